@@ -1,0 +1,383 @@
+// extern "C" boundary of the engine (declared in include/fideslib_hip.h).
+// Every call converts C++ exceptions into a status code plus a thread-local
+// message, the same contract as the reference's CudaError / OPENFHE_THROW
+// surfaced through an int (src/CudaUtils.cuh:706-750).
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fideslib_hip.h"
+#include "fh_keyswitch.hpp"
+
+struct fh_context {
+  fh::Context* c;
+};
+
+struct fh_key {
+  fh::DBuf data;
+  const fh::Context* owner;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return FH_OK;
+  } catch (const fh::Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return FH_E_NOMEM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return FH_E_INTERNAL;
+  }
+}
+
+fh::Context& C(fh_context* h) {
+  if (!h || !h->c) throw fh::Error(FH_E_INVALID, "null context");
+  return *h->c;
+}
+const fh::Context& C(const fh_context* h) {
+  if (!h || !h->c) throw fh::Error(FH_E_INVALID, "null context");
+  return *h->c;
+}
+
+void need(const void* p, const char* what) {
+  if (!p) throw fh::Error(FH_E_INVALID, std::string("null pointer: ") + what);
+}
+
+std::vector<int> pids_or_identity(const fh::Context& c, const int32_t* ids, int n) {
+  if (n < 0 || n > fh::kMaxLimbs) throw fh::Error(FH_E_INVALID, "num_limbs out of range");
+  std::vector<int> v(n);
+  for (int i = 0; i < n; ++i) {
+    v[i] = ids ? ids[i] : i;
+    if (v[i] < 0 || v[i] >= c.np) throw fh::Error(FH_E_INVALID, "prime id out of range");
+  }
+  return v;
+}
+
+void check_level(const fh::Context& c, int level) {
+  if (level < 0 || level >= c.L1) throw fh::Error(FH_E_INVALID, "level out of range");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fh_last_error(void) { return g_err.c_str(); }
+int fh_version(void) { return (1 << 16) | 0; }
+
+int fh_context_create(int log_n, int num_q, int num_p, int dnum, const uint64_t* moduli, fh_context** out) {
+  return guard([&] {
+    need(moduli, "moduli");
+    need(out, "out");
+    std::vector<uint64_t> m(moduli, moduli + num_q + num_p);
+    auto* h = new fh_context{nullptr};
+    try {
+      h->c = new fh::Context(log_n, num_q, num_p, dnum, m);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int fh_context_create_generated(int log_n, int num_q, int num_p, int dnum, int first_bits, int scale_bits, int aux_bits, fh_context** out) {
+  return guard([&] {
+    need(out, "out");
+    if (log_n < 10 || log_n > 16 || num_q < 1 || num_q > fh::kMaxPrimes || dnum < 1) throw fh::Error(FH_E_PARAM, "bad parameters");
+    if (first_bits < 20 || first_bits > 61 || scale_bits < 20 || scale_bits > 61 || aux_bits < 20 || aux_bits > 61)
+      throw fh::Error(FH_E_PARAM, "modulus sizes must be in [20, 61] bits");
+    int K = num_p;
+    if (K <= 0) {
+      auto q = fh::h_gen_primes(log_n, num_q, first_bits, scale_bits, 0, aux_bits);
+      K = fh::h_default_num_special(q, num_q, dnum, aux_bits);
+    }
+    auto m = fh::h_gen_primes(log_n, num_q, first_bits, scale_bits, K, aux_bits);
+    auto* h = new fh_context{nullptr};
+    try {
+      h->c = new fh::Context(log_n, num_q, K, dnum, m);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int fh_context_destroy(fh_context* ctx) {
+  return guard([&] {
+    if (!ctx) return;
+    delete ctx->c;
+    delete ctx;
+  });
+}
+
+int fh_context_shape(const fh_context* ctx, int* out5) {
+  return guard([&] {
+    const auto& c = C(ctx);
+    need(out5, "out");
+    out5[0] = c.logN; out5[1] = c.L1; out5[2] = c.K; out5[3] = c.dnum; out5[4] = c.alpha;
+  });
+}
+
+int fh_context_moduli(const fh_context* ctx, uint64_t* moduli, uint64_t* psi) {
+  return guard([&] {
+    const auto& c = C(ctx);
+    for (int i = 0; i < c.np; ++i) {
+      if (moduli) moduli[i] = c.q[i];
+      if (psi) psi[i] = c.psi[i];
+    }
+  });
+}
+
+int fh_synchronize(fh_context* ctx) {
+  return guard([&] { FH_HIP(hipStreamSynchronize(C(ctx).stream)); });
+}
+
+int fh_device_alloc(fh_context* ctx, size_t bytes, void** out) {
+  return guard([&] {
+    need(out, "out");
+    *out = C(ctx).pool.get(bytes);
+  });
+}
+
+int fh_device_free(fh_context* ctx, void* ptr) {
+  return guard([&] {
+    auto& c = C(ctx);
+    // the pool reuses blocks in stream order; make sure no queued work still reads it
+    FH_HIP(hipStreamSynchronize(c.stream));
+    c.pool.put(ptr);
+  });
+}
+
+int fh_copy_h2d(fh_context* ctx, void* dst, const void* src, size_t bytes) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(dst, "dst"); need(src, "src");
+    FH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c.stream));
+    FH_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int fh_copy_d2h(fh_context* ctx, void* dst, const void* src, size_t bytes) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(dst, "dst"); need(src, "src");
+    FH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream));
+    FH_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int fh_copy_d2d(fh_context* ctx, void* dst, const void* src, size_t bytes) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(dst, "dst"); need(src, "src");
+    FH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c.stream));
+  });
+}
+
+int fh_ntt_forward(fh_context* ctx, uint64_t* polys, const int32_t* prime_ids, int num_limbs, int num_polys, size_t poly_stride) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(polys, "polys");
+    if (num_polys < 1) throw fh::Error(FH_E_INVALID, "num_polys must be >= 1");
+    auto p = pids_or_identity(c, prime_ids, num_limbs);
+    uint64_t stride = poly_stride ? poly_stride : (uint64_t)num_limbs * c.N;
+    fh::ntt_fwd(c, fh::make_set(polys, num_limbs, p.data(), num_polys, stride), c.stream);
+  });
+}
+
+int fh_ntt_inverse(fh_context* ctx, uint64_t* polys, const int32_t* prime_ids, int num_limbs, int num_polys, size_t poly_stride) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(polys, "polys");
+    if (num_polys < 1) throw fh::Error(FH_E_INVALID, "num_polys must be >= 1");
+    auto p = pids_or_identity(c, prime_ids, num_limbs);
+    uint64_t stride = poly_stride ? poly_stride : (uint64_t)num_limbs * c.N;
+    fh::ntt_inv(c, fh::make_set(polys, num_limbs, p.data(), num_polys, stride), c.stream);
+  });
+}
+
+int fh_eltwise_binary(fh_context* ctx, int op, const uint64_t* a, const uint64_t* b, uint64_t* out, const int32_t* prime_ids, int num_limbs) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(a, "a"); need(b, "b"); need(out, "out");
+    if (op < 0 || op > 2) throw fh::Error(FH_E_INVALID, "unknown element-wise op");
+    auto p = pids_or_identity(c, prime_ids, num_limbs);
+    fh::ew_binary(c, op, fh::make_set(const_cast<uint64_t*>(a), num_limbs, p.data()),
+                  fh::make_set(const_cast<uint64_t*>(b), num_limbs, p.data()), fh::make_set(out, num_limbs, p.data()), c.stream);
+  });
+}
+
+int fh_eltwise_scalar(fh_context* ctx, int op, const uint64_t* a, const uint64_t* scalars, uint64_t* out, const int32_t* prime_ids, int num_limbs) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(a, "a"); need(out, "out");
+    if (op < 0 || op > 2) throw fh::Error(FH_E_INVALID, "unknown scalar op");
+    if (op != 2) need(scalars, "scalars");
+    auto p = pids_or_identity(c, prime_ids, num_limbs);
+    fh::ew_unary(c, op, fh::make_set(const_cast<uint64_t*>(a), num_limbs, p.data()), fh::make_set(out, num_limbs, p.data()), scalars, c.stream);
+  });
+}
+
+int fh_tensor(fh_context* ctx, const uint64_t* ct1, const uint64_t* ct2, int level, uint64_t* out3) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(ct1, "ct1"); need(ct2, "ct2"); need(out3, "out3");
+    check_level(c, level);
+    fh::ew_tensor(c, ct1, ct2, out3, level + 1, c.stream);
+  });
+}
+
+int fh_rescale(fh_context* ctx, const uint64_t* ct, int level, int ncomp, uint64_t* out) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(ct, "ct"); need(out, "out");
+    check_level(c, level);
+    if (level < 1) throw fh::Error(FH_E_INVALID, "cannot rescale at level 0");
+    if (ncomp < 1 || ncomp > 3) throw fh::Error(FH_E_INVALID, "ncomp must be 1..3");
+    fh::rescale(c, ct, level, ncomp, out, c.stream);
+  });
+}
+
+int fh_key_create(fh_context* ctx, const uint64_t* host_key, fh_key** out) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(host_key, "host_key"); need(out, "out");
+    size_t words = (size_t)c.dnum * 2 * c.np * c.N;
+    auto* k = new fh_key{fh::DBuf(c.pool, words), &c};
+    hipError_t e = hipMemcpyAsync(k->data.p, host_key, words * 8, hipMemcpyHostToDevice, c.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+    if (e != hipSuccess) {
+      delete k;
+      throw fh::Error(FH_E_DEVICE, std::string("key upload failed: ") + hipGetErrorString(e));
+    }
+    *out = k;
+  });
+}
+
+int fh_key_destroy(fh_context* ctx, fh_key* key) {
+  return guard([&] {
+    if (!key) return;
+    auto& c = C(ctx);
+    FH_HIP(hipStreamSynchronize(c.stream));
+    delete key;
+  });
+}
+
+static const uint64_t* key_of(const fh::Context& c, const fh_key* k) {
+  if (!k) throw fh::Error(FH_E_INVALID, "null key");
+  if (k->owner != &c) throw fh::Error(FH_E_INVALID, "key belongs to another context");
+  return k->data.p;
+}
+
+int fh_num_digits(const fh_context* ctx, int level) {
+  try {
+    const auto& c = C(ctx);
+    check_level(c, level);
+    return c.num_digits(level);
+  } catch (const fh::Error& e) {
+    g_err = e.what();
+    return e.code;
+  }
+}
+
+int fh_modup(fh_context* ctx, const uint64_t* poly, int level, uint64_t* digits) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(poly, "poly"); need(digits, "digits");
+    check_level(c, level);
+    fh::modup(c, poly, level, digits, c.stream);
+  });
+}
+
+int fh_key_inner_product(fh_context* ctx, const uint64_t* digits, int level, const fh_key* key, const uint64_t* c0,
+                         uint32_t galois, uint64_t* out0, uint64_t* out1) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(digits, "digits"); need(out0, "out0"); need(out1, "out1");
+    check_level(c, level);
+    if ((galois & 1) == 0 || galois >= 2 * c.N) throw fh::Error(FH_E_INVALID, "galois index must be odd and < 2N");
+    fh::ksk_inner(c, digits, level, key_of(c, key), out0, out1, c0, galois, c.stream);
+  });
+}
+
+int fh_moddown(fh_context* ctx, const uint64_t* poly_ext, int level, int num_polys, uint64_t* out) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(poly_ext, "poly_ext"); need(out, "out");
+    check_level(c, level);
+    if (num_polys < 1) throw fh::Error(FH_E_INVALID, "num_polys must be >= 1");
+    fh::moddown(c, poly_ext, level, out, num_polys, 0, 0, c.stream);
+  });
+}
+
+int fh_automorph(fh_context* ctx, const uint64_t* poly, int num_limbs, uint32_t galois, uint64_t* out) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(poly, "poly"); need(out, "out");
+    if (num_limbs < 1) throw fh::Error(FH_E_INVALID, "num_limbs must be >= 1");
+    fh::automorph(c, poly, num_limbs, galois, out, c.stream);
+  });
+}
+
+int fh_galois_index(const fh_context* ctx, int32_t rotation, uint32_t* out) {
+  return guard([&] {
+    const auto& c = C(ctx);
+    need(out, "out");
+    uint64_t m = 2ull * c.N;
+    if (rotation == 0) { *out = 1; return; }
+    uint64_t g = rotation > 0 ? 5 : fh::h_inv_odd(5, m);
+    uint64_t r = rotation > 0 ? (uint64_t)rotation : (uint64_t)(-(int64_t)rotation);
+    *out = (uint32_t)fh::h_powmod(g, (uint64_t)r, m);
+  });
+}
+
+int fh_relinearize(fh_context* ctx, const uint64_t* ct3, int level, const fh_key* relin_key, uint64_t* out) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(ct3, "ct3"); need(out, "out");
+    check_level(c, level);
+    fh::relinearize(c, ct3, level, key_of(c, relin_key), out, c.stream);
+  });
+}
+
+int fh_eval_mult(fh_context* ctx, const uint64_t* ct1, const uint64_t* ct2, int level, const fh_key* relin_key, uint64_t* out) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(ct1, "ct1"); need(ct2, "ct2"); need(out, "out");
+    check_level(c, level);
+    fh::eval_mult(c, ct1, ct2, level, key_of(c, relin_key), out, c.stream);
+  });
+}
+
+int fh_eval_rotate(fh_context* ctx, const uint64_t* ct, int level, const fh_key* rot_key, uint32_t galois, uint64_t* out) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(ct, "ct"); need(out, "out");
+    check_level(c, level);
+    if ((galois & 1) == 0 || galois >= 2 * c.N) throw fh::Error(FH_E_INVALID, "galois index must be odd and < 2N");
+    fh::eval_rotate(c, ct, level, key_of(c, rot_key), galois, out, c.stream);
+  });
+}
+
+int fh_eval_rotate_ext(fh_context* ctx, const uint64_t* ct, int level, const fh_key* rot_key, uint32_t galois, int add_c0,
+                       uint64_t* out_ext) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(ct, "ct"); need(out_ext, "out_ext");
+    check_level(c, level);
+    if ((galois & 1) == 0 || galois >= 2 * c.N) throw fh::Error(FH_E_INVALID, "galois index must be odd and < 2N");
+    fh::rotate_ext(c, ct, level, key_of(c, rot_key), galois, add_c0 != 0, out_ext, c.stream);
+  });
+}
+
+}  // extern "C"

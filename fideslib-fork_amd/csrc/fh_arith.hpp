@@ -1,0 +1,86 @@
+// Device modular arithmetic over 64-bit RNS words for gfx950.
+//
+// Every residue is a uint64 below q < 2^62. gfx950 has no 64x64->128 multiply,
+// so products are assembled from 32-bit multiplies by the compiler
+// (__umul64hi). Products by a fixed twiddle use Shoup's precomputed quotient;
+// products of two variables use Barrett reduction of the 128-bit product.
+// Reference: src/ModMult.cuh:42-95, 199-213 (Barrett/Shoup), src/AddSub.cuh:37-50.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fh {
+
+// Per-prime constants, one entry per prime index, resident in device memory.
+struct PrimeConst {
+  uint64_t q;
+  uint64_t q2;        // 2q
+  uint64_t mu;        // Barrett: floor(2^(2k) / q), k = bit length of q
+  uint32_t k;         // bit length of q
+  uint32_t pad;
+  uint64_t ninv;      // N^-1 mod q
+  uint64_t ninv_sh;   // Shoup quotient of ninv
+  uint64_t r64;       // 2^64 mod q
+  uint64_t r64_sh;    // Shoup quotient of r64
+  double qd;          // q as double (fp64 path, q < 2^51)
+  double qinv;        // 1/q rounded
+};
+
+__device__ __forceinline__ uint64_t mulhi(uint64_t a, uint64_t b) { return __umul64hi(a, b); }
+
+// a*w mod q in [0, 2q) for any a < 2^64 and w < q, with wp = floor(w 2^64 / q).
+__device__ __forceinline__ uint64_t shoup_lazy(uint64_t a, uint64_t w, uint64_t wp, uint64_t q) {
+  return a * w - mulhi(a, wp) * q;
+}
+
+__device__ __forceinline__ uint64_t reduce2(uint64_t a, uint64_t q) { return a >= q ? a - q : a; }
+
+__device__ __forceinline__ uint64_t shoup(uint64_t a, uint64_t w, uint64_t wp, uint64_t q) {
+  return reduce2(shoup_lazy(a, w, wp, q), q);
+}
+
+__device__ __forceinline__ uint64_t add_mod(uint64_t a, uint64_t b, uint64_t q) {
+  uint64_t s = a + b;
+  return s >= q ? s - q : s;
+}
+
+__device__ __forceinline__ uint64_t sub_mod(uint64_t a, uint64_t b, uint64_t q) {
+  return a >= b ? a - b : a + q - b;
+}
+
+// Barrett reduction of the 128-bit value (hi, lo) < q^2, result in [0, q).
+// t = x >> (k-1) < 2^(k+1); qhat = (t * mu) >> (k+1); x - qhat q < 3q.
+__device__ __forceinline__ uint64_t barrett128(uint64_t hi, uint64_t lo, const PrimeConst& p) {
+  const uint32_t k = p.k;
+  uint64_t t = (hi << (65 - k)) | (lo >> (k - 1));
+  uint64_t ph = mulhi(t, p.mu), pl = t * p.mu;
+  uint64_t qh = (ph << (63 - k)) | (pl >> (k + 1));
+  uint64_t r = lo - qh * p.q;
+  r = reduce2(r, p.q2);
+  return reduce2(r, p.q);
+}
+
+__device__ __forceinline__ uint64_t mul_mod(uint64_t a, uint64_t b, const PrimeConst& p) {
+  return barrett128(mulhi(a, b), a * b, p);
+}
+
+// 128-bit accumulator for sums of products (basis conversion, inner products).
+struct U128 {
+  uint64_t lo, hi;
+  __device__ __forceinline__ void zero() { lo = 0; hi = 0; }
+  __device__ __forceinline__ void mac(uint64_t a, uint64_t b) {
+    uint64_t pl = a * b, ph = mulhi(a, b);
+    uint64_t nl = lo + pl;
+    hi += ph + (nl < lo ? 1 : 0);
+    lo = nl;
+  }
+};
+
+// (hi, lo) mod q for any 128-bit value: x = hi [2^64]_q + lo (mod q).
+__device__ __forceinline__ uint64_t mod128(uint64_t hi, uint64_t lo, const PrimeConst& p) {
+  uint64_t h = shoup(hi, p.r64, p.r64_sh, p.q);  // [0, q)
+  uint64_t l = barrett128(0, lo, p);              // lo < 2^64 < q^2
+  return add_mod(h, l, p.q);
+}
+
+}  // namespace fh

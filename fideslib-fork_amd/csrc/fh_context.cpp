@@ -1,0 +1,351 @@
+// Host construction of the device context: product-side number theory, twiddle
+// tables and the basis-conversion constants of OpenFHE's CKKS-RNS parameters.
+// Reference: src/ConstantsGPU.cu:131-698 (SetupConstants), constants copied there
+// from OpenFHE CryptoParametersCKKSRNS (src/ConstantsGPU.cu:441-636).
+#include "fh_context.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace fh {
+
+typedef unsigned __int128 u128;
+
+uint64_t h_mulmod(uint64_t a, uint64_t b, uint64_t q) { return (uint64_t)(((u128)a * b) % q); }
+
+uint64_t h_powmod(uint64_t a, uint64_t e, uint64_t q) {
+  uint64_t r = 1 % q;
+  a %= q;
+  for (; e; e >>= 1) {
+    if (e & 1) r = h_mulmod(r, a, q);
+    a = h_mulmod(a, a, q);
+  }
+  return r;
+}
+
+uint64_t h_invmod(uint64_t a, uint64_t q) {
+  // q prime in every use here: Fermat
+  return h_powmod(a % q, q - 2, q);
+}
+
+uint64_t h_shoup(uint64_t w, uint64_t q) { return (uint64_t)(((u128)w << 64) / q); }
+
+uint64_t h_inv_odd(uint64_t g, uint64_t m) {
+  // Newton iteration x <- x (2 - g x) doubles the correct low bits; g x = 1 mod 8 for x = g
+  uint64_t x = g;
+  for (int i = 0; i < 6; ++i) x *= 2 - g * x;
+  return x & (m - 1);
+}
+
+bool h_is_prime(uint64_t n) {
+  if (n < 2) return false;
+  const uint64_t bases[] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+  for (uint64_t b : bases) {
+    if (n == b) return true;
+    if (n % b == 0) return false;
+  }
+  uint64_t d = n - 1;
+  int s = 0;
+  while (!(d & 1)) { d >>= 1; ++s; }
+  for (uint64_t b : bases) {
+    uint64_t x = h_powmod(b, d, n);
+    if (x == 1 || x == n - 1) continue;
+    bool composite = true;
+    for (int r = 1; r < s && composite; ++r) {
+      x = h_mulmod(x, x, n);
+      if (x == n - 1) composite = false;
+    }
+    if (composite) return false;
+  }
+  return true;
+}
+
+// Smallest primitive m-th root of unity mod q (m a power of two): OpenFHE
+// RootOfUnity returns the minimum over all primitive roots.
+uint64_t h_min_root_of_unity(uint64_t m, uint64_t q) {
+  if ((q - 1) % m) throw Error(-2, "modulus is not 1 mod 2N");
+  // m | q-1 with m a power of two: x^((q-1)/m) has order m iff x is a non-residue-like witness;
+  // test candidates until the m/2-th power is -1.
+  uint64_t w = 0;
+  for (uint64_t g = 2;; ++g) {
+    w = h_powmod(g, (q - 1) / m, q);
+    if (h_powmod(w, m / 2, q) == q - 1) break;
+  }
+  uint64_t w2 = h_mulmod(w, w, q), best = w, cur = w;
+  for (uint64_t k = 3; k < m; k += 2) {
+    cur = h_mulmod(cur, w2, q);
+    best = std::min(best, cur);
+  }
+  return best;
+}
+
+static uint64_t first_prime(int bits, uint64_t m) {
+  uint64_t q = (uint64_t)1 << bits, r = q % m, c = q + 1 - r + (r ? m : 0);
+  while (!h_is_prime(c)) c += m;
+  return c;
+}
+static uint64_t last_prime(int bits, uint64_t m) {
+  uint64_t q = (uint64_t)1 << bits, r = q % m, c = q + 1 - r - m;
+  while (!h_is_prime(c)) c -= m;
+  return c;
+}
+static uint64_t prev_prime(uint64_t q, uint64_t m) {
+  uint64_t c = q - m;
+  while (!h_is_prime(c)) c -= m;
+  return c;
+}
+static uint64_t next_prime(uint64_t q, uint64_t m) {
+  uint64_t c = q + m;
+  while (!h_is_prime(c)) c += m;
+  return c;
+}
+
+std::vector<uint64_t> h_gen_primes(int logN, int L1, int first_bits, int scale_bits, int K, int aux_bits) {
+  uint64_t m = (uint64_t)2 << logN;
+  std::vector<uint64_t> out(L1 + K);
+  uint64_t top = first_prime(scale_bits, m), qn = top, qp = top;
+  out[L1 - 1] = top;
+  for (int i = L1 - 2, cnt = 0; i >= 1; --i, ++cnt) {
+    if (cnt % 2 == 0) out[i] = qp = prev_prime(qp, m);
+    else out[i] = qn = next_prime(qn, m);
+  }
+  if (L1 > 1) out[0] = (first_bits == scale_bits) ? next_prime(qn, m) : last_prime(first_bits, m);
+  uint64_t pp = first_prime(aux_bits, m);
+  for (int k = 0; k < K; ++k) {
+    bool inq;
+    do {
+      pp = prev_prime(pp, m);
+      inq = std::find(out.begin(), out.begin() + L1, pp) != out.begin() + L1;
+    } while (inq);
+    out[L1 + k] = pp;
+  }
+  return out;
+}
+
+int h_default_num_special(const std::vector<uint64_t>& q, int L1, int dnum, int aux_bits) {
+  int alpha = (L1 + dnum - 1) / dnum, maxbits = 0;
+  for (int d = 0; d < dnum; ++d) {
+    std::vector<uint64_t> acc{1};
+    for (int i = d * alpha; i < std::min((d + 1) * alpha, L1); ++i) {
+      u128 carry = 0;
+      for (auto& w : acc) {
+        u128 v = (u128)w * q[i] + carry;
+        w = (uint64_t)v;
+        carry = v >> 64;
+      }
+      if (carry) acc.push_back((uint64_t)carry);
+    }
+    int bits = 64 * ((int)acc.size() - 1) + (64 - __builtin_clzll(acc.back()));
+    maxbits = std::max(maxbits, bits);
+  }
+  return (maxbits + aux_bits - 1) / aux_bits;
+}
+
+// ---- Pool ----
+Pool::~Pool() { release_all(); }
+
+void* Pool::get(size_t bytes) {
+  bytes = (bytes + 255) & ~(size_t)255;
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = free_.find(bytes);
+  if (it != free_.end()) {
+    void* p = it->second;
+    free_.erase(it);
+    cached_ -= bytes;
+    in_use_ += bytes;
+    return p;
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    // give back cached blocks and retry once
+    for (auto& kv : free_) { (void)hipFree(kv.second); sizes_.erase(kv.second); }
+    free_.clear();
+    cached_ = 0;
+    e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) throw Error(-4, "device allocation of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+  }
+  sizes_[p] = bytes;
+  in_use_ += bytes;
+  return p;
+}
+
+void Pool::put(void* p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = sizes_.find(p);
+  if (it == sizes_.end()) return;
+  free_.emplace(it->second, p);
+  in_use_ -= it->second;
+  cached_ += it->second;
+}
+
+void Pool::release_all() {
+  std::lock_guard<std::mutex> lk(mu_);
+  for (auto& kv : free_) { (void)hipFree(kv.second); sizes_.erase(kv.second); }
+  free_.clear();
+  cached_ = 0;
+}
+
+// ---- Context ----
+template <class T>
+T* Context::upload(const std::vector<T>& v) {
+  T* d = nullptr;
+  size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
+  FH_HIP(hipMalloc(&d, bytes));
+  owned.push_back(d);
+  if (!v.empty()) FH_HIP(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return d;
+}
+
+static uint32_t bitrev(uint32_t x, int bits) {
+  uint32_t r = 0;
+  for (int i = 0; i < bits; ++i) { r = (r << 1) | (x & 1); x >>= 1; }
+  return r;
+}
+
+Context::Context(int logN_, int L1_, int K_, int dnum_, const std::vector<uint64_t>& primes)
+    : logN(logN_), N(1u << logN_), L1(L1_), K(K_), dnum(dnum_) {
+  if (logN < 10 || logN > 16) throw Error(-1, "logN must be in [10, 16]");
+  if (L1 < 1 || K < 0 || L1 + K > kMaxPrimes) throw Error(-1, "prime counts out of range");
+  if (dnum < 1 || dnum > L1) throw Error(-1, "dnum out of range");
+  if ((int)primes.size() != L1 + K) throw Error(-1, "primes size must be L1 + K");
+  alpha = (L1 + dnum - 1) / dnum;
+  if (L1 - alpha * (dnum - 1) <= 0) throw Error(-1, "cannot distribute the Q towers into dnum digits");
+  np = L1 + K;
+  q = primes;
+  const uint64_t m = 2ull * N;
+  for (int i = 0; i < np; ++i) {
+    if (q[i] >= (1ull << 62) || q[i] % m != 1 || !h_is_prime(q[i])) throw Error(-1, "modulus " + std::to_string(i) + " is not an NTT-friendly prime < 2^62");
+    for (int j = 0; j < i; ++j) if (q[j] == q[i]) throw Error(-1, "duplicate modulus");
+  }
+  psi.resize(np);
+  std::vector<PrimeConst> pc(np);
+  std::vector<uint64_t> tw((size_t)np * N), twsh((size_t)np * N), itw((size_t)np * N), itwsh((size_t)np * N);
+  std::vector<double> twd((size_t)np * N), itwd((size_t)np * N);
+  for (int i = 0; i < np; ++i) {
+    const uint64_t qi = q[i];
+    psi[i] = h_min_root_of_unity(m, qi);
+    PrimeConst& P = pc[i];
+    P.q = qi;
+    P.q2 = 2 * qi;
+    P.k = 64 - __builtin_clzll(qi);
+    P.mu = (uint64_t)(((u128)1 << (2 * P.k)) / qi);
+    P.ninv = h_invmod(N % qi, qi);
+    P.ninv_sh = h_shoup(P.ninv, qi);
+    P.r64 = (uint64_t)(((u128)1 << 64) % qi);
+    P.r64_sh = h_shoup(P.r64, qi);
+    P.qd = (double)qi;
+    P.qinv = 1.0 / (double)qi;
+    // twiddles: tw[brev(j)] = psi^j (OpenFHE ChineseRemainderTransformFTTNat::PreCompute)
+    const uint64_t pinv = h_invmod(psi[i], qi);
+    uint64_t x = 1, xi = 1;
+    for (uint32_t j = 0; j < N; ++j) {
+      size_t r = (size_t)i * N + bitrev(j, logN);
+      tw[r] = x;
+      itw[r] = xi;
+      twsh[r] = h_shoup(x, qi);
+      itwsh[r] = h_shoup(xi, qi);
+      twd[r] = (double)x;
+      itwd[r] = (double)xi;
+      x = h_mulmod(x, psi[i], qi);
+      xi = h_mulmod(xi, pinv, qi);
+    }
+  }
+  d_prime = upload(pc);
+  d_tw = upload(tw);
+  d_tw_sh = upload(twsh);
+  d_itw = upload(itw);
+  d_itw_sh = upload(itwsh);
+  d_twd = upload(twd);
+  d_itwd = upload(itwd);
+
+  // ---- ModUp tables (OpenFHE PartQlHatInvModq / PartQlHatModp) ----
+  const int E = L1 + K;
+  std::vector<uint64_t> qhinv((size_t)L1 * dnum * alpha, 0), qhinv_sh(qhinv.size(), 0);
+  std::vector<uint64_t> qhat((size_t)L1 * dnum * alpha * E, 0);
+  for (int l = 0; l < L1; ++l) {
+    for (int d = 0; d < num_digits(l); ++d) {
+      int start = d * alpha, end = std::min(start + alpha, l + 1);
+      for (int i = start; i < end; ++i) {
+        uint64_t qi = q[i], prod = 1;
+        for (int j = start; j < end; ++j) if (j != i) prod = h_mulmod(prod, q[j] % qi, qi);
+        size_t o = ((size_t)l * dnum + d) * alpha + (i - start);
+        qhinv[o] = h_invmod(prod, qi);
+        qhinv_sh[o] = h_shoup(qhinv[o], qi);
+        for (int pos = 0; pos < l + 1 + K; ++pos) {
+          uint64_t pj = q[ext_prime(l, pos)], pr = 1;
+          for (int j = start; j < end; ++j) if (j != i) pr = h_mulmod(pr, q[j] % pj, pj);
+          qhat[o * E + pos] = pr;
+        }
+      }
+    }
+  }
+  d_modup_qhinv = upload(qhinv);
+  d_modup_qhinv_sh = upload(qhinv_sh);
+  d_modup_qhat = upload(qhat);
+
+  // ---- ModDown tables (OpenFHE PHatInvModp, PHatModq, PInvModq) ----
+  std::vector<uint64_t> phinv(std::max(K, 1), 0), phinv_sh(std::max(K, 1), 0), phat((size_t)std::max(K, 1) * L1, 0);
+  for (int k = 0; k < K; ++k) {
+    uint64_t pk = q[L1 + k], prod = 1;
+    for (int j = 0; j < K; ++j) if (j != k) prod = h_mulmod(prod, q[L1 + j] % pk, pk);
+    phinv[k] = h_invmod(prod, pk);
+    phinv_sh[k] = h_shoup(phinv[k], pk);
+    for (int i = 0; i < L1; ++i) {
+      uint64_t qi = q[i], pr = 1;
+      for (int j = 0; j < K; ++j) if (j != k) pr = h_mulmod(pr, q[L1 + j] % qi, qi);
+      phat[(size_t)k * L1 + i] = pr;
+    }
+  }
+  h_pmod.assign(L1, 0);
+  h_pinv.assign(L1, 0);
+  std::vector<uint64_t> pinv_sh(L1), pmod_sh(L1);
+  for (int i = 0; i < L1; ++i) {
+    uint64_t qi = q[i], pr = 1;
+    for (int j = 0; j < K; ++j) pr = h_mulmod(pr, q[L1 + j] % qi, qi);
+    h_pmod[i] = pr;
+    h_pinv[i] = h_invmod(pr, qi);
+    pinv_sh[i] = h_shoup(h_pinv[i], qi);
+    pmod_sh[i] = h_shoup(pr, qi);
+  }
+  d_md_phinv = upload(phinv);
+  d_md_phinv_sh = upload(phinv_sh);
+  d_md_phat = upload(phat);
+  d_md_pinv = upload(h_pinv);
+  d_md_pinv_sh = upload(pinv_sh);
+  d_pmod = upload(h_pmod);
+  d_pmod_sh = upload(pmod_sh);
+
+  // ---- Rescale tables. OpenFHE QlQlInvModqlDivqlModq[l][i] = floor([Q_{l-1}^-1]_{q_l} Q_{l-1} / q_l) mod q_i.
+  // With u = [Q_{l-1}^-1]_{q_l}, Q_{l-1} u = 1 + q_l R exactly, so R = -q_l^-1 (mod q_i) for i < l.
+  std::vector<uint64_t> qlinv((size_t)L1 * L1, 0), qlinv_sh(qlinv.size(), 0), QlQl(qlinv.size(), 0), QlQl_sh(qlinv.size(), 0);
+  for (int l = 1; l < L1; ++l) {
+    for (int i = 0; i < l; ++i) {
+      uint64_t qi = q[i];
+      uint64_t v = h_invmod(q[l] % qi, qi);
+      size_t o = (size_t)l * L1 + i;
+      qlinv[o] = v;
+      qlinv_sh[o] = h_shoup(v, qi);
+      QlQl[o] = v ? qi - v : 0;
+      QlQl_sh[o] = h_shoup(QlQl[o], qi);
+    }
+  }
+  d_rs_qlinv = upload(qlinv);
+  d_rs_qlinv_sh = upload(qlinv_sh);
+  d_rs_QlQl = upload(QlQl);
+  d_rs_QlQl_sh = upload(QlQl_sh);
+
+  FH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+}
+
+Context::~Context() {
+  if (stream) {
+    (void)hipStreamSynchronize(stream);
+    (void)hipStreamDestroy(stream);
+  }
+  pool.release_all();
+  for (void* p : owned) (void)hipFree(p);
+}
+
+}  // namespace fh

@@ -1,0 +1,146 @@
+// Device context for the RNS-CKKS evaluation path: primes, twiddle tables and
+// every basis-conversion constant, resident in HBM for the context's lifetime.
+//
+// Prime indices: 0..L1-1 are the Q chain (q_0 first), L1..L1+K-1 the special
+// primes P. A polynomial over Q_l ∪ P keeps its limbs in the order
+// q_0..q_l, p_0..p_{K-1}; every limb is N contiguous uint64 words in the
+// bit-reversed EVALUATION order of OpenFHE (no reordering anywhere).
+// Reference: src/ConstantsGPU.cuh:49-198 (constants), src/CKKS/Context.cuh:56-424.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fh_arith.hpp"
+
+namespace fh {
+
+constexpr int kMaxPrimes = 72;   // Q + P primes a context may hold
+constexpr int kMaxLimbs = 72;    // limbs of one polynomial in one launch descriptor
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define FH_HIP(call)                                                                           \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      throw ::fh::Error(-5, std::string("HIP error: ") + hipGetErrorString(e_) + " at " +      \
+                                 __FILE__ + ":" + std::to_string(__LINE__));                   \
+  } while (0)
+
+// ---- host number theory (product-side; u128 arithmetic) ----
+uint64_t h_mulmod(uint64_t a, uint64_t b, uint64_t q);
+uint64_t h_powmod(uint64_t a, uint64_t e, uint64_t q);
+uint64_t h_invmod(uint64_t a, uint64_t q);
+uint64_t h_shoup(uint64_t w, uint64_t q);  // floor(w 2^64 / q)
+uint64_t h_inv_odd(uint64_t g, uint64_t m); // inverse of odd g modulo a power of two m
+bool h_is_prime(uint64_t n);
+uint64_t h_min_root_of_unity(uint64_t m, uint64_t q);
+// OpenFHE FIXEDMANUAL modulus chain (L1 primes) + K special primes
+std::vector<uint64_t> h_gen_primes(int logN, int L1, int first_bits, int scale_bits, int K, int aux_bits);
+int h_default_num_special(const std::vector<uint64_t>& q, int L1, int dnum, int aux_bits);
+
+// Simple size-bucketed caching allocator over hipMalloc (stream-ordered reuse on the context stream).
+class Pool {
+ public:
+  ~Pool();
+  void* get(size_t bytes);
+  void put(void* p);
+  void release_all();
+  size_t bytes_in_use() const { return in_use_; }
+  size_t bytes_cached() const { return cached_; }
+
+ private:
+  std::mutex mu_;
+  std::multimap<size_t, void*> free_;
+  std::map<void*, size_t> sizes_;
+  size_t in_use_ = 0, cached_ = 0;
+};
+
+// RAII device buffer from the pool.
+struct DBuf {
+  Pool* pool = nullptr;
+  uint64_t* p = nullptr;
+  size_t words = 0;
+  DBuf() = default;
+  DBuf(Pool& pl, size_t w) : pool(&pl), p((uint64_t*)pl.get(w * 8)), words(w) {}
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : pool(o.pool), p(o.p), words(o.words) { o.p = nullptr; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    reset();
+    pool = o.pool; p = o.p; words = o.words; o.p = nullptr;
+    return *this;
+  }
+  ~DBuf() { reset(); }
+  void reset() { if (p && pool) pool->put(p); p = nullptr; }
+};
+
+struct Context {
+  int logN = 0;
+  uint32_t N = 0;
+  int L1 = 0, K = 0, dnum = 0, alpha = 0, np = 0;
+  std::vector<uint64_t> q, psi;
+
+  // device tables
+  PrimeConst* d_prime = nullptr;   // [np]
+  uint64_t* d_tw = nullptr;        // [np][N] psi^brev(k)
+  uint64_t* d_tw_sh = nullptr;     // Shoup quotients
+  uint64_t* d_itw = nullptr;       // [np][N] psi^-brev(k)
+  uint64_t* d_itw_sh = nullptr;
+  double* d_twd = nullptr;         // [np][N] twiddles as doubles (fp64 NTT path, q < 2^51)
+  double* d_itwd = nullptr;
+
+  // ModUp: for level l and digit d, source prescale [Qhat_i^-1]_{q_i} (+Shoup), and
+  // [Qhat_i]_{p_j} for every extended position j of level l.
+  uint64_t* d_modup_qhinv = nullptr;     // [L1][dnum][alpha]
+  uint64_t* d_modup_qhinv_sh = nullptr;  // [L1][dnum][alpha]
+  uint64_t* d_modup_qhat = nullptr;      // [L1][dnum][alpha][L1+K]
+  // ModDown
+  uint64_t* d_md_phinv = nullptr;        // [K]     [P^_k^-1]_{p_k}
+  uint64_t* d_md_phinv_sh = nullptr;     // [K]
+  uint64_t* d_md_phat = nullptr;         // [K][L1] [P^_k]_{q_i}
+  uint64_t* d_md_pinv = nullptr;         // [L1]    [P^-1]_{q_i}
+  uint64_t* d_md_pinv_sh = nullptr;
+  uint64_t* d_pmod = nullptr;            // [L1]    [P]_{q_i}
+  uint64_t* d_pmod_sh = nullptr;
+  // Rescale
+  uint64_t* d_rs_qlinv = nullptr;        // [L1][L1] [q_l^-1]_{q_i}
+  uint64_t* d_rs_qlinv_sh = nullptr;
+  uint64_t* d_rs_QlQl = nullptr;         // [L1][L1] QlQlInvModqlDivqlModq
+  uint64_t* d_rs_QlQl_sh = nullptr;
+
+  // host copies of the conversion constants (for table checks and host helpers)
+  std::vector<uint64_t> h_pmod, h_pinv;
+
+  hipStream_t stream = nullptr;
+  Pool pool;
+  std::vector<void*> owned;   // table allocations freed at destruction
+
+  Context(int logN, int L1, int K, int dnum, const std::vector<uint64_t>& primes);
+  ~Context();
+  Context(const Context&) = delete;
+  Context& operator=(const Context&) = delete;
+
+  int num_digits(int l) const {
+    int b = (l + 1 + alpha - 1) / alpha;
+    return b > dnum ? dnum : b;
+  }
+  int ext_prime(int l, int pos) const { return pos <= l ? pos : L1 + (pos - l - 1); }
+  size_t limb_words() const { return N; }
+
+ private:
+  template <class T>
+  T* upload(const std::vector<T>& v);
+};
+
+}  // namespace fh

@@ -1,0 +1,146 @@
+// Element-wise RNS kernels: add, sub, negate, multiply, scalar ops and the
+// CKKS tensor product. Each block walks one limb (blockIdx.y) with 16-byte
+// vector loads; these kernels are HBM-bound by construction.
+// Reference: src/AddSub.cuh:12-50, src/CKKS/ElemenwiseBatchKernels.cuh:13-99,
+// src/CKKS/ElemenwiseBatchKernels.cu:1363-1510 (binomialMult_ / binomialSquare_).
+#include "fh_ops.hpp"
+
+namespace fh {
+
+namespace {
+
+constexpr int kEwThreads = 256;
+
+struct V2 {
+  uint64_t x, y;
+};
+
+__device__ __forceinline__ V2 ld2(const uint64_t* p) {
+  ulonglong2 v = *reinterpret_cast<const ulonglong2*>(p);
+  return V2{v.x, v.y};
+}
+__device__ __forceinline__ void st2(uint64_t* p, uint64_t a, uint64_t b) {
+  *reinterpret_cast<ulonglong2*>(p) = make_ulonglong2(a, b);
+}
+
+// out = op(a, b) limb-wise; a, b, out share the prime map of `a`.
+template <int OP>
+__global__ __launch_bounds__(kEwThreads) void k_binary(LimbSet a, LimbSet b, LimbSet o, const PrimeConst* pr, uint32_t N) {
+  uint64_t *pa, *pb, *po;
+  int pid, pid2, pid3;
+  SetAddr{a}.at(blockIdx.y, N, pa, pid);
+  SetAddr{b}.at(blockIdx.y, N, pb, pid2);
+  SetAddr{o}.at(blockIdx.y, N, po, pid3);
+  const PrimeConst P = pr[pid];
+  for (uint32_t i = 2 * (blockIdx.x * blockDim.x + threadIdx.x); i < N; i += 2 * gridDim.x * blockDim.x) {
+    V2 x = ld2(pa + i), y = ld2(pb + i);
+    uint64_t r0, r1;
+    if constexpr (OP == 0) { r0 = add_mod(x.x, y.x, P.q); r1 = add_mod(x.y, y.y, P.q); }
+    else if constexpr (OP == 1) { r0 = sub_mod(x.x, y.x, P.q); r1 = sub_mod(x.y, y.y, P.q); }
+    else { r0 = mul_mod(x.x, y.x, P); r1 = mul_mod(x.y, y.y, P); }
+    st2(po + i, r0, r1);
+  }
+}
+
+struct Scalars {
+  uint64_t v[kMaxLimbs];
+  uint64_t sh[kMaxLimbs];
+};
+
+// OP 0: a + s_limb ; OP 1: a * s_limb (Shoup) ; OP 2: -a
+template <int OP>
+__global__ __launch_bounds__(kEwThreads) void k_unary(LimbSet a, LimbSet o, Scalars sc, const PrimeConst* pr, uint32_t N) {
+  uint64_t *pa, *po;
+  int pid, pid2;
+  SetAddr{a}.at(blockIdx.y, N, pa, pid);
+  SetAddr{o}.at(blockIdx.y, N, po, pid2);
+  const uint64_t q = pr[pid].q;
+  const uint32_t limb = blockIdx.y % a.nlimbs;
+  const uint64_t s = sc.v[limb], ssh = sc.sh[limb];
+  for (uint32_t i = 2 * (blockIdx.x * blockDim.x + threadIdx.x); i < N; i += 2 * gridDim.x * blockDim.x) {
+    V2 x = ld2(pa + i);
+    uint64_t r0, r1;
+    if constexpr (OP == 0) { r0 = add_mod(x.x, s, q); r1 = add_mod(x.y, s, q); }
+    else if constexpr (OP == 1) { r0 = shoup(x.x, s, ssh, q); r1 = shoup(x.y, s, ssh, q); }
+    else { r0 = x.x ? q - x.x : 0; r1 = x.y ? q - x.y : 0; }
+    st2(po + i, r0, r1);
+  }
+}
+
+// (c0, c1) x (d0, d1) -> (c0 d0, c0 d1 + c1 d0, c1 d1); square when SQ.
+template <bool SQ>
+__global__ __launch_bounds__(kEwThreads) void k_tensor(const uint64_t* c, const uint64_t* d, uint64_t* o, uint32_t nl, const PrimeConst* pr, uint32_t N) {
+  const uint32_t limb = blockIdx.y;
+  const PrimeConst P = pr[limb];
+  const size_t comp = (size_t)nl * N, off = (size_t)limb * N;
+  const uint64_t *c0 = c + off, *c1 = c + comp + off, *d0 = d + off, *d1 = d + comp + off;
+  uint64_t *o0 = o + off, *o1 = o + comp + off, *o2 = o + 2 * comp + off;
+  for (uint32_t i = 2 * (blockIdx.x * blockDim.x + threadIdx.x); i < N; i += 2 * gridDim.x * blockDim.x) {
+    V2 a0 = ld2(c0 + i), a1 = ld2(c1 + i);
+    V2 b0 = SQ ? a0 : ld2(d0 + i), b1 = SQ ? a1 : ld2(d1 + i);
+    uint64_t x0 = mul_mod(a0.x, b0.x, P), y0 = mul_mod(a0.y, b0.y, P);
+    uint64_t x2 = mul_mod(a1.x, b1.x, P), y2 = mul_mod(a1.y, b1.y, P);
+    uint64_t x1, y1;
+    if constexpr (SQ) {
+      x1 = mul_mod(a0.x, a1.x, P); x1 = add_mod(x1, x1, P.q);
+      y1 = mul_mod(a0.y, a1.y, P); y1 = add_mod(y1, y1, P.q);
+    } else {
+      // c0 d1 + c1 d0 accumulated in 128 bits, one reduction
+      U128 s; s.zero(); s.mac(a0.x, b1.x); s.mac(a1.x, b0.x); x1 = mod128(s.hi, s.lo, P);
+      U128 t; t.zero(); t.mac(a0.y, b1.y); t.mac(a1.y, b0.y); y1 = mod128(t.hi, t.lo, P);
+    }
+    st2(o0 + i, x0, y0);
+    st2(o1 + i, x1, y1);
+    st2(o2 + i, x2, y2);
+  }
+}
+
+inline dim3 ew_grid(uint32_t N, uint32_t nlimbs) {
+  uint32_t per_block = 2 * kEwThreads;
+  uint32_t gx = (N + per_block - 1) / per_block;
+  if (gx > 64) gx = 64;
+  return dim3(gx, nlimbs);
+}
+
+}  // namespace
+
+void ew_binary(const Context& c, int op, const LimbSet& a, const LimbSet& b, const LimbSet& o, hipStream_t st) {
+  uint32_t n = a.nlimbs * a.npolys;
+  if (!n) return;
+  dim3 g = ew_grid(c.N, n);
+  switch (op) {
+    case 0: k_binary<0><<<g, kEwThreads, 0, st>>>(a, b, o, c.d_prime, c.N); break;
+    case 1: k_binary<1><<<g, kEwThreads, 0, st>>>(a, b, o, c.d_prime, c.N); break;
+    default: k_binary<2><<<g, kEwThreads, 0, st>>>(a, b, o, c.d_prime, c.N); break;
+  }
+  FH_HIP(hipGetLastError());
+}
+
+void ew_unary(const Context& c, int op, const LimbSet& a, const LimbSet& o, const uint64_t* scalars, hipStream_t st) {
+  uint32_t n = a.nlimbs * a.npolys;
+  if (!n) return;
+  Scalars sc{};
+  for (uint32_t i = 0; i < a.nlimbs; ++i) {
+    uint64_t q = c.q[a.pid[i]];
+    uint64_t v = scalars ? scalars[i] % q : 0;
+    sc.v[i] = v;
+    sc.sh[i] = h_shoup(v, q);
+  }
+  dim3 g = ew_grid(c.N, n);
+  switch (op) {
+    case 0: k_unary<0><<<g, kEwThreads, 0, st>>>(a, o, sc, c.d_prime, c.N); break;
+    case 1: k_unary<1><<<g, kEwThreads, 0, st>>>(a, o, sc, c.d_prime, c.N); break;
+    default: k_unary<2><<<g, kEwThreads, 0, st>>>(a, o, sc, c.d_prime, c.N); break;
+  }
+  FH_HIP(hipGetLastError());
+}
+
+void ew_tensor(const Context& c, const uint64_t* ct1, const uint64_t* ct2, uint64_t* out3, int nlimbs, hipStream_t st) {
+  if (nlimbs <= 0) return;
+  dim3 g = ew_grid(c.N, nlimbs);
+  if (ct1 == ct2) k_tensor<true><<<g, kEwThreads, 0, st>>>(ct1, ct1, out3, nlimbs, c.d_prime, c.N);
+  else k_tensor<false><<<g, kEwThreads, 0, st>>>(ct1, ct2, out3, nlimbs, c.d_prime, c.N);
+  FH_HIP(hipGetLastError());
+}
+
+}  // namespace fh

@@ -1,0 +1,101 @@
+// Launch descriptors shared by the HIP kernels and their host launchers.
+//
+// A kernel addresses the limbs it works on through an Addr object passed by
+// value in the kernel arguments (no per-call pointer-table upload):
+//   SetAddr - npolys polynomials of nlimbs limbs each, equally strided, one prime
+//             map shared by all of them (blockIdx.y = poly * nlimbs + limb);
+//   JobAddr - a device-resident table of (limb pointer, prime) jobs, used for
+//             batches of ciphertexts at different levels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fh_arith.hpp"
+#include "fh_context.hpp"
+
+namespace fh {
+
+struct LimbSet {
+  uint64_t* base;
+  uint64_t poly_stride;  // words between consecutive polynomials
+  uint32_t nlimbs;
+  uint32_t npolys;
+  uint8_t pid[kMaxLimbs];
+};
+
+struct LimbJob {
+  uint64_t* ptr;
+  uint32_t pid;
+  uint32_t pad;
+};
+
+struct SetAddr {
+  LimbSet s;
+  __device__ __forceinline__ void at(uint32_t y, uint32_t N, uint64_t*& p, int& pid) const {
+    uint32_t limb = y % s.nlimbs, poly = y / s.nlimbs;
+    p = s.base + (size_t)poly * s.poly_stride + (size_t)limb * N;
+    pid = s.pid[limb];
+  }
+};
+
+struct JobAddr {
+  const LimbJob* jobs;
+  __device__ __forceinline__ void at(uint32_t y, uint32_t, uint64_t*& p, int& pid) const {
+    p = jobs[y].ptr;
+    pid = (int)jobs[y].pid;
+  }
+};
+
+// Target limbs of a ModUp output (beta digits, each over Q_l ∪ P, ext = l+1+K limbs):
+// job y enumerates, digit by digit, every extended position outside the digit's own primes.
+struct ModUpAddr {
+  uint64_t* base;
+  uint32_t ext, lq, alpha, L1, beta;
+  __device__ __forceinline__ void at(uint32_t y, uint32_t N, uint64_t*& p, int& pid) const {
+    const uint32_t full = ext - alpha;
+    uint32_t d = y / full, t = y - d * full;
+    if (d >= beta - 1) { d = beta - 1; t = y - (beta - 1) * full; }
+    const uint32_t start = d * alpha;
+    const uint32_t sz = alpha < lq - start ? alpha : lq - start;
+    const uint32_t pos = t < start ? t : t + sz;
+    p = base + ((size_t)d * ext + pos) * N;
+    pid = (int)(pos < lq ? pos : L1 + (pos - lq));
+  }
+  __host__ uint32_t count() const { return beta * ext - lq; }
+};
+
+struct NttTables {
+  const PrimeConst* prime;
+  const uint64_t* tw;
+  const uint64_t* tw_sh;
+  const uint64_t* itw;
+  const uint64_t* itw_sh;
+  uint32_t N;
+  uint32_t logN;
+};
+
+inline NttTables ntt_tables(const Context& c) {
+  return NttTables{c.d_prime, c.d_tw, c.d_tw_sh, c.d_itw, c.d_itw_sh, c.N, (uint32_t)c.logN};
+}
+
+inline LimbSet make_set(uint64_t* base, uint32_t nlimbs, const int* pids, uint32_t npolys = 1, uint64_t poly_stride = 0) {
+  if (nlimbs > (uint32_t)kMaxLimbs) throw Error(-1, "too many limbs for one launch descriptor");
+  LimbSet s{};
+  s.base = base;
+  s.nlimbs = nlimbs;
+  s.npolys = npolys;
+  s.poly_stride = poly_stride;
+  for (uint32_t i = 0; i < nlimbs; ++i) s.pid[i] = (uint8_t)pids[i];
+  return s;
+}
+
+// ---- launchers (fh_ntt.hip) ----
+void ntt_fwd(const Context& c, const LimbSet& s, hipStream_t st);
+void ntt_inv(const Context& c, const LimbSet& s, hipStream_t st);
+void ntt_fwd(const Context& c, const LimbSet& in, const LimbSet& out, hipStream_t st);  // out-of-place
+void ntt_inv(const Context& c, const LimbSet& in, const LimbSet& out, hipStream_t st);
+void ntt_fwd_modup(const Context& c, const ModUpAddr& a, uint32_t n, hipStream_t st);
+void ntt_fwd_jobs(const Context& c, const LimbJob* d_jobs, uint32_t njobs, hipStream_t st);
+void ntt_inv_jobs(const Context& c, const LimbJob* d_jobs, uint32_t njobs, hipStream_t st);
+
+}  // namespace fh

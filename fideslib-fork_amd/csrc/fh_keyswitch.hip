@@ -1,0 +1,368 @@
+// Rescale, hybrid key switching (ModUp, key inner product, ModDown) and the
+// automorphism, plus the ciphertext compositions built from them: relinearized
+// EvalMult, EvalRotate in the hoisted formulation, the extended (Q_l ∪ P)
+// rotation used by hoisted linear transforms.
+//
+// Reference: src/CKKS/LimbPartition.cu:711-797 (rescale), :851-968 (modup),
+// :1345-1556 (dotKSK), :2145-2217 (moddown), :2224-2281 (automorph);
+// src/CKKS/Conv.cu:27-430 (ModDown2 / DecompAndModUpConv);
+// src/CKKS/ElemenwiseBatchKernels.cu:202-377 (fusedDotKSK_2_, hoistedRotateDotKSK_2_);
+// src/Rotation.cuh:14-34 (automorph_slot); src/CKKS/Ciphertext.cpp:569-607,
+// 618-622, 846-866, 1102-1168 (op compositions).
+#include <algorithm>
+
+#include "fh_keyswitch.hpp"
+
+namespace fh {
+
+namespace {
+
+constexpr int kT = 256;
+constexpr int kMaxAlpha = 32;
+constexpr int kMaxK = 32;
+
+__device__ __forceinline__ uint32_t brev_bits(uint32_t x, int bits) { return __brev(x) >> (32 - bits); }
+
+// automorphism gather index in bit-reversed EVALUATION order:
+// out[j] = in[map(j)], map(brev(i)) = brev(((2i+1) k mod 2N) >> 1)   (OpenFHE PrecomputeAutoMap)
+__device__ __forceinline__ uint32_t auto_src(uint32_t j, uint32_t k, int logN) {
+  uint32_t i = brev_bits(j, logN);
+  uint32_t e = ((2 * i + 1) * k) & ((2u << logN) - 1);
+  return brev_bits(e >> 1, logN);
+}
+
+// ----------------------------------------------------------------------------------- ModUp conversion
+struct ModUpParams {
+  uint32_t N, lq, ext, alpha, L1, K, beta, level, dnum;
+  const PrimeConst* prime;
+  const uint64_t* qhinv;     // [L1][dnum][alpha]
+  const uint64_t* qhinv_sh;
+  const uint64_t* qhat;      // [L1][dnum][alpha][L1+K]
+};
+
+// blockIdx.y = digit. in_coef: INTT of the input (lq limbs), in_eval: the input itself.
+// Writes the digit's own limbs (copied in EVAL form) and the converted coefficient-form
+// residues for every other extended position.
+__global__ __launch_bounds__(kT) void k_modup_conv(const uint64_t* __restrict__ in_coef, const uint64_t* __restrict__ in_eval,
+                                                    uint64_t* __restrict__ out, ModUpParams mp) {
+  __shared__ uint64_t s_qhat[kMaxAlpha * kMaxPrimes];
+  __shared__ uint64_t s_inv[kMaxAlpha], s_invsh[kMaxAlpha], s_q[kMaxAlpha];
+  const uint32_t d = blockIdx.y;
+  const uint32_t start = d * mp.alpha;
+  const uint32_t sz = min(mp.alpha, mp.lq - start);
+  const uint32_t E = mp.L1 + mp.K;
+  const size_t tb = ((size_t)mp.level * mp.dnum + d) * mp.alpha;
+  for (uint32_t i = threadIdx.x; i < sz * mp.ext; i += blockDim.x) {
+    uint32_t si = i / mp.ext, pos = i % mp.ext;
+    s_qhat[si * mp.ext + pos] = mp.qhat[(tb + si) * E + pos];
+  }
+  for (uint32_t i = threadIdx.x; i < sz; i += blockDim.x) {
+    s_inv[i] = mp.qhinv[tb + i];
+    s_invsh[i] = mp.qhinv_sh[tb + i];
+    s_q[i] = mp.prime[start + i].q;
+  }
+  __syncthreads();
+  uint64_t* od = out + (size_t)d * mp.ext * mp.N;
+  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < mp.N; n += gridDim.x * blockDim.x) {
+    uint64_t y[kMaxAlpha];
+#pragma unroll 4
+    for (uint32_t i = 0; i < sz; ++i) {
+      const size_t o = (size_t)(start + i) * mp.N + n;
+      y[i] = shoup(in_coef[o], s_inv[i], s_invsh[i], s_q[i]);
+      od[o] = in_eval[o];
+    }
+    for (uint32_t pos = 0; pos < mp.ext; ++pos) {
+      if (pos == start) { pos += sz - 1; continue; }
+      const uint32_t p = pos < mp.lq ? pos : mp.L1 + (pos - mp.lq);
+      U128 acc;
+      acc.zero();
+      for (uint32_t i = 0; i < sz; ++i) acc.mac(y[i], s_qhat[i * mp.ext + pos]);
+      od[(size_t)pos * mp.N + n] = mod128(acc.hi, acc.lo, mp.prime[p]);
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------------- key inner product
+struct KskParams {
+  uint32_t N, lq, ext, L1, K, beta, logN;
+  uint32_t galois;          // automorphism index (1 = none)
+  const PrimeConst* prime;
+  const uint64_t* key;      // [dnum][2][L1+K][N]
+  const uint64_t* c0;       // optional: c0 over Q_l, added as P*c0 to out0's Q limbs
+  const uint64_t* pmod;     // [L1] [P]_{q_i}
+  const uint64_t* pmod_sh;
+};
+
+// blockIdx.y = extended position. out0 = sum_d digit_d b_d (+ P c0), out1 = sum_d digit_d a_d,
+// both written through the automorphism (scatter to the image slot).
+__global__ __launch_bounds__(kT) void k_ksk_inner(const uint64_t* __restrict__ digits, uint64_t* __restrict__ out0,
+                                                   uint64_t* __restrict__ out1, KskParams kp) {
+  const uint32_t pos = blockIdx.y;
+  const uint32_t p = pos < kp.lq ? pos : kp.L1 + (pos - kp.lq);
+  const PrimeConst P = kp.prime[p];
+  const uint32_t E = kp.L1 + kp.K;
+  const size_t N = kp.N;
+  const bool addc0 = kp.c0 != nullptr && pos < kp.lq;
+  const uint64_t pm = addc0 ? kp.pmod[pos] : 0, pmsh = addc0 ? kp.pmod_sh[pos] : 0;
+  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < kp.N; n += gridDim.x * blockDim.x) {
+    U128 a0, a1;
+    a0.zero();
+    a1.zero();
+    for (uint32_t d = 0; d < kp.beta; ++d) {
+      const uint64_t x = digits[((size_t)d * kp.ext + pos) * N + n];
+      const uint64_t kb = kp.key[(((size_t)d * 2 + 0) * E + p) * N + n];
+      const uint64_t ka = kp.key[(((size_t)d * 2 + 1) * E + p) * N + n];
+      a0.mac(x, kb);
+      a1.mac(x, ka);
+    }
+    uint64_t r0 = mod128(a0.hi, a0.lo, P), r1 = mod128(a1.hi, a1.lo, P);
+    if (addc0) r0 = add_mod(r0, shoup(kp.c0[(size_t)pos * N + n], pm, pmsh, P.q), P.q);
+    // gather form: out[j] = in[auto_src(j)]  <=>  scatter: out[auto_dst(n)] = in[n]
+    uint32_t j = n;
+    if (kp.galois != 1) {
+      // invert the gather map: find j with auto_src(j) = n. auto_src with k^-1 is the inverse map.
+      j = auto_src(n, kp.galois, kp.logN);
+    }
+    out0[(size_t)pos * N + j] = r0;
+    out1[(size_t)pos * N + j] = r1;
+  }
+}
+
+// ----------------------------------------------------------------------------------- ModDown
+struct ModDownParams {
+  uint32_t N, lq, ext, L1, K;
+  const PrimeConst* prime;
+  const uint64_t* phinv;     // [K]
+  const uint64_t* phinv_sh;
+  const uint64_t* phat;      // [K][L1]
+  const uint64_t* pinv;      // [L1]
+  const uint64_t* pinv_sh;
+};
+
+// blockIdx.y = poly. pcoef: INTT of the K special limbs of each poly (K limbs per poly);
+// conv: lq limbs per poly, coefficient form.
+__global__ __launch_bounds__(kT) void k_moddown_conv(const uint64_t* __restrict__ pcoef, uint64_t* __restrict__ conv, ModDownParams mp) {
+  __shared__ uint64_t s_phat[kMaxK * kMaxPrimes];
+  __shared__ uint64_t s_inv[kMaxK], s_invsh[kMaxK], s_p[kMaxK];
+  for (uint32_t i = threadIdx.x; i < mp.K * mp.lq; i += blockDim.x) {
+    uint32_t k = i / mp.lq, t = i % mp.lq;
+    s_phat[k * mp.lq + t] = mp.phat[(size_t)k * mp.L1 + t];
+  }
+  for (uint32_t k = threadIdx.x; k < mp.K; k += blockDim.x) {
+    s_inv[k] = mp.phinv[k];
+    s_invsh[k] = mp.phinv_sh[k];
+    s_p[k] = mp.prime[mp.L1 + k].q;
+  }
+  __syncthreads();
+  const uint64_t* pc = pcoef + (size_t)blockIdx.y * mp.K * mp.N;
+  uint64_t* cv = conv + (size_t)blockIdx.y * mp.lq * mp.N;
+  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < mp.N; n += gridDim.x * blockDim.x) {
+    uint64_t y[kMaxK];
+#pragma unroll 4
+    for (uint32_t k = 0; k < mp.K; ++k) y[k] = shoup(pc[(size_t)k * mp.N + n], s_inv[k], s_invsh[k], s_p[k]);
+    for (uint32_t i = 0; i < mp.lq; ++i) {
+      U128 acc;
+      acc.zero();
+      for (uint32_t k = 0; k < mp.K; ++k) acc.mac(y[k], s_phat[k * mp.lq + i]);
+      cv[(size_t)i * mp.N + n] = mod128(acc.hi, acc.lo, mp.prime[i]);
+    }
+  }
+}
+
+// out_i = (x_i - conv_i) * [P^-1]_{q_i}; blockIdx.y = poly * lq + i. x has ext limbs per poly.
+__global__ __launch_bounds__(kT) void k_moddown_final(const uint64_t* __restrict__ x, const uint64_t* __restrict__ conv,
+                                                       uint64_t* __restrict__ out, ModDownParams mp, uint64_t x_stride, uint64_t out_stride) {
+  const uint32_t poly = blockIdx.y / mp.lq, i = blockIdx.y % mp.lq;
+  const uint64_t q = mp.prime[i].q, w = mp.pinv[i], wsh = mp.pinv_sh[i];
+  const uint64_t* xi = x + poly * x_stride + (size_t)i * mp.N;
+  const uint64_t* ci = conv + ((size_t)poly * mp.lq + i) * mp.N;
+  uint64_t* oi = out + poly * out_stride + (size_t)i * mp.N;
+  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < mp.N; n += gridDim.x * blockDim.x)
+    oi[n] = shoup(sub_mod(xi[n], ci[n], q), w, wsh, q);
+}
+
+// copy the K special limbs of each poly (ext limbs per poly, stride x_stride) to a packed buffer
+__global__ __launch_bounds__(kT) void k_gather_special(const uint64_t* __restrict__ x, uint64_t* __restrict__ out, uint32_t N, uint32_t lq,
+                                                        uint32_t K, uint64_t x_stride) {
+  const uint32_t poly = blockIdx.y / K, k = blockIdx.y % K;
+  const uint64_t* src = x + poly * x_stride + (size_t)(lq + k) * N;
+  uint64_t* dst = out + ((size_t)poly * K + k) * N;
+  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) dst[n] = src[n];
+}
+
+// ----------------------------------------------------------------------------------- Rescale
+struct RescaleParams {
+  uint32_t N, l, L1;
+  const PrimeConst* prime;
+  const uint64_t* qlinv;     // [L1][L1]
+  const uint64_t* qlinv_sh;
+  const uint64_t* QlQl;      // [L1][L1]
+  const uint64_t* QlQl_sh;
+};
+
+// blockIdx.y = comp * l + i: e_i = SwitchModulus_{q_l -> q_i}(top) * QlQl[l][i] (coefficient form)
+__global__ __launch_bounds__(kT) void k_rescale_expand(const uint64_t* __restrict__ top, uint64_t* __restrict__ e, RescaleParams rp) {
+  const uint32_t comp = blockIdx.y / rp.l, i = blockIdx.y % rp.l;
+  const uint64_t ql = rp.prime[rp.l].q, qi = rp.prime[i].q;
+  const uint64_t half = ql >> 1;
+  const size_t o = (size_t)rp.l * rp.L1 + i;
+  const uint64_t f = rp.QlQl[o], fsh = rp.QlQl_sh[o];
+  // OpenFHE NativeVector::SwitchModulus, old modulus q_l > or < q_i
+  const uint64_t diff = ql > qi ? qi - (ql % qi) : qi - ql;
+  const uint64_t* t = top + (size_t)comp * rp.N;
+  uint64_t* ei = e + ((size_t)comp * rp.l + i) * rp.N;
+  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < rp.N; n += gridDim.x * blockDim.x) {
+    uint64_t v = t[n];
+    if (v > half) v += diff;
+    if (v >= qi) v %= qi;
+    ei[n] = shoup(v, f, fsh, qi);
+  }
+}
+
+// out_i = x_i * [q_l^-1]_{q_i} + e_i; blockIdx.y = comp * l + i. x: (l+1) limbs per comp.
+__global__ __launch_bounds__(kT) void k_rescale_final(const uint64_t* __restrict__ x, const uint64_t* __restrict__ e,
+                                                       uint64_t* __restrict__ out, RescaleParams rp) {
+  const uint32_t comp = blockIdx.y / rp.l, i = blockIdx.y % rp.l;
+  const uint64_t qi = rp.prime[i].q;
+  const size_t o = (size_t)rp.l * rp.L1 + i;
+  const uint64_t w = rp.qlinv[o], wsh = rp.qlinv_sh[o];
+  const uint64_t* xi = x + ((size_t)comp * (rp.l + 1) + i) * rp.N;
+  const uint64_t* ei = e + ((size_t)comp * rp.l + i) * rp.N;
+  uint64_t* oi = out + ((size_t)comp * rp.l + i) * rp.N;
+  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < rp.N; n += gridDim.x * blockDim.x)
+    oi[n] = add_mod(shoup(xi[n], w, wsh, qi), ei[n], qi);
+}
+
+// copy limb l of each component (stride (l+1) limbs) into a packed buffer
+__global__ __launch_bounds__(kT) void k_take_top(const uint64_t* __restrict__ x, uint64_t* __restrict__ out, uint32_t N, uint32_t l) {
+  const uint64_t* src = x + ((size_t)blockIdx.y * (l + 1) + l) * N;
+  uint64_t* dst = out + (size_t)blockIdx.y * N;
+  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) dst[n] = src[n];
+}
+
+// ----------------------------------------------------------------------------------- automorphism
+__global__ __launch_bounds__(kT) void k_automorph(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint32_t N, uint32_t logN,
+                                                   uint32_t k) {
+  const uint64_t* a = in + (size_t)blockIdx.y * N;
+  uint64_t* o = out + (size_t)blockIdx.y * N;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < N; j += gridDim.x * blockDim.x) o[j] = a[auto_src(j, k, logN)];
+}
+
+inline dim3 grid_n(uint32_t N, uint32_t y, uint32_t per_thread = 1) {
+  uint32_t gx = (N / per_thread + kT - 1) / kT;
+  return dim3(std::max(1u, std::min(gx, 256u)), y);
+}
+
+}  // namespace
+
+// ===================================================================================== host side
+
+void modup(Context& c, const uint64_t* x, int l, uint64_t* digits, hipStream_t st) {
+  const uint32_t lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);
+  if (c.K < 1) throw Error(-1, "key switching needs at least one special prime");
+  if ((uint32_t)c.alpha > (uint32_t)kMaxAlpha) throw Error(-3, "digit size above kernel limit");
+  std::vector<int> ids(lq);
+  for (uint32_t i = 0; i < lq; ++i) ids[i] = i;
+  DBuf tmp(c.pool, (size_t)lq * c.N);
+  ntt_inv(c, make_set(const_cast<uint64_t*>(x), lq, ids.data()), make_set(tmp.p, lq, ids.data()), st);
+  ModUpParams mp{c.N, lq, ext, (uint32_t)c.alpha, (uint32_t)c.L1, (uint32_t)c.K, beta, (uint32_t)l, (uint32_t)c.dnum,
+                 c.d_prime, c.d_modup_qhinv, c.d_modup_qhinv_sh, c.d_modup_qhat};
+  k_modup_conv<<<grid_n(c.N, beta), kT, 0, st>>>(tmp.p, x, digits, mp);
+  FH_HIP(hipGetLastError());
+  ModUpAddr a{digits, ext, lq, (uint32_t)c.alpha, (uint32_t)c.L1, beta};
+  ntt_fwd_modup(c, a, a.count(), st);
+}
+
+void ksk_inner(Context& c, const uint64_t* digits, int l, const uint64_t* key, uint64_t* out0, uint64_t* out1,
+               const uint64_t* c0, uint32_t galois, hipStream_t st) {
+  const uint32_t lq = l + 1, ext = lq + c.K;
+  KskParams kp{c.N, lq, ext, (uint32_t)c.L1, (uint32_t)c.K, (uint32_t)c.num_digits(l), (uint32_t)c.logN, galois,
+               c.d_prime, key, c0, c.d_pmod, c.d_pmod_sh};
+  if (galois != 1) {
+    // the kernel scatters through the inverse map: pass k^-1 so that out[j] = in[src_k(j)]
+    kp.galois = (uint32_t)h_inv_odd(galois, 2ull * c.N);
+  }
+  k_ksk_inner<<<grid_n(c.N, ext), kT, 0, st>>>(digits, out0, out1, kp);
+  FH_HIP(hipGetLastError());
+}
+
+void moddown(Context& c, const uint64_t* x, int l, uint64_t* out, int npolys, uint64_t x_stride, uint64_t out_stride, hipStream_t st) {
+  const uint32_t lq = l + 1, ext = lq + c.K, K = c.K;
+  if (K > (uint32_t)kMaxK) throw Error(-3, "special prime count above kernel limit");
+  if (!x_stride) x_stride = (uint64_t)ext * c.N;
+  if (!out_stride) out_stride = (uint64_t)lq * c.N;
+  DBuf pc(c.pool, (size_t)npolys * K * c.N), conv(c.pool, (size_t)npolys * lq * c.N);
+  k_gather_special<<<grid_n(c.N, npolys * K), kT, 0, st>>>(x, pc.p, c.N, lq, K, x_stride);
+  std::vector<int> pids(K), qids(lq);
+  for (uint32_t k = 0; k < K; ++k) pids[k] = c.L1 + k;
+  for (uint32_t i = 0; i < lq; ++i) qids[i] = i;
+  ntt_inv(c, make_set(pc.p, K, pids.data(), npolys, (uint64_t)K * c.N), st);
+  ModDownParams mp{c.N, lq, ext, (uint32_t)c.L1, K, c.d_prime, c.d_md_phinv, c.d_md_phinv_sh, c.d_md_phat, c.d_md_pinv, c.d_md_pinv_sh};
+  k_moddown_conv<<<grid_n(c.N, npolys), kT, 0, st>>>(pc.p, conv.p, mp);
+  FH_HIP(hipGetLastError());
+  ntt_fwd(c, make_set(conv.p, lq, qids.data(), npolys, (uint64_t)lq * c.N), st);
+  k_moddown_final<<<grid_n(c.N, npolys * lq), kT, 0, st>>>(x, conv.p, out, mp, x_stride, out_stride);
+  FH_HIP(hipGetLastError());
+}
+
+void rescale(Context& c, const uint64_t* x, int l, int ncomp, uint64_t* out, hipStream_t st) {
+  if (l < 1) throw Error(-1, "cannot rescale a level-0 ciphertext");
+  DBuf top(c.pool, (size_t)ncomp * c.N), e(c.pool, (size_t)ncomp * l * c.N);
+  k_take_top<<<grid_n(c.N, ncomp), kT, 0, st>>>(x, top.p, c.N, l);
+  int pid_top = l;
+  ntt_inv(c, make_set(top.p, 1, &pid_top, ncomp, c.N), st);
+  RescaleParams rp{c.N, (uint32_t)l, (uint32_t)c.L1, c.d_prime, c.d_rs_qlinv, c.d_rs_qlinv_sh, c.d_rs_QlQl, c.d_rs_QlQl_sh};
+  k_rescale_expand<<<grid_n(c.N, ncomp * l), kT, 0, st>>>(top.p, e.p, rp);
+  FH_HIP(hipGetLastError());
+  std::vector<int> ids(l);
+  for (int i = 0; i < l; ++i) ids[i] = i;
+  ntt_fwd(c, make_set(e.p, l, ids.data(), ncomp, (uint64_t)l * c.N), st);
+  k_rescale_final<<<grid_n(c.N, ncomp * l), kT, 0, st>>>(x, e.p, out, rp);
+  FH_HIP(hipGetLastError());
+}
+
+void automorph(Context& c, const uint64_t* x, int nlimbs, uint32_t k, uint64_t* out, hipStream_t st) {
+  if (x == out) throw Error(-1, "automorphism cannot run in place");
+  if ((k & 1) == 0 || k >= 2 * c.N) throw Error(-1, "automorphism index must be odd and < 2N");
+  k_automorph<<<grid_n(c.N, nlimbs), kT, 0, st>>>(x, out, c.N, c.logN, k);
+  FH_HIP(hipGetLastError());
+}
+
+// ks = ModDown(<ModUp(c2), key>); out = (c0 + ks0, c1 + ks1)
+void relinearize(Context& c, const uint64_t* ct3, int l, const uint64_t* key, uint64_t* out2, hipStream_t st) {
+  const uint32_t lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);
+  const size_t P = (size_t)lq * c.N;
+  DBuf dig(c.pool, (size_t)beta * ext * c.N), t(c.pool, (size_t)2 * ext * c.N), ks(c.pool, 2 * P);
+  modup(c, ct3 + 2 * P, l, dig.p, st);
+  ksk_inner(c, dig.p, l, key, t.p, t.p + (size_t)ext * c.N, nullptr, 1, st);
+  moddown(c, t.p, l, ks.p, 2, 0, 0, st);
+  std::vector<int> ids(lq);
+  for (uint32_t i = 0; i < lq; ++i) ids[i] = i;
+  ew_binary(c, 0, make_set(const_cast<uint64_t*>(ct3), lq, ids.data(), 2, P), make_set(ks.p, lq, ids.data(), 2, P),
+            make_set(out2, lq, ids.data(), 2, P), st);
+}
+
+void eval_mult(Context& c, const uint64_t* ct1, const uint64_t* ct2, int l, const uint64_t* key, uint64_t* out2, hipStream_t st) {
+  const size_t P = (size_t)(l + 1) * c.N;
+  DBuf t(c.pool, 3 * P);
+  ew_tensor(c, ct1, ct2, t.p, l + 1, st);
+  relinearize(c, t.p, l, key, out2, st);
+}
+
+// hoisted formulation: ext key switch of c1 (+ P c0), automorph, result over Q_l ∪ P
+void rotate_ext(Context& c, const uint64_t* ct, int l, const uint64_t* key, uint32_t galois, bool add_c0, uint64_t* out_ext, hipStream_t st) {
+  const uint32_t lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);
+  const size_t P = (size_t)lq * c.N;
+  DBuf dig(c.pool, (size_t)beta * ext * c.N);
+  modup(c, ct + P, l, dig.p, st);
+  ksk_inner(c, dig.p, l, key, out_ext, out_ext + (size_t)ext * c.N, add_c0 ? ct : nullptr, galois, st);
+}
+
+void eval_rotate(Context& c, const uint64_t* ct, int l, const uint64_t* key, uint32_t galois, uint64_t* out2, hipStream_t st) {
+  const uint32_t lq = l + 1, ext = lq + c.K;
+  DBuf e(c.pool, (size_t)2 * ext * c.N);
+  rotate_ext(c, ct, l, key, galois, true, e.p, st);
+  moddown(c, e.p, l, out2, 2, 0, 0, st);
+}
+
+}  // namespace fh

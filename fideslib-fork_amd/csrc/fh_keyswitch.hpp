@@ -1,0 +1,26 @@
+// Rescale, hybrid key switching, automorphism and the ciphertext-level
+// compositions (fh_keyswitch.hip). Buffers are device pointers in the layout of
+// include/fideslib_hip.h; keys are device copies of [dnum][b, a][L1+K][N].
+#pragma once
+#include "fh_ops.hpp"
+
+namespace fh {
+
+// digits: beta x (l+1+K) limbs (EVAL)
+void modup(Context& c, const uint64_t* x, int l, uint64_t* digits, hipStream_t st);
+// out0/out1: (l+1+K) limbs. c0 (optional, l+1 limbs) is folded as P*c0 into out0's Q limbs;
+// galois != 1 writes both outputs through the automorphism X -> X^galois.
+void ksk_inner(Context& c, const uint64_t* digits, int l, const uint64_t* key, uint64_t* out0, uint64_t* out1,
+               const uint64_t* c0, uint32_t galois, hipStream_t st);
+// npolys polynomials over Q_l ∪ P (x_stride words apart, 0 = packed) -> over Q_l (out_stride apart)
+void moddown(Context& c, const uint64_t* x, int l, uint64_t* out, int npolys, uint64_t x_stride, uint64_t out_stride, hipStream_t st);
+// ncomp components over Q_l -> over Q_{l-1}
+void rescale(Context& c, const uint64_t* x, int l, int ncomp, uint64_t* out, hipStream_t st);
+void automorph(Context& c, const uint64_t* x, int nlimbs, uint32_t k, uint64_t* out, hipStream_t st);
+
+void relinearize(Context& c, const uint64_t* ct3, int l, const uint64_t* key, uint64_t* out2, hipStream_t st);
+void eval_mult(Context& c, const uint64_t* ct1, const uint64_t* ct2, int l, const uint64_t* key, uint64_t* out2, hipStream_t st);
+void rotate_ext(Context& c, const uint64_t* ct, int l, const uint64_t* key, uint32_t galois, bool add_c0, uint64_t* out_ext, hipStream_t st);
+void eval_rotate(Context& c, const uint64_t* ct, int l, const uint64_t* key, uint32_t galois, uint64_t* out2, hipStream_t st);
+
+}  // namespace fh

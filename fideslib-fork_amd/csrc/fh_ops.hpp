@@ -1,0 +1,16 @@
+// Host launchers of the evaluation kernels (element-wise, rescale, key
+// switching, automorphism) and the ciphertext-level compositions built on them.
+#pragma once
+#include "fh_kernels.hpp"
+
+namespace fh {
+
+// ---- element-wise (fh_eltwise.hip) ----
+// op: 0 add, 1 sub, 2 mul
+void ew_binary(const Context& c, int op, const LimbSet& a, const LimbSet& b, const LimbSet& o, hipStream_t st);
+// op: 0 add scalar, 1 mul scalar, 2 negate (scalars per limb, may be null for negate)
+void ew_unary(const Context& c, int op, const LimbSet& a, const LimbSet& o, const uint64_t* scalars, hipStream_t st);
+// ct1, ct2: 2 x nlimbs over Q_l; out3: 3 x nlimbs. ct1 == ct2 selects the square form.
+void ew_tensor(const Context& c, const uint64_t* ct1, const uint64_t* ct2, uint64_t* out3, int nlimbs, hipStream_t st);
+
+}  // namespace fh

@@ -1,0 +1,244 @@
+"""Python binding of the C ABI in include/fideslib_hip.h (libfideslib_hip.so).
+
+Plumbing for tests and the benchmark: numpy <-> device buffers and thin wrappers that call the
+HIP kernels through ctypes. There is no CPU fallback: if the shared library is missing or no GPU
+is present, every call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libfideslib_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "fideslib_hip.h")
+
+_lib = None
+
+
+class FHError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libfideslib_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FHError(f"{LIB_PATH} is missing: build it with `make -C fideslib-fork_amd`")
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    P, I, U, S, VP = C.c_void_p, C.c_int, C.c_uint64, C.c_size_t, C.c_void_p
+    PP = C.POINTER(C.c_void_p)
+    u64p, i32p, ip = C.POINTER(C.c_uint64), C.POINTER(C.c_int32), C.POINTER(C.c_int)
+    sig = {
+        "fh_last_error": (C.c_char_p, []),
+        "fh_version": (I, []),
+        "fh_context_create": (I, [I, I, I, I, u64p, PP]),
+        "fh_context_create_generated": (I, [I, I, I, I, I, I, I, PP]),
+        "fh_context_destroy": (I, [P]),
+        "fh_context_shape": (I, [P, ip]),
+        "fh_context_moduli": (I, [P, u64p, u64p]),
+        "fh_synchronize": (I, [P]),
+        "fh_device_alloc": (I, [P, S, PP]),
+        "fh_device_free": (I, [P, VP]),
+        "fh_copy_h2d": (I, [P, VP, VP, S]),
+        "fh_copy_d2h": (I, [P, VP, VP, S]),
+        "fh_copy_d2d": (I, [P, VP, VP, S]),
+        "fh_ntt_forward": (I, [P, VP, i32p, I, I, S]),
+        "fh_ntt_inverse": (I, [P, VP, i32p, I, I, S]),
+        "fh_eltwise_binary": (I, [P, I, VP, VP, VP, i32p, I]),
+        "fh_eltwise_scalar": (I, [P, I, VP, u64p, VP, i32p, I]),
+        "fh_tensor": (I, [P, VP, VP, I, VP]),
+        "fh_rescale": (I, [P, VP, I, I, VP]),
+        "fh_key_create": (I, [P, u64p, PP]),
+        "fh_key_destroy": (I, [P, P]),
+        "fh_num_digits": (I, [P, I]),
+        "fh_modup": (I, [P, VP, I, VP]),
+        "fh_key_inner_product": (I, [P, VP, I, P, VP, C.c_uint32, VP, VP]),
+        "fh_moddown": (I, [P, VP, I, I, VP]),
+        "fh_automorph": (I, [P, VP, I, C.c_uint32, VP]),
+        "fh_galois_index": (I, [P, C.c_int32, C.POINTER(C.c_uint32)]),
+        "fh_relinearize": (I, [P, VP, I, P, VP]),
+        "fh_eval_mult": (I, [P, VP, VP, I, P, VP]),
+        "fh_eval_rotate": (I, [P, VP, I, P, C.c_uint32, VP]),
+        "fh_eval_rotate_ext": (I, [P, VP, I, P, C.c_uint32, I, VP]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def _check(rc):
+    if rc != 0:
+        raise FHError(f"fideslib_hip error {rc}: {load().fh_last_error().decode()}")
+
+
+def _u64p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+
+def _i32(ids):
+    if ids is None:
+        return None
+    arr = (C.c_int32 * len(ids))(*[int(i) for i in ids])
+    return arr
+
+
+class DeviceArray:
+    """A device allocation of uint64 words with a numpy-like shape (freed with the context pool)."""
+
+    def __init__(self, ctx, shape):
+        self.ctx = ctx
+        self.shape = tuple(shape)
+        self.words = int(np.prod(self.shape))
+        p = C.c_void_p()
+        _check(load().fh_device_alloc(ctx.h, max(self.words, 1) * 8, C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host, dtype=np.uint64)
+        assert host.size == self.words, (host.shape, self.shape)
+        _check(load().fh_copy_h2d(self.ctx.h, self.ptr, host.ctypes.data, self.words * 8))
+        return self
+
+    def download(self):
+        out = np.empty(self.shape, dtype=np.uint64)
+        _check(load().fh_copy_d2h(self.ctx.h, out.ctypes.data, self.ptr, self.words * 8))
+        return out
+
+    def addr(self, word_offset=0):
+        return self.ptr + 8 * word_offset
+
+    def free(self):
+        if self.ptr:
+            _check(load().fh_device_free(self.ctx.h, self.ptr))
+            self.ptr = None
+
+
+class Context:
+    """A device context (primes, tables, stream, pool)."""
+
+    def __init__(self, log_n, num_q, num_p, dnum, moduli=None, first_bits=60, scale_bits=50, aux_bits=60):
+        L = load()
+        h = C.c_void_p()
+        if moduli is not None:
+            m = np.ascontiguousarray(moduli, dtype=np.uint64)
+            _check(L.fh_context_create(log_n, num_q, num_p, dnum, _u64p(m), C.byref(h)))
+        else:
+            _check(L.fh_context_create_generated(log_n, num_q, num_p, dnum, first_bits, scale_bits, aux_bits, C.byref(h)))
+        self.h = h.value
+        shape = (C.c_int * 5)()
+        _check(L.fh_context_shape(self.h, shape))
+        self.log_n, self.L1, self.K, self.dnum, self.alpha = list(shape)
+        self.N = 1 << self.log_n
+        self.moduli = np.zeros(self.L1 + self.K, dtype=np.uint64)
+        self.psi = np.zeros(self.L1 + self.K, dtype=np.uint64)
+        _check(L.fh_context_moduli(self.h, _u64p(self.moduli), _u64p(self.psi)))
+
+    def close(self):
+        if self.h:
+            _check(load().fh_context_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        _check(load().fh_synchronize(self.h))
+
+    def array(self, shape):
+        return DeviceArray(self, shape)
+
+    def to_device(self, host):
+        host = np.ascontiguousarray(host, dtype=np.uint64)
+        return DeviceArray(self, host.shape).upload(host)
+
+    # ---- NTT ----
+    def ntt_forward(self, d, prime_ids=None, num_limbs=None, num_polys=1, poly_stride=0, offset=0):
+        nl = num_limbs if num_limbs is not None else (len(prime_ids) if prime_ids is not None else d.shape[0])
+        _check(load().fh_ntt_forward(self.h, d.addr(offset), _i32(prime_ids), nl, num_polys, poly_stride))
+
+    def ntt_inverse(self, d, prime_ids=None, num_limbs=None, num_polys=1, poly_stride=0, offset=0):
+        nl = num_limbs if num_limbs is not None else (len(prime_ids) if prime_ids is not None else d.shape[0])
+        _check(load().fh_ntt_inverse(self.h, d.addr(offset), _i32(prime_ids), nl, num_polys, poly_stride))
+
+    # ---- element-wise ----
+    def eltwise(self, op, a, b, out, prime_ids=None):
+        nl = len(prime_ids) if prime_ids is not None else a.shape[0]
+        _check(load().fh_eltwise_binary(self.h, op, a.ptr, b.ptr, out.ptr, _i32(prime_ids), nl))
+
+    def scalar(self, op, a, scalars, out, prime_ids=None):
+        nl = len(prime_ids) if prime_ids is not None else a.shape[0]
+        s = None if scalars is None else np.ascontiguousarray(scalars, dtype=np.uint64)
+        _check(load().fh_eltwise_scalar(self.h, op, a.ptr, None if s is None else _u64p(s), out.ptr, _i32(prime_ids), nl))
+
+    def tensor(self, ct1, ct2, level, out3):
+        _check(load().fh_tensor(self.h, ct1.ptr, ct2.ptr, level, out3.ptr))
+
+    # ---- rescale / key switching ----
+    def rescale(self, ct, level, ncomp, out):
+        _check(load().fh_rescale(self.h, ct.ptr, level, ncomp, out.ptr))
+
+    def key(self, host_key):
+        return Key(self, host_key)
+
+    def num_digits(self, level):
+        r = load().fh_num_digits(self.h, level)
+        if r < 0:
+            _check(r)
+        return r
+
+    def modup(self, poly, level, digits, offset=0):
+        _check(load().fh_modup(self.h, poly.addr(offset), level, digits.ptr))
+
+    def key_inner_product(self, digits, level, key, out0, out1, c0=None, galois=1):
+        _check(load().fh_key_inner_product(self.h, digits.ptr, level, key.h, None if c0 is None else c0.ptr,
+                                           galois, out0.ptr, out1.ptr))
+
+    def moddown(self, x, level, num_polys, out):
+        _check(load().fh_moddown(self.h, x.ptr, level, num_polys, out.ptr))
+
+    def automorph(self, x, num_limbs, galois, out):
+        _check(load().fh_automorph(self.h, x.ptr, num_limbs, galois, out.ptr))
+
+    def galois_index(self, rotation):
+        v = C.c_uint32()
+        _check(load().fh_galois_index(self.h, rotation, C.byref(v)))
+        return v.value
+
+    def relinearize(self, ct3, level, key, out):
+        _check(load().fh_relinearize(self.h, ct3.ptr, level, key.h, out.ptr))
+
+    def eval_mult(self, ct1, ct2, level, key, out):
+        _check(load().fh_eval_mult(self.h, ct1.ptr, ct2.ptr, level, key.h, out.ptr))
+
+    def eval_rotate(self, ct, level, key, galois, out):
+        _check(load().fh_eval_rotate(self.h, ct.ptr, level, key.h, galois, out.ptr))
+
+    def eval_rotate_ext(self, ct, level, key, galois, out_ext, add_c0=True):
+        _check(load().fh_eval_rotate_ext(self.h, ct.ptr, level, key.h, galois, int(add_c0), out_ext.ptr))
+
+
+class Key:
+    """A key-switching key resident on the device ([dnum][b, a][L1+K][N] upload layout)."""
+
+    def __init__(self, ctx, host_key):
+        host_key = np.ascontiguousarray(host_key, dtype=np.uint64)
+        assert host_key.size == ctx.dnum * 2 * (ctx.L1 + ctx.K) * ctx.N
+        h = C.c_void_p()
+        _check(load().fh_key_create(ctx.h, _u64p(host_key), C.byref(h)))
+        self.ctx, self.h = ctx, h.value
+
+    def free(self):
+        if self.h:
+            _check(load().fh_key_destroy(self.ctx.h, self.h))
+            self.h = None

@@ -1,0 +1,137 @@
+/*
+ * CPU oracle for the RNS-CKKS evaluation path — TEST INFRASTRUCTURE ONLY.
+ *
+ * This library is a plain-C restatement of the arithmetic that the reference
+ * (FIDESlib, and the patched OpenFHE 1.5.1 fork it is bit-compatible with) runs
+ * on the hot path: negacyclic NTT/INTT in OpenFHE's bit-reversed EVALUATION
+ * order, RNS modular arithmetic, Rescale, hybrid key switching (ModUp, key inner
+ * product, ModDown), automorphisms and the hoisted rotation formulation, and the
+ * BSGS linear transform. Only the test suite, the smoke check and the CPU
+ * baseline of the benchmark load it; the product never links or calls it.
+ *
+ * Layout: every polynomial is limb-major, one contiguous N-word array per limb,
+ * values canonical in [0, q). A ciphertext is its components back to back.
+ * Prime indices: 0..L1-1 are the Q chain (q_0 first), L1..L1+K-1 the special
+ * primes P. A polynomial "over Q_l" has l+1 limbs (primes 0..l); "over Q_l∪P"
+ * has l+1+K limbs (primes 0..l then L1..L1+K-1).
+ *
+ * Parity status per function is stated in oracle/README.md and in the tests
+ * that pin each function (tests/test_oracle_*.py).
+ */
+#ifndef CKKS_ORACLE_H
+#define CKKS_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAXP 72
+
+typedef struct orc_ctx orc_ctx;
+
+/* ---- scalar number theory ---- */
+uint64_t orc_mulmod(uint64_t a, uint64_t b, uint64_t q);
+uint64_t orc_powmod(uint64_t a, uint64_t e, uint64_t q);
+uint64_t orc_invmod(uint64_t a, uint64_t q);
+int orc_is_prime(uint64_t n);
+uint64_t orc_first_prime(int bits, uint64_t m);
+uint64_t orc_last_prime(int bits, uint64_t m);
+uint64_t orc_previous_prime(uint64_t q, uint64_t m);
+uint64_t orc_next_prime(uint64_t q, uint64_t m);
+uint64_t orc_root_of_unity(uint64_t m, uint64_t q);
+/* FIXEDMANUAL CKKS modulus chain (L1 primes) followed by K special primes. */
+int orc_gen_primes(int logN, int L1, int first_bits, int scale_bits, int K, int aux_bits, uint64_t* out);
+/* OpenFHE's rule for the special-prime count: ceil(max digit bit length / aux_bits). */
+int orc_default_num_special(const uint64_t* q, int L1, int dnum, int aux_bits);
+
+/* ---- context ---- */
+orc_ctx* orc_create(int logN, int L1, int K, int dnum, const uint64_t* primes);
+void orc_destroy(orc_ctx* c);
+uint64_t orc_prime(const orc_ctx* c, int idx);
+uint64_t orc_psi(const orc_ctx* c, int idx);
+int orc_alpha(const orc_ctx* c);
+const uint64_t* orc_twiddles(const orc_ctx* c, int idx);     /* psi^brev(k), k < N */
+const uint64_t* orc_inv_twiddles(const orc_ctx* c, int idx); /* psi^-brev(k) */
+/* ModUp/ModDown/Rescale tables, exposed for pinning */
+uint64_t orc_rescale_ql_inv(const orc_ctx* c, int l, int i);      /* [q_l^-1]_{q_i}, i<l */
+uint64_t orc_rescale_QlQlInv(const orc_ctx* c, int l, int i);     /* OpenFHE QlQlInvModqlDivqlModq */
+uint64_t orc_P_mod(const orc_ctx* c, int i);                      /* [P]_{q_i} */
+uint64_t orc_P_inv_mod(const orc_ctx* c, int i);                  /* [P^-1]_{q_i} */
+
+/* ---- NTT (in place, per limb; pids[i] is the prime index of limb i) ---- */
+void orc_ntt_fwd(const orc_ctx* c, uint64_t* a, const int* pids, int nlimbs);
+void orc_ntt_inv(const orc_ctx* c, uint64_t* a, const int* pids, int nlimbs);
+/* single-prime versions with explicit tables (used by the q=17 known-answer test) */
+void orc_ntt_fwd_raw(uint64_t* a, uint64_t N, uint64_t q, const uint64_t* tw);
+void orc_ntt_inv_raw(uint64_t* a, uint64_t N, uint64_t q, const uint64_t* itw, uint64_t ninv);
+void orc_make_tables(uint64_t N, uint64_t q, uint64_t psi, uint64_t* tw, uint64_t* itw);
+
+/* ---- element-wise (over Q_l: prime i for limb i) ---- */
+void orc_add(const orc_ctx* c, const uint64_t* a, const uint64_t* b, uint64_t* out, int nlimbs);
+void orc_sub(const orc_ctx* c, const uint64_t* a, const uint64_t* b, uint64_t* out, int nlimbs);
+void orc_mul(const orc_ctx* c, const uint64_t* a, const uint64_t* b, uint64_t* out, int nlimbs);
+void orc_neg(const orc_ctx* c, const uint64_t* a, uint64_t* out, int nlimbs);
+/* multiply limb i by scalars[i] */
+void orc_mul_scalar(const orc_ctx* c, const uint64_t* a, const uint64_t* scalars, uint64_t* out, int nlimbs);
+void orc_add_scalar(const orc_ctx* c, const uint64_t* a, const uint64_t* scalars, uint64_t* out, int nlimbs);
+/* tensor: (c0,c1)x(d0,d1) -> (c0d0, c0d1+c1d0, c1d1) */
+void orc_tensor(const orc_ctx* c, const uint64_t* ct1, const uint64_t* ct2, uint64_t* out3, int nlimbs);
+void orc_square(const orc_ctx* c, const uint64_t* ct1, uint64_t* out3, int nlimbs);
+
+/* ---- Rescale: poly over Q_l (EVAL) -> over Q_{l-1} (EVAL) ---- */
+void orc_rescale_poly(const orc_ctx* c, const uint64_t* x, int l, uint64_t* out);
+
+/* ---- hybrid key switching ---- */
+int orc_num_digits(const orc_ctx* c, int l);
+/* ModUp of x over Q_l (EVAL): out = beta digits, each over Q_l∪P (EVAL) */
+void orc_modup(const orc_ctx* c, const uint64_t* x, int l, uint64_t* out);
+/* key inner product. key layout: [dnum][2 (b then a)][L1+K limbs][N] (EVAL).
+ * out0 = sum_d digit_d * b_d, out1 = sum_d digit_d * a_d, both over Q_l∪P */
+void orc_ksk_inner(const orc_ctx* c, const uint64_t* digits, int l, const uint64_t* key, uint64_t* out0, uint64_t* out1);
+/* ModDown of x over Q_l∪P (EVAL) -> over Q_l (EVAL) */
+void orc_moddown(const orc_ctx* c, const uint64_t* x, int l, uint64_t* out);
+/* automorphism X -> X^k in EVALUATION form, gather form of OpenFHE's PrecomputeAutoMap */
+void orc_automorph(const orc_ctx* c, const uint64_t* x, int nlimbs, uint32_t k, uint64_t* out);
+uint32_t orc_rot_index(const orc_ctx* c, int r); /* 5^r mod 2N (r may be negative) */
+
+/* ---- ciphertext-level ops (ct = 2 or 3 components over Q_l) ---- */
+/* ks = ModDown(<ModUp(c2), evk>); out = (c0 + ks0, c1 + ks1) */
+void orc_relinearize(const orc_ctx* c, const uint64_t* ct3, int l, const uint64_t* key, uint64_t* out2);
+void orc_eval_mult(const orc_ctx* c, const uint64_t* ct1, const uint64_t* ct2, int l, const uint64_t* key, uint64_t* out2);
+void orc_rescale_ct(const orc_ctx* c, const uint64_t* ct, int l, int ncomp, uint64_t* out);
+/* hoisted-formulation rotation: ext key switch of c1, fold P*c0, automorph, ModDown */
+void orc_eval_rotate(const orc_ctx* c, const uint64_t* ct, int l, const uint64_t* key, uint32_t k, uint64_t* out);
+/* rotation that stays in Q_l∪P (no ModDown): out is 2 x (l+1+K) limbs */
+void orc_eval_rotate_ext(const orc_ctx* c, const uint64_t* ct, int l, const uint64_t* key, uint32_t k, int add_c0, uint64_t* out_ext);
+
+/* ---- key generation / encryption (seeded, mt19937_64) ---- */
+/* ternary secret, EVAL over Q∪P (L1+K limbs); hamming weight h (0 = uniform ternary) */
+void orc_gen_secret(const orc_ctx* c, uint64_t seed, int h, uint64_t* s);
+/* switching key from s_old (EVAL over Q, L1 limbs) to s_new (EVAL over Q∪P) */
+void orc_gen_switch_key(const orc_ctx* c, const uint64_t* s_old, const uint64_t* s_new, uint64_t seed, uint64_t* key);
+void orc_gen_relin_key(const orc_ctx* c, const uint64_t* s, uint64_t seed, uint64_t* key);
+void orc_gen_rot_key(const orc_ctx* c, const uint64_t* s, uint32_t k, uint64_t seed, uint64_t* key);
+/* CKKS encode of n complex slots (re,im interleaved) at scale, over Q_l (EVAL) */
+void orc_encode(const orc_ctx* c, const double* slots, int n, double scale, int l, uint64_t* out);
+/* same, but over Q_l ∪ P and multiplied by P (the extended-basis plaintext form) */
+void orc_encode_ext(const orc_ctx* c, const double* slots, int n, double scale, int l, uint64_t* out);
+/* decode a plaintext poly over Q_l (EVAL) into n complex slots */
+void orc_decode(const orc_ctx* c, const uint64_t* m, int l, int n, double scale, double* slots);
+void orc_encrypt(const orc_ctx* c, const uint64_t* m, int l, const uint64_t* s, uint64_t seed, uint64_t* ct);
+void orc_decrypt(const orc_ctx* c, const uint64_t* ct, int l, int ncomp, const uint64_t* s, uint64_t* m);
+
+/* ---- random data (uniform limbs) ---- */
+void orc_random_poly(const orc_ctx* c, uint64_t seed, const int* pids, int nlimbs, uint64_t* out);
+
+/* ---- mt19937_64 reference stream (to check the generator itself) ---- */
+uint64_t orc_mt_first(uint64_t seed, int skip);
+
+/* ---- threads used by the parallel loops (OpenMP) ---- */
+int orc_num_threads(void);
+void orc_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,306 @@
+"""ctypes binding of the CPU oracle (oracle/ckks_oracle.c) — TEST INFRASTRUCTURE ONLY.
+
+Loaded by tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py, always as the
+checker. Arrays are numpy uint64, limb-major (one N-word row per limb), canonical residues.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libckks_oracle.so")
+_lib = None
+
+u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        V, P, U, I, D, U32 = None, C.c_void_p, C.c_uint64, C.c_int, C.c_double, C.c_uint32
+        sig = {
+            "orc_mulmod": (U, [U, U, U]),
+            "orc_powmod": (U, [U, U, U]),
+            "orc_invmod": (U, [U, U]),
+            "orc_is_prime": (I, [U]),
+            "orc_first_prime": (U, [I, U]),
+            "orc_last_prime": (U, [I, U]),
+            "orc_previous_prime": (U, [U, U]),
+            "orc_next_prime": (U, [U, U]),
+            "orc_root_of_unity": (U, [U, U]),
+            "orc_gen_primes": (I, [I, I, I, I, I, I, u64p]),
+            "orc_default_num_special": (I, [u64p, I, I, I]),
+            "orc_create": (P, [I, I, I, I, u64p]),
+            "orc_destroy": (V, [P]),
+            "orc_prime": (U, [P, I]),
+            "orc_psi": (U, [P, I]),
+            "orc_alpha": (I, [P]),
+            "orc_twiddles": (C.POINTER(C.c_uint64), [P, I]),
+            "orc_inv_twiddles": (C.POINTER(C.c_uint64), [P, I]),
+            "orc_rescale_ql_inv": (U, [P, I, I]),
+            "orc_rescale_QlQlInv": (U, [P, I, I]),
+            "orc_P_mod": (U, [P, I]),
+            "orc_P_inv_mod": (U, [P, I]),
+            "orc_ntt_fwd": (V, [P, u64p, i32p, I]),
+            "orc_ntt_inv": (V, [P, u64p, i32p, I]),
+            "orc_ntt_fwd_raw": (V, [u64p, U, U, u64p]),
+            "orc_ntt_inv_raw": (V, [u64p, U, U, u64p, U]),
+            "orc_make_tables": (V, [U, U, U, u64p, u64p]),
+            "orc_add": (V, [P, u64p, u64p, u64p, I]),
+            "orc_sub": (V, [P, u64p, u64p, u64p, I]),
+            "orc_mul": (V, [P, u64p, u64p, u64p, I]),
+            "orc_neg": (V, [P, u64p, u64p, I]),
+            "orc_mul_scalar": (V, [P, u64p, u64p, u64p, I]),
+            "orc_add_scalar": (V, [P, u64p, u64p, u64p, I]),
+            "orc_tensor": (V, [P, u64p, u64p, u64p, I]),
+            "orc_square": (V, [P, u64p, u64p, I]),
+            "orc_rescale_poly": (V, [P, u64p, I, u64p]),
+            "orc_rescale_ct": (V, [P, u64p, I, I, u64p]),
+            "orc_num_digits": (I, [P, I]),
+            "orc_modup": (V, [P, u64p, I, u64p]),
+            "orc_ksk_inner": (V, [P, u64p, I, u64p, u64p, u64p]),
+            "orc_moddown": (V, [P, u64p, I, u64p]),
+            "orc_automorph": (V, [P, u64p, I, U32, u64p]),
+            "orc_rot_index": (U32, [P, I]),
+            "orc_relinearize": (V, [P, u64p, I, u64p, u64p]),
+            "orc_eval_mult": (V, [P, u64p, u64p, I, u64p, u64p]),
+            "orc_eval_rotate": (V, [P, u64p, I, u64p, U32, u64p]),
+            "orc_eval_rotate_ext": (V, [P, u64p, I, u64p, U32, I, u64p]),
+            "orc_gen_secret": (V, [P, U, I, u64p]),
+            "orc_gen_switch_key": (V, [P, u64p, u64p, U, u64p]),
+            "orc_gen_relin_key": (V, [P, u64p, U, u64p]),
+            "orc_gen_rot_key": (V, [P, u64p, U32, U, u64p]),
+            "orc_encode": (V, [P, f64p, I, D, I, u64p]),
+            "orc_encode_ext": (V, [P, f64p, I, D, I, u64p]),
+            "orc_decode": (V, [P, u64p, I, I, D, f64p]),
+            "orc_encrypt": (V, [P, u64p, I, u64p, U, u64p]),
+            "orc_decrypt": (V, [P, u64p, I, I, u64p, u64p]),
+            "orc_random_poly": (V, [P, U, i32p, I, u64p]),
+            "orc_mt_first": (U, [U, I]),
+            "orc_num_threads": (I, []),
+            "orc_set_threads": (V, [I]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def gen_primes(logN, L1, first_bits, scale_bits, K, aux_bits=60):
+    out = np.zeros(L1 + K, dtype=np.uint64)
+    rc = lib().orc_gen_primes(logN, L1, first_bits, scale_bits, K, aux_bits, out)
+    assert rc == 0
+    return out
+
+
+class Ctx:
+    """Oracle context over the primes q_0..q_{L1-1} (Q) and p_0..p_{K-1} (P)."""
+
+    def __init__(self, logN, primes, L1, K, dnum):
+        self.logN, self.N, self.L1, self.K, self.dnum = logN, 1 << logN, L1, K, dnum
+        self.primes = np.ascontiguousarray(primes, dtype=np.uint64)
+        self.h = lib().orc_create(logN, L1, K, dnum, self.primes)
+        assert self.h
+        self.alpha = lib().orc_alpha(self.h)
+        self.psi = [int(lib().orc_psi(self.h, i)) for i in range(L1 + K)]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    # ---- helpers ----
+    def q(self, i):
+        return int(self.primes[i])
+
+    def zeros(self, nlimbs, ncomp=1):
+        return np.zeros((ncomp * nlimbs, self.N), dtype=np.uint64)
+
+    def ext_pids(self, l):
+        return list(range(l + 1)) + [self.L1 + k for k in range(self.K)]
+
+    def num_digits(self, l):
+        return lib().orc_num_digits(self.h, l)
+
+    def twiddles(self, i):
+        p = lib().orc_twiddles(self.h, i)
+        return np.ctypeslib.as_array(p, shape=(self.N,)).copy()
+
+    def inv_twiddles(self, i):
+        p = lib().orc_inv_twiddles(self.h, i)
+        return np.ctypeslib.as_array(p, shape=(self.N,)).copy()
+
+    def random_poly(self, seed, pids):
+        pids = np.ascontiguousarray(pids, dtype=np.int32)
+        out = np.zeros((len(pids), self.N), dtype=np.uint64)
+        lib().orc_random_poly(self.h, seed, pids, len(pids), out)
+        return out
+
+    # ---- transforms ----
+    def ntt(self, a, pids):
+        a = np.ascontiguousarray(a, dtype=np.uint64).copy()
+        lib().orc_ntt_fwd(self.h, a, np.ascontiguousarray(pids, dtype=np.int32), len(pids))
+        return a
+
+    def intt(self, a, pids):
+        a = np.ascontiguousarray(a, dtype=np.uint64).copy()
+        lib().orc_ntt_inv(self.h, a, np.ascontiguousarray(pids, dtype=np.int32), len(pids))
+        return a
+
+    # ---- element-wise ----
+    def _bin(self, fn, a, b):
+        out = np.zeros_like(a)
+        getattr(lib(), fn)(self.h, np.ascontiguousarray(a), np.ascontiguousarray(b), out, a.shape[0])
+        return out
+
+    def add(self, a, b):
+        return self._bin("orc_add", a, b)
+
+    def sub(self, a, b):
+        return self._bin("orc_sub", a, b)
+
+    def mul(self, a, b):
+        return self._bin("orc_mul", a, b)
+
+    def neg(self, a):
+        out = np.zeros_like(a)
+        lib().orc_neg(self.h, np.ascontiguousarray(a), out, a.shape[0])
+        return out
+
+    def mul_scalar(self, a, s):
+        out = np.zeros_like(a)
+        lib().orc_mul_scalar(self.h, np.ascontiguousarray(a), np.ascontiguousarray(s, dtype=np.uint64), out, a.shape[0])
+        return out
+
+    def add_scalar(self, a, s):
+        out = np.zeros_like(a)
+        lib().orc_add_scalar(self.h, np.ascontiguousarray(a), np.ascontiguousarray(s, dtype=np.uint64), out, a.shape[0])
+        return out
+
+    def tensor(self, ct1, ct2, l):
+        out = self.zeros(l + 1, 3)
+        lib().orc_tensor(self.h, np.ascontiguousarray(ct1), np.ascontiguousarray(ct2), out, l + 1)
+        return out
+
+    # ---- rescale / key switching ----
+    def rescale_poly(self, x, l):
+        out = self.zeros(l)
+        lib().orc_rescale_poly(self.h, np.ascontiguousarray(x), l, out)
+        return out
+
+    def rescale_ct(self, ct, l, ncomp=2):
+        out = self.zeros(l, ncomp)
+        lib().orc_rescale_ct(self.h, np.ascontiguousarray(ct), l, ncomp, out)
+        return out
+
+    def modup(self, x, l):
+        beta = self.num_digits(l)
+        out = np.zeros((beta * (l + 1 + self.K), self.N), dtype=np.uint64)
+        lib().orc_modup(self.h, np.ascontiguousarray(x), l, out)
+        return out
+
+    def ksk_inner(self, digits, l, key):
+        ext = l + 1 + self.K
+        o0 = np.zeros((ext, self.N), dtype=np.uint64)
+        o1 = np.zeros((ext, self.N), dtype=np.uint64)
+        lib().orc_ksk_inner(self.h, np.ascontiguousarray(digits), l, np.ascontiguousarray(key), o0, o1)
+        return o0, o1
+
+    def moddown(self, x, l):
+        out = self.zeros(l + 1)
+        lib().orc_moddown(self.h, np.ascontiguousarray(x), l, out)
+        return out
+
+    def automorph(self, x, k):
+        out = np.zeros_like(x)
+        lib().orc_automorph(self.h, np.ascontiguousarray(x), x.shape[0], k, out)
+        return out
+
+    def rot_index(self, r):
+        return int(lib().orc_rot_index(self.h, r))
+
+    def relinearize(self, ct3, l, key):
+        out = self.zeros(l + 1, 2)
+        lib().orc_relinearize(self.h, np.ascontiguousarray(ct3), l, np.ascontiguousarray(key), out)
+        return out
+
+    def eval_mult(self, ct1, ct2, l, key):
+        out = self.zeros(l + 1, 2)
+        lib().orc_eval_mult(self.h, np.ascontiguousarray(ct1), np.ascontiguousarray(ct2), l, np.ascontiguousarray(key), out)
+        return out
+
+    def eval_rotate(self, ct, l, key, k):
+        out = self.zeros(l + 1, 2)
+        lib().orc_eval_rotate(self.h, np.ascontiguousarray(ct), l, np.ascontiguousarray(key), k, out)
+        return out
+
+    def eval_rotate_ext(self, ct, l, key, k, add_c0=True):
+        out = np.zeros((2 * (l + 1 + self.K), self.N), dtype=np.uint64)
+        lib().orc_eval_rotate_ext(self.h, np.ascontiguousarray(ct), l, np.ascontiguousarray(key), k, int(add_c0), out)
+        return out
+
+    # ---- keys / encoding / encryption ----
+    def gen_secret(self, seed, h=0):
+        s = self.zeros(self.L1 + self.K)
+        lib().orc_gen_secret(self.h, seed, h, s)
+        return s
+
+    def key_shape(self):
+        return (self.dnum * 2 * (self.L1 + self.K), self.N)
+
+    def gen_relin_key(self, s, seed):
+        key = np.zeros(self.key_shape(), dtype=np.uint64)
+        lib().orc_gen_relin_key(self.h, s, seed, key)
+        return key
+
+    def gen_rot_key(self, s, k, seed):
+        key = np.zeros(self.key_shape(), dtype=np.uint64)
+        lib().orc_gen_rot_key(self.h, s, k, seed, key)
+        return key
+
+    def gen_switch_key(self, s_old, s_new, seed):
+        key = np.zeros(self.key_shape(), dtype=np.uint64)
+        lib().orc_gen_switch_key(self.h, np.ascontiguousarray(s_old), np.ascontiguousarray(s_new), seed, key)
+        return key
+
+    def encode(self, z, scale, l):
+        z = np.asarray(z, dtype=np.complex128)
+        flat = np.ascontiguousarray(np.stack([z.real, z.imag], axis=1).reshape(-1))
+        out = self.zeros(l + 1)
+        lib().orc_encode(self.h, flat, len(z), scale, l, out)
+        return out
+
+    def encode_ext(self, z, scale, l):
+        z = np.asarray(z, dtype=np.complex128)
+        flat = np.ascontiguousarray(np.stack([z.real, z.imag], axis=1).reshape(-1))
+        out = self.zeros(l + 1 + self.K)
+        lib().orc_encode_ext(self.h, flat, len(z), scale, l, out)
+        return out
+
+    def decode(self, m, l, n, scale):
+        out = np.zeros(2 * n, dtype=np.float64)
+        lib().orc_decode(self.h, np.ascontiguousarray(m), l, n, scale, out)
+        return out[0::2] + 1j * out[1::2]
+
+    def encrypt(self, m, l, s, seed):
+        ct = self.zeros(l + 1, 2)
+        lib().orc_encrypt(self.h, np.ascontiguousarray(m), l, s, seed, ct)
+        return ct
+
+    def decrypt(self, ct, l, s, ncomp=2):
+        m = self.zeros(l + 1)
+        lib().orc_decrypt(self.h, np.ascontiguousarray(ct), l, ncomp, s, m)
+        return m

@@ -1,0 +1,213 @@
+"""Rescale, ModUp, key inner product, ModDown, automorphism, EvalMult and EvalRotate on the GPU
+against the CPU oracle, bit-exact on every limb (plus decrypted-slot checks for the full ops)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _ids(o, l):
+    return list(range(l + 1))
+
+
+@pytest.mark.parametrize("cfgname,level,ncomp", [("cfg1", 3, 2), ("cfg1", 1, 3), ("cfg2", 24, 2), ("cfg2", 9, 2)])
+def test_rescale(cfgname, level, ncomp, gctx1, gctx2, octx1, octx2):
+    g, o = (gctx1, octx1) if cfgname == "cfg1" else (gctx2, octx2)
+    x = np.concatenate([o.random_poly(40 + k, _ids(o, level)) for k in range(ncomp)])
+    d = g.to_device(x)
+    out = g.array((ncomp * level, o.N))
+    g.rescale(d, level, ncomp, out)
+    assert np.array_equal(out.download(), o.rescale_ct(x, level, ncomp))
+    d.free(); out.free()
+
+
+@pytest.mark.parametrize("cfgname,level", [("cfg1", 3), ("cfg1", 2), ("cfg1", 0), ("cfg2", 24), ("cfg2", 17), ("cfg2", 8), ("cfg2", 3)])
+def test_modup(cfgname, level, gctx1, gctx2, octx1, octx2):
+    g, o = (gctx1, octx1) if cfgname == "cfg1" else (gctx2, octx2)
+    x = o.random_poly(50 + level, _ids(o, level))
+    beta = o.num_digits(level)
+    assert g.num_digits(level) == beta
+    d = g.to_device(x)
+    dig = g.array((beta * (level + 1 + o.K), o.N))
+    g.modup(d, level, dig)
+    assert np.array_equal(dig.download(), o.modup(x, level))
+    d.free(); dig.free()
+
+
+@pytest.mark.parametrize("cfgname,level,galois,with_c0", [("cfg1", 3, 1, False), ("cfg1", 2, 5, True), ("cfg2", 24, 25, True), ("cfg2", 13, 1, False)])
+def test_key_inner_product(cfgname, level, galois, with_c0, gctx1, gctx2, octx1, octx2):
+    g, o = (gctx1, octx1) if cfgname == "cfg1" else (gctx2, octx2)
+    ext = level + 1 + o.K
+    beta = o.num_digits(level)
+    pids = o.ext_pids(level)
+    digits = np.concatenate([o.random_poly(60 + d, pids) for d in range(beta)])
+    full = list(range(o.L1 + o.K))
+    key = np.concatenate([o.random_poly(70 + i, full) for i in range(2 * o.dnum)])
+    c0 = o.random_poly(80, _ids(o, level))
+    dk = g.key(key)
+    dd = g.to_device(digits)
+    dc0 = g.to_device(c0)
+    o0, o1 = g.array((ext, o.N)), g.array((ext, o.N))
+    g.key_inner_product(dd, level, dk, o0, o1, c0=dc0 if with_c0 else None, galois=galois)
+    e0, e1 = o.ksk_inner(digits, level, key)
+    if with_c0:
+        Pm = np.array([int(o.primes[i]) for i in range(level + 1)], dtype=object)
+        from oracle.oracle import lib
+        pmod = [int(lib().orc_P_mod(o.h, i)) for i in range(level + 1)]
+        e0 = e0.copy()
+        e0[: level + 1] = o.add(e0[: level + 1], o.mul_scalar(c0, np.array(pmod, dtype=np.uint64)))
+    if galois != 1:
+        e0, e1 = o.automorph(e0, galois), o.automorph(e1, galois)
+    assert np.array_equal(o0.download(), e0)
+    assert np.array_equal(o1.download(), e1)
+    for x in (dd, dc0, o0, o1):
+        x.free()
+    dk.free()
+
+
+@pytest.mark.parametrize("cfgname,level,npolys", [("cfg1", 3, 2), ("cfg1", 0, 1), ("cfg2", 24, 2), ("cfg2", 11, 1)])
+def test_moddown(cfgname, level, npolys, gctx1, gctx2, octx1, octx2):
+    g, o = (gctx1, octx1) if cfgname == "cfg1" else (gctx2, octx2)
+    ext = level + 1 + o.K
+    x = np.concatenate([o.random_poly(90 + k, o.ext_pids(level)) for k in range(npolys)])
+    d = g.to_device(x)
+    out = g.array((npolys * (level + 1), o.N))
+    g.moddown(d, level, npolys, out)
+    exp = np.concatenate([o.moddown(x[k * ext:(k + 1) * ext], level) for k in range(npolys)])
+    assert np.array_equal(out.download(), exp)
+    d.free(); out.free()
+
+
+@pytest.mark.parametrize("rot", [1, -1, 7, 1000, "conj"])
+def test_automorph(rot, gctx1, octx1):
+    g, o = gctx1, octx1
+    k = 2 * o.N - 1 if rot == "conj" else o.rot_index(rot)
+    if rot != "conj":
+        assert g.galois_index(rot) == k
+    x = o.random_poly(5, list(range(o.L1)))
+    d, out = g.to_device(x), g.array(x.shape)
+    g.automorph(d, o.L1, k, out)
+    assert np.array_equal(out.download(), o.automorph(x, k))
+    d.free(); out.free()
+
+
+def _keys(o, seed=1):
+    s = o.gen_secret(seed)
+    return s
+
+
+def _slots(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+
+
+@pytest.fixture(scope="module")
+def cfg1_material(octx1):
+    o = octx1
+    s = o.gen_secret(11)
+    relin = o.gen_relin_key(s, 12)
+    k1 = o.rot_index(1)
+    rot1 = o.gen_rot_key(s, k1, 13)
+    n = o.N // 2
+    z1, z2 = _slots(n, 1), _slots(n, 2)
+    l = o.L1 - 1
+    ct1 = o.encrypt(o.encode(z1, 2.0 ** 50, l), l, s, 21)
+    ct2 = o.encrypt(o.encode(z2, 2.0 ** 50, l), l, s, 22)
+    return dict(s=s, relin=relin, rot1=rot1, k1=k1, z1=z1, z2=z2, ct1=ct1, ct2=ct2, l=l)
+
+
+def test_cfg1_mult_rescale_rotate_chain(gctx1, octx1, cfg1_material):
+    """cfg1 pipeline: EvalMult + relinearize -> Rescale -> EvalRotate(+1); every limb bit-exact,
+    decrypted slots equal z1*z2 rotated by one within 2^-25."""
+    g, o, m = gctx1, octx1, cfg1_material
+    l = m["l"]
+    d1, d2 = g.to_device(m["ct1"]), g.to_device(m["ct2"])
+    krel, krot = g.key(m["relin"]), g.key(m["rot1"])
+    prod = g.array((2 * (l + 1), o.N))
+    g.eval_mult(d1, d2, l, krel, prod)
+    exp_prod = o.eval_mult(m["ct1"], m["ct2"], l, m["relin"])
+    assert np.array_equal(prod.download(), exp_prod)
+    rs = g.array((2 * l, o.N))
+    g.rescale(prod, l, 2, rs)
+    exp_rs = o.rescale_ct(exp_prod, l, 2)
+    assert np.array_equal(rs.download(), exp_rs)
+    rot = g.array((2 * l, o.N))
+    g.eval_rotate(rs, l - 1, krot, m["k1"], rot)
+    exp_rot = o.eval_rotate(exp_rs, l - 1, m["rot1"], m["k1"])
+    got = rot.download()
+    assert np.array_equal(got, exp_rot)
+    # decrypt: scale after rescale = 2^100 / q_l
+    scale = 2.0 ** 100 / float(o.primes[l])
+    dec = o.decode(o.decrypt(got, l - 1, m["s"]), l - 1, o.N // 2, scale)
+    want = np.roll(m["z1"] * m["z2"], -1)
+    err = np.max(np.abs(dec - want))
+    assert err < 2.0 ** -25, err
+    for x in (d1, d2, prod, rs, rot):
+        x.free()
+    krel.free(); krot.free()
+
+
+def test_relinearize_degree2(gctx1, octx1, cfg1_material):
+    g, o, m = gctx1, octx1, cfg1_material
+    l = m["l"]
+    t = o.tensor(m["ct1"], m["ct2"], l)
+    dt = g.to_device(t)
+    k = g.key(m["relin"])
+    out = g.array((2 * (l + 1), o.N))
+    g.relinearize(dt, l, k, out)
+    assert np.array_equal(out.download(), o.relinearize(t, l, m["relin"]))
+    dt.free(); out.free(); k.free()
+
+
+def test_rotate_ext_matches(gctx1, octx1, cfg1_material):
+    g, o, m = gctx1, octx1, cfg1_material
+    l = m["l"]
+    d = g.to_device(m["ct1"])
+    k = g.key(m["rot1"])
+    ext = l + 1 + o.K
+    for add_c0 in (True, False):
+        out = g.array((2 * ext, o.N))
+        g.eval_rotate_ext(d, l, k, m["k1"], out, add_c0=add_c0)
+        assert np.array_equal(out.download(), o.eval_rotate_ext(m["ct1"], l, m["rot1"], m["k1"], add_c0))
+        out.free()
+    d.free(); k.free()
+
+
+@pytest.fixture(scope="module")
+def cfg2_material(octx2):
+    o = octx2
+    s = o.gen_secret(31)
+    return dict(s=s, relin=o.gen_relin_key(s, 32))
+
+
+@pytest.mark.parametrize("level", [24, 12])
+def test_cfg2_eval_mult(level, gctx2, octx2, cfg2_material):
+    g, o, m = gctx2, octx2, cfg2_material
+    ids = _ids(o, level)
+    ct1 = np.concatenate([o.random_poly(200 + level, ids), o.random_poly(201 + level, ids)])
+    ct2 = np.concatenate([o.random_poly(202 + level, ids), o.random_poly(203 + level, ids)])
+    d1, d2 = g.to_device(ct1), g.to_device(ct2)
+    k = g.key(m["relin"])
+    out = g.array((2 * (level + 1), o.N))
+    g.eval_mult(d1, d2, level, k, out)
+    assert np.array_equal(out.download(), o.eval_mult(ct1, ct2, level, m["relin"]))
+    d1.free(); d2.free(); out.free(); k.free()
+
+
+@pytest.mark.parametrize("rot", [1, 64])
+def test_cfg2_eval_rotate_decrypts(rot, gctx2, octx2, cfg2_material):
+    g, o, m = gctx2, octx2, cfg2_material
+    level = 24
+    kidx = o.rot_index(rot)
+    rk = o.gen_rot_key(m["s"], kidx, 40 + rot)
+    z = _slots(o.N // 2, 5)
+    ct = o.encrypt(o.encode(z, 2.0 ** 50, level), level, m["s"], 41)
+    d = g.to_device(ct)
+    k = g.key(rk)
+    out = g.array(ct.shape)
+    g.eval_rotate(d, level, k, kidx, out)
+    got = out.download()
+    assert np.array_equal(got, o.eval_rotate(ct, level, rk, kidx))
+    dec = o.decode(o.decrypt(got, level, m["s"]), level, o.N // 2, 2.0 ** 50)
+    assert np.max(np.abs(dec - np.roll(z, -rot))) < 2.0 ** -30
+    d.free(); out.free(); k.free()

@@ -1,0 +1,344 @@
+"""Pins the CPU oracle (oracle/ckks_oracle.c) against the reference's known-answer vector, closed
+forms computed here with Python big integers, and decryption invariants. CPU only."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+def is_prime(n):
+    if n < 2:
+        return False
+    for p in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        if n % p == 0:
+            return n == p
+    d, s = n - 1, 0
+    while d % 2 == 0:
+        d //= 2
+        s += 1
+    for a in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        x = pow(a, d, n)
+        if x in (1, n - 1):
+            continue
+        for _ in range(s - 1):
+            x = x * x % n
+            if x == n - 1:
+                break
+        else:
+            return False
+    return True
+
+
+def brev(x, bits):
+    return int(format(x, "0%db" % bits)[::-1], 2)
+
+
+def small_ctx(logN=10, L1=4, K=2, dnum=2):
+    primes = O.gen_primes(logN, L1, 60, 50, K)
+    return O.Ctx(logN, primes, L1, K, dnum)
+
+
+@pytest.fixture(scope="module")
+def sc():
+    return small_ctx()
+
+
+def test_mt19937_64_stream():
+    # the 10000th output of std::mt19937_64 with the default seed (C++ standard, [rand.predef])
+    assert O.lib().orc_mt_first(5489, 9999) == 9981545732273789042
+
+
+@pytest.mark.parametrize("logN,L1,K", [(13, 4, 2), (16, 25, 9)])
+def test_prime_chain_rule(logN, L1, K):
+    m = 2 << logN
+    p = [int(x) for x in O.gen_primes(logN, L1, 60, 50, K)]
+    assert all(is_prime(x) and x % m == 1 for x in p)
+    assert len(set(p)) == len(p)
+    # top of the chain: the first prime = 1 mod 2N above 2^50
+    top = p[L1 - 1]
+    assert top > 2 ** 50 and all(not is_prime(c) for c in range(2 ** 50 + 1, top, m))
+    # q_0: the last prime = 1 mod 2N below 2^60
+    assert p[0] < 2 ** 60 and all(not is_prime(c) for c in range(p[0] + m, 2 ** 60, m))
+    # alternation: PreviousPrime / NextPrime walking down from the top
+    lo, hi = top, top
+    for cnt, i in enumerate(range(L1 - 2, 0, -1)):
+        if cnt % 2 == 0:
+            c = lo - m
+            while not is_prime(c):
+                c -= m
+            lo = c
+        else:
+            c = hi + m
+            while not is_prime(c):
+                c += m
+            hi = c
+        assert p[i] == c
+    # special primes: descending primes = 1 mod 2N below 2^60, skipping the Q primes
+    c = 2 ** 60 + 1
+    got = []
+    while len(got) < K:
+        c -= m
+        if is_prime(c) and c not in p[:L1]:
+            got.append(c)
+    assert p[L1:] == got
+
+
+def test_default_special_count():
+    p = O.gen_primes(13, 4, 60, 50, 0)
+    assert O.lib().orc_default_num_special(np.ascontiguousarray(p[:4]), 4, 2, 60) == 2
+    p2 = O.gen_primes(16, 25, 60, 50, 0)
+    digit_bits = max(int(np.prod([int(x) for x in p2[d * 9:(d + 1) * 9]], dtype=object)).bit_length() for d in range(3))
+    assert O.lib().orc_default_num_special(np.ascontiguousarray(p2[:25]), 25, 3, 60) == -(-digit_bits // 60)
+
+
+def test_root_of_unity_is_minimal_primitive():
+    for q, m in ((17, 16), (97, 32), (12289, 2048)):
+        r = int(O.lib().orc_root_of_unity(m, q))
+        prim = [x for x in range(2, q) if pow(x, m // 2, q) == q - 1 and pow(x, m, q) == 1]
+        assert r == min(prim)
+    c = small_ctx(11, 2, 1, 1)
+    for i in range(3):
+        q, psi, m = c.q(i), c.psi[i], 2 * c.N
+        assert pow(psi, m // 2, q) == q - 1
+        w = psi
+        for k in range(3, 200, 2):  # no smaller root among a prefix of odd powers
+            w = w * psi * psi % q
+            assert pow(psi, k, q) >= psi
+
+
+def test_kat_q17_n8():
+    """Reference known-answer case (test/NegacyclicNTTtests.cu:31-122): q=17, N=8,
+    v = {1,2,3,7,5,4,1,2}: round trip, and NTT product == schoolbook negacyclic product."""
+    q, N = 17, 8
+    psi = int(O.lib().orc_root_of_unity(2 * N, q))
+    assert psi == 3
+    tw, itw = np.zeros(N, np.uint64), np.zeros(N, np.uint64)
+    O.lib().orc_make_tables(N, q, psi, tw, itw)
+    v = np.array([1, 2, 3, 7, 5, 4, 1, 2], dtype=np.uint64)
+    a = v.copy()
+    O.lib().orc_ntt_fwd_raw(a, N, q, tw)
+    # closed form: A[j] = sum_i v_i psi^((2 brev(j)+1) i)
+    for j in range(N):
+        e = 2 * brev(j, 3) + 1
+        assert int(a[j]) == sum(int(v[i]) * pow(psi, e * i, q) for i in range(N)) % q
+    b = a.copy()
+    O.lib().orc_ntt_inv_raw(b, N, q, itw, pow(N, q - 2, q))
+    assert np.array_equal(b, v)
+    w = np.array([3, 0, 16, 1, 1, 5, 2, 9], dtype=np.uint64)
+    wa = w.copy()
+    O.lib().orc_ntt_fwd_raw(wa, N, q, tw)
+    prod = (a * wa) % q
+    O.lib().orc_ntt_inv_raw(prod, N, q, itw, pow(N, q - 2, q))
+    school = [0] * N
+    for i in range(N):
+        for k in range(N):
+            t = int(v[i]) * int(w[k])
+            if i + k < N:
+                school[i + k] += t
+            else:
+                school[i + k - N] -= t
+    assert [int(x) for x in prod] == [s % q for s in school]
+
+
+def test_ntt_closed_form_and_convolution(sc):
+    c = sc
+    N = c.N
+    for pid in (0, 1, c.L1):  # 60-bit, 50-bit and special prime
+        q, psi = c.q(pid), c.psi[pid]
+        x = c.random_poly(3 + pid, [pid])
+        X = c.ntt(x, [pid])
+        xi = [int(t) for t in x[0]]
+        for j in (0, 1, 2, 3, N // 2, N - 1, 777):
+            w = pow(psi, 2 * brev(j, c.logN) + 1, q)
+            acc, p = 0, 1
+            for i in range(N):
+                acc += xi[i] * p
+                p = p * w % q
+            assert acc % q == int(X[0, j])
+        assert np.array_equal(c.intt(X, [pid]), x)
+    # negacyclic convolution: NTT(a) * NTT(b) -> a * b mod X^N + 1 (small operands)
+    rng = np.random.default_rng(0)
+    a = rng.integers(-50, 50, N)
+    b = rng.integers(-50, 50, N)
+    full = np.convolve(a, b)
+    neg = full[:N].copy()
+    neg[: N - 1] -= full[N:]
+    q = c.q(1)
+    A = c.ntt(np.array([a % q], dtype=np.uint64), [1])
+    B = c.ntt(np.array([b % q], dtype=np.uint64), [1])
+    Pm = c.intt(c.mul(np.vstack([A, A]), np.vstack([B, B]))[1:2], [1])  # limb 1 = prime 1
+    assert [int(t) for t in Pm[0]] == [int(t) % q for t in neg]
+
+
+def crt(res, mods):
+    M = 1
+    for m in mods:
+        M *= m
+    x = 0
+    for r, m in zip(res, mods):
+        Mi = M // m
+        x += int(r) * Mi * pow(Mi, -1, m)
+    return x % M, M
+
+
+def test_modup_closed_form(sc):
+    c = sc
+    N, K = c.N, c.K
+    for l in (3, 2, 0):
+        x = c.random_poly(10 + l, list(range(l + 1)))
+        dig = c.modup(x, l)
+        ext = l + 1 + K
+        xc = c.intt(x, list(range(l + 1)))
+        pids = c.ext_pids(l)
+        for d in range(c.num_digits(l)):
+            start, end = d * c.alpha, min((d + 1) * c.alpha, l + 1)
+            blk = dig[d * ext:(d + 1) * ext]
+            assert np.array_equal(blk[start:end], x[start:end])  # own limbs untouched (EVAL)
+            blkc = c.intt(blk, pids)
+            qs = [c.q(i) for i in range(start, end)]
+            Qd = 1
+            for qq in qs:
+                Qd *= qq
+            for n in (0, 1, N // 3, N - 1):
+                # fast base conversion: V = sum_i [x_i (Qd/q_i)^-1]_{q_i} (Qd/q_i)
+                V = 0
+                for i, qq in zip(range(start, end), qs):
+                    Qh = Qd // qq
+                    V += (int(xc[i, n]) * pow(Qh, -1, qq) % qq) * Qh
+                X, _ = crt([xc[i, n] for i in range(start, end)], qs)
+                assert (V - X) % Qd == 0 and 0 <= (V - X) // Qd < len(qs)
+                for pos in range(ext):
+                    if start <= pos < end:
+                        continue
+                    assert int(blkc[pos, n]) == V % c.q(pids[pos])
+
+
+def test_moddown_closed_form(sc):
+    c = sc
+    N, K, L1 = c.N, c.K, c.L1
+    for l in (3, 1):
+        pids = c.ext_pids(l)
+        x = c.random_poly(20 + l, pids)
+        out = c.moddown(x, l)
+        xc = c.intt(x, pids)
+        oc = c.intt(out, list(range(l + 1)))
+        ps = [c.q(L1 + k) for k in range(K)]
+        P = 1
+        for p in ps:
+            P *= p
+        for n in (0, 5, N - 1):
+            V = 0
+            for k, p in enumerate(ps):
+                Ph = P // p
+                V += (int(xc[l + 1 + k, n]) * pow(Ph, -1, p) % p) * Ph
+            for i in range(l + 1):
+                qi = c.q(i)
+                assert int(oc[i, n]) == (int(xc[i, n]) - V) * pow(P, -1, qi) % qi
+        # approximate division: ModDown(X) = round-ish(X / P) within K
+        for n in (2, 9):
+            X, M = crt([xc[j, n] for j in range(len(pids))], [c.q(p) for p in pids])
+            Y, Mq = crt([oc[i, n] for i in range(l + 1)], [c.q(i) for i in range(l + 1)])
+            diff = (Y - X // P) % Mq
+            assert min(diff, Mq - diff) <= K
+
+
+def test_rescale_closed_form(sc):
+    c = sc
+    L1 = c.L1
+    for l in (3, 2, 1):
+        # OpenFHE QlQlInvModqlDivqlModq: floor([Q_{l-1}^-1]_{q_l} Q_{l-1} / q_l) mod q_i
+        Q = 1
+        for i in range(l):
+            Q *= c.q(i)
+        R = (pow(Q, -1, c.q(l)) * Q) // c.q(l)
+        for i in range(l):
+            assert int(O.lib().orc_rescale_QlQlInv(c.h, l, i)) == R % c.q(i)
+            assert int(O.lib().orc_rescale_ql_inv(c.h, l, i)) == pow(c.q(l), -1, c.q(i))
+        x = c.random_poly(30 + l, list(range(l + 1)))
+        out = c.rescale_poly(x, l)
+        xc = c.intt(x, list(range(l + 1)))
+        oc = c.intt(out, list(range(l)))
+        ql = c.q(l)
+        for n in (0, 1, c.N - 1, 100):
+            X, M = crt([xc[i, n] for i in range(l + 1)], [c.q(i) for i in range(l + 1)])
+            r = X % ql
+            if r > ql // 2:
+                r -= ql
+            Y = (X - r) // ql
+            for i in range(l):
+                assert int(oc[i, n]) == Y % c.q(i)
+
+
+def test_automorph_coefficient_definition(sc):
+    c = sc
+    N = c.N
+    a = np.random.default_rng(4).integers(-5, 6, N)
+    q = c.q(0)
+    A = c.ntt(np.array([a % q], dtype=np.uint64), [0])
+    for k in (5, c.rot_index(3), c.rot_index(-2), 2 * N - 1):
+        got = c.intt(c.automorph(A, k), [0])[0]
+        exp = [0] * N
+        for i in range(N):
+            e = i * k % (2 * N)
+            if e < N:
+                exp[e] += int(a[i])
+            else:
+                exp[e - N] -= int(a[i])
+        assert [int(t) for t in got] == [t % q for t in exp]
+    assert c.rot_index(0) == 1
+    assert c.rot_index(1) == 5 and c.rot_index(2) == 25
+    assert c.rot_index(-1) * 5 % (2 * N) == 1
+
+
+def _slots(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+
+
+def test_encode_decode_encrypt_roundtrip(sc):
+    c = sc
+    n = c.N // 2
+    z = _slots(n, 1)
+    l = c.L1 - 1
+    m = c.encode(z, 2.0 ** 40, l)
+    assert np.max(np.abs(c.decode(m, l, n, 2.0 ** 40) - z)) < 1e-9
+    s = c.gen_secret(3)
+    ct = c.encrypt(m, l, s, 4)
+    assert np.max(np.abs(c.decode(c.decrypt(ct, l, s), l, n, 2.0 ** 40) - z)) < 1e-8
+    # sparse packing
+    z8 = _slots(8, 2)
+    m8 = c.encode(z8, 2.0 ** 40, l)
+    assert np.max(np.abs(c.decode(m8, l, 8, 2.0 ** 40) - z8)) < 1e-9
+
+
+def test_keyswitch_decrypt_invariants():
+    """relinearized product and hoisted rotations decrypt to the expected slots (cfg1 shape)."""
+    c = small_ctx(13, 4, 2, 2)
+    n, l, Dl = c.N // 2, 3, 2.0 ** 50
+    s = c.gen_secret(7)
+    z1, z2 = _slots(n, 8), _slots(n, 9)
+    ct1 = c.encrypt(c.encode(z1, Dl, l), l, s, 10)
+    ct2 = c.encrypt(c.encode(z2, Dl, l), l, s, 11)
+    rk = c.gen_relin_key(s, 12)
+    prod = c.eval_mult(ct1, ct2, l, rk)
+    dec = c.decode(c.decrypt(prod, l, s), l, n, Dl * Dl)
+    assert np.max(np.abs(dec - z1 * z2)) < 1e-9
+    # the degree-2 tensor decrypts the same (relinearization only adds key-switch noise)
+    t = c.tensor(ct1, ct2, l)
+    dec3 = c.decode(c.decrypt(t, l, s, 3), l, n, Dl * Dl)
+    assert np.max(np.abs(dec3 - z1 * z2)) < 1e-9
+    rs = c.rescale_ct(prod, l)
+    dec = c.decode(c.decrypt(rs, l - 1, s), l - 1, n, Dl * Dl / float(c.q(l)))
+    assert np.max(np.abs(dec - z1 * z2)) < 1e-7
+    for r in (1, -3, 17):
+        k = c.rot_index(r)
+        key = c.gen_rot_key(s, k, 100 + r)
+        rot = c.eval_rotate(ct1, l, key, k)
+        dec = c.decode(c.decrypt(rot, l, s), l, n, Dl)
+        assert np.max(np.abs(dec - np.roll(z1, -r))) < 1e-9
+    # conjugation
+    kc = 2 * c.N - 1
+    key = c.gen_rot_key(s, kc, 200)
+    conj = c.eval_rotate(ct1, l, key, kc)
+    dec = c.decode(c.decrypt(conj, l, s), l, n, Dl)
+    assert np.max(np.abs(dec - np.conj(z1))) < 1e-9
