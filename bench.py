@@ -1,0 +1,265 @@
+#!/usr/bin/env python3
+"""Benchmark of the RNS-CKKS hot path on one MI355X.
+
+Workload (the production single-op configuration): ring dimension N = 2^16, 25 Q limbs
+(one 60-bit + 24 x 50-bit primes), dnum = 3 (9 special primes), a batch of 64 ciphertexts at the
+top level, 16 keyed rotation steps. One step rotates every ciphertext of the batch by a step drawn
+(seeded) from the 16 keyed steps through the batched hybrid key switch (fh_rotate_batch). Inputs
+and keys are synthetic uniform residues generated on the device; the same batch is also run one
+rotation at a time (fh_eval_rotate) and the two results are compared bit for bit.
+
+Also reported: NTT GB/s (forward + inverse over all 64 x 2 x 25 limbs), EvalMult+Rescale ops/s,
+the roofline of the dominant kernel (HIP-event timing of its launches inside the timed region) and
+the CPU oracle timed on a bounded sample of the same rotation workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fideslib-fork_amd"))
+
+METRIC = "HKS rotations/s N=2^16 L=24, batched vs per-op, bit-exact; NTT GB/s; p99 ms"
+HBM_PEAK_GBS = 8000.0
+LOGN, L1, K, DNUM = 16, 25, 9, 3
+STEPS16 = [1 << i for i in range(15)] + [3]
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU oracle sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def algorithmic_bytes(family, N, lq, ext, beta, B, groups):
+    """HBM bytes one launch of a kernel family must move at this shape (8-byte words)."""
+    limb = 8 * N
+    if family.startswith("ntt_"):
+        return None  # depends on the launch; handled by the caller
+    if family == "ksk_batch":
+        digits = B * beta * ext
+        keys = groups * 2 * beta * ext
+        return limb * (digits + keys + B * lq + 2 * B * ext)
+    if family == "modup_conv":
+        return limb * B * (2 * lq + beta * ext - lq)
+    if family == "moddown_conv":
+        return limb * 2 * B * (K + lq)
+    if family == "moddown_final":
+        return limb * 2 * B * 3 * lq
+    return None
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": a.gpus}))
+        sys.exit(2)
+    import fides_hip as fh
+
+    ctx = fh.Context(LOGN, L1, K, DNUM)
+    N, B, lvl = ctx.N, a.batch, L1 - 1
+    lq, ext, beta = lvl + 1, lvl + 1 + K, ctx.num_digits(lvl)
+    ct_words = 2 * lq * N
+    rng = np.random.default_rng(a.seed)
+
+    # ---- synthetic inputs, resident in HBM ----
+    qids = list(range(lq))
+    cts = ctx.array((B * 2 * lq, N))
+    ctx.fill_uniform(cts, qids, num_polys=2 * B, seed=a.seed)
+    full = list(range(L1 + K))
+    gal, dev_keys = [], []
+    kbuf = ctx.array((DNUM * 2 * (L1 + K), N))
+    for i, r in enumerate(STEPS16):
+        ctx.fill_uniform(kbuf, full, num_polys=DNUM * 2, seed=a.seed + 100 + i)
+        dev_keys.append(ctx.key_from_device(kbuf))
+        gal.append(ctx.galois_index(r))
+    kbuf.free()
+    out_b = ctx.array((B * 2 * lq, N))
+    out_p = ctx.array((B * 2 * lq, N))
+
+    def choose(step):
+        r = np.random.default_rng(a.seed * 1000 + step).integers(0, len(STEPS16), B)
+        return [dev_keys[i] for i in r], [gal[i] for i in r], len(set(r.tolist()))
+
+    # ---- bit-exactness: batched == per-op on the same inputs ----
+    ks, gs, _ = choose(0)
+    ctx.rotate_batch(cts, B, lvl, ks, gs, out_b)
+    for b in range(B):
+        ctx_ct = _view(fh, ctx, cts, b * ct_words, (2 * lq, N))
+        ctx_out = _view(fh, ctx, out_p, b * ct_words, (2 * lq, N))
+        ctx.eval_rotate(ctx_ct, lvl, ks[b], gs[b], ctx_out)
+    ctx.sync()
+    check = min(B, 8)
+    xb = out_b.download()[: check * 2 * lq]
+    xp = out_p.download()[: check * 2 * lq]
+    bit_exact = bool(np.array_equal(xb, xp))
+
+    # ---- timed: batched rotations ----
+    for w in range(a.warmup):
+        ks, gs, _ = choose(w + 1)
+        ctx.rotate_batch(cts, B, lvl, ks, gs, out_b)
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_reset()
+    groups_total = 0
+    t0 = time.perf_counter()
+    for s in range(a.steps):
+        ks, gs, ng = choose(1000 + s)
+        groups_total += ng
+        ctx.rotate_batch(cts, B, lvl, ks, gs, out_b)
+    ctx.sync()
+    t_batched = time.perf_counter() - t0
+    fams = ["ntt_inv_rows", "ntt_inv_cols", "ntt_fwd_cols", "ntt_fwd_rows", "modup_conv", "ksk_batch",
+            "gather_special", "moddown_conv", "moddown_final"]
+    prof = {f: ctx.profile_read(f) for f in fams}
+    ctx.profile(False)
+    ms_per_step = 1000.0 * t_batched / a.steps
+    rot_per_s = B * a.steps / t_batched
+
+    # ---- per-op rotations (same inputs, one call per ciphertext) ----
+    per_steps = max(1, min(a.steps, 3))
+    for s in range(1):
+        ks, gs, _ = choose(2000)
+        for b in range(B):
+            ctx.eval_rotate(_view(fh, ctx, cts, b * ct_words, (2 * lq, N)), lvl, ks[b], gs[b],
+                            _view(fh, ctx, out_p, b * ct_words, (2 * lq, N)))
+    ctx.sync()
+    t0 = time.perf_counter()
+    for s in range(per_steps):
+        ks, gs, _ = choose(3000 + s)
+        for b in range(B):
+            ctx.eval_rotate(_view(fh, ctx, cts, b * ct_words, (2 * lq, N)), lvl, ks[b], gs[b],
+                            _view(fh, ctx, out_p, b * ct_words, (2 * lq, N)))
+    ctx.sync()
+    per_op_rot_s = B * per_steps / (time.perf_counter() - t0)
+
+    # ---- NTT GB/s: forward + inverse over all 64 x 2 x 25 limbs ----
+    ctx.ntt_forward(out_b, qids, num_polys=2 * B)
+    ctx.ntt_inverse(out_b, qids, num_polys=2 * B)
+    ctx.sync()
+    nrep = 5
+    ctx.profile(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(nrep):
+        ctx.ntt_forward(out_b, qids, num_polys=2 * B)
+        ctx.ntt_inverse(out_b, qids, num_polys=2 * B)
+    ctx.sync()
+    t_ntt = (time.perf_counter() - t0) / nrep
+    nprof = {f: ctx.profile_read(f) for f in ("ntt_fwd_cols", "ntt_fwd_rows", "ntt_inv_rows", "ntt_inv_cols")}
+    ctx.profile(False)
+    ntt_limbs = 2 * B * lq
+    ntt_bytes = 2 * ntt_limbs * 16 * N  # forward + inverse, 16N bytes per limb-transform
+    ntt_gbs = ntt_bytes / t_ntt / 1e9
+
+    # ---- EvalMult + Rescale (relinearization key = first rotation key's buffer shape) ----
+    prod = ctx.array((2 * lq, N))
+    rs = ctx.array((2 * (lq - 1), N))
+    ctx.eval_mult(_view(fh, ctx, cts, 0, (2 * lq, N)), _view(fh, ctx, cts, ct_words, (2 * lq, N)), lvl, dev_keys[0], prod)
+    ctx.sync()
+    nmul = 16
+    t0 = time.perf_counter()
+    for i in range(nmul):
+        x = _view(fh, ctx, cts, (2 * i % B) * ct_words, (2 * lq, N))
+        y = _view(fh, ctx, cts, ((2 * i + 1) % B) * ct_words, (2 * lq, N))
+        ctx.eval_mult(x, y, lvl, dev_keys[0], prod)
+        ctx.rescale(prod, lvl, 2, rs)
+    ctx.sync()
+    mult_rescale_s = nmul / (time.perf_counter() - t0)
+
+    # ---- roofline of the dominant kernel of the timed rotation step ----
+    dom = max(prof, key=lambda f: prof[f][1])
+    launches, tot = prof[dom]
+    avg_ms = tot / max(launches, 1)
+    groups_avg = groups_total / a.steps
+    if dom.startswith("ntt_"):
+        # per launch: every limb of the launch read and written once
+        per_step_limbs = {"ntt_inv_rows": B * lq + 2 * B * K, "ntt_inv_cols": B * lq + 2 * B * K,
+                          "ntt_fwd_cols": B * (beta * ext - lq) + 2 * B * lq, "ntt_fwd_rows": B * (beta * ext - lq) + 2 * B * lq}
+        alg = 16 * N * per_step_limbs[dom] * a.steps / max(launches, 1)
+    else:
+        alg = algorithmic_bytes(dom, N, lq, ext, beta, B, groups_avg)
+    achieved = alg / (avg_ms * 1e-3) / 1e9 if alg else None
+    roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": None,
+            "avg_launch_ms": round(avg_ms, 4), "launches": launches,
+            "kernel_ms_per_step": {f: round(prof[f][1] / a.steps, 3) for f in fams}}
+    ntt_kernel_ms = {f: round(nprof[f][1] / max(nprof[f][0], 1), 4) for f in nprof}
+
+    # ---- CPU baseline: the oracle on a bounded sample of the same rotation ----
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(a)
+
+    line = {
+        "metric": METRIC,
+        "value": round(rot_per_s, 1),
+        "unit": "rotations/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (uniform residues and keys generated on the device)",
+        "config": {"workload": "batched hybrid key-switch rotation, N=2^16, 25 limbs, dnum=3, K=9, 64 ciphertexts, 16 keyed steps",
+                   "ring_dim": N, "limbs": lq, "dnum": DNUM, "special_primes": K, "batch": B, "keyed_steps": len(STEPS16)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "per_op_rotations_per_s": round(per_op_rot_s, 1),
+        "batched_over_per_op": round(rot_per_s / per_op_rot_s, 3),
+        "bit_exact_batched_vs_per_op": bit_exact,
+        "ntt_gbps": round(ntt_gbs, 1),
+        "ntt_roofline_frac": round(ntt_gbs / HBM_PEAK_GBS, 4),
+        "ntt_ms_fwd_inv": round(1000 * t_ntt, 3),
+        "ntt_avg_launch_ms": ntt_kernel_ms,
+        "eval_mult_rescale_per_s": round(mult_rescale_s, 1),
+    }
+    print(json.dumps(line))
+    ctx.close()
+
+
+def _view(fh, ctx, arr, word_offset, shape):
+    """A non-owning DeviceArray aliasing part of `arr`."""
+    v = fh.DeviceArray.__new__(fh.DeviceArray)
+    v.ctx, v.shape = ctx, tuple(shape)
+    v.words = int(np.prod(shape))
+    v.ptr = arr.ptr + 8 * word_offset
+    return v
+
+
+def cpu_baseline(a):
+    """The CPU oracle timed on the GPU host: hoisted-formulation rotations at the same shape."""
+    from oracle import oracle as O
+
+    primes = O.gen_primes(LOGN, L1, 60, 50, K)
+    o = O.Ctx(LOGN, primes, L1, K, DNUM)
+    lvl = L1 - 1
+    ct = np.concatenate([o.random_poly(1, list(range(L1))), o.random_poly(2, list(range(L1)))])
+    key = o.random_poly(3, list(range(L1 + K)) * (2 * DNUM))
+    k = o.rot_index(1)
+    o.eval_rotate(ct, lvl, key, k)  # warm-up
+    n, t0 = 0, time.perf_counter()
+    while n < 64 and time.perf_counter() - t0 < a.cpu_seconds:
+        o.eval_rotate(ct, lvl, key, k)
+        n += 1
+    dt = time.perf_counter() - t0
+    threads = O.lib().orc_num_threads()
+    return {"value": round(n / dt, 3), "unit": "rotations/s", "cores": threads, "kind": "port",
+            "sample": f"{n} rotations of one N=2^16, 25-limb ciphertext (dnum=3, K=9) by the C oracle, {dt:.1f} s"}
+
+
+if __name__ == "__main__":
+    main()

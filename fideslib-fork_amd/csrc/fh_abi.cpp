@@ -254,8 +254,15 @@ int fh_key_create(fh_context* ctx, const uint64_t* host_key, fh_key** out) {
     need(host_key, "host_key"); need(out, "out");
     size_t words = (size_t)c.dnum * 2 * c.np * c.N;
     auto* k = new fh_key{fh::DBuf(c.pool, words), &c};
-    hipError_t e = hipMemcpyAsync(k->data.p, host_key, words * 8, hipMemcpyHostToDevice, c.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+    // host or device source (unified addressing resolves the direction)
+    hipError_t e = hipMemcpyAsync(k->data.p, host_key, words * 8, hipMemcpyDefault, c.stream);
+    if (e == hipSuccess) {
+      // resident keys are kept in Montgomery form: the inner products reduce with one REDC
+      std::vector<int> ids(c.np);
+      for (int i = 0; i < c.np; ++i) ids[i] = i;
+      fh::to_montgomery(c, fh::make_set(k->data.p, c.np, ids.data(), 2 * c.dnum, (uint64_t)c.np * c.N), c.stream);
+      e = hipStreamSynchronize(c.stream);
+    }
     if (e != hipSuccess) {
       delete k;
       throw fh::Error(FH_E_DEVICE, std::string("key upload failed: ") + hipGetErrorString(e));
@@ -377,6 +384,78 @@ int fh_eval_rotate_ext(fh_context* ctx, const uint64_t* ct, int level, const fh_
     check_level(c, level);
     if ((galois & 1) == 0 || galois >= 2 * c.N) throw fh::Error(FH_E_INVALID, "galois index must be odd and < 2N");
     fh::rotate_ext(c, ct, level, key_of(c, rot_key), galois, add_c0 != 0, out_ext, c.stream);
+  });
+}
+
+int fh_rotate_batch(fh_context* ctx, int count, const uint64_t* cts, size_t ct_stride, int level, const fh_key* const* keys,
+                    const uint32_t* galois, uint64_t* out) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(cts, "cts"); need(keys, "keys"); need(galois, "galois"); need(out, "out");
+    check_level(c, level);
+    if (count < 0) throw fh::Error(FH_E_INVALID, "count must be >= 0");
+    std::vector<const uint64_t*> kp(count);
+    for (int i = 0; i < count; ++i) kp[i] = key_of(c, keys[i]);
+    fh::rotate_batch(c, count, cts, ct_stride, level, kp.data(), galois, true, true, out, c.stream);
+  });
+}
+
+int fh_rotate_hoisted(fh_context* ctx, const uint64_t* ct, int level, int n, const fh_key* const* keys, const uint32_t* galois,
+                      int add_c0, int moddown, uint64_t* out) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(ct, "ct"); need(keys, "keys"); need(galois, "galois"); need(out, "out");
+    check_level(c, level);
+    if (n < 0) throw fh::Error(FH_E_INVALID, "n must be >= 0");
+    std::vector<const uint64_t*> kp(n);
+    for (int i = 0; i < n; ++i) kp[i] = key_of(c, keys[i]);
+    fh::rotate_hoisted(c, ct, level, n, kp.data(), galois, add_c0 != 0, moddown != 0, out, c.stream);
+  });
+}
+
+int fh_fill_uniform(fh_context* ctx, uint64_t* polys, const int32_t* prime_ids, int num_limbs, int num_polys, size_t poly_stride,
+                    uint64_t seed) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(polys, "polys");
+    if (num_polys < 1) throw fh::Error(FH_E_INVALID, "num_polys must be >= 1");
+    auto p = pids_or_identity(c, prime_ids, num_limbs);
+    uint64_t stride = poly_stride ? poly_stride : (uint64_t)num_limbs * c.N;
+    fh::fill_uniform(c, fh::make_set(polys, num_limbs, p.data(), num_polys, stride), seed, c.stream);
+  });
+}
+
+int fh_profile_enable(fh_context* ctx, int enable) {
+  return guard([&] { C(ctx).prof.on = enable != 0; });
+}
+
+int fh_profile_reset(fh_context* ctx) {
+  return guard([&] {
+    auto& c = C(ctx);
+    FH_HIP(hipStreamSynchronize(c.stream));
+    std::lock_guard<std::mutex> lk(c.prof.mu);
+    for (auto& r : c.prof.recs) { c.prof.give(r.a); c.prof.give(r.b); }
+    c.prof.recs.clear();
+  });
+}
+
+int fh_profile_read(fh_context* ctx, const char* name, int* launches, double* total_ms) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(name, "name"); need(launches, "launches"); need(total_ms, "total_ms");
+    FH_HIP(hipStreamSynchronize(c.stream));
+    std::lock_guard<std::mutex> lk(c.prof.mu);
+    int n = 0;
+    double t = 0;
+    for (auto& r : c.prof.recs) {
+      if (std::strcmp(r.name, name) != 0) continue;
+      float ms = 0;
+      FH_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      ++n;
+      t += ms;
+    }
+    *launches = n;
+    *total_ms = t;
   });
 }
 

@@ -17,13 +17,16 @@ struct PrimeConst {
   uint64_t q2;        // 2q
   uint64_t mu;        // Barrett: floor(2^(2k) / q), k = bit length of q
   uint32_t k;         // bit length of q
-  uint32_t pad;
+  uint32_t fp64;      // 1 if the fp64 NTT path is exact for q (q < 2^50.2)
   uint64_t ninv;      // N^-1 mod q
   uint64_t ninv_sh;   // Shoup quotient of ninv
   uint64_t r64;       // 2^64 mod q
   uint64_t r64_sh;    // Shoup quotient of r64
-  double qd;          // q as double (fp64 path, q < 2^51)
+  double qd;          // q as double (fp64 path)
   double qinv;        // 1/q rounded
+  double ninv_d;      // N^-1 mod q, centred into (-q/2, q/2], as double
+  uint64_t mont_ninv; // -q^-1 mod 2^64 (Montgomery reduction)
+  uint64_t mont_r2;   // 2^128 mod q (to Montgomery form)
 };
 
 __device__ __forceinline__ uint64_t mulhi(uint64_t a, uint64_t b) { return __umul64hi(a, b); }
@@ -75,6 +78,21 @@ struct U128 {
     lo = nl;
   }
 };
+
+// Montgomery reduction: for T = hi 2^64 + lo < q 2^64, returns T 2^-64 mod q in [0, q).
+// m = lo (-q^-1) mod 2^64 makes T + m q divisible by 2^64; the low word of the sum is zero,
+// so its carry is 1 exactly when lo != 0.
+__device__ __forceinline__ uint64_t redc(uint64_t hi, uint64_t lo, uint64_t q, uint64_t ninv) {
+  const uint64_t m = lo * ninv;
+  uint64_t t = hi + mulhi(m, q) + (lo != 0 ? 1 : 0);
+  return t >= q ? t - q : t;
+}
+__device__ __forceinline__ uint64_t redc(uint64_t hi, uint64_t lo, const PrimeConst& p) { return redc(hi, lo, p.q, p.mont_ninv); }
+
+// a * b * 2^-64 mod q for a, b < q (Montgomery product)
+__device__ __forceinline__ uint64_t mont_mul(uint64_t a, uint64_t b, const PrimeConst& p) {
+  return redc(mulhi(a, b), a * b, p);
+}
 
 // (hi, lo) mod q for any 128-bit value: x = hi [2^64]_q + lo (mod q).
 __device__ __forceinline__ uint64_t mod128(uint64_t hi, uint64_t lo, const PrimeConst& p) {

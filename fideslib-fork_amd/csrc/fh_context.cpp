@@ -187,6 +187,37 @@ void Pool::release_all() {
   cached_ = 0;
 }
 
+// ---- Profiler ----
+hipEvent_t Profiler::take() {
+  if (!spare.empty()) {
+    hipEvent_t e = spare.back();
+    spare.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  FH_HIP(hipEventCreate(&e));
+  return e;
+}
+
+Profiler::~Profiler() {
+  for (auto& r : recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+  for (auto e : spare) (void)hipEventDestroy(e);
+}
+
+ProfScope::ProfScope(const Context& ctx, const char* n, hipStream_t s) : c(ctx), name(n), st(s) {
+  if (!c.prof.on) return;
+  std::lock_guard<std::mutex> lk(c.prof.mu);
+  b = c.prof.take();
+  FH_HIP(hipEventRecord(b, st));
+}
+
+ProfScope::~ProfScope() {
+  if (!b) return;
+  std::lock_guard<std::mutex> lk(c.prof.mu);
+  hipEvent_t e = c.prof.take();
+  if (hipEventRecord(e, st) == hipSuccess) c.prof.recs.push_back(Profiler::Rec{name, b, e});
+}
+
 // ---- Context ----
 template <class T>
 T* Context::upload(const std::vector<T>& v) {
@@ -237,6 +268,11 @@ Context::Context(int logN_, int L1_, int K_, int dnum_, const std::vector<uint64
     P.r64_sh = h_shoup(P.r64, qi);
     P.qd = (double)qi;
     P.qinv = 1.0 / (double)qi;
+    // fp64 NTT: every intermediate stays below 2^53 when q < 2^50.2 (values are bounded by ~3.7q)
+    P.fp64 = ((double)qi < 1.1487 * (double)(1ull << 50)) ? 1u : 0u;
+    P.ninv_d = P.ninv > qi / 2 ? (double)P.ninv - (double)qi : (double)P.ninv;
+    P.mont_ninv = (uint64_t)0 - h_inv_odd(qi, 0);  // modulus 2^64: mask of m-1 = all ones
+    P.mont_r2 = (uint64_t)(((((u128)1 << 64) % qi) * (((u128)1 << 64) % qi)) % qi);
     // twiddles: tw[brev(j)] = psi^j (OpenFHE ChineseRemainderTransformFTTNat::PreCompute)
     const uint64_t pinv = h_invmod(psi[i], qi);
     uint64_t x = 1, xi = 1;
@@ -246,8 +282,9 @@ Context::Context(int logN_, int L1_, int K_, int dnum_, const std::vector<uint64
       itw[r] = xi;
       twsh[r] = h_shoup(x, qi);
       itwsh[r] = h_shoup(xi, qi);
-      twd[r] = (double)x;
-      itwd[r] = (double)xi;
+      // centred representatives: |w| <= q/2 halves the product magnitude in the fp64 path
+      twd[r] = x > qi / 2 ? (double)x - (double)qi : (double)x;
+      itwd[r] = xi > qi / 2 ? (double)xi - (double)qi : (double)xi;
       x = h_mulmod(x, psi[i], qi);
       xi = h_mulmod(xi, pinv, qi);
     }
@@ -284,6 +321,16 @@ Context::Context(int logN_, int L1_, int K_, int dnum_, const std::vector<uint64
   d_modup_qhinv = upload(qhinv);
   d_modup_qhinv_sh = upload(qhinv_sh);
   d_modup_qhat = upload(qhat);
+  // Montgomery forms [Qhat_i 2^64]_{p_j}: the conversion reduces its 128-bit sum with one REDC
+  std::vector<uint64_t> qhat_m(qhat.size(), 0);
+  for (int l = 0; l < L1; ++l)
+    for (int d = 0; d < num_digits(l); ++d)
+      for (int i = 0; i < alpha; ++i)
+        for (int pos = 0; pos < l + 1 + K; ++pos) {
+          size_t o = (((size_t)l * dnum + d) * alpha + i) * E + pos;
+          qhat_m[o] = (uint64_t)(((u128)qhat[o] << 64) % q[ext_prime(l, pos)]);
+        }
+  d_modup_qhat_m = upload(qhat_m);
 
   // ---- ModDown tables (OpenFHE PHatInvModp, PHatModq, PInvModq) ----
   std::vector<uint64_t> phinv(std::max(K, 1), 0), phinv_sh(std::max(K, 1), 0), phat((size_t)std::max(K, 1) * L1, 0);
@@ -312,6 +359,15 @@ Context::Context(int logN_, int L1_, int K_, int dnum_, const std::vector<uint64
   d_md_phinv = upload(phinv);
   d_md_phinv_sh = upload(phinv_sh);
   d_md_phat = upload(phat);
+  std::vector<uint64_t> phat_m(phat.size(), 0);
+  for (int k = 0; k < K; ++k)
+    for (int i = 0; i < L1; ++i) phat_m[(size_t)k * L1 + i] = (uint64_t)(((u128)phat[(size_t)k * L1 + i] << 64) % q[i]);
+  d_md_phat_m = upload(phat_m);
+  // REDC needs (sum of products) < q 2^64: true when count * max prime <= 2^64
+  uint64_t qmax = *std::max_element(q.begin(), q.end());
+  int qbits = 64 - __builtin_clzll(qmax);
+  const u128 lim = (u128)1 << 64;
+  redc_conv_ok = ((u128)alpha << qbits) <= lim && ((u128)std::max(K, 1) << qbits) <= lim;
   d_md_pinv = upload(h_pinv);
   d_md_pinv_sh = upload(pinv_sh);
   d_pmod = upload(h_pmod);

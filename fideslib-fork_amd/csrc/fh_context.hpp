@@ -85,6 +85,33 @@ struct DBuf {
   void reset() { if (p && pool) pool->put(p); p = nullptr; }
 };
 
+// Per-kernel-family launch timing with HIP events on the launching stream
+// (enabled on demand; the reference wraps ops in NVTX ranges, src/CudaUtils.cuh:664-694).
+struct Profiler {
+  struct Rec {
+    const char* name;
+    hipEvent_t a, b;
+  };
+  bool on = false;
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> spare;
+  std::mutex mu;
+  hipEvent_t take();
+  void give(hipEvent_t e) { spare.push_back(e); }
+  ~Profiler();
+};
+
+struct Context;
+// Records an event pair around the launches issued in its scope when profiling is on.
+struct ProfScope {
+  const Context& c;
+  const char* name;
+  hipStream_t st;
+  hipEvent_t b = nullptr;
+  ProfScope(const Context& ctx, const char* n, hipStream_t s);
+  ~ProfScope();
+};
+
 struct Context {
   int logN = 0;
   uint32_t N = 0;
@@ -105,6 +132,9 @@ struct Context {
   uint64_t* d_modup_qhinv = nullptr;     // [L1][dnum][alpha]
   uint64_t* d_modup_qhinv_sh = nullptr;  // [L1][dnum][alpha]
   uint64_t* d_modup_qhat = nullptr;      // [L1][dnum][alpha][L1+K]
+  uint64_t* d_modup_qhat_m = nullptr;    // same, Montgomery form [Qhat_i 2^64]_{p_j}
+  uint64_t* d_md_phat_m = nullptr;       // [K][L1] Montgomery form [P^_k 2^64]_{q_i}
+  bool redc_conv_ok = false;             // conversion sums fit a single Montgomery reduction
   // ModDown
   uint64_t* d_md_phinv = nullptr;        // [K]     [P^_k^-1]_{p_k}
   uint64_t* d_md_phinv_sh = nullptr;     // [K]
@@ -124,6 +154,7 @@ struct Context {
 
   hipStream_t stream = nullptr;
   Pool pool;
+  mutable Profiler prof;
   std::vector<void*> owned;   // table allocations freed at destruction
 
   Context(int logN, int L1, int K, int dnum, const std::vector<uint64_t>& primes);

@@ -55,7 +55,7 @@ __global__ __launch_bounds__(kEwThreads) void k_unary(LimbSet a, LimbSet o, Scal
   SetAddr{a}.at(blockIdx.y, N, pa, pid);
   SetAddr{o}.at(blockIdx.y, N, po, pid2);
   const uint64_t q = pr[pid].q;
-  const uint32_t limb = blockIdx.y % a.nlimbs;
+  const uint32_t limb = blockIdx.y / a.npolys;
   const uint64_t s = sc.v[limb], ssh = sc.sh[limb];
   for (uint32_t i = 2 * (blockIdx.x * blockDim.x + threadIdx.x); i < N; i += 2 * gridDim.x * blockDim.x) {
     V2 x = ld2(pa + i);
@@ -95,6 +95,26 @@ __global__ __launch_bounds__(kEwThreads) void k_tensor(const uint64_t* c, const 
   }
 }
 
+// splitmix64 finalizer: a counter-based stream, one 64-bit word per (seed, limb job, index)
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(kEwThreads) void k_fill_uniform(LimbSet s, const PrimeConst* pr, uint32_t N, uint64_t seed) {
+  uint64_t* p;
+  int pid;
+  SetAddr{s}.at(blockIdx.y, N, p, pid);
+  const PrimeConst P = pr[pid];
+  const uint64_t key = mix64(seed ^ ((uint64_t)blockIdx.y << 32));
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    const uint64_t r = mix64(key + i);
+    p[i] = barrett128(0, r, P);  // r mod q (uniform up to a 2^-13 bias, irrelevant for synthetic data)
+  }
+}
+
 inline dim3 ew_grid(uint32_t N, uint32_t nlimbs) {
   uint32_t per_block = 2 * kEwThreads;
   uint32_t gx = (N + per_block - 1) / per_block;
@@ -108,6 +128,7 @@ void ew_binary(const Context& c, int op, const LimbSet& a, const LimbSet& b, con
   uint32_t n = a.nlimbs * a.npolys;
   if (!n) return;
   dim3 g = ew_grid(c.N, n);
+  ProfScope ps_(c, "eltwise", st);
   switch (op) {
     case 0: k_binary<0><<<g, kEwThreads, 0, st>>>(a, b, o, c.d_prime, c.N); break;
     case 1: k_binary<1><<<g, kEwThreads, 0, st>>>(a, b, o, c.d_prime, c.N); break;
@@ -127,6 +148,7 @@ void ew_unary(const Context& c, int op, const LimbSet& a, const LimbSet& o, cons
     sc.sh[i] = h_shoup(v, q);
   }
   dim3 g = ew_grid(c.N, n);
+  ProfScope ps_(c, "eltwise", st);
   switch (op) {
     case 0: k_unary<0><<<g, kEwThreads, 0, st>>>(a, o, sc, c.d_prime, c.N); break;
     case 1: k_unary<1><<<g, kEwThreads, 0, st>>>(a, o, sc, c.d_prime, c.N); break;
@@ -135,9 +157,33 @@ void ew_unary(const Context& c, int op, const LimbSet& a, const LimbSet& o, cons
   FH_HIP(hipGetLastError());
 }
 
+__global__ __launch_bounds__(kEwThreads) void k_to_mont(LimbSet s, const PrimeConst* pr, uint32_t N) {
+  uint64_t* p;
+  int pid;
+  SetAddr{s}.at(blockIdx.y, N, p, pid);
+  const PrimeConst P = pr[pid];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) p[i] = mont_mul(p[i], P.mont_r2, P);
+}
+
+void to_montgomery(const Context& c, const LimbSet& s, hipStream_t st) {
+  uint32_t n = s.nlimbs * s.npolys;
+  if (!n) return;
+  k_to_mont<<<ew_grid(c.N, n), kEwThreads, 0, st>>>(s, c.d_prime, c.N);
+  FH_HIP(hipGetLastError());
+}
+
+void fill_uniform(const Context& c, const LimbSet& s, uint64_t seed, hipStream_t st) {
+  uint32_t n = s.nlimbs * s.npolys;
+  if (!n) return;
+  if (n > 65535u) throw Error(-1, "too many limbs in one fill");
+  k_fill_uniform<<<ew_grid(c.N, n), kEwThreads, 0, st>>>(s, c.d_prime, c.N, seed);
+  FH_HIP(hipGetLastError());
+}
+
 void ew_tensor(const Context& c, const uint64_t* ct1, const uint64_t* ct2, uint64_t* out3, int nlimbs, hipStream_t st) {
   if (nlimbs <= 0) return;
   dim3 g = ew_grid(c.N, nlimbs);
+  ProfScope ps_(c, "tensor", st);
   if (ct1 == ct2) k_tensor<true><<<g, kEwThreads, 0, st>>>(ct1, ct1, out3, nlimbs, c.d_prime, c.N);
   else k_tensor<false><<<g, kEwThreads, 0, st>>>(ct1, ct2, out3, nlimbs, c.d_prime, c.N);
   FH_HIP(hipGetLastError());

@@ -31,8 +31,9 @@ struct LimbJob {
 
 struct SetAddr {
   LimbSet s;
+  // limb-major job order: consecutive blocks work on the same prime (its twiddles stay in L2)
   __device__ __forceinline__ void at(uint32_t y, uint32_t N, uint64_t*& p, int& pid) const {
-    uint32_t limb = y % s.nlimbs, poly = y / s.nlimbs;
+    uint32_t limb = y / s.npolys, poly = y % s.npolys;
     p = s.base + (size_t)poly * s.poly_stride + (size_t)limb * N;
     pid = s.pid[limb];
   }
@@ -51,14 +52,18 @@ struct JobAddr {
 struct ModUpAddr {
   uint64_t* base;
   uint32_t ext, lq, alpha, L1, beta;
+  uint32_t per;           // target limbs per polynomial (count()); jobs of poly b follow poly b-1
+  uint64_t poly_stride;   // words between consecutive polynomials' digit blocks
   __device__ __forceinline__ void at(uint32_t y, uint32_t N, uint64_t*& p, int& pid) const {
+    const uint32_t b = y / per;
+    y -= b * per;
     const uint32_t full = ext - alpha;
     uint32_t d = y / full, t = y - d * full;
     if (d >= beta - 1) { d = beta - 1; t = y - (beta - 1) * full; }
     const uint32_t start = d * alpha;
     const uint32_t sz = alpha < lq - start ? alpha : lq - start;
     const uint32_t pos = t < start ? t : t + sz;
-    p = base + ((size_t)d * ext + pos) * N;
+    p = base + b * poly_stride + ((size_t)d * ext + pos) * N;
     pid = (int)(pos < lq ? pos : L1 + (pos - lq));
   }
   __host__ uint32_t count() const { return beta * ext - lq; }
@@ -70,12 +75,14 @@ struct NttTables {
   const uint64_t* tw_sh;
   const uint64_t* itw;
   const uint64_t* itw_sh;
+  const double* twd;   // centred twiddles as doubles (fp64 path)
+  const double* itwd;
   uint32_t N;
   uint32_t logN;
 };
 
 inline NttTables ntt_tables(const Context& c) {
-  return NttTables{c.d_prime, c.d_tw, c.d_tw_sh, c.d_itw, c.d_itw_sh, c.N, (uint32_t)c.logN};
+  return NttTables{c.d_prime, c.d_tw, c.d_tw_sh, c.d_itw, c.d_itw_sh, c.d_twd, c.d_itwd, c.N, (uint32_t)c.logN};
 }
 
 inline LimbSet make_set(uint64_t* base, uint32_t nlimbs, const int* pids, uint32_t npolys = 1, uint64_t poly_stride = 0) {

@@ -34,10 +34,12 @@ __device__ __forceinline__ uint32_t auto_src(uint32_t j, uint32_t k, int logN) {
 // ----------------------------------------------------------------------------------- ModUp conversion
 struct ModUpParams {
   uint32_t N, lq, ext, alpha, L1, K, beta, level, dnum;
+  uint64_t in_stride;        // words between consecutive input polys (EVAL form)
   const PrimeConst* prime;
   const uint64_t* qhinv;     // [L1][dnum][alpha]
   const uint64_t* qhinv_sh;
   const uint64_t* qhat;      // [L1][dnum][alpha][L1+K]
+  const uint64_t* qhat_m;    // Montgomery form of qhat
 };
 
 // blockIdx.y = digit. in_coef: INTT of the input (lq limbs), in_eval: the input itself.
@@ -47,7 +49,11 @@ __global__ __launch_bounds__(kT) void k_modup_conv(const uint64_t* __restrict__ 
                                                     uint64_t* __restrict__ out, ModUpParams mp) {
   __shared__ uint64_t s_qhat[kMaxAlpha * kMaxPrimes];
   __shared__ uint64_t s_inv[kMaxAlpha], s_invsh[kMaxAlpha], s_q[kMaxAlpha];
-  const uint32_t d = blockIdx.y;
+  // blockIdx.y = poly * beta + digit; in_coef is packed (lq limbs per poly), out is beta*ext limbs per poly
+  const uint32_t d = blockIdx.y % mp.beta, b = blockIdx.y / mp.beta;
+  in_coef += (size_t)b * mp.lq * mp.N;
+  in_eval += (size_t)b * mp.in_stride;
+  out += (size_t)b * mp.beta * mp.ext * mp.N;
   const uint32_t start = d * mp.alpha;
   const uint32_t sz = min(mp.alpha, mp.lq - start);
   const uint32_t E = mp.L1 + mp.K;
@@ -78,6 +84,76 @@ __global__ __launch_bounds__(kT) void k_modup_conv(const uint64_t* __restrict__ 
       acc.zero();
       for (uint32_t i = 0; i < sz; ++i) acc.mac(y[i], s_qhat[i * mp.ext + pos]);
       od[(size_t)pos * mp.N + n] = mod128(acc.hi, acc.lo, mp.prime[p]);
+    }
+  }
+}
+
+// Register-resident form for digit sizes up to SZ: two consecutive coefficients per lane (16-byte
+// accesses), the sources prescaled once, each target's 128-bit sum reduced with one Montgomery
+// reduction against the Montgomery-form matrix [Qhat_i 2^64]_{p_j}.
+template <int SZ>
+__global__ __launch_bounds__(kT) void k_modup_conv_r(const uint64_t* __restrict__ in_coef, const uint64_t* __restrict__ in_eval,
+                                                      uint64_t* __restrict__ out, ModUpParams mp) {
+  __shared__ uint64_t s_qhat[SZ * kMaxPrimes];
+  __shared__ uint64_t s_pq[kMaxPrimes], s_pn[kMaxPrimes];
+  const uint32_t d = blockIdx.y % mp.beta, b = blockIdx.y / mp.beta;
+  in_coef += (size_t)b * mp.lq * mp.N;
+  in_eval += (size_t)b * mp.in_stride;
+  out += (size_t)b * mp.beta * mp.ext * mp.N;
+  const uint32_t start = d * mp.alpha;
+  const uint32_t sz = min(mp.alpha, mp.lq - start);
+  const uint32_t E = mp.L1 + mp.K;
+  const size_t tb = ((size_t)mp.level * mp.dnum + d) * mp.alpha;
+  for (uint32_t i = threadIdx.x; i < sz * mp.ext; i += blockDim.x) {
+    uint32_t si = i / mp.ext, pos = i % mp.ext;
+    s_qhat[si * mp.ext + pos] = mp.qhat_m[(tb + si) * E + pos];
+  }
+  for (uint32_t pos = threadIdx.x; pos < mp.ext; pos += blockDim.x) {
+    const PrimeConst& P = mp.prime[pos < mp.lq ? pos : mp.L1 + (pos - mp.lq)];
+    s_pq[pos] = P.q;
+    s_pn[pos] = P.mont_ninv;
+  }
+  uint64_t inv[SZ], invsh[SZ], qs[SZ];
+#pragma unroll
+  for (int i = 0; i < SZ; ++i) {
+    if (i < (int)sz) {
+      inv[i] = mp.qhinv[tb + i];
+      invsh[i] = mp.qhinv_sh[tb + i];
+      qs[i] = mp.prime[start + i].q;
+    }
+  }
+  __syncthreads();
+  uint64_t* od = out + (size_t)d * mp.ext * mp.N;
+  for (uint32_t n = 2 * (blockIdx.x * blockDim.x + threadIdx.x); n < mp.N; n += 2 * gridDim.x * blockDim.x) {
+    uint64_t y0[SZ], y1[SZ];
+#pragma unroll
+    for (int i = 0; i < SZ; ++i) {
+      if (i < (int)sz) {
+        const size_t o = (size_t)(start + i) * mp.N + n;
+        const ulonglong2 c = *reinterpret_cast<const ulonglong2*>(in_coef + o);
+        y0[i] = shoup(c.x, inv[i], invsh[i], qs[i]);
+        y1[i] = shoup(c.y, inv[i], invsh[i], qs[i]);
+        *reinterpret_cast<ulonglong2*>(od + o) = *reinterpret_cast<const ulonglong2*>(in_eval + o);
+      }
+    }
+    for (uint32_t pos = 0; pos < mp.ext; ++pos) {
+      if (pos == start) {
+        pos += sz - 1;
+        continue;
+      }
+      U128 a0, a1;
+      a0.zero();
+      a1.zero();
+#pragma unroll
+      for (int i = 0; i < SZ; ++i) {
+        if (i < (int)sz) {
+          const uint64_t c = s_qhat[i * mp.ext + pos];
+          a0.mac(y0[i], c);
+          a1.mac(y1[i], c);
+        }
+      }
+      const uint64_t pq = s_pq[pos], pn = s_pn[pos];
+      *reinterpret_cast<ulonglong2*>(od + (size_t)pos * mp.N + n) = make_ulonglong2(redc(a0.hi, a0.lo, pq, pn), redc(a1.hi, a1.lo, pq, pn));
     }
   }
 }
@@ -115,7 +191,7 @@ __global__ __launch_bounds__(kT) void k_ksk_inner(const uint64_t* __restrict__ d
       a0.mac(x, kb);
       a1.mac(x, ka);
     }
-    uint64_t r0 = mod128(a0.hi, a0.lo, P), r1 = mod128(a1.hi, a1.lo, P);
+    uint64_t r0 = redc(a0.hi, a0.lo, P), r1 = redc(a1.hi, a1.lo, P);  // keys are in Montgomery form
     if (addc0) r0 = add_mod(r0, shoup(kp.c0[(size_t)pos * N + n], pm, pmsh, P.q), P.q);
     // gather form: out[j] = in[auto_src(j)]  <=>  scatter: out[auto_dst(n)] = in[n]
     uint32_t j = n;
@@ -128,6 +204,77 @@ __global__ __launch_bounds__(kT) void k_ksk_inner(const uint64_t* __restrict__ d
   }
 }
 
+// ----------------------------------------------------------------------------------- batched key inner product
+// Ciphertexts that share a rotation key form a group; a thread loads the key words of its
+// (position, coefficient) once and applies them to every ciphertext of the group, so the key
+// (the dominant HBM stream of a rotation) is read once per group instead of once per ciphertext.
+constexpr int kMaxGroups = 64;
+constexpr int kMaxBatch = 256;
+constexpr int kMaxDnum = 8;
+
+struct RotGroups {
+  const uint64_t* key[kMaxGroups];
+  uint32_t gal_inv[kMaxGroups];   // inverse automorphism index (scatter form), 1 = none
+  uint16_t first[kMaxGroups], count[kMaxGroups];
+  uint16_t order[kMaxBatch];      // batch index of the i-th ciphertext in group order
+};
+
+struct KskBatchParams {
+  uint32_t N, lq, ext, L1, K, beta, logN;
+  uint64_t ct_stride;       // words between input ciphertexts (c0 at offset 0)
+  uint64_t dig_stride;      // words between digit blocks of consecutive ciphertexts
+  uint64_t out_stride;      // words between output ext ciphertexts (2 * ext limbs)
+  int add_c0;
+  const PrimeConst* prime;
+  const uint64_t* cts;
+  const uint64_t* pmod;
+  const uint64_t* pmod_sh;
+};
+
+__global__ __launch_bounds__(kT) void k_ksk_batch(const uint64_t* __restrict__ digits, uint64_t* __restrict__ out, KskBatchParams kp,
+                                                   RotGroups g) {
+  const uint32_t pos = blockIdx.y, grp = blockIdx.z;
+  const uint32_t p = pos < kp.lq ? pos : kp.L1 + (pos - kp.lq);
+  const PrimeConst P = kp.prime[p];
+  const uint32_t E = kp.L1 + kp.K;
+  const size_t N = kp.N;
+  const uint64_t* key = g.key[grp];
+  const uint32_t kinv = g.gal_inv[grp];
+  const bool addc0 = kp.add_c0 && pos < kp.lq;
+  const uint64_t pm = addc0 ? kp.pmod[pos] : 0, pmsh = addc0 ? kp.pmod_sh[pos] : 0;
+  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < kp.N; n += gridDim.x * blockDim.x) {
+    uint64_t kb[kMaxDnum], ka[kMaxDnum];
+#pragma unroll
+    for (uint32_t d = 0; d < kMaxDnum; ++d) {
+      if (d < kp.beta) {
+        kb[d] = key[(((size_t)d * 2 + 0) * E + p) * N + n];
+        ka[d] = key[(((size_t)d * 2 + 1) * E + p) * N + n];
+      }
+    }
+    const uint32_t j = kinv != 1 ? auto_src(n, kinv, kp.logN) : n;
+    for (uint32_t t = 0; t < g.count[grp]; ++t) {
+      const uint32_t b = g.order[g.first[grp] + t];
+      const uint64_t* dg = digits + b * kp.dig_stride + (size_t)pos * N + n;
+      U128 a0, a1;
+      a0.zero();
+      a1.zero();
+#pragma unroll
+      for (uint32_t d = 0; d < kMaxDnum; ++d) {
+        if (d < kp.beta) {
+          const uint64_t x = dg[(size_t)d * kp.ext * N];
+          a0.mac(x, kb[d]);
+          a1.mac(x, ka[d]);
+        }
+      }
+      uint64_t r0 = redc(a0.hi, a0.lo, P), r1 = redc(a1.hi, a1.lo, P);  // keys are in Montgomery form
+      if (addc0) r0 = add_mod(r0, shoup(kp.cts[b * kp.ct_stride + (size_t)pos * N + n], pm, pmsh, P.q), P.q);
+      uint64_t* o = out + b * kp.out_stride;
+      o[(size_t)pos * N + j] = r0;
+      o[((size_t)kp.ext + pos) * N + j] = r1;
+    }
+  }
+}
+
 // ----------------------------------------------------------------------------------- ModDown
 struct ModDownParams {
   uint32_t N, lq, ext, L1, K;
@@ -135,6 +282,7 @@ struct ModDownParams {
   const uint64_t* phinv;     // [K]
   const uint64_t* phinv_sh;
   const uint64_t* phat;      // [K][L1]
+  const uint64_t* phat_m;    // Montgomery form of phat
   const uint64_t* pinv;      // [L1]
   const uint64_t* pinv_sh;
 };
@@ -165,6 +313,60 @@ __global__ __launch_bounds__(kT) void k_moddown_conv(const uint64_t* __restrict_
       acc.zero();
       for (uint32_t k = 0; k < mp.K; ++k) acc.mac(y[k], s_phat[k * mp.lq + i]);
       cv[(size_t)i * mp.N + n] = mod128(acc.hi, acc.lo, mp.prime[i]);
+    }
+  }
+}
+
+// Register-resident form for K <= KK special primes: two coefficients per lane, Montgomery-form
+// matrix [P^_k 2^64]_{q_i}, one Montgomery reduction per output residue.
+template <int KK>
+__global__ __launch_bounds__(kT) void k_moddown_conv_r(const uint64_t* __restrict__ pcoef, uint64_t* __restrict__ conv, ModDownParams mp) {
+  __shared__ uint64_t s_phat[KK * kMaxPrimes];
+  __shared__ uint64_t s_q[kMaxPrimes], s_n[kMaxPrimes];
+  for (uint32_t i = threadIdx.x; i < mp.K * mp.lq; i += blockDim.x) {
+    uint32_t k = i / mp.lq, t = i % mp.lq;
+    s_phat[k * mp.lq + t] = mp.phat_m[(size_t)k * mp.L1 + t];
+  }
+  for (uint32_t i = threadIdx.x; i < mp.lq; i += blockDim.x) {
+    s_q[i] = mp.prime[i].q;
+    s_n[i] = mp.prime[i].mont_ninv;
+  }
+  uint64_t inv[KK], invsh[KK], ps[KK];
+#pragma unroll
+  for (int k = 0; k < KK; ++k) {
+    if (k < (int)mp.K) {
+      inv[k] = mp.phinv[k];
+      invsh[k] = mp.phinv_sh[k];
+      ps[k] = mp.prime[mp.L1 + k].q;
+    }
+  }
+  __syncthreads();
+  const uint64_t* pc = pcoef + (size_t)blockIdx.y * mp.K * mp.N;
+  uint64_t* cv = conv + (size_t)blockIdx.y * mp.lq * mp.N;
+  for (uint32_t n = 2 * (blockIdx.x * blockDim.x + threadIdx.x); n < mp.N; n += 2 * gridDim.x * blockDim.x) {
+    uint64_t y0[KK], y1[KK];
+#pragma unroll
+    for (int k = 0; k < KK; ++k) {
+      if (k < (int)mp.K) {
+        const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(pc + (size_t)k * mp.N + n);
+        y0[k] = shoup(v.x, inv[k], invsh[k], ps[k]);
+        y1[k] = shoup(v.y, inv[k], invsh[k], ps[k]);
+      }
+    }
+    for (uint32_t i = 0; i < mp.lq; ++i) {
+      U128 a0, a1;
+      a0.zero();
+      a1.zero();
+#pragma unroll
+      for (int k = 0; k < KK; ++k) {
+        if (k < (int)mp.K) {
+          const uint64_t c = s_phat[k * mp.lq + i];
+          a0.mac(y0[k], c);
+          a1.mac(y1[k], c);
+        }
+      }
+      const uint64_t q = s_q[i], nv = s_n[i];
+      *reinterpret_cast<ulonglong2*>(cv + (size_t)i * mp.N + n) = make_ulonglong2(redc(a0.hi, a0.lo, q, nv), redc(a1.hi, a1.lo, q, nv));
     }
   }
 }
@@ -257,20 +459,34 @@ inline dim3 grid_n(uint32_t N, uint32_t y, uint32_t per_thread = 1) {
 
 // ===================================================================================== host side
 
-void modup(Context& c, const uint64_t* x, int l, uint64_t* digits, hipStream_t st) {
+void modup(Context& c, const uint64_t* x, int l, uint64_t* digits, hipStream_t st) { modup_batch(c, x, l, 1, 0, digits, st); }
+
+void modup_batch(Context& c, const uint64_t* x, int l, int npolys, uint64_t x_stride, uint64_t* digits, hipStream_t st) {
   const uint32_t lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);
   if (c.K < 1) throw Error(-1, "key switching needs at least one special prime");
   if ((uint32_t)c.alpha > (uint32_t)kMaxAlpha) throw Error(-3, "digit size above kernel limit");
+  if (npolys < 1) return;
+  if (!x_stride) x_stride = (uint64_t)lq * c.N;
   std::vector<int> ids(lq);
   for (uint32_t i = 0; i < lq; ++i) ids[i] = i;
-  DBuf tmp(c.pool, (size_t)lq * c.N);
-  ntt_inv(c, make_set(const_cast<uint64_t*>(x), lq, ids.data()), make_set(tmp.p, lq, ids.data()), st);
+  DBuf tmp(c.pool, (size_t)npolys * lq * c.N);
+  ntt_inv(c, make_set(const_cast<uint64_t*>(x), lq, ids.data(), npolys, x_stride),
+          make_set(tmp.p, lq, ids.data(), npolys, (uint64_t)lq * c.N), st);
   ModUpParams mp{c.N, lq, ext, (uint32_t)c.alpha, (uint32_t)c.L1, (uint32_t)c.K, beta, (uint32_t)l, (uint32_t)c.dnum,
-                 c.d_prime, c.d_modup_qhinv, c.d_modup_qhinv_sh, c.d_modup_qhat};
-  k_modup_conv<<<grid_n(c.N, beta), kT, 0, st>>>(tmp.p, x, digits, mp);
+                 x_stride, c.d_prime, c.d_modup_qhinv, c.d_modup_qhinv_sh, c.d_modup_qhat, c.d_modup_qhat_m};
+  {
+    ProfScope ps_(c, "modup_conv", st);
+    const dim3 g2 = grid_n(c.N, beta * npolys, 2);
+    if (!c.redc_conv_ok || c.alpha > 16) k_modup_conv<<<grid_n(c.N, beta * npolys), kT, 0, st>>>(tmp.p, x, digits, mp);
+    else if (c.alpha <= 4) k_modup_conv_r<4><<<g2, kT, 0, st>>>(tmp.p, x, digits, mp);
+    else if (c.alpha <= 8) k_modup_conv_r<8><<<g2, kT, 0, st>>>(tmp.p, x, digits, mp);
+    else if (c.alpha <= 12) k_modup_conv_r<12><<<g2, kT, 0, st>>>(tmp.p, x, digits, mp);
+    else k_modup_conv_r<16><<<g2, kT, 0, st>>>(tmp.p, x, digits, mp);
+  }
   FH_HIP(hipGetLastError());
-  ModUpAddr a{digits, ext, lq, (uint32_t)c.alpha, (uint32_t)c.L1, beta};
-  ntt_fwd_modup(c, a, a.count(), st);
+  ModUpAddr a{digits, ext, lq, (uint32_t)c.alpha, (uint32_t)c.L1, beta, 0, (uint64_t)beta * ext * c.N};
+  a.per = a.count();
+  ntt_fwd_modup(c, a, a.per * npolys, st);
 }
 
 void ksk_inner(Context& c, const uint64_t* digits, int l, const uint64_t* key, uint64_t* out0, uint64_t* out1,
@@ -282,7 +498,7 @@ void ksk_inner(Context& c, const uint64_t* digits, int l, const uint64_t* key, u
     // the kernel scatters through the inverse map: pass k^-1 so that out[j] = in[src_k(j)]
     kp.galois = (uint32_t)h_inv_odd(galois, 2ull * c.N);
   }
-  k_ksk_inner<<<grid_n(c.N, ext), kT, 0, st>>>(digits, out0, out1, kp);
+  { ProfScope ps_(c, "ksk_inner", st); k_ksk_inner<<<grid_n(c.N, ext), kT, 0, st>>>(digits, out0, out1, kp); }
   FH_HIP(hipGetLastError());
 }
 
@@ -292,39 +508,48 @@ void moddown(Context& c, const uint64_t* x, int l, uint64_t* out, int npolys, ui
   if (!x_stride) x_stride = (uint64_t)ext * c.N;
   if (!out_stride) out_stride = (uint64_t)lq * c.N;
   DBuf pc(c.pool, (size_t)npolys * K * c.N), conv(c.pool, (size_t)npolys * lq * c.N);
-  k_gather_special<<<grid_n(c.N, npolys * K), kT, 0, st>>>(x, pc.p, c.N, lq, K, x_stride);
+  { ProfScope ps_(c, "gather_special", st); k_gather_special<<<grid_n(c.N, npolys * K), kT, 0, st>>>(x, pc.p, c.N, lq, K, x_stride); }
   std::vector<int> pids(K), qids(lq);
   for (uint32_t k = 0; k < K; ++k) pids[k] = c.L1 + k;
   for (uint32_t i = 0; i < lq; ++i) qids[i] = i;
   ntt_inv(c, make_set(pc.p, K, pids.data(), npolys, (uint64_t)K * c.N), st);
-  ModDownParams mp{c.N, lq, ext, (uint32_t)c.L1, K, c.d_prime, c.d_md_phinv, c.d_md_phinv_sh, c.d_md_phat, c.d_md_pinv, c.d_md_pinv_sh};
-  k_moddown_conv<<<grid_n(c.N, npolys), kT, 0, st>>>(pc.p, conv.p, mp);
+  ModDownParams mp{c.N, lq, ext, (uint32_t)c.L1, K, c.d_prime, c.d_md_phinv, c.d_md_phinv_sh, c.d_md_phat, c.d_md_phat_m,
+                   c.d_md_pinv, c.d_md_pinv_sh};
+  {
+    ProfScope ps_(c, "moddown_conv", st);
+    const dim3 g2 = grid_n(c.N, npolys, 2);
+    if (!c.redc_conv_ok || K > 16) k_moddown_conv<<<grid_n(c.N, npolys), kT, 0, st>>>(pc.p, conv.p, mp);
+    else if (K <= 4) k_moddown_conv_r<4><<<g2, kT, 0, st>>>(pc.p, conv.p, mp);
+    else if (K <= 8) k_moddown_conv_r<8><<<g2, kT, 0, st>>>(pc.p, conv.p, mp);
+    else if (K <= 12) k_moddown_conv_r<12><<<g2, kT, 0, st>>>(pc.p, conv.p, mp);
+    else k_moddown_conv_r<16><<<g2, kT, 0, st>>>(pc.p, conv.p, mp);
+  }
   FH_HIP(hipGetLastError());
   ntt_fwd(c, make_set(conv.p, lq, qids.data(), npolys, (uint64_t)lq * c.N), st);
-  k_moddown_final<<<grid_n(c.N, npolys * lq), kT, 0, st>>>(x, conv.p, out, mp, x_stride, out_stride);
+  { ProfScope ps_(c, "moddown_final", st); k_moddown_final<<<grid_n(c.N, npolys * lq), kT, 0, st>>>(x, conv.p, out, mp, x_stride, out_stride); }
   FH_HIP(hipGetLastError());
 }
 
 void rescale(Context& c, const uint64_t* x, int l, int ncomp, uint64_t* out, hipStream_t st) {
   if (l < 1) throw Error(-1, "cannot rescale a level-0 ciphertext");
   DBuf top(c.pool, (size_t)ncomp * c.N), e(c.pool, (size_t)ncomp * l * c.N);
-  k_take_top<<<grid_n(c.N, ncomp), kT, 0, st>>>(x, top.p, c.N, l);
+  { ProfScope ps_(c, "take_top", st); k_take_top<<<grid_n(c.N, ncomp), kT, 0, st>>>(x, top.p, c.N, l); }
   int pid_top = l;
   ntt_inv(c, make_set(top.p, 1, &pid_top, ncomp, c.N), st);
   RescaleParams rp{c.N, (uint32_t)l, (uint32_t)c.L1, c.d_prime, c.d_rs_qlinv, c.d_rs_qlinv_sh, c.d_rs_QlQl, c.d_rs_QlQl_sh};
-  k_rescale_expand<<<grid_n(c.N, ncomp * l), kT, 0, st>>>(top.p, e.p, rp);
+  { ProfScope ps_(c, "rescale_expand", st); k_rescale_expand<<<grid_n(c.N, ncomp * l), kT, 0, st>>>(top.p, e.p, rp); }
   FH_HIP(hipGetLastError());
   std::vector<int> ids(l);
   for (int i = 0; i < l; ++i) ids[i] = i;
   ntt_fwd(c, make_set(e.p, l, ids.data(), ncomp, (uint64_t)l * c.N), st);
-  k_rescale_final<<<grid_n(c.N, ncomp * l), kT, 0, st>>>(x, e.p, out, rp);
+  { ProfScope ps_(c, "rescale_final", st); k_rescale_final<<<grid_n(c.N, ncomp * l), kT, 0, st>>>(x, e.p, out, rp); }
   FH_HIP(hipGetLastError());
 }
 
 void automorph(Context& c, const uint64_t* x, int nlimbs, uint32_t k, uint64_t* out, hipStream_t st) {
   if (x == out) throw Error(-1, "automorphism cannot run in place");
   if ((k & 1) == 0 || k >= 2 * c.N) throw Error(-1, "automorphism index must be odd and < 2N");
-  k_automorph<<<grid_n(c.N, nlimbs), kT, 0, st>>>(x, out, c.N, c.logN, k);
+  { ProfScope ps_(c, "automorph", st); k_automorph<<<grid_n(c.N, nlimbs), kT, 0, st>>>(x, out, c.N, c.logN, k); }
   FH_HIP(hipGetLastError());
 }
 
@@ -363,6 +588,94 @@ void eval_rotate(Context& c, const uint64_t* ct, int l, const uint64_t* key, uin
   DBuf e(c.pool, (size_t)2 * ext * c.N);
   rotate_ext(c, ct, l, key, galois, true, e.p, st);
   moddown(c, e.p, l, out2, 2, 0, 0, st);
+}
+
+namespace {
+
+// Key inner products (+ P*c0, automorphism) for B ciphertexts whose digits are already ModUp'ed,
+// grouped by shared (key, galois); then optional ModDown of the 2B extended results.
+void rotate_core(Context& c, int nb, const uint64_t* digits, uint64_t dig_stride, const uint64_t* xs, uint64_t ct_stride, int l,
+                 const uint64_t* const* keys, const uint32_t* galois, bool add_c0, bool do_moddown, uint64_t* out, hipStream_t st) {
+  const uint32_t lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);
+  DBuf ext_buf;
+  uint64_t* eo;
+  if (do_moddown) {
+    ext_buf = DBuf(c.pool, (size_t)nb * 2 * ext * c.N);
+    eo = ext_buf.p;
+  } else {
+    eo = out;
+  }
+  std::vector<int> idx(nb);
+  for (int i = 0; i < nb; ++i) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+    if (keys[a] != keys[b]) return keys[a] < keys[b];
+    return galois[a] < galois[b];
+  });
+  KskBatchParams kp{c.N, lq, ext, (uint32_t)c.L1, (uint32_t)c.K, beta, (uint32_t)c.logN, ct_stride,
+                    dig_stride, 2ull * ext * c.N, add_c0 ? 1 : 0, c.d_prime, xs, c.d_pmod, c.d_pmod_sh};
+  int i = 0;
+  while (i < nb) {
+    RotGroups g{};
+    int ng = 0, base = i;
+    while (i < nb && ng < kMaxGroups) {
+      const uint64_t* k = keys[idx[i]];
+      uint32_t gl = galois[idx[i]];
+      if (!k) throw Error(-1, "null key");
+      if ((gl & 1) == 0 || gl >= 2 * c.N) throw Error(-1, "galois index must be odd and < 2N");
+      g.key[ng] = k;
+      g.gal_inv[ng] = gl == 1 ? 1 : (uint32_t)h_inv_odd(gl, 2ull * c.N);
+      g.first[ng] = (uint16_t)(i - base);
+      int cnt = 0;
+      while (i < nb && keys[idx[i]] == k && galois[idx[i]] == gl) {
+        g.order[i - base] = (uint16_t)idx[i];
+        ++i;
+        ++cnt;
+      }
+      g.count[ng] = (uint16_t)cnt;
+      ++ng;
+    }
+    ProfScope ps_(c, "ksk_batch", st);
+    k_ksk_batch<<<dim3(grid_n(c.N, 1).x, ext, ng), kT, 0, st>>>(digits, eo, kp, g);
+    FH_HIP(hipGetLastError());
+  }
+  if (do_moddown) moddown(c, eo, l, out, 2 * nb, 0, 0, st);
+}
+
+}  // namespace
+
+// B ciphertexts at level l (ct_stride words apart, 0 = packed), each with its own (key, galois);
+// ciphertexts that share a key are grouped so the key is streamed once per group. out: B packed
+// results, over Q_l (2 x lq limbs each) when do_moddown, else over Q_l ∪ P (2 x ext limbs each).
+void rotate_batch(Context& c, int B, const uint64_t* cts, uint64_t ct_stride, int l, const uint64_t* const* keys,
+                  const uint32_t* galois, bool add_c0, bool do_moddown, uint64_t* out, hipStream_t st) {
+  if (B <= 0) return;
+  const uint32_t lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);
+  if (beta > (uint32_t)kMaxDnum) throw Error(-3, "dnum above kernel limit");
+  if (!ct_stride) ct_stride = 2ull * lq * c.N;
+  const uint64_t out_ct = do_moddown ? 2ull * lq * c.N : 2ull * ext * c.N;
+  for (int b0 = 0; b0 < B; b0 += kMaxBatch) {
+    const int nb = std::min(B - b0, kMaxBatch);
+    const uint64_t* xs = cts + (size_t)b0 * ct_stride;
+    DBuf dig(c.pool, (size_t)nb * beta * ext * c.N);
+    modup_batch(c, xs + (size_t)lq * c.N, l, nb, ct_stride, dig.p, st);
+    rotate_core(c, nb, dig.p, (uint64_t)beta * ext * c.N, xs, ct_stride, l, keys + b0, galois + b0, add_c0, do_moddown,
+                out + (size_t)b0 * out_ct, st);
+  }
+}
+
+// n rotations of ONE ciphertext from a single shared ModUp (hoisted rotation).
+void rotate_hoisted(Context& c, const uint64_t* ct, int l, int n, const uint64_t* const* keys, const uint32_t* galois, bool add_c0,
+                    bool do_moddown, uint64_t* out, hipStream_t st) {
+  if (n <= 0) return;
+  const uint32_t lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);
+  if (beta > (uint32_t)kMaxDnum) throw Error(-3, "dnum above kernel limit");
+  const uint64_t out_ct = do_moddown ? 2ull * lq * c.N : 2ull * ext * c.N;
+  DBuf dig(c.pool, (size_t)beta * ext * c.N);
+  modup(c, ct + (size_t)lq * c.N, l, dig.p, st);
+  for (int b0 = 0; b0 < n; b0 += kMaxBatch) {
+    const int nb = std::min(n - b0, kMaxBatch);
+    rotate_core(c, nb, dig.p, 0, ct, 0, l, keys + b0, galois + b0, add_c0, do_moddown, out + (size_t)b0 * out_ct, st);
+  }
 }
 
 }  // namespace fh

@@ -8,6 +8,14 @@ namespace fh {
 
 // digits: beta x (l+1+K) limbs (EVAL)
 void modup(Context& c, const uint64_t* x, int l, uint64_t* digits, hipStream_t st);
+// npolys inputs x_stride words apart (0 = packed) -> npolys packed digit blocks
+void modup_batch(Context& c, const uint64_t* x, int l, int npolys, uint64_t x_stride, uint64_t* digits, hipStream_t st);
+// B rotations sharing nothing but the level; ciphertexts with equal (key, galois) share the key stream
+void rotate_batch(Context& c, int B, const uint64_t* cts, uint64_t ct_stride, int l, const uint64_t* const* keys,
+                  const uint32_t* galois, bool add_c0, bool do_moddown, uint64_t* out, hipStream_t st);
+// n rotations of one ciphertext sharing a single ModUp (keys[i] / galois[i] per rotation)
+void rotate_hoisted(Context& c, const uint64_t* ct, int l, int n, const uint64_t* const* keys, const uint32_t* galois, bool add_c0,
+                    bool do_moddown, uint64_t* out, hipStream_t st);
 // out0/out1: (l+1+K) limbs. c0 (optional, l+1 limbs) is folded as P*c0 into out0's Q limbs;
 // galois != 1 writes both outputs through the automorphism X -> X^galois.
 void ksk_inner(Context& c, const uint64_t* digits, int l, const uint64_t* key, uint64_t* out0, uint64_t* out1,

@@ -6,46 +6,185 @@
 // a final N^-1. Reference: src/NTT.cu:42-310 (INTT), 331-756 (NTT),
 // src/CKKS/LimbPartition.cu:433-550 (launch), src/NTThelper.cuh:21-72 (butterflies).
 //
-// MI355X design: N = S1 * S2 (S1 = 2^l1 "column" points, S2 = 2^l2 "row"
-// points). The first l1 stages only couple elements of the same column
-// (stride S2), the last l2 stages only elements of the same row (contiguous),
-// so a limb is transformed in two launches. Inside a launch each 64-wide
-// wavefront lane owns 16 elements of one sub-transform: four radix-2 stages run
-// in registers, one LDS exchange re-partitions the sub-transform, and the
-// remaining stages run in registers again. Global loads and stores are
-// coalesced in both passes (rows are staged through LDS where the register
-// layout is not). Arithmetic is Harvey-lazy Shoup: forward values live in
-// [0, 4q), inverse values in [0, 2q); the last stage of each transform
-// returns canonical residues.
+// MI355X design
+//  * N = S1 * S2 (S1 = 2^l1 "column" points, S2 = 2^l2 "row" points). The first
+//    l1 stages only couple elements of one column (stride S2), the last l2
+//    stages only elements of one row (contiguous), so a limb is transformed by
+//    two launches. Each lane owns 16 elements of one sub-transform: four radix-2
+//    stages run in registers, one LDS exchange re-partitions the sub-transform,
+//    the remaining stages run in registers again. Global loads and stores are
+//    coalesced in both passes (rows are staged through LDS where the register
+//    layout is not).
+//  * Two exact arithmetics, chosen per limb by its prime:
+//      - fp64 (q < 2^50.2, the 50-bit scaling primes): residues are doubles in a
+//        signed lazy range; a product is split exactly with FMA (h = yw,
+//        l = fma(y, w, -h)), the quotient is rint(h / q) and the remainder
+//        fma(-c, q, h) + l is exact. gfx950 issues fp64 FMA at full rate, about
+//        three times the rate of a 64-bit Shoup product built from 32-bit
+//        multiplies. Values are re-centred into (-q/2, q/2] every few stages so
+//        every intermediate stays an exact integer below 2^53.
+//      - integer Harvey-lazy Shoup for the 60-bit primes: forward values in
+//        [0, 4q), inverse values in [0, 2q).
+//    Both are exact, so the canonical outputs are identical.
+//  * Between the two launches the limb holds an internal form (lazy integers or
+//    doubles). Large batches are split into chunks of limbs small enough that
+//    the intermediate written by the first launch is still in the MALL when the
+//    second launch reads it back.
+#include <algorithm>
+
 #include "fh_kernels.hpp"
 
 namespace fh {
 
 namespace {
 
-__device__ __forceinline__ void fwd_bfly(uint64_t& X, uint64_t& Y, uint64_t w, uint64_t wp, uint64_t q, uint64_t q2) {
-  uint64_t x = X >= q2 ? X - q2 : X;        // [0, 2q)
-  uint64_t t = shoup_lazy(Y, w, wp, q);     // [0, 2q)
-  X = x + t;                                // [0, 4q)
-  Y = x - t + q2;                           // [0, 4q)
-}
-
-__device__ __forceinline__ void inv_bfly(uint64_t& X, uint64_t& Y, uint64_t w, uint64_t wp, uint64_t q, uint64_t q2) {
-  uint64_t u = X + Y;                       // X, Y in [0, 2q)
-  u = u >= q2 ? u - q2 : u;                 // [0, 2q)
-  uint64_t v = X - Y + q2;                  // [0, 4q)
-  X = u;
-  Y = shoup_lazy(v, w, wp, q);              // [0, 2q)
-}
-
-__device__ __forceinline__ uint64_t canon4(uint64_t x, uint64_t q, uint64_t q2) {
-  x = x >= q2 ? x - q2 : x;
-  return x >= q ? x - q : x;
-}
-
 constexpr int kThreads = 256;
 
-// ---------------------------------------------------------------- forward, columns (first l1 stages)
+// ------------------------------------------------------------------------------- integer arithmetic
+struct IntA {
+  using T = uint64_t;
+  uint64_t q, q2;
+  struct W {
+    uint64_t w, sh;
+  };
+  __device__ __forceinline__ W tw(const uint64_t* t, const uint64_t* tsh, uint32_t i) const { return W{t[i], tsh[i]}; }
+  __device__ __forceinline__ T from_canon(uint64_t v) const { return v; }
+  __device__ __forceinline__ T from_mid(uint64_t v) const { return v; }
+  __device__ __forceinline__ uint64_t to_mid(T v) const { return v; }
+  __device__ __forceinline__ void fwd(T& X, T& Y, W w) const {
+    uint64_t x = X >= q2 ? X - q2 : X;         // [0, 2q)
+    uint64_t t = shoup_lazy(Y, w.w, w.sh, q);  // [0, 2q)
+    X = x + t;                                 // [0, 4q)
+    Y = x - t + q2;                            // [0, 4q)
+  }
+  __device__ __forceinline__ void inv(T& X, T& Y, W w) const {
+    uint64_t u = X + Y;                        // X, Y in [0, 2q)
+    u = u >= q2 ? u - q2 : u;
+    uint64_t v = X - Y + q2;                   // [0, 4q)
+    X = u;
+    Y = shoup_lazy(v, w.w, w.sh, q);           // [0, 2q)
+  }
+  __device__ __forceinline__ void recenter(T&) const {}
+  __device__ __forceinline__ uint64_t canon(T x) const {
+    x = x >= q2 ? x - q2 : x;
+    return x >= q ? x - q : x;
+  }
+};
+
+// ------------------------------------------------------------------------------- fp64 arithmetic
+struct FpA {
+  using T = double;
+  using W = double;
+  double q, qinv, half;
+  __device__ __forceinline__ W tw(const double* t, const double*, uint32_t i) const { return t[i]; }
+  // y * w mod q for |y| < 4q, |w| <= q/2: exact, result in about (-1.3q, 1.3q)
+  __device__ __forceinline__ double mulmod(double y, double w) const {
+    const double h = y * w;
+    const double l = fma(y, w, -h);
+    const double c = rint(h * qinv);
+    const double d = fma(-c, q, h);
+    return d + l;
+  }
+  __device__ __forceinline__ T from_canon(uint64_t v) const {
+    const double x = (double)v;
+    return x > half ? x - q : x;
+  }
+  __device__ __forceinline__ T from_mid(uint64_t v) const { return __longlong_as_double((long long)v); }
+  __device__ __forceinline__ uint64_t to_mid(T v) const { return (uint64_t)__double_as_longlong(v); }
+  __device__ __forceinline__ void fwd(T& X, T& Y, W w) const {
+    const double t = mulmod(Y, w);
+    Y = X - t;
+    X = X + t;
+  }
+  __device__ __forceinline__ void inv(T& X, T& Y, W w) const {
+    const double u = X + Y, v = X - Y;
+    X = u;
+    Y = mulmod(v, w);
+  }
+  __device__ __forceinline__ void recenter(T& x) const { x = fma(-rint(x * qinv), q, x); }
+  __device__ __forceinline__ uint64_t canon(T x) const {
+    recenter(x);
+    x = x < 0 ? x + q : x;
+    x = x >= q ? x - q : x;
+    return (uint64_t)x;
+  }
+};
+
+// per-limb arithmetic selection
+struct Sel {
+  bool fp;
+  IntA ia;
+  FpA fa;
+  const uint64_t *tw, *twsh;
+  const double* twd;
+};
+
+__device__ __forceinline__ Sel select(const NttTables& tb, int pid, bool inverse) {
+  const PrimeConst& P = tb.prime[pid];
+  Sel s;
+  s.fp = P.fp64 != 0;
+  s.ia = IntA{P.q, 2 * P.q};
+  s.fa = FpA{P.qd, P.qinv, 0.5 * P.qd};
+  const size_t o = (size_t)pid * tb.N;
+  s.tw = (inverse ? tb.itw : tb.tw) + o;
+  s.twsh = (inverse ? tb.itw_sh : tb.tw_sh) + o;
+  s.twd = (inverse ? tb.itwd : tb.twd) + o;
+  return s;
+}
+
+// ------------------------------------------------------------------------------- forward, columns
+// First l1 stages. Input: canonical (in place or out of place), output: internal form.
+template <int LS, class A, class TwT>
+__device__ __forceinline__ void fwd_cols_body(const A& ar, const TwT* sw, const TwT* swp, const uint64_t* p, uint64_t* po,
+                                              uint64_t* tile, uint32_t N2, int C) {
+  constexpr int S = 1 << LS, T = S / 16, PADS = S + 1;
+  const int tid = threadIdx.x;
+  const int cc = tid % C, r0 = tid / C;
+  const uint32_t col = blockIdx.x * C + cc;
+  typename A::T x[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) x[k] = ar.from_canon(p[(size_t)(r0 + T * k) * N2 + col]);
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int h = 8 >> s;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (k & h) continue;
+      const int idx = (1 << s) + (k >> (4 - s));
+      ar.fwd(x[k], x[k + h], ar.tw(sw, swp, idx));
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) ar.recenter(x[k]);
+  if constexpr (LS > 4) {
+    uint64_t* t = tile + cc * PADS;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) t[r0 + T * k] = ar.to_mid(x[k]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(t[16 * r0 + k]);
+#pragma unroll
+    for (int s = 4; s < LS; ++s) {
+      const int h = S >> (s + 1);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        if (k & h) continue;
+        const int idx = (1 << s) + ((16 * r0 + k) >> (LS - s));
+        ar.fwd(x[k], x[k + h], ar.tw(sw, swp, idx));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      ar.recenter(x[k]);
+      po[(size_t)(16 * r0 + k) * N2 + col] = ar.to_mid(x[k]);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) po[(size_t)(r0 + T * k) * N2 + col] = ar.to_mid(x[k]);
+  }
+}
+
 template <int LS, class Addr, class AddrOut>
 __global__ __launch_bounds__(kThreads) void k_ntt_fwd_cols(Addr ad, AddrOut ao, NttTables tb) {
   constexpr int S = 1 << LS, T = S / 16, PADS = S + 1, CMAX = kThreads / T;
@@ -55,71 +194,32 @@ __global__ __launch_bounds__(kThreads) void k_ntt_fwd_cols(Addr ad, AddrOut ao, 
   int pid, pido;
   ad.at(blockIdx.y, tb.N, p, pid);
   ao.at(blockIdx.y, tb.N, po, pido);
-  const uint64_t q = tb.prime[pid].q, q2 = 2 * q;
-  const uint64_t* tw = tb.tw + (size_t)pid * tb.N;
-  const uint64_t* twp = tb.tw_sh + (size_t)pid * tb.N;
-  const int tid = threadIdx.x, C = blockDim.x / T;
-  for (int i = tid; i < S; i += blockDim.x) { sw[i] = tw[i]; swp[i] = twp[i]; }
+  const Sel sel = select(tb, pid, false);
+  const int C = blockDim.x / T;
   const uint32_t N2 = tb.N >> LS;
-  const int cc = tid % C, r0 = tid / C;
-  const uint32_t col = blockIdx.x * C + cc;
-  uint64_t x[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) x[k] = p[(size_t)(r0 + T * k) * N2 + col];
-  __syncthreads();
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const int h = 8 >> s;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      if (k & h) continue;
-      const int idx = (1 << s) + (k >> (4 - s));
-      fwd_bfly(x[k], x[k + h], sw[idx], swp[idx], q, q2);
-    }
-  }
-  if constexpr (LS > 4) {
-    uint64_t* t = tile + cc * PADS;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) t[r0 + T * k] = x[k];
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = t[16 * r0 + k];
-#pragma unroll
-    for (int s = 4; s < LS; ++s) {
-      const int h = S >> (s + 1);
-#pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        if (k & h) continue;
-        const int idx = (1 << s) + ((16 * r0 + k) >> (LS - s));
-        fwd_bfly(x[k], x[k + h], sw[idx], swp[idx], q, q2);
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < 16; ++k) po[(size_t)(16 * r0 + k) * N2 + col] = x[k];
+  if (sel.fp) {
+    double* swd = reinterpret_cast<double*>(sw);
+    for (int i = threadIdx.x; i < S; i += blockDim.x) swd[i] = sel.twd[i];
+    fwd_cols_body<LS>(sel.fa, (const double*)swd, (const double*)swd, p, po, tile, N2, C);
   } else {
-#pragma unroll
-    for (int k = 0; k < 16; ++k) po[(size_t)(r0 + T * k) * N2 + col] = x[k];
+    for (int i = threadIdx.x; i < S; i += blockDim.x) { sw[i] = sel.tw[i]; swp[i] = sel.twsh[i]; }
+    fwd_cols_body<LS>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, p, po, tile, N2, C);
   }
 }
 
-// ---------------------------------------------------------------- forward, rows (last l2 stages)
-template <int LS, class Addr>
-__global__ __launch_bounds__(kThreads) void k_ntt_fwd_rows(Addr ad, NttTables tb, int l1) {
-  constexpr int S = 1 << LS, T = S / 16, PADS = S + 1, RMAX = kThreads / T;
-  __shared__ uint64_t tile[RMAX * PADS];
-  uint64_t* p;
-  int pid;
-  ad.at(blockIdx.y, tb.N, p, pid);
-  const uint64_t q = tb.prime[pid].q, q2 = 2 * q;
-  const uint64_t* tw = tb.tw + (size_t)pid * tb.N;
-  const uint64_t* twp = tb.tw_sh + (size_t)pid * tb.N;
-  const int tid = threadIdx.x, R = blockDim.x / T;
+// ------------------------------------------------------------------------------- forward, rows
+// Last l2 stages. Input: internal form, output: canonical residues.
+template <int LS, class A, class TwT>
+__device__ __forceinline__ void fwd_rows_body(const A& ar, const TwT* tw, const TwT* twp, uint64_t* p, uint64_t* tile, int l1,
+                                              int R) {
+  constexpr int S = 1 << LS, T = S / 16, PADS = S + 1;
+  const int tid = threadIdx.x;
   const int rr = tid / T, c0 = tid % T;
   const uint32_t rho = blockIdx.x * R + rr;
   uint64_t* row = p + (size_t)rho * S;
-  uint64_t x[16];
+  typename A::T x[16];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) x[k] = row[c0 + T * k];
+  for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(row[c0 + T * k]);
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     const int h = 8 >> s;
@@ -127,16 +227,19 @@ __global__ __launch_bounds__(kThreads) void k_ntt_fwd_rows(Addr ad, NttTables tb
     for (int k = 0; k < 16; ++k) {
       if (k & h) continue;
       const uint32_t idx = (1u << (l1 + s)) + (rho << s) + (k >> (4 - s));
-      fwd_bfly(x[k], x[k + h], tw[idx], twp[idx], q, q2);
+      ar.fwd(x[k], x[k + h], ar.tw(tw, twp, idx));
     }
   }
   uint64_t* t = tile + rr * PADS;
   if constexpr (LS > 4) {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) t[c0 + T * k] = x[k];
+    for (int k = 0; k < 16; ++k) {
+      ar.recenter(x[k]);
+      t[c0 + T * k] = ar.to_mid(x[k]);
+    }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = t[16 * c0 + k];
+    for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(t[16 * c0 + k]);
 #pragma unroll
     for (int s = 4; s < LS; ++s) {
       const int h = S >> (s + 1);
@@ -144,22 +247,98 @@ __global__ __launch_bounds__(kThreads) void k_ntt_fwd_rows(Addr ad, NttTables tb
       for (int k = 0; k < 16; ++k) {
         if (k & h) continue;
         const uint32_t idx = (1u << (l1 + s)) + (rho << s) + ((16 * c0 + k) >> (LS - s));
-        fwd_bfly(x[k], x[k + h], tw[idx], twp[idx], q, q2);
+        ar.fwd(x[k], x[k + h], ar.tw(tw, twp, idx));
       }
     }
     // each thread rewrites exactly the LDS words it read: no barrier needed before the store
 #pragma unroll
-    for (int k = 0; k < 16; ++k) t[16 * c0 + k] = canon4(x[k], q, q2);
+    for (int k = 0; k < 16; ++k) t[16 * c0 + k] = ar.canon(x[k]);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 16; ++k) row[c0 + T * k] = t[c0 + T * k];
   } else {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) row[c0 + T * k] = canon4(x[k], q, q2);
+    for (int k = 0; k < 16; ++k) row[c0 + T * k] = ar.canon(x[k]);
   }
 }
 
-// ---------------------------------------------------------------- inverse, rows (first: last l2 stages reversed)
+template <int LS, class Addr>
+__global__ __launch_bounds__(kThreads) void k_ntt_fwd_rows(Addr ad, NttTables tb, int l1) {
+  constexpr int S = 1 << LS, T = S / 16, PADS = S + 1, RMAX = kThreads / T;
+  __shared__ uint64_t tile[RMAX * PADS];
+  uint64_t* p;
+  int pid;
+  ad.at(blockIdx.y, tb.N, p, pid);
+  const Sel sel = select(tb, pid, false);
+  const int R = blockDim.x / T;
+  if (sel.fp) fwd_rows_body<LS>(sel.fa, sel.twd, sel.twd, p, tile, l1, R);
+  else fwd_rows_body<LS>(sel.ia, sel.tw, sel.twsh, p, tile, l1, R);
+}
+
+// ------------------------------------------------------------------------------- inverse, rows
+// First launch of the INTT (last l2 stages, reversed). Input: canonical, output: internal form.
+// fp64 values are re-centred after every second stage (Gentleman-Sande sums double per stage).
+template <int LS, class A, class TwT>
+__device__ __forceinline__ void inv_rows_body(const A& ar, const TwT* tw, const TwT* twp, const uint64_t* p, uint64_t* po,
+                                              uint64_t* tile, int l1, int R) {
+  constexpr int S = 1 << LS, T = S / 16, PADS = S + 1;
+  const int tid = threadIdx.x;
+  const int rr = tid / T, c0 = tid % T;
+  const uint32_t rho = blockIdx.x * R + rr;
+  const uint64_t* row = p + (size_t)rho * S;
+  uint64_t* rowo = po + (size_t)rho * S;
+  uint64_t* t = tile + rr * PADS;
+  typename A::T x[16];
+  if constexpr (LS > 4) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) t[c0 + T * k] = row[c0 + T * k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x[k] = ar.from_canon(t[16 * c0 + k]);
+#pragma unroll
+    for (int s = LS - 1; s >= 4; --s) {
+      const int h = S >> (s + 1);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        if (k & h) continue;
+        const uint32_t idx = (1u << (l1 + s)) + (rho << s) + ((16 * c0 + k) >> (LS - s));
+        ar.inv(x[k], x[k + h], ar.tw(tw, twp, idx));
+      }
+      if (((LS - s) & 1) == 0) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) ar.recenter(x[k]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      ar.recenter(x[k]);
+      t[16 * c0 + k] = ar.to_mid(x[k]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(t[c0 + T * k]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x[k] = ar.from_canon(row[c0 + T * k]);
+  }
+#pragma unroll
+  for (int s = 3; s >= 0; --s) {
+    const int h = 8 >> s;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (k & h) continue;
+      const uint32_t idx = (1u << (l1 + s)) + (rho << s) + (k >> (4 - s));
+      ar.inv(x[k], x[k + h], ar.tw(tw, twp, idx));
+    }
+    if ((s & 1) == 0) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) ar.recenter(x[k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) rowo[c0 + T * k] = ar.to_mid(x[k]);
+}
+
 template <int LS, class Addr, class AddrOut>
 __global__ __launch_bounds__(kThreads) void k_ntt_inv_rows(Addr ad, AddrOut ao, NttTables tb, int l1) {
   constexpr int S = 1 << LS, T = S / 16, PADS = S + 1, RMAX = kThreads / T;
@@ -168,78 +347,27 @@ __global__ __launch_bounds__(kThreads) void k_ntt_inv_rows(Addr ad, AddrOut ao, 
   int pid, pido;
   ad.at(blockIdx.y, tb.N, p, pid);
   ao.at(blockIdx.y, tb.N, po, pido);
-  const uint64_t q = tb.prime[pid].q, q2 = 2 * q;
-  const uint64_t* tw = tb.itw + (size_t)pid * tb.N;
-  const uint64_t* twp = tb.itw_sh + (size_t)pid * tb.N;
-  const int tid = threadIdx.x, R = blockDim.x / T;
-  const int rr = tid / T, c0 = tid % T;
-  const uint32_t rho = blockIdx.x * R + rr;
-  const uint64_t* row = p + (size_t)rho * S;
-  uint64_t* rowo = po + (size_t)rho * S;
-  uint64_t* t = tile + rr * PADS;
-  uint64_t x[16];
-  if constexpr (LS > 4) {
-#pragma unroll
-    for (int k = 0; k < 16; ++k) t[c0 + T * k] = row[c0 + T * k];
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = t[16 * c0 + k];
-#pragma unroll
-    for (int s = LS - 1; s >= 4; --s) {
-      const int h = S >> (s + 1);
-#pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        if (k & h) continue;
-        const uint32_t idx = (1u << (l1 + s)) + (rho << s) + ((16 * c0 + k) >> (LS - s));
-        inv_bfly(x[k], x[k + h], tw[idx], twp[idx], q, q2);
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < 16; ++k) t[16 * c0 + k] = x[k];
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = t[c0 + T * k];
-  } else {
-#pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = row[c0 + T * k];
-  }
-#pragma unroll
-  for (int s = 3; s >= 0; --s) {
-    const int h = 8 >> s;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      if (k & h) continue;
-      const uint32_t idx = (1u << (l1 + s)) + (rho << s) + (k >> (4 - s));
-      inv_bfly(x[k], x[k + h], tw[idx], twp[idx], q, q2);
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < 16; ++k) rowo[c0 + T * k] = x[k];
+  const Sel sel = select(tb, pid, true);
+  const int R = blockDim.x / T;
+  if (sel.fp) inv_rows_body<LS>(sel.fa, sel.twd, sel.twd, p, po, tile, l1, R);
+  else inv_rows_body<LS>(sel.ia, sel.tw, sel.twsh, p, po, tile, l1, R);
 }
 
-// ---------------------------------------------------------------- inverse, columns (second: first l1 stages reversed, N^-1)
-template <int LS, class Addr>
-__global__ __launch_bounds__(kThreads) void k_ntt_inv_cols(Addr ad, NttTables tb) {
-  constexpr int S = 1 << LS, T = S / 16, PADS = S + 1, CMAX = kThreads / T;
-  __shared__ uint64_t tile[CMAX * PADS];
-  __shared__ uint64_t sw[S], swp[S];
-  uint64_t* p;
-  int pid;
-  ad.at(blockIdx.y, tb.N, p, pid);
-  const PrimeConst& P = tb.prime[pid];
-  const uint64_t q = P.q, q2 = 2 * q, ninv = P.ninv, ninv_sh = P.ninv_sh;
-  const uint64_t* tw = tb.itw + (size_t)pid * tb.N;
-  const uint64_t* twp = tb.itw_sh + (size_t)pid * tb.N;
-  const int tid = threadIdx.x, C = blockDim.x / T;
-  for (int i = tid; i < S; i += blockDim.x) { sw[i] = tw[i]; swp[i] = twp[i]; }
-  const uint32_t N2 = tb.N >> LS;
+// ------------------------------------------------------------------------------- inverse, columns
+// Second launch of the INTT (first l1 stages, reversed) with the N^-1 scaling. Input: internal
+// form, output: canonical residues.
+template <int LS, class A, class TwT, class Fin>
+__device__ __forceinline__ void inv_cols_body(const A& ar, const TwT* sw, const TwT* swp, uint64_t* p, uint64_t* tile, uint32_t N2,
+                                              int C, const Fin& fin) {
+  constexpr int S = 1 << LS, T = S / 16, PADS = S + 1;
+  const int tid = threadIdx.x;
   const int cc = tid % C, r0 = tid / C;
   const uint32_t col = blockIdx.x * C + cc;
-  uint64_t x[16];
+  typename A::T x[16];
   uint64_t* t = tile + cc * PADS;
   if constexpr (LS > 4) {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = p[(size_t)(16 * r0 + k) * N2 + col];
+    for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(p[(size_t)(16 * r0 + k) * N2 + col]);
     __syncthreads();
 #pragma unroll
     for (int s = LS - 1; s >= 4; --s) {
@@ -248,17 +376,24 @@ __global__ __launch_bounds__(kThreads) void k_ntt_inv_cols(Addr ad, NttTables tb
       for (int k = 0; k < 16; ++k) {
         if (k & h) continue;
         const int idx = (1 << s) + ((16 * r0 + k) >> (LS - s));
-        inv_bfly(x[k], x[k + h], sw[idx], swp[idx], q, q2);
+        ar.inv(x[k], x[k + h], ar.tw(sw, swp, idx));
+      }
+      if (((LS - s) & 1) == 0) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) ar.recenter(x[k]);
       }
     }
 #pragma unroll
-    for (int k = 0; k < 16; ++k) t[16 * r0 + k] = x[k];
+    for (int k = 0; k < 16; ++k) {
+      ar.recenter(x[k]);
+      t[16 * r0 + k] = ar.to_mid(x[k]);
+    }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = t[r0 + T * k];
+    for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(t[r0 + T * k]);
   } else {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = p[(size_t)(r0 + T * k) * N2 + col];
+    for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(p[(size_t)(r0 + T * k) * N2 + col]);
     __syncthreads();
   }
 #pragma unroll
@@ -268,16 +403,54 @@ __global__ __launch_bounds__(kThreads) void k_ntt_inv_cols(Addr ad, NttTables tb
     for (int k = 0; k < 16; ++k) {
       if (k & h) continue;
       const int idx = (1 << s) + (k >> (4 - s));
-      inv_bfly(x[k], x[k + h], sw[idx], swp[idx], q, q2);
+      ar.inv(x[k], x[k + h], ar.tw(sw, swp, idx));
+    }
+    if ((s & 1) == 0) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) ar.recenter(x[k]);
     }
   }
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    uint64_t v = shoup_lazy(x[k], ninv, ninv_sh, q);
-    p[(size_t)(r0 + T * k) * N2 + col] = v >= q ? v - q : v;
+  for (int k = 0; k < 16; ++k) p[(size_t)(r0 + T * k) * N2 + col] = fin(x[k]);
+}
+
+struct IntFin {
+  IntA a;
+  uint64_t s, ssh;
+  __device__ __forceinline__ uint64_t operator()(uint64_t v) const {
+    uint64_t r = shoup_lazy(v, s, ssh, a.q);
+    return r >= a.q ? r - a.q : r;
+  }
+};
+struct FpFin {
+  FpA a;
+  double s;
+  __device__ __forceinline__ uint64_t operator()(double v) const { return a.canon(a.mulmod(v, s)); }
+};
+
+template <int LS, class Addr>
+__global__ __launch_bounds__(kThreads) void k_ntt_inv_cols(Addr ad, NttTables tb) {
+  constexpr int S = 1 << LS, T = S / 16, PADS = S + 1, CMAX = kThreads / T;
+  __shared__ uint64_t tile[CMAX * PADS];
+  __shared__ uint64_t sw[S], swp[S];
+  uint64_t* p;
+  int pid;
+  ad.at(blockIdx.y, tb.N, p, pid);
+  const Sel sel = select(tb, pid, true);
+  const PrimeConst& P = tb.prime[pid];
+  const int C = blockDim.x / T;
+  const uint32_t N2 = tb.N >> LS;
+  if (sel.fp) {
+    double* swd = reinterpret_cast<double*>(sw);
+    for (int i = threadIdx.x; i < S; i += blockDim.x) swd[i] = sel.twd[i];
+    inv_cols_body<LS>(sel.fa, (const double*)swd, (const double*)swd, p, tile, N2, C, FpFin{sel.fa, P.ninv_d});
+  } else {
+    for (int i = threadIdx.x; i < S; i += blockDim.x) { sw[i] = sel.tw[i]; swp[i] = sel.twsh[i]; }
+    inv_cols_body<LS>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, p, tile, N2, C, IntFin{sel.ia, P.ninv, P.ninv_sh});
   }
 }
 
+// ------------------------------------------------------------------------------- launch
 template <int LS>
 inline dim3 col_block(uint32_t N) {
   constexpr int T = (1 << LS) / 16;
@@ -294,40 +467,57 @@ inline dim3 row_block(uint32_t N) {
   return dim3(R * T);
 }
 
+// Offset addressing of a chunk of a larger job space (blockIdx.y + base).
+template <class Addr>
+struct Shift {
+  Addr a;
+  uint32_t base;
+  __device__ __forceinline__ void at(uint32_t y, uint32_t N, uint64_t*& p, int& pid) const { a.at(y + base, N, p, pid); }
+};
+
+// limbs per chunk: the intermediate of one chunk stays MALL-resident between the two launches
+inline uint32_t chunk_limbs(uint32_t N) {
+  const size_t budget = 96ull << 20;
+  return (uint32_t)std::max<size_t>(1, budget / (8ull * N));
+}
+
 template <class Addr, class AddrOut, int L1S, int L2S>
 void fwd_impl(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlimbs, hipStream_t st) {
   NttTables tb = ntt_tables(c);
-  dim3 b1 = col_block<L1S>(c.N);
-  uint32_t C = b1.x / ((1 << L1S) / 16);
-  dim3 g1((c.N >> L1S) / C, nlimbs);
-  k_ntt_fwd_cols<L1S, Addr, AddrOut><<<g1, b1, 0, st>>>(ad, ao, tb);
-  dim3 b2 = row_block<L2S>(c.N);
-  uint32_t R = b2.x / ((1 << L2S) / 16);
-  dim3 g2((c.N >> L2S) / R, nlimbs);
-  k_ntt_fwd_rows<L2S, AddrOut><<<g2, b2, 0, st>>>(ao, tb, L1S);
+  dim3 b1 = col_block<L1S>(c.N), b2 = row_block<L2S>(c.N);
+  uint32_t C = b1.x / ((1 << L1S) / 16), R = b2.x / ((1 << L2S) / 16);
+  const uint32_t chunk = chunk_limbs(c.N);
+  for (uint32_t base = 0; base < nlimbs; base += chunk) {
+    uint32_t n = std::min(chunk, nlimbs - base);
+    Shift<Addr> sa{ad, base};
+    Shift<AddrOut> so{ao, base};
+    { ProfScope ps_(c, "ntt_fwd_cols", st); k_ntt_fwd_cols<L1S><<<dim3((c.N >> L1S) / C, n), b1, 0, st>>>(sa, so, tb); }
+    { ProfScope ps_(c, "ntt_fwd_rows", st); k_ntt_fwd_rows<L2S><<<dim3((c.N >> L2S) / R, n), b2, 0, st>>>(so, tb, L1S); }
+  }
 }
 
 template <class Addr, class AddrOut, int L1S, int L2S>
 void inv_impl(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlimbs, hipStream_t st) {
   NttTables tb = ntt_tables(c);
-  dim3 b2 = row_block<L2S>(c.N);
-  uint32_t R = b2.x / ((1 << L2S) / 16);
-  dim3 g2((c.N >> L2S) / R, nlimbs);
-  k_ntt_inv_rows<L2S, Addr, AddrOut><<<g2, b2, 0, st>>>(ad, ao, tb, L1S);
-  dim3 b1 = col_block<L1S>(c.N);
-  uint32_t C = b1.x / ((1 << L1S) / 16);
-  dim3 g1((c.N >> L1S) / C, nlimbs);
-  k_ntt_inv_cols<L1S, AddrOut><<<g1, b1, 0, st>>>(ao, tb);
+  dim3 b1 = col_block<L1S>(c.N), b2 = row_block<L2S>(c.N);
+  uint32_t C = b1.x / ((1 << L1S) / 16), R = b2.x / ((1 << L2S) / 16);
+  const uint32_t chunk = chunk_limbs(c.N);
+  for (uint32_t base = 0; base < nlimbs; base += chunk) {
+    uint32_t n = std::min(chunk, nlimbs - base);
+    Shift<Addr> sa{ad, base};
+    Shift<AddrOut> so{ao, base};
+    { ProfScope ps_(c, "ntt_inv_rows", st); k_ntt_inv_rows<L2S><<<dim3((c.N >> L2S) / R, n), b2, 0, st>>>(sa, so, tb, L1S); }
+    { ProfScope ps_(c, "ntt_inv_cols", st); k_ntt_inv_cols<L1S><<<dim3((c.N >> L1S) / C, n), b1, 0, st>>>(so, tb); }
+  }
 }
 
 template <class Addr, class AddrOut>
 void dispatch(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlimbs, hipStream_t st, bool fwd) {
   if (nlimbs == 0) return;
-  if (nlimbs > 65535u) throw Error(-1, "too many limbs in one NTT launch");
-#define FH_NTT_CASE(LN, A, B)                                   \
-  case LN:                                                      \
+#define FH_NTT_CASE(LN, A, B)                                      \
+  case LN:                                                         \
     if (fwd) fwd_impl<Addr, AddrOut, A, B>(c, ad, ao, nlimbs, st); \
-    else inv_impl<Addr, AddrOut, A, B>(c, ad, ao, nlimbs, st);  \
+    else inv_impl<Addr, AddrOut, A, B>(c, ad, ao, nlimbs, st);     \
     break;
   switch (c.logN) {
     FH_NTT_CASE(10, 5, 5)

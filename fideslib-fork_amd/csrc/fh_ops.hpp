@@ -12,5 +12,9 @@ void ew_binary(const Context& c, int op, const LimbSet& a, const LimbSet& b, con
 void ew_unary(const Context& c, int op, const LimbSet& a, const LimbSet& o, const uint64_t* scalars, hipStream_t st);
 // ct1, ct2: 2 x nlimbs over Q_l; out3: 3 x nlimbs. ct1 == ct2 selects the square form.
 void ew_tensor(const Context& c, const uint64_t* ct1, const uint64_t* ct2, uint64_t* out3, int nlimbs, hipStream_t st);
+// in place x -> x 2^64 mod q (resident key-switching keys are kept in Montgomery form)
+void to_montgomery(const Context& c, const LimbSet& s, hipStream_t st);
+// uniform residues mod each limb's prime (synthetic inputs)
+void fill_uniform(const Context& c, const LimbSet& s, uint64_t seed, hipStream_t st);
 
 }  // namespace fh

@@ -67,6 +67,12 @@ def _declare(L):
         "fh_eval_mult": (I, [P, VP, VP, I, P, VP]),
         "fh_eval_rotate": (I, [P, VP, I, P, C.c_uint32, VP]),
         "fh_eval_rotate_ext": (I, [P, VP, I, P, C.c_uint32, I, VP]),
+        "fh_rotate_batch": (I, [P, I, VP, S, I, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), VP]),
+        "fh_rotate_hoisted": (I, [P, VP, I, I, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), I, I, VP]),
+        "fh_fill_uniform": (I, [P, VP, i32p, I, I, S, U]),
+        "fh_profile_enable": (I, [P, I]),
+        "fh_profile_reset": (I, [P]),
+        "fh_profile_read": (I, [P, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -191,6 +197,15 @@ class Context:
     def key(self, host_key):
         return Key(self, host_key)
 
+    def key_from_device(self, dev):
+        """A key copied from a device buffer in the upload layout."""
+        assert dev.words == self.dnum * 2 * (self.L1 + self.K) * self.N
+        k = Key.__new__(Key)
+        h = C.c_void_p()
+        _check(load().fh_key_create(self.h, C.cast(dev.ptr, C.POINTER(C.c_uint64)), C.byref(h)))
+        k.ctx, k.h = self, h.value
+        return k
+
     def num_digits(self, level):
         r = load().fh_num_digits(self.h, level)
         if r < 0:
@@ -226,6 +241,33 @@ class Context:
 
     def eval_rotate_ext(self, ct, level, key, galois, out_ext, add_c0=True):
         _check(load().fh_eval_rotate_ext(self.h, ct.ptr, level, key.h, galois, int(add_c0), out_ext.ptr))
+
+    def rotate_batch(self, cts, count, level, keys, galois, out, ct_stride=0, offset=0):
+        kp = (C.c_void_p * count)(*[k.h for k in keys])
+        gl = (C.c_uint32 * count)(*galois)
+        _check(load().fh_rotate_batch(self.h, count, cts.addr(offset), ct_stride, level, kp, gl, out.ptr))
+
+    def rotate_hoisted(self, ct, level, keys, galois, out, add_c0=True, moddown=True):
+        n = len(keys)
+        kp = (C.c_void_p * n)(*[k.h for k in keys])
+        gl = (C.c_uint32 * n)(*galois)
+        _check(load().fh_rotate_hoisted(self.h, ct.ptr, level, n, kp, gl, int(add_c0), int(moddown), out.ptr))
+
+    def fill_uniform(self, d, prime_ids=None, num_limbs=None, num_polys=1, poly_stride=0, seed=0, offset=0):
+        nl = num_limbs if num_limbs is not None else (len(prime_ids) if prime_ids is not None else d.shape[0])
+        _check(load().fh_fill_uniform(self.h, d.addr(offset), _i32(prime_ids), nl, num_polys, poly_stride, seed))
+
+    # ---- profiling (per-kernel-family HIP event timing) ----
+    def profile(self, on=True):
+        _check(load().fh_profile_enable(self.h, int(on)))
+
+    def profile_reset(self):
+        _check(load().fh_profile_reset(self.h))
+
+    def profile_read(self, name):
+        n, t = C.c_int(), C.c_double()
+        _check(load().fh_profile_read(self.h, name.encode(), C.byref(n), C.byref(t)))
+        return n.value, t.value
 
 
 class Key:
