@@ -413,6 +413,31 @@ int fh_rotate_hoisted(fh_context* ctx, const uint64_t* ct, int level, int n, con
   });
 }
 
+int fh_linear_transform(fh_context* ctx, const uint64_t* ct, int level, int row_size, int b_step, int stride, int offset, int slots,
+                        const uint64_t* diags, const fh_key* const* baby_keys, const uint32_t* baby_galois, const fh_key* giant_key,
+                        uint32_t giant_galois, const fh_key* offset_key, uint32_t offset_galois, uint64_t* out) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(ct, "ct"); need(diags, "diags"); need(out, "out");
+    check_level(c, level);
+    if (b_step < 1 || b_step > 16) throw fh::Error(FH_E_INVALID, "b_step must be in [1, 16]");
+    if (slots < 1 || slots > (int)c.N / 2) throw fh::Error(FH_E_INVALID, "slots out of range");
+    std::vector<const uint64_t*> bk(b_step, nullptr);
+    std::vector<uint32_t> bg(b_step, 1);
+    for (int i = 0; i < b_step; ++i) {
+      if (((long long)i * stride) % slots == 0) continue;
+      need(baby_keys, "baby_keys");
+      need(baby_galois, "baby_galois");
+      bk[i] = key_of(c, baby_keys[i]);
+      bg[i] = baby_galois[i];
+    }
+    const uint64_t* gk = giant_key ? key_of(c, giant_key) : nullptr;
+    const uint64_t* ok = offset_key ? key_of(c, offset_key) : nullptr;
+    fh::linear_transform(c, ct, level, row_size, b_step, stride, offset, slots, diags, bk.data(), bg.data(), gk, giant_galois, ok,
+                         offset_galois, out, c.stream);
+  });
+}
+
 int fh_fill_uniform(fh_context* ctx, uint64_t* polys, const int32_t* prime_ids, int num_limbs, int num_polys, size_t poly_stride,
                     uint64_t seed) {
   return guard([&] {

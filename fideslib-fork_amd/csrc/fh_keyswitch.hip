@@ -165,6 +165,7 @@ struct KskParams {
   const PrimeConst* prime;
   const uint64_t* key;      // [dnum][2][L1+K][N]
   const uint64_t* c0;       // optional: c0 over Q_l, added as P*c0 to out0's Q limbs
+  int c0_ext;               // 1: c0 is already P-scaled over Q_l ∪ P, added on every limb
   const uint64_t* pmod;     // [L1] [P]_{q_i}
   const uint64_t* pmod_sh;
 };
@@ -178,7 +179,8 @@ __global__ __launch_bounds__(kT) void k_ksk_inner(const uint64_t* __restrict__ d
   const PrimeConst P = kp.prime[p];
   const uint32_t E = kp.L1 + kp.K;
   const size_t N = kp.N;
-  const bool addc0 = kp.c0 != nullptr && pos < kp.lq;
+  const bool addc0 = kp.c0 != nullptr && !kp.c0_ext && pos < kp.lq;
+  const bool addext = kp.c0 != nullptr && kp.c0_ext;
   const uint64_t pm = addc0 ? kp.pmod[pos] : 0, pmsh = addc0 ? kp.pmod_sh[pos] : 0;
   for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < kp.N; n += gridDim.x * blockDim.x) {
     U128 a0, a1;
@@ -193,6 +195,7 @@ __global__ __launch_bounds__(kT) void k_ksk_inner(const uint64_t* __restrict__ d
     }
     uint64_t r0 = redc(a0.hi, a0.lo, P), r1 = redc(a1.hi, a1.lo, P);  // keys are in Montgomery form
     if (addc0) r0 = add_mod(r0, shoup(kp.c0[(size_t)pos * N + n], pm, pmsh, P.q), P.q);
+    if (addext) r0 = add_mod(r0, kp.c0[(size_t)pos * N + n], P.q);
     // gather form: out[j] = in[auto_src(j)]  <=>  scatter: out[auto_dst(n)] = in[n]
     uint32_t j = n;
     if (kp.galois != 1) {
@@ -490,10 +493,10 @@ void modup_batch(Context& c, const uint64_t* x, int l, int npolys, uint64_t x_st
 }
 
 void ksk_inner(Context& c, const uint64_t* digits, int l, const uint64_t* key, uint64_t* out0, uint64_t* out1,
-               const uint64_t* c0, uint32_t galois, hipStream_t st) {
+               const uint64_t* c0, uint32_t galois, hipStream_t st, bool c0_ext) {
   const uint32_t lq = l + 1, ext = lq + c.K;
   KskParams kp{c.N, lq, ext, (uint32_t)c.L1, (uint32_t)c.K, (uint32_t)c.num_digits(l), (uint32_t)c.logN, galois,
-               c.d_prime, key, c0, c.d_pmod, c.d_pmod_sh};
+               c.d_prime, key, c0, c0_ext ? 1 : 0, c.d_pmod, c.d_pmod_sh};
   if (galois != 1) {
     // the kernel scatters through the inverse map: pass k^-1 so that out[j] = in[src_k(j)]
     kp.galois = (uint32_t)h_inv_odd(galois, 2ull * c.N);

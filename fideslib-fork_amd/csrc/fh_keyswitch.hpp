@@ -18,8 +18,14 @@ void rotate_hoisted(Context& c, const uint64_t* ct, int l, int n, const uint64_t
                     bool do_moddown, uint64_t* out, hipStream_t st);
 // out0/out1: (l+1+K) limbs. c0 (optional, l+1 limbs) is folded as P*c0 into out0's Q limbs;
 // galois != 1 writes both outputs through the automorphism X -> X^galois.
+// c0_ext: c0 is already P-scaled over Q_l ∪ P (l+1+K limbs) and is added on every limb.
 void ksk_inner(Context& c, const uint64_t* digits, int l, const uint64_t* key, uint64_t* out0, uint64_t* out1,
-               const uint64_t* c0, uint32_t galois, hipStream_t st);
+               const uint64_t* c0, uint32_t galois, hipStream_t st, bool c0_ext = false);
+// BSGS linear transform in the extended-basis formulation (fh_lt.hip); keys[i] for the baby step
+// i*stride (null where i*stride = 0 mod slots), giant/offset keys may be null when unused.
+void linear_transform(Context& c, const uint64_t* ct, int l, int rowSize, int bStep, int stride, int offset, int slots,
+                      const uint64_t* diags, const uint64_t* const* baby_keys, const uint32_t* baby_gal, const uint64_t* giant_key,
+                      uint32_t giant_gal, const uint64_t* off_key, uint32_t off_gal, uint64_t* out, hipStream_t st);
 // npolys polynomials over Q_l ∪ P (x_stride words apart, 0 = packed) -> over Q_l (out_stride apart)
 void moddown(Context& c, const uint64_t* x, int l, uint64_t* out, int npolys, uint64_t x_stride, uint64_t out_stride, hipStream_t st);
 // ncomp components over Q_l -> over Q_{l-1}

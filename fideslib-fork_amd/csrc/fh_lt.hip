@@ -1,0 +1,168 @@
+// Hoisted BSGS linear transform (plaintext-diagonal matrix-vector product) in the
+// extended-basis formulation of the reference: baby-step rotations share one ModUp and
+// stay in Q_l ∪ P, one multiply-accumulate kernel forms every giant-step partial sum, and
+// the giant steps are folded by Horner's rule with only c1 brought back to Q_l before each
+// giant rotation (the extended c0 enters the key-switch product directly).
+// Reference: src/CKKS/LinearTransform.cu:65-254 (FUSED branch, ONLY_C1), MAC kernel
+// dotProductLtBatchedPt2___ (src/CKKS/ElemenwiseBatchKernels.cu:785-835) via
+// LTdotProductPtBatch (src/CKKS/LimbPartitionBatch.cu:266-524), Ciphertext::extend
+// (src/CKKS/Ciphertext.cpp:1087-1100), contract api/CryptoContext.hpp:460-517.
+#include <algorithm>
+
+#include "fh_keyswitch.hpp"
+
+namespace fh {
+
+namespace {
+
+constexpr int kT = 256;
+constexpr int kMaxBaby = 16;
+
+// out[comp][pos] = [P]_{q_pos} x[comp][pos] for Q limbs, 0 for the special limbs (the P-scaled
+// extended form of a ciphertext over Q_l).
+__global__ __launch_bounds__(kT) void k_extend_p(const uint64_t* __restrict__ x, uint64_t* __restrict__ out, uint32_t N, uint32_t lq,
+                                                  uint32_t ext, const PrimeConst* prime, const uint64_t* pmod, const uint64_t* pmod_sh) {
+  const uint32_t comp = blockIdx.y / ext, pos = blockIdx.y % ext;
+  uint64_t* o = out + ((size_t)comp * ext + pos) * N;
+  if (pos >= lq) {
+    for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) o[n] = 0;
+    return;
+  }
+  const uint64_t q = prime[pos].q, w = pmod[pos], wsh = pmod_sh[pos];
+  const uint64_t* xi = x + ((size_t)comp * lq + pos) * N;
+  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) o[n] = shoup(xi[n], w, wsh, q);
+}
+
+struct MacParams {
+  uint32_t N, lq, ext, L1, rowSize, bStep, gStep;
+  const PrimeConst* prime;
+};
+
+// R[j][comp][pos] = sum_{i: bStep j + i < rowSize} fast[i][comp][pos] (.) diag[bStep j + i][pos]
+// One lane holds the bStep baby-step words of both components for its (position, coefficient)
+// and streams the diagonals once.
+template <int BB>
+__global__ __launch_bounds__(kT) void k_lt_mac(const uint64_t* __restrict__ fast, const uint64_t* __restrict__ diags,
+                                                uint64_t* __restrict__ R, MacParams mp) {
+  const uint32_t pos = blockIdx.y;
+  const PrimeConst P = mp.prime[pos < mp.lq ? pos : mp.L1 + (pos - mp.lq)];
+  const size_t N = mp.N, E = (size_t)mp.ext * N;
+  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < mp.N; n += gridDim.x * blockDim.x) {
+    uint64_t f0[BB], f1[BB];
+#pragma unroll
+    for (int i = 0; i < BB; ++i) {
+      if (i < (int)mp.bStep) {
+        f0[i] = fast[((size_t)i * 2 + 0) * E + (size_t)pos * N + n];
+        f1[i] = fast[((size_t)i * 2 + 1) * E + (size_t)pos * N + n];
+      }
+    }
+    for (uint32_t j = 0; j < mp.gStep; ++j) {
+      U128 a0, a1;
+      a0.zero();
+      a1.zero();
+#pragma unroll
+      for (int i = 0; i < BB; ++i) {
+        const uint32_t k = j * mp.bStep + i;
+        if (i < (int)mp.bStep && k < mp.rowSize) {
+          const uint64_t d = diags[(size_t)k * E + (size_t)pos * N + n];
+          a0.mac(f0[i], d);
+          a1.mac(f1[i], d);
+        }
+      }
+      R[((size_t)j * 2 + 0) * E + (size_t)pos * N + n] = mod128(a0.hi, a0.lo, P);
+      R[((size_t)j * 2 + 1) * E + (size_t)pos * N + n] = mod128(a1.hi, a1.lo, P);
+    }
+  }
+}
+
+inline dim3 grid_n(uint32_t N, uint32_t y) { return dim3(std::max(1u, std::min((N + kT - 1) / kT, 256u)), y); }
+
+}  // namespace
+
+void linear_transform(Context& c, const uint64_t* ct, int l, int rowSize, int bStep, int stride, int offset, int slots,
+                      const uint64_t* diags, const uint64_t* const* baby_keys, const uint32_t* baby_gal, const uint64_t* giant_key,
+                      uint32_t giant_gal, const uint64_t* off_key, uint32_t off_gal, uint64_t* out, hipStream_t st) {
+  if (bStep < 1 || bStep > kMaxBaby) throw Error(-1, "bStep must be in [1, 16]");
+  if (rowSize < 1) throw Error(-1, "rowSize must be >= 1");
+  if (slots < 1 || slots > (int)c.N / 2) throw Error(-1, "slots out of range");
+  const uint32_t lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);
+  const size_t N = c.N, E = (size_t)ext * N;
+  const int gStep = (rowSize + bStep - 1) / bStep;
+  const bool giant = ((long long)bStep * stride) % slots != 0;
+  if (gStep > 1 && giant && !giant_key) throw Error(-1, "giant-step key required");
+  if (offset != 0 && !off_key) throw Error(-1, "offset key required");
+
+  // ---- baby steps: identity in P-scaled extended form, the others from one shared ModUp ----
+  DBuf fast(c.pool, (size_t)bStep * 2 * E);
+  std::vector<int> rot_idx;
+  std::vector<const uint64_t*> rk;
+  std::vector<uint32_t> rg;
+  for (int i = 0; i < bStep; ++i) {
+    if (((long long)i * stride) % slots == 0) {
+      ProfScope ps_(c, "lt_extend", st);
+      k_extend_p<<<grid_n(c.N, 2 * ext), kT, 0, st>>>(ct, fast.p + (size_t)i * 2 * E, c.N, lq, ext, c.d_prime, c.d_pmod, c.d_pmod_sh);
+      FH_HIP(hipGetLastError());
+    } else {
+      if (!baby_keys[i]) throw Error(-1, "baby-step key missing");
+      rot_idx.push_back(i);
+      rk.push_back(baby_keys[i]);
+      rg.push_back(baby_gal[i]);
+    }
+  }
+  if (!rot_idx.empty()) {
+    bool contiguous = true;
+    for (size_t t = 1; t < rot_idx.size(); ++t) contiguous &= rot_idx[t] == rot_idx[t - 1] + 1;
+    if (contiguous) {
+      rotate_hoisted(c, ct, l, (int)rot_idx.size(), rk.data(), rg.data(), true, false, fast.p + (size_t)rot_idx[0] * 2 * E, st);
+    } else {
+      DBuf tmp(c.pool, rot_idx.size() * 2 * E);
+      rotate_hoisted(c, ct, l, (int)rot_idx.size(), rk.data(), rg.data(), true, false, tmp.p, st);
+      for (size_t t = 0; t < rot_idx.size(); ++t)
+        FH_HIP(hipMemcpyAsync(fast.p + (size_t)rot_idx[t] * 2 * E, tmp.p + t * 2 * E, 2 * E * 8, hipMemcpyDeviceToDevice, st));
+    }
+  }
+
+  // ---- all giant-step partial sums in one multiply-accumulate pass ----
+  DBuf R(c.pool, (size_t)gStep * 2 * E);
+  MacParams mp{c.N, lq, ext, (uint32_t)c.L1, (uint32_t)rowSize, (uint32_t)bStep, (uint32_t)gStep, c.d_prime};
+  {
+    ProfScope ps_(c, "lt_mac", st);
+    if (bStep <= 4) k_lt_mac<4><<<grid_n(c.N, ext), kT, 0, st>>>(fast.p, diags, R.p, mp);
+    else if (bStep <= 8) k_lt_mac<8><<<grid_n(c.N, ext), kT, 0, st>>>(fast.p, diags, R.p, mp);
+    else k_lt_mac<16><<<grid_n(c.N, ext), kT, 0, st>>>(fast.p, diags, R.p, mp);
+    FH_HIP(hipGetLastError());
+  }
+
+  // ---- Horner over the giant steps ----
+  std::vector<int> epids(ext);
+  for (uint32_t p = 0; p < ext; ++p) epids[p] = c.ext_prime(l, p);
+  DBuf acc(c.pool, 2 * E), c1q(c.pool, (size_t)lq * N), dig(c.pool, (size_t)beta * E);
+  bool have_acc = false;
+  for (int j = gStep - 1; j >= 0; --j) {
+    uint64_t* Rj = R.p + (size_t)j * 2 * E;
+    if (have_acc) {
+      LimbSet a = make_set(Rj, ext, epids.data(), 2, E), b = make_set(acc.p, ext, epids.data(), 2, E);
+      ew_binary(c, 0, a, b, a, st);
+    }
+    if (j > 0) {
+      if (giant) {
+        moddown(c, Rj + E, l, c1q.p, 1, 0, 0, st);
+        modup(c, c1q.p, l, dig.p, st);
+        ksk_inner(c, dig.p, l, giant_key, acc.p, acc.p + E, Rj, giant_gal, st, true);
+      } else {
+        FH_HIP(hipMemcpyAsync(acc.p, Rj, 2 * E * 8, hipMemcpyDeviceToDevice, st));
+      }
+      have_acc = true;
+    } else if (offset != 0) {
+      moddown(c, Rj + E, l, c1q.p, 1, 0, 0, st);
+      modup(c, c1q.p, l, dig.p, st);
+      DBuf t(c.pool, 2 * E);
+      ksk_inner(c, dig.p, l, off_key, t.p, t.p + E, Rj, off_gal, st, true);
+      moddown(c, t.p, l, out, 2, 0, 0, st);
+    } else {
+      moddown(c, Rj, l, out, 2, 0, 0, st);
+    }
+  }
+}
+
+}  // namespace fh

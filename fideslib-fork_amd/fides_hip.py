@@ -70,6 +70,8 @@ def _declare(L):
         "fh_rotate_batch": (I, [P, I, VP, S, I, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), VP]),
         "fh_rotate_hoisted": (I, [P, VP, I, I, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), I, I, VP]),
         "fh_fill_uniform": (I, [P, VP, i32p, I, I, S, U]),
+        "fh_linear_transform": (I, [P, VP, I, I, I, I, I, I, VP, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), P, C.c_uint32, P,
+                                    C.c_uint32, VP]),
         "fh_profile_enable": (I, [P, I]),
         "fh_profile_reset": (I, [P]),
         "fh_profile_read": (I, [P, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
@@ -256,6 +258,15 @@ class Context:
     def fill_uniform(self, d, prime_ids=None, num_limbs=None, num_polys=1, poly_stride=0, seed=0, offset=0):
         nl = num_limbs if num_limbs is not None else (len(prime_ids) if prime_ids is not None else d.shape[0])
         _check(load().fh_fill_uniform(self.h, d.addr(offset), _i32(prime_ids), nl, num_polys, poly_stride, seed))
+
+    def linear_transform(self, ct, level, row_size, b_step, stride, diags, baby_keys, baby_galois, out, giant_key=None, giant_galois=1,
+                         offset_key=None, offset_galois=1, offset=0, slots=None):
+        slots = slots or self.N // 2
+        bk = (C.c_void_p * b_step)(*[(k.h if k is not None else None) for k in baby_keys])
+        bg = (C.c_uint32 * b_step)(*[int(g) for g in baby_galois])
+        _check(load().fh_linear_transform(self.h, ct.ptr, level, row_size, b_step, stride, offset, slots, diags.ptr, bk, bg,
+                                          giant_key.h if giant_key is not None else None, giant_galois,
+                                          offset_key.h if offset_key is not None else None, offset_galois, out.ptr))
 
     # ---- profiling (per-kernel-family HIP event timing) ----
     def profile(self, on=True):

@@ -722,28 +722,125 @@ void orc_eval_mult(const orc_ctx* c, const u64* ct1, const u64* ct2, int l, cons
  * with addFirst, then KeySwitchDown): t0 = sum digit*b + P*c0 (Q limbs), t1 = sum digit*a,
  * both automorphed in Q_l∪P. Reference: src/CKKS/Ciphertext.cpp:1102-1168,
  * src/CKKS/ElemenwiseBatchKernels.cu:277-377 (P*c0 term :315-319). */
-void orc_eval_rotate_ext(const orc_ctx* c, const u64* ct, int l, const u64* key, uint32_t k, int add_c0, u64* out_ext) {
+/* Rotation core. c0_mode 0: no c0; 1: c0 over Q_l folded as [P]_{q_i} c0; 2: c0 already in the
+ * P-scaled extended form over Q_l∪P, added on every limb (reference hoistedRotateDotKSK_2_,
+ * src/CKKS/ElemenwiseBatchKernels.cu:302-307, the c0_modup branch). */
+static void rotate_core(const orc_ctx* c, const u64* c0, int c0_mode, const u64* c1, int l, const u64* key, uint32_t k, u64* out_ext) {
   u64 N = c->N;
-  size_t P = (size_t)(l + 1) * N;
   int ext = l + 1 + c->K;
   int beta = orc_num_digits(c, l);
   u64* dig = (u64*)malloc((size_t)beta * ext * N * 8);
   u64* t0 = (u64*)malloc((size_t)ext * N * 8);
   u64* t1 = (u64*)malloc((size_t)ext * N * 8);
-  orc_modup(c, ct + P, l, dig);
+  orc_modup(c, c1, l, dig);
   orc_ksk_inner(c, dig, l, key, t0, t1);
-  if (add_c0) {
+  if (c0_mode == 1) {
     for (int i = 0; i <= l; ++i) {
       u64 qi = c->q[i];
       for (u64 j = 0; j < N; ++j) {
         size_t x = (size_t)i * N + j;
-        t0[x] = modadd(t0[x], orc_mulmod(ct[x], c->Pmod[i], qi), qi);
+        t0[x] = modadd(t0[x], orc_mulmod(c0[x], c->Pmod[i], qi), qi);
+      }
+    }
+  } else if (c0_mode == 2) {
+    for (int pos = 0; pos < ext; ++pos) {
+      u64 q = c->q[ext_prime(c, l, pos)];
+      for (u64 j = 0; j < N; ++j) {
+        size_t x = (size_t)pos * N + j;
+        t0[x] = modadd(t0[x], c0[x], q);
       }
     }
   }
   orc_automorph(c, t0, ext, k, out_ext);
   orc_automorph(c, t1, ext, k, out_ext + (size_t)ext * N);
   free(dig); free(t0); free(t1);
+}
+
+void orc_eval_rotate_ext(const orc_ctx* c, const u64* ct, int l, const u64* key, uint32_t k, int add_c0, u64* out_ext) {
+  rotate_core(c, ct, add_c0 ? 1 : 0, ct + (size_t)(l + 1) * c->N, l, key, k, out_ext);
+}
+
+/* BSGS linear transform in the reference's extended-basis formulation
+ * (src/CKKS/LinearTransform.cu:65-254, FUSED branch, ONLY_C1 = true):
+ *   fast[0] = (P c0, P c1) over Q_l∪P (Ciphertext::extend, src/CKKS/Ciphertext.cpp:1087-1100),
+ *   fast[i] = hoisted rotation by i*stride kept in Q_l∪P (no ModDown);
+ *   R_j = sum_{i, bStep j + i < rowSize} fast[i] (.) diag[bStep j + i]   (MAC, both components);
+ *   Horner from j = gStep-1 down: R_j += acc; for j > 0 and a non-trivial giant step, ModDown c1
+ *   only and rotate by bStep*stride with the extended c0 folded directly; at j = 0 rotate by
+ *   offset (then ModDown both) or ModDown both.
+ * diags: rowSize plaintexts over Q_l∪P (l+1+K limbs each). baby_keys: bStep keys (entry i used
+ * for i*stride != 0 mod slots), each dnum*2*(L1+K) limbs. Output: 2 x (l+1) limbs. */
+void orc_linear_transform(const orc_ctx* c, const u64* ct, int l, int rowSize, int bStep, int stride, int offset, int slots,
+                          const u64* diags, const u64* baby_keys, const uint32_t* baby_k, const u64* giant_key, uint32_t giant_k,
+                          const u64* off_key, uint32_t off_k, u64* out) {
+  u64 N = c->N;
+  int ext = l + 1 + c->K;
+  size_t E = (size_t)ext * N, Q = (size_t)(l + 1) * N;
+  size_t key_words = (size_t)c->dnum * 2 * c->np * N;
+  int gStep = (rowSize + bStep - 1) / bStep;
+  u64* fast = (u64*)calloc((size_t)bStep * 2 * E, 8);
+  for (int i = 0; i < bStep; ++i) {
+    u64* f = fast + (size_t)i * 2 * E;
+    if (((long long)i * stride) % slots == 0) {
+      for (int comp = 0; comp < 2; ++comp)
+        for (int p = 0; p <= l; ++p) {
+          u64 qp = c->q[p];
+          for (u64 j = 0; j < N; ++j) f[comp * E + (size_t)p * N + j] = orc_mulmod(ct[comp * Q + (size_t)p * N + j], c->Pmod[p], qp);
+        }
+    } else {
+      rotate_core(c, ct, 1, ct + Q, l, baby_keys + (size_t)i * key_words, baby_k[i], f);
+    }
+  }
+  u64* acc = (u64*)calloc(2 * E, 8);
+  u64* R = (u64*)malloc(2 * E * 8);
+  u64* c1q = (u64*)malloc(Q * 8);
+  int have_acc = 0;
+  for (int j = gStep - 1; j >= 0; --j) {
+    memset(R, 0, 2 * E * 8);
+    for (int i = 0; i < bStep && bStep * j + i < rowSize; ++i) {
+      const u64* d = diags + (size_t)(bStep * j + i) * E;
+      const u64* f = fast + (size_t)i * 2 * E;
+      for (int comp = 0; comp < 2; ++comp)
+#pragma omp parallel for
+        for (int pos = 0; pos < ext; ++pos) {
+          u64 q = c->q[ext_prime(c, l, pos)];
+          for (u64 n = 0; n < N; ++n) {
+            size_t x = (size_t)pos * N + n;
+            R[comp * E + x] = modadd(R[comp * E + x], orc_mulmod(f[comp * E + x], d[x], q), q);
+          }
+        }
+    }
+    if (have_acc) {
+      for (int comp = 0; comp < 2; ++comp)
+        for (int pos = 0; pos < ext; ++pos) {
+          u64 q = c->q[ext_prime(c, l, pos)];
+          for (u64 n = 0; n < N; ++n) {
+            size_t x = comp * E + (size_t)pos * N + n;
+            R[x] = modadd(R[x], acc[x], q);
+          }
+        }
+    }
+    if (j > 0) {
+      if (((long long)bStep * stride) % slots != 0) {
+        orc_moddown(c, R + E, l, c1q);
+        rotate_core(c, R, 2, c1q, l, giant_key, giant_k, acc);
+      } else {
+        memcpy(acc, R, 2 * E * 8);
+      }
+      have_acc = 1;
+    } else if (offset != 0) {
+      orc_moddown(c, R + E, l, c1q);
+      u64* t = (u64*)malloc(2 * E * 8);
+      rotate_core(c, R, 2, c1q, l, off_key, off_k, t);
+      orc_moddown(c, t, l, out);
+      orc_moddown(c, t + E, l, out + Q);
+      free(t);
+    } else {
+      orc_moddown(c, R, l, out);
+      orc_moddown(c, R + E, l, out + Q);
+    }
+  }
+  free(fast); free(acc); free(R); free(c1q);
 }
 
 void orc_eval_rotate(const orc_ctx* c, const u64* ct, int l, const u64* key, uint32_t k, u64* out) {

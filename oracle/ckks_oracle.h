@@ -105,6 +105,11 @@ void orc_eval_rotate(const orc_ctx* c, const uint64_t* ct, int l, const uint64_t
 /* rotation that stays in Q_l∪P (no ModDown): out is 2 x (l+1+K) limbs */
 void orc_eval_rotate_ext(const orc_ctx* c, const uint64_t* ct, int l, const uint64_t* key, uint32_t k, int add_c0, uint64_t* out_ext);
 
+/* BSGS linear transform in the reference's extended-basis (hoisted) formulation; see the .c file */
+void orc_linear_transform(const orc_ctx* c, const uint64_t* ct, int l, int rowSize, int bStep, int stride, int offset, int slots,
+                          const uint64_t* diags, const uint64_t* baby_keys, const uint32_t* baby_k, const uint64_t* giant_key,
+                          uint32_t giant_k, const uint64_t* off_key, uint32_t off_k, uint64_t* out);
+
 /* ---- key generation / encryption (seeded, mt19937_64) ---- */
 /* ternary secret, EVAL over Q∪P (L1+K limbs); hamming weight h (0 = uniform ternary) */
 void orc_gen_secret(const orc_ctx* c, uint64_t seed, int h, uint64_t* s);

@@ -77,6 +77,8 @@ def lib():
             "orc_eval_mult": (V, [P, u64p, u64p, I, u64p, u64p]),
             "orc_eval_rotate": (V, [P, u64p, I, u64p, U32, u64p]),
             "orc_eval_rotate_ext": (V, [P, u64p, I, u64p, U32, I, u64p]),
+            "orc_linear_transform": (V, [P, u64p, I, I, I, I, I, I, u64p, u64p, np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"),
+                                         u64p, U32, u64p, U32, u64p]),
             "orc_gen_secret": (V, [P, U, I, u64p]),
             "orc_gen_switch_key": (V, [P, u64p, u64p, U, u64p]),
             "orc_gen_relin_key": (V, [P, u64p, U, u64p]),
@@ -250,6 +252,21 @@ class Ctx:
     def eval_rotate_ext(self, ct, l, key, k, add_c0=True):
         out = np.zeros((2 * (l + 1 + self.K), self.N), dtype=np.uint64)
         lib().orc_eval_rotate_ext(self.h, np.ascontiguousarray(ct), l, np.ascontiguousarray(key), k, int(add_c0), out)
+        return out
+
+    def linear_transform(self, ct, l, diags, bStep, stride, baby_keys, baby_k, giant_key=None, giant_k=1, off_key=None, off_k=1,
+                         offset=0, slots=None):
+        """baby_keys: list of bStep keys (None where unused); diags: rowSize x (l+1+K) limbs."""
+        slots = slots or self.N // 2
+        rowSize = diags.shape[0] // (l + 1 + self.K)
+        zero = np.zeros(self.key_shape(), dtype=np.uint64)
+        bk = np.ascontiguousarray(np.concatenate([zero if k is None else k for k in baby_keys]))
+        bkk = np.ascontiguousarray(np.array([int(x) for x in baby_k], dtype=np.uint32))
+        gk = zero if giant_key is None else np.ascontiguousarray(giant_key)
+        ok = zero if off_key is None else np.ascontiguousarray(off_key)
+        out = self.zeros(l + 1, 2)
+        lib().orc_linear_transform(self.h, np.ascontiguousarray(ct), l, rowSize, bStep, stride, offset, slots,
+                                   np.ascontiguousarray(diags), bk, bkk, gk, giant_k, ok, off_k, out)
         return out
 
     # ---- keys / encoding / encryption ----

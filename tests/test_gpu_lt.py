@@ -1,0 +1,56 @@
+"""Hoisted BSGS linear transform on the GPU against the oracle's restatement of the reference's
+extended-basis formulation: every limb bit-exact, decrypted result = sum_k Rot(z, k*stride + offset) D_k."""
+import numpy as np
+import pytest
+
+from conftest import make_oracle_ctx
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(o, s, z, l, rowSize, bStep, stride, offset, scale_pt, seed):
+    n = o.N // 2
+    rng = np.random.default_rng(seed)
+    D = rng.uniform(-1, 1, (rowSize, n)) + 1j * rng.uniform(-1, 1, (rowSize, n))
+    diags = np.concatenate([o.encode_ext(np.roll(D[k], bStep * (k // bStep) * stride + offset), scale_pt, l) for k in range(rowSize)])
+    want = sum(np.roll(z, -(k * stride + offset)) * D[k] for k in range(rowSize))
+    return diags, want
+
+
+def _run(g, o, s, l, rowSize, bStep, stride, offset, scale_pt=2.0 ** 30, seed=0, check_decrypt=True):
+    n = o.N // 2
+    z = np.random.default_rng(seed + 1).uniform(-1, 1, n) + 1j * np.random.default_rng(seed + 2).uniform(-1, 1, n)
+    ct = o.encrypt(o.encode(z, 2.0 ** 50, l), l, s, seed + 3)
+    diags, want = _case(o, s, z, l, rowSize, bStep, stride, offset, scale_pt, seed)
+    bk = [None] + [o.gen_rot_key(s, o.rot_index(i * stride), 900 + i) for i in range(1, bStep)]
+    bkk = [1] + [o.rot_index(i * stride) for i in range(1, bStep)]
+    gk = o.gen_rot_key(s, o.rot_index(bStep * stride), 950)
+    ok = o.gen_rot_key(s, o.rot_index(offset), 960) if offset else None
+    exp = o.linear_transform(ct, l, diags, bStep, stride, bk, bkk, gk, o.rot_index(bStep * stride), ok,
+                             o.rot_index(offset) if offset else 1, offset)
+    dct, dd = g.to_device(ct), g.to_device(diags)
+    dbk = [None] + [g.key(k) for k in bk[1:]]
+    dgk = g.key(gk)
+    dok = g.key(ok) if offset else None
+    out = g.array(ct.shape)
+    g.linear_transform(dct, l, rowSize, bStep, stride, dd, dbk, bkk, out, giant_key=dgk, giant_galois=o.rot_index(bStep * stride),
+                       offset_key=dok, offset_galois=o.rot_index(offset) if offset else 1, offset=offset)
+    got = out.download()
+    assert np.array_equal(got, exp), "linear transform differs from the oracle"
+    if check_decrypt:
+        dec = o.decode(o.decrypt(got, l, s), l, n, 2.0 ** 50 * scale_pt)
+        assert np.max(np.abs(dec - want)) < 1e-5
+    for x in [dct, dd, out] + dbk[1:] + [dgk] + ([dok] if dok else []):
+        x.free()
+
+
+@pytest.mark.parametrize("rowSize,bStep,offset", [(12, 4, 0), (10, 4, 3), (8, 8, 0), (16, 4, 5)])
+def test_linear_transform_cfg1(rowSize, bStep, offset, gctx1, octx1):
+    s = octx1.gen_secret(70)
+    _run(gctx1, octx1, s, 3, rowSize, bStep, 1, offset, seed=rowSize + offset)
+
+
+def test_linear_transform_cfg4_shape(gctx2, octx2):
+    """The cfg4 shape: N=2^16, level 20 (21 limbs), 64 diagonals, bStep = gStep = 8, stride 1."""
+    s = octx2.gen_secret(71)
+    _run(gctx2, octx2, s, 20, 64, 8, 1, 0, seed=4, check_decrypt=True)

@@ -342,3 +342,30 @@ def test_keyswitch_decrypt_invariants():
     conj = c.eval_rotate(ct1, l, key, kc)
     dec = c.decode(c.decrypt(conj, l, s), l, n, Dl)
     assert np.max(np.abs(dec - np.conj(z1))) < 1e-9
+
+
+def _lt_case(c, s, z, l, rowSize, bStep, stride, offset, seed=0, scale_pt=2.0 ** 30):
+    n = c.N // 2
+    rng = np.random.default_rng(seed)
+    D = rng.uniform(-1, 1, (rowSize, n)) + 1j * rng.uniform(-1, 1, (rowSize, n))
+    diags = np.concatenate([c.encode_ext(np.roll(D[k], bStep * (k // bStep) * stride + offset), scale_pt, l) for k in range(rowSize)])
+    bk = [None] + [c.gen_rot_key(s, c.rot_index(i * stride), 500 + i) for i in range(1, bStep)]
+    bkk = [1] + [c.rot_index(i * stride) for i in range(1, bStep)]
+    gk = c.gen_rot_key(s, c.rot_index(bStep * stride), 600)
+    ok = c.gen_rot_key(s, c.rot_index(offset), 700) if offset else None
+    want = sum(np.roll(z, -(k * stride + offset)) * D[k] for k in range(rowSize))
+    return diags, bk, bkk, gk, ok, want
+
+
+@pytest.mark.parametrize("rowSize,bStep,offset", [(12, 4, 0), (10, 4, 3), (8, 8, 0)])
+def test_linear_transform_decrypts(rowSize, bStep, offset):
+    """BSGS linear transform (extended-basis formulation) decrypts to sum_k rot(z, k + offset) * D_k."""
+    c = small_ctx(10, 4, 2, 2)
+    n, l, Dl = c.N // 2, 3, 2.0 ** 50
+    s = c.gen_secret(17)
+    z = _slots(n, 18)
+    ct = c.encrypt(c.encode(z, Dl, l), l, s, 19)
+    diags, bk, bkk, gk, ok, want = _lt_case(c, s, z, l, rowSize, bStep, 1, offset)
+    out = c.linear_transform(ct, l, diags, bStep, 1, bk, bkk, gk, c.rot_index(bStep), ok, c.rot_index(offset) if offset else 1, offset)
+    dec = c.decode(c.decrypt(out, l, s), l, n, Dl * 2.0 ** 30)
+    assert np.max(np.abs(dec - want)) < 1e-6
