@@ -450,6 +450,28 @@ int fh_fill_uniform(fh_context* ctx, uint64_t* polys, const int32_t* prime_ids, 
   });
 }
 
+static int* tune_slot(fh::Tune& t, const char* name) {
+  if (!name) throw fh::Error(FH_E_INVALID, "null tunable name");
+  if (!std::strcmp(name, "ntt_chunk_mb")) return &t.ntt_chunk_mb;
+  throw fh::Error(FH_E_INVALID, std::string("unknown tunable: ") + name);
+}
+
+int fh_tune_set(fh_context* ctx, const char* name, int value) {
+  return guard([&] {
+    auto& c = C(ctx);
+    if (value < 1) throw fh::Error(FH_E_INVALID, "tunable values must be >= 1");
+    *tune_slot(c.tune, name) = value;
+  });
+}
+
+int fh_tune_get(const fh_context* ctx, const char* name, int* value) {
+  return guard([&] {
+    auto& c = const_cast<fh::Context&>(C(ctx));
+    need(value, "value");
+    *value = *tune_slot(c.tune, name);
+  });
+}
+
 int fh_profile_enable(fh_context* ctx, int enable) {
   return guard([&] { C(ctx).prof.on = enable != 0; });
 }

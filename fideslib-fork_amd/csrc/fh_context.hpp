@@ -101,6 +101,11 @@ struct Profiler {
   ~Profiler();
 };
 
+// Launch-shape tunables (set by the autotuner or by the caller through fh_tune_set).
+struct Tune {
+  int ntt_chunk_mb = 96;     // limbs per NTT launch pair, in MB of limb data (MALL residency)
+};
+
 struct Context;
 // Records an event pair around the launches issued in its scope when profiling is on.
 struct ProfScope {
@@ -155,6 +160,7 @@ struct Context {
   hipStream_t stream = nullptr;
   Pool pool;
   mutable Profiler prof;
+  Tune tune;
   std::vector<void*> owned;   // table allocations freed at destruction
 
   Context(int logN, int L1, int K, int dnum, const std::vector<uint64_t>& primes);

@@ -245,35 +245,51 @@ __global__ __launch_bounds__(kT) void k_ksk_batch(const uint64_t* __restrict__ d
   const uint32_t kinv = g.gal_inv[grp];
   const bool addc0 = kp.add_c0 && pos < kp.lq;
   const uint64_t pm = addc0 ? kp.pmod[pos] : 0, pmsh = addc0 ? kp.pmod_sh[pos] : 0;
-  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < kp.N; n += gridDim.x * blockDim.x) {
-    uint64_t kb[kMaxDnum], ka[kMaxDnum];
+  // two consecutive coefficients per lane: 16-byte loads of keys, digits and c0
+  for (uint32_t n = 2 * (blockIdx.x * blockDim.x + threadIdx.x); n < kp.N; n += 2 * gridDim.x * blockDim.x) {
+    ulonglong2 kb[kMaxDnum], ka[kMaxDnum];
 #pragma unroll
     for (uint32_t d = 0; d < kMaxDnum; ++d) {
       if (d < kp.beta) {
-        kb[d] = key[(((size_t)d * 2 + 0) * E + p) * N + n];
-        ka[d] = key[(((size_t)d * 2 + 1) * E + p) * N + n];
+        kb[d] = *reinterpret_cast<const ulonglong2*>(key + (((size_t)d * 2 + 0) * E + p) * N + n);
+        ka[d] = *reinterpret_cast<const ulonglong2*>(key + (((size_t)d * 2 + 1) * E + p) * N + n);
       }
     }
-    const uint32_t j = kinv != 1 ? auto_src(n, kinv, kp.logN) : n;
-    for (uint32_t t = 0; t < g.count[grp]; ++t) {
-      const uint32_t b = g.order[g.first[grp] + t];
+    const uint32_t j0 = kinv != 1 ? auto_src(n, kinv, kp.logN) : n;
+    const uint32_t j1 = kinv != 1 ? auto_src(n + 1, kinv, kp.logN) : n + 1;
+    const uint32_t cnt = g.count[grp], first = g.first[grp];
+#pragma unroll 2
+    for (uint32_t t = 0; t < cnt; ++t) {
+      const uint32_t b = g.order[first + t];
       const uint64_t* dg = digits + b * kp.dig_stride + (size_t)pos * N + n;
-      U128 a0, a1;
+      U128 a0, a1, b0, b1;
       a0.zero();
       a1.zero();
+      b0.zero();
+      b1.zero();
 #pragma unroll
       for (uint32_t d = 0; d < kMaxDnum; ++d) {
         if (d < kp.beta) {
-          const uint64_t x = dg[(size_t)d * kp.ext * N];
-          a0.mac(x, kb[d]);
-          a1.mac(x, ka[d]);
+          const ulonglong2 x = *reinterpret_cast<const ulonglong2*>(dg + (size_t)d * kp.ext * N);
+          a0.mac(x.x, kb[d].x);
+          a1.mac(x.x, ka[d].x);
+          b0.mac(x.y, kb[d].y);
+          b1.mac(x.y, ka[d].y);
         }
       }
-      uint64_t r0 = redc(a0.hi, a0.lo, P), r1 = redc(a1.hi, a1.lo, P);  // keys are in Montgomery form
-      if (addc0) r0 = add_mod(r0, shoup(kp.cts[b * kp.ct_stride + (size_t)pos * N + n], pm, pmsh, P.q), P.q);
+      // keys are in Montgomery form: one REDC gives sum digit * key mod p
+      uint64_t r0 = redc(a0.hi, a0.lo, P), r1 = redc(a1.hi, a1.lo, P);
+      uint64_t s0 = redc(b0.hi, b0.lo, P), s1 = redc(b1.hi, b1.lo, P);
+      if (addc0) {
+        const ulonglong2 c = *reinterpret_cast<const ulonglong2*>(kp.cts + b * kp.ct_stride + (size_t)pos * N + n);
+        r0 = add_mod(r0, shoup(c.x, pm, pmsh, P.q), P.q);
+        s0 = add_mod(s0, shoup(c.y, pm, pmsh, P.q), P.q);
+      }
       uint64_t* o = out + b * kp.out_stride;
-      o[(size_t)pos * N + j] = r0;
-      o[((size_t)kp.ext + pos) * N + j] = r1;
+      o[(size_t)pos * N + j0] = r0;
+      o[((size_t)kp.ext + pos) * N + j0] = r1;
+      o[(size_t)pos * N + j1] = s0;
+      o[((size_t)kp.ext + pos) * N + j1] = s1;
     }
   }
 }
@@ -638,7 +654,7 @@ void rotate_core(Context& c, int nb, const uint64_t* digits, uint64_t dig_stride
       ++ng;
     }
     ProfScope ps_(c, "ksk_batch", st);
-    k_ksk_batch<<<dim3(grid_n(c.N, 1).x, ext, ng), kT, 0, st>>>(digits, eo, kp, g);
+    k_ksk_batch<<<dim3(grid_n(c.N, 1, 2).x, ext, ng), kT, 0, st>>>(digits, eo, kp, g);
     FH_HIP(hipGetLastError());
   }
   if (do_moddown) moddown(c, eo, l, out, 2 * nb, 0, 0, st);

@@ -476,9 +476,9 @@ struct Shift {
 };
 
 // limbs per chunk: the intermediate of one chunk stays MALL-resident between the two launches
-inline uint32_t chunk_limbs(uint32_t N) {
-  const size_t budget = 96ull << 20;
-  return (uint32_t)std::max<size_t>(1, budget / (8ull * N));
+inline uint32_t chunk_limbs(const Context& c) {
+  const size_t budget = (size_t)std::max(1, c.tune.ntt_chunk_mb) << 20;
+  return (uint32_t)std::max<size_t>(1, budget / (8ull * c.N));
 }
 
 template <class Addr, class AddrOut, int L1S, int L2S>
@@ -486,7 +486,7 @@ void fwd_impl(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlim
   NttTables tb = ntt_tables(c);
   dim3 b1 = col_block<L1S>(c.N), b2 = row_block<L2S>(c.N);
   uint32_t C = b1.x / ((1 << L1S) / 16), R = b2.x / ((1 << L2S) / 16);
-  const uint32_t chunk = chunk_limbs(c.N);
+  const uint32_t chunk = chunk_limbs(c);
   for (uint32_t base = 0; base < nlimbs; base += chunk) {
     uint32_t n = std::min(chunk, nlimbs - base);
     Shift<Addr> sa{ad, base};
@@ -501,7 +501,7 @@ void inv_impl(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlim
   NttTables tb = ntt_tables(c);
   dim3 b1 = col_block<L1S>(c.N), b2 = row_block<L2S>(c.N);
   uint32_t C = b1.x / ((1 << L1S) / 16), R = b2.x / ((1 << L2S) / 16);
-  const uint32_t chunk = chunk_limbs(c.N);
+  const uint32_t chunk = chunk_limbs(c);
   for (uint32_t base = 0; base < nlimbs; base += chunk) {
     uint32_t n = std::min(chunk, nlimbs - base);
     Shift<Addr> sa{ad, base};

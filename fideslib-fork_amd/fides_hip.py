@@ -72,6 +72,8 @@ def _declare(L):
         "fh_fill_uniform": (I, [P, VP, i32p, I, I, S, U]),
         "fh_linear_transform": (I, [P, VP, I, I, I, I, I, I, VP, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), P, C.c_uint32, P,
                                     C.c_uint32, VP]),
+        "fh_tune_set": (I, [P, C.c_char_p, I]),
+        "fh_tune_get": (I, [P, C.c_char_p, C.POINTER(C.c_int)]),
         "fh_profile_enable": (I, [P, I]),
         "fh_profile_reset": (I, [P]),
         "fh_profile_read": (I, [P, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
@@ -267,6 +269,15 @@ class Context:
         _check(load().fh_linear_transform(self.h, ct.ptr, level, row_size, b_step, stride, offset, slots, diags.ptr, bk, bg,
                                           giant_key.h if giant_key is not None else None, giant_galois,
                                           offset_key.h if offset_key is not None else None, offset_galois, out.ptr))
+
+    # ---- launch-shape tunables ----
+    def tune_set(self, name, value):
+        _check(load().fh_tune_set(self.h, name.encode(), int(value)))
+
+    def tune_get(self, name):
+        v = C.c_int()
+        _check(load().fh_tune_get(self.h, name.encode(), C.byref(v)))
+        return v.value
 
     # ---- profiling (per-kernel-family HIP event timing) ----
     def profile(self, on=True):
