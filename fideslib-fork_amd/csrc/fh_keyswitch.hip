@@ -113,14 +113,12 @@ __global__ __launch_bounds__(kT) void k_modup_conv_r(const uint64_t* __restrict_
     s_pq[pos] = P.q;
     s_pn[pos] = P.mont_ninv;
   }
-  uint64_t inv[SZ], invsh[SZ], qs[SZ];
-#pragma unroll
-  for (int i = 0; i < SZ; ++i) {
-    if (i < (int)sz) {
-      inv[i] = mp.qhinv[tb + i];
-      invsh[i] = mp.qhinv_sh[tb + i];
-      qs[i] = mp.prime[start + i].q;
-    }
+  // per-source constants live in LDS: held as uniform scalars they exhaust the SGPR file
+  __shared__ uint64_t inv[SZ], invsh[SZ], qs[SZ];
+  for (uint32_t i = threadIdx.x; i < sz; i += blockDim.x) {
+    inv[i] = mp.qhinv[tb + i];
+    invsh[i] = mp.qhinv_sh[tb + i];
+    qs[i] = mp.prime[start + i].q;
   }
   __syncthreads();
   uint64_t* od = out + (size_t)d * mp.ext * mp.N;
@@ -350,14 +348,11 @@ __global__ __launch_bounds__(kT) void k_moddown_conv_r(const uint64_t* __restric
     s_q[i] = mp.prime[i].q;
     s_n[i] = mp.prime[i].mont_ninv;
   }
-  uint64_t inv[KK], invsh[KK], ps[KK];
-#pragma unroll
-  for (int k = 0; k < KK; ++k) {
-    if (k < (int)mp.K) {
-      inv[k] = mp.phinv[k];
-      invsh[k] = mp.phinv_sh[k];
-      ps[k] = mp.prime[mp.L1 + k].q;
-    }
+  __shared__ uint64_t inv[KK], invsh[KK], ps[KK];
+  for (uint32_t k = threadIdx.x; k < mp.K; k += blockDim.x) {
+    inv[k] = mp.phinv[k];
+    invsh[k] = mp.phinv_sh[k];
+    ps[k] = mp.prime[mp.L1 + k].q;
   }
   __syncthreads();
   const uint64_t* pc = pcoef + (size_t)blockIdx.y * mp.K * mp.N;
