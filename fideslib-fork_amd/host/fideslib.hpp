@@ -1,0 +1,230 @@
+// fideslib — host C++ facade of the MI355X RNS-CKKS engine.
+//
+// Mirrors the reference's public surface for the evaluation path
+// (api/CryptoContext.hpp:32-691, api/Ciphertext.hpp, api/CCParams.hpp:37-62): a CryptoContextImpl
+// built from CCParams, KeyGen / EvalMultKeyGen / EvalRotateKeyGen, CKKS packed encoding,
+// Encrypt / Decrypt, and the operations EvalAdd / EvalSub / EvalMult (ciphertext, plaintext and
+// scalar operands), Relinearize, Rescale, EvalRotate, EvalFastRotation (hoisted), EvalRotateMany
+// and LinearTransform. Every device operation goes through the C ABI of include/fideslib_hip.h;
+// this layer owns the CKKS metadata (level, scale, noise degree, slots) the way the reference's
+// CiphertextImpl does (src/CKKS/Ciphertext.cpp:163-237, 569-607) and the scalar CRT encodings of
+// ElemForEvalMult / ElemForEvalAddOrSub (src/CKKS/Context.cu:669-863).
+//
+// Scope: the FIXEDMANUAL scaling technique (the caller rescales), HYBRID key switching,
+// uniform-ternary or sparse-ternary secrets. Errors are thrown as std::runtime_error, the
+// reference facade's contract.
+#pragma once
+#include <complex>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/fideslib_hip.h"
+
+namespace fideslib {
+
+enum class ScalingTechnique { FIXEDMANUAL };
+enum class SecretKeyDist { UNIFORM_TERNARY, SPARSE_TERNARY };
+
+// api/CCParams.hpp:37-62 (the setters this path reads)
+class CCParams {
+ public:
+  void SetMultiplicativeDepth(uint32_t d) { depth_ = d; }
+  void SetScalingModSize(uint32_t b) { scale_bits_ = b; }
+  void SetFirstModSize(uint32_t b) { first_bits_ = b; }
+  void SetRingDim(uint32_t n) { ring_dim_ = n; }
+  void SetNumLargeDigits(uint32_t d) { dnum_ = d; }
+  void SetScalingTechnique(ScalingTechnique t) { tech_ = t; }
+  void SetBatchSize(uint32_t s) { batch_ = s; }
+  void SetSecretKeyDist(SecretKeyDist d) { dist_ = d; }
+  void SetHammingWeight(uint32_t h) { hw_ = h; }
+  // Not in OpenFHE's CCParams: the number of special primes (0 = OpenFHE's rule) and the seed
+  // of the deterministic sampler (keys, encryption randomness) used by this build.
+  void SetNumSpecialPrimes(uint32_t k) { num_special_ = k; }
+  void SetSeed(uint64_t s) { seed_ = s; }
+
+  uint32_t depth_ = 1, scale_bits_ = 50, first_bits_ = 60, ring_dim_ = 1 << 13, dnum_ = 1, batch_ = 0, hw_ = 32;
+  uint32_t num_special_ = 0;
+  uint64_t seed_ = 1;
+  ScalingTechnique tech_ = ScalingTechnique::FIXEDMANUAL;
+  SecretKeyDist dist_ = SecretKeyDist::UNIFORM_TERNARY;
+};
+
+class CryptoContextImpl;
+
+// Owner of the native context: destroyed when the CryptoContextImpl and every buffer allocated
+// from it are gone, so ciphertexts may outlive the CryptoContext object.
+struct NativeContext {
+  fh_context* ctx = nullptr;
+  explicit NativeContext(fh_context* c) : ctx(c) {}
+  NativeContext(const NativeContext&) = delete;
+  NativeContext& operator=(const NativeContext&) = delete;
+  ~NativeContext();
+};
+
+// A device-resident buffer from the engine's pool.
+struct DeviceBuffer {
+  std::shared_ptr<NativeContext> owner;
+  uint64_t* ptr = nullptr;
+  size_t words = 0;
+  DeviceBuffer() = default;
+  DeviceBuffer(const std::shared_ptr<NativeContext>& c, size_t w);
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  ~DeviceBuffer();
+};
+
+// Ciphertext: ncomp components over Q_l (l+1 limbs each), device resident.
+class CiphertextImpl {
+ public:
+  std::shared_ptr<DeviceBuffer> data;
+  uint32_t towers = 0;         // l + 1
+  uint32_t ncomp = 2;
+  uint32_t noiseScaleDeg = 1;
+  double scalingFactor = 1.0;
+  uint32_t slots = 0;
+  // OpenFHE level: towers consumed from the top of the chain
+  uint32_t GetLevel(uint32_t L1) const { return L1 - towers; }
+  uint32_t GetNoiseScaleDeg() const { return noiseScaleDeg; }
+  double GetScalingFactor() const { return scalingFactor; }
+  uint32_t GetSlots() const { return slots; }
+  size_t NumberCiphertextElements() const { return ncomp; }
+};
+
+// Plaintext: one polynomial over Q_l (or Q_l ∪ P when extended), plus the encoded values.
+class PlaintextImpl {
+ public:
+  std::shared_ptr<DeviceBuffer> data;
+  uint32_t towers = 0;
+  bool extended = false;
+  uint32_t noiseScaleDeg = 1;
+  double scalingFactor = 1.0;
+  uint32_t slots = 0;
+  std::vector<std::complex<double>> values;
+  const std::vector<std::complex<double>>& GetCKKSPackedValue() const { return values; }
+  void SetLength(size_t n) { values.resize(n); }
+};
+
+class PrivateKeyImpl {
+ public:
+  std::shared_ptr<DeviceBuffer> s;   // EVAL over Q ∪ P
+};
+class PublicKeyImpl {
+ public:
+  std::shared_ptr<DeviceBuffer> b, a;  // over Q: b = -a s + e
+};
+
+using Ciphertext = std::shared_ptr<CiphertextImpl>;
+using Plaintext = std::shared_ptr<PlaintextImpl>;
+using PrivateKey = std::shared_ptr<PrivateKeyImpl>;
+using PublicKey = std::shared_ptr<PublicKeyImpl>;
+struct KeyPair {
+  PublicKey publicKey;
+  PrivateKey secretKey;
+};
+
+class CryptoContextImpl;
+using CryptoContext = std::shared_ptr<CryptoContextImpl>;
+CryptoContext GenCryptoContext(const CCParams& params);
+
+class CryptoContextImpl {
+ public:
+  explicit CryptoContextImpl(const CCParams& p);
+  ~CryptoContextImpl();
+
+  uint32_t GetRingDimension() const { return N_; }
+  uint32_t GetCyclotomicOrder() const { return 2 * N_; }
+  uint32_t GetNumQ() const { return L1_; }
+  const std::vector<uint64_t>& GetModuli() const { return moduli_; }
+  fh_context* Native() const { return ctx_; }
+
+  // ---- keys (api/CryptoContext.hpp:84-88)
+  KeyPair KeyGen();
+  void EvalMultKeyGen(const PrivateKey& sk);
+  void EvalRotateKeyGen(const PrivateKey& sk, const std::vector<int32_t>& steps);
+  void EvalConjugateKeyGen(const PrivateKey& sk);
+  uint32_t FindAutomorphismIndex(int32_t index) const;
+
+  // ---- encoding (api/CryptoContext.hpp:140-172)
+  Plaintext MakeCKKSPackedPlaintext(const std::vector<std::complex<double>>& v, size_t noiseScaleDeg = 1, uint32_t level = 0,
+                                    uint32_t slots = 0);
+  Plaintext MakeCKKSPackedPlaintext(const std::vector<double>& v, size_t noiseScaleDeg = 1, uint32_t level = 0, uint32_t slots = 0);
+  Plaintext MakeCKKSPackedPlaintextExtended(const std::vector<std::complex<double>>& v, size_t noiseScaleDeg = 1,
+                                            uint32_t level = 0, uint32_t slots = 0);
+
+  // ---- encryption (api/CryptoContext.hpp:285-288)
+  Ciphertext Encrypt(const PrivateKey& sk, const Plaintext& pt);
+  Ciphertext Encrypt(const PublicKey& pk, const Plaintext& pt);
+  Plaintext Decrypt(const PrivateKey& sk, const Ciphertext& ct);
+
+  // ---- arithmetic (api/CryptoContext.hpp:306-365)
+  Ciphertext EvalNegate(const Ciphertext& ct);
+  Ciphertext EvalAdd(const Ciphertext& a, const Ciphertext& b);
+  Ciphertext EvalAdd(const Ciphertext& a, const Plaintext& pt);
+  Ciphertext EvalAdd(const Ciphertext& a, double scalar);
+  Ciphertext EvalSub(const Ciphertext& a, const Ciphertext& b);
+  Ciphertext EvalSub(const Ciphertext& a, const Plaintext& pt);
+  Ciphertext EvalSub(const Ciphertext& a, double scalar);
+  Ciphertext EvalMult(const Ciphertext& a, const Ciphertext& b);
+  Ciphertext EvalMult(const Ciphertext& a, const Plaintext& pt);
+  Ciphertext EvalMult(const Ciphertext& a, double scalar);
+  Ciphertext EvalMultNoRelin(const Ciphertext& a, const Ciphertext& b);
+  Ciphertext Relinearize(const Ciphertext& ct);
+  Ciphertext EvalSquare(const Ciphertext& ct);
+  Ciphertext EvalAddMany(const std::vector<Ciphertext>& cts);
+  Ciphertext Rescale(const Ciphertext& ct);
+  Ciphertext LevelReduce(const Ciphertext& ct, uint32_t levels);
+
+  // ---- rotations (api/CryptoContext.hpp:367-404)
+  Ciphertext EvalRotate(const Ciphertext& ct, int32_t index);
+  Ciphertext EvalConjugate(const Ciphertext& ct);
+  std::shared_ptr<void> EvalFastRotationPrecompute(const Ciphertext& ct);
+  Ciphertext EvalFastRotation(const Ciphertext& ct, int32_t index, uint32_t m, const std::shared_ptr<void>& precomp);
+  std::vector<Ciphertext> EvalFastRotation(const Ciphertext& ct, const std::vector<int32_t>& indices, uint32_t m,
+                                           const std::shared_ptr<void>& precomp);
+  std::vector<Ciphertext> EvalRotateMany(const std::vector<Ciphertext>& cts, int32_t index);
+
+  // ---- linear transform (api/CryptoContext.hpp:460-517): diagonals built with
+  // MakeCKKSPackedPlaintextExtended at the ciphertext's level, D_k pre-rotated by
+  // -(bStep*(k/bStep)*stride + offset); result at the input level, noise degree + 1.
+  Ciphertext LinearTransform(const Ciphertext& ct, int rowSize, int bStep, const std::vector<Plaintext>& diagonals, int stride = 1,
+                             int offset = 0);
+
+  // ---- scalar CRT encodings (src/CKKS/Context.cu:669-863), exposed for tests
+  std::vector<uint64_t> ElemForEvalMult(uint32_t towers, double operand) const;
+  std::vector<uint64_t> ElemForEvalAddOrSub(uint32_t towers, double operand, uint32_t noiseDeg) const;
+
+  void Synchronize() const;
+  double ScalingFactor() const { return scale_; }
+
+ private:
+  void check(int rc) const;
+  Ciphertext newCt(uint32_t towers, uint32_t ncomp) const;
+  Ciphertext adjusted(const Ciphertext& ct, uint32_t towers) const;
+  void encodeInto(const std::vector<std::complex<double>>& v, uint32_t slots, double scale, const std::vector<int32_t>& pids,
+                  uint64_t* dev) const;
+  std::shared_ptr<DeviceBuffer> smallToEval(const std::vector<int64_t>& coeffs, const std::vector<int32_t>& pids) const;
+  std::shared_ptr<DeviceBuffer> uniform(const std::vector<int32_t>& pids);
+  std::vector<int64_t> gaussian();
+  fh_key* switchKey(const DeviceBuffer& s_old, const DeviceBuffer& s_new);
+  const fh_key* rotKey(uint32_t galois) const;
+  uint32_t galoisFor(int32_t index) const;
+  void automorphismKey(const PrivateKey& sk, uint32_t galois);
+
+  std::shared_ptr<NativeContext> native_;
+  fh_context* ctx_ = nullptr;
+  uint32_t N_ = 0, L1_ = 0, K_ = 0, dnum_ = 0, alpha_ = 0;
+  double scale_ = 0;
+  std::vector<uint64_t> moduli_;
+  std::vector<uint64_t> pmodq_;
+  CCParams params_;
+  struct Rng;
+  std::unique_ptr<Rng> rng_;
+  fh_key* relin_ = nullptr;
+  std::map<uint32_t, fh_key*> rot_;
+};
+
+}  // namespace fideslib

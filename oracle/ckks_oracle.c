@@ -1175,6 +1175,33 @@ void orc_decrypt(const orc_ctx* c, const u64* ct, int l, int ncomp, const u64* s
   }
 }
 
+/* Public-key encryption as in OpenFHE 1.5.1 PKERNS::EncryptZeroCore (an un-vendored dependency,
+ * reached from api/CryptoContext.cpp:900-921): v ternary, e0, e1 Gaussian; c0 = b v + e0 + m,
+ * c1 = a v + e1 over Q_l. The sampler streams are this build's (mt19937_64), not OpenFHE's. */
+void orc_encrypt_pk(const orc_ctx* c, const u64* m, int l, const u64* pk_b, const u64* pk_a, u64 seed, u64* ct) {
+  u64 N = c->N;
+  size_t P = (size_t)(l + 1) * N;
+  mt64* s = (mt64*)malloc(sizeof(mt64));
+  mt_seed(s, seed);
+  int64_t* w = (int64_t*)malloc(3 * N * sizeof(int64_t));
+  for (u64 j = 0; j < N; ++j) w[j] = (int64_t)(mt_next(s) % 3) - 1;
+  for (u64 j = 0; j < 2 * N; ++j) w[N + j] = mt_gauss(s);
+  u64* ev = (u64*)malloc(3 * P * 8);
+  int pids[ORC_MAXP];
+  for (int i = 0; i <= l; ++i) pids[i] = i;
+  for (int t = 0; t < 3; ++t) small_to_eval(c, w + (size_t)t * N, pids, l + 1, ev + (size_t)t * P);
+  const u64 *v = ev, *e0 = ev + P, *e1 = ev + 2 * P;
+  for (int i = 0; i <= l; ++i) {
+    u64 q = c->q[i];
+    for (u64 j = 0; j < N; ++j) {
+      size_t x = (size_t)i * N + j;
+      ct[x] = modadd(modadd(orc_mulmod(pk_b[x], v[x], q), e0[x], q), m[x], q);
+      ct[P + x] = modadd(orc_mulmod(pk_a[x], v[x], q), e1[x], q);
+    }
+  }
+  free(ev); free(w); free(s);
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

@@ -119,12 +119,15 @@ void orc_gen_relin_key(const orc_ctx* c, const uint64_t* s, uint64_t seed, uint6
 void orc_gen_rot_key(const orc_ctx* c, const uint64_t* s, uint32_t k, uint64_t seed, uint64_t* key);
 /* CKKS encode of n complex slots (re,im interleaved) at scale, over Q_l (EVAL) */
 void orc_encode(const orc_ctx* c, const double* slots, int n, double scale, int l, uint64_t* out);
-/* same, but over Q_l ∪ P and multiplied by P (the extended-basis plaintext form) */
+/* the same polynomial over Q_l ∪ P (the extended-basis plaintext form of the linear transform) */
 void orc_encode_ext(const orc_ctx* c, const double* slots, int n, double scale, int l, uint64_t* out);
 /* decode a plaintext poly over Q_l (EVAL) into n complex slots */
 void orc_decode(const orc_ctx* c, const uint64_t* m, int l, int n, double scale, double* slots);
 void orc_encrypt(const orc_ctx* c, const uint64_t* m, int l, const uint64_t* s, uint64_t seed, uint64_t* ct);
 void orc_decrypt(const orc_ctx* c, const uint64_t* ct, int l, int ncomp, const uint64_t* s, uint64_t* m);
+/* public-key encryption with pk = (b, a) over Q (L1 limbs each, b = -a s + e):
+ * v ternary, c0 = b v + e0 + m, c1 = a v + e1 (draw order v, e0, e1) */
+void orc_encrypt_pk(const orc_ctx* c, const uint64_t* m, int l, const uint64_t* pk_b, const uint64_t* pk_a, uint64_t seed, uint64_t* ct);
 
 /* ---- random data (uniform limbs) ---- */
 void orc_random_poly(const orc_ctx* c, uint64_t seed, const int* pids, int nlimbs, uint64_t* out);

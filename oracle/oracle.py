@@ -4,6 +4,7 @@ Loaded by tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py, 
 checker. Arrays are numpy uint64, limb-major (one N-word row per limb), canonical residues.
 """
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -88,6 +89,7 @@ def lib():
             "orc_decode": (V, [P, u64p, I, I, D, f64p]),
             "orc_encrypt": (V, [P, u64p, I, u64p, U, u64p]),
             "orc_decrypt": (V, [P, u64p, I, I, u64p, u64p]),
+            "orc_encrypt_pk": (V, [P, u64p, I, u64p, u64p, U, u64p]),
             "orc_random_poly": (V, [P, U, i32p, I, u64p]),
             "orc_mt_first": (U, [U, I]),
             "orc_num_threads": (I, []),
@@ -99,6 +101,59 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+# ---- scalar CRT encodings (src/CKKS/Context.cu:669-760 and 777-863) ----
+_MAX_LOG_STEP = 60  # OpenFHE LargeScalingFactorConstants::MAX_LOG_STEP
+
+
+def _log_approx(operand, sc_factor, max_bits):
+    res = abs(operand * sc_factor)
+    if res > 0:
+        log_sf = int(math.ceil(math.log2(res)))
+        return log_sf - min(log_sf, max_bits)
+    return 0
+
+
+def _approx_residues(log_approx, moduli):
+    """2^log_approx mod q_i, built in steps of at most 2^60 (Context.cu:733-748)."""
+    out = [1] * len(moduli)
+    while log_approx > 0:
+        step = min(log_approx, _MAX_LOG_STEP)
+        out = [(o * (1 << step)) % q for o, q in zip(out, moduli)]
+        log_approx -= step
+    return out
+
+
+def elem_for_eval_mult(moduli, operand, sc_factor):
+    """ContextData::ElemForEvalMult (src/CKKS/Context.cu:669-760): the residues of
+    round(operand * scFactor) for towers with the given moduli; past 2^125 the product is rounded
+    at 2^logApprox and scaled back up by CRT multiplication. Python floats are IEEE doubles
+    without FMA, the rounding sequence the reference documents."""
+    log_approx = _log_approx(operand, sc_factor, 125)
+    approx_factor = 2.0 ** log_approx
+    large = int(operand / approx_factor * sc_factor + 0.5)  # C++ truncation toward zero
+    f = [large % q for q in moduli]  # Python's % is already the non-negative residue
+    if log_approx > 0:
+        f = [(a * b) % q for a, b, q in zip(f, _approx_residues(log_approx, moduli), moduli)]
+    return np.array(f, dtype=np.uint64)
+
+
+def elem_for_eval_add(moduli, operand, sc_factor, noise_deg):
+    """ContextData::ElemForEvalAddOrSub (src/CKKS/Context.cu:777-863) for operand >= 0: the
+    residues of round(operand * scFactor) times round(scFactor)^(noise_deg - 1)."""
+    if operand < 0:
+        raise ValueError("the reference encodes |operand| and subtracts for negative constants")
+    log_approx = _log_approx(operand, sc_factor, 61)
+    approx_factor = 2.0 ** log_approx
+    sc = int(operand * sc_factor / approx_factor + 0.5)
+    c = [sc % q for q in moduli]
+    if log_approx > 0:
+        c = [(a * b) % q for a, b, q in zip(c, _approx_residues(log_approx, moduli), moduli)]
+    isf = int(sc_factor + 0.5)
+    for _ in range(1, noise_deg):
+        c = [(a * isf) % q for a, q in zip(c, moduli)]
+    return np.array(c, dtype=np.uint64)
 
 
 def gen_primes(logN, L1, first_bits, scale_bits, K, aux_bits=60):
@@ -315,6 +370,11 @@ class Ctx:
     def encrypt(self, m, l, s, seed):
         ct = self.zeros(l + 1, 2)
         lib().orc_encrypt(self.h, np.ascontiguousarray(m), l, s, seed, ct)
+        return ct
+
+    def encrypt_pk(self, m, l, pk_b, pk_a, seed):
+        ct = self.zeros(l + 1, 2)
+        lib().orc_encrypt_pk(self.h, np.ascontiguousarray(m), l, np.ascontiguousarray(pk_b), np.ascontiguousarray(pk_a), seed, ct)
         return ct
 
     def decrypt(self, ct, l, s, ncomp=2):

@@ -369,3 +369,77 @@ def test_linear_transform_decrypts(rowSize, bStep, offset):
     out = c.linear_transform(ct, l, diags, bStep, 1, bk, bkk, gk, c.rot_index(bStep), ok, c.rot_index(offset) if offset else 1, offset)
     dec = c.decode(c.decrypt(out, l, s), l, n, Dl * 2.0 ** 30)
     assert np.max(np.abs(dec - want)) < 1e-6
+
+
+def test_public_key_encryption_decrypts(sc):
+    """pk = (b, a) = an encryption of zero; c0 + c1 s = v e + e0 + e1 s + m, well inside the noise."""
+    c = sc
+    n, l, Dl = c.N // 2, c.L1 - 1, 2.0 ** 40
+    s = c.gen_secret(21)
+    pk = c.encrypt(c.zeros(c.L1), c.L1 - 1, s, 22)
+    z = _slots(n, 23)
+    ct = c.encrypt_pk(c.encode(z, Dl, l), l, pk[:c.L1], pk[c.L1:], 24)
+    assert np.max(np.abs(c.decode(c.decrypt(ct, l, s), l, n, Dl) - z)) < 1e-7
+    # the same message at a lower level uses the key's first towers
+    ct2 = c.encrypt_pk(c.encode(z, Dl, 1), 1, pk[:c.L1], pk[c.L1:], 25)
+    assert np.max(np.abs(c.decode(c.decrypt(ct2, 1, s), 1, n, Dl) - z)) < 1e-7
+    # a dropped noise or message term is visible: the ciphertext is not an encryption of zero
+    assert np.max(np.abs(c.decode(c.decrypt(ct, l, s), l, n, Dl))) > 0.5
+
+
+def _crt_centered(res, moduli):
+    """the centred integer with the given residues (Python big integers)"""
+    Q = 1
+    for q in moduli:
+        Q *= q
+    x = 0
+    for r, q in zip(res, moduli):
+        Qi = Q // q
+        x += int(r) * Qi * pow(Qi, -1, q)
+    x %= Q
+    return x - Q if x > Q // 2 else x
+
+
+def _round_like_double(fr):
+    """trunc(fl(fr + 1/2)) for an exact rational fr: the reference's `(int)(x + 0.5)` on doubles"""
+    from fractions import Fraction
+    return int(float(fr + Fraction(1, 2)))
+
+
+@pytest.mark.parametrize("operand", [2.5, -0.75, 1e-9, 0.1, -1234.5678, 3.0e30, -7.25e25, 0.0, 2.0 ** 70])
+def test_elem_for_eval_mult_closed_form(sc, operand):
+    """ElemForEvalMult: the residues are those of one integer V (CRT over all towers) with
+    V = round(operand * scale) when that fits in 125 bits, else V = 2^k * round(operand*scale/2^k)
+    with k = ceil(log2|operand*scale|) - 125: divisible by 2^k and within 2^k of operand*scale."""
+    from fractions import Fraction
+    q = [c_ for c_ in (sc.q(i) for i in range(sc.L1))]
+    scale = 2.0 ** 50
+    V = _crt_centered(O.elem_for_eval_mult(q, operand, scale), q)
+    x = Fraction(operand) * Fraction(scale)
+    if abs(x) < Fraction(2) ** 125:
+        assert V == _round_like_double(x)
+    else:
+        import math
+        k = math.ceil(math.log2(abs(float(x)))) - 125
+        assert k > 0 and V % (1 << k) == 0 and abs(V - x) <= (1 << k)
+
+
+@pytest.mark.parametrize("operand,deg", [(v, d) for v in (0.25, 1.5, 0.0, 1e-7, 1e12) for d in (1, 2, 3)] + [(3.3e20, 1), (3.3e20, 2)])
+def test_elem_for_eval_add_closed_form(sc, operand, deg):
+    """ElemForEvalAddOrSub: V = round(operand*scale) * round(scale)^(deg-1) below 2^61; above,
+    round(operand*scale) is taken at 2^k granularity, k = ceil(log2(operand*scale)) - 61."""
+    from fractions import Fraction
+    import math
+    q = [sc.q(i) for i in range(sc.L1)]
+    scale = 2.0 ** 50
+    x = Fraction(operand) * Fraction(scale)
+    sf = (1 << 50) ** (deg - 1)
+    assert x * sf * 4 < math.prod(q)  # the residues determine the integer
+    V = _crt_centered(O.elem_for_eval_add(q, operand, scale, deg), q)
+    assert V % sf == 0
+    W = V // sf
+    if x < Fraction(2) ** 61:
+        assert W == _round_like_double(x)
+    else:
+        k = math.ceil(math.log2(float(x))) - 61
+        assert k > 0 and W % (1 << k) == 0 and abs(W - x) <= (1 << k)
