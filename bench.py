@@ -9,8 +9,11 @@ and keys are synthetic uniform residues generated on the device; the same batch 
 rotation at a time (fh_eval_rotate) and the two results are compared bit for bit.
 
 Also reported: NTT GB/s (forward + inverse over all 64 x 2 x 25 limbs), EvalMult+Rescale ops/s,
-the roofline of the dominant kernel (HIP-event timing of its launches inside the timed region) and
-the CPU oracle timed on a bounded sample of the same rotation workload.
+a mixed-level op stream (256 ciphertexts at levels 1..24; rotations, multiplications with
+relinearization and Rescale, additions) run one operation per call and as one fused batch (eager
+and HIP-graph replayed), the roofline of the dominant kernel (HIP-event timing of its launches
+inside the timed region) and the CPU oracle timed on a bounded sample of the same rotation
+workload.
 """
 import argparse
 import json
@@ -38,6 +41,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--stream-ops", type=int, default=256, help="ciphertexts of the mixed-level op stream (0 = skip)")
+    ap.add_argument("--lt-count", type=int, default=32, help="linear transforms of the BSGS workload (0 = skip)")
     return ap.parse_args()
 
 
@@ -54,8 +59,6 @@ def algorithmic_bytes(family, N, lq, ext, beta, B, groups):
         return limb * B * (2 * lq + beta * ext - lq)
     if family == "moddown_conv":
         return limb * 2 * B * (K + lq)
-    if family == "moddown_final":
-        return limb * 2 * B * 3 * lq
     return None
 
 
@@ -120,7 +123,7 @@ def main():
     ctx.sync()
     t_batched = time.perf_counter() - t0
     fams = ["ntt_inv_rows", "ntt_inv_cols", "ntt_fwd_cols", "ntt_fwd_rows", "modup_conv", "ksk_batch",
-            "gather_special", "moddown_conv", "moddown_final"]
+            "moddown_conv", "ntt_fwd_rows_moddown"]
     prof = {f: ctx.profile_read(f) for f in fams}
     ctx.profile(False)
     ms_per_step = 1000.0 * t_batched / a.steps
@@ -184,9 +187,11 @@ def main():
     groups_avg = groups_total / a.steps
     if dom.startswith("ntt_"):
         # per launch: every limb of the launch read and written once
-        per_step_limbs = {"ntt_inv_rows": B * lq + 2 * B * K, "ntt_inv_cols": B * lq + 2 * B * K,
-                          "ntt_fwd_cols": B * (beta * ext - lq) + 2 * B * lq, "ntt_fwd_rows": B * (beta * ext - lq) + 2 * B * lq}
-        alg = 16 * N * per_step_limbs[dom] * a.steps / max(launches, 1)
+        # (the ModDown rows launch also reads x_i and writes the result: 24N bytes per limb)
+        per_step_bytes = {"ntt_inv_rows": 16 * N * (B * lq + 2 * B * K), "ntt_inv_cols": 16 * N * (B * lq + 2 * B * K),
+                          "ntt_fwd_cols": 16 * N * (B * (beta * ext - lq) + 2 * B * lq),
+                          "ntt_fwd_rows": 16 * N * B * (beta * ext - lq), "ntt_fwd_rows_moddown": 24 * N * 2 * B * lq}
+        alg = per_step_bytes[dom] * a.steps / max(launches, 1)
     else:
         alg = algorithmic_bytes(dom, N, lq, ext, beta, B, groups_avg)
     achieved = alg / (avg_ms * 1e-3) / 1e9 if alg else None
@@ -195,6 +200,10 @@ def main():
             "avg_launch_ms": round(avg_ms, 4), "launches": launches,
             "kernel_ms_per_step": {f: round(prof[f][1] / a.steps, 3) for f in fams}}
     ntt_kernel_ms = {f: round(nprof[f][1] / max(nprof[f][0], 1), 4) for f in nprof}
+
+    # ---- mixed-level op stream: per-op vs one batch (eager and graph-replayed) ----
+    stream = uneven_stream(fh, ctx, dev_keys, gal, a) if a.stream_ops > 0 else None
+    lt = lt_workload(fh, ctx, a) if a.lt_count > 0 else None
 
     # ---- CPU baseline: the oracle on a bounded sample of the same rotation ----
     cpu = None
@@ -226,9 +235,135 @@ def main():
         "ntt_ms_fwd_inv": round(1000 * t_ntt, 3),
         "ntt_avg_launch_ms": ntt_kernel_ms,
         "eval_mult_rescale_per_s": round(mult_rescale_s, 1),
+        "uneven_stream": stream,
+        "linear_transform": lt,
     }
     print(json.dumps(line))
     ctx.close()
+
+
+def uneven_stream(fh, ctx, rot_keys, rot_gal, a):
+    """A seeded stream of ciphertexts at levels uniform in [1, 24]: 40% EvalRotate (16 keyed steps),
+    40% EvalMult + relinearization + Rescale, 20% EvalAdd. Timed one operation per call and as one
+    mixed-level batch (eager, and replayed from a captured HIP graph); outputs compared bit for bit."""
+    N, n = ctx.N, a.stream_ops
+    rng = np.random.default_rng(a.seed + 7)
+    kinds = rng.choice(3, size=n, p=[0.4, 0.4, 0.2])          # 0 rotate, 1 mult+rescale, 2 add
+    levels = rng.integers(1, L1, size=n)
+    keyi = rng.integers(0, len(rot_keys), size=n)
+    relin = rot_keys[0]  # a key of the relinearization shape (synthetic residues)
+    ins, outs_p, outs_b = [], [], []
+    for i in range(n):
+        lq = int(levels[i]) + 1
+        x = ctx.array((2 * lq, N))
+        y = ctx.array((2 * lq, N))
+        ctx.fill_uniform(x, list(range(lq)) * 2, num_limbs=2 * lq, seed=a.seed + 10000 + 2 * i)
+        ctx.fill_uniform(y, list(range(lq)) * 2, num_limbs=2 * lq, seed=a.seed + 10001 + 2 * i)
+        olimbs = 2 * (lq - 1) if kinds[i] == 1 else 2 * lq
+        ins.append((x, y))
+        outs_p.append(ctx.array((olimbs, N)))
+        outs_b.append(ctx.array((olimbs, N)))
+    tmp = ctx.array((2 * L1, N))
+    lib = fh.load()
+
+    def per_op():
+        for i in range(n):
+            l = int(levels[i])
+            x, y = ins[i]
+            o = outs_p[i]
+            if kinds[i] == 0:
+                ctx.eval_rotate(x, l, rot_keys[keyi[i]], rot_gal[keyi[i]], o)
+            elif kinds[i] == 1:
+                ctx.eval_mult(x, y, l, relin, tmp)
+                ctx.rescale(tmp, l, 2, o)
+            else:
+                ids = list(range(l + 1)) * 2
+                ctx.eltwise(0, x, y, o, prime_ids=ids)
+
+    b = fh.Batch(ctx)
+    for i in range(n):
+        l = int(levels[i])
+        x, y = ins[i]
+        if kinds[i] == 0:
+            b.rotate(x.ptr, l, rot_keys[keyi[i]], rot_gal[keyi[i]], outs_b[i].ptr)
+        elif kinds[i] == 1:
+            b.mult(x.ptr, y.ptr, l, relin, True, outs_b[i].ptr)
+        else:
+            b.add(x.ptr, y.ptr, l, 2, outs_b[i].ptr)
+    b.finalize()
+    _, scratch = b.info()
+
+    def timed(fn, reps):
+        fn()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ctx.sync()
+        return (time.perf_counter() - t0) / reps
+
+    t_per = timed(per_op, 2)
+    t_eager = timed(lambda: b.run(graph=False), 3)
+    t_graph = timed(lambda: b.run(graph=True), 3)
+    same = all(np.array_equal(outs_p[i].download(), outs_b[i].download()) for i in range(n))
+    for x, y in ins:
+        x.free()
+        y.free()
+    for o in outs_p + outs_b:
+        o.free()
+    tmp.free()
+    b.free()
+    counts = {k: int((kinds == j).sum()) for j, k in enumerate(("rotate", "mult_relin_rescale", "add"))}
+    return {"ops": n, "mix": counts, "levels": "uniform in [1, 24]",
+            "per_op_ops_per_s": round(n / t_per, 1), "batched_ops_per_s": round(n / t_eager, 1),
+            "batched_graph_ops_per_s": round(n / t_graph, 1), "batched_over_per_op": round(t_per / t_eager, 3),
+            "graph_over_per_op": round(t_per / t_graph, 3), "bit_exact_batched_vs_per_op": bool(same),
+            "batch_scratch_gib": round(scratch / 2 ** 30, 2)}
+
+
+def lt_workload(fh, ctx, a):
+    """Hoisted BSGS plaintext-matrix x ciphertext product: level 20 (21 limbs), 64 nonzero
+    diagonals resident over Q_20 ∪ P (960 MiB), bStep = gStep = 8, stride 1; a batch of lt_count
+    ciphertexts against the one resident diagonal set. Synthetic residues for keys, diagonals and
+    ciphertexts (the LT arithmetic is data-independent)."""
+    N, lvl, row, bstep = ctx.N, 20, 64, 8
+    lq, ext = lvl + 1, lvl + 1 + K
+    full = list(range(L1 + K))
+    kbuf = ctx.array((DNUM * 2 * (L1 + K), N))
+    keys = {}
+    for i, r in enumerate(list(range(1, bstep)) + [bstep]):
+        ctx.fill_uniform(kbuf, full, num_polys=DNUM * 2, seed=a.seed + 500 + i)
+        keys[r] = ctx.key_from_device(kbuf)
+    kbuf.free()
+    ext_ids = list(range(lq)) + [L1 + k for k in range(K)]
+    diags = ctx.array((row * ext, N))
+    ctx.fill_uniform(diags, ext_ids, num_polys=row, seed=a.seed + 600)
+    cts = ctx.array((a.lt_count * 2 * lq, N))
+    ctx.fill_uniform(cts, list(range(lq)), num_polys=2 * a.lt_count, seed=a.seed + 700)
+    out = ctx.array((2 * lq, N))
+    bk = [None] + [keys[i] for i in range(1, bstep)]
+    bg = [1] + [ctx.galois_index(i) for i in range(1, bstep)]
+    gk, gg = keys[bstep], ctx.galois_index(bstep)
+
+    def one(i):
+        ctx.linear_transform(_view(fh, ctx, cts, i * 2 * lq * N, (2 * lq, N)), lvl, row, bstep, 1, diags, bk, bg, out,
+                             giant_key=gk, giant_galois=gg)
+
+    one(0)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for i in range(a.lt_count):
+        one(i)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    for x in [diags, cts, out] + list(keys.values()):
+        x.free()
+    # algorithmic bytes per transform: 15 key switches' keys, 64 diagonals, ciphertext in/out
+    beta = ctx.num_digits(lvl)
+    alg = 8 * N * (15 * 2 * beta * ext + row * ext + 4 * lq)
+    return {"transforms": a.lt_count, "level": lvl, "diagonals": row, "b_step": bstep, "g_step": row // bstep,
+            "transforms_per_s": round(a.lt_count / dt, 2), "ms_per_transform": round(1000 * dt / a.lt_count, 3),
+            "algorithmic_gb_per_transform": round(alg / 1e9, 3)}
 
 
 def _view(fh, ctx, arr, word_offset, shape):

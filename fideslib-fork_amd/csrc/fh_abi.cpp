@@ -7,7 +7,9 @@
 #include <vector>
 
 #include "../../include/fideslib_hip.h"
+#include "fh_batch.hpp"
 #include "fh_keyswitch.hpp"
+#include "fh_serve.hpp"
 
 struct fh_context {
   fh::Context* c;
@@ -447,6 +449,122 @@ int fh_fill_uniform(fh_context* ctx, uint64_t* polys, const int32_t* prime_ids, 
     auto p = pids_or_identity(c, prime_ids, num_limbs);
     uint64_t stride = poly_stride ? poly_stride : (uint64_t)num_limbs * c.N;
     fh::fill_uniform(c, fh::make_set(polys, num_limbs, p.data(), num_polys, stride), seed, c.stream);
+  });
+}
+
+struct fh_batch {
+  fh_context* ctx;
+  fh::Batch b;
+};
+
+static fh_batch& B(fh_batch* h) {
+  if (!h) throw fh::Error(FH_E_INVALID, "null batch");
+  return *h;
+}
+
+int fh_batch_create(fh_context* ctx, fh_batch** out) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(out, "out");
+    *out = new fh_batch{ctx, fh::Batch(c)};
+  });
+}
+
+int fh_batch_destroy(fh_batch* b) {
+  return guard([&] {
+    if (!b) return;
+    // replays may still be in flight on the context stream
+    (void)hipStreamSynchronize(C(b->ctx).stream);
+    delete b;
+  });
+}
+
+int fh_batch_rotate(fh_batch* b, const uint64_t* ct, int level, const fh_key* key, uint32_t galois, uint64_t* out) {
+  return guard([&] {
+    auto& h = B(b);
+    auto& c = C(h.ctx);
+    h.b.add_rotate(ct, level, key_of(c, key), galois, out);
+  });
+}
+
+int fh_batch_mult(fh_batch* b, const uint64_t* a, const uint64_t* bb, int level, const fh_key* relin_key, int rescale, uint64_t* out) {
+  return guard([&] {
+    auto& h = B(b);
+    auto& c = C(h.ctx);
+    h.b.add_mult(a, bb, level, key_of(c, relin_key), rescale != 0, out);
+  });
+}
+
+int fh_batch_add(fh_batch* b, const uint64_t* a, const uint64_t* bb, int level, int ncomp, uint64_t* out) {
+  return guard([&] { B(b).b.add_add(a, bb, level, ncomp, out); });
+}
+
+int fh_batch_mult_plain(fh_batch* b, const uint64_t* a, const uint64_t* pt, int level, int rescale, uint64_t* out) {
+  return guard([&] { B(b).b.add_mult_plain(a, pt, level, rescale != 0, out); });
+}
+
+int fh_batch_add_plain(fh_batch* b, const uint64_t* a, const uint64_t* pt, int level, uint64_t* out) {
+  return guard([&] { B(b).b.add_add_plain(a, pt, level, out); });
+}
+
+int fh_batch_level_reduce(fh_batch* b, const uint64_t* a, int level_in, int level_out, int ncomp, uint64_t* out) {
+  return guard([&] { B(b).b.add_level_reduce(a, level_in, level_out, ncomp, out); });
+}
+
+int fh_batch_reset(fh_batch* b) {
+  return guard([&] { B(b).b.reset(); });
+}
+
+int fh_batch_finalize(fh_batch* b) {
+  return guard([&] { B(b).b.finalize(); });
+}
+
+int fh_batch_run(fh_batch* b, int use_graph) {
+  return guard([&] {
+    auto& h = B(b);
+    h.b.run(C(h.ctx).stream, use_graph != 0);
+  });
+}
+
+int fh_batch_info(const fh_batch* b, size_t* items, size_t* scratch_bytes) {
+  return guard([&] {
+    if (!b) throw fh::Error(FH_E_INVALID, "null batch");
+    if (items) *items = b->b.items();
+    if (scratch_bytes) *scratch_bytes = b->b.scratch_bytes();
+  });
+}
+
+int fh_serve_run(fh_context* ctx, const fh_serve_config* config, const fh_key* const* rot_keys, const uint32_t* rot_galois,
+                 const fh_key* relin_key, fh_serve_result* result) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(config, "config"); need(rot_keys, "rot_keys"); need(rot_galois, "rot_galois"); need(result, "result");
+    fh::ServeConfig cfg;
+    cfg.rate_per_s = config->rate_per_s;
+    cfg.requests = config->requests;
+    cfg.start_level = config->start_level;
+    cfg.hbm_budget_gb = config->hbm_budget_gb;
+    cfg.mode = config->mode;
+    cfg.inference_share = config->inference_share;
+    cfg.seed = config->seed;
+    cfg.checksum = config->checksum != 0;
+    if (cfg.mode != 0 && cfg.mode != 1) throw fh::Error(FH_E_INVALID, "mode must be 0 (batched) or 1 (per operation)");
+    const uint64_t* rk[8];
+    for (int k = 0; k < 8; ++k) rk[k] = key_of(c, rot_keys[k]);
+    fh::ServeResult r = fh::serve_run(c, cfg, rk, rot_galois, key_of(c, relin_key));
+    result->p50_ms = r.p50_ms;
+    result->p99_ms = r.p99_ms;
+    result->mean_ms = r.mean_ms;
+    result->max_ms = r.max_ms;
+    result->requests_per_s = r.requests_per_s;
+    result->wall_s = r.wall_s;
+    result->mean_batch_ops = r.mean_batch_ops;
+    result->ticks = r.ticks;
+    result->completed = r.completed;
+    result->inference = r.inference;
+    result->polynomial = r.polynomial;
+    result->max_active = r.max_active;
+    result->checksum = r.checksum;
   });
 }
 

@@ -79,6 +79,56 @@ struct U128 {
   }
 };
 
+// Exact sum of up to 16 products y c with y, c < 2^60 from 30-bit halves (y = y1 2^30 + y0):
+// every partial product y_a c_b is below 2^60, so four plain 64-bit sums of at most 16 terms
+// cannot overflow, and each term costs four 32x32->64 multiply-adds with no carry handling.
+// The 128-bit value s00 + (s01 + s10) 2^30 + s11 2^60 is assembled once.
+struct Acc30 {
+  uint64_t s00, s01, s10, s11;
+  __device__ __forceinline__ void zero() { s00 = s01 = s10 = s11 = 0; }
+  __device__ __forceinline__ void mac(uint32_t y0, uint32_t y1, uint32_t c0, uint32_t c1) {
+    s00 += (uint64_t)y0 * c0;
+    s01 += (uint64_t)y0 * c1;
+    s10 += (uint64_t)y1 * c0;
+    s11 += (uint64_t)y1 * c1;
+  }
+  __device__ __forceinline__ void get(uint64_t& hi, uint64_t& lo) const {
+    const uint64_t m = s01 + s10;
+    const uint64_t mc = m < s01 ? 1 : 0;  // carry of the middle sum, weight 2^94
+    uint64_t h = (m >> 34) + (mc << 30) + (s11 >> 4);
+    const uint64_t l1 = m << 30, l2 = s11 << 60;
+    const uint64_t l = s00 + l1;
+    h += l < l1 ? 1 : 0;
+    const uint64_t l3 = l + l2;
+    h += l3 < l2 ? 1 : 0;
+    lo = l3;
+    hi = h;
+  }
+};
+
+// Multiply-accumulate policies of the basis conversions and key inner products. Operands are
+// prepared once (prep) and reused across every product they enter.
+struct MacWide {   // any residues below 2^64: 128-bit product per term
+  using Op = uint64_t;
+  U128 a;
+  __device__ __forceinline__ static Op prep(uint64_t v) { return v; }
+  __device__ __forceinline__ static Op zero_op() { return 0; }
+  __device__ __forceinline__ void zero() { a.zero(); }
+  __device__ __forceinline__ void mac(Op y, Op c) { a.mac(y, c); }
+  __device__ __forceinline__ void get(uint64_t& hi, uint64_t& lo) const { hi = a.hi; lo = a.lo; }
+};
+struct Mac30 {     // residues below 2^60, at most 16 terms per sum
+  struct Op {
+    uint32_t lo, hi;
+  };
+  Acc30 a;
+  __device__ __forceinline__ static Op prep(uint64_t v) { return Op{(uint32_t)v & 0x3FFFFFFFu, (uint32_t)(v >> 30)}; }
+  __device__ __forceinline__ static Op zero_op() { return Op{0, 0}; }
+  __device__ __forceinline__ void zero() { a.zero(); }
+  __device__ __forceinline__ void mac(Op y, Op c) { a.mac(y.lo, y.hi, c.lo, c.hi); }
+  __device__ __forceinline__ void get(uint64_t& hi, uint64_t& lo) const { a.get(hi, lo); }
+};
+
 // Montgomery reduction: for T = hi 2^64 + lo < q 2^64, returns T 2^-64 mod q in [0, q).
 // m = lo (-q^-1) mod 2^64 makes T + m q divisible by 2^64; the low word of the sum is zero,
 // so its carry is 1 exactly when lo != 0.

@@ -368,6 +368,7 @@ Context::Context(int logN_, int L1_, int K_, int dnum_, const std::vector<uint64
   int qbits = 64 - __builtin_clzll(qmax);
   const u128 lim = (u128)1 << 64;
   redc_conv_ok = ((u128)alpha << qbits) <= lim && ((u128)std::max(K, 1) << qbits) <= lim;
+  split_mac_ok = qbits <= 60 && alpha <= 16 && K <= 16 && dnum <= 16;
   d_md_pinv = upload(h_pinv);
   d_md_pinv_sh = upload(pinv_sh);
   d_pmod = upload(h_pmod);

@@ -103,7 +103,9 @@ struct Profiler {
 
 // Launch-shape tunables (set by the autotuner or by the caller through fh_tune_set).
 struct Tune {
-  int ntt_chunk_mb = 96;     // limbs per NTT launch pair, in MB of limb data (MALL residency)
+  // limb data per NTT launch pair, in MB. Measured at N=2^16 (3200 limbs): 96 MB chunks take 3.97 ms
+  // per forward+inverse, one launch pair over all limbs 3.23 ms; launch tails outweigh MALL reuse.
+  int ntt_chunk_mb = 1 << 16;
 };
 
 struct Context;
@@ -140,6 +142,7 @@ struct Context {
   uint64_t* d_modup_qhat_m = nullptr;    // same, Montgomery form [Qhat_i 2^64]_{p_j}
   uint64_t* d_md_phat_m = nullptr;       // [K][L1] Montgomery form [P^_k 2^64]_{q_i}
   bool redc_conv_ok = false;             // conversion sums fit a single Montgomery reduction
+  bool split_mac_ok = false;             // every prime < 2^60 and alpha, K <= 16: 30-bit split products
   // ModDown
   uint64_t* d_md_phinv = nullptr;        // [K]     [P^_k^-1]_{p_k}
   uint64_t* d_md_phinv_sh = nullptr;     // [K]

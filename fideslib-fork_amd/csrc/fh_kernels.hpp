@@ -29,6 +29,26 @@ struct LimbJob {
   uint32_t pad;
 };
 
+__device__ __forceinline__ uint32_t brev_bits(uint32_t x, int bits) { return __brev(x) >> (32 - bits); }
+
+// automorphism gather index in bit-reversed EVALUATION order:
+// out[j] = in[map(j)], map(brev(i)) = brev(((2i+1) k mod 2N) >> 1)   (OpenFHE PrecomputeAutoMap)
+__device__ __forceinline__ uint32_t auto_src(uint32_t j, uint32_t k, int logN) {
+  uint32_t i = brev_bits(j, logN);
+  uint32_t e = ((2 * i + 1) * k) & ((2u << logN) - 1);
+  return brev_bits(e >> 1, logN);
+}
+
+// Epilogue entry of a fused forward NTT over a job table (same index as the NTT job):
+// ModDown: out = (x - v) [P^-1]_{q_pid} (+ add); Rescale: out = x [q_l^-1]_{q_pid} + v.
+struct EpiJob {
+  const uint64_t* x;
+  uint64_t* out;
+  const uint64_t* add;
+  uint32_t pid;
+  uint32_t l;
+};
+
 struct SetAddr {
   LimbSet s;
   // limb-major job order: consecutive blocks work on the same prime (its twiddles stay in L2)
@@ -96,6 +116,20 @@ inline LimbSet make_set(uint64_t* base, uint32_t nlimbs, const int* pids, uint32
   return s;
 }
 
+// Launch KERNEL<SZ, M> with the smallest compiled SZ >= n (n <= 16): the register-resident basis
+// conversions are instantiated for source counts 2, 4, 8, 9, 12 and 16.
+#define FH_SZ_DISPATCH(n, KERNEL, M, LAUNCH, ARGS)                     \
+  {                                                                     \
+    const int n_ = (n);                                                 \
+    if (n_ <= 2) KERNEL<2, M><<<FH_UNPACK LAUNCH>>> ARGS;               \
+    else if (n_ <= 4) KERNEL<4, M><<<FH_UNPACK LAUNCH>>> ARGS;          \
+    else if (n_ <= 8) KERNEL<8, M><<<FH_UNPACK LAUNCH>>> ARGS;          \
+    else if (n_ <= 9) KERNEL<9, M><<<FH_UNPACK LAUNCH>>> ARGS;          \
+    else if (n_ <= 12) KERNEL<12, M><<<FH_UNPACK LAUNCH>>> ARGS;        \
+    else KERNEL<16, M><<<FH_UNPACK LAUNCH>>> ARGS;                      \
+  }
+#define FH_UNPACK(...) __VA_ARGS__
+
 // ---- launchers (fh_ntt.hip) ----
 void ntt_fwd(const Context& c, const LimbSet& s, hipStream_t st);
 void ntt_inv(const Context& c, const LimbSet& s, hipStream_t st);
@@ -103,6 +137,19 @@ void ntt_fwd(const Context& c, const LimbSet& in, const LimbSet& out, hipStream_
 void ntt_inv(const Context& c, const LimbSet& in, const LimbSet& out, hipStream_t st);
 void ntt_fwd_modup(const Context& c, const ModUpAddr& a, uint32_t n, hipStream_t st);
 void ntt_fwd_jobs(const Context& c, const LimbJob* d_jobs, uint32_t njobs, hipStream_t st);
+// Forward NTT of the ModDown conversion (conv: npolys x lq limbs over Q_l, coefficient form, used as
+// scratch) fused with the ModDown combination: out_i = (x_i - NTT(conv)_i) * [P^-1]_{q_i}.
+void ntt_fwd_moddown(const Context& c, const LimbSet& conv, const uint64_t* x, uint64_t x_stride, uint64_t* out, uint64_t out_stride,
+                     hipStream_t st);
+// Forward NTT of the Rescale correction e (npolys x l limbs, coefficient form, scratch) fused with
+// the Rescale combination: out_i = x_i * [q_l^-1]_{q_i} + NTT(e)_i.
+void ntt_fwd_rescale(const Context& c, const LimbSet& e, const uint64_t* x, uint32_t l, uint64_t* out, hipStream_t st);
 void ntt_inv_jobs(const Context& c, const LimbJob* d_jobs, uint32_t njobs, hipStream_t st);
+// out-of-place job forms: job y reads in[y].ptr and writes out[y].ptr (same prime)
+void ntt_fwd_jobs(const Context& c, const LimbJob* d_in, const LimbJob* d_out, uint32_t njobs, hipStream_t st);
+void ntt_inv_jobs(const Context& c, const LimbJob* d_in, const LimbJob* d_out, uint32_t njobs, hipStream_t st);
+// in-place forward NTTs of the job limbs (scratch) with the ModDown / Rescale epilogue of d_epi[y]
+void ntt_fwd_jobs_moddown(const Context& c, const LimbJob* d_jobs, const EpiJob* d_epi, uint32_t njobs, hipStream_t st);
+void ntt_fwd_jobs_rescale(const Context& c, const LimbJob* d_jobs, const EpiJob* d_epi, uint32_t njobs, hipStream_t st);
 
 }  // namespace fh

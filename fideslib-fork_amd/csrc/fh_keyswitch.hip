@@ -21,16 +21,6 @@ constexpr int kT = 256;
 constexpr int kMaxAlpha = 32;
 constexpr int kMaxK = 32;
 
-__device__ __forceinline__ uint32_t brev_bits(uint32_t x, int bits) { return __brev(x) >> (32 - bits); }
-
-// automorphism gather index in bit-reversed EVALUATION order:
-// out[j] = in[map(j)], map(brev(i)) = brev(((2i+1) k mod 2N) >> 1)   (OpenFHE PrecomputeAutoMap)
-__device__ __forceinline__ uint32_t auto_src(uint32_t j, uint32_t k, int logN) {
-  uint32_t i = brev_bits(j, logN);
-  uint32_t e = ((2 * i + 1) * k) & ((2u << logN) - 1);
-  return brev_bits(e >> 1, logN);
-}
-
 // ----------------------------------------------------------------------------------- ModUp conversion
 struct ModUpParams {
   uint32_t N, lq, ext, alpha, L1, K, beta, level, dnum;
@@ -89,12 +79,15 @@ __global__ __launch_bounds__(kT) void k_modup_conv(const uint64_t* __restrict__ 
 }
 
 // Register-resident form for digit sizes up to SZ: two consecutive coefficients per lane (16-byte
-// accesses), the sources prescaled once, each target's 128-bit sum reduced with one Montgomery
-// reduction against the Montgomery-form matrix [Qhat_i 2^64]_{p_j}.
-template <int SZ>
+// accesses), the sources prescaled once and prepared for the multiply-accumulate policy M, each
+// target's 128-bit sum reduced with one Montgomery reduction against the Montgomery-form matrix
+// [Qhat_i 2^64]_{p_j}. Sources beyond the digit's size are zero, so the product loop has no
+// branches.
+template <int SZ, class M>
 __global__ __launch_bounds__(kT) void k_modup_conv_r(const uint64_t* __restrict__ in_coef, const uint64_t* __restrict__ in_eval,
                                                       uint64_t* __restrict__ out, ModUpParams mp) {
-  __shared__ uint64_t s_qhat[SZ * kMaxPrimes];
+  using Op = typename M::Op;
+  __shared__ Op s_qhat[SZ * kMaxPrimes];
   __shared__ uint64_t s_pq[kMaxPrimes], s_pn[kMaxPrimes];
   const uint32_t d = blockIdx.y % mp.beta, b = blockIdx.y / mp.beta;
   in_coef += (size_t)b * mp.lq * mp.N;
@@ -104,9 +97,9 @@ __global__ __launch_bounds__(kT) void k_modup_conv_r(const uint64_t* __restrict_
   const uint32_t sz = min(mp.alpha, mp.lq - start);
   const uint32_t E = mp.L1 + mp.K;
   const size_t tb = ((size_t)mp.level * mp.dnum + d) * mp.alpha;
-  for (uint32_t i = threadIdx.x; i < sz * mp.ext; i += blockDim.x) {
-    uint32_t si = i / mp.ext, pos = i % mp.ext;
-    s_qhat[si * mp.ext + pos] = mp.qhat_m[(tb + si) * E + pos];
+  for (uint32_t i = threadIdx.x; i < SZ * mp.ext; i += blockDim.x) {
+    const uint32_t si = i / mp.ext, pos = i % mp.ext;
+    s_qhat[si * mp.ext + pos] = si < sz ? M::prep(mp.qhat_m[(tb + si) * E + pos]) : M::zero_op();
   }
   for (uint32_t pos = threadIdx.x; pos < mp.ext; pos += blockDim.x) {
     const PrimeConst& P = mp.prime[pos < mp.lq ? pos : mp.L1 + (pos - mp.lq)];
@@ -123,15 +116,18 @@ __global__ __launch_bounds__(kT) void k_modup_conv_r(const uint64_t* __restrict_
   __syncthreads();
   uint64_t* od = out + (size_t)d * mp.ext * mp.N;
   for (uint32_t n = 2 * (blockIdx.x * blockDim.x + threadIdx.x); n < mp.N; n += 2 * gridDim.x * blockDim.x) {
-    uint64_t y0[SZ], y1[SZ];
+    Op y0[SZ], y1[SZ];
 #pragma unroll
     for (int i = 0; i < SZ; ++i) {
       if (i < (int)sz) {
         const size_t o = (size_t)(start + i) * mp.N + n;
         const ulonglong2 c = *reinterpret_cast<const ulonglong2*>(in_coef + o);
-        y0[i] = shoup(c.x, inv[i], invsh[i], qs[i]);
-        y1[i] = shoup(c.y, inv[i], invsh[i], qs[i]);
+        y0[i] = M::prep(shoup(c.x, inv[i], invsh[i], qs[i]));
+        y1[i] = M::prep(shoup(c.y, inv[i], invsh[i], qs[i]));
         *reinterpret_cast<ulonglong2*>(od + o) = *reinterpret_cast<const ulonglong2*>(in_eval + o);
+      } else {
+        y0[i] = M::zero_op();
+        y1[i] = M::zero_op();
       }
     }
     for (uint32_t pos = 0; pos < mp.ext; ++pos) {
@@ -139,19 +135,20 @@ __global__ __launch_bounds__(kT) void k_modup_conv_r(const uint64_t* __restrict_
         pos += sz - 1;
         continue;
       }
-      U128 a0, a1;
+      M a0, a1;
       a0.zero();
       a1.zero();
 #pragma unroll
       for (int i = 0; i < SZ; ++i) {
-        if (i < (int)sz) {
-          const uint64_t c = s_qhat[i * mp.ext + pos];
-          a0.mac(y0[i], c);
-          a1.mac(y1[i], c);
-        }
+        const Op c = s_qhat[i * mp.ext + pos];
+        a0.mac(y0[i], c);
+        a1.mac(y1[i], c);
       }
       const uint64_t pq = s_pq[pos], pn = s_pn[pos];
-      *reinterpret_cast<ulonglong2*>(od + (size_t)pos * mp.N + n) = make_ulonglong2(redc(a0.hi, a0.lo, pq, pn), redc(a1.hi, a1.lo, pq, pn));
+      uint64_t h0, l0, h1, l1;
+      a0.get(h0, l0);
+      a1.get(h1, l1);
+      *reinterpret_cast<ulonglong2*>(od + (size_t)pos * mp.N + n) = make_ulonglong2(redc(h0, l0, pq, pn), redc(h1, l1, pq, pn));
     }
   }
 }
@@ -170,6 +167,7 @@ struct KskParams {
 
 // blockIdx.y = extended position. out0 = sum_d digit_d b_d (+ P c0), out1 = sum_d digit_d a_d,
 // both written through the automorphism (scatter to the image slot).
+template <class M>
 __global__ __launch_bounds__(kT) void k_ksk_inner(const uint64_t* __restrict__ digits, uint64_t* __restrict__ out0,
                                                    uint64_t* __restrict__ out1, KskParams kp) {
   const uint32_t pos = blockIdx.y;
@@ -181,17 +179,18 @@ __global__ __launch_bounds__(kT) void k_ksk_inner(const uint64_t* __restrict__ d
   const bool addext = kp.c0 != nullptr && kp.c0_ext;
   const uint64_t pm = addc0 ? kp.pmod[pos] : 0, pmsh = addc0 ? kp.pmod_sh[pos] : 0;
   for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < kp.N; n += gridDim.x * blockDim.x) {
-    U128 a0, a1;
+    M a0, a1;
     a0.zero();
     a1.zero();
     for (uint32_t d = 0; d < kp.beta; ++d) {
-      const uint64_t x = digits[((size_t)d * kp.ext + pos) * N + n];
-      const uint64_t kb = kp.key[(((size_t)d * 2 + 0) * E + p) * N + n];
-      const uint64_t ka = kp.key[(((size_t)d * 2 + 1) * E + p) * N + n];
-      a0.mac(x, kb);
-      a1.mac(x, ka);
+      const auto x = M::prep(digits[((size_t)d * kp.ext + pos) * N + n]);
+      a0.mac(x, M::prep(kp.key[(((size_t)d * 2 + 0) * E + p) * N + n]));
+      a1.mac(x, M::prep(kp.key[(((size_t)d * 2 + 1) * E + p) * N + n]));
     }
-    uint64_t r0 = redc(a0.hi, a0.lo, P), r1 = redc(a1.hi, a1.lo, P);  // keys are in Montgomery form
+    uint64_t h0, l0, h1, l1;
+    a0.get(h0, l0);
+    a1.get(h1, l1);
+    uint64_t r0 = redc(h0, l0, P), r1 = redc(h1, l1, P);  // keys are in Montgomery form
     if (addc0) r0 = add_mod(r0, shoup(kp.c0[(size_t)pos * N + n], pm, pmsh, P.q), P.q);
     if (addext) r0 = add_mod(r0, kp.c0[(size_t)pos * N + n], P.q);
     // gather form: out[j] = in[auto_src(j)]  <=>  scatter: out[auto_dst(n)] = in[n]
@@ -232,8 +231,10 @@ struct KskBatchParams {
   const uint64_t* pmod_sh;
 };
 
+template <class M>
 __global__ __launch_bounds__(kT) void k_ksk_batch(const uint64_t* __restrict__ digits, uint64_t* __restrict__ out, KskBatchParams kp,
                                                    RotGroups g) {
+  using Op = typename M::Op;
   const uint32_t pos = blockIdx.y, grp = blockIdx.z;
   const uint32_t p = pos < kp.lq ? pos : kp.L1 + (pos - kp.lq);
   const PrimeConst P = kp.prime[p];
@@ -245,12 +246,16 @@ __global__ __launch_bounds__(kT) void k_ksk_batch(const uint64_t* __restrict__ d
   const uint64_t pm = addc0 ? kp.pmod[pos] : 0, pmsh = addc0 ? kp.pmod_sh[pos] : 0;
   // two consecutive coefficients per lane: 16-byte loads of keys, digits and c0
   for (uint32_t n = 2 * (blockIdx.x * blockDim.x + threadIdx.x); n < kp.N; n += 2 * gridDim.x * blockDim.x) {
-    ulonglong2 kb[kMaxDnum], ka[kMaxDnum];
+    Op kb0[kMaxDnum], kb1[kMaxDnum], ka0[kMaxDnum], ka1[kMaxDnum];
 #pragma unroll
     for (uint32_t d = 0; d < kMaxDnum; ++d) {
       if (d < kp.beta) {
-        kb[d] = *reinterpret_cast<const ulonglong2*>(key + (((size_t)d * 2 + 0) * E + p) * N + n);
-        ka[d] = *reinterpret_cast<const ulonglong2*>(key + (((size_t)d * 2 + 1) * E + p) * N + n);
+        const ulonglong2 b = *reinterpret_cast<const ulonglong2*>(key + (((size_t)d * 2 + 0) * E + p) * N + n);
+        const ulonglong2 a = *reinterpret_cast<const ulonglong2*>(key + (((size_t)d * 2 + 1) * E + p) * N + n);
+        kb0[d] = M::prep(b.x);
+        kb1[d] = M::prep(b.y);
+        ka0[d] = M::prep(a.x);
+        ka1[d] = M::prep(a.y);
       }
     }
     const uint32_t j0 = kinv != 1 ? auto_src(n, kinv, kp.logN) : n;
@@ -260,7 +265,7 @@ __global__ __launch_bounds__(kT) void k_ksk_batch(const uint64_t* __restrict__ d
     for (uint32_t t = 0; t < cnt; ++t) {
       const uint32_t b = g.order[first + t];
       const uint64_t* dg = digits + b * kp.dig_stride + (size_t)pos * N + n;
-      U128 a0, a1, b0, b1;
+      M a0, a1, b0, b1;
       a0.zero();
       a1.zero();
       b0.zero();
@@ -269,15 +274,23 @@ __global__ __launch_bounds__(kT) void k_ksk_batch(const uint64_t* __restrict__ d
       for (uint32_t d = 0; d < kMaxDnum; ++d) {
         if (d < kp.beta) {
           const ulonglong2 x = *reinterpret_cast<const ulonglong2*>(dg + (size_t)d * kp.ext * N);
-          a0.mac(x.x, kb[d].x);
-          a1.mac(x.x, ka[d].x);
-          b0.mac(x.y, kb[d].y);
-          b1.mac(x.y, ka[d].y);
+          const Op x0 = M::prep(x.x), x1 = M::prep(x.y);
+          a0.mac(x0, kb0[d]);
+          a1.mac(x0, ka0[d]);
+          b0.mac(x1, kb1[d]);
+          b1.mac(x1, ka1[d]);
         }
       }
       // keys are in Montgomery form: one REDC gives sum digit * key mod p
-      uint64_t r0 = redc(a0.hi, a0.lo, P), r1 = redc(a1.hi, a1.lo, P);
-      uint64_t s0 = redc(b0.hi, b0.lo, P), s1 = redc(b1.hi, b1.lo, P);
+      uint64_t h, l;
+      a0.get(h, l);
+      uint64_t r0 = redc(h, l, P);
+      a1.get(h, l);
+      uint64_t r1 = redc(h, l, P);
+      b0.get(h, l);
+      uint64_t s0 = redc(h, l, P);
+      b1.get(h, l);
+      uint64_t s1 = redc(h, l, P);
       if (addc0) {
         const ulonglong2 c = *reinterpret_cast<const ulonglong2*>(kp.cts + b * kp.ct_stride + (size_t)pos * N + n);
         r0 = add_mod(r0, shoup(c.x, pm, pmsh, P.q), P.q);
@@ -335,14 +348,15 @@ __global__ __launch_bounds__(kT) void k_moddown_conv(const uint64_t* __restrict_
 }
 
 // Register-resident form for K <= KK special primes: two coefficients per lane, Montgomery-form
-// matrix [P^_k 2^64]_{q_i}, one Montgomery reduction per output residue.
-template <int KK>
+// matrix [P^_k 2^64]_{q_i}, one Montgomery reduction per output residue; sources beyond K are zero.
+template <int KK, class M>
 __global__ __launch_bounds__(kT) void k_moddown_conv_r(const uint64_t* __restrict__ pcoef, uint64_t* __restrict__ conv, ModDownParams mp) {
-  __shared__ uint64_t s_phat[KK * kMaxPrimes];
+  using Op = typename M::Op;
+  __shared__ Op s_phat[KK * kMaxPrimes];
   __shared__ uint64_t s_q[kMaxPrimes], s_n[kMaxPrimes];
-  for (uint32_t i = threadIdx.x; i < mp.K * mp.lq; i += blockDim.x) {
+  for (uint32_t i = threadIdx.x; i < KK * mp.lq; i += blockDim.x) {
     uint32_t k = i / mp.lq, t = i % mp.lq;
-    s_phat[k * mp.lq + t] = mp.phat_m[(size_t)k * mp.L1 + t];
+    s_phat[k * mp.lq + t] = k < mp.K ? M::prep(mp.phat_m[(size_t)k * mp.L1 + t]) : M::zero_op();
   }
   for (uint32_t i = threadIdx.x; i < mp.lq; i += blockDim.x) {
     s_q[i] = mp.prime[i].q;
@@ -358,52 +372,35 @@ __global__ __launch_bounds__(kT) void k_moddown_conv_r(const uint64_t* __restric
   const uint64_t* pc = pcoef + (size_t)blockIdx.y * mp.K * mp.N;
   uint64_t* cv = conv + (size_t)blockIdx.y * mp.lq * mp.N;
   for (uint32_t n = 2 * (blockIdx.x * blockDim.x + threadIdx.x); n < mp.N; n += 2 * gridDim.x * blockDim.x) {
-    uint64_t y0[KK], y1[KK];
+    Op y0[KK], y1[KK];
 #pragma unroll
     for (int k = 0; k < KK; ++k) {
       if (k < (int)mp.K) {
         const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(pc + (size_t)k * mp.N + n);
-        y0[k] = shoup(v.x, inv[k], invsh[k], ps[k]);
-        y1[k] = shoup(v.y, inv[k], invsh[k], ps[k]);
+        y0[k] = M::prep(shoup(v.x, inv[k], invsh[k], ps[k]));
+        y1[k] = M::prep(shoup(v.y, inv[k], invsh[k], ps[k]));
+      } else {
+        y0[k] = M::zero_op();
+        y1[k] = M::zero_op();
       }
     }
     for (uint32_t i = 0; i < mp.lq; ++i) {
-      U128 a0, a1;
+      M a0, a1;
       a0.zero();
       a1.zero();
 #pragma unroll
       for (int k = 0; k < KK; ++k) {
-        if (k < (int)mp.K) {
-          const uint64_t c = s_phat[k * mp.lq + i];
-          a0.mac(y0[k], c);
-          a1.mac(y1[k], c);
-        }
+        const Op c = s_phat[k * mp.lq + i];
+        a0.mac(y0[k], c);
+        a1.mac(y1[k], c);
       }
       const uint64_t q = s_q[i], nv = s_n[i];
-      *reinterpret_cast<ulonglong2*>(cv + (size_t)i * mp.N + n) = make_ulonglong2(redc(a0.hi, a0.lo, q, nv), redc(a1.hi, a1.lo, q, nv));
+      uint64_t h0, l0, h1, l1;
+      a0.get(h0, l0);
+      a1.get(h1, l1);
+      *reinterpret_cast<ulonglong2*>(cv + (size_t)i * mp.N + n) = make_ulonglong2(redc(h0, l0, q, nv), redc(h1, l1, q, nv));
     }
   }
-}
-
-// out_i = (x_i - conv_i) * [P^-1]_{q_i}; blockIdx.y = poly * lq + i. x has ext limbs per poly.
-__global__ __launch_bounds__(kT) void k_moddown_final(const uint64_t* __restrict__ x, const uint64_t* __restrict__ conv,
-                                                       uint64_t* __restrict__ out, ModDownParams mp, uint64_t x_stride, uint64_t out_stride) {
-  const uint32_t poly = blockIdx.y / mp.lq, i = blockIdx.y % mp.lq;
-  const uint64_t q = mp.prime[i].q, w = mp.pinv[i], wsh = mp.pinv_sh[i];
-  const uint64_t* xi = x + poly * x_stride + (size_t)i * mp.N;
-  const uint64_t* ci = conv + ((size_t)poly * mp.lq + i) * mp.N;
-  uint64_t* oi = out + poly * out_stride + (size_t)i * mp.N;
-  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < mp.N; n += gridDim.x * blockDim.x)
-    oi[n] = shoup(sub_mod(xi[n], ci[n], q), w, wsh, q);
-}
-
-// copy the K special limbs of each poly (ext limbs per poly, stride x_stride) to a packed buffer
-__global__ __launch_bounds__(kT) void k_gather_special(const uint64_t* __restrict__ x, uint64_t* __restrict__ out, uint32_t N, uint32_t lq,
-                                                        uint32_t K, uint64_t x_stride) {
-  const uint32_t poly = blockIdx.y / K, k = blockIdx.y % K;
-  const uint64_t* src = x + poly * x_stride + (size_t)(lq + k) * N;
-  uint64_t* dst = out + ((size_t)poly * K + k) * N;
-  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) dst[n] = src[n];
 }
 
 // ----------------------------------------------------------------------------------- Rescale
@@ -433,27 +430,6 @@ __global__ __launch_bounds__(kT) void k_rescale_expand(const uint64_t* __restric
     if (v >= qi) v %= qi;
     ei[n] = shoup(v, f, fsh, qi);
   }
-}
-
-// out_i = x_i * [q_l^-1]_{q_i} + e_i; blockIdx.y = comp * l + i. x: (l+1) limbs per comp.
-__global__ __launch_bounds__(kT) void k_rescale_final(const uint64_t* __restrict__ x, const uint64_t* __restrict__ e,
-                                                       uint64_t* __restrict__ out, RescaleParams rp) {
-  const uint32_t comp = blockIdx.y / rp.l, i = blockIdx.y % rp.l;
-  const uint64_t qi = rp.prime[i].q;
-  const size_t o = (size_t)rp.l * rp.L1 + i;
-  const uint64_t w = rp.qlinv[o], wsh = rp.qlinv_sh[o];
-  const uint64_t* xi = x + ((size_t)comp * (rp.l + 1) + i) * rp.N;
-  const uint64_t* ei = e + ((size_t)comp * rp.l + i) * rp.N;
-  uint64_t* oi = out + ((size_t)comp * rp.l + i) * rp.N;
-  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < rp.N; n += gridDim.x * blockDim.x)
-    oi[n] = add_mod(shoup(xi[n], w, wsh, qi), ei[n], qi);
-}
-
-// copy limb l of each component (stride (l+1) limbs) into a packed buffer
-__global__ __launch_bounds__(kT) void k_take_top(const uint64_t* __restrict__ x, uint64_t* __restrict__ out, uint32_t N, uint32_t l) {
-  const uint64_t* src = x + ((size_t)blockIdx.y * (l + 1) + l) * N;
-  uint64_t* dst = out + (size_t)blockIdx.y * N;
-  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) dst[n] = src[n];
 }
 
 // ----------------------------------------------------------------------------------- automorphism
@@ -492,10 +468,8 @@ void modup_batch(Context& c, const uint64_t* x, int l, int npolys, uint64_t x_st
     ProfScope ps_(c, "modup_conv", st);
     const dim3 g2 = grid_n(c.N, beta * npolys, 2);
     if (!c.redc_conv_ok || c.alpha > 16) k_modup_conv<<<grid_n(c.N, beta * npolys), kT, 0, st>>>(tmp.p, x, digits, mp);
-    else if (c.alpha <= 4) k_modup_conv_r<4><<<g2, kT, 0, st>>>(tmp.p, x, digits, mp);
-    else if (c.alpha <= 8) k_modup_conv_r<8><<<g2, kT, 0, st>>>(tmp.p, x, digits, mp);
-    else if (c.alpha <= 12) k_modup_conv_r<12><<<g2, kT, 0, st>>>(tmp.p, x, digits, mp);
-    else k_modup_conv_r<16><<<g2, kT, 0, st>>>(tmp.p, x, digits, mp);
+    else if (c.split_mac_ok) FH_SZ_DISPATCH(c.alpha, k_modup_conv_r, Mac30, (g2, kT, 0, st), (tmp.p, x, digits, mp))
+    else FH_SZ_DISPATCH(c.alpha, k_modup_conv_r, MacWide, (g2, kT, 0, st), (tmp.p, x, digits, mp))
   }
   FH_HIP(hipGetLastError());
   ModUpAddr a{digits, ext, lq, (uint32_t)c.alpha, (uint32_t)c.L1, beta, 0, (uint64_t)beta * ext * c.N};
@@ -512,52 +486,76 @@ void ksk_inner(Context& c, const uint64_t* digits, int l, const uint64_t* key, u
     // the kernel scatters through the inverse map: pass k^-1 so that out[j] = in[src_k(j)]
     kp.galois = (uint32_t)h_inv_odd(galois, 2ull * c.N);
   }
-  { ProfScope ps_(c, "ksk_inner", st); k_ksk_inner<<<grid_n(c.N, ext), kT, 0, st>>>(digits, out0, out1, kp); }
+  {
+    ProfScope ps_(c, "ksk_inner", st);
+    if (c.split_mac_ok) k_ksk_inner<Mac30><<<grid_n(c.N, ext), kT, 0, st>>>(digits, out0, out1, kp);
+    else k_ksk_inner<MacWide><<<grid_n(c.N, ext), kT, 0, st>>>(digits, out0, out1, kp);
+  }
   FH_HIP(hipGetLastError());
 }
+
+namespace {
+// [a, a + an) and [b, b + bn) words overlap
+inline bool overlaps(const uint64_t* a, size_t an, const uint64_t* b, size_t bn) { return a < b + bn && b < a + an; }
+}  // namespace
 
 void moddown(Context& c, const uint64_t* x, int l, uint64_t* out, int npolys, uint64_t x_stride, uint64_t out_stride, hipStream_t st) {
   const uint32_t lq = l + 1, ext = lq + c.K, K = c.K;
   if (K > (uint32_t)kMaxK) throw Error(-3, "special prime count above kernel limit");
+  if (npolys < 1) return;
   if (!x_stride) x_stride = (uint64_t)ext * c.N;
   if (!out_stride) out_stride = (uint64_t)lq * c.N;
+  const size_t xn = (size_t)(npolys - 1) * x_stride + (size_t)ext * c.N, on = (size_t)(npolys - 1) * out_stride + (size_t)lq * c.N;
+  if (overlaps(x, xn, out, on)) {
+    // the fused epilogue reads x while other blocks write out: go through a separate buffer
+    DBuf tmp(c.pool, (size_t)npolys * lq * c.N);
+    moddown(c, x, l, tmp.p, npolys, x_stride, (uint64_t)lq * c.N, st);
+    for (int b = 0; b < npolys; ++b)
+      FH_HIP(hipMemcpyAsync(out + b * out_stride, tmp.p + (size_t)b * lq * c.N, (size_t)lq * c.N * 8, hipMemcpyDeviceToDevice, st));
+    return;
+  }
   DBuf pc(c.pool, (size_t)npolys * K * c.N), conv(c.pool, (size_t)npolys * lq * c.N);
-  { ProfScope ps_(c, "gather_special", st); k_gather_special<<<grid_n(c.N, npolys * K), kT, 0, st>>>(x, pc.p, c.N, lq, K, x_stride); }
   std::vector<int> pids(K), qids(lq);
   for (uint32_t k = 0; k < K; ++k) pids[k] = c.L1 + k;
   for (uint32_t i = 0; i < lq; ++i) qids[i] = i;
-  ntt_inv(c, make_set(pc.p, K, pids.data(), npolys, (uint64_t)K * c.N), st);
+  // INTT of the special limbs straight out of the extended polys into a packed buffer
+  ntt_inv(c, make_set(const_cast<uint64_t*>(x) + (size_t)lq * c.N, K, pids.data(), npolys, x_stride),
+          make_set(pc.p, K, pids.data(), npolys, (uint64_t)K * c.N), st);
   ModDownParams mp{c.N, lq, ext, (uint32_t)c.L1, K, c.d_prime, c.d_md_phinv, c.d_md_phinv_sh, c.d_md_phat, c.d_md_phat_m,
                    c.d_md_pinv, c.d_md_pinv_sh};
   {
     ProfScope ps_(c, "moddown_conv", st);
     const dim3 g2 = grid_n(c.N, npolys, 2);
     if (!c.redc_conv_ok || K > 16) k_moddown_conv<<<grid_n(c.N, npolys), kT, 0, st>>>(pc.p, conv.p, mp);
-    else if (K <= 4) k_moddown_conv_r<4><<<g2, kT, 0, st>>>(pc.p, conv.p, mp);
-    else if (K <= 8) k_moddown_conv_r<8><<<g2, kT, 0, st>>>(pc.p, conv.p, mp);
-    else if (K <= 12) k_moddown_conv_r<12><<<g2, kT, 0, st>>>(pc.p, conv.p, mp);
-    else k_moddown_conv_r<16><<<g2, kT, 0, st>>>(pc.p, conv.p, mp);
+    else if (c.split_mac_ok) FH_SZ_DISPATCH((int)K, k_moddown_conv_r, Mac30, (g2, kT, 0, st), (pc.p, conv.p, mp))
+    else FH_SZ_DISPATCH((int)K, k_moddown_conv_r, MacWide, (g2, kT, 0, st), (pc.p, conv.p, mp))
   }
   FH_HIP(hipGetLastError());
-  ntt_fwd(c, make_set(conv.p, lq, qids.data(), npolys, (uint64_t)lq * c.N), st);
-  { ProfScope ps_(c, "moddown_final", st); k_moddown_final<<<grid_n(c.N, npolys * lq), kT, 0, st>>>(x, conv.p, out, mp, x_stride, out_stride); }
-  FH_HIP(hipGetLastError());
+  // forward NTT of the conversion with (x_i - conv_i) * P^-1 fused into its last launch
+  ntt_fwd_moddown(c, make_set(conv.p, lq, qids.data(), npolys, (uint64_t)lq * c.N), x, x_stride, out, out_stride, st);
 }
 
 void rescale(Context& c, const uint64_t* x, int l, int ncomp, uint64_t* out, hipStream_t st) {
   if (l < 1) throw Error(-1, "cannot rescale a level-0 ciphertext");
+  if (ncomp < 1) return;
+  if (overlaps(x, (size_t)ncomp * (l + 1) * c.N, out, (size_t)ncomp * l * c.N)) {
+    DBuf tmp(c.pool, (size_t)ncomp * l * c.N);
+    rescale(c, x, l, ncomp, tmp.p, st);
+    FH_HIP(hipMemcpyAsync(out, tmp.p, (size_t)ncomp * l * c.N * 8, hipMemcpyDeviceToDevice, st));
+    return;
+  }
   DBuf top(c.pool, (size_t)ncomp * c.N), e(c.pool, (size_t)ncomp * l * c.N);
-  { ProfScope ps_(c, "take_top", st); k_take_top<<<grid_n(c.N, ncomp), kT, 0, st>>>(x, top.p, c.N, l); }
   int pid_top = l;
-  ntt_inv(c, make_set(top.p, 1, &pid_top, ncomp, c.N), st);
+  // INTT of the top limb of every component straight into a packed buffer
+  ntt_inv(c, make_set(const_cast<uint64_t*>(x) + (size_t)l * c.N, 1, &pid_top, ncomp, (uint64_t)(l + 1) * c.N),
+          make_set(top.p, 1, &pid_top, ncomp, c.N), st);
   RescaleParams rp{c.N, (uint32_t)l, (uint32_t)c.L1, c.d_prime, c.d_rs_qlinv, c.d_rs_qlinv_sh, c.d_rs_QlQl, c.d_rs_QlQl_sh};
   { ProfScope ps_(c, "rescale_expand", st); k_rescale_expand<<<grid_n(c.N, ncomp * l), kT, 0, st>>>(top.p, e.p, rp); }
   FH_HIP(hipGetLastError());
   std::vector<int> ids(l);
   for (int i = 0; i < l; ++i) ids[i] = i;
-  ntt_fwd(c, make_set(e.p, l, ids.data(), ncomp, (uint64_t)l * c.N), st);
-  { ProfScope ps_(c, "rescale_final", st); k_rescale_final<<<grid_n(c.N, ncomp * l), kT, 0, st>>>(x, e.p, out, rp); }
-  FH_HIP(hipGetLastError());
+  // forward NTT of the correction with x_i * q_l^-1 + e_i fused into its last launch
+  ntt_fwd_rescale(c, make_set(e.p, l, ids.data(), ncomp, (uint64_t)l * c.N), x, (uint32_t)l, out, st);
 }
 
 void automorph(Context& c, const uint64_t* x, int nlimbs, uint32_t k, uint64_t* out, hipStream_t st) {
@@ -649,7 +647,9 @@ void rotate_core(Context& c, int nb, const uint64_t* digits, uint64_t dig_stride
       ++ng;
     }
     ProfScope ps_(c, "ksk_batch", st);
-    k_ksk_batch<<<dim3(grid_n(c.N, 1, 2).x, ext, ng), kT, 0, st>>>(digits, eo, kp, g);
+    const dim3 gk(grid_n(c.N, 1, 2).x, ext, ng);
+    if (c.split_mac_ok) k_ksk_batch<Mac30><<<gk, kT, 0, st>>>(digits, eo, kp, g);
+    else k_ksk_batch<MacWide><<<gk, kT, 0, st>>>(digits, eo, kp, g);
     FH_HIP(hipGetLastError());
   }
   if (do_moddown) moddown(c, eo, l, out, 2 * nb, 0, 0, st);

@@ -27,9 +27,11 @@
 //        [0, 4q), inverse values in [0, 2q).
 //    Both are exact, so the canonical outputs are identical.
 //  * Between the two launches the limb holds an internal form (lazy integers or
-//    doubles). Large batches are split into chunks of limbs small enough that
-//    the intermediate written by the first launch is still in the MALL when the
-//    second launch reads it back.
+//    doubles). A batch may be split into chunks of limbs (tunable "ntt_chunk_mb")
+//    so that the intermediate is still in the MALL when the second launch reads
+//    it back; by default one launch pair covers the whole batch.
+//  * The last forward launch hands its outputs to an epilogue, which fuses the
+//    element-wise step that consumes them (ModDown, Rescale).
 #include <algorithm>
 
 #include "fh_kernels.hpp"
@@ -209,14 +211,15 @@ __global__ __launch_bounds__(kThreads) void k_ntt_fwd_cols(Addr ad, AddrOut ao, 
 
 // ------------------------------------------------------------------------------- forward, rows
 // Last l2 stages. Input: internal form, output: canonical residues.
-template <int LS, class A, class TwT>
+template <int LS, class A, class TwT, class Epi>
 __device__ __forceinline__ void fwd_rows_body(const A& ar, const TwT* tw, const TwT* twp, uint64_t* p, uint64_t* tile, int l1,
-                                              int R) {
+                                              int R, const Epi& epi) {
   constexpr int S = 1 << LS, T = S / 16, PADS = S + 1;
   const int tid = threadIdx.x;
   const int rr = tid / T, c0 = tid % T;
   const uint32_t rho = blockIdx.x * R + rr;
   uint64_t* row = p + (size_t)rho * S;
+  const uint32_t rbase = rho * S;
   typename A::T x[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(row[c0 + T * k]);
@@ -255,24 +258,83 @@ __device__ __forceinline__ void fwd_rows_body(const A& ar, const TwT* tw, const 
     for (int k = 0; k < 16; ++k) t[16 * c0 + k] = ar.canon(x[k]);
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 16; ++k) row[c0 + T * k] = t[c0 + T * k];
+    for (int k = 0; k < 16; ++k) epi.store(row, rbase, c0 + T * k, t[c0 + T * k]);
   } else {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) row[c0 + T * k] = ar.canon(x[k]);
+    for (int k = 0; k < 16; ++k) epi.store(row, rbase, c0 + T * k, ar.canon(x[k]));
   }
 }
 
-template <int LS, class Addr>
-__global__ __launch_bounds__(kThreads) void k_ntt_fwd_rows(Addr ad, NttTables tb, int l1) {
+// ------------------------------------------------------------------------------- forward epilogues
+// The last launch of a forward NTT hands every canonical output word to an epilogue, which
+// either stores it in place (EpiStore) or fuses the element-wise step that consumes it: the
+// ModDown combination (x_i - v) * [P^-1]_{q_i} (src/NTTfusions.cuh:237-252) or the Rescale
+// combination x_i * [q_l^-1]_{q_i} + v (src/NTTfusions.cuh:214-235). Job y addresses
+// (poly, limb) = (y % npolys, y / npolys), the order of SetAddr.
+struct EpiStore {
+  __device__ __forceinline__ void init(uint32_t, uint32_t, const PrimeConst*) {}
+  __device__ __forceinline__ void store(uint64_t* row, uint32_t, uint32_t j, uint64_t v) const { row[j] = v; }
+};
+
+struct EpiModDown {
+  const uint64_t* x;       // ext polys, limb i of poly b at x + b * x_stride + i * N
+  uint64_t x_stride;
+  uint64_t* out;           // Q_l polys, limb i of poly b at out + b * out_stride + i * N
+  uint64_t out_stride;
+  const uint64_t* w;       // [P^-1]_{q_i}, Shoup companions
+  const uint64_t* wsh;
+  uint32_t npolys;
+  // per block
+  const uint64_t* xi;
+  uint64_t* oi;
+  uint64_t q, wi, wish;
+  __device__ __forceinline__ void init(uint32_t y, uint32_t N, const PrimeConst* prime) {
+    const uint32_t i = y / npolys, b = y % npolys;
+    xi = x + b * x_stride + (size_t)i * N;
+    oi = out + b * out_stride + (size_t)i * N;
+    q = prime[i].q;
+    wi = w[i];
+    wish = wsh[i];
+  }
+  __device__ __forceinline__ void store(uint64_t*, uint32_t rbase, uint32_t j, uint64_t v) const {
+    oi[rbase + j] = shoup(sub_mod(xi[rbase + j], v, q), wi, wish, q);
+  }
+};
+
+struct EpiRescale {
+  const uint64_t* x;       // ncomp components of l+1 limbs each
+  uint64_t* out;           // ncomp components of l limbs each
+  const uint64_t* w;       // [q_l^-1]_{q_i} row of level l, Shoup companions
+  const uint64_t* wsh;
+  uint32_t l, npolys;
+  const uint64_t* xi;
+  uint64_t* oi;
+  uint64_t q, wi, wish;
+  __device__ __forceinline__ void init(uint32_t y, uint32_t N, const PrimeConst* prime) {
+    const uint32_t i = y / npolys, b = y % npolys;
+    xi = x + ((size_t)b * (l + 1) + i) * N;
+    oi = out + ((size_t)b * l + i) * N;
+    q = prime[i].q;
+    wi = w[i];
+    wish = wsh[i];
+  }
+  __device__ __forceinline__ void store(uint64_t*, uint32_t rbase, uint32_t j, uint64_t v) const {
+    oi[rbase + j] = add_mod(shoup(xi[rbase + j], wi, wish, q), v, q);
+  }
+};
+
+template <int LS, class Addr, class Epi>
+__global__ __launch_bounds__(kThreads) void k_ntt_fwd_rows(Addr ad, NttTables tb, int l1, Epi epi) {
   constexpr int S = 1 << LS, T = S / 16, PADS = S + 1, RMAX = kThreads / T;
   __shared__ uint64_t tile[RMAX * PADS];
   uint64_t* p;
   int pid;
   ad.at(blockIdx.y, tb.N, p, pid);
+  epi.init(blockIdx.y, tb.N, tb.prime);
   const Sel sel = select(tb, pid, false);
   const int R = blockDim.x / T;
-  if (sel.fp) fwd_rows_body<LS>(sel.fa, sel.twd, sel.twd, p, tile, l1, R);
-  else fwd_rows_body<LS>(sel.ia, sel.tw, sel.twsh, p, tile, l1, R);
+  if (sel.fp) fwd_rows_body<LS>(sel.fa, sel.twd, sel.twd, p, tile, l1, R, epi);
+  else fwd_rows_body<LS>(sel.ia, sel.tw, sel.twsh, p, tile, l1, R, epi);
 }
 
 // ------------------------------------------------------------------------------- inverse, rows
@@ -475,14 +537,70 @@ struct Shift {
   __device__ __forceinline__ void at(uint32_t y, uint32_t N, uint64_t*& p, int& pid) const { a.at(y + base, N, p, pid); }
 };
 
+// Epilogues of a batch at mixed levels: one table entry per limb job (same index as the NTT job).
+struct EpiModDownJobs {
+  const EpiJob* jobs;
+  const uint64_t* w;       // [P^-1]_{q_i} by prime, Shoup companions
+  const uint64_t* wsh;
+  const uint64_t* xi;
+  uint64_t* oi;
+  const uint64_t* ai;
+  uint64_t q, wi, wish;
+  __device__ __forceinline__ void init(uint32_t y, uint32_t, const PrimeConst* prime) {
+    const EpiJob j = jobs[y];
+    xi = j.x;
+    oi = j.out;
+    ai = j.add;
+    q = prime[j.pid].q;
+    wi = w[j.pid];
+    wish = wsh[j.pid];
+  }
+  __device__ __forceinline__ void store(uint64_t*, uint32_t rbase, uint32_t jj, uint64_t v) const {
+    uint64_t r = shoup(sub_mod(xi[rbase + jj], v, q), wi, wish, q);
+    if (ai) r = add_mod(ai[rbase + jj], r, q);
+    oi[rbase + jj] = r;
+  }
+};
+
+struct EpiRescaleJobs {
+  const EpiJob* jobs;
+  const uint64_t* w;       // [L1][L1] [q_l^-1]_{q_i}, Shoup companions
+  const uint64_t* wsh;
+  uint32_t L1;
+  const uint64_t* xi;
+  uint64_t* oi;
+  uint64_t q, wi, wish;
+  __device__ __forceinline__ void init(uint32_t y, uint32_t, const PrimeConst* prime) {
+    const EpiJob j = jobs[y];
+    xi = j.x;
+    oi = j.out;
+    q = prime[j.pid].q;
+    const size_t o = (size_t)j.l * L1 + j.pid;
+    wi = w[o];
+    wish = wsh[o];
+  }
+  __device__ __forceinline__ void store(uint64_t*, uint32_t rbase, uint32_t jj, uint64_t v) const {
+    oi[rbase + jj] = add_mod(shoup(xi[rbase + jj], wi, wish, q), v, q);
+  }
+};
+
+template <class Epi>
+struct ShiftEpi {
+  Epi e;
+  uint32_t base;
+  __device__ __forceinline__ void init(uint32_t y, uint32_t N, const PrimeConst* prime) { e.init(y + base, N, prime); }
+  __device__ __forceinline__ void store(uint64_t* row, uint32_t rbase, uint32_t j, uint64_t v) const { e.store(row, rbase, j, v); }
+};
+
 // limbs per chunk: the intermediate of one chunk stays MALL-resident between the two launches
 inline uint32_t chunk_limbs(const Context& c) {
   const size_t budget = (size_t)std::max(1, c.tune.ntt_chunk_mb) << 20;
-  return (uint32_t)std::max<size_t>(1, budget / (8ull * c.N));
+  // at most 65535 limbs per launch (grid y limit)
+  return (uint32_t)std::min<size_t>(65535, std::max<size_t>(1, budget / (8ull * c.N)));
 }
 
-template <class Addr, class AddrOut, int L1S, int L2S>
-void fwd_impl(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlimbs, hipStream_t st) {
+template <class Addr, class AddrOut, int L1S, int L2S, class Epi>
+void fwd_impl(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlimbs, hipStream_t st, const Epi& epi, const char* rows_name) {
   NttTables tb = ntt_tables(c);
   dim3 b1 = col_block<L1S>(c.N), b2 = row_block<L2S>(c.N);
   uint32_t C = b1.x / ((1 << L1S) / 16), R = b2.x / ((1 << L2S) / 16);
@@ -491,8 +609,9 @@ void fwd_impl(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlim
     uint32_t n = std::min(chunk, nlimbs - base);
     Shift<Addr> sa{ad, base};
     Shift<AddrOut> so{ao, base};
+    ShiftEpi<Epi> se{epi, base};
     { ProfScope ps_(c, "ntt_fwd_cols", st); k_ntt_fwd_cols<L1S><<<dim3((c.N >> L1S) / C, n), b1, 0, st>>>(sa, so, tb); }
-    { ProfScope ps_(c, "ntt_fwd_rows", st); k_ntt_fwd_rows<L2S><<<dim3((c.N >> L2S) / R, n), b2, 0, st>>>(so, tb, L1S); }
+    { ProfScope ps_(c, rows_name, st); k_ntt_fwd_rows<L2S><<<dim3((c.N >> L2S) / R, n), b2, 0, st>>>(so, tb, L1S, se); }
   }
 }
 
@@ -511,13 +630,14 @@ void inv_impl(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlim
   }
 }
 
-template <class Addr, class AddrOut>
-void dispatch(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlimbs, hipStream_t st, bool fwd) {
+template <class Addr, class AddrOut, class Epi = EpiStore>
+void dispatch(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlimbs, hipStream_t st, bool fwd, const Epi& epi = Epi{},
+              const char* rows_name = "ntt_fwd_rows") {
   if (nlimbs == 0) return;
-#define FH_NTT_CASE(LN, A, B)                                      \
-  case LN:                                                         \
-    if (fwd) fwd_impl<Addr, AddrOut, A, B>(c, ad, ao, nlimbs, st); \
-    else inv_impl<Addr, AddrOut, A, B>(c, ad, ao, nlimbs, st);     \
+#define FH_NTT_CASE(LN, A, B)                                                          \
+  case LN:                                                                             \
+    if (fwd) fwd_impl<Addr, AddrOut, A, B, Epi>(c, ad, ao, nlimbs, st, epi, rows_name); \
+    else inv_impl<Addr, AddrOut, A, B>(c, ad, ao, nlimbs, st);                         \
     break;
   switch (c.logN) {
     FH_NTT_CASE(10, 5, 5)
@@ -548,6 +668,32 @@ void ntt_inv(const Context& c, const LimbSet& in, const LimbSet& out, hipStream_
 }
 void ntt_fwd_jobs(const Context& c, const LimbJob* j, uint32_t n, hipStream_t st) { dispatch(c, JobAddr{j}, JobAddr{j}, n, st, true); }
 void ntt_inv_jobs(const Context& c, const LimbJob* j, uint32_t n, hipStream_t st) { dispatch(c, JobAddr{j}, JobAddr{j}, n, st, false); }
+void ntt_fwd_jobs(const Context& c, const LimbJob* in, const LimbJob* out, uint32_t n, hipStream_t st) {
+  dispatch(c, JobAddr{in}, JobAddr{out}, n, st, true);
+}
+void ntt_inv_jobs(const Context& c, const LimbJob* in, const LimbJob* out, uint32_t n, hipStream_t st) {
+  dispatch(c, JobAddr{in}, JobAddr{out}, n, st, false);
+}
+void ntt_fwd_jobs_moddown(const Context& c, const LimbJob* jobs, const EpiJob* epi, uint32_t n, hipStream_t st) {
+  EpiModDownJobs e{epi, c.d_md_pinv, c.d_md_pinv_sh, nullptr, nullptr, nullptr, 0, 0, 0};
+  dispatch(c, JobAddr{jobs}, JobAddr{jobs}, n, st, true, e, "ntt_fwd_rows_moddown");
+}
+void ntt_fwd_jobs_rescale(const Context& c, const LimbJob* jobs, const EpiJob* epi, uint32_t n, hipStream_t st) {
+  EpiRescaleJobs e{epi, c.d_rs_qlinv, c.d_rs_qlinv_sh, (uint32_t)c.L1, nullptr, nullptr, 0, 0, 0};
+  dispatch(c, JobAddr{jobs}, JobAddr{jobs}, n, st, true, e, "ntt_fwd_rows_rescale");
+}
 void ntt_fwd_modup(const Context& c, const ModUpAddr& a, uint32_t n, hipStream_t st) { dispatch(c, a, a, n, st, true); }
+
+void ntt_fwd_moddown(const Context& c, const LimbSet& conv, const uint64_t* x, uint64_t x_stride, uint64_t* out, uint64_t out_stride,
+                     hipStream_t st) {
+  EpiModDown e{x, x_stride, out, out_stride, c.d_md_pinv, c.d_md_pinv_sh, conv.npolys, nullptr, nullptr, 0, 0, 0};
+  dispatch(c, SetAddr{conv}, SetAddr{conv}, conv.nlimbs * conv.npolys, st, true, e, "ntt_fwd_rows_moddown");
+}
+
+void ntt_fwd_rescale(const Context& c, const LimbSet& e, const uint64_t* x, uint32_t l, uint64_t* out, hipStream_t st) {
+  const size_t o = (size_t)l * c.L1;
+  EpiRescale ep{x, out, c.d_rs_qlinv + o, c.d_rs_qlinv_sh + o, l, e.npolys, nullptr, nullptr, 0, 0, 0};
+  dispatch(c, SetAddr{e}, SetAddr{e}, e.nlimbs * e.npolys, st, true, ep, "ntt_fwd_rows_rescale");
+}
 
 }  // namespace fh

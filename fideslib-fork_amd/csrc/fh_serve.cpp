@@ -1,0 +1,255 @@
+// Continuous-batching serving loop (see fh_serve.hpp).
+#include "fh_serve.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <deque>
+#include <memory>
+#include <random>
+#include <thread>
+
+#include "fh_keyswitch.hpp"
+
+namespace fh {
+
+namespace {
+
+enum OpKind : uint8_t { kMulPtRs, kRotate, kAdd, kAddPt, kDrop, kMultRs };
+
+// One step of a request program. Registers: 0 = input x, 1 = t, 2 = r, 3 = y, 4 = aligned argument.
+struct Step {
+  OpKind kind;
+  uint8_t dst, a, b;   // register indices (b: second operand register, or the plaintext index)
+  uint8_t rot;         // rotation key index (kRotate)
+  bool plain_b;        // b names a plaintext
+};
+
+// inference: t = x * w (+Rescale); 8 x (r = Rot(t, 2^k); t = t + r); Horner degree 7 in t
+std::vector<Step> inference_program() {
+  std::vector<Step> p;
+  p.push_back({kMulPtRs, 1, 0, 0, 0, true});  // plaintext 0 = weights
+  for (int k = 0; k < 8; ++k) {
+    p.push_back({kRotate, 2, 1, 0, (uint8_t)k, false});
+    p.push_back({kAdd, 1, 1, 2, 0, false});
+  }
+  p.push_back({kMulPtRs, 3, 1, 8, 0, true});  // y = t * c7
+  p.push_back({kAddPt, 3, 3, 7, 0, true});    // y += c6
+  for (int i = 5; i >= 0; --i) {
+    p.push_back({kDrop, 4, 1, 3, 0, false});  // align t to y's level
+    p.push_back({kMultRs, 3, 3, 4, 0, false});
+    p.push_back({kAddPt, 3, 3, (uint8_t)(1 + i), 0, true});
+  }
+  return p;
+}
+
+// polynomial: Horner degree 6 in x
+std::vector<Step> polynomial_program() {
+  std::vector<Step> p;
+  p.push_back({kMulPtRs, 3, 0, 7, 0, true});  // y = x * c6
+  p.push_back({kAddPt, 3, 3, 6, 0, true});    // y += c5
+  for (int i = 4; i >= 0; --i) {
+    p.push_back({kDrop, 4, 0, 3, 0, false});
+    p.push_back({kMultRs, 3, 3, 4, 0, false});
+    p.push_back({kAddPt, 3, 3, (uint8_t)(1 + i), 0, true});
+  }
+  return p;
+}
+
+struct Request {
+  int id = 0;
+  bool inference = true;
+  double arrival = 0;
+  size_t pc = 0;
+  int level[5] = {0, 0, 0, 0, 0};
+  DBuf reg[5];
+};
+
+}  // namespace
+
+ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const* rot_keys, const uint32_t* rot_galois,
+                      const uint64_t* relin_key) {
+  if (cfg.requests < 1 || cfg.rate_per_s <= 0) throw Error(-1, "serving needs requests >= 1 and a positive rate");
+  const int L = cfg.start_level;
+  if (L < 9 || L >= c.L1) throw Error(-1, "start level must leave room for the 8 levels a request consumes");
+  for (int k = 0; k < 8; ++k)
+    if (!rot_keys[k]) throw Error(-1, "null rotation key");
+  if (!relin_key) throw Error(-1, "null relinearization key");
+  const size_t N = c.N;
+  const size_t reg_words = 2ull * (L + 1) * N;
+  const hipStream_t st = c.stream;
+
+  // shared plaintexts: 0 = weights, 1..8 = polynomial coefficients c0..c7 (synthetic residues)
+  DBuf pts(c.pool, 9 * (size_t)(L + 1) * N);
+  {
+    std::vector<int> ids(L + 1);
+    for (int i = 0; i <= L; ++i) ids[i] = i;
+    fill_uniform(c, make_set(pts.p, L + 1, ids.data(), 9, (uint64_t)(L + 1) * N), cfg.seed ^ 0x5eedULL, st);
+  }
+  auto pt = [&](int i) { return pts.p + (size_t)i * (L + 1) * N; };
+  // per-operation mode scratch (one multiplication at a time)
+  DBuf tmp(c.pool, reg_words);
+
+  const std::vector<Step> progs[2] = {polynomial_program(), inference_program()};
+
+  // arrivals
+  std::mt19937_64 rng(cfg.seed);
+  std::uniform_real_distribution<double> U(0.0, 1.0);
+  std::vector<double> arrival(cfg.requests);
+  std::vector<bool> kind(cfg.requests);
+  double t = 0;
+  for (int i = 0; i < cfg.requests; ++i) {
+    t += -std::log(1.0 - U(rng)) / cfg.rate_per_s;
+    arrival[i] = t;
+    kind[i] = U(rng) < cfg.inference_share;
+  }
+  const size_t per_request = 5 * reg_words * 8;
+  const int max_active = std::max(1, (int)std::min<double>(1e6, cfg.hbm_budget_gb * 1e9 / (double)per_request));
+
+  ServeResult res;
+  std::vector<double> lat;
+  lat.reserve(cfg.requests);
+  std::deque<std::unique_ptr<Request>> active;
+  Batch batch(c);
+  int next = 0;
+  double ops_total = 0;
+  using clock = std::chrono::steady_clock;
+  FH_HIP(hipStreamSynchronize(st));
+  const auto t0 = clock::now();
+  auto now = [&] { return std::chrono::duration<double>(clock::now() - t0).count(); };
+
+  while (res.completed < cfg.requests) {
+    const double tn = now();
+    while (next < cfg.requests && arrival[next] <= tn && (int)active.size() < max_active) {
+      std::unique_ptr<Request> r(new Request);
+      r->id = next;
+      r->inference = kind[next];
+      r->arrival = arrival[next];
+      for (int k = 0; k < 5; ++k) r->reg[k] = DBuf(c.pool, reg_words);
+      std::vector<int> ids(L + 1);
+      for (int i = 0; i <= L; ++i) ids[i] = i;
+      // the request's input ciphertext (synthetic residues at the start level)
+      fill_uniform(c, make_set(r->reg[0].p, L + 1, ids.data(), 2, (uint64_t)(L + 1) * N), cfg.seed * 1000003ull + next, st);
+      r->level[0] = L;
+      active.push_back(std::move(r));
+      ++next;
+    }
+    res.max_active = std::max(res.max_active, (int)active.size());
+    if (active.empty()) {
+      if (next < cfg.requests) {
+        const double wait = arrival[next] - now();
+        if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+      }
+      continue;
+    }
+    // one tick: the next step of every admitted request
+    if (cfg.mode == 0) batch.reset();
+    for (auto& rp : active) {
+      Request* r = rp.get();
+      const Step& s = progs[r->inference ? 1 : 0][r->pc];
+      const int la = r->level[s.a];
+      uint64_t* a = r->reg[s.a].p;
+      uint64_t* d = r->reg[s.dst].p;
+      const uint64_t* b = s.plain_b ? pt(s.b) : r->reg[s.b].p;
+      const size_t lq = la + 1;
+      int ld = la;
+      switch (s.kind) {
+        case kMulPtRs:
+          ld = la - 1;
+          if (cfg.mode == 0) batch.add_mult_plain(a, b, la, true, d);
+          else {
+            std::vector<int> ids(lq);
+            for (size_t i = 0; i < lq; ++i) ids[i] = (int)i;
+            ew_binary(c, 2, make_set(a, lq, ids.data(), 2, lq * N), make_set(const_cast<uint64_t*>(b), lq, ids.data(), 2, 0),
+                      make_set(tmp.p, lq, ids.data(), 2, lq * N), st);
+            rescale(c, tmp.p, la, 2, d, st);
+          }
+          break;
+        case kRotate:
+          if (cfg.mode == 0) batch.add_rotate(a, la, rot_keys[s.rot], rot_galois[s.rot], d);
+          else eval_rotate(c, a, la, rot_keys[s.rot], rot_galois[s.rot], d, st);
+          break;
+        case kAdd:
+          if (r->level[s.b] != la) throw Error(-6, "serving program adds operands at different levels");
+          if (cfg.mode == 0) batch.add_add(a, b, la, 2, d);
+          else {
+            std::vector<int> ids(2 * lq);
+            for (size_t i = 0; i < 2 * lq; ++i) ids[i] = (int)(i % lq);
+            ew_binary(c, 0, make_set(a, 2 * lq, ids.data()), make_set(const_cast<uint64_t*>(b), 2 * lq, ids.data()),
+                      make_set(d, 2 * lq, ids.data()), st);
+          }
+          break;
+        case kAddPt:
+          if (cfg.mode == 0) batch.add_add_plain(a, b, la, d);
+          else {
+            std::vector<int> ids(lq);
+            for (size_t i = 0; i < lq; ++i) ids[i] = (int)i;
+            ew_binary(c, 0, make_set(a, lq, ids.data()), make_set(const_cast<uint64_t*>(b), lq, ids.data()), make_set(d, lq, ids.data()), st);
+            if (d != a) FH_HIP(hipMemcpyAsync(d + lq * N, a + lq * N, lq * N * 8, hipMemcpyDeviceToDevice, st));
+          }
+          break;
+        case kDrop: {
+          ld = r->level[s.b];  // the level of the register to align with
+          if (cfg.mode == 0) batch.add_level_reduce(a, la, ld, 2, d);
+          else {
+            const size_t lo = ld + 1;
+            for (int comp = 0; comp < 2; ++comp)
+              FH_HIP(hipMemcpyAsync(d + comp * lo * N, a + comp * lq * N, lo * N * 8, hipMemcpyDeviceToDevice, st));
+          }
+          break;
+        }
+        case kMultRs:
+          if (r->level[s.b] != la) throw Error(-6, "serving program multiplies operands at different levels");
+          ld = la - 1;
+          if (cfg.mode == 0) batch.add_mult(a, b, la, relin_key, true, d);
+          else {
+            eval_mult(c, a, b, la, relin_key, tmp.p, st);
+            rescale(c, tmp.p, la, 2, d, st);
+          }
+          break;
+      }
+      r->level[s.dst] = ld;
+    }
+    ops_total += (double)active.size();
+    if (cfg.mode == 0) {
+      batch.finalize();
+      batch.run(st, false);
+    }
+    FH_HIP(hipStreamSynchronize(st));
+    const double td = now();
+    ++res.ticks;
+    for (auto it = active.begin(); it != active.end();) {
+      Request* r = it->get();
+      if (++r->pc == progs[r->inference ? 1 : 0].size()) {
+        lat.push_back(td - r->arrival);
+        if (cfg.checksum) {
+          std::vector<uint64_t> h(2ull * (r->level[3] + 1) * N);
+          FH_HIP(hipMemcpy(h.data(), r->reg[3].p, h.size() * 8, hipMemcpyDeviceToHost));
+          for (uint64_t w : h) res.checksum += w;
+        }
+        ++res.completed;
+        (r->inference ? res.inference : res.polynomial)++;
+        it = active.erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
+  res.wall_s = now();
+  std::sort(lat.begin(), lat.end());
+  auto pct = [&](double p) {
+    const size_t i = std::min(lat.size() - 1, (size_t)std::ceil(p * lat.size()) - (p > 0 ? 1 : 0));
+    return 1000.0 * lat[i];
+  };
+  res.p50_ms = pct(0.50);
+  res.p99_ms = pct(0.99);
+  res.max_ms = 1000.0 * lat.back();
+  double sum = 0;
+  for (double v : lat) sum += v;
+  res.mean_ms = 1000.0 * sum / lat.size();
+  res.requests_per_s = cfg.requests / (res.wall_s - arrival[0]);
+  res.mean_batch_ops = ops_total / std::max(1, res.ticks);
+  return res;
+}
+
+}  // namespace fh

@@ -1,0 +1,47 @@
+// Continuous-batching serving loop over the mixed-level batch engine (fh_serve.cpp).
+//
+// Requests arrive as a seeded Poisson process. Each request is a fixed program of dependent
+// ciphertext operations on its own working registers (device buffers taken from the context pool
+// at admission and returned at completion):
+//   inference ("dot product + degree-7 polynomial"): x * w (plaintext weights) + Rescale, a
+//     rotate-and-sum over 8 doublings (rotations by 1, 2, ..., 128 and additions), then Horner's
+//     rule for a degree-7 polynomial (ciphertext products with relinearization and Rescale,
+//     plaintext coefficient additions, level alignment of the argument);
+//   polynomial ("depth-6 polynomial"): Horner's rule for a degree-6 polynomial of the input.
+// Admission is bounded by an HBM budget (each request's register footprint). Every scheduler
+// tick gathers the next operation of every admitted request into one batch, whatever its kind
+// and level, and executes it as one launch sequence (or, in the per-operation baseline mode, one
+// operation call at a time); requests that arrive during a tick join at the next one.
+// The reference has no request loop; its closest analogue is the concurrent-ops lanes of
+// api/ConcurrentOps.hpp:39-62 and api/CryptoContext.hpp:174-195.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "fh_batch.hpp"
+
+namespace fh {
+
+struct ServeConfig {
+  double rate_per_s = 100.0;   // mean arrival rate
+  int requests = 200;
+  int start_level = 24;        // level of every incoming ciphertext
+  double hbm_budget_gb = 64.0; // admission budget for request registers
+  int mode = 0;                // 0: one fused batch per tick, 1: one call per operation
+  double inference_share = 0.74;
+  uint64_t seed = 1;
+  bool checksum = false;       // sum every request's result words (to compare the two modes)
+};
+
+struct ServeResult {
+  double p50_ms = 0, p99_ms = 0, mean_ms = 0, max_ms = 0;
+  double requests_per_s = 0, wall_s = 0, mean_batch_ops = 0;
+  int ticks = 0, completed = 0, inference = 0, polynomial = 0, max_active = 0;
+  uint64_t checksum = 0;       // sum of all result words mod 2^64 (when requested)
+};
+
+// rot_keys / rot_galois: the 8 keys of rotations by 2^0 .. 2^7.
+ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const* rot_keys, const uint32_t* rot_galois,
+                      const uint64_t* relin_key);
+
+}  // namespace fh

@@ -72,6 +72,19 @@ def _declare(L):
         "fh_fill_uniform": (I, [P, VP, i32p, I, I, S, U]),
         "fh_linear_transform": (I, [P, VP, I, I, I, I, I, I, VP, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), P, C.c_uint32, P,
                                     C.c_uint32, VP]),
+        "fh_batch_create": (I, [P, PP]),
+        "fh_batch_destroy": (I, [P]),
+        "fh_batch_rotate": (I, [P, VP, I, P, C.c_uint32, VP]),
+        "fh_batch_mult": (I, [P, VP, VP, I, P, I, VP]),
+        "fh_batch_add": (I, [P, VP, VP, I, I, VP]),
+        "fh_batch_mult_plain": (I, [P, VP, VP, I, I, VP]),
+        "fh_batch_add_plain": (I, [P, VP, VP, I, VP]),
+        "fh_batch_level_reduce": (I, [P, VP, I, I, I, VP]),
+        "fh_batch_reset": (I, [P]),
+        "fh_batch_finalize": (I, [P]),
+        "fh_batch_run": (I, [P, I]),
+        "fh_batch_info": (I, [P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+        "fh_serve_run": (I, [P, C.POINTER(ServeConfig), C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), P, C.POINTER(ServeResult)]),
         "fh_tune_set": (I, [P, C.c_char_p, I]),
         "fh_tune_get": (I, [P, C.c_char_p, C.POINTER(C.c_int)]),
         "fh_profile_enable": (I, [P, I]),
@@ -82,6 +95,18 @@ def _declare(L):
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("rate_per_s", C.c_double), ("requests", C.c_int), ("start_level", C.c_int), ("hbm_budget_gb", C.c_double),
+                ("mode", C.c_int), ("inference_share", C.c_double), ("seed", C.c_uint64), ("checksum", C.c_int)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
+                ("requests_per_s", C.c_double), ("wall_s", C.c_double), ("mean_batch_ops", C.c_double), ("ticks", C.c_int),
+                ("completed", C.c_int), ("inference", C.c_int), ("polynomial", C.c_int), ("max_active", C.c_int),
+                ("checksum", C.c_uint64)]
 
 
 def _check(rc):
@@ -270,6 +295,17 @@ class Context:
                                           giant_key.h if giant_key is not None else None, giant_galois,
                                           offset_key.h if offset_key is not None else None, offset_galois, out.ptr))
 
+    # ---- serving loop ----
+    def serve(self, rot_keys, rot_galois, relin_key, rate_per_s, requests, start_level=None, mode=0, hbm_budget_gb=64.0,
+              inference_share=0.74, seed=1, checksum=False):
+        cfg = ServeConfig(rate_per_s, requests, self.L1 - 1 if start_level is None else start_level, hbm_budget_gb, mode,
+                          inference_share, seed, int(checksum))
+        kp = (C.c_void_p * 8)(*[k.h for k in rot_keys])
+        gl = (C.c_uint32 * 8)(*[int(g) for g in rot_galois])
+        r = ServeResult()
+        _check(load().fh_serve_run(self.h, C.byref(cfg), kp, gl, relin_key.h, C.byref(r)))
+        return {f: getattr(r, f) for f, _ in ServeResult._fields_}
+
     # ---- launch-shape tunables ----
     def tune_set(self, name, value):
         _check(load().fh_tune_set(self.h, name.encode(), int(value)))
@@ -290,6 +326,55 @@ class Context:
         n, t = C.c_int(), C.c_double()
         _check(load().fh_profile_read(self.h, name.encode(), C.byref(n), C.byref(t)))
         return n.value, t.value
+
+
+class Batch:
+    """A mixed-level batch of EvalRotate / EvalMult(+Rescale) / EvalAdd items (fh_batch_*).
+
+    Items name device addresses (ints, e.g. DeviceArray.addr()); finalize() uploads the job
+    tables, run() issues the whole batch, graph=True captures it once and replays it."""
+
+    def __init__(self, ctx):
+        h = C.c_void_p()
+        _check(load().fh_batch_create(ctx.h, C.byref(h)))
+        self.ctx, self.h = ctx, h.value
+
+    def rotate(self, ct_addr, level, key, galois, out_addr):
+        _check(load().fh_batch_rotate(self.h, ct_addr, level, key.h, galois, out_addr))
+
+    def mult(self, a_addr, b_addr, level, relin_key, rescale, out_addr):
+        _check(load().fh_batch_mult(self.h, a_addr, b_addr, level, relin_key.h, int(rescale), out_addr))
+
+    def add(self, a_addr, b_addr, level, ncomp, out_addr):
+        _check(load().fh_batch_add(self.h, a_addr, b_addr, level, ncomp, out_addr))
+
+    def mult_plain(self, a_addr, pt_addr, level, rescale, out_addr):
+        _check(load().fh_batch_mult_plain(self.h, a_addr, pt_addr, level, int(rescale), out_addr))
+
+    def add_plain(self, a_addr, pt_addr, level, out_addr):
+        _check(load().fh_batch_add_plain(self.h, a_addr, pt_addr, level, out_addr))
+
+    def level_reduce(self, a_addr, level_in, level_out, ncomp, out_addr):
+        _check(load().fh_batch_level_reduce(self.h, a_addr, level_in, level_out, ncomp, out_addr))
+
+    def reset(self):
+        _check(load().fh_batch_reset(self.h))
+
+    def finalize(self):
+        _check(load().fh_batch_finalize(self.h))
+
+    def run(self, graph=False):
+        _check(load().fh_batch_run(self.h, int(graph)))
+
+    def info(self):
+        n, b = C.c_size_t(), C.c_size_t()
+        _check(load().fh_batch_info(self.h, C.byref(n), C.byref(b)))
+        return n.value, b.value
+
+    def free(self):
+        if self.h:
+            _check(load().fh_batch_destroy(self.h))
+            self.h = None
 
 
 class Key:
