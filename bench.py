@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stream-ops", type=int, default=256, help="ciphertexts of the mixed-level op stream (0 = skip)")
     ap.add_argument("--lt-count", type=int, default=32, help="linear transforms of the BSGS workload (0 = skip)")
+    ap.add_argument("--serve-requests", type=int, default=200, help="requests per serving session (0 = skip)")
+    ap.add_argument("--serve-rates", type=str, default="50,150,400", help="arrival rates (requests/s) of the serving sweep")
+    ap.add_argument("--copy-mb", type=int, default=256, help="device copy of this size first (counter calibration; 0 = skip)")
     return ap.parse_args()
 
 
@@ -74,6 +77,21 @@ def main():
     lq, ext, beta = lvl + 1, lvl + 1 + K, ctx.num_digits(lvl)
     ct_words = 2 * lq * N
     rng = np.random.default_rng(a.seed)
+
+    # ---- a plain device copy of known size: the reference point that scales hardware byte counters ----
+    copy_gbs = None
+    if a.copy_mb > 0:
+        words = a.copy_mb * (1 << 17)
+        src, dst = ctx.array((words,)), ctx.array((words,))
+        lib = fh.load()
+        lib.fh_copy_d2d(ctx.h, dst.ptr, src.ptr, words * 8)
+        ctx.sync()
+        t0 = time.perf_counter()
+        lib.fh_copy_d2d(ctx.h, dst.ptr, src.ptr, words * 8)
+        ctx.sync()
+        copy_gbs = round(2 * words * 8 / (time.perf_counter() - t0) / 1e9, 1)
+        src.free()
+        dst.free()
 
     # ---- synthetic inputs, resident in HBM ----
     qids = list(range(lq))
@@ -204,6 +222,7 @@ def main():
     # ---- mixed-level op stream: per-op vs one batch (eager and graph-replayed) ----
     stream = uneven_stream(fh, ctx, dev_keys, gal, a) if a.stream_ops > 0 else None
     lt = lt_workload(fh, ctx, a) if a.lt_count > 0 else None
+    serving = serving_sweep(ctx, dev_keys[:8], gal[:8], dev_keys[0], a) if a.serve_requests > 0 else None
 
     # ---- CPU baseline: the oracle on a bounded sample of the same rotation ----
     cpu = None
@@ -235,8 +254,10 @@ def main():
         "ntt_ms_fwd_inv": round(1000 * t_ntt, 3),
         "ntt_avg_launch_ms": ntt_kernel_ms,
         "eval_mult_rescale_per_s": round(mult_rescale_s, 1),
+        "device_copy_gbps": copy_gbs,
         "uneven_stream": stream,
         "linear_transform": lt,
+        "serving": serving,
     }
     print(json.dumps(line))
     ctx.close()
@@ -364,6 +385,25 @@ def lt_workload(fh, ctx, a):
     return {"transforms": a.lt_count, "level": lvl, "diagonals": row, "b_step": bstep, "g_step": row // bstep,
             "transforms_per_s": round(a.lt_count / dt, 2), "ms_per_transform": round(1000 * dt / a.lt_count, 3),
             "algorithmic_gb_per_transform": round(alg / 1e9, 3)}
+
+
+def serving_sweep(ctx, rot_keys, rot_gal, relin, a):
+    """Continuous-batching serving sessions (fh_serve_run) at N=2^16, requests arriving at level 24:
+    74% inference (x*w + Rescale, rotate-and-sum over 8 doublings, degree-7 polynomial), 26%
+    depth-6 polynomial; p50/p99 latency and requests/s per arrival rate, batched, plus the
+    one-call-per-operation baseline at the middle rate. Keys are synthetic residues."""
+    assert STEPS16[:8] == [1 << k for k in range(8)]
+    rates = [float(r) for r in a.serve_rates.split(",") if r]
+    out = {"requests": a.serve_requests, "mix": "74% inference (dot product + degree-7 polynomial), 26% depth-6 polynomial",
+           "bootstrap_share": "not served (bootstrapping not built)", "sessions": []}
+    keep = ("p50_ms", "p99_ms", "mean_ms", "requests_per_s", "mean_batch_ops", "max_active", "ticks")
+    for i, rate in enumerate(rates):
+        r = ctx.serve(rot_keys, rot_gal, relin, rate, a.serve_requests, mode=0, seed=a.seed + i)
+        out["sessions"].append(dict({"rate_per_s": rate, "mode": "batched"}, **{k: round(r[k], 3) for k in keep}))
+    mid = rates[len(rates) // 2]
+    r = ctx.serve(rot_keys, rot_gal, relin, mid, a.serve_requests, mode=1, seed=a.seed + len(rates) // 2)
+    out["sessions"].append(dict({"rate_per_s": mid, "mode": "per_op"}, **{k: round(r[k], 3) for k in keep}))
+    return out
 
 
 def _view(fh, ctx, arr, word_offset, shape):

@@ -352,6 +352,23 @@ __global__ __launch_bounds__(kT) void k_rescale_expand_jobs(const RsJob* __restr
   }
 }
 
+// Reorder NTT job lists prime-major (stable), permuting a companion list the same way:
+// consecutive workgroups then share one prime's twiddle table in L2.
+template <class T>
+void sort_by_prime(std::vector<LimbJob>& jobs, std::vector<T>* comp) {
+  std::vector<size_t> idx(jobs.size());
+  for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return jobs[a].pid < jobs[b].pid; });
+  std::vector<LimbJob> j2(jobs.size());
+  for (size_t i = 0; i < idx.size(); ++i) j2[i] = jobs[idx[i]];
+  jobs.swap(j2);
+  if (comp) {
+    std::vector<T> c2(comp->size());
+    for (size_t i = 0; i < idx.size(); ++i) c2[i] = (*comp)[idx[i]];
+    comp->swap(c2);
+  }
+}
+
 // byte-addressed table builder: sections aligned to 64 bytes, uploaded in one copy
 struct TableBuilder {
   std::vector<uint8_t> bytes;
@@ -749,6 +766,13 @@ void Batch::finalize() {
       }
     }
   }
+
+  sort_by_prime(ks_in, &ks_out);
+  sort_by_prime<LimbJob>(modup_ntt, nullptr);
+  sort_by_prime(md_in, &md_out);
+  sort_by_prime(md_ntt, &md_epi);
+  sort_by_prime(rs_in, &rs_out);
+  sort_by_prime(rs_ntt, &rs_epi);
 
   TableBuilder tb;
   const size_t o_mulpt = tb.put(pj);

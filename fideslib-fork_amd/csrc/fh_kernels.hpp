@@ -67,26 +67,37 @@ struct JobAddr {
   }
 };
 
-// Target limbs of a ModUp output (beta digits, each over Q_l ∪ P, ext = l+1+K limbs):
-// job y enumerates, digit by digit, every extended position outside the digit's own primes.
+// Target limbs of a ModUp output (npolys polynomials, each beta digits over Q_l ∪ P, ext = l+1+K
+// limbs per digit). Jobs run prime-major: every (poly, digit) target of extended position 0, then
+// position 1, ..., so consecutive workgroups share one prime's twiddle table in L2. A Q position
+// is a target of every digit but its own (beta - 1 jobs per poly), a P position of all beta.
 struct ModUpAddr {
   uint64_t* base;
   uint32_t ext, lq, alpha, L1, beta;
-  uint32_t per;           // target limbs per polynomial (count()); jobs of poly b follow poly b-1
+  uint32_t npolys;
   uint64_t poly_stride;   // words between consecutive polynomials' digit blocks
   __device__ __forceinline__ void at(uint32_t y, uint32_t N, uint64_t*& p, int& pid) const {
-    const uint32_t b = y / per;
-    y -= b * per;
-    const uint32_t full = ext - alpha;
-    uint32_t d = y / full, t = y - d * full;
-    if (d >= beta - 1) { d = beta - 1; t = y - (beta - 1) * full; }
-    const uint32_t start = d * alpha;
-    const uint32_t sz = alpha < lq - start ? alpha : lq - start;
-    const uint32_t pos = t < start ? t : t + sz;
-    p = base + b * poly_stride + ((size_t)d * ext + pos) * N;
+    const uint32_t qjobs = npolys * (beta - 1);  // jobs per Q position
+    const uint32_t y1 = qjobs * lq;
+    uint32_t pos, r, d;
+    if (y < y1) {
+      pos = y / qjobs;
+      r = y - pos * qjobs;
+      const uint32_t b = r / (beta - 1), dd = r - b * (beta - 1), own = pos / alpha;
+      d = dd < own ? dd : dd + 1;
+      p = base + b * poly_stride + ((size_t)d * ext + pos) * N;
+    } else {
+      const uint32_t pj = npolys * beta;           // jobs per P position
+      const uint32_t t = (y - y1) / pj;
+      pos = lq + t;
+      r = (y - y1) - t * pj;
+      const uint32_t b = r / beta;
+      d = r - b * beta;
+      p = base + b * poly_stride + ((size_t)d * ext + pos) * N;
+    }
     pid = (int)(pos < lq ? pos : L1 + (pos - lq));
   }
-  __host__ uint32_t count() const { return beta * ext - lq; }
+  __host__ uint32_t count() const { return npolys * (beta * ext - lq); }
 };
 
 struct NttTables {

@@ -472,9 +472,8 @@ void modup_batch(Context& c, const uint64_t* x, int l, int npolys, uint64_t x_st
     else FH_SZ_DISPATCH(c.alpha, k_modup_conv_r, MacWide, (g2, kT, 0, st), (tmp.p, x, digits, mp))
   }
   FH_HIP(hipGetLastError());
-  ModUpAddr a{digits, ext, lq, (uint32_t)c.alpha, (uint32_t)c.L1, beta, 0, (uint64_t)beta * ext * c.N};
-  a.per = a.count();
-  ntt_fwd_modup(c, a, a.per * npolys, st);
+  ModUpAddr a{digits, ext, lq, (uint32_t)c.alpha, (uint32_t)c.L1, beta, (uint32_t)npolys, (uint64_t)beta * ext * c.N};
+  ntt_fwd_modup(c, a, a.count(), st);
 }
 
 void ksk_inner(Context& c, const uint64_t* digits, int l, const uint64_t* key, uint64_t* out0, uint64_t* out1,
