@@ -344,9 +344,10 @@ def uneven_stream(fh, ctx, rot_keys, rot_gal, a):
 
 def lt_workload(fh, ctx, a):
     """Hoisted BSGS plaintext-matrix x ciphertext product: level 20 (21 limbs), 64 nonzero
-    diagonals resident over Q_20 ∪ P (960 MiB), bStep = gStep = 8, stride 1; a batch of lt_count
-    ciphertexts against the one resident diagonal set. Synthetic residues for keys, diagonals and
-    ciphertexts (the LT arithmetic is data-independent)."""
+    diagonals resident over Q_20 ∪ P (960 MiB), bStep = gStep = 8, stride 1; lt_count ciphertexts
+    against the one resident diagonal set, one call per ciphertext and as one batched call
+    (bit-exact comparison). Synthetic residues for keys, diagonals and ciphertexts (the LT
+    arithmetic is data-independent)."""
     N, lvl, row, bstep = ctx.N, 20, 64, 8
     lq, ext = lvl + 1, lvl + 1 + K
     full = list(range(L1 + K))
@@ -367,9 +368,11 @@ def lt_workload(fh, ctx, a):
     gk, gg = keys[bstep], ctx.galois_index(bstep)
 
     def one(i):
-        ctx.linear_transform(_view(fh, ctx, cts, i * 2 * lq * N, (2 * lq, N)), lvl, row, bstep, 1, diags, bk, bg, out,
-                             giant_key=gk, giant_galois=gg)
+        ctx.linear_transform(_view(fh, ctx, cts, i * 2 * lq * N, (2 * lq, N)), lvl, row, bstep, 1, diags, bk, bg,
+                             _view(fh, ctx, outs, i * 2 * lq * N, (2 * lq, N)), giant_key=gk, giant_galois=gg)
 
+    outs = ctx.array((a.lt_count * 2 * lq, N))
+    outb = ctx.array((a.lt_count * 2 * lq, N))
     one(0)
     ctx.sync()
     t0 = time.perf_counter()
@@ -377,14 +380,31 @@ def lt_workload(fh, ctx, a):
         one(i)
     ctx.sync()
     dt = time.perf_counter() - t0
-    for x in [diags, cts, out] + list(keys.values()):
+
+    def batched():
+        ctx.linear_transform_batch(cts, a.lt_count, lvl, row, bstep, 1, diags, bk, bg, outb, giant_key=gk, giant_galois=gg)
+
+    batched()
+    ctx.sync()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batched()
+    ctx.sync()
+    dtb = (time.perf_counter() - t0) / reps
+    same = bool(np.array_equal(outs.download(), outb.download()))
+    for x in [diags, cts, out, outs, outb] + list(keys.values()):
         x.free()
     # algorithmic bytes per transform: 15 key switches' keys, 64 diagonals, ciphertext in/out
     beta = ctx.num_digits(lvl)
     alg = 8 * N * (15 * 2 * beta * ext + row * ext + 4 * lq)
+    # batched: keys and diagonals read once for the whole batch
+    alg_b = 8 * N * (15 * 2 * beta * ext + row * ext + a.lt_count * 4 * lq)
     return {"transforms": a.lt_count, "level": lvl, "diagonals": row, "b_step": bstep, "g_step": row // bstep,
-            "transforms_per_s": round(a.lt_count / dt, 2), "ms_per_transform": round(1000 * dt / a.lt_count, 3),
-            "algorithmic_gb_per_transform": round(alg / 1e9, 3)}
+            "per_call_transforms_per_s": round(a.lt_count / dt, 2), "per_call_ms_per_transform": round(1000 * dt / a.lt_count, 3),
+            "batched_transforms_per_s": round(a.lt_count / dtb, 2), "batched_ms_per_transform": round(1000 * dtb / a.lt_count, 3),
+            "batched_over_per_call": round(dt / dtb, 3), "bit_exact_batched_vs_per_call": same,
+            "algorithmic_gb_per_transform": round(alg / 1e9, 3), "algorithmic_gb_per_batch": round(alg_b / 1e9, 3)}
 
 
 def serving_sweep(ctx, rot_keys, rot_gal, relin, a):

@@ -122,6 +122,15 @@ int fh_context_destroy(fh_context* ctx) {
   });
 }
 
+int fh_context_set_reduced_noise(fh_context* ctx, int on) {
+  return guard([&] {
+    auto& c = C(ctx);
+    if (on && !c.rn_conv_ok) throw fh::Error(FH_E_UNSUPPORTED, "ReducedNoise conversions need (alpha + 1) * max prime <= 2^64");
+    FH_HIP(hipStreamSynchronize(c.stream));  // work already queued keeps the mode it was issued with
+    c.reduced_noise = on != 0;
+  });
+}
+
 int fh_context_shape(const fh_context* ctx, int* out5) {
   return guard([&] {
     const auto& c = C(ctx);
@@ -437,6 +446,33 @@ int fh_linear_transform(fh_context* ctx, const uint64_t* ct, int level, int row_
     const uint64_t* ok = offset_key ? key_of(c, offset_key) : nullptr;
     fh::linear_transform(c, ct, level, row_size, b_step, stride, offset, slots, diags, bk.data(), bg.data(), gk, giant_galois, ok,
                          offset_galois, out, c.stream);
+  });
+}
+
+int fh_linear_transform_batch(fh_context* ctx, int count, const uint64_t* cts, size_t ct_stride, int level, int row_size, int b_step,
+                              int stride, int offset, int slots, const uint64_t* diags, const fh_key* const* baby_keys,
+                              const uint32_t* baby_galois, const fh_key* giant_key, uint32_t giant_galois, const fh_key* offset_key,
+                              uint32_t offset_galois, uint64_t* out) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(cts, "cts"); need(diags, "diags"); need(out, "out");
+    check_level(c, level);
+    if (count < 0) throw fh::Error(FH_E_INVALID, "count must be >= 0");
+    if (b_step < 1 || b_step > 16) throw fh::Error(FH_E_INVALID, "b_step must be in [1, 16]");
+    if (slots < 1 || slots > (int)c.N / 2) throw fh::Error(FH_E_INVALID, "slots out of range");
+    std::vector<const uint64_t*> bk(b_step, nullptr);
+    std::vector<uint32_t> bg(b_step, 1);
+    for (int i = 0; i < b_step; ++i) {
+      if (((long long)i * stride) % slots == 0) continue;
+      need(baby_keys, "baby_keys");
+      need(baby_galois, "baby_galois");
+      bk[i] = key_of(c, baby_keys[i]);
+      bg[i] = baby_galois[i];
+    }
+    const uint64_t* gk = giant_key ? key_of(c, giant_key) : nullptr;
+    const uint64_t* ok = offset_key ? key_of(c, offset_key) : nullptr;
+    fh::linear_transform_batch(c, count, cts, ct_stride, level, row_size, b_step, stride, offset, slots, diags, bk.data(), bg.data(), gk,
+                               giant_galois, ok, offset_galois, out, c.stream);
   });
 }
 

@@ -100,6 +100,8 @@ struct ConvConst {
   const uint64_t* phinv;     // [K]
   const uint64_t* phinv_sh;
   const uint64_t* phat_m;    // [K][L1], Montgomery form
+  const uint64_t* negq_m;    // ReducedNoise corrections (null = off), as in fh_keyswitch.hip
+  const uint64_t* negp_m;
 };
 
 // k_modup_conv_r of fh_keyswitch.hip with the level and digit read from the job
@@ -118,10 +120,13 @@ __global__ __launch_bounds__(kT) void k_modup_conv_jobs(const ModUpJob* __restri
     const uint32_t si = i / ext, pos = i % ext;
     s_qhat[si * ext + pos] = si < sz ? M::prep(cc.qhat_m[(tb + si) * E + pos]) : M::zero_op();
   }
+  __shared__ Op s_negq[kMaxPrimes];
+  const bool rn = cc.negq_m != nullptr;
   for (uint32_t pos = threadIdx.x; pos < ext; pos += blockDim.x) {
     const PrimeConst& P = cc.prime[pos < lq ? pos : cc.L1 + (pos - lq)];
     s_pq[pos] = P.q;
     s_pn[pos] = P.mont_ninv;
+    if (rn) s_negq[pos] = M::prep(cc.negq_m[((size_t)j.l * cc.dnum + j.d) * E + pos]);
   }
   for (uint32_t i = threadIdx.x; i < sz; i += blockDim.x) {
     inv[i] = cc.qhinv[tb + i];
@@ -131,19 +136,24 @@ __global__ __launch_bounds__(kT) void k_modup_conv_jobs(const ModUpJob* __restri
   __syncthreads();
   for (uint32_t n = 2 * (blockIdx.x * blockDim.x + threadIdx.x); n < N; n += 2 * gridDim.x * blockDim.x) {
     Op y0[SZ], y1[SZ];
+    uint32_t lift0 = 0, lift1 = 0;
 #pragma unroll
     for (int i = 0; i < SZ; ++i) {
       if (i < (int)sz) {
         const size_t o = (size_t)(start + i) * N + n;
         const ulonglong2 c = *reinterpret_cast<const ulonglong2*>(j.coef + o);
-        y0[i] = M::prep(shoup(c.x, inv[i], invsh[i], qs[i]));
-        y1[i] = M::prep(shoup(c.y, inv[i], invsh[i], qs[i]));
+        const uint64_t v0 = shoup(c.x, inv[i], invsh[i], qs[i]), v1 = shoup(c.y, inv[i], invsh[i], qs[i]);
+        lift0 += v0 > (qs[i] >> 1);
+        lift1 += v1 > (qs[i] >> 1);
+        y0[i] = M::prep(v0);
+        y1[i] = M::prep(v1);
         *reinterpret_cast<ulonglong2*>(j.out + o) = *reinterpret_cast<const ulonglong2*>(j.eval + o);
       } else {
         y0[i] = M::zero_op();
         y1[i] = M::zero_op();
       }
     }
+    const Op l0 = M::prep(lift0), l1 = M::prep(lift1);
     for (uint32_t pos = 0; pos < ext; ++pos) {
       if (pos == start) {
         pos += sz - 1;
@@ -157,6 +167,10 @@ __global__ __launch_bounds__(kT) void k_modup_conv_jobs(const ModUpJob* __restri
         const Op c = s_qhat[i * ext + pos];
         a0.mac(y0[i], c);
         a1.mac(y1[i], c);
+      }
+      if (rn) {
+        a0.mac(l0, s_negq[pos]);
+        a1.mac(l1, s_negq[pos]);
       }
       const uint64_t pq = s_pq[pos], pn = s_pn[pos];
       uint64_t h0, l0, h1, l1;
@@ -285,9 +299,12 @@ __global__ __launch_bounds__(kT) void k_moddown_conv_jobs(const MdJob* __restric
     const uint32_t k = i / lq, t = i % lq;
     s_phat[k * lq + t] = k < K ? M::prep(cc.phat_m[(size_t)k * cc.L1 + t]) : M::zero_op();
   }
+  __shared__ Op s_negp[kMaxPrimes];
+  const bool rn = cc.negp_m != nullptr;
   for (uint32_t i = threadIdx.x; i < lq; i += blockDim.x) {
     s_q[i] = cc.prime[i].q;
     s_n[i] = cc.prime[i].mont_ninv;
+    if (rn) s_negp[i] = M::prep(cc.negp_m[i]);
   }
   for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) {
     inv[k] = cc.phinv[k];
@@ -297,17 +314,22 @@ __global__ __launch_bounds__(kT) void k_moddown_conv_jobs(const MdJob* __restric
   __syncthreads();
   for (uint32_t n = 2 * (blockIdx.x * blockDim.x + threadIdx.x); n < N; n += 2 * gridDim.x * blockDim.x) {
     Op y0[KK], y1[KK];
+    uint32_t lift0 = 0, lift1 = 0;
 #pragma unroll
     for (int k = 0; k < KK; ++k) {
       if (k < (int)K) {
         const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(j.pc + (size_t)k * N + n);
-        y0[k] = M::prep(shoup(v.x, inv[k], invsh[k], ps[k]));
-        y1[k] = M::prep(shoup(v.y, inv[k], invsh[k], ps[k]));
+        const uint64_t v0 = shoup(v.x, inv[k], invsh[k], ps[k]), v1 = shoup(v.y, inv[k], invsh[k], ps[k]);
+        lift0 += v0 > (ps[k] >> 1);
+        lift1 += v1 > (ps[k] >> 1);
+        y0[k] = M::prep(v0);
+        y1[k] = M::prep(v1);
       } else {
         y0[k] = M::zero_op();
         y1[k] = M::zero_op();
       }
     }
+    const Op l0 = M::prep(lift0), l1 = M::prep(lift1);
     for (uint32_t i = 0; i < lq; ++i) {
       M a0, a1;
       a0.zero();
@@ -317,6 +339,10 @@ __global__ __launch_bounds__(kT) void k_moddown_conv_jobs(const MdJob* __restric
         const Op c = s_phat[k * lq + i];
         a0.mac(y0[k], c);
         a1.mac(y1[k], c);
+      }
+      if (rn) {
+        a0.mac(l0, s_negp[i]);
+        a1.mac(l1, s_negp[i]);
       }
       const uint64_t q = s_q[i], nv = s_n[i];
       uint64_t h0, l0, h1, l1;
@@ -563,6 +589,7 @@ void Batch::finalize() {
   if (!P.rots.empty() || !P.mults.empty()) {
     if (K < 1) throw Error(-1, "key switching needs at least one special prime");
     if (!c.redc_conv_ok || c.alpha > 16 || K > 16) throw Error(-3, "batched key switching needs alpha, K <= 16");
+    if (c.reduced_noise && !c.rn_conv_ok) throw Error(-3, "ReducedNoise conversions need (alpha + 1) * max prime <= 2^64");
     if (c.dnum > kMaxDnumB) throw Error(-3, "dnum above kernel limit");
   }
 
@@ -720,18 +747,21 @@ void Batch::finalize() {
       }
     }
   }
-  // groups of key-switch items sharing (key, galois)
+  // groups of key-switch items sharing (key, galois), levels descending inside a group. A group is
+  // cut into chunks of at most kGroupChunk items (each chunk reads the key once), so no workgroup
+  // runs far longer than the others, and the chunks are launched largest first.
+  constexpr uint32_t kGroupChunk = 16;
   std::vector<size_t> order(ks.size());
   for (size_t i = 0; i < ks.size(); ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
-    return std::make_tuple(ks[a].key, ks[a].galois) < std::make_tuple(ks[b].key, ks[b].galois);
+    return std::make_tuple(ks[a].key, ks[a].galois, -ks[a].l) < std::make_tuple(ks[b].key, ks[b].galois, -ks[b].l);
   });
   std::vector<KskItem> ki;
   std::vector<KskGroup> kg;
   for (size_t t = 0; t < order.size();) {
     const Ks& k0 = ks[order[t]];
     KskGroup g{k0.key, k0.galois == 1 ? 1u : (uint32_t)h_inv_odd(k0.galois, 2ull * c.N), (uint32_t)ki.size(), 0, 0};
-    while (t < order.size() && ks[order[t]].key == k0.key && ks[order[t]].galois == k0.galois) {
+    while (t < order.size() && ks[order[t]].key == k0.key && ks[order[t]].galois == k0.galois && g.count < kGroupChunk) {
       const Ks& k = ks[order[t]];
       ki.push_back(KskItem{A + k.dig, k.c0, A + k.ext, (uint32_t)k.l, 0});
       g.max_lq = std::max<uint32_t>(g.max_lq, k.l + 1);
@@ -740,6 +770,7 @@ void Batch::finalize() {
     }
     kg.push_back(g);
   }
+  std::stable_sort(kg.begin(), kg.end(), [](const KskGroup& a, const KskGroup& b) { return a.count * a.max_lq > b.count * b.max_lq; });
   // rescale sources: relinearized products and plaintext products that asked for a Rescale
   struct RsSrc {
     uint32_t l;
@@ -850,11 +881,12 @@ void launch_all(Context& c, BatchPlan& P, hipStream_t st) {
   if (P.n_ks_intt) {
     ntt_inv_jobs(c, P.t_ks_in, P.t_ks_out, P.n_ks_intt, st);
     ConvConst cc{N, (uint32_t)c.L1, (uint32_t)c.K, (uint32_t)c.alpha, (uint32_t)c.dnum, c.d_prime, c.d_modup_qhinv, c.d_modup_qhinv_sh,
-                 c.d_modup_qhat_m, c.d_md_phinv, c.d_md_phinv_sh, c.d_md_phat_m};
+                 c.d_modup_qhat_m, c.d_md_phinv, c.d_md_phinv_sh, c.d_md_phat_m, c.reduced_noise ? c.d_modup_negq_m : nullptr,
+                 c.reduced_noise ? c.d_md_negp_m : nullptr};
     {
       ProfScope ps_(c, "modup_conv", st);
       const dim3 g = grid_tiles(N, P.n_modup, 2);
-      if (c.split_mac_ok) FH_SZ_DISPATCH(c.alpha, k_modup_conv_jobs, Mac30, (g, kT, 0, st), (P.t_modup, cc))
+      if (c.mac30_ok()) FH_SZ_DISPATCH(c.alpha, k_modup_conv_jobs, Mac30, (g, kT, 0, st), (P.t_modup, cc))
       else FH_SZ_DISPATCH(c.alpha, k_modup_conv_jobs, MacWide, (g, kT, 0, st), (P.t_modup, cc))
     }
     FH_HIP(hipGetLastError());
@@ -871,7 +903,7 @@ void launch_all(Context& c, BatchPlan& P, hipStream_t st) {
     {
       ProfScope ps_(c, "moddown_conv", st);
       const dim3 g = grid_tiles(N, P.n_md, 2);
-      if (c.split_mac_ok) FH_SZ_DISPATCH(c.K, k_moddown_conv_jobs, Mac30, (g, kT, 0, st), (P.t_md, cc))
+      if (c.mac30_ok()) FH_SZ_DISPATCH(c.K, k_moddown_conv_jobs, Mac30, (g, kT, 0, st), (P.t_md, cc))
       else FH_SZ_DISPATCH(c.K, k_moddown_conv_jobs, MacWide, (g, kT, 0, st), (P.t_md, cc))
     }
     FH_HIP(hipGetLastError());

@@ -296,6 +296,39 @@ Context::Context(int logN_, int L1_, int K_, int dnum_, const std::vector<uint64
   d_itw_sh = upload(itwsh);
   d_twd = upload(twd);
   d_itwd = upload(itwd);
+  {
+    // row twists (see the header): tau_r = psi^e_r, e_r = 1 + 2 brev_l1(r) - S1 (mod 2N)
+    ntt_l1 = logN / 2;
+    const uint32_t S1 = 1u << ntt_l1, S2 = N >> ntt_l1;
+    std::vector<uint64_t> ts((size_t)np * N), tssh(ts.size()), its(ts.size()), itssh(ts.size());
+    std::vector<double> tsd(ts.size()), itsd(ts.size());
+    for (int i = 0; i < np; ++i) {
+      const uint64_t qi = q[i];
+      for (uint32_t r = 0; r < S1; ++r) {
+        const int64_t e = (1 + 2 * (int64_t)bitrev(r, ntt_l1) - (int64_t)S1) % (int64_t)m;
+        const uint64_t eu = (uint64_t)(e < 0 ? e + (int64_t)m : e);
+        const uint64_t step = h_powmod(psi[i], eu, qi), istep = h_powmod(psi[i], (m - eu) % m, qi);
+        uint64_t x = 1, xi = 1;
+        for (uint32_t c = 0; c < S2; ++c) {
+          const size_t o = (size_t)i * N + (size_t)r * S2 + c;
+          ts[o] = x;
+          its[o] = xi;
+          tssh[o] = h_shoup(x, qi);
+          itssh[o] = h_shoup(xi, qi);
+          tsd[o] = x > qi / 2 ? (double)x - (double)qi : (double)x;
+          itsd[o] = xi > qi / 2 ? (double)xi - (double)qi : (double)xi;
+          x = h_mulmod(x, step, qi);
+          xi = h_mulmod(xi, istep, qi);
+        }
+      }
+    }
+    d_twist = upload(ts);
+    d_twist_sh = upload(tssh);
+    d_itwist = upload(its);
+    d_itwist_sh = upload(itssh);
+    d_twistd = upload(tsd);
+    d_itwistd = upload(itsd);
+  }
 
   // ---- ModUp tables (OpenFHE PartQlHatInvModq / PartQlHatModp) ----
   const int E = L1 + K;
@@ -369,6 +402,34 @@ Context::Context(int logN_, int L1_, int K_, int dnum_, const std::vector<uint64
   const u128 lim = (u128)1 << 64;
   redc_conv_ok = ((u128)alpha << qbits) <= lim && ((u128)std::max(K, 1) << qbits) <= lim;
   split_mac_ok = qbits <= 60 && alpha <= 16 && K <= 16 && dnum <= 16;
+  rn_conv_ok = ((u128)(alpha + 1) << qbits) <= lim && ((u128)(std::max(K, 1) + 1) << qbits) <= lim;
+  {
+    // ReducedNoise corrections in Montgomery form: [-Q_d 2^64]_{p_j} per (level, digit, target
+    // position) and [-P 2^64]_{q_i}
+    const int E = L1 + K;
+    std::vector<uint64_t> negq((size_t)L1 * dnum * E, 0), negp(L1, 0);
+    for (int l = 0; l < L1; ++l) {
+      const int lq = l + 1, ext = lq + K;
+      for (int d = 0; d < dnum; ++d) {
+        const int start = d * alpha, end = std::min(start + alpha, lq);
+        if (start >= end) continue;
+        for (int pos = 0; pos < ext; ++pos) {
+          if (pos >= start && pos < end) continue;
+          const uint64_t p = q[ext_prime(l, pos)];
+          uint64_t qd = 1;
+          for (int j = start; j < end; ++j) qd = h_mulmod(qd, q[j] % p, p);
+          const uint64_t neg = qd ? p - qd : 0;
+          negq[((size_t)l * dnum + d) * E + pos] = (uint64_t)(((u128)neg << 64) % p);
+        }
+      }
+    }
+    for (int i = 0; i < L1; ++i) {
+      const uint64_t neg = h_pmod[i] ? q[i] - h_pmod[i] : 0;
+      negp[i] = (uint64_t)(((u128)neg << 64) % q[i]);
+    }
+    d_modup_negq_m = upload(negq);
+    d_md_negp_m = upload(negp);
+  }
   d_md_pinv = upload(h_pinv);
   d_md_pinv_sh = upload(pinv_sh);
   d_pmod = upload(h_pmod);

@@ -133,6 +133,16 @@ struct Context {
   uint64_t* d_itw_sh = nullptr;
   double* d_twd = nullptr;         // [np][N] twiddles as doubles (fp64 NTT path, q < 2^51)
   double* d_itwd = nullptr;
+  // Row twists of the two-pass NTT (N = S1 x S2, S1 = 2^ntt_l1): after the column stages, row r
+  // is finished by a standard S2-point negacyclic transform of x_c tau_r^c, with
+  // tau_r = psi^(1 + 2 brev_l1(r) - S1); twist[r S2 + c] = tau_r^c, itwist its inverse.
+  int ntt_l1 = 0;
+  uint64_t* d_twist = nullptr;     // [np][N]
+  uint64_t* d_twist_sh = nullptr;
+  uint64_t* d_itwist = nullptr;
+  uint64_t* d_itwist_sh = nullptr;
+  double* d_twistd = nullptr;      // centred doubles (fp64 path)
+  double* d_itwistd = nullptr;
 
   // ModUp: for level l and digit d, source prescale [Qhat_i^-1]_{q_i} (+Shoup), and
   // [Qhat_i]_{p_j} for every extended position j of level l.
@@ -143,6 +153,13 @@ struct Context {
   uint64_t* d_md_phat_m = nullptr;       // [K][L1] Montgomery form [P^_k 2^64]_{q_i}
   bool redc_conv_ok = false;             // conversion sums fit a single Montgomery reduction
   bool split_mac_ok = false;             // every prime < 2^60 and alpha, K <= 16: 30-bit split products
+  // ReducedNoise basis conversions (the reference engine's reducedNoise option): prescaled
+  // residues above q/2 read as v - q, i.e. the sums gain -count * Q_src. Off by default.
+  bool reduced_noise = false;
+  bool rn_conv_ok = false;               // one more term still fits a single Montgomery reduction
+  uint64_t* d_modup_negq_m = nullptr;    // [L1][dnum][L1+K] Montgomery form of [-Q_d]_{p_j}
+  uint64_t* d_md_negp_m = nullptr;       // [L1] Montgomery form of [-P]_{q_i}
+  bool mac30_ok() const { return split_mac_ok && (!reduced_noise || (alpha < 16 && K < 16)); }
   // ModDown
   uint64_t* d_md_phinv = nullptr;        // [K]     [P^_k^-1]_{p_k}
   uint64_t* d_md_phinv_sh = nullptr;     // [K]

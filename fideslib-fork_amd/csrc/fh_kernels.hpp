@@ -108,12 +108,19 @@ struct NttTables {
   const uint64_t* itw_sh;
   const double* twd;   // centred twiddles as doubles (fp64 path)
   const double* itwd;
+  const uint64_t* twist;     // row twists [np][N] and their inverses (Shoup companions, doubles)
+  const uint64_t* twist_sh;
+  const uint64_t* itwist;
+  const uint64_t* itwist_sh;
+  const double* twistd;
+  const double* itwistd;
   uint32_t N;
   uint32_t logN;
 };
 
 inline NttTables ntt_tables(const Context& c) {
-  return NttTables{c.d_prime, c.d_tw, c.d_tw_sh, c.d_itw, c.d_itw_sh, c.d_twd, c.d_itwd, c.N, (uint32_t)c.logN};
+  return NttTables{c.d_prime, c.d_tw, c.d_tw_sh, c.d_itw, c.d_itw_sh, c.d_twd, c.d_itwd, c.d_twist, c.d_twist_sh,
+                   c.d_itwist, c.d_itwist_sh, c.d_twistd, c.d_itwistd, c.N, (uint32_t)c.logN};
 }
 
 inline LimbSet make_set(uint64_t* base, uint32_t nlimbs, const int* pids, uint32_t npolys = 1, uint64_t poly_stride = 0) {

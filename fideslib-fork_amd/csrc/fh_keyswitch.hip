@@ -30,6 +30,7 @@ struct ModUpParams {
   const uint64_t* qhinv_sh;
   const uint64_t* qhat;      // [L1][dnum][alpha][L1+K]
   const uint64_t* qhat_m;    // Montgomery form of qhat
+  const uint64_t* negq_m;    // ReducedNoise: [L1][dnum][L1+K] Montgomery form of [-Q_d]_{p_j}; null = off
 };
 
 // blockIdx.y = digit. in_coef: INTT of the input (lq limbs), in_eval: the input itself.
@@ -101,10 +102,13 @@ __global__ __launch_bounds__(kT) void k_modup_conv_r(const uint64_t* __restrict_
     const uint32_t si = i / mp.ext, pos = i % mp.ext;
     s_qhat[si * mp.ext + pos] = si < sz ? M::prep(mp.qhat_m[(tb + si) * E + pos]) : M::zero_op();
   }
+  __shared__ Op s_negq[kMaxPrimes];
+  const bool rn = mp.negq_m != nullptr;
   for (uint32_t pos = threadIdx.x; pos < mp.ext; pos += blockDim.x) {
     const PrimeConst& P = mp.prime[pos < mp.lq ? pos : mp.L1 + (pos - mp.lq)];
     s_pq[pos] = P.q;
     s_pn[pos] = P.mont_ninv;
+    if (rn) s_negq[pos] = M::prep(mp.negq_m[((size_t)mp.level * mp.dnum + d) * E + pos]);
   }
   // per-source constants live in LDS: held as uniform scalars they exhaust the SGPR file
   __shared__ uint64_t inv[SZ], invsh[SZ], qs[SZ];
@@ -117,19 +121,24 @@ __global__ __launch_bounds__(kT) void k_modup_conv_r(const uint64_t* __restrict_
   uint64_t* od = out + (size_t)d * mp.ext * mp.N;
   for (uint32_t n = 2 * (blockIdx.x * blockDim.x + threadIdx.x); n < mp.N; n += 2 * gridDim.x * blockDim.x) {
     Op y0[SZ], y1[SZ];
+    uint32_t lift0 = 0, lift1 = 0;  // ReducedNoise: sources above q_i/2 (read as v - q_i)
 #pragma unroll
     for (int i = 0; i < SZ; ++i) {
       if (i < (int)sz) {
         const size_t o = (size_t)(start + i) * mp.N + n;
         const ulonglong2 c = *reinterpret_cast<const ulonglong2*>(in_coef + o);
-        y0[i] = M::prep(shoup(c.x, inv[i], invsh[i], qs[i]));
-        y1[i] = M::prep(shoup(c.y, inv[i], invsh[i], qs[i]));
+        const uint64_t v0 = shoup(c.x, inv[i], invsh[i], qs[i]), v1 = shoup(c.y, inv[i], invsh[i], qs[i]);
+        lift0 += v0 > (qs[i] >> 1);
+        lift1 += v1 > (qs[i] >> 1);
+        y0[i] = M::prep(v0);
+        y1[i] = M::prep(v1);
         *reinterpret_cast<ulonglong2*>(od + o) = *reinterpret_cast<const ulonglong2*>(in_eval + o);
       } else {
         y0[i] = M::zero_op();
         y1[i] = M::zero_op();
       }
     }
+    const Op l0 = M::prep(lift0), l1 = M::prep(lift1);
     for (uint32_t pos = 0; pos < mp.ext; ++pos) {
       if (pos == start) {
         pos += sz - 1;
@@ -143,6 +152,10 @@ __global__ __launch_bounds__(kT) void k_modup_conv_r(const uint64_t* __restrict_
         const Op c = s_qhat[i * mp.ext + pos];
         a0.mac(y0[i], c);
         a1.mac(y1[i], c);
+      }
+      if (rn) {
+        a0.mac(l0, s_negq[pos]);
+        a1.mac(l1, s_negq[pos]);
       }
       const uint64_t pq = s_pq[pos], pn = s_pn[pos];
       uint64_t h0, l0, h1, l1;
@@ -221,10 +234,12 @@ struct RotGroups {
 
 struct KskBatchParams {
   uint32_t N, lq, ext, L1, K, beta, logN;
-  uint64_t ct_stride;       // words between input ciphertexts (c0 at offset 0)
-  uint64_t dig_stride;      // words between digit blocks of consecutive ciphertexts
+  uint64_t ct_stride;       // words between the c0 sources of consecutive source ciphertexts
+  uint64_t dig_stride;      // words between digit blocks of consecutive source ciphertexts
   uint64_t out_stride;      // words between output ext ciphertexts (2 * ext limbs)
-  int add_c0;
+  uint32_t per_src;         // items per source: item b reads the digits and c0 of source b / per_src
+  int add_c0;               // fold c0: P-scaled over Q_l (c0_ext = 0) or as given over Q_l ∪ P (c0_ext = 1)
+  int c0_ext;
   const PrimeConst* prime;
   const uint64_t* cts;
   const uint64_t* pmod;
@@ -242,7 +257,8 @@ __global__ __launch_bounds__(kT) void k_ksk_batch(const uint64_t* __restrict__ d
   const size_t N = kp.N;
   const uint64_t* key = g.key[grp];
   const uint32_t kinv = g.gal_inv[grp];
-  const bool addc0 = kp.add_c0 && pos < kp.lq;
+  const bool addc0 = kp.add_c0 && !kp.c0_ext && pos < kp.lq;
+  const bool addext = kp.add_c0 && kp.c0_ext;
   const uint64_t pm = addc0 ? kp.pmod[pos] : 0, pmsh = addc0 ? kp.pmod_sh[pos] : 0;
   // two consecutive coefficients per lane: 16-byte loads of keys, digits and c0
   for (uint32_t n = 2 * (blockIdx.x * blockDim.x + threadIdx.x); n < kp.N; n += 2 * gridDim.x * blockDim.x) {
@@ -264,7 +280,8 @@ __global__ __launch_bounds__(kT) void k_ksk_batch(const uint64_t* __restrict__ d
 #pragma unroll 2
     for (uint32_t t = 0; t < cnt; ++t) {
       const uint32_t b = g.order[first + t];
-      const uint64_t* dg = digits + b * kp.dig_stride + (size_t)pos * N + n;
+      const uint32_t src = b / kp.per_src;
+      const uint64_t* dg = digits + src * kp.dig_stride + (size_t)pos * N + n;
       M a0, a1, b0, b1;
       a0.zero();
       a1.zero();
@@ -292,9 +309,13 @@ __global__ __launch_bounds__(kT) void k_ksk_batch(const uint64_t* __restrict__ d
       b1.get(h, l);
       uint64_t s1 = redc(h, l, P);
       if (addc0) {
-        const ulonglong2 c = *reinterpret_cast<const ulonglong2*>(kp.cts + b * kp.ct_stride + (size_t)pos * N + n);
+        const ulonglong2 c = *reinterpret_cast<const ulonglong2*>(kp.cts + src * kp.ct_stride + (size_t)pos * N + n);
         r0 = add_mod(r0, shoup(c.x, pm, pmsh, P.q), P.q);
         s0 = add_mod(s0, shoup(c.y, pm, pmsh, P.q), P.q);
+      } else if (addext) {
+        const ulonglong2 c = *reinterpret_cast<const ulonglong2*>(kp.cts + src * kp.ct_stride + (size_t)pos * N + n);
+        r0 = add_mod(r0, c.x, P.q);
+        s0 = add_mod(s0, c.y, P.q);
       }
       uint64_t* o = out + b * kp.out_stride;
       o[(size_t)pos * N + j0] = r0;
@@ -315,6 +336,7 @@ struct ModDownParams {
   const uint64_t* phat_m;    // Montgomery form of phat
   const uint64_t* pinv;      // [L1]
   const uint64_t* pinv_sh;
+  const uint64_t* negp_m;    // ReducedNoise: [L1] Montgomery form of [-P]_{q_i}; null = off
 };
 
 // blockIdx.y = poly. pcoef: INTT of the K special limbs of each poly (K limbs per poly);
@@ -358,9 +380,12 @@ __global__ __launch_bounds__(kT) void k_moddown_conv_r(const uint64_t* __restric
     uint32_t k = i / mp.lq, t = i % mp.lq;
     s_phat[k * mp.lq + t] = k < mp.K ? M::prep(mp.phat_m[(size_t)k * mp.L1 + t]) : M::zero_op();
   }
+  __shared__ Op s_negp[kMaxPrimes];
+  const bool rn = mp.negp_m != nullptr;
   for (uint32_t i = threadIdx.x; i < mp.lq; i += blockDim.x) {
     s_q[i] = mp.prime[i].q;
     s_n[i] = mp.prime[i].mont_ninv;
+    if (rn) s_negp[i] = M::prep(mp.negp_m[i]);
   }
   __shared__ uint64_t inv[KK], invsh[KK], ps[KK];
   for (uint32_t k = threadIdx.x; k < mp.K; k += blockDim.x) {
@@ -373,17 +398,22 @@ __global__ __launch_bounds__(kT) void k_moddown_conv_r(const uint64_t* __restric
   uint64_t* cv = conv + (size_t)blockIdx.y * mp.lq * mp.N;
   for (uint32_t n = 2 * (blockIdx.x * blockDim.x + threadIdx.x); n < mp.N; n += 2 * gridDim.x * blockDim.x) {
     Op y0[KK], y1[KK];
+    uint32_t lift0 = 0, lift1 = 0;  // ReducedNoise: sources above p_k/2 (read as v - p_k)
 #pragma unroll
     for (int k = 0; k < KK; ++k) {
       if (k < (int)mp.K) {
         const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(pc + (size_t)k * mp.N + n);
-        y0[k] = M::prep(shoup(v.x, inv[k], invsh[k], ps[k]));
-        y1[k] = M::prep(shoup(v.y, inv[k], invsh[k], ps[k]));
+        const uint64_t v0 = shoup(v.x, inv[k], invsh[k], ps[k]), v1 = shoup(v.y, inv[k], invsh[k], ps[k]);
+        lift0 += v0 > (ps[k] >> 1);
+        lift1 += v1 > (ps[k] >> 1);
+        y0[k] = M::prep(v0);
+        y1[k] = M::prep(v1);
       } else {
         y0[k] = M::zero_op();
         y1[k] = M::zero_op();
       }
     }
+    const Op l0 = M::prep(lift0), l1 = M::prep(lift1);
     for (uint32_t i = 0; i < mp.lq; ++i) {
       M a0, a1;
       a0.zero();
@@ -393,6 +423,10 @@ __global__ __launch_bounds__(kT) void k_moddown_conv_r(const uint64_t* __restric
         const Op c = s_phat[k * mp.lq + i];
         a0.mac(y0[k], c);
         a1.mac(y1[k], c);
+      }
+      if (rn) {
+        a0.mac(l0, s_negp[i]);
+        a1.mac(l1, s_negp[i]);
       }
       const uint64_t q = s_q[i], nv = s_n[i];
       uint64_t h0, l0, h1, l1;
@@ -463,12 +497,14 @@ void modup_batch(Context& c, const uint64_t* x, int l, int npolys, uint64_t x_st
   ntt_inv(c, make_set(const_cast<uint64_t*>(x), lq, ids.data(), npolys, x_stride),
           make_set(tmp.p, lq, ids.data(), npolys, (uint64_t)lq * c.N), st);
   ModUpParams mp{c.N, lq, ext, (uint32_t)c.alpha, (uint32_t)c.L1, (uint32_t)c.K, beta, (uint32_t)l, (uint32_t)c.dnum,
-                 x_stride, c.d_prime, c.d_modup_qhinv, c.d_modup_qhinv_sh, c.d_modup_qhat, c.d_modup_qhat_m};
+                 x_stride, c.d_prime, c.d_modup_qhinv, c.d_modup_qhinv_sh, c.d_modup_qhat, c.d_modup_qhat_m,
+                 c.reduced_noise ? c.d_modup_negq_m : nullptr};
+  if (c.reduced_noise && (!c.rn_conv_ok || c.alpha > 16)) throw Error(-3, "ReducedNoise conversions need (alpha + 1) * max prime <= 2^64");
   {
     ProfScope ps_(c, "modup_conv", st);
     const dim3 g2 = grid_n(c.N, beta * npolys, 2);
     if (!c.redc_conv_ok || c.alpha > 16) k_modup_conv<<<grid_n(c.N, beta * npolys), kT, 0, st>>>(tmp.p, x, digits, mp);
-    else if (c.split_mac_ok) FH_SZ_DISPATCH(c.alpha, k_modup_conv_r, Mac30, (g2, kT, 0, st), (tmp.p, x, digits, mp))
+    else if (c.mac30_ok()) FH_SZ_DISPATCH(c.alpha, k_modup_conv_r, Mac30, (g2, kT, 0, st), (tmp.p, x, digits, mp))
     else FH_SZ_DISPATCH(c.alpha, k_modup_conv_r, MacWide, (g2, kT, 0, st), (tmp.p, x, digits, mp))
   }
   FH_HIP(hipGetLastError());
@@ -521,12 +557,13 @@ void moddown(Context& c, const uint64_t* x, int l, uint64_t* out, int npolys, ui
   ntt_inv(c, make_set(const_cast<uint64_t*>(x) + (size_t)lq * c.N, K, pids.data(), npolys, x_stride),
           make_set(pc.p, K, pids.data(), npolys, (uint64_t)K * c.N), st);
   ModDownParams mp{c.N, lq, ext, (uint32_t)c.L1, K, c.d_prime, c.d_md_phinv, c.d_md_phinv_sh, c.d_md_phat, c.d_md_phat_m,
-                   c.d_md_pinv, c.d_md_pinv_sh};
+                   c.d_md_pinv, c.d_md_pinv_sh, c.reduced_noise ? c.d_md_negp_m : nullptr};
+  if (c.reduced_noise && (!c.rn_conv_ok || K > 16)) throw Error(-3, "ReducedNoise conversions need (K + 1) * max prime <= 2^64");
   {
     ProfScope ps_(c, "moddown_conv", st);
     const dim3 g2 = grid_n(c.N, npolys, 2);
     if (!c.redc_conv_ok || K > 16) k_moddown_conv<<<grid_n(c.N, npolys), kT, 0, st>>>(pc.p, conv.p, mp);
-    else if (c.split_mac_ok) FH_SZ_DISPATCH((int)K, k_moddown_conv_r, Mac30, (g2, kT, 0, st), (pc.p, conv.p, mp))
+    else if (c.mac30_ok()) FH_SZ_DISPATCH((int)K, k_moddown_conv_r, Mac30, (g2, kT, 0, st), (pc.p, conv.p, mp))
     else FH_SZ_DISPATCH((int)K, k_moddown_conv_r, MacWide, (g2, kT, 0, st), (pc.p, conv.p, mp))
   }
   FH_HIP(hipGetLastError());
@@ -601,34 +638,31 @@ void eval_rotate(Context& c, const uint64_t* ct, int l, const uint64_t* key, uin
   moddown(c, e.p, l, out2, 2, 0, 0, st);
 }
 
-namespace {
-
-// Key inner products (+ P*c0, automorphism) for B ciphertexts whose digits are already ModUp'ed,
-// grouped by shared (key, galois); then optional ModDown of the 2B extended results.
-void rotate_core(Context& c, int nb, const uint64_t* digits, uint64_t dig_stride, const uint64_t* xs, uint64_t ct_stride, int l,
-                 const uint64_t* const* keys, const uint32_t* galois, bool add_c0, bool do_moddown, uint64_t* out, hipStream_t st) {
+// Key inner products of nb items at level l, grouped by shared (key, galois) so each key limb is
+// read once per group. Item b reads the digits (and c0) of source b / per_src and writes its
+// extended result (2 x (l+1+K) limbs) at out + b * out_stride, through the automorphism.
+// add_c0 folds c0: P c0 over Q_l (c0_ext false) or an extended c0 added on every limb.
+void ksk_items(Context& c, int nb, int per_src, const uint64_t* digits, uint64_t dig_stride, const uint64_t* c0s, uint64_t c0_stride,
+               bool add_c0, bool c0_ext, int l, const uint64_t* const* keys, const uint32_t* galois, uint64_t* eo, uint64_t out_stride,
+               hipStream_t st) {
   const uint32_t lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);
-  DBuf ext_buf;
-  uint64_t* eo;
-  if (do_moddown) {
-    ext_buf = DBuf(c.pool, (size_t)nb * 2 * ext * c.N);
-    eo = ext_buf.p;
-  } else {
-    eo = out;
-  }
+  if (beta > (uint32_t)kMaxDnum) throw Error(-3, "dnum above kernel limit");
+  if (per_src < 1) throw Error(-1, "per_src must be >= 1");
+  if (nb > 65535) throw Error(-3, "too many key-switch items in one call");
   std::vector<int> idx(nb);
   for (int i = 0; i < nb; ++i) idx[i] = i;
   std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
     if (keys[a] != keys[b]) return keys[a] < keys[b];
     return galois[a] < galois[b];
   });
-  KskBatchParams kp{c.N, lq, ext, (uint32_t)c.L1, (uint32_t)c.K, beta, (uint32_t)c.logN, ct_stride,
-                    dig_stride, 2ull * ext * c.N, add_c0 ? 1 : 0, c.d_prime, xs, c.d_pmod, c.d_pmod_sh};
+  KskBatchParams kp{c.N, lq, ext, (uint32_t)c.L1, (uint32_t)c.K, beta, (uint32_t)c.logN, c0_stride, dig_stride, out_stride,
+                    (uint32_t)per_src, add_c0 ? 1 : 0, c0_ext ? 1 : 0, c.d_prime, c0s, c.d_pmod, c.d_pmod_sh};
   int i = 0;
   while (i < nb) {
     RotGroups g{};
     int ng = 0, base = i;
-    while (i < nb && ng < kMaxGroups) {
+    // one launch takes at most kMaxGroups groups and kMaxBatch items (a group may continue in the next)
+    while (i < nb && ng < kMaxGroups && i - base < kMaxBatch) {
       const uint64_t* k = keys[idx[i]];
       uint32_t gl = galois[idx[i]];
       if (!k) throw Error(-1, "null key");
@@ -637,7 +671,7 @@ void rotate_core(Context& c, int nb, const uint64_t* digits, uint64_t dig_stride
       g.gal_inv[ng] = gl == 1 ? 1 : (uint32_t)h_inv_odd(gl, 2ull * c.N);
       g.first[ng] = (uint16_t)(i - base);
       int cnt = 0;
-      while (i < nb && keys[idx[i]] == k && galois[idx[i]] == gl) {
+      while (i < nb && i - base < kMaxBatch && keys[idx[i]] == k && galois[idx[i]] == gl) {
         g.order[i - base] = (uint16_t)idx[i];
         ++i;
         ++cnt;
@@ -651,6 +685,24 @@ void rotate_core(Context& c, int nb, const uint64_t* digits, uint64_t dig_stride
     else k_ksk_batch<MacWide><<<gk, kT, 0, st>>>(digits, eo, kp, g);
     FH_HIP(hipGetLastError());
   }
+}
+
+namespace {
+
+// Key inner products (+ P*c0, automorphism) for B ciphertexts whose digits are already ModUp'ed,
+// grouped by shared (key, galois); then optional ModDown of the 2B extended results.
+void rotate_core(Context& c, int nb, const uint64_t* digits, uint64_t dig_stride, const uint64_t* xs, uint64_t ct_stride, int l,
+                 const uint64_t* const* keys, const uint32_t* galois, bool add_c0, bool do_moddown, uint64_t* out, hipStream_t st) {
+  const uint32_t lq = l + 1, ext = lq + c.K;
+  DBuf ext_buf;
+  uint64_t* eo;
+  if (do_moddown) {
+    ext_buf = DBuf(c.pool, (size_t)nb * 2 * ext * c.N);
+    eo = ext_buf.p;
+  } else {
+    eo = out;
+  }
+  ksk_items(c, nb, 1, digits, dig_stride, xs, ct_stride, add_c0, false, l, keys, galois, eo, 2ull * ext * c.N, st);
   if (do_moddown) moddown(c, eo, l, out, 2 * nb, 0, 0, st);
 }
 

@@ -26,6 +26,19 @@ void ksk_inner(Context& c, const uint64_t* digits, int l, const uint64_t* key, u
 void linear_transform(Context& c, const uint64_t* ct, int l, int rowSize, int bStep, int stride, int offset, int slots,
                       const uint64_t* diags, const uint64_t* const* baby_keys, const uint32_t* baby_gal, const uint64_t* giant_key,
                       uint32_t giant_gal, const uint64_t* off_key, uint32_t off_gal, uint64_t* out, hipStream_t st);
+// Key inner products of nb items grouped by (key, galois): item b uses the digits (dig_stride
+// apart) and c0 (c0_stride apart) of source b / per_src and writes 2 x (l+1+K) limbs at
+// eo + b * out_stride. add_c0 folds c0 as P c0 over Q_l, or (c0_ext) as given over Q_l ∪ P.
+void ksk_items(Context& c, int nb, int per_src, const uint64_t* digits, uint64_t dig_stride, const uint64_t* c0s, uint64_t c0_stride,
+               bool add_c0, bool c0_ext, int l, const uint64_t* const* keys, const uint32_t* galois, uint64_t* eo, uint64_t out_stride,
+               hipStream_t st);
+// count linear transforms of ciphertexts at level l (ct_stride words apart, 0 = packed) sharing one
+// diagonal set and key set; outputs packed (2 x (l+1) limbs each). Keys and diagonals are read once
+// per batch instead of once per ciphertext.
+void linear_transform_batch(Context& c, int count, const uint64_t* cts, uint64_t ct_stride, int l, int rowSize, int bStep, int stride,
+                            int offset, int slots, const uint64_t* diags, const uint64_t* const* baby_keys, const uint32_t* baby_gal,
+                            const uint64_t* giant_key, uint32_t giant_gal, const uint64_t* off_key, uint32_t off_gal, uint64_t* out,
+                            hipStream_t st);
 // npolys polynomials over Q_l ∪ P (x_stride words apart, 0 = packed) -> over Q_l (out_stride apart)
 void moddown(Context& c, const uint64_t* x, int l, uint64_t* out, int npolys, uint64_t x_stride, uint64_t out_stride, hipStream_t st);
 // ncomp components over Q_l -> over Q_{l-1}

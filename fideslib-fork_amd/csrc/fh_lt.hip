@@ -77,6 +77,76 @@ __global__ __launch_bounds__(kT) void k_lt_mac(const uint64_t* __restrict__ fast
 
 inline dim3 grid_n(uint32_t N, uint32_t y) { return dim3(std::max(1u, std::min((N + kT - 1) / kT, 256u)), y); }
 
+// k_extend_p for B ciphertexts: blockIdx.y = (b, comp, pos)
+__global__ __launch_bounds__(kT) void k_extend_p_b(const uint64_t* __restrict__ x, uint64_t x_stride, uint64_t* __restrict__ out,
+                                                    uint64_t out_stride, uint32_t N, uint32_t lq, uint32_t ext, const PrimeConst* prime,
+                                                    const uint64_t* pmod, const uint64_t* pmod_sh) {
+  const uint32_t b = blockIdx.y / (2 * ext), r = blockIdx.y % (2 * ext);
+  const uint32_t comp = r / ext, pos = r % ext;
+  uint64_t* o = out + b * out_stride + ((size_t)comp * ext + pos) * N;
+  if (pos >= lq) {
+    for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) o[n] = 0;
+    return;
+  }
+  const uint64_t q = prime[pos].q, w = pmod[pos], wsh = pmod_sh[pos];
+  const uint64_t* xi = x + b * x_stride + ((size_t)comp * lq + pos) * N;
+  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) o[n] = shoup(xi[n], w, wsh, q);
+}
+
+constexpr int kMacTile = 64;   // coefficients per workgroup of the batched MAC
+constexpr int kMacLanes = kT / kMacTile;
+
+struct MacBatchParams {
+  uint32_t N, lq, ext, L1, rowSize, bStep, gStep, B, nrot;
+  const PrimeConst* prime;
+  int8_t slot[kMaxBaby];   // baby step i: rotation slot (>= 0) or the P-scaled identity (-1)
+};
+
+// Giant-step partial sums for B ciphertexts sharing one diagonal set. blockIdx.y = extended
+// position, blockIdx.x = a 64-coefficient tile: the tile of every diagonal is staged in LDS once
+// and applied to all B ciphertexts (four ciphertexts in flight per workgroup), so the diagonals
+// are read once per batch. R[b][j][comp][pos] = sum_i fast_b[i][comp][pos] (.) diag[bStep j + i][pos].
+template <int BB>
+__global__ __launch_bounds__(kT) void k_lt_mac_b(const uint64_t* __restrict__ rot, const uint64_t* __restrict__ ident,
+                                                  const uint64_t* __restrict__ diags, uint64_t* __restrict__ R, MacBatchParams mp) {
+  extern __shared__ uint64_t sd[];   // rowSize x kMacTile
+  const uint32_t pos = blockIdx.y;
+  const PrimeConst P = mp.prime[pos < mp.lq ? pos : mp.L1 + (pos - mp.lq)];
+  const size_t N = mp.N, E = (size_t)mp.ext * N;
+  const uint32_t lane = threadIdx.x % kMacTile, cl = threadIdx.x / kMacTile;
+  const uint32_t n = blockIdx.x * kMacTile + lane;
+  for (uint32_t k = cl; k < mp.rowSize; k += kMacLanes) sd[k * kMacTile + lane] = diags[k * E + (size_t)pos * N + n];
+  __syncthreads();
+  for (uint32_t b = cl; b < mp.B; b += kMacLanes) {
+    uint64_t f0[BB], f1[BB];
+#pragma unroll
+    for (int i = 0; i < BB; ++i) {
+      if (i < (int)mp.bStep) {
+        const uint64_t* src = mp.slot[i] >= 0 ? rot + ((size_t)b * mp.nrot + mp.slot[i]) * 2 * E : ident + (size_t)b * 2 * E;
+        f0[i] = src[(size_t)pos * N + n];
+        f1[i] = src[E + (size_t)pos * N + n];
+      }
+    }
+    uint64_t* Rb = R + (size_t)b * mp.gStep * 2 * E;
+    for (uint32_t j = 0; j < mp.gStep; ++j) {
+      U128 a0, a1;
+      a0.zero();
+      a1.zero();
+#pragma unroll
+      for (int i = 0; i < BB; ++i) {
+        const uint32_t k = j * mp.bStep + i;
+        if (i < (int)mp.bStep && k < mp.rowSize) {
+          const uint64_t d = sd[k * kMacTile + lane];
+          a0.mac(f0[i], d);
+          a1.mac(f1[i], d);
+        }
+      }
+      Rb[((size_t)j * 2 + 0) * E + (size_t)pos * N + n] = mod128(a0.hi, a0.lo, P);
+      Rb[((size_t)j * 2 + 1) * E + (size_t)pos * N + n] = mod128(a1.hi, a1.lo, P);
+    }
+  }
+}
+
 }  // namespace
 
 void linear_transform(Context& c, const uint64_t* ct, int l, int rowSize, int bStep, int stride, int offset, int slots,
@@ -161,6 +231,116 @@ void linear_transform(Context& c, const uint64_t* ct, int l, int rowSize, int bS
       moddown(c, t.p, l, out, 2, 0, 0, st);
     } else {
       moddown(c, Rj, l, out, 2, 0, 0, st);
+    }
+  }
+}
+
+// The same transform for count ciphertexts sharing the diagonal set and keys: the baby-step
+// rotations of all ciphertexts form one key-switch batch (each baby key read once for all of
+// them), one multiply-accumulate launch reads the diagonals once for the whole batch, and every
+// giant step key-switches all ciphertexts with one read of the giant key. Results are bit-identical
+// to count calls of linear_transform.
+void linear_transform_batch(Context& c, int count, const uint64_t* cts, uint64_t ct_stride, int l, int rowSize, int bStep, int stride,
+                            int offset, int slots, const uint64_t* diags, const uint64_t* const* baby_keys, const uint32_t* baby_gal,
+                            const uint64_t* giant_key, uint32_t giant_gal, const uint64_t* off_key, uint32_t off_gal, uint64_t* out,
+                            hipStream_t st) {
+  if (count < 1) return;
+  if (bStep < 1 || bStep > kMaxBaby) throw Error(-1, "bStep must be in [1, 16]");
+  if (rowSize < 1) throw Error(-1, "rowSize must be >= 1");
+  if ((size_t)rowSize * kMacTile * 8 > 64 * 1024) throw Error(-3, "batched linear transform takes at most 128 diagonals");
+  if (slots < 1 || slots > (int)c.N / 2) throw Error(-1, "slots out of range");
+  const uint32_t lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);
+  const size_t N = c.N, E = (size_t)ext * N;
+  if (!ct_stride) ct_stride = 2ull * lq * N;
+  const int gStep = (rowSize + bStep - 1) / bStep;
+  const bool giant = ((long long)bStep * stride) % slots != 0;
+  if (gStep > 1 && giant && !giant_key) throw Error(-1, "giant-step key required");
+  if (offset != 0 && !off_key) throw Error(-1, "offset key required");
+  const uint32_t B = (uint32_t)count;
+
+  // ---- baby steps ----
+  MacBatchParams mp{c.N, lq, ext, (uint32_t)c.L1, (uint32_t)rowSize, (uint32_t)bStep, (uint32_t)gStep, B, 0, c.d_prime, {}};
+  std::vector<int> rot_idx;
+  bool has_ident = false;
+  for (int i = 0; i < bStep; ++i) {
+    if (((long long)i * stride) % slots == 0) {
+      mp.slot[i] = -1;
+      has_ident = true;
+    } else {
+      if (!baby_keys[i]) throw Error(-1, "baby-step key missing");
+      mp.slot[i] = (int8_t)rot_idx.size();
+      rot_idx.push_back(i);
+    }
+  }
+  const uint32_t nrot = (uint32_t)rot_idx.size();
+  mp.nrot = nrot;
+  DBuf ident, rot, dig(c.pool, (size_t)B * beta * E);
+  if (has_ident) {
+    ident = DBuf(c.pool, (size_t)B * 2 * E);
+    ProfScope ps_(c, "lt_extend", st);
+    k_extend_p_b<<<grid_n(c.N, B * 2 * ext), kT, 0, st>>>(cts, ct_stride, ident.p, 2 * E, c.N, lq, ext, c.d_prime, c.d_pmod, c.d_pmod_sh);
+    FH_HIP(hipGetLastError());
+  }
+  if (nrot) {
+    rot = DBuf(c.pool, (size_t)B * nrot * 2 * E);
+    modup_batch(c, cts + (size_t)lq * N, l, (int)B, ct_stride, dig.p, st);
+    std::vector<const uint64_t*> ks((size_t)B * nrot);
+    std::vector<uint32_t> gs((size_t)B * nrot);
+    for (uint32_t b = 0; b < B; ++b)
+      for (uint32_t r = 0; r < nrot; ++r) {
+        ks[(size_t)b * nrot + r] = baby_keys[rot_idx[r]];
+        gs[(size_t)b * nrot + r] = baby_gal[rot_idx[r]];
+      }
+    ksk_items(c, (int)(B * nrot), (int)nrot, dig.p, (uint64_t)beta * E, cts, ct_stride, true, false, l, ks.data(), gs.data(), rot.p, 2 * E,
+              st);
+  }
+
+  // ---- giant-step partial sums: diagonals read once for the batch ----
+  DBuf R(c.pool, (size_t)B * gStep * 2 * E);
+  {
+    ProfScope ps_(c, "lt_mac", st);
+    const dim3 g(c.N / kMacTile, ext);
+    const size_t lds = (size_t)rowSize * kMacTile * 8;
+    if (bStep <= 4) k_lt_mac_b<4><<<g, kT, lds, st>>>(rot.p, ident.p, diags, R.p, mp);
+    else if (bStep <= 8) k_lt_mac_b<8><<<g, kT, lds, st>>>(rot.p, ident.p, diags, R.p, mp);
+    else k_lt_mac_b<16><<<g, kT, lds, st>>>(rot.p, ident.p, diags, R.p, mp);
+    FH_HIP(hipGetLastError());
+  }
+
+  // ---- Horner over the giant steps, all ciphertexts together ----
+  std::vector<int> epids(ext);
+  for (uint32_t p = 0; p < ext; ++p) epids[p] = c.ext_prime(l, p);
+  const uint64_t rstride = (uint64_t)gStep * 2 * E;  // between ciphertexts in R
+  DBuf acc(c.pool, (size_t)B * 2 * E), c1q(c.pool, (size_t)B * lq * N);
+  std::vector<const uint64_t*> gk(B, giant_key), okv(B, off_key);
+  std::vector<uint32_t> gg(B, giant_gal), og(B, off_gal);
+  bool have_acc = false;
+  for (int j = gStep - 1; j >= 0; --j) {
+    uint64_t* Rj = R.p + (size_t)j * 2 * E;
+    if (have_acc) {
+      for (int comp = 0; comp < 2; ++comp) {
+        LimbSet a = make_set(Rj + comp * E, ext, epids.data(), B, rstride), b = make_set(acc.p + comp * E, ext, epids.data(), B, 2 * E);
+        ew_binary(c, 0, a, b, a, st);
+      }
+    }
+    if (j > 0) {
+      if (giant) {
+        moddown(c, Rj + E, l, c1q.p, (int)B, rstride, (uint64_t)lq * N, st);
+        modup_batch(c, c1q.p, l, (int)B, (uint64_t)lq * N, dig.p, st);
+        ksk_items(c, (int)B, 1, dig.p, (uint64_t)beta * E, Rj, rstride, true, true, l, gk.data(), gg.data(), acc.p, 2 * E, st);
+      } else {
+        FH_HIP(hipMemcpy2DAsync(acc.p, 2 * E * 8, Rj, rstride * 8, 2 * E * 8, B, hipMemcpyDeviceToDevice, st));
+      }
+      have_acc = true;
+    } else if (offset != 0) {
+      moddown(c, Rj + E, l, c1q.p, (int)B, rstride, (uint64_t)lq * N, st);
+      modup_batch(c, c1q.p, l, (int)B, (uint64_t)lq * N, dig.p, st);
+      DBuf t(c.pool, (size_t)B * 2 * E);
+      ksk_items(c, (int)B, 1, dig.p, (uint64_t)beta * E, Rj, rstride, true, true, l, okv.data(), og.data(), t.p, 2 * E, st);
+      moddown(c, t.p, l, out, (int)(2 * B), E, (uint64_t)lq * N, st);
+    } else {
+      moddown(c, Rj, l, out, (int)B, rstride, 2ull * lq * N, st);
+      moddown(c, Rj + E, l, out + (size_t)lq * N, (int)B, rstride, 2ull * lq * N, st);
     }
   }
 }

@@ -10,7 +10,10 @@
 //  * N = S1 * S2 (S1 = 2^l1 "column" points, S2 = 2^l2 "row" points). The first
 //    l1 stages only couple elements of one column (stride S2), the last l2
 //    stages only elements of one row (contiguous), so a limb is transformed by
-//    two launches. Each lane owns 16 elements of one sub-transform: four radix-2
+//    two launches. Row rho's stages equal a standard S2-point negacyclic
+//    transform of x_c tau_rho^c (tau_rho = psi^(1 + 2 brev(rho) - S1)), so the row
+//    launch multiplies by a per-element twist table once and then uses the
+//    same S2 twiddles (kept in LDS) for every row, like the column launch. Each lane owns 16 elements of one sub-transform: four radix-2
 //    stages run in registers, one LDS exchange re-partitions the sub-transform,
 //    the remaining stages run in registers again. Global loads and stores are
 //    coalesced in both passes (rows are staged through LDS where the register
@@ -67,6 +70,8 @@ struct IntA {
     Y = shoup_lazy(v, w.w, w.sh, q);           // [0, 2q)
   }
   __device__ __forceinline__ void recenter(T&) const {}
+  // x * w for any x < 2^64, result in [0, 2q)
+  __device__ __forceinline__ T mul(T x, W w) const { return shoup_lazy(x, w.w, w.sh, q); }
   __device__ __forceinline__ uint64_t canon(T x) const {
     x = x >= q2 ? x - q2 : x;
     return x >= q ? x - q : x;
@@ -104,6 +109,12 @@ struct FpA {
     Y = mulmod(v, w);
   }
   __device__ __forceinline__ void recenter(T& x) const { x = fma(-rint(x * qinv), q, x); }
+  // x * w re-centred into [-q/2, q/2]
+  __device__ __forceinline__ T mul(T x, W w) const {
+    T r = mulmod(x, w);
+    recenter(r);
+    return r;
+  }
   __device__ __forceinline__ uint64_t canon(T x) const {
     recenter(x);
     x = x < 0 ? x + q : x;
@@ -119,6 +130,8 @@ struct Sel {
   FpA fa;
   const uint64_t *tw, *twsh;
   const double* twd;
+  const uint64_t *ts, *tssh;   // row twists of the limb's prime (forward: tau^c, inverse: tau^-c)
+  const double* tsd;
 };
 
 __device__ __forceinline__ Sel select(const NttTables& tb, int pid, bool inverse) {
@@ -131,6 +144,9 @@ __device__ __forceinline__ Sel select(const NttTables& tb, int pid, bool inverse
   s.tw = (inverse ? tb.itw : tb.tw) + o;
   s.twsh = (inverse ? tb.itw_sh : tb.tw_sh) + o;
   s.twd = (inverse ? tb.itwd : tb.twd) + o;
+  s.ts = (inverse ? tb.itwist : tb.twist) + o;
+  s.tssh = (inverse ? tb.itwist_sh : tb.twist_sh) + o;
+  s.tsd = (inverse ? tb.itwistd : tb.twistd) + o;
   return s;
 }
 
@@ -209,12 +225,28 @@ __global__ __launch_bounds__(kThreads) void k_ntt_fwd_cols(Addr ad, AddrOut ao, 
   }
 }
 
+// ------------------------------------------------------------------------------- row tiles in LDS
+// A row pass writes its tile row-major (word c0 + T k) and reads it transposed (word 16 c0 + k),
+// i.e. at a stride of 16 words across lanes, which maps pairs of lanes onto the same LDS banks.
+// One pad word every 16 words makes that stride 17: the T lanes of a row then hit distinct bank
+// pairs in both patterns, and a row length of 17 T words (= T mod 32) places the rows sharing a
+// half-wave on disjoint banks.
+__device__ __forceinline__ int rsw(int e) { return e + (e >> 4); }
+template <int LS>
+struct RowTile {
+  static constexpr int S = 1 << LS;
+  static constexpr int PAD = S + S / 16;   // words per row in LDS; rsw(S - 1) = PAD - 2
+};
+
 // ------------------------------------------------------------------------------- forward, rows
-// Last l2 stages. Input: internal form, output: canonical residues.
+// Last l2 stages. Row rho of the intermediate is finished by a standard S2-point negacyclic
+// transform of x_c tau_rho^c (c = position in the row): one coalesced twist multiply per element,
+// then stages whose twiddles are the first S2 entries of the table, the same for every row, read
+// from LDS. Input: internal form, output: canonical residues handed to the epilogue.
 template <int LS, class A, class TwT, class Epi>
-__device__ __forceinline__ void fwd_rows_body(const A& ar, const TwT* tw, const TwT* twp, uint64_t* p, uint64_t* tile, int l1,
-                                              int R, const Epi& epi) {
-  constexpr int S = 1 << LS, T = S / 16, PADS = S + 1;
+__device__ __forceinline__ void fwd_rows_body(const A& ar, const TwT* sw, const TwT* swp, const TwT* ts, const TwT* tsp, uint64_t* p,
+                                              uint64_t* tile, int R, const Epi& epi) {
+  constexpr int S = 1 << LS, T = S / 16, PADS = RowTile<LS>::PAD;
   const int tid = threadIdx.x;
   const int rr = tid / T, c0 = tid % T;
   const uint32_t rho = blockIdx.x * R + rr;
@@ -224,13 +256,15 @@ __device__ __forceinline__ void fwd_rows_body(const A& ar, const TwT* tw, const 
 #pragma unroll
   for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(row[c0 + T * k]);
 #pragma unroll
+  for (int k = 0; k < 16; ++k) x[k] = ar.mul(x[k], ar.tw(ts + rbase, tsp + rbase, c0 + T * k));
+#pragma unroll
   for (int s = 0; s < 4; ++s) {
     const int h = 8 >> s;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       if (k & h) continue;
-      const uint32_t idx = (1u << (l1 + s)) + (rho << s) + (k >> (4 - s));
-      ar.fwd(x[k], x[k + h], ar.tw(tw, twp, idx));
+      const int idx = (1 << s) + (k >> (4 - s));
+      ar.fwd(x[k], x[k + h], ar.tw(sw, swp, idx));
     }
   }
   uint64_t* t = tile + rr * PADS;
@@ -238,27 +272,27 @@ __device__ __forceinline__ void fwd_rows_body(const A& ar, const TwT* tw, const 
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       ar.recenter(x[k]);
-      t[c0 + T * k] = ar.to_mid(x[k]);
+      t[rsw(c0 + T * k)] = ar.to_mid(x[k]);
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(t[16 * c0 + k]);
+    for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(t[rsw(16 * c0 + k)]);
 #pragma unroll
     for (int s = 4; s < LS; ++s) {
       const int h = S >> (s + 1);
 #pragma unroll
       for (int k = 0; k < 16; ++k) {
         if (k & h) continue;
-        const uint32_t idx = (1u << (l1 + s)) + (rho << s) + ((16 * c0 + k) >> (LS - s));
-        ar.fwd(x[k], x[k + h], ar.tw(tw, twp, idx));
+        const int idx = (1 << s) + ((16 * c0 + k) >> (LS - s));
+        ar.fwd(x[k], x[k + h], ar.tw(sw, swp, idx));
       }
     }
     // each thread rewrites exactly the LDS words it read: no barrier needed before the store
 #pragma unroll
-    for (int k = 0; k < 16; ++k) t[16 * c0 + k] = ar.canon(x[k]);
+    for (int k = 0; k < 16; ++k) t[rsw(16 * c0 + k)] = ar.canon(x[k]);
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 16; ++k) epi.store(row, rbase, c0 + T * k, t[c0 + T * k]);
+    for (int k = 0; k < 16; ++k) epi.store(row, rbase, c0 + T * k, t[rsw(c0 + T * k)]);
   } else {
 #pragma unroll
     for (int k = 0; k < 16; ++k) epi.store(row, rbase, c0 + T * k, ar.canon(x[k]));
@@ -324,47 +358,59 @@ struct EpiRescale {
 };
 
 template <int LS, class Addr, class Epi>
-__global__ __launch_bounds__(kThreads) void k_ntt_fwd_rows(Addr ad, NttTables tb, int l1, Epi epi) {
-  constexpr int S = 1 << LS, T = S / 16, PADS = S + 1, RMAX = kThreads / T;
+__global__ __launch_bounds__(kThreads) void k_ntt_fwd_rows(Addr ad, NttTables tb, Epi epi) {
+  constexpr int S = 1 << LS, T = S / 16, PADS = RowTile<LS>::PAD, RMAX = kThreads / T;
   __shared__ uint64_t tile[RMAX * PADS];
+  __shared__ uint64_t sw[S], swp[S];
   uint64_t* p;
   int pid;
   ad.at(blockIdx.y, tb.N, p, pid);
   epi.init(blockIdx.y, tb.N, tb.prime);
   const Sel sel = select(tb, pid, false);
   const int R = blockDim.x / T;
-  if (sel.fp) fwd_rows_body<LS>(sel.fa, sel.twd, sel.twd, p, tile, l1, R, epi);
-  else fwd_rows_body<LS>(sel.ia, sel.tw, sel.twsh, p, tile, l1, R, epi);
+  if (sel.fp) {
+    double* swd = reinterpret_cast<double*>(sw);
+    for (int i = threadIdx.x; i < S; i += blockDim.x) swd[i] = sel.twd[i];
+    __syncthreads();
+    fwd_rows_body<LS>(sel.fa, (const double*)swd, (const double*)swd, sel.tsd, sel.tsd, p, tile, R, epi);
+  } else {
+    for (int i = threadIdx.x; i < S; i += blockDim.x) { sw[i] = sel.tw[i]; swp[i] = sel.twsh[i]; }
+    __syncthreads();
+    fwd_rows_body<LS>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, sel.ts, sel.tssh, p, tile, R, epi);
+  }
 }
 
 // ------------------------------------------------------------------------------- inverse, rows
-// First launch of the INTT (last l2 stages, reversed). Input: canonical, output: internal form.
-// fp64 values are re-centred after every second stage (Gentleman-Sande sums double per stage).
+// First launch of the INTT: the inverse of the forward row transform, i.e. the standard S2-point
+// Gentleman-Sande stages (first S2 entries of the inverse table, from LDS) followed by one
+// coalesced multiply by tau_rho^-c. Input: canonical, output: internal form. fp64 values are
+// re-centred after every second stage (Gentleman-Sande sums double per stage).
 template <int LS, class A, class TwT>
-__device__ __forceinline__ void inv_rows_body(const A& ar, const TwT* tw, const TwT* twp, const uint64_t* p, uint64_t* po,
-                                              uint64_t* tile, int l1, int R) {
-  constexpr int S = 1 << LS, T = S / 16, PADS = S + 1;
+__device__ __forceinline__ void inv_rows_body(const A& ar, const TwT* sw, const TwT* swp, const TwT* ts, const TwT* tsp, const uint64_t* p,
+                                              uint64_t* po, uint64_t* tile, int R) {
+  constexpr int S = 1 << LS, T = S / 16, PADS = RowTile<LS>::PAD;
   const int tid = threadIdx.x;
   const int rr = tid / T, c0 = tid % T;
   const uint32_t rho = blockIdx.x * R + rr;
   const uint64_t* row = p + (size_t)rho * S;
   uint64_t* rowo = po + (size_t)rho * S;
+  const uint32_t rbase = rho * S;
   uint64_t* t = tile + rr * PADS;
   typename A::T x[16];
   if constexpr (LS > 4) {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) t[c0 + T * k] = row[c0 + T * k];
+    for (int k = 0; k < 16; ++k) t[rsw(c0 + T * k)] = row[c0 + T * k];
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = ar.from_canon(t[16 * c0 + k]);
+    for (int k = 0; k < 16; ++k) x[k] = ar.from_canon(t[rsw(16 * c0 + k)]);
 #pragma unroll
     for (int s = LS - 1; s >= 4; --s) {
       const int h = S >> (s + 1);
 #pragma unroll
       for (int k = 0; k < 16; ++k) {
         if (k & h) continue;
-        const uint32_t idx = (1u << (l1 + s)) + (rho << s) + ((16 * c0 + k) >> (LS - s));
-        ar.inv(x[k], x[k + h], ar.tw(tw, twp, idx));
+        const int idx = (1 << s) + ((16 * c0 + k) >> (LS - s));
+        ar.inv(x[k], x[k + h], ar.tw(sw, swp, idx));
       }
       if (((LS - s) & 1) == 0) {
 #pragma unroll
@@ -374,11 +420,11 @@ __device__ __forceinline__ void inv_rows_body(const A& ar, const TwT* tw, const 
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       ar.recenter(x[k]);
-      t[16 * c0 + k] = ar.to_mid(x[k]);
+      t[rsw(16 * c0 + k)] = ar.to_mid(x[k]);
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(t[c0 + T * k]);
+    for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(t[rsw(c0 + T * k)]);
   } else {
 #pragma unroll
     for (int k = 0; k < 16; ++k) x[k] = ar.from_canon(row[c0 + T * k]);
@@ -389,8 +435,8 @@ __device__ __forceinline__ void inv_rows_body(const A& ar, const TwT* tw, const 
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       if (k & h) continue;
-      const uint32_t idx = (1u << (l1 + s)) + (rho << s) + (k >> (4 - s));
-      ar.inv(x[k], x[k + h], ar.tw(tw, twp, idx));
+      const int idx = (1 << s) + (k >> (4 - s));
+      ar.inv(x[k], x[k + h], ar.tw(sw, swp, idx));
     }
     if ((s & 1) == 0) {
 #pragma unroll
@@ -398,21 +444,30 @@ __device__ __forceinline__ void inv_rows_body(const A& ar, const TwT* tw, const 
     }
   }
 #pragma unroll
-  for (int k = 0; k < 16; ++k) rowo[c0 + T * k] = ar.to_mid(x[k]);
+  for (int k = 0; k < 16; ++k) rowo[c0 + T * k] = ar.to_mid(ar.mul(x[k], ar.tw(ts + rbase, tsp + rbase, c0 + T * k)));
 }
 
 template <int LS, class Addr, class AddrOut>
-__global__ __launch_bounds__(kThreads) void k_ntt_inv_rows(Addr ad, AddrOut ao, NttTables tb, int l1) {
-  constexpr int S = 1 << LS, T = S / 16, PADS = S + 1, RMAX = kThreads / T;
+__global__ __launch_bounds__(kThreads) void k_ntt_inv_rows(Addr ad, AddrOut ao, NttTables tb) {
+  constexpr int S = 1 << LS, T = S / 16, PADS = RowTile<LS>::PAD, RMAX = kThreads / T;
   __shared__ uint64_t tile[RMAX * PADS];
+  __shared__ uint64_t sw[S], swp[S];
   uint64_t *p, *po;
   int pid, pido;
   ad.at(blockIdx.y, tb.N, p, pid);
   ao.at(blockIdx.y, tb.N, po, pido);
   const Sel sel = select(tb, pid, true);
   const int R = blockDim.x / T;
-  if (sel.fp) inv_rows_body<LS>(sel.fa, sel.twd, sel.twd, p, po, tile, l1, R);
-  else inv_rows_body<LS>(sel.ia, sel.tw, sel.twsh, p, po, tile, l1, R);
+  if (sel.fp) {
+    double* swd = reinterpret_cast<double*>(sw);
+    for (int i = threadIdx.x; i < S; i += blockDim.x) swd[i] = sel.twd[i];
+    __syncthreads();
+    inv_rows_body<LS>(sel.fa, (const double*)swd, (const double*)swd, sel.tsd, sel.tsd, p, po, tile, R);
+  } else {
+    for (int i = threadIdx.x; i < S; i += blockDim.x) { sw[i] = sel.tw[i]; swp[i] = sel.twsh[i]; }
+    __syncthreads();
+    inv_rows_body<LS>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, sel.ts, sel.tssh, p, po, tile, R);
+  }
 }
 
 // ------------------------------------------------------------------------------- inverse, columns
@@ -611,7 +666,7 @@ void fwd_impl(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlim
     Shift<AddrOut> so{ao, base};
     ShiftEpi<Epi> se{epi, base};
     { ProfScope ps_(c, "ntt_fwd_cols", st); k_ntt_fwd_cols<L1S><<<dim3((c.N >> L1S) / C, n), b1, 0, st>>>(sa, so, tb); }
-    { ProfScope ps_(c, rows_name, st); k_ntt_fwd_rows<L2S><<<dim3((c.N >> L2S) / R, n), b2, 0, st>>>(so, tb, L1S, se); }
+    { ProfScope ps_(c, rows_name, st); k_ntt_fwd_rows<L2S><<<dim3((c.N >> L2S) / R, n), b2, 0, st>>>(so, tb, se); }
   }
 }
 
@@ -625,7 +680,7 @@ void inv_impl(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlim
     uint32_t n = std::min(chunk, nlimbs - base);
     Shift<Addr> sa{ad, base};
     Shift<AddrOut> so{ao, base};
-    { ProfScope ps_(c, "ntt_inv_rows", st); k_ntt_inv_rows<L2S><<<dim3((c.N >> L2S) / R, n), b2, 0, st>>>(sa, so, tb, L1S); }
+    { ProfScope ps_(c, "ntt_inv_rows", st); k_ntt_inv_rows<L2S><<<dim3((c.N >> L2S) / R, n), b2, 0, st>>>(sa, so, tb); }
     { ProfScope ps_(c, "ntt_inv_cols", st); k_ntt_inv_cols<L1S><<<dim3((c.N >> L1S) / C, n), b1, 0, st>>>(so, tb); }
   }
 }

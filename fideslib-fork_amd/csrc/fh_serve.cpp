@@ -142,8 +142,10 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
       }
       continue;
     }
-    // one tick: the next step of every admitted request
-    if (cfg.mode == 0) batch.reset();
+    // one tick: the next step of every admitted request. A tick of one or two operations is
+    // issued directly (the batch's table upload would cost more than it saves).
+    const bool use_batch = cfg.mode == 0 && active.size() >= 3;
+    if (use_batch) batch.reset();
     for (auto& rp : active) {
       Request* r = rp.get();
       const Step& s = progs[r->inference ? 1 : 0][r->pc];
@@ -156,7 +158,7 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
       switch (s.kind) {
         case kMulPtRs:
           ld = la - 1;
-          if (cfg.mode == 0) batch.add_mult_plain(a, b, la, true, d);
+          if (use_batch) batch.add_mult_plain(a, b, la, true, d);
           else {
             std::vector<int> ids(lq);
             for (size_t i = 0; i < lq; ++i) ids[i] = (int)i;
@@ -166,12 +168,12 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
           }
           break;
         case kRotate:
-          if (cfg.mode == 0) batch.add_rotate(a, la, rot_keys[s.rot], rot_galois[s.rot], d);
+          if (use_batch) batch.add_rotate(a, la, rot_keys[s.rot], rot_galois[s.rot], d);
           else eval_rotate(c, a, la, rot_keys[s.rot], rot_galois[s.rot], d, st);
           break;
         case kAdd:
           if (r->level[s.b] != la) throw Error(-6, "serving program adds operands at different levels");
-          if (cfg.mode == 0) batch.add_add(a, b, la, 2, d);
+          if (use_batch) batch.add_add(a, b, la, 2, d);
           else {
             std::vector<int> ids(2 * lq);
             for (size_t i = 0; i < 2 * lq; ++i) ids[i] = (int)(i % lq);
@@ -180,7 +182,7 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
           }
           break;
         case kAddPt:
-          if (cfg.mode == 0) batch.add_add_plain(a, b, la, d);
+          if (use_batch) batch.add_add_plain(a, b, la, d);
           else {
             std::vector<int> ids(lq);
             for (size_t i = 0; i < lq; ++i) ids[i] = (int)i;
@@ -190,7 +192,7 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
           break;
         case kDrop: {
           ld = r->level[s.b];  // the level of the register to align with
-          if (cfg.mode == 0) batch.add_level_reduce(a, la, ld, 2, d);
+          if (use_batch) batch.add_level_reduce(a, la, ld, 2, d);
           else {
             const size_t lo = ld + 1;
             for (int comp = 0; comp < 2; ++comp)
@@ -201,7 +203,7 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
         case kMultRs:
           if (r->level[s.b] != la) throw Error(-6, "serving program multiplies operands at different levels");
           ld = la - 1;
-          if (cfg.mode == 0) batch.add_mult(a, b, la, relin_key, true, d);
+          if (use_batch) batch.add_mult(a, b, la, relin_key, true, d);
           else {
             eval_mult(c, a, b, la, relin_key, tmp.p, st);
             rescale(c, tmp.p, la, 2, d, st);
@@ -211,7 +213,7 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
       r->level[s.dst] = ld;
     }
     ops_total += (double)active.size();
-    if (cfg.mode == 0) {
+    if (use_batch) {
       batch.finalize();
       batch.run(st, false);
     }

@@ -27,7 +27,8 @@ struct ServeConfig {
   int requests = 200;
   int start_level = 24;        // level of every incoming ciphertext
   double hbm_budget_gb = 64.0; // admission budget for request registers
-  int mode = 0;                // 0: one fused batch per tick, 1: one call per operation
+  int mode = 0;                // 0: one fused batch per tick (ticks of 1-2 operations issued directly),
+                               // 1: one call per operation
   double inference_share = 0.74;
   uint64_t seed = 1;
   bool checksum = false;       // sum every request's result words (to compare the two modes)

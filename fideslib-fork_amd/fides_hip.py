@@ -42,6 +42,7 @@ def _declare(L):
         "fh_context_create_generated": (I, [I, I, I, I, I, I, I, PP]),
         "fh_context_destroy": (I, [P]),
         "fh_context_shape": (I, [P, ip]),
+        "fh_context_set_reduced_noise": (I, [P, I]),
         "fh_context_moduli": (I, [P, u64p, u64p]),
         "fh_synchronize": (I, [P]),
         "fh_device_alloc": (I, [P, S, PP]),
@@ -85,6 +86,8 @@ def _declare(L):
         "fh_batch_run": (I, [P, I]),
         "fh_batch_info": (I, [P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
         "fh_serve_run": (I, [P, C.POINTER(ServeConfig), C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), P, C.POINTER(ServeResult)]),
+        "fh_linear_transform_batch": (I, [P, I, VP, S, I, I, I, I, I, I, VP, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), P,
+                                          C.c_uint32, P, C.c_uint32, VP]),
         "fh_tune_set": (I, [P, C.c_char_p, I]),
         "fh_tune_get": (I, [P, C.c_char_p, C.POINTER(C.c_int)]),
         "fh_profile_enable": (I, [P, I]),
@@ -186,6 +189,9 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_reduced_noise(self, on):
+        _check(load().fh_context_set_reduced_noise(self.h, int(on)))
 
     def sync(self):
         _check(load().fh_synchronize(self.h))
@@ -294,6 +300,15 @@ class Context:
         _check(load().fh_linear_transform(self.h, ct.ptr, level, row_size, b_step, stride, offset, slots, diags.ptr, bk, bg,
                                           giant_key.h if giant_key is not None else None, giant_galois,
                                           offset_key.h if offset_key is not None else None, offset_galois, out.ptr))
+
+    def linear_transform_batch(self, cts, count, level, row_size, b_step, stride, diags, baby_keys, baby_galois, out, giant_key=None,
+                               giant_galois=1, offset_key=None, offset_galois=1, offset=0, slots=None, ct_stride=0):
+        slots = slots or self.N // 2
+        bk = (C.c_void_p * b_step)(*[(k.h if k is not None else None) for k in baby_keys])
+        bg = (C.c_uint32 * b_step)(*[int(g) for g in baby_galois])
+        _check(load().fh_linear_transform_batch(self.h, count, cts.ptr, ct_stride, level, row_size, b_step, stride, offset, slots,
+                                                diags.ptr, bk, bg, giant_key.h if giant_key is not None else None, giant_galois,
+                                                offset_key.h if offset_key is not None else None, offset_galois, out.ptr))
 
     # ---- serving loop ----
     def serve(self, rot_keys, rot_galois, relin_key, rate_per_s, requests, start_level=None, mode=0, hbm_budget_gb=64.0,

@@ -250,6 +250,9 @@ struct orc_ctx {
   /* Rescale: [q_l^-1]_{q_i}, QlQlInvModqlDivqlModq[l][i] */
   u64* qlInv;   /* [L1][L1] */
   u64* QlQlInv; /* [L1][L1] */
+  /* ReducedNoise basis conversions (src/CKKS/ReducedNoise.cuh:12-37): a prescaled source residue
+   * above q/2 counts as v - q, i.e. the sum gains -count * Q_src */
+  int reduced_noise;
 };
 
 static int brev(u64 x, int bits) {
@@ -567,6 +570,15 @@ void orc_modup(const orc_ctx* c, const u64* x, int l, u64* out) {
       for (u64 k = 0; k < N; ++k) y[(size_t)i * N + k] = orc_mulmod(y[(size_t)i * N + k], qhinv[i], qi);
     }
     u64* od = out + (size_t)d * ext * N;
+    /* ReducedNoise: per coefficient, the number of prescaled sources above q_i/2 */
+    unsigned char* lifted = NULL;
+    if (c->reduced_noise) {
+      lifted = (unsigned char*)calloc(N, 1);
+      for (int i = 0; i < sz; ++i) {
+        u64 half = c->q[start + i] >> 1;
+        for (u64 k = 0; k < N; ++k) lifted[k] += y[(size_t)i * N + k] > half;
+      }
+    }
 #pragma omp parallel for schedule(dynamic)
     for (int pos = 0; pos < ext; ++pos) {
       int p = ext_prime(c, l, pos);
@@ -582,13 +594,21 @@ void orc_modup(const orc_ctx* c, const u64* x, int l, u64* out) {
         for (int j = start; j < end; ++j) if (j != start + i) pr = orc_mulmod(pr, c->q[j] % pj, pj);
         qh[i] = pr;
       }
+      u64 negQ = 0;  /* [-Q_d]_{p_j}, Q_d the product of the digit's active primes */
+      if (lifted) {
+        u64 qd = 1;
+        for (int j = start; j < end; ++j) qd = orc_mulmod(qd, c->q[j] % pj, pj);
+        negQ = qd ? pj - qd : 0;
+      }
       for (u64 k = 0; k < N; ++k) {
         u128 s = 0;
         for (int i = 0; i < sz; ++i) s += (u128)y[(size_t)i * N + k] * qh[i];
+        if (lifted) s += (u128)lifted[k] * negQ;
         o[k] = (u64)(s % pj);
       }
       ntt1(c, o, p);
     }
+    free(lifted);
     free(y);
   }
 }
@@ -636,13 +656,23 @@ void orc_moddown(const orc_ctx* c, const u64* x, int l, u64* out) {
     intt1(c, y + (size_t)k * N, p);
     for (u64 j = 0; j < N; ++j) y[(size_t)k * N + j] = orc_mulmod(y[(size_t)k * N + j], c->PhatInv[k], pk);
   }
+  unsigned char* lifted = NULL;
+  if (c->reduced_noise) {
+    lifted = (unsigned char*)calloc(N, 1);
+    for (int k = 0; k < K; ++k) {
+      u64 half = c->q[c->L1 + k] >> 1;
+      for (u64 j = 0; j < N; ++j) lifted[j] += y[(size_t)k * N + j] > half;
+    }
+  }
 #pragma omp parallel for
   for (int i = 0; i <= l; ++i) {
     u64 qi = c->q[i];
+    u64 negP = c->Pmod[i] ? qi - c->Pmod[i] : 0;  /* [-P]_{q_i} */
     u64* conv = (u64*)malloc(N * 8);
     for (u64 j = 0; j < N; ++j) {
       u128 s = 0;
       for (int k = 0; k < K; ++k) s += (u128)y[(size_t)k * N + j] * c->PhatModq[(size_t)k * c->L1 + i];
+      if (lifted) s += (u128)lifted[j] * negP;
       conv[j] = (u64)(s % qi);
     }
     ntt1(c, conv, i);
@@ -651,8 +681,11 @@ void orc_moddown(const orc_ctx* c, const u64* x, int l, u64* out) {
     for (u64 j = 0; j < N; ++j) oi[j] = orc_mulmod(modsub(xi[j], conv[j], qi), c->PInv[i], qi);
     free(conv);
   }
+  free(lifted);
   free(y);
 }
+
+void orc_set_reduced_noise(orc_ctx* c, int on) { c->reduced_noise = on != 0; }
 
 /* OpenFHE PrecomputeAutoMap + AutomorphismTransform (EVALUATION form, gather):
  * out[brev(j)] = x[brev(((2j+1) k mod 2N) >> 1)].

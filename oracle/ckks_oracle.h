@@ -47,6 +47,9 @@ int orc_default_num_special(const uint64_t* q, int L1, int dnum, int aux_bits);
 
 /* ---- context ---- */
 orc_ctx* orc_create(int logN, int L1, int K, int dnum, const uint64_t* primes);
+/* ReducedNoise basis conversions in ModUp / ModDown (prescaled residues above q/2 read as v - q;
+ * src/CKKS/ReducedNoise.cuh:12-37, src/CKKS/Conv.cu:76-89, 356-381); off by default */
+void orc_set_reduced_noise(orc_ctx* c, int on);
 void orc_destroy(orc_ctx* c);
 uint64_t orc_prime(const orc_ctx* c, int idx);
 uint64_t orc_psi(const orc_ctx* c, int idx);

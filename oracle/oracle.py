@@ -92,6 +92,7 @@ def lib():
             "orc_encrypt_pk": (V, [P, u64p, I, u64p, u64p, U, u64p]),
             "orc_random_poly": (V, [P, U, i32p, I, u64p]),
             "orc_mt_first": (U, [U, I]),
+            "orc_set_reduced_noise": (V, [P, I]),
             "orc_num_threads": (I, []),
             "orc_set_threads": (V, [I]),
         }
@@ -178,6 +179,10 @@ class Ctx:
         if getattr(self, "h", None):
             lib().orc_destroy(self.h)
             self.h = None
+
+    def set_reduced_noise(self, on):
+        """ReducedNoise ModUp/ModDown conversions (prescaled residues above q/2 read as v - q)."""
+        lib().orc_set_reduced_noise(self.h, int(on))
 
     # ---- helpers ----
     def q(self, i):

@@ -54,3 +54,43 @@ def test_linear_transform_cfg4_shape(gctx2, octx2):
     """The cfg4 shape: N=2^16, level 20 (21 limbs), 64 diagonals, bStep = gStep = 8, stride 1."""
     s = octx2.gen_secret(71)
     _run(gctx2, octx2, s, 20, 64, 8, 1, 0, seed=4, check_decrypt=True)
+
+
+def _batch_case(g, o, s, l, rowSize, bStep, stride, offset, count, seed, scale_pt=2.0 ** 30):
+    """count ciphertexts through fh_linear_transform_batch: each result equals the oracle's LT."""
+    n = o.N // 2
+    rng = np.random.default_rng(seed)
+    D = rng.uniform(-1, 1, (rowSize, n)) + 1j * rng.uniform(-1, 1, (rowSize, n))
+    diags = np.concatenate([o.encode_ext(np.roll(D[k], bStep * (k // bStep) * stride + offset), scale_pt, l) for k in range(rowSize)])
+    bk = [None] + [o.gen_rot_key(s, o.rot_index(i * stride), 900 + i) for i in range(1, bStep)]
+    bkk = [1] + [o.rot_index(i * stride) for i in range(1, bStep)]
+    gk = o.gen_rot_key(s, o.rot_index(bStep * stride), 950)
+    ok = o.gen_rot_key(s, o.rot_index(offset), 960) if offset else None
+    cts = [o.encrypt(o.encode(rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n), 2.0 ** 50, l), l, s, seed + 10 + b) for b in range(count)]
+    exp = [o.linear_transform(ct, l, diags, bStep, stride, bk, bkk, gk, o.rot_index(bStep * stride), ok,
+                              o.rot_index(offset) if offset else 1, offset) for ct in cts]
+    dcts, dd = g.to_device(np.concatenate(cts)), g.to_device(diags)
+    dbk = [None] + [g.key(k) for k in bk[1:]]
+    dgk = g.key(gk)
+    dok = g.key(ok) if offset else None
+    out = g.array((count * 2 * (l + 1), o.N))
+    g.linear_transform_batch(dcts, count, l, rowSize, bStep, stride, dd, dbk, bkk, out, giant_key=dgk,
+                             giant_galois=o.rot_index(bStep * stride), offset_key=dok,
+                             offset_galois=o.rot_index(offset) if offset else 1, offset=offset)
+    got = out.download().reshape(count, 2 * (l + 1), o.N)
+    for b in range(count):
+        assert np.array_equal(got[b], exp[b]), f"batched linear transform, ciphertext {b}, differs from the oracle"
+    for x in [dcts, dd, out] + dbk[1:] + [dgk] + ([dok] if dok else []):
+        x.free()
+
+
+@pytest.mark.parametrize("rowSize,bStep,offset,count", [(12, 4, 0, 3), (10, 4, 3, 5), (8, 8, 0, 2), (16, 4, 5, 4)])
+def test_linear_transform_batch_cfg1(rowSize, bStep, offset, count, gctx1, octx1):
+    s = octx1.gen_secret(72)
+    _batch_case(gctx1, octx1, s, 3, rowSize, bStep, 1, offset, count, seed=3 * rowSize + offset)
+
+
+def test_linear_transform_batch_cfg4_shape(gctx2, octx2):
+    """The LT workload's shape: N=2^16, level 20, 64 diagonals, bStep = gStep = 8, two ciphertexts."""
+    s = octx2.gen_secret(73)
+    _batch_case(gctx2, octx2, s, 20, 64, 8, 1, 0, 2, seed=5)

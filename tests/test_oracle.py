@@ -242,6 +242,59 @@ def test_moddown_closed_form(sc):
             assert min(diff, Mq - diff) <= K
 
 
+def test_reduced_noise_conversions_closed_form():
+    """ReducedNoise: every prescaled source y above q/2 enters the conversion as y - q, so ModUp
+    yields sum_i centred(y_i) Qhat_i mod p_j (and differs from the plain conversion exactly by
+    -count * Q_d), and ModDown converts with centred sources too."""
+    c = small_ctx()
+    c.set_reduced_noise(True)
+    N, K, L1 = c.N, c.K, c.L1
+    cen = lambda v, q: v - q if v > q // 2 else v
+    for l in (3, 1):
+        x = c.random_poly(40 + l, list(range(l + 1)))
+        dig = c.modup(x, l)
+        ext = l + 1 + K
+        xc = c.intt(x, list(range(l + 1)))
+        pids = c.ext_pids(l)
+        lifted_seen = 0
+        for d in range(c.num_digits(l)):
+            start, end = d * c.alpha, min((d + 1) * c.alpha, l + 1)
+            blkc = c.intt(dig[d * ext:(d + 1) * ext], pids)
+            qs = [c.q(i) for i in range(start, end)]
+            Qd = 1
+            for qq in qs:
+                Qd *= qq
+            for n in (0, 3, N // 2, N - 1):
+                ys = [int(xc[i, n]) * pow(Qd // qq, -1, qq) % qq for i, qq in zip(range(start, end), qs)]
+                V = sum(cen(y, qq) * (Qd // qq) for y, qq in zip(ys, qs))
+                lifted_seen += sum(y > qq // 2 for y, qq in zip(ys, qs))
+                X, _ = crt([xc[i, n] for i in range(start, end)], qs)
+                assert (V - X) % Qd == 0 and abs((V - X) // Qd) <= len(qs)
+                for pos in range(ext):
+                    if start <= pos < end:
+                        continue
+                    assert int(blkc[pos, n]) == V % c.q(pids[pos])
+        assert lifted_seen > 0
+        pe = c.ext_pids(l)
+        xe = c.random_poly(60 + l, pe)
+        out = c.moddown(xe, l)
+        xec = c.intt(xe, pe)
+        oc = c.intt(out, list(range(l + 1)))
+        ps = [c.q(L1 + k) for k in range(K)]
+        P = 1
+        for p in ps:
+            P *= p
+        for n in (1, 7, N - 2):
+            V = sum(cen(int(xec[l + 1 + k, n]) * pow(P // p, -1, p) % p, p) * (P // p) for k, p in enumerate(ps))
+            for i in range(l + 1):
+                qi = c.q(i)
+                assert int(oc[i, n]) == (int(xec[i, n]) - V) * pow(P, -1, qi) % qi
+    # switched off again: the plain conversion
+    c.set_reduced_noise(False)
+    x = c.random_poly(41, list(range(4)))
+    assert np.array_equal(c.modup(x, 3), small_ctx().modup(x, 3))
+
+
 def test_rescale_closed_form(sc):
     c = sc
     L1 = c.L1
