@@ -105,7 +105,7 @@ struct ConvConst {
 };
 
 // k_modup_conv_r of fh_keyswitch.hip with the level and digit read from the job
-template <int SZ, class M>
+template <int SZ, class M, bool RN>
 __global__ __launch_bounds__(kT) void k_modup_conv_jobs(const ModUpJob* __restrict__ jobs, ConvConst cc) {
   using Op = typename M::Op;
   __shared__ Op s_qhat[SZ * kMaxPrimes];
@@ -120,13 +120,12 @@ __global__ __launch_bounds__(kT) void k_modup_conv_jobs(const ModUpJob* __restri
     const uint32_t si = i / ext, pos = i % ext;
     s_qhat[si * ext + pos] = si < sz ? M::prep(cc.qhat_m[(tb + si) * E + pos]) : M::zero_op();
   }
-  __shared__ Op s_negq[kMaxPrimes];
-  const bool rn = cc.negq_m != nullptr;
+  __shared__ Op s_negq[RN ? kMaxPrimes : 1];
   for (uint32_t pos = threadIdx.x; pos < ext; pos += blockDim.x) {
     const PrimeConst& P = cc.prime[pos < lq ? pos : cc.L1 + (pos - lq)];
     s_pq[pos] = P.q;
     s_pn[pos] = P.mont_ninv;
-    if (rn) s_negq[pos] = M::prep(cc.negq_m[((size_t)j.l * cc.dnum + j.d) * E + pos]);
+    if constexpr (RN) s_negq[pos] = M::prep(cc.negq_m[((size_t)j.l * cc.dnum + j.d) * E + pos]);
   }
   for (uint32_t i = threadIdx.x; i < sz; i += blockDim.x) {
     inv[i] = cc.qhinv[tb + i];
@@ -143,8 +142,10 @@ __global__ __launch_bounds__(kT) void k_modup_conv_jobs(const ModUpJob* __restri
         const size_t o = (size_t)(start + i) * N + n;
         const ulonglong2 c = *reinterpret_cast<const ulonglong2*>(j.coef + o);
         const uint64_t v0 = shoup(c.x, inv[i], invsh[i], qs[i]), v1 = shoup(c.y, inv[i], invsh[i], qs[i]);
-        lift0 += v0 > (qs[i] >> 1);
-        lift1 += v1 > (qs[i] >> 1);
+        if constexpr (RN) {
+          lift0 += v0 > (qs[i] >> 1);
+          lift1 += v1 > (qs[i] >> 1);
+        }
         y0[i] = M::prep(v0);
         y1[i] = M::prep(v1);
         *reinterpret_cast<ulonglong2*>(j.out + o) = *reinterpret_cast<const ulonglong2*>(j.eval + o);
@@ -168,7 +169,7 @@ __global__ __launch_bounds__(kT) void k_modup_conv_jobs(const ModUpJob* __restri
         a0.mac(y0[i], c);
         a1.mac(y1[i], c);
       }
-      if (rn) {
+      if constexpr (RN) {
         a0.mac(l0, s_negq[pos]);
         a1.mac(l1, s_negq[pos]);
       }
@@ -287,7 +288,7 @@ struct MdJob {
 };
 
 // k_moddown_conv_r of fh_keyswitch.hip with l+1 read from the job
-template <int KK, class M>
+template <int KK, class M, bool RN>
 __global__ __launch_bounds__(kT) void k_moddown_conv_jobs(const MdJob* __restrict__ jobs, ConvConst cc) {
   using Op = typename M::Op;
   __shared__ Op s_phat[KK * kMaxPrimes];
@@ -299,12 +300,11 @@ __global__ __launch_bounds__(kT) void k_moddown_conv_jobs(const MdJob* __restric
     const uint32_t k = i / lq, t = i % lq;
     s_phat[k * lq + t] = k < K ? M::prep(cc.phat_m[(size_t)k * cc.L1 + t]) : M::zero_op();
   }
-  __shared__ Op s_negp[kMaxPrimes];
-  const bool rn = cc.negp_m != nullptr;
+  __shared__ Op s_negp[RN ? kMaxPrimes : 1];
   for (uint32_t i = threadIdx.x; i < lq; i += blockDim.x) {
     s_q[i] = cc.prime[i].q;
     s_n[i] = cc.prime[i].mont_ninv;
-    if (rn) s_negp[i] = M::prep(cc.negp_m[i]);
+    if constexpr (RN) s_negp[i] = M::prep(cc.negp_m[i]);
   }
   for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) {
     inv[k] = cc.phinv[k];
@@ -320,8 +320,10 @@ __global__ __launch_bounds__(kT) void k_moddown_conv_jobs(const MdJob* __restric
       if (k < (int)K) {
         const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(j.pc + (size_t)k * N + n);
         const uint64_t v0 = shoup(v.x, inv[k], invsh[k], ps[k]), v1 = shoup(v.y, inv[k], invsh[k], ps[k]);
-        lift0 += v0 > (ps[k] >> 1);
-        lift1 += v1 > (ps[k] >> 1);
+        if constexpr (RN) {
+          lift0 += v0 > (ps[k] >> 1);
+          lift1 += v1 > (ps[k] >> 1);
+        }
         y0[k] = M::prep(v0);
         y1[k] = M::prep(v1);
       } else {
@@ -340,7 +342,7 @@ __global__ __launch_bounds__(kT) void k_moddown_conv_jobs(const MdJob* __restric
         a0.mac(y0[k], c);
         a1.mac(y1[k], c);
       }
-      if (rn) {
+      if constexpr (RN) {
         a0.mac(l0, s_negp[i]);
         a1.mac(l1, s_negp[i]);
       }
@@ -886,8 +888,7 @@ void launch_all(Context& c, BatchPlan& P, hipStream_t st) {
     {
       ProfScope ps_(c, "modup_conv", st);
       const dim3 g = grid_tiles(N, P.n_modup, 2);
-      if (c.mac30_ok()) FH_SZ_DISPATCH(c.alpha, k_modup_conv_jobs, Mac30, (g, kT, 0, st), (P.t_modup, cc))
-      else FH_SZ_DISPATCH(c.alpha, k_modup_conv_jobs, MacWide, (g, kT, 0, st), (P.t_modup, cc))
+      FH_CONV_DISPATCH(c, c.alpha, k_modup_conv_jobs, (g, kT, 0, st), (P.t_modup, cc))
     }
     FH_HIP(hipGetLastError());
     ntt_fwd_jobs(c, P.t_modup_ntt, P.n_modup_ntt, st);
@@ -903,8 +904,7 @@ void launch_all(Context& c, BatchPlan& P, hipStream_t st) {
     {
       ProfScope ps_(c, "moddown_conv", st);
       const dim3 g = grid_tiles(N, P.n_md, 2);
-      if (c.mac30_ok()) FH_SZ_DISPATCH(c.K, k_moddown_conv_jobs, Mac30, (g, kT, 0, st), (P.t_md, cc))
-      else FH_SZ_DISPATCH(c.K, k_moddown_conv_jobs, MacWide, (g, kT, 0, st), (P.t_md, cc))
+      FH_CONV_DISPATCH(c, c.K, k_moddown_conv_jobs, (g, kT, 0, st), (P.t_md, cc))
     }
     FH_HIP(hipGetLastError());
     ntt_fwd_jobs_moddown(c, P.t_md_ntt, P.t_md_epi, P.n_md_ntt, st);

@@ -134,19 +134,30 @@ inline LimbSet make_set(uint64_t* base, uint32_t nlimbs, const int* pids, uint32
   return s;
 }
 
-// Launch KERNEL<SZ, M> with the smallest compiled SZ >= n (n <= 16): the register-resident basis
-// conversions are instantiated for source counts 2, 4, 8, 9, 12 and 16.
-#define FH_SZ_DISPATCH(n, KERNEL, M, LAUNCH, ARGS)                     \
+// Launch KERNEL<SZ, M, RN> with the smallest compiled SZ >= n (n <= 16): the register-resident
+// basis conversions are instantiated for source counts 2, 4, 8, 9, 12 and 16.
+#define FH_SZ_DISPATCH(n, KERNEL, M, RN, LAUNCH, ARGS)                   \
   {                                                                     \
     const int n_ = (n);                                                 \
-    if (n_ <= 2) KERNEL<2, M><<<FH_UNPACK LAUNCH>>> ARGS;               \
-    else if (n_ <= 4) KERNEL<4, M><<<FH_UNPACK LAUNCH>>> ARGS;          \
-    else if (n_ <= 8) KERNEL<8, M><<<FH_UNPACK LAUNCH>>> ARGS;          \
-    else if (n_ <= 9) KERNEL<9, M><<<FH_UNPACK LAUNCH>>> ARGS;          \
-    else if (n_ <= 12) KERNEL<12, M><<<FH_UNPACK LAUNCH>>> ARGS;        \
-    else KERNEL<16, M><<<FH_UNPACK LAUNCH>>> ARGS;                      \
+    if (n_ <= 2) KERNEL<2, M, RN><<<FH_UNPACK LAUNCH>>> ARGS;           \
+    else if (n_ <= 4) KERNEL<4, M, RN><<<FH_UNPACK LAUNCH>>> ARGS;      \
+    else if (n_ <= 8) KERNEL<8, M, RN><<<FH_UNPACK LAUNCH>>> ARGS;      \
+    else if (n_ <= 9) KERNEL<9, M, RN><<<FH_UNPACK LAUNCH>>> ARGS;      \
+    else if (n_ <= 12) KERNEL<12, M, RN><<<FH_UNPACK LAUNCH>>> ARGS;    \
+    else KERNEL<16, M, RN><<<FH_UNPACK LAUNCH>>> ARGS;                  \
   }
 #define FH_UNPACK(...) __VA_ARGS__
+// Pick the multiply-accumulate policy and the ReducedNoise variant of a conversion kernel.
+#define FH_CONV_DISPATCH(ctx, n, KERNEL, LAUNCH, ARGS)                                          \
+  {                                                                                            \
+    if ((ctx).mac30_ok()) {                                                                    \
+      if ((ctx).reduced_noise) FH_SZ_DISPATCH(n, KERNEL, Mac30, true, LAUNCH, ARGS)            \
+      else FH_SZ_DISPATCH(n, KERNEL, Mac30, false, LAUNCH, ARGS)                               \
+    } else {                                                                                   \
+      if ((ctx).reduced_noise) FH_SZ_DISPATCH(n, KERNEL, MacWide, true, LAUNCH, ARGS)          \
+      else FH_SZ_DISPATCH(n, KERNEL, MacWide, false, LAUNCH, ARGS)                             \
+    }                                                                                          \
+  }
 
 // ---- launchers (fh_ntt.hip) ----
 void ntt_fwd(const Context& c, const LimbSet& s, hipStream_t st);

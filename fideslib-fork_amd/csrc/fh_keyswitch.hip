@@ -84,7 +84,7 @@ __global__ __launch_bounds__(kT) void k_modup_conv(const uint64_t* __restrict__ 
 // target's 128-bit sum reduced with one Montgomery reduction against the Montgomery-form matrix
 // [Qhat_i 2^64]_{p_j}. Sources beyond the digit's size are zero, so the product loop has no
 // branches.
-template <int SZ, class M>
+template <int SZ, class M, bool RN>
 __global__ __launch_bounds__(kT) void k_modup_conv_r(const uint64_t* __restrict__ in_coef, const uint64_t* __restrict__ in_eval,
                                                       uint64_t* __restrict__ out, ModUpParams mp) {
   using Op = typename M::Op;
@@ -102,13 +102,12 @@ __global__ __launch_bounds__(kT) void k_modup_conv_r(const uint64_t* __restrict_
     const uint32_t si = i / mp.ext, pos = i % mp.ext;
     s_qhat[si * mp.ext + pos] = si < sz ? M::prep(mp.qhat_m[(tb + si) * E + pos]) : M::zero_op();
   }
-  __shared__ Op s_negq[kMaxPrimes];
-  const bool rn = mp.negq_m != nullptr;
+  __shared__ Op s_negq[RN ? kMaxPrimes : 1];
   for (uint32_t pos = threadIdx.x; pos < mp.ext; pos += blockDim.x) {
     const PrimeConst& P = mp.prime[pos < mp.lq ? pos : mp.L1 + (pos - mp.lq)];
     s_pq[pos] = P.q;
     s_pn[pos] = P.mont_ninv;
-    if (rn) s_negq[pos] = M::prep(mp.negq_m[((size_t)mp.level * mp.dnum + d) * E + pos]);
+    if constexpr (RN) s_negq[pos] = M::prep(mp.negq_m[((size_t)mp.level * mp.dnum + d) * E + pos]);
   }
   // per-source constants live in LDS: held as uniform scalars they exhaust the SGPR file
   __shared__ uint64_t inv[SZ], invsh[SZ], qs[SZ];
@@ -128,8 +127,10 @@ __global__ __launch_bounds__(kT) void k_modup_conv_r(const uint64_t* __restrict_
         const size_t o = (size_t)(start + i) * mp.N + n;
         const ulonglong2 c = *reinterpret_cast<const ulonglong2*>(in_coef + o);
         const uint64_t v0 = shoup(c.x, inv[i], invsh[i], qs[i]), v1 = shoup(c.y, inv[i], invsh[i], qs[i]);
-        lift0 += v0 > (qs[i] >> 1);
-        lift1 += v1 > (qs[i] >> 1);
+        if constexpr (RN) {
+          lift0 += v0 > (qs[i] >> 1);
+          lift1 += v1 > (qs[i] >> 1);
+        }
         y0[i] = M::prep(v0);
         y1[i] = M::prep(v1);
         *reinterpret_cast<ulonglong2*>(od + o) = *reinterpret_cast<const ulonglong2*>(in_eval + o);
@@ -153,7 +154,7 @@ __global__ __launch_bounds__(kT) void k_modup_conv_r(const uint64_t* __restrict_
         a0.mac(y0[i], c);
         a1.mac(y1[i], c);
       }
-      if (rn) {
+      if constexpr (RN) {
         a0.mac(l0, s_negq[pos]);
         a1.mac(l1, s_negq[pos]);
       }
@@ -371,7 +372,7 @@ __global__ __launch_bounds__(kT) void k_moddown_conv(const uint64_t* __restrict_
 
 // Register-resident form for K <= KK special primes: two coefficients per lane, Montgomery-form
 // matrix [P^_k 2^64]_{q_i}, one Montgomery reduction per output residue; sources beyond K are zero.
-template <int KK, class M>
+template <int KK, class M, bool RN>
 __global__ __launch_bounds__(kT) void k_moddown_conv_r(const uint64_t* __restrict__ pcoef, uint64_t* __restrict__ conv, ModDownParams mp) {
   using Op = typename M::Op;
   __shared__ Op s_phat[KK * kMaxPrimes];
@@ -380,12 +381,11 @@ __global__ __launch_bounds__(kT) void k_moddown_conv_r(const uint64_t* __restric
     uint32_t k = i / mp.lq, t = i % mp.lq;
     s_phat[k * mp.lq + t] = k < mp.K ? M::prep(mp.phat_m[(size_t)k * mp.L1 + t]) : M::zero_op();
   }
-  __shared__ Op s_negp[kMaxPrimes];
-  const bool rn = mp.negp_m != nullptr;
+  __shared__ Op s_negp[RN ? kMaxPrimes : 1];
   for (uint32_t i = threadIdx.x; i < mp.lq; i += blockDim.x) {
     s_q[i] = mp.prime[i].q;
     s_n[i] = mp.prime[i].mont_ninv;
-    if (rn) s_negp[i] = M::prep(mp.negp_m[i]);
+    if constexpr (RN) s_negp[i] = M::prep(mp.negp_m[i]);
   }
   __shared__ uint64_t inv[KK], invsh[KK], ps[KK];
   for (uint32_t k = threadIdx.x; k < mp.K; k += blockDim.x) {
@@ -404,8 +404,10 @@ __global__ __launch_bounds__(kT) void k_moddown_conv_r(const uint64_t* __restric
       if (k < (int)mp.K) {
         const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(pc + (size_t)k * mp.N + n);
         const uint64_t v0 = shoup(v.x, inv[k], invsh[k], ps[k]), v1 = shoup(v.y, inv[k], invsh[k], ps[k]);
-        lift0 += v0 > (ps[k] >> 1);
-        lift1 += v1 > (ps[k] >> 1);
+        if constexpr (RN) {
+          lift0 += v0 > (ps[k] >> 1);
+          lift1 += v1 > (ps[k] >> 1);
+        }
         y0[k] = M::prep(v0);
         y1[k] = M::prep(v1);
       } else {
@@ -424,7 +426,7 @@ __global__ __launch_bounds__(kT) void k_moddown_conv_r(const uint64_t* __restric
         a0.mac(y0[k], c);
         a1.mac(y1[k], c);
       }
-      if (rn) {
+      if constexpr (RN) {
         a0.mac(l0, s_negp[i]);
         a1.mac(l1, s_negp[i]);
       }
@@ -504,8 +506,7 @@ void modup_batch(Context& c, const uint64_t* x, int l, int npolys, uint64_t x_st
     ProfScope ps_(c, "modup_conv", st);
     const dim3 g2 = grid_n(c.N, beta * npolys, 2);
     if (!c.redc_conv_ok || c.alpha > 16) k_modup_conv<<<grid_n(c.N, beta * npolys), kT, 0, st>>>(tmp.p, x, digits, mp);
-    else if (c.mac30_ok()) FH_SZ_DISPATCH(c.alpha, k_modup_conv_r, Mac30, (g2, kT, 0, st), (tmp.p, x, digits, mp))
-    else FH_SZ_DISPATCH(c.alpha, k_modup_conv_r, MacWide, (g2, kT, 0, st), (tmp.p, x, digits, mp))
+    else FH_CONV_DISPATCH(c, c.alpha, k_modup_conv_r, (g2, kT, 0, st), (tmp.p, x, digits, mp))
   }
   FH_HIP(hipGetLastError());
   ModUpAddr a{digits, ext, lq, (uint32_t)c.alpha, (uint32_t)c.L1, beta, (uint32_t)npolys, (uint64_t)beta * ext * c.N};
@@ -563,8 +564,7 @@ void moddown(Context& c, const uint64_t* x, int l, uint64_t* out, int npolys, ui
     ProfScope ps_(c, "moddown_conv", st);
     const dim3 g2 = grid_n(c.N, npolys, 2);
     if (!c.redc_conv_ok || K > 16) k_moddown_conv<<<grid_n(c.N, npolys), kT, 0, st>>>(pc.p, conv.p, mp);
-    else if (c.mac30_ok()) FH_SZ_DISPATCH((int)K, k_moddown_conv_r, Mac30, (g2, kT, 0, st), (pc.p, conv.p, mp))
-    else FH_SZ_DISPATCH((int)K, k_moddown_conv_r, MacWide, (g2, kT, 0, st), (pc.p, conv.p, mp))
+    else FH_CONV_DISPATCH(c, (int)K, k_moddown_conv_r, (g2, kT, 0, st), (pc.p, conv.p, mp))
   }
   FH_HIP(hipGetLastError());
   // forward NTT of the conversion with (x_i - conv_i) * P^-1 fused into its last launch
