@@ -604,17 +604,25 @@ int fh_serve_run(fh_context* ctx, const fh_serve_config* config, const fh_key* c
   });
 }
 
-static int* tune_slot(fh::Tune& t, const char* name) {
+// tunable slot and its smallest accepted value
+static int* tune_slot(fh::Tune& t, const char* name, int* min_value = nullptr) {
   if (!name) throw fh::Error(FH_E_INVALID, "null tunable name");
-  if (!std::strcmp(name, "ntt_chunk_mb")) return &t.ntt_chunk_mb;
-  throw fh::Error(FH_E_INVALID, std::string("unknown tunable: ") + name);
+  int lo = 1;
+  int* slot = nullptr;
+  if (!std::strcmp(name, "ntt_chunk_mb")) slot = &t.ntt_chunk_mb;
+  else if (!std::strcmp(name, "ntt_mix")) slot = &t.ntt_mix, lo = 0;
+  if (!slot) throw fh::Error(FH_E_INVALID, std::string("unknown tunable: ") + name);
+  if (min_value) *min_value = lo;
+  return slot;
 }
 
 int fh_tune_set(fh_context* ctx, const char* name, int value) {
   return guard([&] {
     auto& c = C(ctx);
-    if (value < 1) throw fh::Error(FH_E_INVALID, "tunable values must be >= 1");
-    *tune_slot(c.tune, name) = value;
+    int lo = 1;
+    int* slot = tune_slot(c.tune, name, &lo);
+    if (value < lo) throw fh::Error(FH_E_INVALID, std::string("tunable ") + name + " must be >= " + std::to_string(lo));
+    *slot = value;
   });
 }
 

@@ -397,6 +397,29 @@ void sort_by_prime(std::vector<LimbJob>& jobs, std::vector<T>* comp) {
   }
 }
 
+// Spread the jobs of integer-arithmetic (60-bit) primes evenly among the fp64 ones, keeping the
+// prime-major order within each class (jobs, comp: as after sort_by_prime).
+template <class T>
+void mix_prime_classes(const Context& c, std::vector<LimbJob>& jobs, std::vector<T>* comp) {
+  std::vector<size_t> a, b;
+  for (size_t i = 0; i < jobs.size(); ++i) (c.fp64[jobs[i].pid] ? a : b).push_back(i);
+  if (a.empty() || b.empty()) return;
+  const size_t total = jobs.size(), nb = b.size();
+  std::vector<size_t> idx(total);
+  for (size_t y = 0; y < total; ++y) {
+    const size_t kb = y * nb / total, kb1 = (y + 1) * nb / total;
+    idx[y] = kb1 > kb ? b[kb] : a[y - kb];
+  }
+  std::vector<LimbJob> j2(total);
+  for (size_t y = 0; y < total; ++y) j2[y] = jobs[idx[y]];
+  jobs.swap(j2);
+  if (comp) {
+    std::vector<T> c2(total);
+    for (size_t y = 0; y < total; ++y) c2[y] = (*comp)[idx[y]];
+    comp->swap(c2);
+  }
+}
+
 // byte-addressed table builder: sections aligned to 64 bytes, uploaded in one copy
 struct TableBuilder {
   std::vector<uint8_t> bytes;
@@ -806,6 +829,14 @@ void Batch::finalize() {
   sort_by_prime(md_ntt, &md_epi);
   sort_by_prime(rs_in, &rs_out);
   sort_by_prime(rs_ntt, &rs_epi);
+  if (c.tune.ntt_mix) {
+    mix_prime_classes(c, ks_in, &ks_out);
+    mix_prime_classes<LimbJob>(c, modup_ntt, nullptr);
+    mix_prime_classes(c, md_in, &md_out);
+    mix_prime_classes(c, md_ntt, &md_epi);
+    mix_prime_classes(c, rs_in, &rs_out);
+    mix_prime_classes(c, rs_ntt, &rs_epi);
+  }
 
   TableBuilder tb;
   const size_t o_mulpt = tb.put(pj);

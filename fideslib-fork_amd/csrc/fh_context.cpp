@@ -251,6 +251,7 @@ Context::Context(int logN_, int L1_, int K_, int dnum_, const std::vector<uint64
     for (int j = 0; j < i; ++j) if (q[j] == q[i]) throw Error(-1, "duplicate modulus");
   }
   psi.resize(np);
+  fp64.assign(np, 0);
   std::vector<PrimeConst> pc(np);
   std::vector<uint64_t> tw((size_t)np * N), twsh((size_t)np * N), itw((size_t)np * N), itwsh((size_t)np * N);
   std::vector<double> twd((size_t)np * N), itwd((size_t)np * N);
@@ -270,6 +271,7 @@ Context::Context(int logN_, int L1_, int K_, int dnum_, const std::vector<uint64
     P.qinv = 1.0 / (double)qi;
     // fp64 NTT: every intermediate stays below 2^53 when q < 2^50.2 (values are bounded by ~3.7q)
     P.fp64 = ((double)qi < 1.1487 * (double)(1ull << 50)) ? 1u : 0u;
+    fp64[i] = (uint8_t)P.fp64;
     P.ninv_d = P.ninv > qi / 2 ? (double)P.ninv - (double)qi : (double)P.ninv;
     P.mont_ninv = (uint64_t)0 - h_inv_odd(qi, 0);  // modulus 2^64: mask of m-1 = all ones
     P.mont_r2 = (uint64_t)(((((u128)1 << 64) % qi) * (((u128)1 << 64) % qi)) % qi);

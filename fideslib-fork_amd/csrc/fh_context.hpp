@@ -106,6 +106,10 @@ struct Tune {
   // limb data per NTT launch pair, in MB. Measured at N=2^16 (3200 limbs): 96 MB chunks take 3.97 ms
   // per forward+inverse, one launch pair over all limbs 3.23 ms; launch tails outweigh MALL reuse.
   int ntt_chunk_mb = 1 << 16;
+  // 1: spread the NTT jobs of 60-bit primes (integer Shoup arithmetic, VALU-heavy) evenly among
+  // the jobs of 50-bit primes (fp64 arithmetic, memory-bound) in ModUp target and batch launches,
+  // instead of running them as one prime-major block at the end.
+  int ntt_mix = 0;
 };
 
 struct Context;
@@ -124,6 +128,7 @@ struct Context {
   uint32_t N = 0;
   int L1 = 0, K = 0, dnum = 0, alpha = 0, np = 0;
   std::vector<uint64_t> q, psi;
+  std::vector<uint8_t> fp64;   // per prime: 1 when its NTT runs in fp64 arithmetic
 
   // device tables
   PrimeConst* d_prime = nullptr;   // [np]

@@ -76,9 +76,16 @@ struct ModUpAddr {
   uint32_t ext, lq, alpha, L1, beta;
   uint32_t npolys;
   uint64_t poly_stride;   // words between consecutive polynomials' digit blocks
+  uint32_t mix = 0;       // 1: spread the P-position jobs evenly among the Q-position jobs
   __device__ __forceinline__ void at(uint32_t y, uint32_t N, uint64_t*& p, int& pid) const {
     const uint32_t qjobs = npolys * (beta - 1);  // jobs per Q position
     const uint32_t y1 = qjobs * lq;
+    if (mix) {
+      // position y holds a P job iff the proportional P count steps between y and y + 1
+      const uint64_t total = (uint64_t)npolys * (beta * ext - lq), nb = total - y1;
+      const uint32_t kb = (uint32_t)(((uint64_t)y * nb) / total), kb1 = (uint32_t)(((uint64_t)(y + 1) * nb) / total);
+      y = kb1 > kb ? y1 + kb : y - kb;
+    }
     uint32_t pos, r, d;
     if (y < y1) {
       pos = y / qjobs;

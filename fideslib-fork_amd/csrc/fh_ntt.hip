@@ -737,7 +737,11 @@ void ntt_fwd_jobs_rescale(const Context& c, const LimbJob* jobs, const EpiJob* e
   EpiRescaleJobs e{epi, c.d_rs_qlinv, c.d_rs_qlinv_sh, (uint32_t)c.L1, nullptr, nullptr, 0, 0, 0};
   dispatch(c, JobAddr{jobs}, JobAddr{jobs}, n, st, true, e, "ntt_fwd_rows_rescale");
 }
-void ntt_fwd_modup(const Context& c, const ModUpAddr& a, uint32_t n, hipStream_t st) { dispatch(c, a, a, n, st, true); }
+void ntt_fwd_modup(const Context& c, const ModUpAddr& a, uint32_t n, hipStream_t st) {
+  ModUpAddr m = a;
+  m.mix = c.tune.ntt_mix ? 1u : 0u;
+  dispatch(c, m, m, n, st, true);
+}
 
 void ntt_fwd_moddown(const Context& c, const LimbSet& conv, const uint64_t* x, uint64_t x_stride, uint64_t* out, uint64_t out_stride,
                      hipStream_t st) {

@@ -14,14 +14,16 @@ pytestmark = pytest.mark.gpu
 CFG = dict(logN=12, L1=8, dnum=3)
 
 
-@pytest.fixture(scope="module")
-def env(oracle_mod, fh):
+# ntt_mix 0: prime-major NTT job tables; 1: jobs of 60-bit primes spread among the 50-bit ones
+@pytest.fixture(scope="module", params=[0, 1], ids=["prime_major", "mixed_classes"])
+def env(request, oracle_mod, fh):
     O = oracle_mod
     K = 2
     primes = O.gen_primes(CFG["logN"], CFG["L1"], 60, 50, K)
     o = O.Ctx(CFG["logN"], primes, CFG["L1"], K, CFG["dnum"])
     g = fh.Context(CFG["logN"], CFG["L1"], K, CFG["dnum"])
     assert [int(x) for x in g.moduli] == [int(x) for x in primes]
+    g.tune_set("ntt_mix", request.param)
     yield o, g
     g.close()
 

@@ -34,6 +34,24 @@ def test_modup(cfgname, level, gctx1, gctx2, octx1, octx2):
     d.free(); dig.free()
 
 
+@pytest.mark.parametrize("cfgname,level", [("cfg1", 3), ("cfg2", 24), ("cfg2", 17), ("cfg2", 3)])
+def test_modup_mixed_job_order(cfgname, level, gctx1, gctx2, octx1, octx2):
+    """ModUp with the target NTT jobs of 60-bit primes spread among the 50-bit ones (ntt_mix=1)."""
+    g, o = (gctx1, octx1) if cfgname == "cfg1" else (gctx2, octx2)
+    x = o.random_poly(60 + level, _ids(o, level))
+    beta = o.num_digits(level)
+    d = g.to_device(x)
+    dig = g.array((beta * (level + 1 + o.K), o.N))
+    g.tune_set("ntt_mix", 1)
+    try:
+        g.modup(d, level, dig)
+        g.sync()
+    finally:
+        g.tune_set("ntt_mix", 0)
+    assert np.array_equal(dig.download(), o.modup(x, level))
+    d.free(); dig.free()
+
+
 @pytest.mark.parametrize("cfgname,level,galois,with_c0", [("cfg1", 3, 1, False), ("cfg1", 2, 5, True), ("cfg2", 24, 25, True), ("cfg2", 13, 1, False)])
 def test_key_inner_product(cfgname, level, galois, with_c0, gctx1, gctx2, octx1, octx2):
     g, o = (gctx1, octx1) if cfgname == "cfg1" else (gctx2, octx2)
@@ -223,9 +241,12 @@ def cfg1_rot_keys(octx1, cfg1_material):
     return steps, ks
 
 
-def test_rotate_batch_matches_per_op(gctx1, octx1, cfg1_material, cfg1_rot_keys):
-    """fh_rotate_batch over 7 ciphertexts with repeated keys == per-ciphertext oracle rotations."""
+@pytest.mark.parametrize("mix", [0, 1])
+def test_rotate_batch_matches_per_op(mix, gctx1, octx1, cfg1_material, cfg1_rot_keys):
+    """fh_rotate_batch over 7 ciphertexts with repeated keys == per-ciphertext oracle rotations
+    (prime-major NTT job order, and with 60-bit-prime jobs spread among the others)."""
     g, o, m = gctx1, octx1, cfg1_material
+    g.tune_set("ntt_mix", mix)
     steps, ks = cfg1_rot_keys
     l = 2
     ids = list(range(l + 1))
@@ -235,7 +256,11 @@ def test_rotate_batch_matches_per_op(gctx1, octx1, cfg1_material, cfg1_rot_keys)
     dk = {r: g.key(ks[r]) for r in steps}
     d = g.to_device(np.concatenate(cts))
     out = g.array((B * 2 * (l + 1), o.N))
-    g.rotate_batch(d, B, l, [dk[r] for r in pick], [o.rot_index(r) for r in pick], out)
+    try:
+        g.rotate_batch(d, B, l, [dk[r] for r in pick], [o.rot_index(r) for r in pick], out)
+        g.sync()
+    finally:
+        g.tune_set("ntt_mix", 0)
     got = out.download().reshape(B, 2 * (l + 1), o.N)
     for b in range(B):
         exp = o.eval_rotate(cts[b], l, ks[pick[b]], o.rot_index(pick[b]))
