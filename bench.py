@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--serve-rates", type=str, default="50,150,400", help="arrival rates (requests/s) of the serving sweep")
     ap.add_argument("--copy-mb", type=int, default=256, help="device copy of this size first (counter calibration; 0 = skip)")
     ap.add_argument("--tune", action="append", default=[], metavar="NAME=VALUE", help="set a context tunable (repeatable)")
+    ap.add_argument("--autotune-polys", type=int, default=32,
+                    help="run the NTT autotuner on this many synthetic polynomials before timing (0 = keep the defaults)")
     return ap.parse_args()
 
 
@@ -74,12 +76,12 @@ def main():
     import fides_hip as fh
 
     ctx = fh.Context(LOGN, L1, K, DNUM)
-    tunables = {}
+    autotune = ctx.autotune(a.autotune_polys) if a.autotune_polys > 0 else None
     for kv in a.tune:
         name, value = kv.split("=", 1)
         ctx.tune_set(name, int(value))
-    for name in ("ntt_chunk_mb", "ntt_mix"):
-        tunables[name] = ctx.tune_get(name)
+    tunables = {name: ctx.tune_get(name) for name in ("ntt_split", "ntt_fwd_cols_threads", "ntt_fwd_rows_threads", "ntt_inv_rows_threads",
+                                                      "ntt_inv_cols_threads", "ntt_chunk_mb", "ntt_mix")}
     N, B, lvl = ctx.N, a.batch, L1 - 1
     lq, ext, beta = lvl + 1, lvl + 1 + K, ctx.num_digits(lvl)
     ct_words = 2 * lq * N
@@ -251,7 +253,7 @@ def main():
         "data": "synthetic (uniform residues and keys generated on the device)",
         "config": {"workload": "batched hybrid key-switch rotation, N=2^16, 25 limbs, dnum=3, K=9, 64 ciphertexts, 16 keyed steps",
                    "ring_dim": N, "limbs": lq, "dnum": DNUM, "special_primes": K, "batch": B, "keyed_steps": len(STEPS16),
-                   "tunables": tunables},
+                   "tunables": tunables, "autotune": autotune},
         "roofline": roof,
         "cpu_baseline": cpu,
         "per_op_rotations_per_s": round(per_op_rot_s, 1),

@@ -609,8 +609,13 @@ static int* tune_slot(fh::Tune& t, const char* name, int* min_value = nullptr) {
   if (!name) throw fh::Error(FH_E_INVALID, "null tunable name");
   int lo = 1;
   int* slot = nullptr;
+  static const char* const kThreadNames[4] = {"ntt_fwd_cols_threads", "ntt_fwd_rows_threads", "ntt_inv_rows_threads",
+                                               "ntt_inv_cols_threads"};
   if (!std::strcmp(name, "ntt_chunk_mb")) slot = &t.ntt_chunk_mb;
   else if (!std::strcmp(name, "ntt_mix")) slot = &t.ntt_mix, lo = 0;
+  else if (!std::strcmp(name, "ntt_split")) slot = &t.ntt_split, lo = 0;
+  for (int i = 0; i < 4 && !slot; ++i)
+    if (!std::strcmp(name, kThreadNames[i])) slot = &t.ntt_threads[i], lo = 256;
   if (!slot) throw fh::Error(FH_E_INVALID, std::string("unknown tunable: ") + name);
   if (min_value) *min_value = lo;
   return slot;
@@ -622,7 +627,27 @@ int fh_tune_set(fh_context* ctx, const char* name, int value) {
     int lo = 1;
     int* slot = tune_slot(c.tune, name, &lo);
     if (value < lo) throw fh::Error(FH_E_INVALID, std::string("tunable ") + name + " must be >= " + std::to_string(lo));
+    if (!std::strcmp(name, "ntt_mix") && value > 1) throw fh::Error(FH_E_INVALID, "ntt_mix must be 0 or 1");
+    if (!std::strcmp(name, "ntt_split") && value != 0 && !fh::ntt_split_ok(c.logN, value))
+      throw fh::Error(FH_E_INVALID, "ntt_split " + std::to_string(value) + " is not built for logN " + std::to_string(c.logN));
+    if (lo == 256 && value != 256 && value != 512) throw fh::Error(FH_E_INVALID, std::string(name) + " must be 256 or 512");
+    if (!std::strcmp(name, "ntt_split")) c.ensure_twists(value ? value : fh::ntt_default_split(c.logN));
     *slot = value;
+  });
+}
+
+int fh_autotune(fh_context* ctx, int polys, fh_autotune_report* report) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(report, "report");
+    const fh::AutotuneResult r = fh::autotune_ntt(c, polys);
+    report->split = r.split;
+    for (int i = 0; i < 4; ++i) report->threads[i] = r.threads[i];
+    report->chunk_mb = r.chunk_mb;
+    report->candidates = r.candidates;
+    report->limbs = r.limbs;
+    report->ms_default = r.ms_default;
+    report->ms_best = r.ms_best;
   });
 }
 

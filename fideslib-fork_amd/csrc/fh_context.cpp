@@ -235,6 +235,45 @@ static uint32_t bitrev(uint32_t x, int bits) {
   return r;
 }
 
+void Context::ensure_twists(int l1) {
+  if (l1 < 0 || l1 > 8 || l1 >= logN) throw Error(-1, "radix split out of range");
+  TwistSet& t = twists[l1];
+  if (t.tw) return;
+  // tau_r = psi^e_r, e_r = 1 + 2 brev_l1(r) - S1 (mod 2N)
+  const uint64_t m = 2ull * N;
+  const uint32_t S1 = 1u << l1, S2 = N >> l1;
+  std::vector<uint64_t> ts((size_t)np * N), tssh(ts.size()), its(ts.size()), itssh(ts.size());
+  std::vector<double> tsd(ts.size()), itsd(ts.size());
+  for (int i = 0; i < np; ++i) {
+    const uint64_t qi = q[i];
+    for (uint32_t r = 0; r < S1; ++r) {
+      const int64_t e = (1 + 2 * (int64_t)bitrev(r, l1) - (int64_t)S1) % (int64_t)m;
+      const uint64_t eu = (uint64_t)(e < 0 ? e + (int64_t)m : e);
+      const uint64_t step = h_powmod(psi[i], eu, qi), istep = h_powmod(psi[i], (m - eu) % m, qi);
+      uint64_t x = 1, xi = 1;
+      for (uint32_t c = 0; c < S2; ++c) {
+        const size_t o = (size_t)i * N + (size_t)r * S2 + c;
+        ts[o] = x;
+        its[o] = xi;
+        tssh[o] = h_shoup(x, qi);
+        itssh[o] = h_shoup(xi, qi);
+        tsd[o] = x > qi / 2 ? (double)x - (double)qi : (double)x;
+        itsd[o] = xi > qi / 2 ? (double)xi - (double)qi : (double)xi;
+        x = h_mulmod(x, step, qi);
+        xi = h_mulmod(xi, istep, qi);
+      }
+    }
+  }
+  TwistSet n;
+  n.tw = upload(ts);
+  n.tw_sh = upload(tssh);
+  n.itw = upload(its);
+  n.itw_sh = upload(itssh);
+  n.twd = upload(tsd);
+  n.itwd = upload(itsd);
+  t = n;
+}
+
 Context::Context(int logN_, int L1_, int K_, int dnum_, const std::vector<uint64_t>& primes)
     : logN(logN_), N(1u << logN_), L1(L1_), K(K_), dnum(dnum_) {
   if (logN < 10 || logN > 16) throw Error(-1, "logN must be in [10, 16]");
@@ -298,39 +337,7 @@ Context::Context(int logN_, int L1_, int K_, int dnum_, const std::vector<uint64
   d_itw_sh = upload(itwsh);
   d_twd = upload(twd);
   d_itwd = upload(itwd);
-  {
-    // row twists (see the header): tau_r = psi^e_r, e_r = 1 + 2 brev_l1(r) - S1 (mod 2N)
-    ntt_l1 = logN / 2;
-    const uint32_t S1 = 1u << ntt_l1, S2 = N >> ntt_l1;
-    std::vector<uint64_t> ts((size_t)np * N), tssh(ts.size()), its(ts.size()), itssh(ts.size());
-    std::vector<double> tsd(ts.size()), itsd(ts.size());
-    for (int i = 0; i < np; ++i) {
-      const uint64_t qi = q[i];
-      for (uint32_t r = 0; r < S1; ++r) {
-        const int64_t e = (1 + 2 * (int64_t)bitrev(r, ntt_l1) - (int64_t)S1) % (int64_t)m;
-        const uint64_t eu = (uint64_t)(e < 0 ? e + (int64_t)m : e);
-        const uint64_t step = h_powmod(psi[i], eu, qi), istep = h_powmod(psi[i], (m - eu) % m, qi);
-        uint64_t x = 1, xi = 1;
-        for (uint32_t c = 0; c < S2; ++c) {
-          const size_t o = (size_t)i * N + (size_t)r * S2 + c;
-          ts[o] = x;
-          its[o] = xi;
-          tssh[o] = h_shoup(x, qi);
-          itssh[o] = h_shoup(xi, qi);
-          tsd[o] = x > qi / 2 ? (double)x - (double)qi : (double)x;
-          itsd[o] = xi > qi / 2 ? (double)xi - (double)qi : (double)xi;
-          x = h_mulmod(x, step, qi);
-          xi = h_mulmod(xi, istep, qi);
-        }
-      }
-    }
-    d_twist = upload(ts);
-    d_twist_sh = upload(tssh);
-    d_itwist = upload(its);
-    d_itwist_sh = upload(itssh);
-    d_twistd = upload(tsd);
-    d_itwistd = upload(itsd);
-  }
+  ensure_twists(logN / 2);
 
   // ---- ModUp tables (OpenFHE PartQlHatInvModq / PartQlHatModp) ----
   const int E = L1 + K;

@@ -110,6 +110,11 @@ struct Tune {
   // the jobs of 50-bit primes (fp64 arithmetic, memory-bound) in ModUp target and batch launches,
   // instead of running them as one prime-major block at the end.
   int ntt_mix = 0;
+  // NTT radix split: log2 of the column-pass length (0: floor(logN / 2)); see ntt_split_ok
+  int ntt_split = 0;
+  // workgroup size (256 or 512 threads) of the forward column, forward row, inverse row and
+  // inverse column passes; 512 applies with the default split only
+  int ntt_threads[4] = {256, 256, 256, 256};
 };
 
 struct Context;
@@ -138,16 +143,20 @@ struct Context {
   uint64_t* d_itw_sh = nullptr;
   double* d_twd = nullptr;         // [np][N] twiddles as doubles (fp64 NTT path, q < 2^51)
   double* d_itwd = nullptr;
-  // Row twists of the two-pass NTT (N = S1 x S2, S1 = 2^ntt_l1): after the column stages, row r
-  // is finished by a standard S2-point negacyclic transform of x_c tau_r^c, with
-  // tau_r = psi^(1 + 2 brev_l1(r) - S1); twist[r S2 + c] = tau_r^c, itwist its inverse.
-  int ntt_l1 = 0;
-  uint64_t* d_twist = nullptr;     // [np][N]
-  uint64_t* d_twist_sh = nullptr;
-  uint64_t* d_itwist = nullptr;
-  uint64_t* d_itwist_sh = nullptr;
-  double* d_twistd = nullptr;      // centred doubles (fp64 path)
-  double* d_itwistd = nullptr;
+  // Row twists of the two-pass NTT for radix split l1 (N = S1 x S2, S1 = 2^l1): after the column
+  // stages, row r is finished by a standard S2-point negacyclic transform of x_c tau_r^c, with
+  // tau_r = psi^(1 + 2 brev_l1(r) - S1); tw[r S2 + c] = tau_r^c, itw its inverse. Built at
+  // creation for the default split, on first use for another (ensure_twists).
+  struct TwistSet {
+    uint64_t* tw = nullptr;        // [np][N]
+    uint64_t* tw_sh = nullptr;
+    uint64_t* itw = nullptr;
+    uint64_t* itw_sh = nullptr;
+    double* twd = nullptr;         // centred doubles (fp64 path)
+    double* itwd = nullptr;
+  };
+  TwistSet twists[9];
+  void ensure_twists(int l1);
 
   // ModUp: for level l and digit d, source prescale [Qhat_i^-1]_{q_i} (+Shoup), and
   // [Qhat_i]_{p_j} for every extended position j of level l.

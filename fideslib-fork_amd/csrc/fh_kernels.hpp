@@ -125,9 +125,21 @@ struct NttTables {
   uint32_t logN;
 };
 
-inline NttTables ntt_tables(const Context& c) {
-  return NttTables{c.d_prime, c.d_tw, c.d_tw_sh, c.d_itw, c.d_itw_sh, c.d_twd, c.d_itwd, c.d_twist, c.d_twist_sh,
-                   c.d_itwist, c.d_itwist_sh, c.d_twistd, c.d_itwistd, c.N, (uint32_t)c.logN};
+// tables of the two-pass NTT at radix split l1 (its row twists must have been built)
+inline NttTables ntt_tables(const Context& c, int l1) {
+  const Context::TwistSet& t = c.twists[l1];
+  if (!t.tw) throw Error(-1, "row twists of this radix split are not built");
+  return NttTables{c.d_prime, c.d_tw, c.d_tw_sh, c.d_itw, c.d_itw_sh, c.d_twd, c.d_itwd, t.tw, t.tw_sh,
+                   t.itw, t.itw_sh, t.twd, t.itwd, c.N, (uint32_t)c.logN};
+}
+
+// NTT radix splits N = 2^l1 (column points) x 2^(logN - l1) (row points) the kernels are built
+// for: both factors in [2^4, 2^8] and within a factor 4 of each other; floor(logN / 2) is the
+// default.
+inline int ntt_default_split(int logN) { return logN / 2; }
+inline bool ntt_split_ok(int logN, int l1) {
+  const int l2 = logN - l1;
+  return l1 >= 4 && l1 <= 8 && l2 >= 4 && l2 <= 8 && (l1 - l2 <= 2) && (l2 - l1 <= 2);
 }
 
 inline LimbSet make_set(uint64_t* base, uint32_t nlimbs, const int* pids, uint32_t npolys = 1, uint64_t poly_stride = 0) {
@@ -172,6 +184,20 @@ void ntt_inv(const Context& c, const LimbSet& s, hipStream_t st);
 void ntt_fwd(const Context& c, const LimbSet& in, const LimbSet& out, hipStream_t st);  // out-of-place
 void ntt_inv(const Context& c, const LimbSet& in, const LimbSet& out, hipStream_t st);
 void ntt_fwd_modup(const Context& c, const ModUpAddr& a, uint32_t n, hipStream_t st);
+
+// NTT launch-shape autotuner: times forward + inverse transforms of `polys` synthetic
+// polynomials over every prime of the context under candidate radix splits, per-pass workgroup
+// sizes and limbs per launch pair, keeps the fastest in c.tune and reports it.
+struct AutotuneResult {
+  int split;            // log2 of the column-pass length
+  int threads[4];       // forward cols, forward rows, inverse rows, inverse cols
+  int chunk_mb;         // limb data per launch pair
+  int candidates;       // configurations timed
+  int limbs;            // limbs per timed transform
+  double ms_default;    // forward + inverse at the default shape
+  double ms_best;       // forward + inverse at the chosen shape
+};
+AutotuneResult autotune_ntt(Context& c, int polys);
 void ntt_fwd_jobs(const Context& c, const LimbJob* d_jobs, uint32_t njobs, hipStream_t st);
 // Forward NTT of the ModDown conversion (conv: npolys x lq limbs over Q_l, coefficient form, used as
 // scratch) fused with the ModDown combination: out_i = (x_i - NTT(conv)_i) * [P^-1]_{q_i}.

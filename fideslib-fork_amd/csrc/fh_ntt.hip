@@ -1,715 +1,9 @@
-// Negacyclic NTT / INTT over 64-bit RNS limbs for gfx950.
-//
-// The transform is OpenFHE's: forward Cooley-Tukey with the bit-reversed table
-// tw[k] = psi^brev(k) (stage m = 1..N/2 uses tw[m + i] for group i), output in
-// bit-reversed EVALUATION order; inverse Gentleman-Sande with psi^-brev(k) and
-// a final N^-1. Reference: src/NTT.cu:42-310 (INTT), 331-756 (NTT),
-// src/CKKS/LimbPartition.cu:433-550 (launch), src/NTThelper.cuh:21-72 (butterflies).
-//
-// MI355X design
-//  * N = S1 * S2 (S1 = 2^l1 "column" points, S2 = 2^l2 "row" points). The first
-//    l1 stages only couple elements of one column (stride S2), the last l2
-//    stages only elements of one row (contiguous), so a limb is transformed by
-//    two launches. Row rho's stages equal a standard S2-point negacyclic
-//    transform of x_c tau_rho^c (tau_rho = psi^(1 + 2 brev(rho) - S1)), so the row
-//    launch multiplies by a per-element twist table once and then uses the
-//    same S2 twiddles (kept in LDS) for every row, like the column launch. Each lane owns 16 elements of one sub-transform: four radix-2
-//    stages run in registers, one LDS exchange re-partitions the sub-transform,
-//    the remaining stages run in registers again. Global loads and stores are
-//    coalesced in both passes (rows are staged through LDS where the register
-//    layout is not).
-//  * Two exact arithmetics, chosen per limb by its prime:
-//      - fp64 (q < 2^50.2, the 50-bit scaling primes): residues are doubles in a
-//        signed lazy range; a product is split exactly with FMA (h = yw,
-//        l = fma(y, w, -h)), the quotient is rint(h / q) and the remainder
-//        fma(-c, q, h) + l is exact. gfx950 issues fp64 FMA at full rate, about
-//        three times the rate of a 64-bit Shoup product built from 32-bit
-//        multiplies. Values are re-centred into (-q/2, q/2] every few stages so
-//        every intermediate stays an exact integer below 2^53.
-//      - integer Harvey-lazy Shoup for the 60-bit primes: forward values in
-//        [0, 4q), inverse values in [0, 2q).
-//    Both are exact, so the canonical outputs are identical.
-//  * Between the two launches the limb holds an internal form (lazy integers or
-//    doubles). A batch may be split into chunks of limbs (tunable "ntt_chunk_mb")
-//    so that the intermediate is still in the MALL when the second launch reads
-//    it back; by default one launch pair covers the whole batch.
-//  * The last forward launch hands its outputs to an epilogue, which fuses the
-//    element-wise step that consumes them (ModDown, Rescale).
-#include <algorithm>
-
-#include "fh_kernels.hpp"
+// NTT / INTT entry points over limb sets (in place and out of place, with the ModDown and Rescale
+// epilogues), and the NTT launch-shape autotuner. The kernels are in fh_ntt_impl.hpp.
+#include "fh_ntt_impl.hpp"
+#include "fh_ops.hpp"
 
 namespace fh {
-
-namespace {
-
-constexpr int kThreads = 256;
-
-// ------------------------------------------------------------------------------- integer arithmetic
-struct IntA {
-  using T = uint64_t;
-  uint64_t q, q2;
-  struct W {
-    uint64_t w, sh;
-  };
-  __device__ __forceinline__ W tw(const uint64_t* t, const uint64_t* tsh, uint32_t i) const { return W{t[i], tsh[i]}; }
-  __device__ __forceinline__ T from_canon(uint64_t v) const { return v; }
-  __device__ __forceinline__ T from_mid(uint64_t v) const { return v; }
-  __device__ __forceinline__ uint64_t to_mid(T v) const { return v; }
-  __device__ __forceinline__ void fwd(T& X, T& Y, W w) const {
-    uint64_t x = X >= q2 ? X - q2 : X;         // [0, 2q)
-    uint64_t t = shoup_lazy(Y, w.w, w.sh, q);  // [0, 2q)
-    X = x + t;                                 // [0, 4q)
-    Y = x - t + q2;                            // [0, 4q)
-  }
-  __device__ __forceinline__ void inv(T& X, T& Y, W w) const {
-    uint64_t u = X + Y;                        // X, Y in [0, 2q)
-    u = u >= q2 ? u - q2 : u;
-    uint64_t v = X - Y + q2;                   // [0, 4q)
-    X = u;
-    Y = shoup_lazy(v, w.w, w.sh, q);           // [0, 2q)
-  }
-  __device__ __forceinline__ void recenter(T&) const {}
-  // x * w for any x < 2^64, result in [0, 2q)
-  __device__ __forceinline__ T mul(T x, W w) const { return shoup_lazy(x, w.w, w.sh, q); }
-  __device__ __forceinline__ uint64_t canon(T x) const {
-    x = x >= q2 ? x - q2 : x;
-    return x >= q ? x - q : x;
-  }
-};
-
-// ------------------------------------------------------------------------------- fp64 arithmetic
-struct FpA {
-  using T = double;
-  using W = double;
-  double q, qinv, half;
-  __device__ __forceinline__ W tw(const double* t, const double*, uint32_t i) const { return t[i]; }
-  // y * w mod q for |y| < 4q, |w| <= q/2: exact, result in about (-1.3q, 1.3q)
-  __device__ __forceinline__ double mulmod(double y, double w) const {
-    const double h = y * w;
-    const double l = fma(y, w, -h);
-    const double c = rint(h * qinv);
-    const double d = fma(-c, q, h);
-    return d + l;
-  }
-  __device__ __forceinline__ T from_canon(uint64_t v) const {
-    const double x = (double)v;
-    return x > half ? x - q : x;
-  }
-  __device__ __forceinline__ T from_mid(uint64_t v) const { return __longlong_as_double((long long)v); }
-  __device__ __forceinline__ uint64_t to_mid(T v) const { return (uint64_t)__double_as_longlong(v); }
-  __device__ __forceinline__ void fwd(T& X, T& Y, W w) const {
-    const double t = mulmod(Y, w);
-    Y = X - t;
-    X = X + t;
-  }
-  __device__ __forceinline__ void inv(T& X, T& Y, W w) const {
-    const double u = X + Y, v = X - Y;
-    X = u;
-    Y = mulmod(v, w);
-  }
-  __device__ __forceinline__ void recenter(T& x) const { x = fma(-rint(x * qinv), q, x); }
-  // x * w re-centred into [-q/2, q/2]
-  __device__ __forceinline__ T mul(T x, W w) const {
-    T r = mulmod(x, w);
-    recenter(r);
-    return r;
-  }
-  __device__ __forceinline__ uint64_t canon(T x) const {
-    recenter(x);
-    x = x < 0 ? x + q : x;
-    x = x >= q ? x - q : x;
-    return (uint64_t)x;
-  }
-};
-
-// per-limb arithmetic selection
-struct Sel {
-  bool fp;
-  IntA ia;
-  FpA fa;
-  const uint64_t *tw, *twsh;
-  const double* twd;
-  const uint64_t *ts, *tssh;   // row twists of the limb's prime (forward: tau^c, inverse: tau^-c)
-  const double* tsd;
-};
-
-__device__ __forceinline__ Sel select(const NttTables& tb, int pid, bool inverse) {
-  const PrimeConst& P = tb.prime[pid];
-  Sel s;
-  s.fp = P.fp64 != 0;
-  s.ia = IntA{P.q, 2 * P.q};
-  s.fa = FpA{P.qd, P.qinv, 0.5 * P.qd};
-  const size_t o = (size_t)pid * tb.N;
-  s.tw = (inverse ? tb.itw : tb.tw) + o;
-  s.twsh = (inverse ? tb.itw_sh : tb.tw_sh) + o;
-  s.twd = (inverse ? tb.itwd : tb.twd) + o;
-  s.ts = (inverse ? tb.itwist : tb.twist) + o;
-  s.tssh = (inverse ? tb.itwist_sh : tb.twist_sh) + o;
-  s.tsd = (inverse ? tb.itwistd : tb.twistd) + o;
-  return s;
-}
-
-// ------------------------------------------------------------------------------- forward, columns
-// First l1 stages. Input: canonical (in place or out of place), output: internal form.
-template <int LS, class A, class TwT>
-__device__ __forceinline__ void fwd_cols_body(const A& ar, const TwT* sw, const TwT* swp, const uint64_t* p, uint64_t* po,
-                                              uint64_t* tile, uint32_t N2, int C) {
-  constexpr int S = 1 << LS, T = S / 16, PADS = S + 1;
-  const int tid = threadIdx.x;
-  const int cc = tid % C, r0 = tid / C;
-  const uint32_t col = blockIdx.x * C + cc;
-  typename A::T x[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) x[k] = ar.from_canon(p[(size_t)(r0 + T * k) * N2 + col]);
-  __syncthreads();
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const int h = 8 >> s;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      if (k & h) continue;
-      const int idx = (1 << s) + (k >> (4 - s));
-      ar.fwd(x[k], x[k + h], ar.tw(sw, swp, idx));
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < 16; ++k) ar.recenter(x[k]);
-  if constexpr (LS > 4) {
-    uint64_t* t = tile + cc * PADS;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) t[r0 + T * k] = ar.to_mid(x[k]);
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(t[16 * r0 + k]);
-#pragma unroll
-    for (int s = 4; s < LS; ++s) {
-      const int h = S >> (s + 1);
-#pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        if (k & h) continue;
-        const int idx = (1 << s) + ((16 * r0 + k) >> (LS - s));
-        ar.fwd(x[k], x[k + h], ar.tw(sw, swp, idx));
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      ar.recenter(x[k]);
-      po[(size_t)(16 * r0 + k) * N2 + col] = ar.to_mid(x[k]);
-    }
-  } else {
-#pragma unroll
-    for (int k = 0; k < 16; ++k) po[(size_t)(r0 + T * k) * N2 + col] = ar.to_mid(x[k]);
-  }
-}
-
-template <int LS, class Addr, class AddrOut>
-__global__ __launch_bounds__(kThreads) void k_ntt_fwd_cols(Addr ad, AddrOut ao, NttTables tb) {
-  constexpr int S = 1 << LS, T = S / 16, PADS = S + 1, CMAX = kThreads / T;
-  __shared__ uint64_t tile[CMAX * PADS];
-  __shared__ uint64_t sw[S], swp[S];
-  uint64_t *p, *po;
-  int pid, pido;
-  ad.at(blockIdx.y, tb.N, p, pid);
-  ao.at(blockIdx.y, tb.N, po, pido);
-  const Sel sel = select(tb, pid, false);
-  const int C = blockDim.x / T;
-  const uint32_t N2 = tb.N >> LS;
-  if (sel.fp) {
-    double* swd = reinterpret_cast<double*>(sw);
-    for (int i = threadIdx.x; i < S; i += blockDim.x) swd[i] = sel.twd[i];
-    fwd_cols_body<LS>(sel.fa, (const double*)swd, (const double*)swd, p, po, tile, N2, C);
-  } else {
-    for (int i = threadIdx.x; i < S; i += blockDim.x) { sw[i] = sel.tw[i]; swp[i] = sel.twsh[i]; }
-    fwd_cols_body<LS>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, p, po, tile, N2, C);
-  }
-}
-
-// ------------------------------------------------------------------------------- row tiles in LDS
-// A row pass writes its tile row-major (word c0 + T k) and reads it transposed (word 16 c0 + k),
-// i.e. at a stride of 16 words across lanes, which maps pairs of lanes onto the same LDS banks.
-// One pad word every 16 words makes that stride 17: the T lanes of a row then hit distinct bank
-// pairs in both patterns, and a row length of 17 T words (= T mod 32) places the rows sharing a
-// half-wave on disjoint banks.
-__device__ __forceinline__ int rsw(int e) { return e + (e >> 4); }
-template <int LS>
-struct RowTile {
-  static constexpr int S = 1 << LS;
-  static constexpr int PAD = S + S / 16;   // words per row in LDS; rsw(S - 1) = PAD - 2
-};
-
-// ------------------------------------------------------------------------------- forward, rows
-// Last l2 stages. Row rho of the intermediate is finished by a standard S2-point negacyclic
-// transform of x_c tau_rho^c (c = position in the row): one coalesced twist multiply per element,
-// then stages whose twiddles are the first S2 entries of the table, the same for every row, read
-// from LDS. Input: internal form, output: canonical residues handed to the epilogue.
-template <int LS, class A, class TwT, class Epi>
-__device__ __forceinline__ void fwd_rows_body(const A& ar, const TwT* sw, const TwT* swp, const TwT* ts, const TwT* tsp, uint64_t* p,
-                                              uint64_t* tile, int R, const Epi& epi) {
-  constexpr int S = 1 << LS, T = S / 16, PADS = RowTile<LS>::PAD;
-  const int tid = threadIdx.x;
-  const int rr = tid / T, c0 = tid % T;
-  const uint32_t rho = blockIdx.x * R + rr;
-  uint64_t* row = p + (size_t)rho * S;
-  const uint32_t rbase = rho * S;
-  typename A::T x[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(row[c0 + T * k]);
-#pragma unroll
-  for (int k = 0; k < 16; ++k) x[k] = ar.mul(x[k], ar.tw(ts + rbase, tsp + rbase, c0 + T * k));
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const int h = 8 >> s;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      if (k & h) continue;
-      const int idx = (1 << s) + (k >> (4 - s));
-      ar.fwd(x[k], x[k + h], ar.tw(sw, swp, idx));
-    }
-  }
-  uint64_t* t = tile + rr * PADS;
-  if constexpr (LS > 4) {
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      ar.recenter(x[k]);
-      t[rsw(c0 + T * k)] = ar.to_mid(x[k]);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(t[rsw(16 * c0 + k)]);
-#pragma unroll
-    for (int s = 4; s < LS; ++s) {
-      const int h = S >> (s + 1);
-#pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        if (k & h) continue;
-        const int idx = (1 << s) + ((16 * c0 + k) >> (LS - s));
-        ar.fwd(x[k], x[k + h], ar.tw(sw, swp, idx));
-      }
-    }
-    // each thread rewrites exactly the LDS words it read: no barrier needed before the store
-#pragma unroll
-    for (int k = 0; k < 16; ++k) t[rsw(16 * c0 + k)] = ar.canon(x[k]);
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 16; ++k) epi.store(row, rbase, c0 + T * k, t[rsw(c0 + T * k)]);
-  } else {
-#pragma unroll
-    for (int k = 0; k < 16; ++k) epi.store(row, rbase, c0 + T * k, ar.canon(x[k]));
-  }
-}
-
-// ------------------------------------------------------------------------------- forward epilogues
-// The last launch of a forward NTT hands every canonical output word to an epilogue, which
-// either stores it in place (EpiStore) or fuses the element-wise step that consumes it: the
-// ModDown combination (x_i - v) * [P^-1]_{q_i} (src/NTTfusions.cuh:237-252) or the Rescale
-// combination x_i * [q_l^-1]_{q_i} + v (src/NTTfusions.cuh:214-235). Job y addresses
-// (poly, limb) = (y % npolys, y / npolys), the order of SetAddr.
-struct EpiStore {
-  __device__ __forceinline__ void init(uint32_t, uint32_t, const PrimeConst*) {}
-  __device__ __forceinline__ void store(uint64_t* row, uint32_t, uint32_t j, uint64_t v) const { row[j] = v; }
-};
-
-struct EpiModDown {
-  const uint64_t* x;       // ext polys, limb i of poly b at x + b * x_stride + i * N
-  uint64_t x_stride;
-  uint64_t* out;           // Q_l polys, limb i of poly b at out + b * out_stride + i * N
-  uint64_t out_stride;
-  const uint64_t* w;       // [P^-1]_{q_i}, Shoup companions
-  const uint64_t* wsh;
-  uint32_t npolys;
-  // per block
-  const uint64_t* xi;
-  uint64_t* oi;
-  uint64_t q, wi, wish;
-  __device__ __forceinline__ void init(uint32_t y, uint32_t N, const PrimeConst* prime) {
-    const uint32_t i = y / npolys, b = y % npolys;
-    xi = x + b * x_stride + (size_t)i * N;
-    oi = out + b * out_stride + (size_t)i * N;
-    q = prime[i].q;
-    wi = w[i];
-    wish = wsh[i];
-  }
-  __device__ __forceinline__ void store(uint64_t*, uint32_t rbase, uint32_t j, uint64_t v) const {
-    oi[rbase + j] = shoup(sub_mod(xi[rbase + j], v, q), wi, wish, q);
-  }
-};
-
-struct EpiRescale {
-  const uint64_t* x;       // ncomp components of l+1 limbs each
-  uint64_t* out;           // ncomp components of l limbs each
-  const uint64_t* w;       // [q_l^-1]_{q_i} row of level l, Shoup companions
-  const uint64_t* wsh;
-  uint32_t l, npolys;
-  const uint64_t* xi;
-  uint64_t* oi;
-  uint64_t q, wi, wish;
-  __device__ __forceinline__ void init(uint32_t y, uint32_t N, const PrimeConst* prime) {
-    const uint32_t i = y / npolys, b = y % npolys;
-    xi = x + ((size_t)b * (l + 1) + i) * N;
-    oi = out + ((size_t)b * l + i) * N;
-    q = prime[i].q;
-    wi = w[i];
-    wish = wsh[i];
-  }
-  __device__ __forceinline__ void store(uint64_t*, uint32_t rbase, uint32_t j, uint64_t v) const {
-    oi[rbase + j] = add_mod(shoup(xi[rbase + j], wi, wish, q), v, q);
-  }
-};
-
-template <int LS, class Addr, class Epi>
-__global__ __launch_bounds__(kThreads) void k_ntt_fwd_rows(Addr ad, NttTables tb, Epi epi) {
-  constexpr int S = 1 << LS, T = S / 16, PADS = RowTile<LS>::PAD, RMAX = kThreads / T;
-  __shared__ uint64_t tile[RMAX * PADS];
-  __shared__ uint64_t sw[S], swp[S];
-  uint64_t* p;
-  int pid;
-  ad.at(blockIdx.y, tb.N, p, pid);
-  epi.init(blockIdx.y, tb.N, tb.prime);
-  const Sel sel = select(tb, pid, false);
-  const int R = blockDim.x / T;
-  if (sel.fp) {
-    double* swd = reinterpret_cast<double*>(sw);
-    for (int i = threadIdx.x; i < S; i += blockDim.x) swd[i] = sel.twd[i];
-    __syncthreads();
-    fwd_rows_body<LS>(sel.fa, (const double*)swd, (const double*)swd, sel.tsd, sel.tsd, p, tile, R, epi);
-  } else {
-    for (int i = threadIdx.x; i < S; i += blockDim.x) { sw[i] = sel.tw[i]; swp[i] = sel.twsh[i]; }
-    __syncthreads();
-    fwd_rows_body<LS>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, sel.ts, sel.tssh, p, tile, R, epi);
-  }
-}
-
-// ------------------------------------------------------------------------------- inverse, rows
-// First launch of the INTT: the inverse of the forward row transform, i.e. the standard S2-point
-// Gentleman-Sande stages (first S2 entries of the inverse table, from LDS) followed by one
-// coalesced multiply by tau_rho^-c. Input: canonical, output: internal form. fp64 values are
-// re-centred after every second stage (Gentleman-Sande sums double per stage).
-template <int LS, class A, class TwT>
-__device__ __forceinline__ void inv_rows_body(const A& ar, const TwT* sw, const TwT* swp, const TwT* ts, const TwT* tsp, const uint64_t* p,
-                                              uint64_t* po, uint64_t* tile, int R) {
-  constexpr int S = 1 << LS, T = S / 16, PADS = RowTile<LS>::PAD;
-  const int tid = threadIdx.x;
-  const int rr = tid / T, c0 = tid % T;
-  const uint32_t rho = blockIdx.x * R + rr;
-  const uint64_t* row = p + (size_t)rho * S;
-  uint64_t* rowo = po + (size_t)rho * S;
-  const uint32_t rbase = rho * S;
-  uint64_t* t = tile + rr * PADS;
-  typename A::T x[16];
-  if constexpr (LS > 4) {
-#pragma unroll
-    for (int k = 0; k < 16; ++k) t[rsw(c0 + T * k)] = row[c0 + T * k];
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = ar.from_canon(t[rsw(16 * c0 + k)]);
-#pragma unroll
-    for (int s = LS - 1; s >= 4; --s) {
-      const int h = S >> (s + 1);
-#pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        if (k & h) continue;
-        const int idx = (1 << s) + ((16 * c0 + k) >> (LS - s));
-        ar.inv(x[k], x[k + h], ar.tw(sw, swp, idx));
-      }
-      if (((LS - s) & 1) == 0) {
-#pragma unroll
-        for (int k = 0; k < 16; ++k) ar.recenter(x[k]);
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      ar.recenter(x[k]);
-      t[rsw(16 * c0 + k)] = ar.to_mid(x[k]);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(t[rsw(c0 + T * k)]);
-  } else {
-#pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = ar.from_canon(row[c0 + T * k]);
-  }
-#pragma unroll
-  for (int s = 3; s >= 0; --s) {
-    const int h = 8 >> s;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      if (k & h) continue;
-      const int idx = (1 << s) + (k >> (4 - s));
-      ar.inv(x[k], x[k + h], ar.tw(sw, swp, idx));
-    }
-    if ((s & 1) == 0) {
-#pragma unroll
-      for (int k = 0; k < 16; ++k) ar.recenter(x[k]);
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < 16; ++k) rowo[c0 + T * k] = ar.to_mid(ar.mul(x[k], ar.tw(ts + rbase, tsp + rbase, c0 + T * k)));
-}
-
-template <int LS, class Addr, class AddrOut>
-__global__ __launch_bounds__(kThreads) void k_ntt_inv_rows(Addr ad, AddrOut ao, NttTables tb) {
-  constexpr int S = 1 << LS, T = S / 16, PADS = RowTile<LS>::PAD, RMAX = kThreads / T;
-  __shared__ uint64_t tile[RMAX * PADS];
-  __shared__ uint64_t sw[S], swp[S];
-  uint64_t *p, *po;
-  int pid, pido;
-  ad.at(blockIdx.y, tb.N, p, pid);
-  ao.at(blockIdx.y, tb.N, po, pido);
-  const Sel sel = select(tb, pid, true);
-  const int R = blockDim.x / T;
-  if (sel.fp) {
-    double* swd = reinterpret_cast<double*>(sw);
-    for (int i = threadIdx.x; i < S; i += blockDim.x) swd[i] = sel.twd[i];
-    __syncthreads();
-    inv_rows_body<LS>(sel.fa, (const double*)swd, (const double*)swd, sel.tsd, sel.tsd, p, po, tile, R);
-  } else {
-    for (int i = threadIdx.x; i < S; i += blockDim.x) { sw[i] = sel.tw[i]; swp[i] = sel.twsh[i]; }
-    __syncthreads();
-    inv_rows_body<LS>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, sel.ts, sel.tssh, p, po, tile, R);
-  }
-}
-
-// ------------------------------------------------------------------------------- inverse, columns
-// Second launch of the INTT (first l1 stages, reversed) with the N^-1 scaling. Input: internal
-// form, output: canonical residues.
-template <int LS, class A, class TwT, class Fin>
-__device__ __forceinline__ void inv_cols_body(const A& ar, const TwT* sw, const TwT* swp, uint64_t* p, uint64_t* tile, uint32_t N2,
-                                              int C, const Fin& fin) {
-  constexpr int S = 1 << LS, T = S / 16, PADS = S + 1;
-  const int tid = threadIdx.x;
-  const int cc = tid % C, r0 = tid / C;
-  const uint32_t col = blockIdx.x * C + cc;
-  typename A::T x[16];
-  uint64_t* t = tile + cc * PADS;
-  if constexpr (LS > 4) {
-#pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(p[(size_t)(16 * r0 + k) * N2 + col]);
-    __syncthreads();
-#pragma unroll
-    for (int s = LS - 1; s >= 4; --s) {
-      const int h = S >> (s + 1);
-#pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        if (k & h) continue;
-        const int idx = (1 << s) + ((16 * r0 + k) >> (LS - s));
-        ar.inv(x[k], x[k + h], ar.tw(sw, swp, idx));
-      }
-      if (((LS - s) & 1) == 0) {
-#pragma unroll
-        for (int k = 0; k < 16; ++k) ar.recenter(x[k]);
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      ar.recenter(x[k]);
-      t[16 * r0 + k] = ar.to_mid(x[k]);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(t[r0 + T * k]);
-  } else {
-#pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(p[(size_t)(r0 + T * k) * N2 + col]);
-    __syncthreads();
-  }
-#pragma unroll
-  for (int s = 3; s >= 0; --s) {
-    const int h = 8 >> s;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      if (k & h) continue;
-      const int idx = (1 << s) + (k >> (4 - s));
-      ar.inv(x[k], x[k + h], ar.tw(sw, swp, idx));
-    }
-    if ((s & 1) == 0) {
-#pragma unroll
-      for (int k = 0; k < 16; ++k) ar.recenter(x[k]);
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < 16; ++k) p[(size_t)(r0 + T * k) * N2 + col] = fin(x[k]);
-}
-
-struct IntFin {
-  IntA a;
-  uint64_t s, ssh;
-  __device__ __forceinline__ uint64_t operator()(uint64_t v) const {
-    uint64_t r = shoup_lazy(v, s, ssh, a.q);
-    return r >= a.q ? r - a.q : r;
-  }
-};
-struct FpFin {
-  FpA a;
-  double s;
-  __device__ __forceinline__ uint64_t operator()(double v) const { return a.canon(a.mulmod(v, s)); }
-};
-
-template <int LS, class Addr>
-__global__ __launch_bounds__(kThreads) void k_ntt_inv_cols(Addr ad, NttTables tb) {
-  constexpr int S = 1 << LS, T = S / 16, PADS = S + 1, CMAX = kThreads / T;
-  __shared__ uint64_t tile[CMAX * PADS];
-  __shared__ uint64_t sw[S], swp[S];
-  uint64_t* p;
-  int pid;
-  ad.at(blockIdx.y, tb.N, p, pid);
-  const Sel sel = select(tb, pid, true);
-  const PrimeConst& P = tb.prime[pid];
-  const int C = blockDim.x / T;
-  const uint32_t N2 = tb.N >> LS;
-  if (sel.fp) {
-    double* swd = reinterpret_cast<double*>(sw);
-    for (int i = threadIdx.x; i < S; i += blockDim.x) swd[i] = sel.twd[i];
-    inv_cols_body<LS>(sel.fa, (const double*)swd, (const double*)swd, p, tile, N2, C, FpFin{sel.fa, P.ninv_d});
-  } else {
-    for (int i = threadIdx.x; i < S; i += blockDim.x) { sw[i] = sel.tw[i]; swp[i] = sel.twsh[i]; }
-    inv_cols_body<LS>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, p, tile, N2, C, IntFin{sel.ia, P.ninv, P.ninv_sh});
-  }
-}
-
-// ------------------------------------------------------------------------------- launch
-template <int LS>
-inline dim3 col_block(uint32_t N) {
-  constexpr int T = (1 << LS) / 16;
-  uint32_t N2 = N >> LS;
-  uint32_t C = std::min<uint32_t>(kThreads / T, N2);
-  return dim3(C * T);
-}
-
-template <int LS>
-inline dim3 row_block(uint32_t N) {
-  constexpr int T = (1 << LS) / 16;
-  uint32_t S1 = N >> LS;  // number of rows
-  uint32_t R = std::min<uint32_t>(kThreads / T, S1);
-  return dim3(R * T);
-}
-
-// Offset addressing of a chunk of a larger job space (blockIdx.y + base).
-template <class Addr>
-struct Shift {
-  Addr a;
-  uint32_t base;
-  __device__ __forceinline__ void at(uint32_t y, uint32_t N, uint64_t*& p, int& pid) const { a.at(y + base, N, p, pid); }
-};
-
-// Epilogues of a batch at mixed levels: one table entry per limb job (same index as the NTT job).
-struct EpiModDownJobs {
-  const EpiJob* jobs;
-  const uint64_t* w;       // [P^-1]_{q_i} by prime, Shoup companions
-  const uint64_t* wsh;
-  const uint64_t* xi;
-  uint64_t* oi;
-  const uint64_t* ai;
-  uint64_t q, wi, wish;
-  __device__ __forceinline__ void init(uint32_t y, uint32_t, const PrimeConst* prime) {
-    const EpiJob j = jobs[y];
-    xi = j.x;
-    oi = j.out;
-    ai = j.add;
-    q = prime[j.pid].q;
-    wi = w[j.pid];
-    wish = wsh[j.pid];
-  }
-  __device__ __forceinline__ void store(uint64_t*, uint32_t rbase, uint32_t jj, uint64_t v) const {
-    uint64_t r = shoup(sub_mod(xi[rbase + jj], v, q), wi, wish, q);
-    if (ai) r = add_mod(ai[rbase + jj], r, q);
-    oi[rbase + jj] = r;
-  }
-};
-
-struct EpiRescaleJobs {
-  const EpiJob* jobs;
-  const uint64_t* w;       // [L1][L1] [q_l^-1]_{q_i}, Shoup companions
-  const uint64_t* wsh;
-  uint32_t L1;
-  const uint64_t* xi;
-  uint64_t* oi;
-  uint64_t q, wi, wish;
-  __device__ __forceinline__ void init(uint32_t y, uint32_t, const PrimeConst* prime) {
-    const EpiJob j = jobs[y];
-    xi = j.x;
-    oi = j.out;
-    q = prime[j.pid].q;
-    const size_t o = (size_t)j.l * L1 + j.pid;
-    wi = w[o];
-    wish = wsh[o];
-  }
-  __device__ __forceinline__ void store(uint64_t*, uint32_t rbase, uint32_t jj, uint64_t v) const {
-    oi[rbase + jj] = add_mod(shoup(xi[rbase + jj], wi, wish, q), v, q);
-  }
-};
-
-template <class Epi>
-struct ShiftEpi {
-  Epi e;
-  uint32_t base;
-  __device__ __forceinline__ void init(uint32_t y, uint32_t N, const PrimeConst* prime) { e.init(y + base, N, prime); }
-  __device__ __forceinline__ void store(uint64_t* row, uint32_t rbase, uint32_t j, uint64_t v) const { e.store(row, rbase, j, v); }
-};
-
-// limbs per chunk: the intermediate of one chunk stays MALL-resident between the two launches
-inline uint32_t chunk_limbs(const Context& c) {
-  const size_t budget = (size_t)std::max(1, c.tune.ntt_chunk_mb) << 20;
-  // at most 65535 limbs per launch (grid y limit)
-  return (uint32_t)std::min<size_t>(65535, std::max<size_t>(1, budget / (8ull * c.N)));
-}
-
-template <class Addr, class AddrOut, int L1S, int L2S, class Epi>
-void fwd_impl(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlimbs, hipStream_t st, const Epi& epi, const char* rows_name) {
-  NttTables tb = ntt_tables(c);
-  dim3 b1 = col_block<L1S>(c.N), b2 = row_block<L2S>(c.N);
-  uint32_t C = b1.x / ((1 << L1S) / 16), R = b2.x / ((1 << L2S) / 16);
-  const uint32_t chunk = chunk_limbs(c);
-  for (uint32_t base = 0; base < nlimbs; base += chunk) {
-    uint32_t n = std::min(chunk, nlimbs - base);
-    Shift<Addr> sa{ad, base};
-    Shift<AddrOut> so{ao, base};
-    ShiftEpi<Epi> se{epi, base};
-    { ProfScope ps_(c, "ntt_fwd_cols", st); k_ntt_fwd_cols<L1S><<<dim3((c.N >> L1S) / C, n), b1, 0, st>>>(sa, so, tb); }
-    { ProfScope ps_(c, rows_name, st); k_ntt_fwd_rows<L2S><<<dim3((c.N >> L2S) / R, n), b2, 0, st>>>(so, tb, se); }
-  }
-}
-
-template <class Addr, class AddrOut, int L1S, int L2S>
-void inv_impl(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlimbs, hipStream_t st) {
-  NttTables tb = ntt_tables(c);
-  dim3 b1 = col_block<L1S>(c.N), b2 = row_block<L2S>(c.N);
-  uint32_t C = b1.x / ((1 << L1S) / 16), R = b2.x / ((1 << L2S) / 16);
-  const uint32_t chunk = chunk_limbs(c);
-  for (uint32_t base = 0; base < nlimbs; base += chunk) {
-    uint32_t n = std::min(chunk, nlimbs - base);
-    Shift<Addr> sa{ad, base};
-    Shift<AddrOut> so{ao, base};
-    { ProfScope ps_(c, "ntt_inv_rows", st); k_ntt_inv_rows<L2S><<<dim3((c.N >> L2S) / R, n), b2, 0, st>>>(sa, so, tb); }
-    { ProfScope ps_(c, "ntt_inv_cols", st); k_ntt_inv_cols<L1S><<<dim3((c.N >> L1S) / C, n), b1, 0, st>>>(so, tb); }
-  }
-}
-
-template <class Addr, class AddrOut, class Epi = EpiStore>
-void dispatch(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlimbs, hipStream_t st, bool fwd, const Epi& epi = Epi{},
-              const char* rows_name = "ntt_fwd_rows") {
-  if (nlimbs == 0) return;
-#define FH_NTT_CASE(LN, A, B)                                                          \
-  case LN:                                                                             \
-    if (fwd) fwd_impl<Addr, AddrOut, A, B, Epi>(c, ad, ao, nlimbs, st, epi, rows_name); \
-    else inv_impl<Addr, AddrOut, A, B>(c, ad, ao, nlimbs, st);                         \
-    break;
-  switch (c.logN) {
-    FH_NTT_CASE(10, 5, 5)
-    FH_NTT_CASE(11, 5, 6)
-    FH_NTT_CASE(12, 6, 6)
-    FH_NTT_CASE(13, 6, 7)
-    FH_NTT_CASE(14, 7, 7)
-    FH_NTT_CASE(15, 7, 8)
-    FH_NTT_CASE(16, 8, 8)
-    default:
-      throw Error(-1, "unsupported ring dimension");
-  }
-#undef FH_NTT_CASE
-  FH_HIP(hipGetLastError());
-}
-
-}  // namespace
 
 void ntt_fwd(const Context& c, const LimbSet& s, hipStream_t st) { dispatch(c, SetAddr{s}, SetAddr{s}, s.nlimbs * s.npolys, st, true); }
 void ntt_inv(const Context& c, const LimbSet& s, hipStream_t st) { dispatch(c, SetAddr{s}, SetAddr{s}, s.nlimbs * s.npolys, st, false); }
@@ -721,28 +15,6 @@ void ntt_inv(const Context& c, const LimbSet& in, const LimbSet& out, hipStream_
   if (in.nlimbs * in.npolys != out.nlimbs * out.npolys) throw Error(-1, "NTT in/out shapes differ");
   dispatch(c, SetAddr{in}, SetAddr{out}, in.nlimbs * in.npolys, st, false);
 }
-void ntt_fwd_jobs(const Context& c, const LimbJob* j, uint32_t n, hipStream_t st) { dispatch(c, JobAddr{j}, JobAddr{j}, n, st, true); }
-void ntt_inv_jobs(const Context& c, const LimbJob* j, uint32_t n, hipStream_t st) { dispatch(c, JobAddr{j}, JobAddr{j}, n, st, false); }
-void ntt_fwd_jobs(const Context& c, const LimbJob* in, const LimbJob* out, uint32_t n, hipStream_t st) {
-  dispatch(c, JobAddr{in}, JobAddr{out}, n, st, true);
-}
-void ntt_inv_jobs(const Context& c, const LimbJob* in, const LimbJob* out, uint32_t n, hipStream_t st) {
-  dispatch(c, JobAddr{in}, JobAddr{out}, n, st, false);
-}
-void ntt_fwd_jobs_moddown(const Context& c, const LimbJob* jobs, const EpiJob* epi, uint32_t n, hipStream_t st) {
-  EpiModDownJobs e{epi, c.d_md_pinv, c.d_md_pinv_sh, nullptr, nullptr, nullptr, 0, 0, 0};
-  dispatch(c, JobAddr{jobs}, JobAddr{jobs}, n, st, true, e, "ntt_fwd_rows_moddown");
-}
-void ntt_fwd_jobs_rescale(const Context& c, const LimbJob* jobs, const EpiJob* epi, uint32_t n, hipStream_t st) {
-  EpiRescaleJobs e{epi, c.d_rs_qlinv, c.d_rs_qlinv_sh, (uint32_t)c.L1, nullptr, nullptr, 0, 0, 0};
-  dispatch(c, JobAddr{jobs}, JobAddr{jobs}, n, st, true, e, "ntt_fwd_rows_rescale");
-}
-void ntt_fwd_modup(const Context& c, const ModUpAddr& a, uint32_t n, hipStream_t st) {
-  ModUpAddr m = a;
-  m.mix = c.tune.ntt_mix ? 1u : 0u;
-  dispatch(c, m, m, n, st, true);
-}
-
 void ntt_fwd_moddown(const Context& c, const LimbSet& conv, const uint64_t* x, uint64_t x_stride, uint64_t* out, uint64_t out_stride,
                      hipStream_t st) {
   EpiModDown e{x, x_stride, out, out_stride, c.d_md_pinv, c.d_md_pinv_sh, conv.npolys, nullptr, nullptr, 0, 0, 0};
@@ -753,6 +25,106 @@ void ntt_fwd_rescale(const Context& c, const LimbSet& e, const uint64_t* x, uint
   const size_t o = (size_t)l * c.L1;
   EpiRescale ep{x, out, c.d_rs_qlinv + o, c.d_rs_qlinv_sh + o, l, e.npolys, nullptr, nullptr, 0, 0, 0};
   dispatch(c, SetAddr{e}, SetAddr{e}, e.nlimbs * e.npolys, st, true, ep, "ntt_fwd_rows_rescale");
+}
+
+// ------------------------------------------------------------------------------- autotuner
+namespace {
+
+// forward + inverse over s, best of `reps` timed runs (ms)
+double time_fwd_inv(Context& c, const LimbSet& s, int reps) {
+  hipEvent_t e0, e1;
+  FH_HIP(hipEventCreate(&e0));
+  FH_HIP(hipEventCreate(&e1));
+  ntt_fwd(c, s, c.stream);
+  ntt_inv(c, s, c.stream);
+  double best = 1e30;
+  for (int r = 0; r < reps; ++r) {
+    FH_HIP(hipEventRecord(e0, c.stream));
+    ntt_fwd(c, s, c.stream);
+    ntt_inv(c, s, c.stream);
+    FH_HIP(hipEventRecord(e1, c.stream));
+    FH_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    FH_HIP(hipEventElapsedTime(&ms, e0, e1));
+    best = std::min(best, (double)ms);
+  }
+  FH_HIP(hipEventDestroy(e0));
+  FH_HIP(hipEventDestroy(e1));
+  return best;
+}
+
+}  // namespace
+
+AutotuneResult autotune_ntt(Context& c, int polys) {
+  if (polys < 1) throw Error(-1, "autotune needs at least one polynomial");
+  // synthetic polynomials over every prime of the context (both arithmetic paths, in the
+  // proportions of an extended basis), resident in HBM
+  std::vector<int> ids(c.np);
+  for (int i = 0; i < c.np; ++i) ids[i] = i;
+  DBuf buf(c.pool, (size_t)polys * c.np * c.N);
+  const LimbSet s = make_set(buf.p, (uint32_t)c.np, ids.data(), (uint32_t)polys, (uint64_t)c.np * c.N);
+  fill_uniform(c, s, 0x7u, c.stream);
+  const bool prof = c.prof.on;
+  c.prof.on = false;
+  const Tune saved = c.tune;
+  AutotuneResult r{};
+  const int reps = 3;
+  auto measure = [&]() {
+    ++r.candidates;
+    return time_fwd_inv(c, s, reps);
+  };
+  try {
+    // 1. radix split (all passes at the default workgroup size)
+    c.tune.ntt_chunk_mb = saved.ntt_chunk_mb;
+    for (int& t : c.tune.ntt_threads) t = kNttThreads;
+    const int l1d = ntt_default_split(c.logN);
+    c.tune.ntt_split = 0;
+    r.ms_default = measure();
+    double best = r.ms_default;
+    int best_split = 0;
+    // a candidate replaces the current choice only when it is clearly faster (timing noise of a
+    // few percent must not flip the shape from run to run)
+    constexpr double kGain = 0.98;
+    for (int l1 = 4; l1 <= 8; ++l1) {
+      if (l1 == l1d || !ntt_split_ok(c.logN, l1)) continue;
+      c.ensure_twists(l1);
+      c.tune.ntt_split = l1;
+      const double t = measure();
+      if (t < best * kGain) best = t, best_split = l1;
+    }
+    c.tune.ntt_split = best_split;
+    // 2. workgroup size per pass (wide tiles exist for the default split)
+    if (best_split == 0) {
+      for (int pass = 0; pass < 4; ++pass) {
+        c.tune.ntt_threads[pass] = kNttWide;
+        const double t = measure();
+        if (t < best * kGain) best = t;
+        else c.tune.ntt_threads[pass] = kNttThreads;
+      }
+    }
+    // 3. limbs per launch pair: the whole set, or chunks of 256 / 128 MB (sized for the MALL) when
+    // the set holds at least two of them; chunking adds launch tails, so it must win by 5%
+    const size_t set_mb = ((size_t)polys * c.np * c.N * 8) >> 20;
+    int best_chunk = c.tune.ntt_chunk_mb;
+    for (int mb : {256, 128}) {
+      if (set_mb < 2 * (size_t)mb || ((size_t)mb << 20) < (size_t)c.N * 8 * c.np) continue;
+      c.tune.ntt_chunk_mb = mb;
+      const double t = measure();
+      if (t < best * 0.95) best = t, best_chunk = mb;
+    }
+    c.tune.ntt_chunk_mb = best_chunk;
+    r.ms_best = best;
+  } catch (...) {
+    c.tune = saved;
+    c.prof.on = prof;
+    throw;
+  }
+  c.prof.on = prof;
+  r.split = c.tune.ntt_split ? c.tune.ntt_split : ntt_default_split(c.logN);
+  for (int i = 0; i < 4; ++i) r.threads[i] = c.tune.ntt_threads[i];
+  r.chunk_mb = c.tune.ntt_chunk_mb;
+  r.limbs = polys * c.np;
+  return r;
 }
 
 }  // namespace fh

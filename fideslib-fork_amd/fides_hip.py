@@ -90,6 +90,7 @@ def _declare(L):
                                           C.c_uint32, P, C.c_uint32, VP]),
         "fh_tune_set": (I, [P, C.c_char_p, I]),
         "fh_tune_get": (I, [P, C.c_char_p, C.POINTER(C.c_int)]),
+        "fh_autotune": (I, [P, I, C.POINTER(AutotuneReport)]),
         "fh_profile_enable": (I, [P, I]),
         "fh_profile_reset": (I, [P]),
         "fh_profile_read": (I, [P, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
@@ -98,6 +99,11 @@ def _declare(L):
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
+
+
+class AutotuneReport(C.Structure):
+    _fields_ = [("split", C.c_int), ("threads", C.c_int * 4), ("chunk_mb", C.c_int), ("candidates", C.c_int), ("limbs", C.c_int),
+                ("ms_default", C.c_double), ("ms_best", C.c_double)]
 
 
 class ServeConfig(C.Structure):
@@ -329,6 +335,15 @@ class Context:
         v = C.c_int()
         _check(load().fh_tune_get(self.h, name.encode(), C.byref(v)))
         return v.value
+
+    def autotune(self, polys):
+        """Choose the NTT launch shape by timing candidates on `polys` synthetic polynomials."""
+        r = AutotuneReport()
+        _check(load().fh_autotune(self.h, int(polys), C.byref(r)))
+        return {"split": r.split, "threads": {"fwd_cols": r.threads[0], "fwd_rows": r.threads[1], "inv_rows": r.threads[2],
+                                              "inv_cols": r.threads[3]},
+                "chunk_mb": r.chunk_mb, "candidates": r.candidates, "limbs": r.limbs, "ms_default": round(r.ms_default, 4),
+                "ms_best": round(r.ms_best, 4)}
 
     # ---- profiling (per-kernel-family HIP event timing) ----
     def profile(self, on=True):

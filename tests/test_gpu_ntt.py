@@ -85,3 +85,106 @@ def test_ntt_all_ring_sizes(fh, oracle_mod, logN):
         d.free()
     finally:
         g.close()
+
+
+# radix splits (log2 of the column-pass length) the kernels are built for, by logN
+SPLITS = {10: [4, 5, 6], 11: [5, 6], 12: [5, 6, 7], 13: [6, 7], 14: [6, 7, 8], 15: [7, 8], 16: [8]}
+THREAD_KNOBS = ["ntt_fwd_cols_threads", "ntt_fwd_rows_threads", "ntt_inv_rows_threads", "ntt_inv_cols_threads"]
+
+
+@pytest.mark.parametrize("logN,l1", [(n, s) for n, ss in SPLITS.items() for s in ss])
+def test_ntt_radix_splits_and_tiles(fh, oracle_mod, logN, l1):
+    """Every radix split gives the oracle's transform; at the default split so do the 512-thread
+    tiles of each pass."""
+    cfg = dict(logN=logN, L1=3, K=1, dnum=1)
+    o = make_oracle_ctx(oracle_mod, cfg)
+    g = fh.Context(logN, 3, 1, 1)
+    try:
+        pids = [0, 1, 2, 3]
+        x = o.random_poly(100 + logN + l1, pids)
+        want = o.ntt(x, pids)
+        g.tune_set("ntt_split", l1)
+        shapes = [[256] * 4]
+        if l1 == logN // 2:
+            shapes += [[512] * 4, [512, 256, 256, 512], [256, 512, 512, 256]]
+        for sh in shapes:
+            for k, v in zip(THREAD_KNOBS, sh):
+                g.tune_set(k, v)
+            print(f"[ntt split] logN={logN} l1={l1} threads={sh}", flush=True)
+            d = g.to_device(x)
+            g.ntt_forward(d, pids)
+            assert np.array_equal(d.download(), want), (l1, sh)
+            g.ntt_inverse(d, pids)
+            assert np.array_equal(d.download(), x), (l1, sh)
+            d.free()
+    finally:
+        g.close()
+
+
+def test_ntt_wide_tiles_chunked_strided(gctx2, octx2):
+    """512-thread tiles with 2-limb launch chunks over strided polynomials at N=2^16."""
+    g, o = gctx2, octx2
+    N = o.N
+    pids = [24, 0, 30, 3, 25]
+    npoly, stride_limbs = 3, 7
+    host = np.zeros((npoly * stride_limbs, N), dtype=np.uint64)
+    for p in range(npoly):
+        host[p * stride_limbs:p * stride_limbs + 5] = o.random_poly(200 + p, pids)
+    saved = {k: g.tune_get(k) for k in THREAD_KNOBS + ["ntt_chunk_mb"]}
+    try:
+        for k in THREAD_KNOBS:
+            g.tune_set(k, 512)
+        g.tune_set("ntt_chunk_mb", 1)
+        d = g.to_device(host)
+        g.ntt_forward(d, pids, num_polys=npoly, poly_stride=stride_limbs * N)
+        got = d.download()
+        for p in range(npoly):
+            blk = slice(p * stride_limbs, p * stride_limbs + 5)
+            assert np.array_equal(got[blk], o.ntt(host[blk], pids))
+        g.ntt_inverse(d, pids, num_polys=npoly, poly_stride=stride_limbs * N)
+        assert np.array_equal(d.download(), host)
+        d.free()
+    finally:
+        for k, v in saved.items():
+            g.tune_set(k, v)
+
+
+def test_tunable_validation(gctx1, fh):
+    g = gctx1
+    with pytest.raises(fh.FHError):
+        g.tune_set("ntt_split", 3)            # below the smallest built split
+    with pytest.raises(fh.FHError):
+        g.tune_set("ntt_split", 9)
+    with pytest.raises(fh.FHError):
+        g.tune_set("ntt_fwd_cols_threads", 384)
+    with pytest.raises(fh.FHError):
+        g.tune_set("ntt_mix", 2)
+    with pytest.raises(fh.FHError):
+        g.tune_set("no_such_knob", 1)
+    assert g.tune_get("ntt_split") == 0
+
+
+@pytest.mark.parametrize("logN,polys", [(12, 16), (16, 4)])
+def test_autotune_choice_is_valid_and_exact(fh, oracle_mod, logN, polys):
+    """The autotuner keeps a built shape no slower than the default, and transforms stay exact."""
+    cfg = dict(logN=logN, L1=3, K=1, dnum=1)
+    o = make_oracle_ctx(oracle_mod, cfg)
+    g = fh.Context(logN, 3, 1, 1)
+    try:
+        r = g.autotune(polys)
+        print(f"[autotune] logN={logN} {r}", flush=True)
+        assert r["split"] in SPLITS[logN]
+        assert set(r["threads"].values()) <= {256, 512}
+        assert r["candidates"] >= 3 and r["limbs"] == polys * 4
+        assert 0 < r["ms_best"] <= r["ms_default"]
+        assert g.tune_get("ntt_split") in (0, r["split"])
+        pids = [0, 1, 2, 3]
+        x = o.random_poly(300 + logN, pids)
+        d = g.to_device(x)
+        g.ntt_forward(d, pids)
+        assert np.array_equal(d.download(), o.ntt(x, pids))
+        g.ntt_inverse(d, pids)
+        assert np.array_equal(d.download(), x)
+        d.free()
+    finally:
+        g.close()
