@@ -47,6 +47,10 @@ def parse():
     ap.add_argument("--serve-rates", type=str, default="50,150,400", help="arrival rates (requests/s) of the serving sweep")
     ap.add_argument("--copy-mb", type=int, default=256, help="device copy of this size first (counter calibration; 0 = skip)")
     ap.add_argument("--tune", action="append", default=[], metavar="NAME=VALUE", help="set a context tunable (repeatable)")
+    ap.add_argument("--rotation-only", action="store_true",
+                    help="time the batched rotation step only (for profiler runs: every dispatch after set-up belongs to it)")
+    ap.add_argument("--traffic-file", type=str, default=os.path.join(ROOT, "profiles", "traffic_rotation_n65536_l25.txt"),
+                    help="rocprofv3 FETCH_SIZE + WRITE_SIZE bytes per launch of each kernel family of this workload")
     ap.add_argument("--autotune-polys", type=int, default=32,
                     help="run the NTT autotuner on this many synthetic polynomials before timing (0 = keep the defaults)")
     return ap.parse_args()
@@ -122,17 +126,19 @@ def main():
         return [dev_keys[i] for i in r], [gal[i] for i in r], len(set(r.tolist()))
 
     # ---- bit-exactness: batched == per-op on the same inputs ----
-    ks, gs, _ = choose(0)
-    ctx.rotate_batch(cts, B, lvl, ks, gs, out_b)
-    for b in range(B):
-        ctx_ct = _view(fh, ctx, cts, b * ct_words, (2 * lq, N))
-        ctx_out = _view(fh, ctx, out_p, b * ct_words, (2 * lq, N))
-        ctx.eval_rotate(ctx_ct, lvl, ks[b], gs[b], ctx_out)
-    ctx.sync()
-    check = min(B, 8)
-    xb = out_b.download()[: check * 2 * lq]
-    xp = out_p.download()[: check * 2 * lq]
-    bit_exact = bool(np.array_equal(xb, xp))
+    bit_exact = None
+    if not a.rotation_only:
+        ks, gs, _ = choose(0)
+        ctx.rotate_batch(cts, B, lvl, ks, gs, out_b)
+        for b in range(B):
+            ctx_ct = _view(fh, ctx, cts, b * ct_words, (2 * lq, N))
+            ctx_out = _view(fh, ctx, out_p, b * ct_words, (2 * lq, N))
+            ctx.eval_rotate(ctx_ct, lvl, ks[b], gs[b], ctx_out)
+        ctx.sync()
+        check = min(B, 8)
+        xb = out_b.download()[: check * 2 * lq]
+        xp = out_p.download()[: check * 2 * lq]
+        bit_exact = bool(np.array_equal(xb, xp))
 
     # ---- timed: batched rotations ----
     for w in range(a.warmup):
@@ -156,56 +162,59 @@ def main():
     ms_per_step = 1000.0 * t_batched / a.steps
     rot_per_s = B * a.steps / t_batched
 
-    # ---- per-op rotations (same inputs, one call per ciphertext) ----
-    per_steps = max(1, min(a.steps, 3))
-    for s in range(1):
-        ks, gs, _ = choose(2000)
-        for b in range(B):
-            ctx.eval_rotate(_view(fh, ctx, cts, b * ct_words, (2 * lq, N)), lvl, ks[b], gs[b],
-                            _view(fh, ctx, out_p, b * ct_words, (2 * lq, N)))
-    ctx.sync()
-    t0 = time.perf_counter()
-    for s in range(per_steps):
-        ks, gs, _ = choose(3000 + s)
-        for b in range(B):
-            ctx.eval_rotate(_view(fh, ctx, cts, b * ct_words, (2 * lq, N)), lvl, ks[b], gs[b],
-                            _view(fh, ctx, out_p, b * ct_words, (2 * lq, N)))
-    ctx.sync()
-    per_op_rot_s = B * per_steps / (time.perf_counter() - t0)
+    per_op_rot_s = ntt_gbs = t_ntt = mult_rescale_s = None
+    nprof = {}
+    if not a.rotation_only:
+        # ---- per-op rotations (same inputs, one call per ciphertext) ----
+        per_steps = max(1, min(a.steps, 3))
+        for s in range(1):
+            ks, gs, _ = choose(2000)
+            for b in range(B):
+                ctx.eval_rotate(_view(fh, ctx, cts, b * ct_words, (2 * lq, N)), lvl, ks[b], gs[b],
+                                _view(fh, ctx, out_p, b * ct_words, (2 * lq, N)))
+        ctx.sync()
+        t0 = time.perf_counter()
+        for s in range(per_steps):
+            ks, gs, _ = choose(3000 + s)
+            for b in range(B):
+                ctx.eval_rotate(_view(fh, ctx, cts, b * ct_words, (2 * lq, N)), lvl, ks[b], gs[b],
+                                _view(fh, ctx, out_p, b * ct_words, (2 * lq, N)))
+        ctx.sync()
+        per_op_rot_s = B * per_steps / (time.perf_counter() - t0)
 
-    # ---- NTT GB/s: forward + inverse over all 64 x 2 x 25 limbs ----
-    ctx.ntt_forward(out_b, qids, num_polys=2 * B)
-    ctx.ntt_inverse(out_b, qids, num_polys=2 * B)
-    ctx.sync()
-    nrep = 5
-    ctx.profile(True)
-    ctx.profile_reset()
-    t0 = time.perf_counter()
-    for _ in range(nrep):
+        # ---- NTT GB/s: forward + inverse over all 64 x 2 x 25 limbs ----
         ctx.ntt_forward(out_b, qids, num_polys=2 * B)
         ctx.ntt_inverse(out_b, qids, num_polys=2 * B)
-    ctx.sync()
-    t_ntt = (time.perf_counter() - t0) / nrep
-    nprof = {f: ctx.profile_read(f) for f in ("ntt_fwd_cols", "ntt_fwd_rows", "ntt_inv_rows", "ntt_inv_cols")}
-    ctx.profile(False)
-    ntt_limbs = 2 * B * lq
-    ntt_bytes = 2 * ntt_limbs * 16 * N  # forward + inverse, 16N bytes per limb-transform
-    ntt_gbs = ntt_bytes / t_ntt / 1e9
+        ctx.sync()
+        nrep = 5
+        ctx.profile(True)
+        ctx.profile_reset()
+        t0 = time.perf_counter()
+        for _ in range(nrep):
+            ctx.ntt_forward(out_b, qids, num_polys=2 * B)
+            ctx.ntt_inverse(out_b, qids, num_polys=2 * B)
+        ctx.sync()
+        t_ntt = (time.perf_counter() - t0) / nrep
+        nprof = {f: ctx.profile_read(f) for f in ("ntt_fwd_cols", "ntt_fwd_rows", "ntt_inv_rows", "ntt_inv_cols")}
+        ctx.profile(False)
+        ntt_limbs = 2 * B * lq
+        ntt_bytes = 2 * ntt_limbs * 16 * N  # forward + inverse, 16N bytes per limb-transform
+        ntt_gbs = ntt_bytes / t_ntt / 1e9
 
-    # ---- EvalMult + Rescale (relinearization key = first rotation key's buffer shape) ----
-    prod = ctx.array((2 * lq, N))
-    rs = ctx.array((2 * (lq - 1), N))
-    ctx.eval_mult(_view(fh, ctx, cts, 0, (2 * lq, N)), _view(fh, ctx, cts, ct_words, (2 * lq, N)), lvl, dev_keys[0], prod)
-    ctx.sync()
-    nmul = 16
-    t0 = time.perf_counter()
-    for i in range(nmul):
-        x = _view(fh, ctx, cts, (2 * i % B) * ct_words, (2 * lq, N))
-        y = _view(fh, ctx, cts, ((2 * i + 1) % B) * ct_words, (2 * lq, N))
-        ctx.eval_mult(x, y, lvl, dev_keys[0], prod)
-        ctx.rescale(prod, lvl, 2, rs)
-    ctx.sync()
-    mult_rescale_s = nmul / (time.perf_counter() - t0)
+        # ---- EvalMult + Rescale (relinearization key = first rotation key's buffer shape) ----
+        prod = ctx.array((2 * lq, N))
+        rs = ctx.array((2 * (lq - 1), N))
+        ctx.eval_mult(_view(fh, ctx, cts, 0, (2 * lq, N)), _view(fh, ctx, cts, ct_words, (2 * lq, N)), lvl, dev_keys[0], prod)
+        ctx.sync()
+        nmul = 16
+        t0 = time.perf_counter()
+        for i in range(nmul):
+            x = _view(fh, ctx, cts, (2 * i % B) * ct_words, (2 * lq, N))
+            y = _view(fh, ctx, cts, ((2 * i + 1) % B) * ct_words, (2 * lq, N))
+            ctx.eval_mult(x, y, lvl, dev_keys[0], prod)
+            ctx.rescale(prod, lvl, 2, rs)
+        ctx.sync()
+        mult_rescale_s = nmul / (time.perf_counter() - t0)
 
     # ---- roofline of the dominant kernel of the timed rotation step ----
     dom = max(prof, key=lambda f: prof[f][1])
@@ -227,15 +236,21 @@ def main():
             "avg_launch_ms": round(avg_ms, 4), "launches": launches,
             "kernel_ms_per_step": {f: round(prof[f][1] / a.steps, 3) for f in fams}}
     ntt_kernel_ms = {f: round(nprof[f][1] / max(nprof[f][0], 1), 4) for f in nprof}
+    # HBM bytes per launch of the dominant kernel, from the rocprofv3 counter runs of this workload
+    traffic = read_traffic(a.traffic_file).get(dom)
+    roof["traffic"] = traffic
+    if traffic:
+        roof["traffic_source"] = os.path.relpath(a.traffic_file, ROOT)
 
     # ---- mixed-level op stream: per-op vs one batch (eager and graph-replayed) ----
-    stream = uneven_stream(fh, ctx, dev_keys, gal, a) if a.stream_ops > 0 else None
-    lt = lt_workload(fh, ctx, a) if a.lt_count > 0 else None
-    serving = serving_sweep(ctx, dev_keys[:8], gal[:8], dev_keys[0], a) if a.serve_requests > 0 else None
+    full = not a.rotation_only
+    stream = uneven_stream(fh, ctx, dev_keys, gal, a) if full and a.stream_ops > 0 else None
+    lt = lt_workload(fh, ctx, a) if full and a.lt_count > 0 else None
+    serving = serving_sweep(ctx, dev_keys[:8], gal[:8], dev_keys[0], a) if full and a.serve_requests > 0 else None
 
     # ---- CPU baseline: the oracle on a bounded sample of the same rotation ----
     cpu = None
-    if not a.no_cpu:
+    if full and not a.no_cpu:
         cpu = cpu_baseline(a)
 
     line = {
@@ -256,14 +271,14 @@ def main():
                    "tunables": tunables, "autotune": autotune},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "per_op_rotations_per_s": round(per_op_rot_s, 1),
-        "batched_over_per_op": round(rot_per_s / per_op_rot_s, 3),
+        "per_op_rotations_per_s": round(per_op_rot_s, 1) if per_op_rot_s else None,
+        "batched_over_per_op": round(rot_per_s / per_op_rot_s, 3) if per_op_rot_s else None,
         "bit_exact_batched_vs_per_op": bit_exact,
-        "ntt_gbps": round(ntt_gbs, 1),
-        "ntt_roofline_frac": round(ntt_gbs / HBM_PEAK_GBS, 4),
-        "ntt_ms_fwd_inv": round(1000 * t_ntt, 3),
+        "ntt_gbps": round(ntt_gbs, 1) if ntt_gbs else None,
+        "ntt_roofline_frac": round(ntt_gbs / HBM_PEAK_GBS, 4) if ntt_gbs else None,
+        "ntt_ms_fwd_inv": round(1000 * t_ntt, 3) if t_ntt else None,
         "ntt_avg_launch_ms": ntt_kernel_ms,
-        "eval_mult_rescale_per_s": round(mult_rescale_s, 1),
+        "eval_mult_rescale_per_s": round(mult_rescale_s, 1) if mult_rescale_s else None,
         "device_copy_gbps": copy_gbs,
         "uneven_stream": stream,
         "linear_transform": lt,
@@ -271,6 +286,21 @@ def main():
     }
     print(json.dumps(line))
     ctx.close()
+
+
+def read_traffic(path):
+    """kernel family -> HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by a device copy of
+    known size), from a profile summary: lines 'family<TAB>bytes', '#' comments."""
+    out = {}
+    if not os.path.exists(path):
+        return out
+    for line in open(path):
+        line = line.strip()
+        if not line or line.startswith("#"):
+            continue
+        fam, val = line.split()[:2]
+        out[fam] = int(val)
+    return out
 
 
 def uneven_stream(fh, ctx, rot_keys, rot_gal, a):
