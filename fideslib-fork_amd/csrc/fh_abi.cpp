@@ -259,6 +259,17 @@ int fh_rescale(fh_context* ctx, const uint64_t* ct, int level, int ncomp, uint64
   });
 }
 
+int fh_mod_raise(fh_context* ctx, const uint64_t* ct, int level_in, int ncomp, int level, uint64_t* out) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(ct, "ct"); need(out, "out");
+    check_level(c, level_in);
+    check_level(c, level);
+    if (ncomp < 1 || ncomp > 3) throw fh::Error(FH_E_INVALID, "ncomp must be 1..3");
+    fh::mod_raise(c, ct, level_in, ncomp, level, out, c.stream);
+  });
+}
+
 int fh_key_create(fh_context* ctx, const uint64_t* host_key, fh_key** out) {
   return guard([&] {
     auto& c = C(ctx);

@@ -468,6 +468,27 @@ __global__ __launch_bounds__(kT) void k_rescale_expand(const uint64_t* __restric
   }
 }
 
+// ----------------------------------------------------------------------------------- ModRaise
+// blockIdx.y = comp * (l+1) + i: out_i = SwitchModulus_{q_0 -> q_i}(x) of the coefficient-form
+// tower 0 of each component (the reference's tower-0 broadcast, ElemenwiseBatchKernels.cu:140-148).
+__global__ __launch_bounds__(kT) void k_mod_raise(const uint64_t* __restrict__ x0, uint64_t* __restrict__ out, uint32_t N, uint32_t l,
+                                                   const PrimeConst* __restrict__ prime) {
+  const uint32_t comp = blockIdx.y / (l + 1), i = blockIdx.y % (l + 1);
+  const uint64_t q0 = prime[0].q, qi = prime[i].q;
+  const uint64_t half = q0 >> 1;
+  const uint64_t diff = q0 > qi ? qi - (q0 % qi) : qi - q0;
+  const uint64_t* t = x0 + (size_t)comp * N;
+  uint64_t* o = out + ((size_t)comp * (l + 1) + i) * N;
+  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+    uint64_t v = t[n];
+    if (i != 0) {
+      if (v > half) v += diff;
+      if (v >= qi) v %= qi;
+    }
+    o[n] = v;
+  }
+}
+
 // ----------------------------------------------------------------------------------- automorphism
 __global__ __launch_bounds__(kT) void k_automorph(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint32_t N, uint32_t logN,
                                                    uint32_t k) {
@@ -592,6 +613,21 @@ void rescale(Context& c, const uint64_t* x, int l, int ncomp, uint64_t* out, hip
   for (int i = 0; i < l; ++i) ids[i] = i;
   // forward NTT of the correction with x_i * q_l^-1 + e_i fused into its last launch
   ntt_fwd_rescale(c, make_set(e.p, l, ids.data(), ncomp, (uint64_t)l * c.N), x, (uint32_t)l, out, st);
+}
+
+void mod_raise(Context& c, const uint64_t* x, int l_in, int ncomp, int l, uint64_t* out, hipStream_t st) {
+  if (l_in < 0 || l < 0 || l >= c.L1) throw Error(-1, "ModRaise level out of range");
+  if (ncomp < 1) return;
+  if (overlaps(x, (size_t)ncomp * (l_in + 1) * c.N, out, (size_t)ncomp * (l + 1) * c.N)) throw Error(-1, "ModRaise cannot run in place");
+  DBuf t0(c.pool, (size_t)ncomp * c.N);
+  int pid0 = 0;
+  // tower 0 of every component to coefficient form, packed
+  ntt_inv(c, make_set(const_cast<uint64_t*>(x), 1, &pid0, ncomp, (uint64_t)(l_in + 1) * c.N), make_set(t0.p, 1, &pid0, ncomp, c.N), st);
+  { ProfScope ps_(c, "mod_raise", st); k_mod_raise<<<grid_n(c.N, ncomp * (l + 1)), kT, 0, st>>>(t0.p, out, c.N, (uint32_t)l, c.d_prime); }
+  FH_HIP(hipGetLastError());
+  std::vector<int> ids(l + 1);
+  for (int i = 0; i <= l; ++i) ids[i] = i;
+  ntt_fwd(c, make_set(out, l + 1, ids.data(), ncomp, (uint64_t)(l + 1) * c.N), st);
 }
 
 void automorph(Context& c, const uint64_t* x, int nlimbs, uint32_t k, uint64_t* out, hipStream_t st) {

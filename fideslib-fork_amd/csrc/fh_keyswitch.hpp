@@ -43,6 +43,10 @@ void linear_transform_batch(Context& c, int count, const uint64_t* cts, uint64_t
 void moddown(Context& c, const uint64_t* x, int l, uint64_t* out, int npolys, uint64_t x_stride, uint64_t out_stride, hipStream_t st);
 // ncomp components over Q_l -> over Q_{l-1}
 void rescale(Context& c, const uint64_t* x, int l, int ncomp, uint64_t* out, hipStream_t st);
+// CKKS bootstrapping ModRaise: tower 0 of ncomp components (of l_in + 1 limbs each) to
+// coefficient form, lifted centred to q_0..q_l (SwitchModulus) and transformed back: out holds
+// ncomp components of l + 1 limbs.
+void mod_raise(Context& c, const uint64_t* x, int l_in, int ncomp, int l, uint64_t* out, hipStream_t st);
 void automorph(Context& c, const uint64_t* x, int nlimbs, uint32_t k, uint64_t* out, hipStream_t st);
 
 void relinearize(Context& c, const uint64_t* ct3, int l, const uint64_t* key, uint64_t* out2, hipStream_t st);

@@ -56,6 +56,7 @@ def _declare(L):
         "fh_eltwise_scalar": (I, [P, I, VP, u64p, VP, i32p, I]),
         "fh_tensor": (I, [P, VP, VP, I, VP]),
         "fh_rescale": (I, [P, VP, I, I, VP]),
+        "fh_mod_raise": (I, [P, VP, I, I, I, VP]),
         "fh_key_create": (I, [P, u64p, PP]),
         "fh_key_destroy": (I, [P, P]),
         "fh_num_digits": (I, [P, I]),
@@ -234,6 +235,9 @@ class Context:
     # ---- rescale / key switching ----
     def rescale(self, ct, level, ncomp, out):
         _check(load().fh_rescale(self.h, ct.ptr, level, ncomp, out.ptr))
+
+    def mod_raise(self, ct, level_in, ncomp, level, out):
+        _check(load().fh_mod_raise(self.h, ct.ptr, level_in, ncomp, level, out.ptr))
 
     def key(self, host_key):
         return Key(self, host_key)

@@ -1,0 +1,308 @@
+// CKKS bootstrapping and Chebyshev series evaluation over the facade.
+//
+// The bootstrap follows the reference's pipeline (src/CKKS/Bootstrap.cu:437-648) at level budget
+// {1, 1}:
+//   ModRaise (fh_mod_raise: tower 0 lifted centred to the whole chain, Bootstrap.cu:708-930)
+//   -> for sparse packing, the partial trace over the rotations by slots * 2^j
+//      (AccumulateBroadcast.cu:122-160), which keeps the coefficients at multiples of N/(2 slots)
+//   -> CoeffsToSlots: one BSGS LinearTransform by the inverse special FFT
+//   -> real and imaginary parts split with a conjugation and a multiplication by -i
+//      (the monomial X^(3N/2), Bootstrap.cu:577-584)
+//   -> EvalMod on both parts: a Chebyshev series of cos((2 pi K y - pi/2) / 2^r) on [-1, 1]
+//      followed by r double-angle steps c <- 2c^2 - 1, which leaves sin(2 pi K y)
+//      (ApproxModEval.cu:341-873); for a sparse secret r = 0 and degree 127 (each double angle
+//      quadruples the series' noise, which the final transform then scales by q_0 / Delta: at
+//      N=2^12 and 64 slots r = 3 with degree 63 measured 11.7 bits, r = 0 with degree 127 14.3
+//      bits, one level cheaper)
+//   -> recombine with i = X^(N/2) -> SlotsToCoeffs: one BSGS LinearTransform by the special FFT.
+// The scale factors are folded into the transforms: CoeffsToSlots multiplies by
+// Delta / (2 gap q_0 K) so that each part enters EvalMod as (I + p / q_0) / K in [-1, 1] (the
+// power of two 2^-e nearest below that factor is applied first as an exact scalar product, so that
+// the diagonals keep entries near 1/slots and their encoding its full precision), and
+// SlotsToCoeffs multiplies by q_0 / (2 pi Delta), undoing sin(2 pi p / q_0) ~ 2 pi p / q_0.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <map>
+
+#include "fideslib.hpp"
+#include "special_fft.hpp"
+
+namespace fideslib {
+
+using detail::Cx;
+using detail::kPi;
+
+struct BootstrapState {
+  uint32_t slots = 0, gap = 1, bStep = 1;
+  double K = 0;          // EvalMod input range: |I| < K
+  int r = 0;             // double-angle steps
+  uint32_t degree = 0;   // Chebyshev degree of the series
+  double pre = 1.0;      // exact power of two applied after the raise
+  std::vector<double> cheb;
+  std::vector<std::complex<double>> cts, stc;   // n x n matrices, row-major
+  std::vector<Plaintext> ctsDiags;              // at the top of the chain
+  std::map<uint32_t, std::vector<Plaintext>> stcDiags;  // by towers
+};
+
+namespace {
+
+// matrix of a linear map on n complex slots, column k = map(e_k)
+std::vector<std::complex<double>> matrix_of(uint32_t n, uint64_t M, bool inverse) {
+  std::vector<std::complex<double>> A((size_t)n * n);
+  for (uint32_t k = 0; k < n; ++k) {
+    std::vector<Cx> v(n, Cx{0.0, 0.0});
+    v[k] = Cx{1.0, 0.0};
+    if (inverse) detail::special_ifft(v, M);
+    else detail::special_fft(v, M);
+    for (uint32_t j = 0; j < n; ++j) A[(size_t)j * n + k] = std::complex<double>(v[j].re, v[j].im);
+  }
+  return A;
+}
+
+// BSGS diagonals of A (y = A x over n slots): diag_k[j] = A[j][(j + k) mod n], rotated by
+// -bStep * floor(k / bStep) as LinearTransform expects (v'[j] = v[(j - s) mod n])
+std::vector<std::vector<std::complex<double>>> bsgs_diagonals(const std::vector<std::complex<double>>& A, uint32_t n, uint32_t bStep) {
+  std::vector<std::vector<std::complex<double>>> D(n, std::vector<std::complex<double>>(n));
+  for (uint32_t k = 0; k < n; ++k) {
+    const uint32_t s = bStep * (k / bStep);
+    for (uint32_t j = 0; j < n; ++j) {
+      const uint32_t src = (j + n - s % n) % n;  // rotation by -s
+      D[k][j] = A[(size_t)src * n + (src + k) % n];
+    }
+  }
+  return D;
+}
+
+struct CosArg {
+  double K;
+  int r;
+};
+double shifted_cos(double y, const void* a) {
+  const CosArg* c = static_cast<const CosArg*>(a);
+  return std::cos((2.0 * kPi * c->K * y - 0.5 * kPi) / std::ldexp(1.0, c->r));
+}
+
+}  // namespace
+
+// ---- Chebyshev series ----
+std::vector<double> CryptoContextImpl::EvalChebyshevCoefficients(double (*f)(double, const void*), const void* arg, double a, double b,
+                                                                 uint32_t degree) {
+  const uint32_t n = degree + 1;
+  std::vector<double> fx(n), c(n);
+  for (uint32_t j = 0; j < n; ++j) {
+    const double x = std::cos(kPi * (j + 0.5) / n);
+    fx[j] = f(0.5 * (b - a) * x + 0.5 * (b + a), arg);
+  }
+  for (uint32_t k = 0; k < n; ++k) {
+    double s = 0.0;
+    for (uint32_t j = 0; j < n; ++j) s += fx[j] * std::cos(kPi * k * (j + 0.5) / n);
+    c[k] = 2.0 * s / n;
+  }
+  return c;
+}
+
+namespace {
+// Evaluation of sum_k c_k T_k(y) (full constant term) by recursive splitting at powers of two:
+// p = q T_M + r with q_0 = c_M, q_j = 2 c_{M+j}, r_k = c_k - c_{2M-k} (T_{M+j} = 2 T_M T_j - T_{M-j});
+// pieces below degree kBaby are linear combinations of T_1 .. T_{kBaby-1}.
+constexpr uint32_t kBaby = 8;
+
+struct ChebEval {
+  CryptoContextImpl& cc;
+  std::map<uint32_t, Ciphertext> T;
+
+  // T_k: T_2k = 2 T_k^2 - 1, T_2k+1 = 2 T_k T_k+1 - T_1, each with one rescale
+  Ciphertext t(uint32_t k) {
+    auto it = T.find(k);
+    if (it != T.end()) return it->second;
+    Ciphertext r;
+    if (k % 2 == 0) {
+      r = cc.Rescale(cc.EvalSquare(t(k / 2)));
+      r = cc.EvalSub(cc.EvalAdd(r, r), 1.0);
+    } else {
+      r = cc.Rescale(cc.EvalMult(t(k / 2), t(k / 2 + 1)));
+      r = cc.EvalSub(cc.EvalAdd(r, r), t(1));
+    }
+    T[k] = r;
+    return r;
+  }
+
+  Ciphertext eval(const std::vector<double>& c) {
+    const uint32_t d = (uint32_t)c.size() - 1;
+    if (d < kBaby) {
+      // level of the deepest T_k used, then one scalar product per term and one rescale
+      uint32_t towers = ~0u;
+      for (uint32_t k = 1; k <= d; ++k) towers = std::min(towers, t(k)->towers);
+      Ciphertext acc;
+      for (uint32_t k = 1; k <= d; ++k) {
+        if (c[k] == 0.0) continue;
+        Ciphertext tk = t(k);
+        Ciphertext term = cc.EvalMult(cc.LevelReduce(tk, tk->towers - towers), c[k]);
+        acc = acc ? cc.EvalAdd(acc, term) : term;
+      }
+      if (!acc) acc = cc.EvalMult(cc.LevelReduce(t(1), t(1)->towers - towers), 0.0);
+      return cc.EvalAdd(cc.Rescale(acc), c[0]);
+    }
+    uint32_t M = 1;
+    while (2 * M <= d) M *= 2;
+    std::vector<double> q(d - M + 1), r(c.begin(), c.begin() + M);
+    q[0] = c[M];
+    for (uint32_t j = 1; j <= d - M; ++j) {
+      q[j] = 2.0 * c[M + j];
+      r[M - j] -= c[M + j];
+    }
+    Ciphertext Q = eval(q), R = eval(r);
+    return cc.EvalAdd(cc.Rescale(cc.EvalMult(Q, t(M))), R);
+  }
+};
+}  // namespace
+
+Ciphertext CryptoContextImpl::EvalChebyshevSeries(const Ciphertext& ct, const std::vector<double>& coeffs, double a, double b) {
+  if (coeffs.size() < 2) throw std::runtime_error("fideslib: EvalChebyshevSeries needs degree >= 1");
+  if (!(b > a)) throw std::runtime_error("fideslib: EvalChebyshevSeries needs a < b");
+  Ciphertext y = ct->noiseScaleDeg == 2 ? Rescale(ct) : ct;
+  if (a != -1.0 || b != 1.0) {
+    // y = (2x - (a + b)) / (b - a)
+    y = Rescale(EvalMult(y, 2.0 / (b - a)));
+    y = EvalAdd(y, -(a + b) / (b - a));
+  }
+  std::vector<double> c = coeffs;
+  c[0] *= 0.5;
+  ChebEval e{*this, {}};
+  e.T[1] = y;
+  return e.eval(c);
+}
+
+// ---- multiplication by a monomial X^power (exact: a plaintext of one +-1 coefficient) ----
+Ciphertext CryptoContextImpl::MultByMonomial(const Ciphertext& ct, uint32_t power) {
+  power %= 2 * N_;
+  const auto key = std::make_pair(power, ct->towers);
+  auto it = monomials_.find(key);
+  if (it == monomials_.end()) {
+    std::vector<int64_t> m(N_, 0);
+    m[power % N_] = power < N_ ? 1 : -1;
+    std::vector<int32_t> pids(ct->towers);
+    for (uint32_t i = 0; i < ct->towers; ++i) pids[i] = (int32_t)i;
+    it = monomials_.emplace(key, smallToEval(m, pids)).first;
+  }
+  auto out = newCt(ct->towers, ct->ncomp);
+  for (uint32_t c = 0; c < ct->ncomp; ++c)
+    check(fh_eltwise_binary(ctx_, 2, ct->data->ptr + (size_t)c * ct->towers * N_, it->second->ptr,
+                            out->data->ptr + (size_t)c * ct->towers * N_, nullptr, (int)ct->towers));
+  out->noiseScaleDeg = ct->noiseScaleDeg;
+  out->scalingFactor = ct->scalingFactor;
+  out->slots = ct->slots;
+  return out;
+}
+
+// ---- bootstrapping ----
+void CryptoContextImpl::EvalBootstrapSetup(const std::vector<uint32_t>& levelBudget, const std::vector<uint32_t>& dim1, uint32_t slots,
+                                           uint32_t correctionFactor) {
+  if (levelBudget.size() != 2 || levelBudget[0] != 1 || levelBudget[1] != 1)
+    throw std::runtime_error("fideslib: EvalBootstrapSetup: this build implements level budget {1, 1}");
+  (void)dim1;
+  if (correctionFactor != 0) throw std::runtime_error("fideslib: EvalBootstrapSetup: correction factors are not implemented");
+  const uint32_t n = slots ? slots : N_ / 2;
+  if (n < 2 || (n & (n - 1)) || n > N_ / 2) throw std::runtime_error("fideslib: bootstrap slots must be a power of two in [2, N/2]");
+  auto st = std::make_shared<BootstrapState>();
+  st->slots = n;
+  st->gap = (N_ / 2) / n;
+  st->bStep = 1;
+  while (st->bStep * st->bStep < n) st->bStep *= 2;
+  // EvalMod range and shape: |I| <= (h + 1) / 2 for a sparse secret of weight h
+  const bool sparse = params_.dist_ == SecretKeyDist::SPARSE_TERNARY;
+  st->K = sparse ? std::ceil(params_.hw_ / 2.0) + 4.0 : 64.0;
+  st->r = sparse ? 0 : 5;
+  st->degree = sparse ? 127 : 63;
+  CosArg arg{st->K, st->r};
+  st->cheb = EvalChebyshevCoefficients(shifted_cos, &arg, -1.0, 1.0, st->degree);
+  const uint64_t M = 2ull * N_;
+  const double q0 = (double)moduli_[0];
+  const double fc = scale_ / (2.0 * st->gap * q0 * st->K), fs = q0 / (2.0 * kPi * scale_);
+  const int e = (int)std::ceil(-std::log2(fc));
+  st->pre = std::ldexp(1.0, -e);
+  const double fd = fc * std::ldexp(1.0, e);
+  st->cts = matrix_of(n, M, true);
+  for (auto& v : st->cts) v *= fd;
+  st->stc = matrix_of(n, M, false);
+  for (auto& v : st->stc) v *= fs;
+  const auto D = bsgs_diagonals(st->cts, n, st->bStep);
+  for (uint32_t k = 0; k < n; ++k) st->ctsDiags.push_back(MakeCKKSPackedPlaintextExtended(D[k], 1, 1, n));
+  boot_ = st;
+}
+
+void CryptoContextImpl::EvalBootstrapKeyGen(const PrivateKey& sk, uint32_t slots) {
+  if (!boot_ || boot_->slots != (slots ? slots : N_ / 2)) throw std::runtime_error("fideslib: EvalBootstrapSetup for these slots first");
+  std::vector<int32_t> steps;
+  for (uint32_t s = boot_->slots; s < N_ / 2; s *= 2) steps.push_back((int32_t)s);  // partial trace
+  for (uint32_t i = 1; i < boot_->bStep; ++i) steps.push_back((int32_t)i);             // baby steps
+  if (boot_->slots > boot_->bStep) steps.push_back((int32_t)boot_->bStep);              // giant step
+  EvalRotateKeyGen(sk, steps);
+  EvalConjugateKeyGen(sk);
+}
+
+BootstrapConstants CryptoContextImpl::GetBootstrapConstants() const {
+  if (!boot_) throw std::runtime_error("fideslib: EvalBootstrapSetup first");
+  return BootstrapConstants{boot_->slots, boot_->bStep, boot_->K, boot_->r, boot_->pre, boot_->cheb, boot_->cts, boot_->stc};
+}
+
+uint32_t CryptoContextImpl::GetBootstrapDepth() const {
+  if (!boot_) throw std::runtime_error("fideslib: EvalBootstrapSetup first");
+  // prescale 1 + CoeffsToSlots 1 + series + double angle r + SlotsToCoeffs 1; the series of degree
+  // d >= 7 takes ceil(log2(d + 1)) + 1 levels (pieces below degree 8 take 4)
+  uint32_t series = 1;
+  while ((1u << series) < boot_->degree + 1) ++series;
+  return 2 + (series + 1) + (uint32_t)boot_->r + 1;
+}
+
+Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0, uint32_t numIterations, uint32_t precision) {
+  if (!boot_) throw std::runtime_error("fideslib: EvalBootstrapSetup first");
+  if (numIterations != 1) throw std::runtime_error("fideslib: EvalBootstrap: one iteration only");
+  (void)precision;
+  if (ct0->ncomp != 2) throw std::runtime_error("fideslib: EvalBootstrap: relinearize first");
+  BootstrapState& st = *boot_;
+  const uint32_t n = st.slots;
+  if (GetBootstrapDepth() + 1 > L1_) throw std::runtime_error("fideslib: EvalBootstrap: the chain is too short for the bootstrap depth");
+  Ciphertext ct = ct0->noiseScaleDeg == 2 ? Rescale(ct0) : ct0;
+  ct = LevelReduce(ct, ct->towers - 1);
+  // 1. raise tower 0 to the whole chain: decrypts to p + q_0 I
+  auto raised = newCt(L1_, 2);
+  check(fh_mod_raise(ctx_, ct->data->ptr, 0, 2, (int)L1_ - 1, raised->data->ptr));
+  raised->noiseScaleDeg = 1;
+  raised->scalingFactor = ct->scalingFactor;
+  raised->slots = n;
+  raised = Rescale(EvalMult(raised, st.pre));
+  // 2. sparse packing: trace onto the coefficients at multiples of gap (times gap)
+  for (uint32_t s = n; s < N_ / 2; s *= 2) raised = EvalAdd(raised, EvalRotate(raised, (int32_t)s));
+  // 3. CoeffsToSlots: slots (I + p / q_0) / (2K), real and imaginary coefficient halves
+  Ciphertext t = Rescale(LinearTransform(raised, (int)n, (int)st.bStep, st.ctsDiags));
+  // 4. real and imaginary parts: t + conj(t), -i (t - conj(t))
+  Ciphertext cj = EvalConjugate(t);
+  Ciphertext re = EvalAdd(t, cj);
+  Ciphertext im = MultByMonomial(EvalSub(t, cj), 3 * N_ / 2);
+  // 5. EvalMod: sin(2 pi K y) on both parts
+  auto evalmod = [&](const Ciphertext& y) {
+    Ciphertext c = EvalChebyshevSeries(y, st.cheb, -1.0, 1.0);
+    for (int i = 0; i < st.r; ++i) {
+      c = Rescale(EvalSquare(c));
+      c = EvalSub(EvalAdd(c, c), 1.0);
+    }
+    return c;
+  };
+  Ciphertext fre = evalmod(re), fim = evalmod(im);
+  // 6. recombine (i = X^(N/2)) and SlotsToCoeffs
+  Ciphertext u = EvalAdd(fre, MultByMonomial(fim, N_ / 2));
+  auto sit = st.stcDiags.find(u->towers);
+  if (sit == st.stcDiags.end()) {
+    const auto D = bsgs_diagonals(st.stc, n, st.bStep);
+    std::vector<Plaintext> pts;
+    for (uint32_t k = 0; k < n; ++k) pts.push_back(MakeCKKSPackedPlaintextExtended(D[k], 1, L1_ - u->towers, n));
+    sit = st.stcDiags.emplace(u->towers, std::move(pts)).first;
+  }
+  Ciphertext out = Rescale(LinearTransform(u, (int)n, (int)st.bStep, sit->second));
+  out->slots = ct0->slots;
+  return out;
+}
+
+}  // namespace fideslib

@@ -1,6 +1,7 @@
 // fideslib host facade: CKKS metadata, key generation, encoding and encryption on the host,
 // every polynomial operation through the C ABI of the MI355X engine.
 #include "fideslib.hpp"
+#include "special_fft.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -12,7 +13,6 @@ namespace fideslib {
 namespace {
 typedef unsigned __int128 u128;
 typedef __int128 i128;
-constexpr double kPi = 3.14159265358979323846;
 
 uint64_t powmod(uint64_t a, uint64_t e, uint64_t q) {
   uint64_t r = 1 % q;
@@ -31,70 +31,10 @@ uint64_t signed_mod(int64_t v, uint64_t q) {
   return r ? q - r : 0;
 }
 
-// ---- CKKS special FFT (Cheon-Kim-Kim-Song), slots at zeta^(5^j); explicit real arithmetic so
-// the rounding sequence is fixed.
-struct Cx {
-  double re, im;
-};
-
-void bitrev(std::vector<Cx>& v) {
-  const int n = (int)v.size();
-  for (int i = 1, j = 0; i < n; ++i) {
-    int bit = n >> 1;
-    for (; j & bit; bit >>= 1) j ^= bit;
-    j ^= bit;
-    if (i < j) std::swap(v[i], v[j]);
-  }
-}
-
-void special_ifft(std::vector<Cx>& v, uint64_t M) {
-  const int n = (int)v.size();
-  for (int len = n; len >= 1; len >>= 1) {
-    const int lenh = len >> 1;
-    const uint64_t lenq = (uint64_t)len << 2;
-    for (int i = 0; i < n; i += len) {
-      uint64_t g = 1;
-      for (int j = 0; j < lenh; ++j) {
-        const uint64_t idx = (lenq - (g % lenq)) * (M / lenq);
-        const double ang = 2.0 * kPi * (double)idx / (double)M;
-        const double wr = std::cos(ang), wi = std::sin(ang);
-        const Cx a = v[i + j], b = v[i + j + lenh];
-        const double ur = a.re + b.re, ui = a.im + b.im;
-        const double dr = a.re - b.re, di = a.im - b.im;
-        v[i + j] = Cx{ur, ui};
-        v[i + j + lenh] = Cx{dr * wr - di * wi, dr * wi + di * wr};
-        g = (g * 5) % M;
-      }
-    }
-  }
-  bitrev(v);
-  for (auto& x : v) {
-    x.re = x.re / n;
-    x.im = x.im / n;
-  }
-}
-
-void special_fft(std::vector<Cx>& v, uint64_t M) {
-  const int n = (int)v.size();
-  bitrev(v);
-  for (int len = 2; len <= n; len <<= 1) {
-    const int lenh = len >> 1;
-    const uint64_t lenq = (uint64_t)len << 2;
-    for (int i = 0; i < n; i += len) {
-      uint64_t g = 1;
-      for (int j = 0; j < lenh; ++j) {
-        const uint64_t idx = (g % lenq) * (M / lenq);
-        const double ang = 2.0 * kPi * (double)idx / (double)M;
-        const double wr = std::cos(ang), wi = std::sin(ang);
-        const Cx a = v[i + j], b = v[i + j + lenh];
-        const double tr = b.re * wr - b.im * wi, ti = b.re * wi + b.im * wr;
-        v[i + j] = Cx{a.re + tr, a.im + ti};
-        v[i + j + lenh] = Cx{a.re - tr, a.im - ti};
-        g = (g * 5) % M;
-      }
-    }
-  }
-}
+using detail::Cx;
+using detail::kPi;
+using detail::special_fft;
+using detail::special_ifft;
 
 int32_t normalize_index(int32_t index, uint32_t slots) {
   int64_t r = index % (int64_t)slots;

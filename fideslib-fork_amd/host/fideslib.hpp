@@ -126,6 +126,16 @@ struct KeyPair {
   PrivateKey secretKey;
 };
 
+// the precomputation of EvalBootstrapSetup (matrices row-major over `slots` slots)
+struct BootstrapConstants {
+  uint32_t slots, bStep;
+  double K;                   // EvalMod input range
+  int r;                      // double-angle steps
+  double pre;                 // power of two applied after the raise
+  std::vector<double> cheb;   // Chebyshev coefficients of cos((2 pi K y - pi/2) / 2^r) on [-1, 1]
+  std::vector<std::complex<double>> cts, stc;
+};
+
 class CryptoContextImpl;
 using CryptoContext = std::shared_ptr<CryptoContextImpl>;
 CryptoContext GenCryptoContext(const CCParams& params);
@@ -193,6 +203,24 @@ class CryptoContextImpl {
   Ciphertext LinearTransform(const Ciphertext& ct, int rowSize, int bStep, const std::vector<Plaintext>& diagonals, int stride = 1,
                              int offset = 0);
 
+  // ---- polynomial evaluation (OpenFHE EvalChebyshevSeries / EvalChebyshevCoefficients):
+  // c_0 / 2 + sum_{k>=1} c_k T_k(y), y = (2x - a - b) / (b - a); the coefficients of f on [a, b]
+  // at degree d are c_k = 2/(d+1) sum_j f(x_j) cos(pi k (j + 1/2) / (d+1)).
+  Ciphertext EvalChebyshevSeries(const Ciphertext& ct, const std::vector<double>& coeffs, double a, double b);
+  static std::vector<double> EvalChebyshevCoefficients(double (*f)(double, const void*), const void* arg, double a, double b,
+                                                       uint32_t degree);
+
+  // ---- bootstrapping (api/CryptoContext.hpp:90-107, 423-424; src/CKKS/Bootstrap.cu:437-648).
+  // Level budget {1, 1}: CoeffsToSlots and SlotsToCoeffs are one BSGS linear transform each over
+  // the `slots` slots. EvalBootstrapKeyGen needs EvalMultKeyGen to have run.
+  void EvalBootstrapSetup(const std::vector<uint32_t>& levelBudget = {1, 1}, const std::vector<uint32_t>& dim1 = {0, 0},
+                          uint32_t slots = 0, uint32_t correctionFactor = 0);
+  void EvalBootstrapKeyGen(const PrivateKey& sk, uint32_t slots);
+  Ciphertext EvalBootstrap(const Ciphertext& ct, uint32_t numIterations = 1, uint32_t precision = 0);
+  // levels one bootstrap consumes after the raise (prescale, CoeffsToSlots, EvalMod, SlotsToCoeffs)
+  uint32_t GetBootstrapDepth() const;
+  BootstrapConstants GetBootstrapConstants() const;
+
   // ---- scalar CRT encodings (src/CKKS/Context.cu:669-863), exposed for tests
   std::vector<uint64_t> ElemForEvalMult(uint32_t towers, double operand) const;
   std::vector<uint64_t> ElemForEvalAddOrSub(uint32_t towers, double operand, uint32_t noiseDeg) const;
@@ -225,6 +253,12 @@ class CryptoContextImpl {
   std::unique_ptr<Rng> rng_;
   fh_key* relin_ = nullptr;
   std::map<uint32_t, fh_key*> rot_;
+
+  // bootstrapping state (bootstrap.cpp)
+  friend struct BootstrapOps;
+  Ciphertext MultByMonomial(const Ciphertext& ct, uint32_t power);
+  std::shared_ptr<struct BootstrapState> boot_;
+  std::map<std::pair<uint32_t, uint32_t>, std::shared_ptr<DeviceBuffer>> monomials_;
 };
 
 }  // namespace fideslib

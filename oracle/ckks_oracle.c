@@ -532,6 +532,25 @@ void orc_rescale_ct(const orc_ctx* c, const u64* ct, int l, int ncomp, u64* out)
     orc_rescale_poly(c, ct + (size_t)k * (l + 1) * c->N, l, out + (size_t)k * l * c->N);
 }
 
+/* CKKS bootstrapping ModRaise, the raise step of src/CKKS/Bootstrap.cu:708-930 (tower 0 of each
+ * component, coefficient form, SwitchModulus to every tower: the broadcast of
+ * src/CKKS/ElemenwiseBatchKernels.cu:140-148), then back to EVALUATION form. ct: ncomp components
+ * of l_in + 1 limbs; out: ncomp components of l + 1 limbs. */
+void orc_mod_raise(const orc_ctx* c, const u64* ct, int l_in, int ncomp, int l, u64* out) {
+  u64 N = c->N;
+  u64* t = (u64*)malloc(N * 8);
+  for (int k = 0; k < ncomp; ++k) {
+    memcpy(t, ct + (size_t)k * (l_in + 1) * N, N * 8);
+    intt1(c, t, 0);
+    for (int i = 0; i <= l; ++i) {
+      u64* o = out + ((size_t)k * (l + 1) + i) * N;
+      for (u64 j = 0; j < N; ++j) o[j] = i == 0 ? t[j] : switch_modulus(t[j], c->q[0], c->q[i]);
+      ntt1(c, o, i);
+    }
+  }
+  free(t);
+}
+
 /* ------------------------------------------------------------------------- */
 /* hybrid key switching                                                      */
 /* ------------------------------------------------------------------------- */

@@ -103,6 +103,8 @@ uint32_t orc_rot_index(const orc_ctx* c, int r); /* 5^r mod 2N (r may be negativ
 void orc_relinearize(const orc_ctx* c, const uint64_t* ct3, int l, const uint64_t* key, uint64_t* out2);
 void orc_eval_mult(const orc_ctx* c, const uint64_t* ct1, const uint64_t* ct2, int l, const uint64_t* key, uint64_t* out2);
 void orc_rescale_ct(const orc_ctx* c, const uint64_t* ct, int l, int ncomp, uint64_t* out);
+/* bootstrapping ModRaise: tower 0 of ncomp components (l_in + 1 limbs each) lifted to l + 1 limbs */
+void orc_mod_raise(const orc_ctx* c, const uint64_t* ct, int l_in, int ncomp, int l, uint64_t* out);
 /* hoisted-formulation rotation: ext key switch of c1, fold P*c0, automorph, ModDown */
 void orc_eval_rotate(const orc_ctx* c, const uint64_t* ct, int l, const uint64_t* key, uint32_t k, uint64_t* out);
 /* rotation that stays in Q_l∪P (no ModDown): out is 2 x (l+1+K) limbs */

@@ -68,6 +68,7 @@ def lib():
             "orc_square": (V, [P, u64p, u64p, I]),
             "orc_rescale_poly": (V, [P, u64p, I, u64p]),
             "orc_rescale_ct": (V, [P, u64p, I, I, u64p]),
+            "orc_mod_raise": (V, [P, u64p, I, I, I, u64p]),
             "orc_num_digits": (I, [P, I]),
             "orc_modup": (V, [P, u64p, I, u64p]),
             "orc_ksk_inner": (V, [P, u64p, I, u64p, u64p, u64p]),
@@ -266,6 +267,13 @@ class Ctx:
     def rescale_ct(self, ct, l, ncomp=2):
         out = self.zeros(l, ncomp)
         lib().orc_rescale_ct(self.h, np.ascontiguousarray(ct), l, ncomp, out)
+        return out
+
+    def mod_raise(self, ct, l_in, l, ncomp=2):
+        """CKKS bootstrapping ModRaise (src/CKKS/Bootstrap.cu:708-930): tower 0 of each component
+        lifted centred to towers 0..l."""
+        out = self.zeros(l + 1, ncomp)
+        lib().orc_mod_raise(self.h, np.ascontiguousarray(ct), l_in, ncomp, l, out)
         return out
 
     def modup(self, x, l):

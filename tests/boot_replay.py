@@ -1,0 +1,291 @@
+"""The facade's CKKS bootstrap and Chebyshev series evaluation restated over the oracle's
+operations (test infrastructure): the same sequence of ModRaise, rotations, linear transforms,
+products, Rescales, scalar encodings and level reductions as
+fideslib-fork_amd/host/bootstrap.cpp, so that a GPU run and this replay can be compared residue by
+residue. Also computes the bootstrap constants the way the facade does (Chebyshev coefficients of
+the shifted cosine, the special-FFT matrices) for CPU-only precision checks.
+"""
+import math
+
+import numpy as np
+
+from oracle import oracle as O
+
+BABY = 8  # pieces below this degree are linear combinations of T_1 .. T_{BABY-1}
+
+
+# ---- constants (mirrors of the facade's double arithmetic) ----
+def cheb_coefficients(f, a, b, degree):
+    n = degree + 1
+    fx = [f(0.5 * (b - a) * math.cos(math.pi * (j + 0.5) / n) + 0.5 * (b + a)) for j in range(n)]
+    c = []
+    for k in range(n):
+        s = 0.0
+        for j in range(n):
+            s += fx[j] * math.cos(math.pi * k * (j + 0.5) / n)
+        c.append(2.0 * s / n)
+    return c
+
+
+def _bitrev(v):
+    n = len(v)
+    j = 0
+    for i in range(1, n):
+        bit = n >> 1
+        while j & bit:
+            j ^= bit
+            bit >>= 1
+        j ^= bit
+        if i < j:
+            v[i], v[j] = v[j], v[i]
+
+
+def special_fft(v, M):
+    v = list(v)
+    n = len(v)
+    _bitrev(v)
+    ln = 2
+    while ln <= n:
+        lh, lq = ln >> 1, ln << 2
+        for i in range(0, n, ln):
+            g = 1
+            for j in range(lh):
+                ang = 2.0 * math.pi * ((g % lq) * (M // lq)) / M
+                wr, wi = math.cos(ang), math.sin(ang)
+                a, b = v[i + j], v[i + j + lh]
+                tr, ti = b[0] * wr - b[1] * wi, b[0] * wi + b[1] * wr
+                v[i + j] = (a[0] + tr, a[1] + ti)
+                v[i + j + lh] = (a[0] - tr, a[1] - ti)
+                g = (g * 5) % M
+        ln <<= 1
+    return v
+
+
+def special_ifft(v, M):
+    v = list(v)
+    n = len(v)
+    ln = n
+    while ln >= 1:
+        lh, lq = ln >> 1, ln << 2
+        for i in range(0, n, ln):
+            g = 1
+            for j in range(lh):
+                ang = 2.0 * math.pi * ((lq - (g % lq)) * (M // lq)) / M
+                wr, wi = math.cos(ang), math.sin(ang)
+                a, b = v[i + j], v[i + j + lh]
+                ur, ui = a[0] + b[0], a[1] + b[1]
+                dr, di = a[0] - b[0], a[1] - b[1]
+                v[i + j] = (ur, ui)
+                v[i + j + lh] = (dr * wr - di * wi, dr * wi + di * wr)
+                g = (g * 5) % M
+        ln >>= 1
+    _bitrev(v)
+    return [(x[0] / n, x[1] / n) for x in v]
+
+
+def matrix_of(n, M, inverse):
+    A = np.zeros((n, n), dtype=np.complex128)
+    for k in range(n):
+        e = [(0.0, 0.0)] * n
+        e[k] = (1.0, 0.0)
+        col = special_ifft(e, M) if inverse else special_fft(e, M)
+        for j in range(n):
+            A[j, k] = complex(col[j][0], col[j][1])
+    return A
+
+
+def bsgs_diagonals(A, bStep):
+    """diag_k[j] = A[j][(j+k) mod n], rotated by -bStep*floor(k/bStep) (LinearTransform's contract)."""
+    n = A.shape[0]
+    D = []
+    for k in range(n):
+        s = bStep * (k // bStep)
+        D.append(np.array([A[(j - s) % n, ((j - s) % n + k) % n] for j in range(n)]))
+    return D
+
+
+def constants(N, n, hw, scale, q0, sparse=True):
+    gap = (N // 2) // n
+    bStep = 1
+    while bStep * bStep < n:
+        bStep *= 2
+    K = math.ceil(hw / 2.0) + 4.0 if sparse else 64.0
+    r = 0 if sparse else 5
+    degree = 127 if sparse else 63
+    cheb = cheb_coefficients(lambda y: math.cos((2.0 * math.pi * K * y - 0.5 * math.pi) / math.ldexp(1.0, r)), -1.0, 1.0, degree)
+    M = 2 * N
+    fc = scale / (2.0 * gap * q0 * K)
+    # an exact power of two 2^-e <= fc is applied as a scalar product before the trace; the
+    # diagonals carry fc 2^e in [1, 2)
+    e = math.ceil(-math.log2(fc))
+    fs = q0 / (2.0 * math.pi * scale)
+    return dict(slots=n, gap=gap, bStep=bStep, K=K, r=r, cheb=cheb, pre=math.ldexp(1.0, -e), cts=matrix_of(n, M, True) * (fc * math.ldexp(1.0, e)),
+                stc=matrix_of(n, M, False) * fs)
+
+
+# ---- ciphertext operations with the facade's metadata rules (towers t, noise degree) ----
+class Ops:
+    def __init__(self, o, scale, relin, rot):
+        self.o, self.scale, self.relin, self.rot = o, scale, relin, rot
+        self.q = [int(o.q(i)) for i in range(o.L1)]
+        self.N = o.N
+        self._mono = {}
+
+    @staticmethod
+    def comp(ct, t, i):
+        return ct[i * t:(i + 1) * t]
+
+    def reduce(self, c, t_to):
+        x, t, d = c
+        if t == t_to:
+            return c
+        assert t > t_to
+        return (np.concatenate([x[:t_to], x[t:t + t_to]]), t_to, d)
+
+    def add(self, a, b):
+        assert a[2] == b[2]
+        t = min(a[1], b[1])
+        a, b = self.reduce(a, t), self.reduce(b, t)
+        o = self.o
+        return (np.concatenate([o.add(self.comp(a[0], t, i), self.comp(b[0], t, i)) for i in range(2)]), t, a[2])
+
+    def neg(self, a):
+        t = a[1]
+        return (np.concatenate([self.o.neg(self.comp(a[0], t, i)) for i in range(2)]), t, a[2])
+
+    def sub(self, a, b):
+        return self.add(a, self.neg(b))
+
+    def add_scalar(self, a, c):
+        x, t, d = a
+        e = O.elem_for_eval_add(self.q[:t], abs(c), self.scale, d)
+        if c < 0:
+            e = np.array([(qi - int(v)) % qi for v, qi in zip(e, self.q[:t])], dtype=np.uint64)
+        return (np.concatenate([self.o.add_scalar(self.comp(x, t, 0), e), self.comp(x, t, 1)]), t, d)
+
+    def mul_scalar(self, a, c):
+        x, t, d = a
+        f = O.elem_for_eval_mult(self.q[:t], c, self.scale)
+        return (np.concatenate([self.o.mul_scalar(self.comp(x, t, i), f) for i in range(2)]), t, d + 1)
+
+    def mult(self, a, b):
+        t = min(a[1], b[1])
+        a, b = self.reduce(a, t), self.reduce(b, t)
+        return (self.o.eval_mult(a[0], b[0], t - 1, self.relin), t, a[2] + b[2])
+
+    def rescale(self, a):
+        x, t, d = a
+        return (self.o.rescale_ct(x, t - 1), t - 1, max(1, d - 1))
+
+    def rotate(self, a, k):
+        x, t, d = a
+        return (self.o.eval_rotate(x, t - 1, self.rot[k], k), t, d)
+
+    def monomial(self, a, power):
+        x, t, d = a
+        N = self.N
+        power %= 2 * N
+        key = (power, t)
+        if key not in self._mono:
+            m = np.zeros((t, N), dtype=np.uint64)
+            for i in range(t):
+                m[i, power % N] = 1 if power < N else self.q[i] - 1
+            self._mono[key] = self.o.ntt(m, list(range(t)))
+        mm = self._mono[key]
+        return (np.concatenate([self.o.mul(self.comp(x, t, i), mm) for i in range(2)]), t, d)
+
+    def linear_transform(self, a, diag_vals, bStep, n):
+        x, t, d = a
+        o, N = self.o, self.N
+        l = t - 1
+        diags = np.concatenate([o.encode_ext(v, self.scale, l) for v in diag_vals])
+        baby_keys, baby_k = [None] * bStep, [1] * bStep
+        for i in range(1, bStep):
+            g = o.rot_index(i)
+            baby_keys[i], baby_k[i] = self.rot[g], g
+        gk, gg = None, 1
+        if n > bStep:
+            gg = o.rot_index(bStep)
+            gk = self.rot[gg]
+        return (o.linear_transform(x, l, diags, bStep, 1, baby_keys, baby_k, gk, gg, slots=n), t, d + 1)
+
+
+def chebyshev_series(ops, y, coeffs, a=-1.0, b=1.0):
+    """EvalChebyshevSeries: c_0 / 2 + sum c_k T_k((2x - a - b) / (b - a))."""
+    if y[2] == 2:
+        y = ops.rescale(y)
+    if a != -1.0 or b != 1.0:
+        y = ops.rescale(ops.mul_scalar(y, 2.0 / (b - a)))
+        y = ops.add_scalar(y, -(a + b) / (b - a))
+    c = list(coeffs)
+    c[0] *= 0.5
+    T = {1: y}
+
+    def t(k):
+        if k in T:
+            return T[k]
+        if k % 2 == 0:
+            r = ops.rescale(ops.mult(t(k // 2), t(k // 2)))
+            r = ops.add_scalar(ops.add(r, r), -1.0)
+        else:
+            r = ops.rescale(ops.mult(t(k // 2), t(k // 2 + 1)))
+            r = ops.sub(ops.add(r, r), t(1))
+        T[k] = r
+        return r
+
+    def ev(c):
+        d = len(c) - 1
+        if d < BABY:
+            towers = min(t(k)[1] for k in range(1, d + 1))
+            acc = None
+            for k in range(1, d + 1):
+                if c[k] == 0.0:
+                    continue
+                term = ops.mul_scalar(ops.reduce(t(k), towers), c[k])
+                acc = term if acc is None else ops.add(acc, term)
+            if acc is None:
+                acc = ops.mul_scalar(ops.reduce(t(1), towers), 0.0)
+            return ops.add_scalar(ops.rescale(acc), c[0])
+        M = 1
+        while 2 * M <= d:
+            M *= 2
+        q = [0.0] * (d - M + 1)
+        r = list(c[:M])
+        q[0] = c[M]
+        for j in range(1, d - M + 1):
+            q[j] = 2.0 * c[M + j]
+            r[M - j] -= c[M + j]
+        Q, R = ev(q), ev(r)
+        return ops.add(ops.rescale(ops.mult(Q, t(M))), R)
+
+    return ev(c)
+
+
+def bootstrap(ops, ct, k):
+    """EvalBootstrap of ct = (residues, towers, noise degree) with constants k."""
+    o, N = ops.o, ops.N
+    n, bStep = k["slots"], k["bStep"]
+    if ct[2] == 2:
+        ct = ops.rescale(ct)
+    ct = ops.reduce(ct, 1)
+    L1 = o.L1
+    raised = (o.mod_raise(ct[0], 0, L1 - 1), L1, 1)
+    raised = ops.rescale(ops.mul_scalar(raised, k["pre"]))
+    s = n
+    while s < N // 2:
+        raised = ops.add(raised, ops.rotate(raised, o.rot_index(s)))
+        s *= 2
+    t = ops.rescale(ops.linear_transform(raised, bsgs_diagonals(k["cts"], bStep), bStep, n))
+    cj = ops.rotate(t, 2 * N - 1)
+    re = ops.add(t, cj)
+    im = ops.monomial(ops.sub(t, cj), 3 * N // 2)
+
+    def evalmod(y):
+        c = chebyshev_series(ops, y, k["cheb"])
+        for _ in range(k["r"]):
+            c = ops.rescale(ops.mult(c, c))
+            c = ops.add_scalar(ops.add(c, c), -1.0)
+        return c
+
+    u = ops.add(evalmod(re), ops.monomial(evalmod(im), N // 2))
+    return ops.rescale(ops.linear_transform(u, bsgs_diagonals(k["stc"], bStep), bStep, n))

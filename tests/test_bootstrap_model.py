@@ -1,0 +1,82 @@
+"""The bootstrap and Chebyshev-series sequences of the facade (restated in boot_replay over the
+oracle) decrypt to the right values: what pins the composition on the CPU. The GPU run of the
+facade is compared with this replay residue by residue in test_gpu_bootstrap.py."""
+import math
+
+import numpy as np
+import pytest
+
+import boot_replay as B
+from oracle import oracle as O
+
+
+def _ctx(logN=10, L1=16, K=6, dnum=3):
+    primes = O.gen_primes(logN, L1, 60, 50, K)
+    return O.Ctx(logN, primes, L1, K, dnum)
+
+
+def _keys(o, s, steps, N, seed):
+    relin = o.gen_relin_key(s, seed)
+    rot = {}
+    for i, g in enumerate(sorted({o.rot_index(r) for r in steps} | {2 * N - 1})):
+        rot[g] = o.gen_rot_key(s, g, seed + 1 + i)
+    return relin, rot
+
+
+def test_special_fft_matrices_are_inverse():
+    n, M = 16, 2 ** 11
+    U, Ui = B.matrix_of(n, M, False), B.matrix_of(n, M, True)
+    assert np.allclose(U @ Ui, np.eye(n), atol=1e-12)
+    # U evaluates at the slot roots: U[j][k] = xi^(5^j k), xi = exp(2 pi i / 4n) (the root of the
+    # subring in X^(N/2n) that n slots occupy)
+    xi = np.exp(2j * math.pi / (4 * n))
+    for j in range(n):
+        r = pow(5, j, 4 * n)
+        for k in range(n):
+            assert abs(U[j, k] - xi ** ((r * k) % (4 * n))) < 1e-12
+
+
+def test_chebyshev_coefficients_closed_form():
+    c = B.cheb_coefficients(lambda x: 3.0 * x ** 3 - x, -1.0, 1.0, 7)   # 3x^3 - x = 0.75 T3 + 1.25 T1
+    want = [0.0, 1.25, 0.0, 0.75, 0, 0, 0, 0]
+    assert np.allclose(c, want, atol=1e-13)
+
+
+def test_chebyshev_series_decrypts():
+    o = _ctx()
+    N, n, scale = o.N, 16, 2.0 ** 50
+    s = o.gen_secret(11, 16)
+    relin, rot = _keys(o, s, [], N, 20)
+    ops = B.Ops(o, scale, relin, rot)
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-2, 2, n)
+    l = o.L1 - 1
+    ct = (o.encrypt(o.encode(x, scale, l), l, s, 5), l + 1, 1)
+    f = lambda v: math.tanh(1.5 * v) + 0.25 * v * v
+    c = B.cheb_coefficients(f, -2.0, 2.0, 27)
+    out = B.chebyshev_series(ops, ct, c, -2.0, 2.0)
+    got = o.decode(o.decrypt(out[0], out[1] - 1, s), out[1] - 1, n, scale).real
+    assert np.max(np.abs(got - [f(v) for v in x])) < 1e-6
+    assert out[1] == l + 1 - 1 - 5     # input scaling 1 + degree 27 (depth 5)
+
+
+@pytest.mark.parametrize("n", [8, 64])
+def test_bootstrap_decrypts(n):
+    o = _ctx()
+    N, scale = o.N, 2.0 ** 50
+    hw = 16
+    s = o.gen_secret(7, hw)
+    k = B.constants(N, n, hw, scale, float(o.q(0)))
+    steps = [n * 2 ** j for j in range(int(math.log2(N // 2 // n)))] + list(range(1, k["bStep"])) + ([k["bStep"]] if n > k["bStep"] else [])
+    relin, rot = _keys(o, s, steps, N, 30)
+    ops = B.Ops(o, scale, relin, rot)
+    rng = np.random.default_rng(n)
+    z = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+    ct = (o.encrypt(o.encode(z, scale, 0), 0, s, 9), 1, 1)
+    out = B.bootstrap(ops, ct, k)
+    l = out[1] - 1
+    got = o.decode(o.decrypt(out[0], l, s), l, n, scale)
+    err = np.max(np.abs(got - z))
+    print(f"bootstrap n={n}: towers {out[1]}, max error {err:.3g} ({-math.log2(err):.1f} bits)")
+    assert out[1] == o.L1 - (2 + 8 + 0 + 1)
+    assert err < 2 ** -12

@@ -1,0 +1,138 @@
+"""Bootstrapping pieces on the GPU against the oracle, bit-exact: ModRaise (tower-0 lift to the
+full chain)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("cfgname,l_in,ncomp", [("cfg1", 0, 2), ("cfg1", 2, 3), ("cfg2", 0, 2), ("cfg2", 5, 2)])
+def test_mod_raise(cfgname, l_in, ncomp, gctx1, gctx2, octx1, octx2):
+    g, o = (gctx1, octx1) if cfgname == "cfg1" else (gctx2, octx2)
+    top = o.L1 - 1
+    ct = np.concatenate([o.random_poly(90 + 7 * k + l_in, list(range(l_in + 1))) for k in range(ncomp)])
+    print(f"[mod_raise] {cfgname} l_in={l_in} ncomp={ncomp}", flush=True)
+    d = g.to_device(ct)
+    for l in (top, top // 2):
+        out = g.array((ncomp * (l + 1), o.N))
+        g.mod_raise(d, l_in, ncomp, l, out)
+        assert np.array_equal(out.download(), o.mod_raise(ct, l_in, l, ncomp)), l
+        out.free()
+    d.free()
+
+
+# ---- the facade's EvalBootstrap / EvalChebyshevSeries (tests/cpp/bootstrap_check.cpp) ----
+import json
+import math
+import os
+import subprocess
+
+import boot_replay as B
+from conftest import PKG
+
+BCHECK = os.path.join(PKG, "_build", "bootstrap_check")
+
+
+@pytest.fixture(scope="module", params=[(12, 8), (12, 64)], ids=["N4096_slots8", "N4096_slots64"])
+def boot_run(request, tmp_path_factory):
+    logN, slots = request.param
+    if not os.path.exists(BCHECK):
+        pytest.fail("bootstrap_check is not built (run __graft_entry__.build())")
+    out = tmp_path_factory.mktemp(f"boot{slots}")
+    print(f"[bootstrap_check] {BCHECK} {out} {logN} {slots}", flush=True)
+    r = subprocess.run([BCHECK, str(out), str(logN), str(slots), "7"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    with open(out / "meta.json") as f:
+        meta = json.load(f)
+    return out, res, meta
+
+
+def test_bootstrap_precision(boot_run):
+    """Decrypted slots after EvalBootstrap are within 2^-12 of the input (N=2^12: ~15.8 bits at 8
+    slots, ~14.3 at 64 in the CPU model); a second bootstrap of the same ciphertext gives the same
+    residues; the result supports another product."""
+    out, res, meta = boot_run
+    print(res)
+    assert res["towers_in"] == 1 and res["towers_out"] == res["L1"] - res["depth"]
+    assert res["boot_err"] < 2 ** -12, res
+    assert res["deterministic"]
+    assert res["square_err"] < 2 ** -10
+    assert res["cheb_err_vs_series"] < 1e-7 and res["cheb_err_vs_function"] < 1e-6
+
+
+def _u64(out, name, rows, N):
+    return np.fromfile(out / f"{name}.u64", dtype=np.uint64).reshape(rows, N)
+
+
+def _c128(out, name):
+    a = np.fromfile(out / f"{name}.f64", dtype=np.float64)
+    return a[0::2] + 1j * a[1::2]
+
+
+def test_bootstrap_matches_oracle_replay(boot_run, oracle_mod):
+    """The oracle replays the session (keys and encryptions from the same seeds, the same
+    bootstrap and series sequences over the dumped constants): every residue agrees."""
+    out, res, meta = boot_run
+    O = oracle_mod
+    logN, n, seed, L1, K, scale = meta["logN"], meta["slots"], meta["seed"], meta["L1"], meta["K"], meta["scale"]
+    N = 1 << logN
+    moduli = np.array([int(v) for v in meta["moduli"]], dtype=np.uint64)
+    o = O.Ctx(logN, moduli, L1, K, meta["dnum"])
+    s = o.gen_secret(seed + 0, meta["hw"])
+    relin = o.gen_relin_key(s, seed + 2)
+    bStep = meta["bStep"]
+    steps = []
+    t = n
+    while t < N // 2:
+        steps.append(t)
+        t *= 2
+    steps += list(range(1, bStep)) + ([bStep] if n > bStep else [])
+    order = []
+    for r in steps:
+        g = o.rot_index(r)
+        if r % (N // 2) and g not in order:
+            order.append(g)
+    order.append(2 * N - 1)
+    rot = {g: o.gen_rot_key(s, g, seed + 3 + i) for i, g in enumerate(order)}
+    enc = seed + 3 + len(order)
+    z = _c128(out, "z")
+    ct = o.encrypt(o.encode(z, scale, 0), 0, s, enc)
+    assert np.array_equal(_u64(out, "ct", 2, N), ct), "input encryption differs"
+    k = dict(slots=n, bStep=bStep, K=meta["Krange"], r=meta["r"], pre=meta["pre"],
+             cheb=list(np.fromfile(out / "cheb.f64", dtype=np.float64)),
+             cts=_c128(out, "cts").reshape(n, n), stc=_c128(out, "stc").reshape(n, n))
+    ops = B.Ops(o, scale, relin, rot)
+    bt = B.bootstrap(ops, (ct, 1, 1), k)
+    assert bt[1] == meta["bt_towers"]
+    got = _u64(out, "bt", 2 * bt[1], N)
+    bad = np.count_nonzero(got != bt[0])
+    assert bad == 0, f"bootstrap: {bad} residues differ from the oracle replay"
+    # the Chebyshev series
+    x = _c128(out, "x")
+    cx = o.encrypt(o.encode(x, scale, L1 - 1), L1 - 1, s, enc + 1)
+    assert np.array_equal(_u64(out, "cx", 2 * L1, N), cx)
+    cy = B.chebyshev_series(ops, (cx, L1, 1), list(np.fromfile(out / "series.f64", dtype=np.float64)), -2.0, 2.0)
+    assert cy[1] == meta["cy_towers"]
+    assert np.array_equal(_u64(out, "cy", 2 * cy[1], N), cy[0]), "Chebyshev series differs from the oracle replay"
+
+
+def test_bootstrap_constants_closed_form(boot_run):
+    """The dumped transforms are the scaled special-FFT matrices and the series approximates the
+    shifted cosine (constants checked against their definitions, independently of the replay)."""
+    out, res, meta = boot_run
+    n, N = meta["slots"], 1 << meta["logN"]
+    q0, scale, Kr, r = float(meta["moduli"][0]), meta["scale"], meta["Krange"], meta["r"]
+    gap = N // 2 // n
+    xi = np.exp(2j * math.pi / (4 * n))
+    U = np.array([[xi ** ((pow(5, j, 4 * n) * kk) % (4 * n)) for kk in range(n)] for j in range(n)])
+    fc = scale / (2 * gap * q0 * Kr)
+    cts = _c128(out, "cts").reshape(n, n)
+    stc = _c128(out, "stc").reshape(n, n)
+    assert np.allclose(cts * meta["pre"], fc * np.linalg.inv(U), rtol=0, atol=1e-12 * fc)
+    assert np.allclose(stc, q0 / (2 * math.pi * scale) * U, rtol=0, atol=1e-9)
+    assert 1.0 <= fc / meta["pre"] < 2.0 and math.log2(meta["pre"]).is_integer()
+    cheb = np.fromfile(out / "cheb.f64", dtype=np.float64)
+    ys = np.linspace(-1, 1, 101)
+    approx = np.polynomial.chebyshev.chebval(ys, np.concatenate([[cheb[0] / 2], cheb[1:]]))
+    assert np.max(np.abs(approx - np.cos((2 * math.pi * Kr * ys - math.pi / 2) / 2 ** r))) < 1e-12

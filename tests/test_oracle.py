@@ -496,3 +496,40 @@ def test_elem_for_eval_add_closed_form(sc, operand, deg):
     else:
         k = math.ceil(math.log2(float(x))) - 61
         assert k > 0 and W % (1 << k) == 0 and abs(W - x) <= (1 << k)
+
+
+def test_mod_raise_closed_form(sc):
+    """Bootstrapping ModRaise: after the raise, tower i of every component holds the centred lift of
+    tower 0's coefficient (v - q_0 when v > q_0 / 2) mod q_i, tower 0 is untouched, and an
+    encryption of m at level 0 decrypts at the top level to m + q_0 I with small integers I."""
+    c = sc
+    top = c.L1 - 1
+    q0 = c.q(0)
+    for l_in in (0, 2):
+        ct = np.concatenate([c.random_poly(70 + l_in + k, list(range(l_in + 1))) for k in range(2)])
+        out = c.mod_raise(ct, l_in, top)
+        for k in range(2):
+            x0 = c.intt(ct[k * (l_in + 1):k * (l_in + 1) + 1], [0])[0]
+            oc = c.intt(out[k * (top + 1):(k + 1) * (top + 1)], list(range(top + 1)))
+            assert np.array_equal(out[k * (top + 1)], ct[k * (l_in + 1)])
+            for n in (0, 1, 5, c.N - 1):
+                v = int(x0[n])
+                cen = v - q0 if v > q0 // 2 else v
+                for i in range(top + 1):
+                    assert int(oc[i, n]) == cen % c.q(i), (l_in, k, n, i)
+    # decryption view: m + q0 * I, |I| small (sparse secret of weight 16)
+    s = c.gen_secret(5, 16)
+    z = _slots(8, 3)
+    m = c.encode(z, 2.0 ** 40, 0)
+    ct0 = c.encrypt(m, 0, s, 9)
+    up = c.mod_raise(ct0, 0, top)
+    dec = c.intt(c.decrypt(up, top, s), list(range(top + 1)))
+    mc = c.intt(c.decrypt(ct0, 0, s), [0])[0]
+    for n in range(0, c.N, 37):
+        X, M = crt([int(dec[i, n]) for i in range(top + 1)], [c.q(i) for i in range(top + 1)])
+        if X > M // 2:
+            X -= M
+        v = int(mc[n])
+        v = v - q0 if v > q0 // 2 else v
+        assert (X - v) % q0 == 0
+        assert abs((X - v) // q0) <= 16
