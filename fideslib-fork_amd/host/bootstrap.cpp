@@ -1,11 +1,14 @@
 // CKKS bootstrapping and Chebyshev series evaluation over the facade.
 //
-// The bootstrap follows the reference's pipeline (src/CKKS/Bootstrap.cu:437-648) at level budget
-// {1, 1}:
+// The bootstrap follows the reference's pipeline (src/CKKS/Bootstrap.cu:437-648):
 //   ModRaise (fh_mod_raise: tower 0 lifted centred to the whole chain, Bootstrap.cu:708-930)
 //   -> for sparse packing, the partial trace over the rotations by slots * 2^j
 //      (AccumulateBroadcast.cu:122-160), which keeps the coefficients at multiples of N/(2 slots)
-//   -> CoeffsToSlots: one BSGS LinearTransform by the inverse special FFT
+//   -> CoeffsToSlots: the inverse special FFT as levelBudget[0] BSGS LinearTransforms, one per
+//      group of butterfly stages (CoeffsToSlots.cu:75-151 applies the reference's factors the
+//      same way); the bit reversal that opens the special FFT is left out of both transforms,
+//      so the slots between them hold the coefficients in bit-reversed order, which EvalMod
+//      (slot-wise) does not see
 //   -> real and imaginary parts split with a conjugation and a multiplication by -i
 //      (the monomial X^(3N/2), Bootstrap.cu:577-584)
 //   -> EvalMod on both parts: a Chebyshev series of cos((2 pi K y - pi/2) / 2^r) on [-1, 1]
@@ -14,7 +17,15 @@
 //      quadruples the series' noise, which the final transform then scales by q_0 / Delta: at
 //      N=2^12 and 64 slots r = 3 with degree 63 measured 11.7 bits, r = 0 with degree 127 14.3
 //      bits, one level cheaper)
-//   -> recombine with i = X^(N/2) -> SlotsToCoeffs: one BSGS LinearTransform by the special FFT.
+//   -> recombine with i = X^(N/2) -> SlotsToCoeffs: the butterfly stages of the special FFT as
+//      levelBudget[1] BSGS LinearTransforms.
+// A group of m stages whose smallest butterfly half-length is h has its nonzero diagonals at the
+// multiples of h within +-(2^m - 1) h: one LinearTransform with stride h, an offset, and at most
+// 2^(m+1) - 1 diagonals.
+// With a scaling factor close to q_0 (e.g. 59-bit scaling primes under a 60-bit q_0), the input is
+// first multiplied by 2^-c and the output by the integer 2^c (the correction factor of
+// Bootstrap.cu:478-497, 640-641), so that the message stays ~2^-10 of q_0 while EvalMod runs at
+// the large scale.
 // The scale factors are folded into the transforms: CoeffsToSlots multiplies by
 // Delta / (2 gap q_0 K) so that each part enters EvalMod as (I + p / q_0) / K in [-1, 1] (the
 // power of two 2^-e nearest below that factor is applied first as an exact scalar product, so that
@@ -34,44 +45,132 @@ using detail::Cx;
 using detail::kPi;
 
 struct BootstrapState {
-  uint32_t slots = 0, gap = 1, bStep = 1;
+  uint32_t slots = 0, gap = 1;
   double K = 0;          // EvalMod input range: |I| < K
   int r = 0;             // double-angle steps
   uint32_t degree = 0;   // Chebyshev degree of the series
   double pre = 1.0;      // exact power of two applied after the raise
+  uint32_t correction = 0;
   std::vector<double> cheb;
-  std::vector<std::complex<double>> cts, stc;   // n x n matrices, row-major
-  std::vector<Plaintext> ctsDiags;              // at the top of the chain
-  std::map<uint32_t, std::vector<Plaintext>> stcDiags;  // by towers
+  std::vector<BootstrapGroup> cts, stc;         // transforms, in the order they are applied
+  std::vector<std::vector<Plaintext>> ctsDiags;  // per CoeffsToSlots group, at its level
+  std::map<uint32_t, std::vector<std::vector<Plaintext>>> stcDiags;  // by the towers entering SlotsToCoeffs
 };
 
 namespace {
 
-// matrix of a linear map on n complex slots, column k = map(e_k)
-std::vector<std::complex<double>> matrix_of(uint32_t n, uint64_t M, bool inverse) {
-  std::vector<std::complex<double>> A((size_t)n * n);
-  for (uint32_t k = 0; k < n; ++k) {
-    std::vector<Cx> v(n, Cx{0.0, 0.0});
-    v[k] = Cx{1.0, 0.0};
-    if (inverse) detail::special_ifft(v, M);
-    else detail::special_fft(v, M);
-    for (uint32_t j = 0; j < n; ++j) A[(size_t)j * n + k] = std::complex<double>(v[j].re, v[j].im);
-  }
-  return A;
+using Diag = std::map<uint32_t, std::vector<std::complex<double>>>;  // offset -> row values
+
+void put(Diag& D, uint32_t d, const std::vector<std::complex<double>>& v) {
+  auto it = D.find(d);
+  if (it == D.end()) D.emplace(d, v);
+  else
+    for (size_t i = 0; i < v.size(); ++i) it->second[i] += v[i];
 }
 
-// BSGS diagonals of A (y = A x over n slots): diag_k[j] = A[j][(j + k) mod n], rotated by
-// -bStep * floor(k / bStep) as LinearTransform expects (v'[j] = v[(j - s) mod n])
-std::vector<std::vector<std::complex<double>>> bsgs_diagonals(const std::vector<std::complex<double>>& A, uint32_t n, uint32_t bStep) {
-  std::vector<std::vector<std::complex<double>>> D(n, std::vector<std::complex<double>>(n));
-  for (uint32_t k = 0; k < n; ++k) {
-    const uint32_t s = bStep * (k / bStep);
-    for (uint32_t j = 0; j < n; ++j) {
-      const uint32_t src = (j + n - s % n) % n;  // rotation by -s
-      D[k][j] = A[(size_t)src * n + (src + k) % n];
+// Butterfly stages of special_fft after its bit reversal (or their inverses), in diagonal form:
+// row r of a stage reads column (r + offset) mod n.
+std::vector<Diag> fft_stages(uint32_t n, uint64_t M, bool inverse) {
+  std::vector<Diag> out;
+  for (uint32_t len = 2; len <= n; len <<= 1) {
+    const uint32_t lh = len >> 1;
+    const uint64_t lq = (uint64_t)len << 2;
+    std::vector<std::complex<double>> d0(n), dp(n), dm(n);
+    for (uint32_t i = 0; i < n; i += len) {
+      uint64_t g = 1;
+      for (uint32_t j = 0; j < lh; ++j) {
+        const uint64_t idx = (g % lq) * (M / lq);
+        const double ang = 2.0 * kPi * (double)idx / (double)M;
+        const std::complex<double> w(std::cos(ang), std::sin(ang));
+        const uint32_t p = i + j, q = i + j + lh;
+        if (inverse) {  // x_p = (y_p + y_q) / 2, x_q = (y_p - y_q) / (2 w)
+          d0[p] = 0.5, dp[p] = 0.5;
+          dm[q] = 0.5 / w, d0[q] = -0.5 / w;
+        } else {        // y_p = x_p + w x_q, y_q = x_p - w x_q
+          d0[p] = 1.0, dp[p] = w;
+          dm[q] = 1.0, d0[q] = -w;
+        }
+        g = (g * 5) % M;
+      }
     }
+    Diag st;
+    put(st, 0, d0);
+    put(st, lh % n, dp);
+    put(st, (n - lh) % n, dm);
+    out.push_back(std::move(st));
   }
-  return D;
+  return out;
+}
+
+// A B in diagonal form: offset d1 + d2, values A_d1[r] B_d2[r + d1]
+Diag compose(const Diag& A, const Diag& B, uint32_t n) {
+  Diag C;
+  for (const auto& a : A)
+    for (const auto& b : B) {
+      std::vector<std::complex<double>> v(n);
+      for (uint32_t r = 0; r < n; ++r) v[r] = a.second[r] * b.second[(r + a.first) % n];
+      put(C, (a.first + b.first) % n, v);
+    }
+  return C;
+}
+
+// LinearTransform shape of a group whose smallest butterfly half-length is `low`
+BootstrapGroup group_transform(const Diag& D, uint32_t n, uint32_t low) {
+  uint32_t span = 0;  // largest |offset| / low
+  for (const auto& d : D) {
+    const uint32_t a = d.first <= n / 2 ? d.first : n - d.first;
+    span = std::max(span, a / low);
+  }
+  BootstrapGroup g;
+  g.stride = low;
+  if (2 * span + 1 >= n / low) {
+    g.rowSize = n / low;
+    g.offset = 0;
+  } else {
+    g.rowSize = 2 * span + 1;
+    g.offset = (n - span * low % n) % n;
+  }
+  g.diags.assign(g.rowSize, std::vector<std::complex<double>>(n));
+  for (uint32_t k = 0; k < g.rowSize; ++k) {
+    auto it = D.find((g.offset + k * low) % n);
+    if (it != D.end()) g.diags[k] = it->second;
+  }
+  return g;
+}
+
+// groups of `budget` sizes (the first take the extra stages), in the order they are applied:
+// CoeffsToSlots from S_L^-1 down, SlotsToCoeffs from S_1 up; the first group carries `factor`
+std::vector<BootstrapGroup> factored(uint32_t n, uint64_t M, uint32_t budget, bool inverse, double factor) {
+  const std::vector<Diag> S = fft_stages(n, M, inverse);
+  const uint32_t L = (uint32_t)S.size();
+  const uint32_t b = std::max(1u, std::min(budget, L));
+  std::vector<uint32_t> order(L);
+  for (uint32_t i = 0; i < L; ++i) order[i] = inverse ? L - 1 - i : i;
+  std::vector<BootstrapGroup> groups;
+  uint32_t pos = 0;
+  for (uint32_t gi = 0; gi < b; ++gi) {
+    const uint32_t m = L / b + (gi < L % b ? 1 : 0);
+    Diag P;
+    put(P, 0, std::vector<std::complex<double>>(n, 1.0));
+    uint32_t low = n;
+    for (uint32_t t = 0; t < m; ++t) {
+      const uint32_t i = order[pos + t];
+      P = compose(S[i], P, n);
+      low = std::min(low, 1u << i);
+    }
+    pos += m;
+    if (groups.empty())
+      for (auto& d : P)
+        for (auto& v : d.second) v *= factor;
+    groups.push_back(group_transform(P, n, low));
+  }
+  return groups;
+}
+
+uint32_t bstep_for(uint32_t rowSize) {
+  uint32_t b = 1;
+  while (b * b < rowSize) b *= 2;
+  return b;
 }
 
 struct CosArg {
@@ -174,6 +273,20 @@ Ciphertext CryptoContextImpl::EvalChebyshevSeries(const Ciphertext& ct, const st
   return e.eval(c);
 }
 
+// ---- multiplication by an integer (no scale change: multIntScalar, ApproxModEval.cu:169-175) ----
+Ciphertext CryptoContextImpl::MultByInteger(const Ciphertext& ct, uint64_t v) {
+  std::vector<uint64_t> f(ct->towers);
+  for (uint32_t i = 0; i < ct->towers; ++i) f[i] = v % moduli_[i];
+  auto out = newCt(ct->towers, ct->ncomp);
+  for (uint32_t c = 0; c < ct->ncomp; ++c)
+    check(fh_eltwise_scalar(ctx_, 1, ct->data->ptr + (size_t)c * ct->towers * N_, f.data(), out->data->ptr + (size_t)c * ct->towers * N_,
+                            nullptr, (int)ct->towers));
+  out->noiseScaleDeg = ct->noiseScaleDeg;
+  out->scalingFactor = ct->scalingFactor;
+  out->slots = ct->slots;
+  return out;
+}
+
 // ---- multiplication by a monomial X^power (exact: a plaintext of one +-1 coefficient) ----
 Ciphertext CryptoContextImpl::MultByMonomial(const Ciphertext& ct, uint32_t power) {
   power %= 2 * N_;
@@ -197,19 +310,34 @@ Ciphertext CryptoContextImpl::MultByMonomial(const Ciphertext& ct, uint32_t powe
 }
 
 // ---- bootstrapping ----
+// a group's diagonals, pre-rotated by -(bStep floor(k / bStep) stride + offset) as LinearTransform
+// expects (v'[j] = v[(j - s) mod n]), encoded extended at OpenFHE level `level`
+std::vector<Plaintext> CryptoContextImpl::groupPlaintexts(const BootstrapGroup& g, uint32_t n, uint32_t level) {
+  const uint32_t b = bstep_for(g.rowSize);
+  std::vector<Plaintext> pts;
+  for (uint32_t k = 0; k < g.rowSize; ++k) {
+    const uint32_t sft = (b * (k / b) * g.stride + g.offset) % n;
+    std::vector<std::complex<double>> v(n);
+    for (uint32_t j = 0; j < n; ++j) v[j] = g.diags[k][(j + n - sft) % n];
+    pts.push_back(MakeCKKSPackedPlaintextExtended(v, 1, level, n));
+  }
+  return pts;
+}
+
 void CryptoContextImpl::EvalBootstrapSetup(const std::vector<uint32_t>& levelBudget, const std::vector<uint32_t>& dim1, uint32_t slots,
                                            uint32_t correctionFactor) {
-  if (levelBudget.size() != 2 || levelBudget[0] != 1 || levelBudget[1] != 1)
-    throw std::runtime_error("fideslib: EvalBootstrapSetup: this build implements level budget {1, 1}");
+  if (levelBudget.size() != 2 || levelBudget[0] < 1 || levelBudget[1] < 1)
+    throw std::runtime_error("fideslib: EvalBootstrapSetup: the level budget is two counts >= 1");
   (void)dim1;
-  if (correctionFactor != 0) throw std::runtime_error("fideslib: EvalBootstrapSetup: correction factors are not implemented");
+  if (correctionFactor > 40) throw std::runtime_error("fideslib: EvalBootstrapSetup: correction factor above 40");
   const uint32_t n = slots ? slots : N_ / 2;
   if (n < 2 || (n & (n - 1)) || n > N_ / 2) throw std::runtime_error("fideslib: bootstrap slots must be a power of two in [2, N/2]");
   auto st = std::make_shared<BootstrapState>();
   st->slots = n;
   st->gap = (N_ / 2) / n;
-  st->bStep = 1;
-  while (st->bStep * st->bStep < n) st->bStep *= 2;
+  // 0 = automatic: keep the message 2^-10 below q_0
+  const int room = (int)params_.first_bits_ - (int)params_.scale_bits_;
+  st->correction = correctionFactor ? correctionFactor : (uint32_t)std::max(0, 10 - room);
   // EvalMod range and shape: |I| <= (h + 1) / 2 for a sparse secret of weight h
   const bool sparse = params_.dist_ == SecretKeyDist::SPARSE_TERNARY;
   st->K = sparse ? std::ceil(params_.hw_ / 2.0) + 4.0 : 64.0;
@@ -223,37 +351,43 @@ void CryptoContextImpl::EvalBootstrapSetup(const std::vector<uint32_t>& levelBud
   const int e = (int)std::ceil(-std::log2(fc));
   st->pre = std::ldexp(1.0, -e);
   const double fd = fc * std::ldexp(1.0, e);
-  st->cts = matrix_of(n, M, true);
-  for (auto& v : st->cts) v *= fd;
-  st->stc = matrix_of(n, M, false);
-  for (auto& v : st->stc) v *= fs;
-  const auto D = bsgs_diagonals(st->cts, n, st->bStep);
-  for (uint32_t k = 0; k < n; ++k) st->ctsDiags.push_back(MakeCKKSPackedPlaintextExtended(D[k], 1, 1, n));
+  st->cts = factored(n, M, levelBudget[0], true, fd);
+  st->stc = factored(n, M, levelBudget[1], false, fs);
+  // CoeffsToSlots group g runs at OpenFHE level 1 + g (after the prescale)
+  for (size_t g = 0; g < st->cts.size(); ++g) st->ctsDiags.push_back(groupPlaintexts(st->cts[g], n, (uint32_t)(1 + g)));
   boot_ = st;
 }
 
 void CryptoContextImpl::EvalBootstrapKeyGen(const PrivateKey& sk, uint32_t slots) {
   if (!boot_ || boot_->slots != (slots ? slots : N_ / 2)) throw std::runtime_error("fideslib: EvalBootstrapSetup for these slots first");
+  const uint32_t n = boot_->slots;
   std::vector<int32_t> steps;
-  for (uint32_t s = boot_->slots; s < N_ / 2; s *= 2) steps.push_back((int32_t)s);  // partial trace
-  for (uint32_t i = 1; i < boot_->bStep; ++i) steps.push_back((int32_t)i);             // baby steps
-  if (boot_->slots > boot_->bStep) steps.push_back((int32_t)boot_->bStep);              // giant step
+  for (uint32_t s = n; s < N_ / 2; s *= 2) steps.push_back((int32_t)s);  // partial trace
+  std::vector<BootstrapGroup> all = boot_->cts;
+  all.insert(all.end(), boot_->stc.begin(), boot_->stc.end());
+  for (const auto& g : all) {
+    const uint32_t b = bstep_for(g.rowSize);
+    for (uint32_t i = 1; i < b; ++i) steps.push_back((int32_t)((i * g.stride) % n));  // baby steps
+    if (g.rowSize > b) steps.push_back((int32_t)((b * g.stride) % n));                // giant step
+    if (g.offset) steps.push_back((int32_t)g.offset);
+  }
   EvalRotateKeyGen(sk, steps);
   EvalConjugateKeyGen(sk);
 }
 
 BootstrapConstants CryptoContextImpl::GetBootstrapConstants() const {
   if (!boot_) throw std::runtime_error("fideslib: EvalBootstrapSetup first");
-  return BootstrapConstants{boot_->slots, boot_->bStep, boot_->K, boot_->r, boot_->pre, boot_->cheb, boot_->cts, boot_->stc};
+  return BootstrapConstants{boot_->slots, boot_->correction, boot_->K, boot_->r, boot_->pre, boot_->cheb, boot_->cts, boot_->stc};
 }
 
 uint32_t CryptoContextImpl::GetBootstrapDepth() const {
   if (!boot_) throw std::runtime_error("fideslib: EvalBootstrapSetup first");
-  // prescale 1 + CoeffsToSlots 1 + series + double angle r + SlotsToCoeffs 1; the series of degree
-  // d >= 7 takes ceil(log2(d + 1)) + 1 levels (pieces below degree 8 take 4)
+  // prescale 1 + one per CoeffsToSlots group + series + double angle r + one per SlotsToCoeffs
+  // group; the series of degree d >= 7 takes ceil(log2(d + 1)) + 1 levels (pieces below degree 8
+  // take 4)
   uint32_t series = 1;
   while ((1u << series) < boot_->degree + 1) ++series;
-  return 2 + (series + 1) + (uint32_t)boot_->r + 1;
+  return 1 + (uint32_t)boot_->cts.size() + (series + 1) + (uint32_t)boot_->r + (uint32_t)boot_->stc.size();
 }
 
 Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0, uint32_t numIterations, uint32_t precision) {
@@ -265,6 +399,10 @@ Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0, uint32_t numI
   const uint32_t n = st.slots;
   if (GetBootstrapDepth() + 1 > L1_) throw std::runtime_error("fideslib: EvalBootstrap: the chain is too short for the bootstrap depth");
   Ciphertext ct = ct0->noiseScaleDeg == 2 ? Rescale(ct0) : ct0;
+  if (st.correction) {
+    if (ct->towers < 2) throw std::runtime_error("fideslib: EvalBootstrap with a correction factor needs two towers");
+    ct = Rescale(EvalMult(ct, std::ldexp(1.0, -(int)st.correction)));
+  }
   ct = LevelReduce(ct, ct->towers - 1);
   // 1. raise tower 0 to the whole chain: decrypts to p + q_0 I
   auto raised = newCt(L1_, 2);
@@ -276,7 +414,10 @@ Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0, uint32_t numI
   // 2. sparse packing: trace onto the coefficients at multiples of gap (times gap)
   for (uint32_t s = n; s < N_ / 2; s *= 2) raised = EvalAdd(raised, EvalRotate(raised, (int32_t)s));
   // 3. CoeffsToSlots: slots (I + p / q_0) / (2K), real and imaginary coefficient halves
-  Ciphertext t = Rescale(LinearTransform(raised, (int)n, (int)st.bStep, st.ctsDiags));
+  Ciphertext t = raised;
+  for (size_t g = 0; g < st.cts.size(); ++g)
+    t = Rescale(LinearTransform(t, (int)st.cts[g].rowSize, (int)bstep_for(st.cts[g].rowSize), st.ctsDiags[g], (int)st.cts[g].stride,
+                                (int)st.cts[g].offset));
   // 4. real and imaginary parts: t + conj(t), -i (t - conj(t))
   Ciphertext cj = EvalConjugate(t);
   Ciphertext re = EvalAdd(t, cj);
@@ -295,12 +436,15 @@ Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0, uint32_t numI
   Ciphertext u = EvalAdd(fre, MultByMonomial(fim, N_ / 2));
   auto sit = st.stcDiags.find(u->towers);
   if (sit == st.stcDiags.end()) {
-    const auto D = bsgs_diagonals(st.stc, n, st.bStep);
-    std::vector<Plaintext> pts;
-    for (uint32_t k = 0; k < n; ++k) pts.push_back(MakeCKKSPackedPlaintextExtended(D[k], 1, L1_ - u->towers, n));
+    std::vector<std::vector<Plaintext>> pts;
+    for (size_t g = 0; g < st.stc.size(); ++g) pts.push_back(groupPlaintexts(st.stc[g], n, L1_ - u->towers + (uint32_t)g));
     sit = st.stcDiags.emplace(u->towers, std::move(pts)).first;
   }
-  Ciphertext out = Rescale(LinearTransform(u, (int)n, (int)st.bStep, sit->second));
+  Ciphertext out = u;
+  for (size_t g = 0; g < st.stc.size(); ++g)
+    out = Rescale(LinearTransform(out, (int)st.stc[g].rowSize, (int)bstep_for(st.stc[g].rowSize), sit->second[g], (int)st.stc[g].stride,
+                                  (int)st.stc[g].offset));
+  if (st.correction) out = MultByInteger(out, 1ull << st.correction);
   out->slots = ct0->slots;
   return out;
 }

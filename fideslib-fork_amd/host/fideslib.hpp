@@ -126,14 +126,22 @@ struct KeyPair {
   PrivateKey secretKey;
 };
 
-// the precomputation of EvalBootstrapSetup (matrices row-major over `slots` slots)
+// one LinearTransform of the factored CoeffsToSlots / SlotsToCoeffs: diagonal k multiplies the
+// slots rotated by offset + k stride (values before the BSGS pre-rotation)
+struct BootstrapGroup {
+  uint32_t rowSize = 0, stride = 1, offset = 0;
+  std::vector<std::vector<std::complex<double>>> diags;
+};
+
+// the precomputation of EvalBootstrapSetup
 struct BootstrapConstants {
-  uint32_t slots, bStep;
+  uint32_t slots;
+  uint32_t correction;        // input scaled by 2^-correction, output by 2^correction
   double K;                   // EvalMod input range
   int r;                      // double-angle steps
   double pre;                 // power of two applied after the raise
   std::vector<double> cheb;   // Chebyshev coefficients of cos((2 pi K y - pi/2) / 2^r) on [-1, 1]
-  std::vector<std::complex<double>> cts, stc;
+  std::vector<BootstrapGroup> cts, stc;  // in the order they are applied
 };
 
 class CryptoContextImpl;
@@ -211,8 +219,9 @@ class CryptoContextImpl {
                                                        uint32_t degree);
 
   // ---- bootstrapping (api/CryptoContext.hpp:90-107, 423-424; src/CKKS/Bootstrap.cu:437-648).
-  // Level budget {1, 1}: CoeffsToSlots and SlotsToCoeffs are one BSGS linear transform each over
-  // the `slots` slots. EvalBootstrapKeyGen needs EvalMultKeyGen to have run.
+  // levelBudget {b1, b2}: CoeffsToSlots and SlotsToCoeffs are b1 and b2 BSGS linear transforms
+  // over the `slots` slots (groups of the special FFT's butterfly stages). EvalBootstrapKeyGen
+  // needs EvalMultKeyGen to have run.
   void EvalBootstrapSetup(const std::vector<uint32_t>& levelBudget = {1, 1}, const std::vector<uint32_t>& dim1 = {0, 0},
                           uint32_t slots = 0, uint32_t correctionFactor = 0);
   void EvalBootstrapKeyGen(const PrivateKey& sk, uint32_t slots);
@@ -257,6 +266,8 @@ class CryptoContextImpl {
   // bootstrapping state (bootstrap.cpp)
   friend struct BootstrapOps;
   Ciphertext MultByMonomial(const Ciphertext& ct, uint32_t power);
+  Ciphertext MultByInteger(const Ciphertext& ct, uint64_t v);
+  std::vector<Plaintext> groupPlaintexts(const BootstrapGroup& g, uint32_t n, uint32_t level);
   std::shared_ptr<struct BootstrapState> boot_;
   std::map<std::pair<uint32_t, uint32_t>, std::shared_ptr<DeviceBuffer>> monomials_;
 };

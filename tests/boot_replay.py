@@ -94,21 +94,119 @@ def matrix_of(n, M, inverse):
     return A
 
 
-def bsgs_diagonals(A, bStep):
-    """diag_k[j] = A[j][(j+k) mod n], rotated by -bStep*floor(k/bStep) (LinearTransform's contract)."""
-    n = A.shape[0]
-    D = []
-    for k in range(n):
-        s = bStep * (k // bStep)
-        D.append(np.array([A[(j - s) % n, ((j - s) % n + k) % n] for j in range(n)]))
-    return D
+# ---- the special FFT factored into butterfly stages (level budgets > 1) ----
+def _put(D, d, v):
+    D[d] = D[d] + v if d in D else v
 
 
-def constants(N, n, hw, scale, q0, sparse=True):
+def fft_stages(n, M, inverse):
+    """Butterfly stages of special_fft after its bit reversal in diagonal form {offset: vector}
+    (row r of the stage reads column r + offset); inverse: the inverse of each stage."""
+    out = []
+    ln = 2
+    while ln <= n:
+        lh, lq = ln >> 1, ln << 2
+        d0, dp, dm = np.zeros(n, complex), np.zeros(n, complex), np.zeros(n, complex)
+        for i in range(0, n, ln):
+            g = 1
+            for j in range(lh):
+                ang = 2.0 * math.pi * ((g % lq) * (M // lq)) / M
+                w = complex(math.cos(ang), math.sin(ang))
+                p, q = i + j, i + j + lh
+                if inverse:   # x_p = (y_p + y_q) / 2, x_q = (y_p - y_q) / (2 w)
+                    d0[p], dp[p] = 0.5, 0.5
+                    dm[q], d0[q] = 0.5 / w, -0.5 / w
+                else:         # y_p = x_p + w x_q, y_q = x_p - w x_q
+                    d0[p], dp[p] = 1.0, w
+                    dm[q], d0[q] = 1.0, -w
+                g = (g * 5) % M
+        st = {0: d0}
+        _put(st, lh % n, dp)
+        _put(st, (-lh) % n, dm)
+        out.append(st)
+        ln <<= 1
+    return out
+
+
+def compose(A, Bm, n):
+    """A B in diagonal form: offset d1 + d2, value A_d1[r] B_d2[r + d1]."""
+    C = {}
+    for d1 in sorted(A):
+        for d2 in sorted(Bm):
+            _put(C, (d1 + d2) % n, A[d1] * np.roll(Bm[d2], -d1))
+    return C
+
+
+def split_levels(L, b):
+    """stage counts of b groups (the first groups take the extra stages)"""
+    b = max(1, min(b, L))
+    return [L // b + (1 if i < L % b else 0) for i in range(b)]
+
+
+def group_transform(D, n, low):
+    """LinearTransform shape of a group whose smallest butterfly half-length is `low`:
+    (rowSize, stride, offset, diagonals d_k for the rotations offset + k stride)."""
+    span = max(abs(((d + n // 2) % n) - n // 2) for d in D) // low   # largest |offset| / stride
+    if 2 * span + 1 >= n // low:
+        rowSize, offset = n // low, 0
+    else:
+        rowSize, offset = 2 * span + 1, (-span * low) % n
+    diags = []
+    for k in range(rowSize):
+        d = (offset + k * low) % n
+        diags.append(D.get(d, np.zeros(n, complex)))
+    return rowSize, low, offset, diags
+
+
+def factored(n, M, budget, inverse, factor):
+    """Groups of the factored CoeffsToSlots (inverse: S_1^-1 .. S_L^-1, applied from S_L^-1) or
+    SlotsToCoeffs (S_L .. S_1, applied from S_1), in the order they are applied; `factor` scales
+    the first group."""
+    S = fft_stages(n, M, inverse)
+    L = len(S)
+    order = list(range(L - 1, -1, -1)) if inverse else list(range(L))
+    groups = []
+    pos = 0
+    for m in split_levels(L, budget):
+        idx = order[pos:pos + m]
+        pos += m
+        P = {0: np.ones(n, complex)}
+        for i in idx:
+            P = compose(S[i], P, n)
+        if not groups:
+            P = {d: v * factor for d, v in P.items()}
+        groups.append(group_transform(P, n, 1 << min(idx)))
+    return groups
+
+
+def bstep_for(rowSize):
+    b = 1
+    while b * b < rowSize:
+        b *= 2
+    return b
+
+
+def rotation_steps(k, N):
+    """slot rotations a bootstrap with constants k uses: the partial trace, then every group's
+    baby steps, giant step and offset"""
+    n = k["slots"]
+    steps = []
+    t = n
+    while t < N // 2:
+        steps.append(t)
+        t *= 2
+    for rowSize, stride, offset, _ in k["cts"] + k["stc"]:
+        b = bstep_for(rowSize)
+        steps += [(i * stride) % n for i in range(1, b)]
+        if rowSize > b:
+            steps.append((b * stride) % n)
+        if offset:
+            steps.append(offset)
+    return [r for r in steps if r % (N // 2)]
+
+
+def constants(N, n, hw, scale, q0, sparse=True, budget=(1, 1), correction=0):
     gap = (N // 2) // n
-    bStep = 1
-    while bStep * bStep < n:
-        bStep *= 2
     K = math.ceil(hw / 2.0) + 4.0 if sparse else 64.0
     r = 0 if sparse else 5
     degree = 127 if sparse else 63
@@ -119,8 +217,8 @@ def constants(N, n, hw, scale, q0, sparse=True):
     # diagonals carry fc 2^e in [1, 2)
     e = math.ceil(-math.log2(fc))
     fs = q0 / (2.0 * math.pi * scale)
-    return dict(slots=n, gap=gap, bStep=bStep, K=K, r=r, cheb=cheb, pre=math.ldexp(1.0, -e), cts=matrix_of(n, M, True) * (fc * math.ldexp(1.0, e)),
-                stc=matrix_of(n, M, False) * fs)
+    return dict(slots=n, gap=gap, K=K, r=r, cheb=cheb, pre=math.ldexp(1.0, -e), correction=correction,
+                cts=factored(n, M, budget[0], True, fc * math.ldexp(1.0, e)), stc=factored(n, M, budget[1], False, fs))
 
 
 # ---- ciphertext operations with the facade's metadata rules (towers t, noise degree) ----
@@ -173,6 +271,11 @@ class Ops:
         a, b = self.reduce(a, t), self.reduce(b, t)
         return (self.o.eval_mult(a[0], b[0], t - 1, self.relin), t, a[2] + b[2])
 
+    def mul_integer(self, a, v):
+        x, t, d = a
+        f = np.array([v % qi for qi in self.q[:t]], dtype=np.uint64)
+        return (np.concatenate([self.o.mul_scalar(self.comp(x, t, i), f) for i in range(2)]), t, d)
+
     def rescale(self, a):
         x, t, d = a
         return (self.o.rescale_ct(x, t - 1), t - 1, max(1, d - 1))
@@ -194,20 +297,32 @@ class Ops:
         mm = self._mono[key]
         return (np.concatenate([self.o.mul(self.comp(x, t, i), mm) for i in range(2)]), t, d)
 
-    def linear_transform(self, a, diag_vals, bStep, n):
+    def linear_transform(self, a, group, n):
+        """LinearTransform over `n` slots of a (rowSize, stride, offset, diagonals) group: the
+        diagonals are pre-rotated by -(bStep floor(k / bStep) stride + offset) and encoded
+        extended at the ciphertext's level."""
         x, t, d = a
-        o, N = self.o, self.N
+        o = self.o
         l = t - 1
-        diags = np.concatenate([o.encode_ext(v, self.scale, l) for v in diag_vals])
+        rowSize, stride, offset, dv = group
+        bStep = bstep_for(rowSize)
+        pre = [np.roll(dv[k], (bStep * (k // bStep) * stride + offset) % n) for k in range(rowSize)]
+        diags = np.concatenate([o.encode_ext(v, self.scale, l) for v in pre])
         baby_keys, baby_k = [None] * bStep, [1] * bStep
         for i in range(1, bStep):
-            g = o.rot_index(i)
+            if (i * stride) % n == 0:
+                continue
+            g = o.rot_index(i * stride)
             baby_keys[i], baby_k[i] = self.rot[g], g
         gk, gg = None, 1
-        if n > bStep:
-            gg = o.rot_index(bStep)
+        if rowSize > bStep and (bStep * stride) % n:
+            gg = o.rot_index(bStep * stride)
             gk = self.rot[gg]
-        return (o.linear_transform(x, l, diags, bStep, 1, baby_keys, baby_k, gk, gg, slots=n), t, d + 1)
+        ok, og = None, 1
+        if offset:
+            og = o.rot_index(offset)
+            ok = self.rot[og]
+        return (o.linear_transform(x, l, diags, bStep, stride, baby_keys, baby_k, gk, gg, ok, og, offset=offset, slots=n), t, d + 1)
 
 
 def chebyshev_series(ops, y, coeffs, a=-1.0, b=1.0):
@@ -264,9 +379,12 @@ def chebyshev_series(ops, y, coeffs, a=-1.0, b=1.0):
 def bootstrap(ops, ct, k):
     """EvalBootstrap of ct = (residues, towers, noise degree) with constants k."""
     o, N = ops.o, ops.N
-    n, bStep = k["slots"], k["bStep"]
+    n = k["slots"]
+    corr = k.get("correction", 0)
     if ct[2] == 2:
         ct = ops.rescale(ct)
+    if corr:
+        ct = ops.rescale(ops.mul_scalar(ct, math.ldexp(1.0, -corr)))
     ct = ops.reduce(ct, 1)
     L1 = o.L1
     raised = (o.mod_raise(ct[0], 0, L1 - 1), L1, 1)
@@ -275,7 +393,9 @@ def bootstrap(ops, ct, k):
     while s < N // 2:
         raised = ops.add(raised, ops.rotate(raised, o.rot_index(s)))
         s *= 2
-    t = ops.rescale(ops.linear_transform(raised, bsgs_diagonals(k["cts"], bStep), bStep, n))
+    t = raised
+    for g in k["cts"]:
+        t = ops.rescale(ops.linear_transform(t, g, n))
     cj = ops.rotate(t, 2 * N - 1)
     re = ops.add(t, cj)
     im = ops.monomial(ops.sub(t, cj), 3 * N // 2)
@@ -288,4 +408,8 @@ def bootstrap(ops, ct, k):
         return c
 
     u = ops.add(evalmod(re), ops.monomial(evalmod(im), N // 2))
-    return ops.rescale(ops.linear_transform(u, bsgs_diagonals(k["stc"], bStep), bStep, n))
+    for g in k["stc"]:
+        u = ops.rescale(ops.linear_transform(u, g, n))
+    if corr:
+        u = ops.mul_integer(u, 1 << corr)
+    return u

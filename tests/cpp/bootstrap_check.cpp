@@ -7,7 +7,8 @@
 // The inputs, outputs and bootstrap constants are written to <out_dir> so that the test can replay
 // the same sequence with the oracle and compare residues.
 //
-// usage: bootstrap_check <out_dir> <logN> <slots> [seed]   (prints one JSON line)
+// usage: bootstrap_check <out_dir> <logN> <slots> [seed] [cts_budget stc_budget depth scale_bits]
+// (prints one JSON line)
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -76,11 +77,14 @@ int main(int argc, char** argv) {
   const int logN = std::atoi(argv[2]);
   const uint32_t slots = (uint32_t)std::atoi(argv[3]);
   const uint64_t seed = argc > 4 ? std::strtoull(argv[4], nullptr, 10) : 7;
+  const uint32_t b1 = argc > 5 ? (uint32_t)std::atoi(argv[5]) : 1, b2 = argc > 6 ? (uint32_t)std::atoi(argv[6]) : 1;
+  const uint32_t depth = argc > 7 ? (uint32_t)std::atoi(argv[7]) : 15;
+  const uint32_t sbits = argc > 8 ? (uint32_t)std::atoi(argv[8]) : 50;
   try {
     CCParams p;
     p.SetRingDim(1u << logN);
-    p.SetMultiplicativeDepth(15);
-    p.SetScalingModSize(50);
+    p.SetMultiplicativeDepth(depth);
+    p.SetScalingModSize(sbits);
     p.SetFirstModSize(60);
     p.SetNumLargeDigits(3);
     p.SetSecretKeyDist(SecretKeyDist::SPARSE_TERNARY);
@@ -90,7 +94,7 @@ int main(int argc, char** argv) {
     auto cc = GenCryptoContext(p);
     auto kp = cc->KeyGen();
     cc->EvalMultKeyGen(kp.secretKey);
-    cc->EvalBootstrapSetup({1, 1}, {0, 0}, slots);
+    cc->EvalBootstrapSetup({b1, b2}, {0, 0}, slots);
     cc->EvalBootstrapKeyGen(kp.secretKey, slots);
     const uint32_t L1 = cc->GetNumQ();
 
@@ -99,8 +103,9 @@ int main(int argc, char** argv) {
     cvec z(slots);
     for (auto& v : z) v = {U(rng), U(rng)};
 
-    // a ciphertext with one tower left, as after a computation
-    auto pt = cc->MakeCKKSPackedPlaintext(z, 1, L1 - 1, slots);
+    // a ciphertext with the towers the bootstrap needs left (one, two with a correction factor)
+    const uint32_t tin = cc->GetBootstrapConstants().correction ? 2 : 1;
+    auto pt = cc->MakeCKKSPackedPlaintext(z, 1, L1 - tin, slots);
     auto ct = cc->Encrypt(kp.secretKey, pt);
     auto bt = cc->EvalBootstrap(ct);
     auto dec = cc->Decrypt(kp.secretKey, bt);
@@ -156,24 +161,35 @@ int main(int argc, char** argv) {
     write_ct(*cc, "cy", cy);
     write_raw("cheb.f64", bc.cheb.data(), bc.cheb.size() * 8);
     write_raw("series.f64", coeffs.data(), coeffs.size() * 8);
-    write_cplx("cts", bc.cts);
-    write_cplx("stc", bc.stc);
+    std::string groups;
+    for (int t = 0; t < 2; ++t) {
+      const auto& G = t ? bc.stc : bc.cts;
+      groups += std::string(t ? ", \"stc\": [" : "\"cts\": [");
+      for (size_t g = 0; g < G.size(); ++g) {
+        std::vector<std::complex<double>> flat;
+        for (const auto& d : G[g].diags) flat.insert(flat.end(), d.begin(), d.end());
+        write_cplx(std::string(t ? "stc" : "cts") + std::to_string(g), flat);
+        groups += (g ? ", " : "") + std::string("[") + std::to_string(G[g].rowSize) + ", " + std::to_string(G[g].stride) + ", " +
+                  std::to_string(G[g].offset) + "]";
+      }
+      groups += "]";
+    }
     {
       FILE* f = std::fopen((g_dir + "/meta.json").c_str(), "w");
       std::fprintf(f, "{\"logN\": %d, \"slots\": %u, \"seed\": %llu, \"L1\": %u, \"K\": %u, \"dnum\": 3, \"hw\": 16, "
-                      "\"scale\": %.17g, \"bStep\": %u, \"Krange\": %.17g, \"r\": %d, \"pre\": %.17g, \"moduli\": [",
-                   logN, slots, (unsigned long long)seed, L1, (unsigned)(cc->GetModuli().size() - L1), cc->ScalingFactor(), bc.bStep, bc.K, bc.r,
-                   bc.pre);
+                      "\"scale\": %.17g, \"Krange\": %.17g, \"r\": %d, \"pre\": %.17g, \"correction\": %u, %s, \"moduli\": [",
+                   logN, slots, (unsigned long long)seed, L1, (unsigned)(cc->GetModuli().size() - L1), cc->ScalingFactor(), bc.K, bc.r, bc.pre,
+                   bc.correction, groups.c_str());
       for (size_t i = 0; i < cc->GetModuli().size(); ++i)
         std::fprintf(f, "%s\"%llu\"", i ? ", " : "", (unsigned long long)cc->GetModuli()[i]);
       std::fprintf(f, "], \"bt_towers\": %u, \"cy_towers\": %u}\n", bt->towers, cy->towers);
       std::fclose(f);
     }
 
-    std::printf("{\"logN\": %d, \"slots\": %u, \"L1\": %u, \"depth\": %u, \"towers_in\": %u, \"towers_out\": %u, "
+    std::printf("{\"logN\": %d, \"slots\": %u, \"budget\": [%u, %u], \"L1\": %u, \"depth\": %u, \"towers_in\": %u, \"towers_out\": %u, "
                 "\"boot_err\": %.6g, \"boot_bits\": %.2f, \"deterministic\": %s, \"square_err\": %.6g, "
                 "\"cheb_towers_out\": %u, \"cheb_err_vs_series\": %.6g, \"cheb_err_vs_function\": %.6g, \"boot_ms\": %.3f}\n",
-                logN, slots, L1, cc->GetBootstrapDepth(), ct->towers, bt->towers, e1, -std::log2(e1), same ? "true" : "false", e2,
+                logN, slots, b1, b2, L1, cc->GetBootstrapDepth(), ct->towers, bt->towers, e1, -std::log2(e1), same ? "true" : "false", e2,
                 cy->towers, e3, e4, ms[1]);
   } catch (const std::exception& ex) {
     std::fprintf(stderr, "error: %s\n", ex.what());

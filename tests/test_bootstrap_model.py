@@ -10,8 +10,8 @@ import boot_replay as B
 from oracle import oracle as O
 
 
-def _ctx(logN=10, L1=16, K=6, dnum=3):
-    primes = O.gen_primes(logN, L1, 60, 50, K)
+def _ctx(logN=10, L1=16, K=6, dnum=3, sbits=50):
+    primes = O.gen_primes(logN, L1, 60, sbits, K)
     return O.Ctx(logN, primes, L1, K, dnum)
 
 
@@ -60,23 +60,45 @@ def test_chebyshev_series_decrypts():
     assert out[1] == l + 1 - 1 - 5     # input scaling 1 + degree 27 (depth 5)
 
 
-@pytest.mark.parametrize("n", [8, 64])
-def test_bootstrap_decrypts(n):
-    o = _ctx()
-    N, scale = o.N, 2.0 ** 50
+def test_factored_transforms_compose_to_the_special_fft():
+    """The butterfly groups of every level budget multiply out to the scaled special FFT (SlotsToCoeffs)
+    and its inverse up to the bit-reversal permutation that the two transforms cancel."""
+    n, M = 64, 2 ** 11
+    U = B.matrix_of(n, M, False)
+    P = np.zeros((n, n))
+    for k in range(n):
+        P[int(format(k, "06b")[::-1], 2), k] = 1
+    for budget in (1, 2, 3, 6):
+        for inverse, f in ((False, 3.0), (True, 0.5)):
+            A = np.eye(n, dtype=complex)
+            for rowSize, stride, offset, dv in B.factored(n, M, budget, inverse, f):
+                G = np.zeros((n, n), dtype=complex)
+                for kk in range(rowSize):
+                    d = (offset + kk * stride) % n
+                    G[np.arange(n), (np.arange(n) + d) % n] += dv[kk]
+                A = G @ A
+            want = f * (np.linalg.inv(U @ np.linalg.inv(P)) if inverse else U @ np.linalg.inv(P))
+            assert np.max(np.abs(A - want)) < 1e-12, (budget, inverse)
+
+
+@pytest.mark.parametrize("n,budget,L1,sbits,corr,bits", [(8, (1, 1), 16, 50, 0, 19), (64, (1, 1), 16, 50, 0, 17), (64, (2, 3), 18, 50, 0, 14),
+                                                        (512, (3, 3), 18, 50, 0, 12), (512, (3, 3), 19, 59, 9, 20)])
+def test_bootstrap_decrypts(n, budget, L1, sbits, corr, bits):
+    o = _ctx(L1=L1, K=9 if sbits > 50 else 6, sbits=sbits)
+    N, scale = o.N, 2.0 ** sbits
     hw = 16
     s = o.gen_secret(7, hw)
-    k = B.constants(N, n, hw, scale, float(o.q(0)))
-    steps = [n * 2 ** j for j in range(int(math.log2(N // 2 // n)))] + list(range(1, k["bStep"])) + ([k["bStep"]] if n > k["bStep"] else [])
-    relin, rot = _keys(o, s, steps, N, 30)
+    k = B.constants(N, n, hw, scale, float(o.q(0)), budget=budget, correction=corr)
+    relin, rot = _keys(o, s, B.rotation_steps(k, N), N, 30)
     ops = B.Ops(o, scale, relin, rot)
     rng = np.random.default_rng(n)
     z = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
-    ct = (o.encrypt(o.encode(z, scale, 0), 0, s, 9), 1, 1)
+    tin = 2 if corr else 1
+    ct = (o.encrypt(o.encode(z, scale, tin - 1), tin - 1, s, 9), tin, 1)
     out = B.bootstrap(ops, ct, k)
     l = out[1] - 1
     got = o.decode(o.decrypt(out[0], l, s), l, n, scale)
     err = np.max(np.abs(got - z))
-    print(f"bootstrap n={n}: towers {out[1]}, max error {err:.3g} ({-math.log2(err):.1f} bits)")
-    assert out[1] == o.L1 - (2 + 8 + 0 + 1)
-    assert err < 2 ** -12
+    print(f"bootstrap n={n} budget={budget}: towers {out[1]}, max error {err:.3g} ({-math.log2(err):.1f} bits)")
+    assert out[1] == o.L1 - (1 + len(k["cts"]) + 8 + 0 + len(k["stc"]))
+    assert err < 2.0 ** -bits
