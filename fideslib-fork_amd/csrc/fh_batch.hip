@@ -86,7 +86,7 @@ __global__ __launch_bounds__(kT) void k_mulpt_jobs(const MulPtJob* __restrict__ 
 // ------------------------------------------------------------------------------ ModUp conversion
 struct ModUpJob {
   const uint64_t* coef;   // INTT of the key-switched polynomial (l+1 limbs)
-  const uint64_t* eval;   // the polynomial itself (l+1 limbs)
+  const uint64_t* eval;   // the polynomial itself (l+1 limbs), copied to the digit's own limbs; null: not copied
   uint64_t* out;          // digit d's block: l+1+K limbs
   uint32_t l, d;
 };
@@ -148,7 +148,7 @@ __global__ __launch_bounds__(kT) void k_modup_conv_jobs(const ModUpJob* __restri
         }
         y0[i] = M::prep(v0);
         y1[i] = M::prep(v1);
-        *reinterpret_cast<ulonglong2*>(j.out + o) = *reinterpret_cast<const ulonglong2*>(j.eval + o);
+        if (j.eval) *reinterpret_cast<ulonglong2*>(j.out + o) = *reinterpret_cast<const ulonglong2*>(j.eval + o);
       } else {
         y0[i] = M::zero_op();
         y1[i] = M::zero_op();
@@ -187,6 +187,7 @@ struct KskItem {
   const uint64_t* dig;    // beta x (l+1+K) limbs
   const uint64_t* c0;     // rotation: c0 over Q_l, folded as P c0; null otherwise
   uint64_t* ext;          // 2 x (l+1+K) limbs
+  const uint64_t* own;    // the key-switched polynomial (EVAL, l+1 limbs): digit d's own limbs
   uint32_t l, pad;
 };
 
@@ -241,6 +242,7 @@ __global__ __launch_bounds__(kT) void k_ksk_jobs(const KskItem* __restrict__ ite
       const uint32_t pos = qprime ? p : lq + (p - kc.L1);
       const uint32_t beta = min(kc.dnum, (lq + kc.alpha - 1) / kc.alpha);
       const uint64_t* dg = it.dig + (size_t)pos * N + n;
+      const uint32_t own_d = it.own && qprime ? pos / kc.alpha : ~0u;
       M a0, a1, b0, b1;
       a0.zero();
       a1.zero();
@@ -249,7 +251,7 @@ __global__ __launch_bounds__(kT) void k_ksk_jobs(const KskItem* __restrict__ ite
 #pragma unroll
       for (uint32_t d = 0; d < kMaxDnumB; ++d) {
         if (d < beta) {
-          const ulonglong2 x = *reinterpret_cast<const ulonglong2*>(dg + (size_t)d * ext * N);
+          const ulonglong2 x = *reinterpret_cast<const ulonglong2*>(d == own_d ? it.own + (size_t)pos * N + n : dg + (size_t)d * ext * N);
           const Op x0 = M::prep(x.x), x1 = M::prep(x.y);
           a0.mac(x0, kb0[d]);
           a1.mac(x0, ka0[d]);
@@ -751,7 +753,7 @@ void Batch::finalize() {
       ks_out.push_back(LimbJob{coef + (size_t)j * N, j, 0});
     }
     for (uint32_t d = 0; d < beta; ++d) {
-      mj.push_back(ModUpJob{coef, k.x, dig + (size_t)d * ext * N, (uint32_t)k.l, d});
+      mj.push_back(ModUpJob{coef, nullptr, dig + (size_t)d * ext * N, (uint32_t)k.l, d});
       const uint32_t start = d * c.alpha, sz = std::min<uint32_t>(c.alpha, lq - start);
       for (uint32_t pos = 0; pos < ext; ++pos) {
         if (pos >= start && pos < start + sz) continue;
@@ -788,7 +790,7 @@ void Batch::finalize() {
     KskGroup g{k0.key, k0.galois == 1 ? 1u : (uint32_t)h_inv_odd(k0.galois, 2ull * c.N), (uint32_t)ki.size(), 0, 0};
     while (t < order.size() && ks[order[t]].key == k0.key && ks[order[t]].galois == k0.galois && g.count < kGroupChunk) {
       const Ks& k = ks[order[t]];
-      ki.push_back(KskItem{A + k.dig, k.c0, A + k.ext, (uint32_t)k.l, 0});
+      ki.push_back(KskItem{A + k.dig, k.c0, A + k.ext, k.x, (uint32_t)k.l, 0});
       g.max_lq = std::max<uint32_t>(g.max_lq, k.l + 1);
       ++g.count;
       ++t;

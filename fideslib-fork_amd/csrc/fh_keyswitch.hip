@@ -43,7 +43,7 @@ __global__ __launch_bounds__(kT) void k_modup_conv(const uint64_t* __restrict__ 
   // blockIdx.y = poly * beta + digit; in_coef is packed (lq limbs per poly), out is beta*ext limbs per poly
   const uint32_t d = blockIdx.y % mp.beta, b = blockIdx.y / mp.beta;
   in_coef += (size_t)b * mp.lq * mp.N;
-  in_eval += (size_t)b * mp.in_stride;
+  if (in_eval) in_eval += (size_t)b * mp.in_stride;
   out += (size_t)b * mp.beta * mp.ext * mp.N;
   const uint32_t start = d * mp.alpha;
   const uint32_t sz = min(mp.alpha, mp.lq - start);
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kT) void k_modup_conv(const uint64_t* __restrict__ 
     for (uint32_t i = 0; i < sz; ++i) {
       const size_t o = (size_t)(start + i) * mp.N + n;
       y[i] = shoup(in_coef[o], s_inv[i], s_invsh[i], s_q[i]);
-      od[o] = in_eval[o];
+      if (in_eval) od[o] = in_eval[o];
     }
     for (uint32_t pos = 0; pos < mp.ext; ++pos) {
       if (pos == start) { pos += sz - 1; continue; }
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kT) void k_modup_conv_r(const uint64_t* __restrict_
   __shared__ uint64_t s_pq[kMaxPrimes], s_pn[kMaxPrimes];
   const uint32_t d = blockIdx.y % mp.beta, b = blockIdx.y / mp.beta;
   in_coef += (size_t)b * mp.lq * mp.N;
-  in_eval += (size_t)b * mp.in_stride;
+  if (in_eval) in_eval += (size_t)b * mp.in_stride;
   out += (size_t)b * mp.beta * mp.ext * mp.N;
   const uint32_t start = d * mp.alpha;
   const uint32_t sz = min(mp.alpha, mp.lq - start);
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(kT) void k_modup_conv_r(const uint64_t* __restrict_
         }
         y0[i] = M::prep(v0);
         y1[i] = M::prep(v1);
-        *reinterpret_cast<ulonglong2*>(od + o) = *reinterpret_cast<const ulonglong2*>(in_eval + o);
+        if (in_eval) *reinterpret_cast<ulonglong2*>(od + o) = *reinterpret_cast<const ulonglong2*>(in_eval + o);
       } else {
         y0[i] = M::zero_op();
         y1[i] = M::zero_op();
@@ -177,6 +177,8 @@ struct KskParams {
   int c0_ext;               // 1: c0 is already P-scaled over Q_l ∪ P, added on every limb
   const uint64_t* pmod;     // [L1] [P]_{q_i}
   const uint64_t* pmod_sh;
+  const uint64_t* own;      // optional: the key-switched polynomial in EVAL form (lq limbs); digit d's
+  uint32_t alpha;           // own limbs are read from it instead of from the digits
 };
 
 // blockIdx.y = extended position. out0 = sum_d digit_d b_d (+ P c0), out1 = sum_d digit_d a_d,
@@ -196,8 +198,9 @@ __global__ __launch_bounds__(kT) void k_ksk_inner(const uint64_t* __restrict__ d
     M a0, a1;
     a0.zero();
     a1.zero();
+    const uint32_t own_d = kp.own && pos < kp.lq ? pos / kp.alpha : ~0u;
     for (uint32_t d = 0; d < kp.beta; ++d) {
-      const auto x = M::prep(digits[((size_t)d * kp.ext + pos) * N + n]);
+      const auto x = M::prep(d == own_d ? kp.own[(size_t)pos * N + n] : digits[((size_t)d * kp.ext + pos) * N + n]);
       a0.mac(x, M::prep(kp.key[(((size_t)d * 2 + 0) * E + p) * N + n]));
       a1.mac(x, M::prep(kp.key[(((size_t)d * 2 + 1) * E + p) * N + n]));
     }
@@ -245,6 +248,9 @@ struct KskBatchParams {
   const uint64_t* cts;
   const uint64_t* pmod;
   const uint64_t* pmod_sh;
+  const uint64_t* own;      // optional: source b's key-switched polynomial (EVAL, lq limbs) at
+  uint64_t own_stride;      // own + b * own_stride; digit d's own limbs are read from it
+  uint32_t alpha;
 };
 
 template <class M>
@@ -283,6 +289,8 @@ __global__ __launch_bounds__(kT) void k_ksk_batch(const uint64_t* __restrict__ d
       const uint32_t b = g.order[first + t];
       const uint32_t src = b / kp.per_src;
       const uint64_t* dg = digits + src * kp.dig_stride + (size_t)pos * N + n;
+      const uint32_t own_d = kp.own && pos < kp.lq ? pos / kp.alpha : ~0u;
+      const uint64_t* ow = kp.own ? kp.own + src * kp.own_stride + (size_t)pos * N + n : nullptr;
       M a0, a1, b0, b1;
       a0.zero();
       a1.zero();
@@ -291,7 +299,7 @@ __global__ __launch_bounds__(kT) void k_ksk_batch(const uint64_t* __restrict__ d
 #pragma unroll
       for (uint32_t d = 0; d < kMaxDnum; ++d) {
         if (d < kp.beta) {
-          const ulonglong2 x = *reinterpret_cast<const ulonglong2*>(dg + (size_t)d * kp.ext * N);
+          const ulonglong2 x = *reinterpret_cast<const ulonglong2*>(d == own_d ? ow : dg + (size_t)d * kp.ext * N);
           const Op x0 = M::prep(x.x), x1 = M::prep(x.y);
           a0.mac(x0, kb0[d]);
           a1.mac(x0, ka0[d]);
@@ -508,7 +516,7 @@ inline dim3 grid_n(uint32_t N, uint32_t y, uint32_t per_thread = 1) {
 
 void modup(Context& c, const uint64_t* x, int l, uint64_t* digits, hipStream_t st) { modup_batch(c, x, l, 1, 0, digits, st); }
 
-void modup_batch(Context& c, const uint64_t* x, int l, int npolys, uint64_t x_stride, uint64_t* digits, hipStream_t st) {
+void modup_batch(Context& c, const uint64_t* x, int l, int npolys, uint64_t x_stride, uint64_t* digits, hipStream_t st, bool copy_own) {
   const uint32_t lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);
   if (c.K < 1) throw Error(-1, "key switching needs at least one special prime");
   if ((uint32_t)c.alpha > (uint32_t)kMaxAlpha) throw Error(-3, "digit size above kernel limit");
@@ -526,8 +534,9 @@ void modup_batch(Context& c, const uint64_t* x, int l, int npolys, uint64_t x_st
   {
     ProfScope ps_(c, "modup_conv", st);
     const dim3 g2 = grid_n(c.N, beta * npolys, 2);
-    if (!c.redc_conv_ok || c.alpha > 16) k_modup_conv<<<grid_n(c.N, beta * npolys), kT, 0, st>>>(tmp.p, x, digits, mp);
-    else FH_CONV_DISPATCH(c, c.alpha, k_modup_conv_r, (g2, kT, 0, st), (tmp.p, x, digits, mp))
+    const uint64_t* own = copy_own ? x : nullptr;
+    if (!c.redc_conv_ok || c.alpha > 16) k_modup_conv<<<grid_n(c.N, beta * npolys), kT, 0, st>>>(tmp.p, own, digits, mp);
+    else FH_CONV_DISPATCH(c, c.alpha, k_modup_conv_r, (g2, kT, 0, st), (tmp.p, own, digits, mp))
   }
   FH_HIP(hipGetLastError());
   ModUpAddr a{digits, ext, lq, (uint32_t)c.alpha, (uint32_t)c.L1, beta, (uint32_t)npolys, (uint64_t)beta * ext * c.N};
@@ -535,10 +544,10 @@ void modup_batch(Context& c, const uint64_t* x, int l, int npolys, uint64_t x_st
 }
 
 void ksk_inner(Context& c, const uint64_t* digits, int l, const uint64_t* key, uint64_t* out0, uint64_t* out1,
-               const uint64_t* c0, uint32_t galois, hipStream_t st, bool c0_ext) {
+               const uint64_t* c0, uint32_t galois, hipStream_t st, bool c0_ext, const uint64_t* own) {
   const uint32_t lq = l + 1, ext = lq + c.K;
   KskParams kp{c.N, lq, ext, (uint32_t)c.L1, (uint32_t)c.K, (uint32_t)c.num_digits(l), (uint32_t)c.logN, galois,
-               c.d_prime, key, c0, c0_ext ? 1 : 0, c.d_pmod, c.d_pmod_sh};
+               c.d_prime, key, c0, c0_ext ? 1 : 0, c.d_pmod, c.d_pmod_sh, own, (uint32_t)c.alpha};
   if (galois != 1) {
     // the kernel scatters through the inverse map: pass k^-1 so that out[j] = in[src_k(j)]
     kp.galois = (uint32_t)h_inv_odd(galois, 2ull * c.N);
@@ -642,8 +651,8 @@ void relinearize(Context& c, const uint64_t* ct3, int l, const uint64_t* key, ui
   const uint32_t lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);
   const size_t P = (size_t)lq * c.N;
   DBuf dig(c.pool, (size_t)beta * ext * c.N), t(c.pool, (size_t)2 * ext * c.N), ks(c.pool, 2 * P);
-  modup(c, ct3 + 2 * P, l, dig.p, st);
-  ksk_inner(c, dig.p, l, key, t.p, t.p + (size_t)ext * c.N, nullptr, 1, st);
+  modup_batch(c, ct3 + 2 * P, l, 1, 0, dig.p, st, false);
+  ksk_inner(c, dig.p, l, key, t.p, t.p + (size_t)ext * c.N, nullptr, 1, st, false, ct3 + 2 * P);
   moddown(c, t.p, l, ks.p, 2, 0, 0, st);
   std::vector<int> ids(lq);
   for (uint32_t i = 0; i < lq; ++i) ids[i] = i;
@@ -663,8 +672,8 @@ void rotate_ext(Context& c, const uint64_t* ct, int l, const uint64_t* key, uint
   const uint32_t lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);
   const size_t P = (size_t)lq * c.N;
   DBuf dig(c.pool, (size_t)beta * ext * c.N);
-  modup(c, ct + P, l, dig.p, st);
-  ksk_inner(c, dig.p, l, key, out_ext, out_ext + (size_t)ext * c.N, add_c0 ? ct : nullptr, galois, st);
+  modup_batch(c, ct + P, l, 1, 0, dig.p, st, false);
+  ksk_inner(c, dig.p, l, key, out_ext, out_ext + (size_t)ext * c.N, add_c0 ? ct : nullptr, galois, st, false, ct + P);
 }
 
 void eval_rotate(Context& c, const uint64_t* ct, int l, const uint64_t* key, uint32_t galois, uint64_t* out2, hipStream_t st) {
@@ -680,7 +689,7 @@ void eval_rotate(Context& c, const uint64_t* ct, int l, const uint64_t* key, uin
 // add_c0 folds c0: P c0 over Q_l (c0_ext false) or an extended c0 added on every limb.
 void ksk_items(Context& c, int nb, int per_src, const uint64_t* digits, uint64_t dig_stride, const uint64_t* c0s, uint64_t c0_stride,
                bool add_c0, bool c0_ext, int l, const uint64_t* const* keys, const uint32_t* galois, uint64_t* eo, uint64_t out_stride,
-               hipStream_t st) {
+               hipStream_t st, const uint64_t* own, uint64_t own_stride) {
   const uint32_t lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);
   if (beta > (uint32_t)kMaxDnum) throw Error(-3, "dnum above kernel limit");
   if (per_src < 1) throw Error(-1, "per_src must be >= 1");
@@ -692,7 +701,8 @@ void ksk_items(Context& c, int nb, int per_src, const uint64_t* digits, uint64_t
     return galois[a] < galois[b];
   });
   KskBatchParams kp{c.N, lq, ext, (uint32_t)c.L1, (uint32_t)c.K, beta, (uint32_t)c.logN, c0_stride, dig_stride, out_stride,
-                    (uint32_t)per_src, add_c0 ? 1 : 0, c0_ext ? 1 : 0, c.d_prime, c0s, c.d_pmod, c.d_pmod_sh};
+                    (uint32_t)per_src, add_c0 ? 1 : 0, c0_ext ? 1 : 0, c.d_prime, c0s, c.d_pmod, c.d_pmod_sh, own, own_stride,
+                    (uint32_t)c.alpha};
   int i = 0;
   while (i < nb) {
     RotGroups g{};
@@ -738,7 +748,8 @@ void rotate_core(Context& c, int nb, const uint64_t* digits, uint64_t dig_stride
   } else {
     eo = out;
   }
-  ksk_items(c, nb, 1, digits, dig_stride, xs, ct_stride, add_c0, false, l, keys, galois, eo, 2ull * ext * c.N, st);
+  ksk_items(c, nb, 1, digits, dig_stride, xs, ct_stride, add_c0, false, l, keys, galois, eo, 2ull * ext * c.N, st, xs + (size_t)lq * c.N,
+            ct_stride);
   if (do_moddown) moddown(c, eo, l, out, 2 * nb, 0, 0, st);
 }
 
@@ -758,7 +769,7 @@ void rotate_batch(Context& c, int B, const uint64_t* cts, uint64_t ct_stride, in
     const int nb = std::min(B - b0, kMaxBatch);
     const uint64_t* xs = cts + (size_t)b0 * ct_stride;
     DBuf dig(c.pool, (size_t)nb * beta * ext * c.N);
-    modup_batch(c, xs + (size_t)lq * c.N, l, nb, ct_stride, dig.p, st);
+    modup_batch(c, xs + (size_t)lq * c.N, l, nb, ct_stride, dig.p, st, false);
     rotate_core(c, nb, dig.p, (uint64_t)beta * ext * c.N, xs, ct_stride, l, keys + b0, galois + b0, add_c0, do_moddown,
                 out + (size_t)b0 * out_ct, st);
   }
@@ -772,7 +783,7 @@ void rotate_hoisted(Context& c, const uint64_t* ct, int l, int n, const uint64_t
   if (beta > (uint32_t)kMaxDnum) throw Error(-3, "dnum above kernel limit");
   const uint64_t out_ct = do_moddown ? 2ull * lq * c.N : 2ull * ext * c.N;
   DBuf dig(c.pool, (size_t)beta * ext * c.N);
-  modup(c, ct + (size_t)lq * c.N, l, dig.p, st);
+  modup_batch(c, ct + (size_t)lq * c.N, l, 1, 0, dig.p, st, false);
   for (int b0 = 0; b0 < n; b0 += kMaxBatch) {
     const int nb = std::min(n - b0, kMaxBatch);
     rotate_core(c, nb, dig.p, 0, ct, 0, l, keys + b0, galois + b0, add_c0, do_moddown, out + (size_t)b0 * out_ct, st);
