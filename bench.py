@@ -249,6 +249,8 @@ def main():
     lt = lt_workload(fh, ctx, a) if full and a.lt_count > 0 else None
     serving = serving_sweep(ctx, dev_keys[:8], gal[:8], dev_keys[0], a) if full and a.serve_requests > 0 else None
     boot = bootstrap_run() if full and not a.no_bootstrap else None
+    if boot is not None and serving is not None:
+        serving["with_bootstrap"] = serving_bootstrap_run(a)
 
     # ---- CPU baseline: the oracle on a bounded sample of the same rotation ----
     cpu = None
@@ -470,6 +472,26 @@ def bootstrap_run():
             "deterministic": res["deterministic"]}
 
 
+def serving_bootstrap_run(a):
+    """A serving session whose arrivals include bootstrap requests (tests/cpp/serve_check.cpp):
+    the bootstrap context above (N=2^16, 25 towers, full-slot bootstrapping), 25% bootstrap, 37.5%
+    inference, 37.5% polynomial requests at 40 requests/s, batched and per-operation; the same
+    checksum from both."""
+    import subprocess
+    exe = os.path.join(ROOT, "fideslib-fork_amd", "_build", "serve_check")
+    if not os.path.exists(exe):
+        return {"error": "serve_check is not built"}
+    args = ["16", "32768", "3", "3", "24", "59", "40", "32", "0.25", str(a.seed), "0.375"]
+    r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        return {"error": (r.stdout + r.stderr)[-500:]}
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    keep = ("p50_ms", "p99_ms", "mean_ms", "requests_per_s", "mean_batch_ops", "bootstraps", "inference", "polynomial")
+    return {"rate_per_s": 40.0, "requests": 32, "bootstrap_share": 0.25, "towers": res["L1"],
+            "batched": {k: res["batched"][k] for k in keep}, "per_op": {k: res["per_op"][k] for k in keep},
+            "same_checksum": res["batched"]["checksum"] == res["per_op"]["checksum"]}
+
+
 def serving_sweep(ctx, rot_keys, rot_gal, relin, a):
     """Continuous-batching serving sessions (fh_serve_run) at N=2^16, requests arriving at level 24:
     74% inference (x*w + Rescale, rotate-and-sum over 8 doublings, degree-7 polynomial), 26%
@@ -478,7 +500,7 @@ def serving_sweep(ctx, rot_keys, rot_gal, relin, a):
     assert STEPS16[:8] == [1 << k for k in range(8)]
     rates = [float(r) for r in a.serve_rates.split(",") if r]
     out = {"requests": a.serve_requests, "mix": "74% inference (dot product + degree-7 polynomial), 26% depth-6 polynomial",
-           "bootstrap_share": "not served (bootstrapping runs through the C++ facade, above the serving loop; see the bootstrap entry)", "sessions": []}
+           "bootstrap_share": "0 in these sessions (synthetic keys); with_bootstrap: a session with bootstrap requests on real keys", "sessions": []}
     keep = ("p50_ms", "p99_ms", "mean_ms", "requests_per_s", "mean_batch_ops", "max_active", "ticks")
     for i, rate in enumerate(rates):
         r = ctx.serve(rot_keys, rot_gal, relin, rate, a.serve_requests, mode=0, seed=a.seed + i)

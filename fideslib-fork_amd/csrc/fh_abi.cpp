@@ -595,6 +595,9 @@ int fh_serve_run(fh_context* ctx, const fh_serve_config* config, const fh_key* c
     cfg.inference_share = config->inference_share;
     cfg.seed = config->seed;
     cfg.checksum = config->checksum != 0;
+    cfg.bootstrap_share = config->bootstrap_share;
+    cfg.bootstrap = config->bootstrap;
+    cfg.bootstrap_user = config->bootstrap_user;
     if (cfg.mode != 0 && cfg.mode != 1) throw fh::Error(FH_E_INVALID, "mode must be 0 (batched) or 1 (per operation)");
     const uint64_t* rk[8];
     for (int k = 0; k < 8; ++k) rk[k] = key_of(c, rot_keys[k]);
@@ -612,6 +615,7 @@ int fh_serve_run(fh_context* ctx, const fh_serve_config* config, const fh_key* c
     result->polynomial = r.polynomial;
     result->max_active = r.max_active;
     result->checksum = r.checksum;
+    result->bootstraps = r.bootstraps;
   });
 }
 

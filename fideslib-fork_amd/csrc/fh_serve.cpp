@@ -15,7 +15,7 @@ namespace fh {
 
 namespace {
 
-enum OpKind : uint8_t { kMulPtRs, kRotate, kAdd, kAddPt, kDrop, kMultRs };
+enum OpKind : uint8_t { kMulPtRs, kRotate, kAdd, kAddPt, kDrop, kMultRs, kBoot };
 
 // One step of a request program. Registers: 0 = input x, 1 = t, 2 = r, 3 = y, 4 = aligned argument.
 struct Step {
@@ -56,9 +56,12 @@ std::vector<Step> polynomial_program() {
   return p;
 }
 
+// bootstrap: y = Bootstrap(x)
+std::vector<Step> bootstrap_program() { return {Step{kBoot, 3, 0, 0, 0, false}}; }
+
 struct Request {
   int id = 0;
-  bool inference = true;
+  int kind = 1;  // 0 polynomial, 1 inference, 2 bootstrap
   double arrival = 0;
   size_t pc = 0;
   int level[5] = {0, 0, 0, 0, 0};
@@ -90,18 +93,23 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
   // per-operation mode scratch (one multiplication at a time)
   DBuf tmp(c.pool, reg_words);
 
-  const std::vector<Step> progs[2] = {polynomial_program(), inference_program()};
+  const std::vector<Step> progs[3] = {polynomial_program(), inference_program(), bootstrap_program()};
+  if (cfg.bootstrap_share < 0 || cfg.bootstrap_share + cfg.inference_share > 1.0) throw Error(-1, "request shares must sum to at most 1");
+  if (cfg.bootstrap_share > 0 && !cfg.bootstrap) throw Error(-1, "bootstrap requests need a bootstrap function");
+  // a bootstrapped register can hold up to the whole chain
+  const size_t boot_words = cfg.bootstrap_share > 0 ? 2ull * c.L1 * N : 0;
 
   // arrivals
   std::mt19937_64 rng(cfg.seed);
   std::uniform_real_distribution<double> U(0.0, 1.0);
   std::vector<double> arrival(cfg.requests);
-  std::vector<bool> kind(cfg.requests);
+  std::vector<int> kind(cfg.requests);
   double t = 0;
   for (int i = 0; i < cfg.requests; ++i) {
     t += -std::log(1.0 - U(rng)) / cfg.rate_per_s;
     arrival[i] = t;
-    kind[i] = U(rng) < cfg.inference_share;
+    const double u = U(rng);
+    kind[i] = u < cfg.inference_share ? 1 : (u >= 1.0 - cfg.bootstrap_share ? 2 : 0);
   }
   const size_t per_request = 5 * reg_words * 8;
   const int max_active = std::max(1, (int)std::min<double>(1e6, cfg.hbm_budget_gb * 1e9 / (double)per_request));
@@ -123,9 +131,9 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
     while (next < cfg.requests && arrival[next] <= tn && (int)active.size() < max_active) {
       std::unique_ptr<Request> r(new Request);
       r->id = next;
-      r->inference = kind[next];
+      r->kind = kind[next];
       r->arrival = arrival[next];
-      for (int k = 0; k < 5; ++k) r->reg[k] = DBuf(c.pool, reg_words);
+      for (int k = 0; k < 5; ++k) r->reg[k] = DBuf(c.pool, k == 3 && r->kind == 2 ? std::max(reg_words, boot_words) : reg_words);
       std::vector<int> ids(L + 1);
       for (int i = 0; i <= L; ++i) ids[i] = i;
       // the request's input ciphertext (synthetic residues at the start level)
@@ -146,9 +154,10 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
     // issued directly (the batch's table upload would cost more than it saves).
     const bool use_batch = cfg.mode == 0 && active.size() >= 3;
     if (use_batch) batch.reset();
+    std::vector<Request*> boots;
     for (auto& rp : active) {
       Request* r = rp.get();
-      const Step& s = progs[r->inference ? 1 : 0][r->pc];
+      const Step& s = progs[r->kind][r->pc];
       const int la = r->level[s.a];
       uint64_t* a = r->reg[s.a].p;
       uint64_t* d = r->reg[s.dst].p;
@@ -200,6 +209,10 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
           }
           break;
         }
+        case kBoot:
+          boots.push_back(r);  // after the tick's other operations
+          ld = -1;
+          break;
         case kMultRs:
           if (r->level[s.b] != la) throw Error(-6, "serving program multiplies operands at different levels");
           ld = la - 1;
@@ -210,19 +223,25 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
           }
           break;
       }
-      r->level[s.dst] = ld;
+      if (s.kind != kBoot) r->level[s.dst] = ld;
     }
     ops_total += (double)active.size();
     if (use_batch) {
       batch.finalize();
       batch.run(st, false);
     }
+    for (Request* r : boots) {
+      int lo = -1;
+      if (cfg.bootstrap(cfg.bootstrap_user, r->reg[0].p, r->level[0], r->reg[3].p, &lo) != 0 || lo < 0)
+        throw Error(-6, "the bootstrap function failed");
+      r->level[3] = lo;
+    }
     FH_HIP(hipStreamSynchronize(st));
     const double td = now();
     ++res.ticks;
     for (auto it = active.begin(); it != active.end();) {
       Request* r = it->get();
-      if (++r->pc == progs[r->inference ? 1 : 0].size()) {
+      if (++r->pc == progs[r->kind].size()) {
         lat.push_back(td - r->arrival);
         if (cfg.checksum) {
           std::vector<uint64_t> h(2ull * (r->level[3] + 1) * N);
@@ -230,7 +249,7 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
           for (uint64_t w : h) res.checksum += w;
         }
         ++res.completed;
-        (r->inference ? res.inference : res.polynomial)++;
+        (r->kind == 1 ? res.inference : r->kind == 2 ? res.bootstraps : res.polynomial)++;
         it = active.erase(it);
       } else {
         ++it;

@@ -7,7 +7,9 @@
 //     rotate-and-sum over 8 doublings (rotations by 1, 2, ..., 128 and additions), then Horner's
 //     rule for a degree-7 polynomial (ciphertext products with relinearization and Rescale,
 //     plaintext coefficient additions, level alignment of the argument);
-//   polynomial ("depth-6 polynomial"): Horner's rule for a degree-6 polynomial of the input.
+//   polynomial ("depth-6 polynomial"): Horner's rule for a degree-6 polynomial of the input;
+//   bootstrap: one bootstrap of the input through a caller-supplied function (the bootstrap is
+//     host-orchestrated above this layer), run directly in the tick that reaches it.
 // Admission is bounded by an HBM budget (each request's register footprint). Every scheduler
 // tick gathers the next operation of every admitted request into one batch, whatever its kind
 // and level, and executes it as one launch sequence (or, in the per-operation baseline mode, one
@@ -32,12 +34,17 @@ struct ServeConfig {
   double inference_share = 0.74;
   uint64_t seed = 1;
   bool checksum = false;       // sum every request's result words (to compare the two modes)
+  // bootstrap requests: a share of the arrivals, each bootstrapping its input (2 x (start_level+1)
+  // limbs) into `out` through `bootstrap`, which returns 0 on success and the output level
+  double bootstrap_share = 0.0;
+  int (*bootstrap)(void* user, const uint64_t* in, int level, uint64_t* out, int* out_level) = nullptr;
+  void* bootstrap_user = nullptr;
 };
 
 struct ServeResult {
   double p50_ms = 0, p99_ms = 0, mean_ms = 0, max_ms = 0;
   double requests_per_s = 0, wall_s = 0, mean_batch_ops = 0;
-  int ticks = 0, completed = 0, inference = 0, polynomial = 0, max_active = 0;
+  int ticks = 0, completed = 0, inference = 0, polynomial = 0, bootstraps = 0, max_active = 0;
   uint64_t checksum = 0;       // sum of all result words mod 2^64 (when requested)
 };
 

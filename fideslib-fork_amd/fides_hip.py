@@ -109,14 +109,15 @@ class AutotuneReport(C.Structure):
 
 class ServeConfig(C.Structure):
     _fields_ = [("rate_per_s", C.c_double), ("requests", C.c_int), ("start_level", C.c_int), ("hbm_budget_gb", C.c_double),
-                ("mode", C.c_int), ("inference_share", C.c_double), ("seed", C.c_uint64), ("checksum", C.c_int)]
+                ("mode", C.c_int), ("inference_share", C.c_double), ("seed", C.c_uint64), ("checksum", C.c_int),
+                ("bootstrap_share", C.c_double), ("bootstrap", C.c_void_p), ("bootstrap_user", C.c_void_p)]
 
 
 class ServeResult(C.Structure):
     _fields_ = [("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
                 ("requests_per_s", C.c_double), ("wall_s", C.c_double), ("mean_batch_ops", C.c_double), ("ticks", C.c_int),
                 ("completed", C.c_int), ("inference", C.c_int), ("polynomial", C.c_int), ("max_active", C.c_int),
-                ("checksum", C.c_uint64)]
+                ("checksum", C.c_uint64), ("bootstraps", C.c_int)]
 
 
 def _check(rc):
@@ -322,9 +323,11 @@ class Context:
 
     # ---- serving loop ----
     def serve(self, rot_keys, rot_galois, relin_key, rate_per_s, requests, start_level=None, mode=0, hbm_budget_gb=64.0,
-              inference_share=0.74, seed=1, checksum=False):
+              inference_share=0.74, seed=1, checksum=False, bootstrap_share=0.0):
+        """bootstrap_share > 0 needs a bootstrap function, which only C/C++ callers pass (the C++
+        facade's EvalBootstrap, tests/cpp/serve_check.cpp); from here it is rejected."""
         cfg = ServeConfig(rate_per_s, requests, self.L1 - 1 if start_level is None else start_level, hbm_budget_gb, mode,
-                          inference_share, seed, int(checksum))
+                          inference_share, seed, int(checksum), float(bootstrap_share), None, None)
         kp = (C.c_void_p * 8)(*[k.h for k in rot_keys])
         gl = (C.c_uint32 * 8)(*[int(g) for g in rot_galois])
         r = ServeResult()

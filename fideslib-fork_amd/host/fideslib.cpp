@@ -428,6 +428,22 @@ Plaintext CryptoContextImpl::Decrypt(const PrivateKey& sk, const Ciphertext& ct)
   return pt;
 }
 
+// ---- raw device residues ----
+Ciphertext CryptoContextImpl::LoadCiphertext(const uint64_t* dev, uint32_t towers, uint32_t ncomp, uint32_t noiseScaleDeg, double scale,
+                                             uint32_t slots) {
+  if (towers < 1 || towers > L1_ || ncomp < 2 || ncomp > 3) throw std::runtime_error("fideslib: LoadCiphertext: shape out of range");
+  auto ct = newCt(towers, ncomp);
+  check(fh_copy_d2d(ctx_, ct->data->ptr, dev, (size_t)ncomp * towers * N_ * 8));
+  ct->noiseScaleDeg = noiseScaleDeg;
+  ct->scalingFactor = scale;
+  ct->slots = slots ? slots : N_ / 2;
+  return ct;
+}
+
+void CryptoContextImpl::StoreCiphertext(const Ciphertext& ct, uint64_t* dev) const {
+  check(fh_copy_d2d(ctx_, dev, ct->data->ptr, (size_t)ct->ncomp * ct->towers * N_ * 8));
+}
+
 // ---- level alignment (FIXEDMANUAL: drop towers, no rescale) ----
 Ciphertext CryptoContextImpl::adjusted(const Ciphertext& ct, uint32_t towers) const {
   if (ct->towers == towers) return ct;

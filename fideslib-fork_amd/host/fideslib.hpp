@@ -230,6 +230,14 @@ class CryptoContextImpl {
   uint32_t GetBootstrapDepth() const;
   BootstrapConstants GetBootstrapConstants() const;
 
+  // ---- raw device residues (Ciphertext::load / store, src/CKKS/Ciphertext.cpp:415-465): ncomp
+  // components of `towers` limbs in EVALUATION order, copied device to device
+  Ciphertext LoadCiphertext(const uint64_t* dev, uint32_t towers, uint32_t ncomp, uint32_t noiseScaleDeg, double scale, uint32_t slots);
+  void StoreCiphertext(const Ciphertext& ct, uint64_t* dev) const;
+  // key handles for C ABI entry points that take them (the serving loop, batches)
+  const fh_key* RotationKeyHandle(int32_t index) const { return rotKey(galoisFor(index)); }
+  const fh_key* RelinKeyHandle() const { return relin_; }
+
   // ---- scalar CRT encodings (src/CKKS/Context.cu:669-863), exposed for tests
   std::vector<uint64_t> ElemForEvalMult(uint32_t towers, double operand) const;
   std::vector<uint64_t> ElemForEvalAddOrSub(uint32_t towers, double operand, uint32_t noiseDeg) const;

@@ -1,0 +1,125 @@
+// Serving sessions with bootstrap requests: the continuous-batching loop (fh_serve_run) over a
+// fideslib facade context whose EvalBootstrap serves the bootstrap requests through the loop's
+// callback. Runs the same seeded session batched (mode 0) and one call per operation (mode 1) and
+// prints both results as one JSON line; tests/test_gpu_serve.py compares the checksums.
+//
+// usage: serve_check <logN> <slots> <cts_budget> <stc_budget> <depth> <scale_bits> <rate_per_s>
+//                    <requests> <bootstrap_share> [seed] [inference_share]
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../fideslib-fork_amd/host/fideslib.hpp"
+
+using namespace fideslib;
+
+struct BootUser {
+  CryptoContextImpl* cc;
+  uint32_t slots;
+  int calls;
+};
+
+// the loop's bootstrap callback: the request register as a facade ciphertext, bootstrapped
+static int boot_cb(void* user, const uint64_t* in, int level, uint64_t* out, int* out_level) {
+  auto* u = static_cast<BootUser*>(user);
+  try {
+    auto ct = u->cc->LoadCiphertext(in, (uint32_t)level + 1, 2, 1, u->cc->ScalingFactor(), u->slots);
+    auto bt = u->cc->EvalBootstrap(ct);
+    u->cc->StoreCiphertext(bt, out);
+    *out_level = (int)bt->towers - 1;
+    ++u->calls;
+    return 0;
+  } catch (const std::exception& ex) {
+    std::fprintf(stderr, "bootstrap callback: %s\n", ex.what());
+    return 1;
+  }
+}
+
+static std::string result_json(const fh_serve_result& r) {
+  char b[512];
+  std::snprintf(b, sizeof b,
+                "{\"p50_ms\": %.4f, \"p99_ms\": %.4f, \"mean_ms\": %.4f, \"max_ms\": %.4f, \"requests_per_s\": %.3f, \"wall_s\": %.4f, "
+                "\"mean_batch_ops\": %.3f, \"ticks\": %d, \"completed\": %d, \"inference\": %d, \"polynomial\": %d, "
+                "\"bootstraps\": %d, \"max_active\": %d, \"checksum\": \"%llu\"}",
+                r.p50_ms, r.p99_ms, r.mean_ms, r.max_ms, r.requests_per_s, r.wall_s, r.mean_batch_ops, r.ticks, r.completed, r.inference,
+                r.polynomial, r.bootstraps, r.max_active, (unsigned long long)r.checksum);
+  return b;
+}
+
+int main(int argc, char** argv) {
+  if (argc < 10) {
+    std::fprintf(stderr, "usage: serve_check <logN> <slots> <b1> <b2> <depth> <scale_bits> <rate> <requests> <boot_share> [seed] [inf_share]\n");
+    return 2;
+  }
+  const int logN = std::atoi(argv[1]);
+  const uint32_t slots = (uint32_t)std::atoi(argv[2]);
+  const uint32_t b1 = (uint32_t)std::atoi(argv[3]), b2 = (uint32_t)std::atoi(argv[4]);
+  const uint32_t depth = (uint32_t)std::atoi(argv[5]), sbits = (uint32_t)std::atoi(argv[6]);
+  const double rate = std::atof(argv[7]);
+  const int requests = std::atoi(argv[8]);
+  const double boot_share = std::atof(argv[9]);
+  const uint64_t seed = argc > 10 ? std::strtoull(argv[10], nullptr, 10) : 5;
+  const double inf_share = argc > 11 ? std::atof(argv[11]) : 0.5;
+  try {
+    CCParams p;
+    p.SetRingDim(1u << logN);
+    p.SetMultiplicativeDepth(depth);
+    p.SetScalingModSize(sbits);
+    p.SetFirstModSize(60);
+    p.SetNumLargeDigits(3);
+    p.SetSecretKeyDist(SecretKeyDist::SPARSE_TERNARY);
+    p.SetHammingWeight(16);
+    p.SetBatchSize(slots);
+    p.SetSeed(seed);
+    auto cc = GenCryptoContext(p);
+    auto kp = cc->KeyGen();
+    cc->EvalMultKeyGen(kp.secretKey);
+    std::vector<int32_t> steps;
+    for (int k = 0; k < 8; ++k) steps.push_back(1 << k);
+    cc->EvalRotateKeyGen(kp.secretKey, steps);
+    cc->EvalBootstrapSetup({b1, b2}, {0, 0}, slots);
+    cc->EvalBootstrapKeyGen(kp.secretKey, slots);
+
+    std::vector<const fh_key*> rk;
+    std::vector<uint32_t> rg;
+    for (int32_t s : steps) {
+      rk.push_back(cc->RotationKeyHandle(s));
+      uint32_t g = 0;
+      if (fh_galois_index(cc->Native(), s, &g) != FH_OK) throw std::runtime_error(fh_last_error());
+      rg.push_back(g);
+    }
+    BootUser user{cc.get(), slots, 0};
+    fh_serve_result res[3];
+    // a warm-up session (per operation, discarded) fills the context's buffer pool and the
+    // bootstrap's first-use state, so that neither measured session pays for them
+    for (int run = 0; run < 3; ++run) {
+      const int mode = run == 0 ? 1 : run - 1;
+      fh_serve_config cfg{};
+      cfg.rate_per_s = rate;
+      cfg.requests = requests;
+      cfg.start_level = (int)cc->GetNumQ() - 1;
+      cfg.hbm_budget_gb = 8.0;
+      cfg.mode = mode;
+      cfg.inference_share = inf_share;
+      cfg.seed = seed;
+      cfg.checksum = 1;
+      cfg.bootstrap_share = boot_share;
+      cfg.bootstrap = boot_cb;
+      cfg.bootstrap_user = &user;
+      if (fh_serve_run(cc->Native(), &cfg, rk.data(), rg.data(), cc->RelinKeyHandle(), &res[run]) != FH_OK) {
+        std::fprintf(stderr, "fh_serve_run: %s\n", fh_last_error());
+        return 1;
+      }
+    }
+    std::printf("{\"logN\": %d, \"slots\": %u, \"L1\": %u, \"boot_depth\": %u, \"rate_per_s\": %.3f, \"requests\": %d, "
+                "\"bootstrap_share\": %.3f, \"callbacks\": %d, \"batched\": %s, \"per_op\": %s}\n",
+                logN, slots, cc->GetNumQ(), cc->GetBootstrapDepth(), rate, requests, boot_share, user.calls, result_json(res[1]).c_str(),
+                result_json(res[2]).c_str());
+  } catch (const std::exception& ex) {
+    std::fprintf(stderr, "error: %s\n", ex.what());
+    return 1;
+  }
+  return 0;
+}

@@ -47,5 +47,43 @@ def test_serving_rejects_bad_config(sctx, fh):
         sctx.serve(rk, rg, relin, 100.0, 4, start_level=5)      # too few levels for a request
     with pytest.raises(fh.FHError):
         sctx.serve(rk, rg, relin, 100.0, 4, mode=3)
+    with pytest.raises(fh.FHError):
+        sctx.serve(rk, rg, relin, 100.0, 4, bootstrap_share=0.2)  # no bootstrap function
+    with pytest.raises(fh.FHError):
+        sctx.serve(rk, rg, relin, 100.0, 4, inference_share=0.9, bootstrap_share=0.2)
     for k in rk + [relin]:
         k.free()
+
+
+# ---- bootstrap requests through the C++ facade (tests/cpp/serve_check.cpp) ----
+import json
+import os
+import subprocess
+
+from conftest import PKG
+
+SCHECK = os.path.join(PKG, "_build", "serve_check")
+
+
+@pytest.mark.parametrize("share", [0.3, 1.0])
+def test_serving_with_bootstrap_requests(share):
+    """A session whose arrivals include bootstrap requests, served by the facade's EvalBootstrap
+    (N=2^12, 8 slots, level budget {1,1}, 16 towers): batched and per-operation execution give the
+    same checksum, every bootstrap request went through the callback once per mode."""
+    if not os.path.exists(SCHECK):
+        pytest.fail("serve_check is not built (run __graft_entry__.build())")
+    args = ["12", "8", "1", "1", "15", "50", "2000", "16", str(share), "5", str(0.5 * (1 - share))]
+    print(f"[serve_check] {SCHECK} {' '.join(args)}", flush=True)
+    r = subprocess.run([SCHECK] + args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    b, p = res["batched"], res["per_op"]
+    for x in (b, p):
+        assert x["completed"] == 16
+        assert x["inference"] + x["polynomial"] + x["bootstraps"] == 16
+        assert 0 < x["p50_ms"] <= x["p99_ms"] <= x["max_ms"]
+    assert b["bootstraps"] == p["bootstraps"] > 0
+    if share == 1.0:
+        assert b["bootstraps"] == 16
+    assert res["callbacks"] == 3 * b["bootstraps"]  # warm-up, batched and per-operation sessions
+    assert b["checksum"] == p["checksum"], "fused batches differ from per-operation execution"
