@@ -455,39 +455,39 @@ def lt_workload(fh, ctx, a):
 def bootstrap_run():
     """EvalBootstrap through the C++ facade (tests/cpp/bootstrap_check.cpp) at N=2^16 with all
     32768 slots, level budget {3, 3}, 59-bit scaling under a 60-bit q_0 (correction factor 9),
-    sparse ternary secret h=16, 25 towers: median latency of three bootstraps and the decryption
-    error of the bootstrapped slots."""
+    sparse ternary secret h=32 (one double angle), 25 towers: median latency of three bootstraps
+    and the decryption error of the bootstrapped slots."""
     import subprocess
     import tempfile
     exe = os.path.join(ROOT, "fideslib-fork_amd", "_build", "bootstrap_check")
     if not os.path.exists(exe):
         return {"error": "bootstrap_check is not built"}
     with tempfile.TemporaryDirectory() as d:
-        r = subprocess.run([exe, d, "16", "32768", "7", "3", "3", "24", "59"], capture_output=True, text=True, timeout=600)
+        r = subprocess.run([exe, d, "16", "32768", "7", "3", "3", "24", "59", "32"], capture_output=True, text=True, timeout=600)
     if r.returncode != 0:
         return {"error": (r.stdout + r.stderr)[-500:]}
     res = json.loads(r.stdout.strip().splitlines()[-1])
-    return {"workload": "N=2^16, 32768 slots, level budget {3,3}, 59-bit scaling, sparse ternary h=16", "ms": res["boot_ms"],
+    return {"workload": "N=2^16, 32768 slots, level budget {3,3}, 59-bit scaling, sparse ternary h=32", "ms": res["boot_ms"],
             "bits": res["boot_bits"], "towers_in": res["towers_in"], "towers_out": res["towers_out"], "depth": res["depth"],
             "deterministic": res["deterministic"]}
 
 
 def serving_bootstrap_run(a):
     """A serving session whose arrivals include bootstrap requests (tests/cpp/serve_check.cpp):
-    the bootstrap context above (N=2^16, 25 towers, full-slot bootstrapping), 25% bootstrap, 37.5%
-    inference, 37.5% polynomial requests at 40 requests/s, batched and per-operation; the same
-    checksum from both."""
+    the bootstrap context above (N=2^16, 25 towers, full-slot bootstrapping, h=32), the mix of 70%
+    inference, 25% polynomial and 5% bootstrap requests at 50 requests/s, batched and per-operation;
+    the same checksum from both."""
     import subprocess
     exe = os.path.join(ROOT, "fideslib-fork_amd", "_build", "serve_check")
     if not os.path.exists(exe):
         return {"error": "serve_check is not built"}
-    args = ["16", "32768", "3", "3", "24", "59", "40", "32", "0.25", str(a.seed), "0.375"]
+    args = ["16", "32768", "3", "3", "24", "59", "50", "96", "0.05", str(a.seed), "0.70", "32"]
     r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=600)
     if r.returncode != 0:
         return {"error": (r.stdout + r.stderr)[-500:]}
     res = json.loads(r.stdout.strip().splitlines()[-1])
     keep = ("p50_ms", "p99_ms", "mean_ms", "requests_per_s", "mean_batch_ops", "bootstraps", "inference", "polynomial")
-    return {"rate_per_s": 40.0, "requests": 32, "bootstrap_share": 0.25, "towers": res["L1"],
+    return {"rate_per_s": 50.0, "requests": 96, "mix": "70% inference, 25% polynomial, 5% bootstrap", "towers": res["L1"],
             "batched": {k: res["batched"][k] for k in keep}, "per_op": {k: res["per_op"][k] for k in keep},
             "same_checksum": res["batched"]["checksum"] == res["per_op"]["checksum"]}
 

@@ -13,7 +13,8 @@
 //      (the monomial X^(3N/2), Bootstrap.cu:577-584)
 //   -> EvalMod on both parts: a Chebyshev series of cos((2 pi K y - pi/2) / 2^r) on [-1, 1]
 //      followed by r double-angle steps c <- 2c^2 - 1, which leaves sin(2 pi K y)
-//      (ApproxModEval.cu:341-873); for a sparse secret r = 0 and degree 127 (each double angle
+//      (ApproxModEval.cu:341-873); for a sparse secret r = 0 with degree 127 up to weight 21 and
+//      255 up to weight 42, double angles only beyond (each double angle
 //      quadruples the series' noise, which the final transform then scales by q_0 / Delta: at
 //      N=2^12 and 64 slots r = 3 with degree 63 measured 11.7 bits, r = 0 with degree 127 14.3
 //      bits, one level cheaper)
@@ -341,8 +342,18 @@ void CryptoContextImpl::EvalBootstrapSetup(const std::vector<uint32_t>& levelBud
   // EvalMod range and shape: |I| <= (h + 1) / 2 for a sparse secret of weight h
   const bool sparse = params_.dist_ == SecretKeyDist::SPARSE_TERNARY;
   st->K = sparse ? std::ceil(params_.hw_ / 2.0) + 4.0 : 64.0;
-  st->r = sparse ? 0 : 5;
-  st->degree = sparse ? 127 : 63;
+  // sparse: a Chebyshev series of sin(a y) of degree d converges to double precision while
+  // a <= 0.63 d (its coefficients fall like the Bessel values J_k(a): ~1e-14 at k = 127 for a = 80),
+  // and each double angle multiplies the series' error by ~4, so: no double angle and degree 127
+  // up to a = 80 (h <= 21), degree 255 up to a = 160 (h <= 42; one more product level, the same
+  // depth as degree 127 plus a double angle), then the fewest double angles that reach a <= 160
+  st->r = 5;
+  st->degree = 63;
+  if (sparse) {
+    st->r = 0;
+    while (2.0 * M_PI * st->K / std::ldexp(1.0, st->r) > 160.0) ++st->r;
+    st->degree = 2.0 * M_PI * st->K / std::ldexp(1.0, st->r) > 80.0 ? 255 : 127;
+  }
   CosArg arg{st->K, st->r};
   st->cheb = EvalChebyshevCoefficients(shifted_cos, &arg, -1.0, 1.0, st->degree);
   const uint64_t M = 2ull * N_;

@@ -171,6 +171,8 @@ class Ctx:
     def __init__(self, logN, primes, L1, K, dnum):
         self.logN, self.N, self.L1, self.K, self.dnum = logN, 1 << logN, L1, K, dnum
         self.primes = np.ascontiguousarray(primes, dtype=np.uint64)
+        if self.primes.ndim != 1 or self.primes.size != L1 + K or L1 < 1 or K < 1 or dnum < 1:
+            raise ValueError(f"oracle context: {self.primes.size} primes for L1={L1}, K={K}, dnum={dnum}")
         self.h = lib().orc_create(logN, L1, K, dnum, self.primes)
         assert self.h
         self.alpha = lib().orc_alpha(self.h)

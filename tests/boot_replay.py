@@ -208,8 +208,12 @@ def rotation_steps(k, N):
 def constants(N, n, hw, scale, q0, sparse=True, budget=(1, 1), correction=0):
     gap = (N // 2) // n
     K = math.ceil(hw / 2.0) + 4.0 if sparse else 64.0
-    r = 0 if sparse else 5
-    degree = 127 if sparse else 63
+    r, degree = 5, 63
+    if sparse:
+        r = 0
+        while 2.0 * math.pi * K / math.ldexp(1.0, r) > 160.0:
+            r += 1
+        degree = 255 if 2.0 * math.pi * K / math.ldexp(1.0, r) > 80.0 else 127
     cheb = cheb_coefficients(lambda y: math.cos((2.0 * math.pi * K * y - 0.5 * math.pi) / math.ldexp(1.0, r)), -1.0, 1.0, degree)
     M = 2 * N
     fc = scale / (2.0 * gap * q0 * K)

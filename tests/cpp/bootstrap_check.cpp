@@ -7,7 +7,7 @@
 // The inputs, outputs and bootstrap constants are written to <out_dir> so that the test can replay
 // the same sequence with the oracle and compare residues.
 //
-// usage: bootstrap_check <out_dir> <logN> <slots> [seed] [cts_budget stc_budget depth scale_bits]
+// usage: bootstrap_check <out_dir> <logN> <slots> [seed] [cts_budget stc_budget depth scale_bits hw]
 // (prints one JSON line)
 #include <algorithm>
 #include <chrono>
@@ -80,6 +80,7 @@ int main(int argc, char** argv) {
   const uint32_t b1 = argc > 5 ? (uint32_t)std::atoi(argv[5]) : 1, b2 = argc > 6 ? (uint32_t)std::atoi(argv[6]) : 1;
   const uint32_t depth = argc > 7 ? (uint32_t)std::atoi(argv[7]) : 15;
   const uint32_t sbits = argc > 8 ? (uint32_t)std::atoi(argv[8]) : 50;
+  const uint32_t hw = argc > 9 ? (uint32_t)std::atoi(argv[9]) : 16;
   try {
     CCParams p;
     p.SetRingDim(1u << logN);
@@ -88,7 +89,7 @@ int main(int argc, char** argv) {
     p.SetFirstModSize(60);
     p.SetNumLargeDigits(3);
     p.SetSecretKeyDist(SecretKeyDist::SPARSE_TERNARY);
-    p.SetHammingWeight(16);
+    p.SetHammingWeight(hw);
     p.SetBatchSize(slots);
     p.SetSeed(seed);
     auto cc = GenCryptoContext(p);
@@ -176,9 +177,9 @@ int main(int argc, char** argv) {
     }
     {
       FILE* f = std::fopen((g_dir + "/meta.json").c_str(), "w");
-      std::fprintf(f, "{\"logN\": %d, \"slots\": %u, \"seed\": %llu, \"L1\": %u, \"K\": %u, \"dnum\": 3, \"hw\": 16, "
+      std::fprintf(f, "{\"logN\": %d, \"slots\": %u, \"seed\": %llu, \"L1\": %u, \"K\": %u, \"dnum\": 3, \"hw\": %u, "
                       "\"scale\": %.17g, \"Krange\": %.17g, \"r\": %d, \"pre\": %.17g, \"correction\": %u, %s, \"moduli\": [",
-                   logN, slots, (unsigned long long)seed, L1, (unsigned)(cc->GetModuli().size() - L1), cc->ScalingFactor(), bc.K, bc.r, bc.pre,
+                   logN, slots, (unsigned long long)seed, L1, (unsigned)(cc->GetModuli().size() - L1), hw, cc->ScalingFactor(), bc.K, bc.r, bc.pre,
                    bc.correction, groups.c_str());
       for (size_t i = 0; i < cc->GetModuli().size(); ++i)
         std::fprintf(f, "%s\"%llu\"", i ? ", " : "", (unsigned long long)cc->GetModuli()[i]);

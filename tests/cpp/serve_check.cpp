@@ -4,7 +4,7 @@
 // prints both results as one JSON line; tests/test_gpu_serve.py compares the checksums.
 //
 // usage: serve_check <logN> <slots> <cts_budget> <stc_budget> <depth> <scale_bits> <rate_per_s>
-//                    <requests> <bootstrap_share> [seed] [inference_share]
+//                    <requests> <bootstrap_share> [seed] [inference_share] [hamming_weight]
 #include <cstdio>
 #include <cstdlib>
 #include <stdexcept>
@@ -50,7 +50,7 @@ static std::string result_json(const fh_serve_result& r) {
 
 int main(int argc, char** argv) {
   if (argc < 10) {
-    std::fprintf(stderr, "usage: serve_check <logN> <slots> <b1> <b2> <depth> <scale_bits> <rate> <requests> <boot_share> [seed] [inf_share]\n");
+    std::fprintf(stderr, "usage: serve_check <logN> <slots> <b1> <b2> <depth> <scale_bits> <rate> <requests> <boot_share> [seed] [inf_share] [hw]\n");
     return 2;
   }
   const int logN = std::atoi(argv[1]);
@@ -62,6 +62,7 @@ int main(int argc, char** argv) {
   const double boot_share = std::atof(argv[9]);
   const uint64_t seed = argc > 10 ? std::strtoull(argv[10], nullptr, 10) : 5;
   const double inf_share = argc > 11 ? std::atof(argv[11]) : 0.5;
+  const uint32_t hw = argc > 12 ? (uint32_t)std::atoi(argv[12]) : 16;
   try {
     CCParams p;
     p.SetRingDim(1u << logN);
@@ -70,7 +71,7 @@ int main(int argc, char** argv) {
     p.SetFirstModSize(60);
     p.SetNumLargeDigits(3);
     p.SetSecretKeyDist(SecretKeyDist::SPARSE_TERNARY);
-    p.SetHammingWeight(16);
+    p.SetHammingWeight(hw);
     p.SetBatchSize(slots);
     p.SetSeed(seed);
     auto cc = GenCryptoContext(p);

@@ -81,12 +81,12 @@ def test_factored_transforms_compose_to_the_special_fft():
             assert np.max(np.abs(A - want)) < 1e-12, (budget, inverse)
 
 
-@pytest.mark.parametrize("n,budget,L1,sbits,corr,bits", [(8, (1, 1), 16, 50, 0, 19), (64, (1, 1), 16, 50, 0, 17), (64, (2, 3), 18, 50, 0, 14),
-                                                        (512, (3, 3), 18, 50, 0, 12), (512, (3, 3), 19, 59, 9, 20)])
-def test_bootstrap_decrypts(n, budget, L1, sbits, corr, bits):
+@pytest.mark.parametrize("n,budget,L1,sbits,corr,bits,hw", [(8, (1, 1), 16, 50, 0, 19, 16), (64, (1, 1), 16, 50, 0, 17, 16),
+                                                           (64, (2, 3), 18, 50, 0, 14, 16), (512, (3, 3), 18, 50, 0, 12, 16),
+                                                           (512, (3, 3), 19, 59, 9, 20, 16), (512, (3, 3), 20, 59, 9, 18, 32)])
+def test_bootstrap_decrypts(n, budget, L1, sbits, corr, bits, hw):
     o = _ctx(L1=L1, K=9 if sbits > 50 else 6, sbits=sbits)
     N, scale = o.N, 2.0 ** sbits
-    hw = 16
     s = o.gen_secret(7, hw)
     k = B.constants(N, n, hw, scale, float(o.q(0)), budget=budget, correction=corr)
     relin, rot = _keys(o, s, B.rotation_steps(k, N), N, 30)
@@ -100,5 +100,6 @@ def test_bootstrap_decrypts(n, budget, L1, sbits, corr, bits):
     got = o.decode(o.decrypt(out[0], l, s), l, n, scale)
     err = np.max(np.abs(got - z))
     print(f"bootstrap n={n} budget={budget}: towers {out[1]}, max error {err:.3g} ({-math.log2(err):.1f} bits)")
-    assert out[1] == o.L1 - (1 + len(k["cts"]) + 8 + 0 + len(k["stc"]))
+    assert k["r"] == 0 and len(k["cheb"]) == (128 if hw <= 21 else 256)
+    assert out[1] == o.L1 - (1 + len(k["cts"]) + (8 if hw <= 21 else 9) + len(k["stc"]))
     assert err < 2.0 ** -bits

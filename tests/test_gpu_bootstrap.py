@@ -34,17 +34,20 @@ BCHECK = os.path.join(PKG, "_build", "bootstrap_check")
 
 
 # (logN, slots, CoeffsToSlots budget, SlotsToCoeffs budget, multiplicative depth, scaling bits,
-# bits required); the CPU model of the same sequences measures 15.8, 15.0, 10.0 and 21.4 bits
-RUNS = [(12, 8, 1, 1, 15, 50, 14), (12, 64, 2, 3, 17, 50, 13), (12, 2048, 3, 3, 18, 50, 9), (12, 2048, 3, 3, 19, 59, 19)]
+# bits required, secret weight); the CPU model of the same sequences measures 15.8, 15.0, 10.0 and
+# 21.4 bits; weight 32 takes a degree-255 series
+RUNS = [(12, 8, 1, 1, 15, 50, 14, 16), (12, 64, 2, 3, 17, 50, 13, 16), (12, 2048, 3, 3, 18, 50, 9, 16), (12, 2048, 3, 3, 19, 59, 19, 16),
+        (12, 2048, 3, 3, 20, 59, 15, 32)]
 
 
-@pytest.fixture(scope="module", params=RUNS, ids=["N4096_slots8_b11", "N4096_slots64_b23", "N4096_full_b33", "N4096_full_b33_scale59"])
+@pytest.fixture(scope="module", params=RUNS, ids=["N4096_slots8_b11", "N4096_slots64_b23", "N4096_full_b33", "N4096_full_b33_scale59",
+                                                  "N4096_full_b33_scale59_h32"])
 def boot_run(request, tmp_path_factory):
-    logN, slots, b1, b2, depth, sbits, bits = request.param
+    logN, slots, b1, b2, depth, sbits, bits, hw = request.param
     if not os.path.exists(BCHECK):
         pytest.fail("bootstrap_check is not built (run __graft_entry__.build())")
     out = tmp_path_factory.mktemp(f"boot{slots}")
-    args = [str(out), str(logN), str(slots), "7", str(b1), str(b2), str(depth), str(sbits)]
+    args = [str(out), str(logN), str(slots), "7", str(b1), str(b2), str(depth), str(sbits), str(hw)]
     print(f"[bootstrap_check] {BCHECK} {' '.join(args)}", flush=True)
     r = subprocess.run([BCHECK] + args, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
@@ -158,6 +161,7 @@ def test_bootstrap_constants_closed_form(boot_run):
         assert np.allclose(product(_groups(out, meta, "stc")), q0 / (2 * math.pi * scale) * UP, rtol=0, atol=1e-9)
     assert 1.0 <= fc / meta["pre"] < 2.0 and math.log2(meta["pre"]).is_integer()
     cheb = np.fromfile(out / "cheb.f64", dtype=np.float64)
+    assert Kr == math.ceil(meta["hw"] / 2) + 4 and r == 0 and len(cheb) == (128 if meta["hw"] <= 21 else 256)
     ys = np.linspace(-1, 1, 101)
     approx = np.polynomial.chebyshev.chebval(ys, np.concatenate([[cheb[0] / 2], cheb[1:]]))
     assert np.max(np.abs(approx - np.cos((2 * math.pi * Kr * ys - math.pi / 2) / 2 ** r))) < 1e-12

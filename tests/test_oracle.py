@@ -533,3 +533,9 @@ def test_mod_raise_closed_form(sc):
         v = v - q0 if v > q0 // 2 else v
         assert (X - v) % q0 == 0
         assert abs((X - v) // q0) <= 16
+
+
+def test_context_rejects_inconsistent_prime_count():
+    primes = O.gen_primes(10, 4, 60, 50, 2)
+    with pytest.raises(ValueError):
+        O.Ctx(10, primes, 4, 3, 2)   # 6 primes cannot be L1=4 plus K=3
