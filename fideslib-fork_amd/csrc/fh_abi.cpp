@@ -239,6 +239,21 @@ int fh_eltwise_scalar(fh_context* ctx, int op, const uint64_t* a, const uint64_t
   });
 }
 
+int fh_linear_wsum(fh_context* ctx, int n, const uint64_t* const* cts, const int32_t* cts_towers, const uint64_t* weights, int towers,
+                   int ncomp, const uint64_t* add_c0, uint64_t* out) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(cts, "cts"); need(cts_towers, "cts_towers"); need(weights, "weights"); need(out, "out");
+    if (n < 1 || n > fh::kMaxWSum) throw fh::Error(FH_E_INVALID, "n must be 1..16");
+    std::vector<uint32_t> limbs(n);
+    for (int k = 0; k < n; ++k) {
+      if (cts_towers[k] < towers) throw fh::Error(FH_E_INVALID, "an input has fewer towers than the output");
+      limbs[k] = (uint32_t)cts_towers[k];
+    }
+    fh::ew_wsum(c, n, cts, limbs.data(), weights, towers, ncomp, add_c0, out, c.stream);
+  });
+}
+
 int fh_tensor(fh_context* ctx, const uint64_t* ct1, const uint64_t* ct2, int level, uint64_t* out3) {
   return guard([&] {
     auto& c = C(ctx);

@@ -95,6 +95,32 @@ __global__ __launch_bounds__(kEwThreads) void k_tensor(const uint64_t* c, const 
   }
 }
 
+// Weighted sum of n ciphertexts over their first `towers` limbs: out_c = sum_k w_k * in_k,c (+ add
+// on component 0). in_k holds ncomp components of limbs_k >= towers limbs; w is [n][towers]
+// residues, add (optional) [towers] residues. One block row per (component, limb).
+struct WSumIn {
+  const uint64_t* p[kMaxWSum];
+  uint32_t limbs[kMaxWSum];
+};
+
+__global__ __launch_bounds__(kEwThreads) void k_wsum(WSumIn in, int n, const uint64_t* __restrict__ w, const uint64_t* __restrict__ add,
+                                                     uint32_t towers, uint64_t* out, const PrimeConst* pr, uint32_t N) {
+  const uint32_t comp = blockIdx.y / towers, limb = blockIdx.y % towers;
+  const PrimeConst P = pr[limb];
+  const uint64_t a0 = add && comp == 0 ? add[limb] : 0;
+  uint64_t* po = out + ((size_t)comp * towers + limb) * N;
+  for (uint32_t i = 2 * (blockIdx.x * blockDim.x + threadIdx.x); i < N; i += 2 * gridDim.x * blockDim.x) {
+    uint64_t r0 = a0, r1 = a0;
+    for (int k = 0; k < n; ++k) {
+      const uint64_t wk = w[(size_t)k * towers + limb];
+      const V2 x = ld2(in.p[k] + ((size_t)comp * in.limbs[k] + limb) * N + i);
+      r0 = add_mod(r0, mul_mod(x.x, wk, P), P.q);
+      r1 = add_mod(r1, mul_mod(x.y, wk, P), P.q);
+    }
+    st2(po + i, r0, r1);
+  }
+}
+
 // splitmix64 finalizer: a counter-based stream, one 64-bit word per (seed, limb job, index)
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   z += 0x9E3779B97F4A7C15ull;
@@ -134,6 +160,31 @@ void ew_binary(const Context& c, int op, const LimbSet& a, const LimbSet& b, con
     case 1: k_binary<1><<<g, kEwThreads, 0, st>>>(a, b, o, c.d_prime, c.N); break;
     default: k_binary<2><<<g, kEwThreads, 0, st>>>(a, b, o, c.d_prime, c.N); break;
   }
+  FH_HIP(hipGetLastError());
+}
+
+void ew_wsum(Context& c, int n, const uint64_t* const* in, const uint32_t* in_limbs, const uint64_t* weights, int towers, int ncomp,
+             const uint64_t* add, uint64_t* out, hipStream_t st) {
+  if (n < 1 || n > kMaxWSum) throw Error(-1, "weighted sum of 1.." + std::to_string(kMaxWSum) + " ciphertexts");
+  if (towers < 1 || towers > c.L1 || ncomp < 1 || ncomp > 3) throw Error(-1, "weighted sum: shape out of range");
+  WSumIn wi{};
+  for (int k = 0; k < n; ++k) {
+    if (!in[k]) throw Error(-1, "weighted sum: null input");
+    if (in_limbs[k] < (uint32_t)towers || in_limbs[k] > (uint32_t)c.L1) throw Error(-1, "weighted sum: an input's limb count is out of range");
+    wi.p[k] = in[k];
+    wi.limbs[k] = in_limbs[k];
+  }
+  // weights (reduced) and the added constant, uploaded in stream order into one pool buffer
+  std::vector<uint64_t> h((size_t)(n + 1) * towers, 0);
+  for (int k = 0; k < n; ++k)
+    for (int i = 0; i < towers; ++i) h[(size_t)k * towers + i] = weights[(size_t)k * towers + i] % c.q[i];
+  if (add)
+    for (int i = 0; i < towers; ++i) h[(size_t)n * towers + i] = add[i] % c.q[i];
+  DBuf d(c.pool, h.size());
+  FH_HIP(hipMemcpyAsync(d.p, h.data(), h.size() * 8, hipMemcpyHostToDevice, st));
+  dim3 g = ew_grid(c.N, (uint32_t)(towers * ncomp));
+  ProfScope ps_(c, "eltwise", st);
+  k_wsum<<<g, kEwThreads, 0, st>>>(wi, n, d.p, add ? d.p + (size_t)n * towers : nullptr, (uint32_t)towers, out, c.d_prime, c.N);
   FH_HIP(hipGetLastError());
 }
 

@@ -231,18 +231,25 @@ struct ChebEval {
   Ciphertext eval(const std::vector<double>& c) {
     const uint32_t d = (uint32_t)c.size() - 1;
     if (d < kBaby) {
-      // level of the deepest T_k used, then one scalar product per term and one rescale
-      uint32_t towers = ~0u;
-      for (uint32_t k = 1; k <= d; ++k) towers = std::min(towers, t(k)->towers);
-      Ciphertext acc;
+      // at the level of the deepest T_k used: one fused weighted sum of the nonzero terms, one
+      // rescale (the residues of the scalar products summed term by term)
+      std::vector<Ciphertext> ts;
+      std::vector<double> ws;
       for (uint32_t k = 1; k <= d; ++k) {
-        if (c[k] == 0.0) continue;
         Ciphertext tk = t(k);
-        Ciphertext term = cc.EvalMult(cc.LevelReduce(tk, tk->towers - towers), c[k]);
-        acc = acc ? cc.EvalAdd(acc, term) : term;
+        if (c[k] == 0.0) continue;
+        ts.push_back(tk);
+        ws.push_back(c[k]);
       }
-      if (!acc) acc = cc.EvalMult(cc.LevelReduce(t(1), t(1)->towers - towers), 0.0);
-      return cc.EvalAdd(cc.Rescale(acc), c[0]);
+      uint32_t towers = ~0u, used = ~0u;
+      for (uint32_t k = 1; k <= d; ++k) towers = std::min(towers, t(k)->towers);
+      for (const auto& x : ts) used = std::min(used, x->towers);
+      if (used > towers) {
+        // the sum's level is that of the deepest T_k, used or not
+        ts.push_back(cc.LevelReduce(t(1), t(1)->towers - towers));
+        ws.push_back(0.0);
+      }
+      return cc.EvalAdd(cc.Rescale(cc.EvalLinearWSum(ts, ws)), c[0]);
     }
     uint32_t M = 1;
     while (2 * M <= d) M *= 2;

@@ -690,6 +690,37 @@ Ciphertext CryptoContextImpl::EvalMult(const Ciphertext& a0, const Ciphertext& b
 
 Ciphertext CryptoContextImpl::EvalSquare(const Ciphertext& ct) { return EvalMult(ct, ct); }
 
+Ciphertext CryptoContextImpl::EvalLinearWSum(const std::vector<Ciphertext>& cts, const std::vector<double>& weights) {
+  if (cts.empty() || cts.size() != weights.size()) throw std::runtime_error("fideslib: EvalLinearWSum needs one weight per ciphertext");
+  uint32_t t = ~0u;
+  for (const auto& c : cts) {
+    if (c->ncomp != cts[0]->ncomp || c->noiseScaleDeg != cts[0]->noiseScaleDeg)
+      throw std::runtime_error("fideslib: EvalLinearWSum: inputs differ in size or scale degree");
+    t = std::min(t, c->towers);
+  }
+  const uint32_t nc = cts[0]->ncomp;
+  Ciphertext acc;
+  for (size_t k0 = 0; k0 < cts.size(); k0 += 16) {
+    const size_t n = std::min<size_t>(16, cts.size() - k0);
+    std::vector<const uint64_t*> ptrs(n);
+    std::vector<int32_t> towers(n);
+    std::vector<uint64_t> w(n * t);
+    for (size_t k = 0; k < n; ++k) {
+      ptrs[k] = cts[k0 + k]->data->ptr;
+      towers[k] = (int32_t)cts[k0 + k]->towers;
+      const auto f = ElemForEvalMult(t, weights[k0 + k]);
+      std::copy(f.begin(), f.end(), w.begin() + k * t);
+    }
+    auto out = newCt(t, nc);
+    check(fh_linear_wsum(ctx_, (int)n, ptrs.data(), towers.data(), w.data(), (int)t, (int)nc, nullptr, out->data->ptr));
+    out->noiseScaleDeg = cts[0]->noiseScaleDeg + 1;
+    out->scalingFactor = cts[0]->scalingFactor * scale_;
+    out->slots = cts[0]->slots;
+    acc = acc ? EvalAdd(acc, out) : out;
+  }
+  return acc;
+}
+
 Ciphertext CryptoContextImpl::EvalAddMany(const std::vector<Ciphertext>& cts) {
   if (cts.empty()) throw std::runtime_error("fideslib: EvalAddMany of an empty list");
   Ciphertext acc = cts[0];

@@ -193,6 +193,10 @@ class CryptoContextImpl {
   Ciphertext Relinearize(const Ciphertext& ct);
   Ciphertext EvalSquare(const Ciphertext& ct);
   Ciphertext EvalAddMany(const std::vector<Ciphertext>& cts);
+  // sum_k weights[k] * cts[k] at the lowest input level, scale degree + 1, not rescaled (OpenFHE's
+  // EvalLinearWSum; the reference's evalLinearWSumMutable, src/CKKS/Ciphertext.cuh:573): one fused
+  // kernel per 16 inputs, the residues those of EvalMult(ct, w) summed
+  Ciphertext EvalLinearWSum(const std::vector<Ciphertext>& cts, const std::vector<double>& weights);
   Ciphertext Rescale(const Ciphertext& ct);
   Ciphertext LevelReduce(const Ciphertext& ct, uint32_t levels);
 

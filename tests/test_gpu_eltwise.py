@@ -32,6 +32,45 @@ def test_scalar(op, gctx1, octx1):
     da.free(); dc.free()
 
 
+@pytest.mark.parametrize("n,towers,with_add", [(1, 5, False), (7, 12, True), (16, 10, True), (3, 25, False)])
+def test_linear_wsum(n, towers, with_add, gctx2, octx2):
+    """fh_linear_wsum against the oracle's scalar products and additions: inputs at mixed tower
+    counts (only the first `towers` limbs of each component count), 2 components."""
+    g, o = gctx2, octx2
+    rng = np.random.default_rng(n)
+    cts, devs, tin = [], [], []
+    for k in range(n):
+        t = towers + int(rng.integers(0, 3)) if towers + 2 <= o.L1 else towers
+        pids = list(range(t))
+        c = np.concatenate([o.random_poly(100 + 2 * k, pids), o.random_poly(101 + 2 * k, pids)])
+        cts.append(c); tin.append(t); devs.append(g.to_device(c))
+    w = np.array([[int(rng.integers(0, int(o.primes[i]))) for i in range(towers)] for _ in range(n)], dtype=np.uint64)
+    add = np.array([int(rng.integers(0, int(o.primes[i]))) for i in range(towers)], dtype=np.uint64) if with_add else None
+    out = g.array((2 * towers, o.N))
+    g.linear_wsum(devs, tin, w, towers, 2, out, add)
+    exp = np.zeros((2 * towers, o.N), dtype=np.uint64)
+    for k in range(n):
+        for comp in range(2):
+            part = cts[k][comp * tin[k]: comp * tin[k] + towers]
+            exp[comp * towers:(comp + 1) * towers] = o.add(exp[comp * towers:(comp + 1) * towers], o.mul_scalar(part, w[k]))
+    if with_add:
+        exp[:towers] = o.add_scalar(exp[:towers], add)
+    assert np.array_equal(out.download(), exp)
+    for x in devs + [out]:
+        x.free()
+
+
+def test_linear_wsum_rejects_bad_shapes(gctx2, fh):
+    g = gctx2
+    a = g.array((2 * 4, g.N))
+    out = g.array((2 * 5, g.N))
+    with pytest.raises(fh.FHError):
+        g.linear_wsum([a], [4], np.ones((1, 5), dtype=np.uint64), 5, 2, out)    # input below the output's towers
+    with pytest.raises(fh.FHError):
+        g.linear_wsum([a] * 17, [4] * 17, np.ones((17, 4), dtype=np.uint64), 4, 2, out)
+    a.free(); out.free()
+
+
 @pytest.mark.parametrize("square", [False, True])
 def test_tensor(square, gctx2, octx2):
     g, o = gctx2, octx2
