@@ -218,11 +218,9 @@ struct ChebEval {
     if (it != T.end()) return it->second;
     Ciphertext r;
     if (k % 2 == 0) {
-      r = cc.Rescale(cc.EvalSquare(t(k / 2)));
-      r = cc.EvalSub(cc.EvalAdd(r, r), 1.0);
+      r = cc.IntegerCombination({cc.Rescale(cc.EvalSquare(t(k / 2)))}, {2}, -1.0);
     } else {
-      r = cc.Rescale(cc.EvalMult(t(k / 2), t(k / 2 + 1)));
-      r = cc.EvalSub(cc.EvalAdd(r, r), t(1));
+      r = cc.IntegerCombination({cc.Rescale(cc.EvalMult(t(k / 2), t(k / 2 + 1))), t(1)}, {2, -1}, 0.0);
     }
     T[k] = r;
     return r;
@@ -249,7 +247,7 @@ struct ChebEval {
         ts.push_back(cc.LevelReduce(t(1), t(1)->towers - towers));
         ws.push_back(0.0);
       }
-      return cc.EvalAdd(cc.Rescale(cc.EvalLinearWSum(ts, ws)), c[0]);
+      return cc.IntegerCombination({cc.Rescale(cc.EvalLinearWSum(ts, ws))}, {1}, c[0]);
     }
     uint32_t M = 1;
     while (2 * M <= d) M *= 2;
@@ -260,7 +258,7 @@ struct ChebEval {
       r[M - j] -= c[M + j];
     }
     Ciphertext Q = eval(q), R = eval(r);
-    return cc.EvalAdd(cc.Rescale(cc.EvalMult(Q, t(M))), R);
+    return cc.IntegerCombination({cc.Rescale(cc.EvalMult(Q, t(M))), R}, {1, 1}, 0.0);
   }
 };
 }  // namespace
@@ -279,6 +277,43 @@ Ciphertext CryptoContextImpl::EvalChebyshevSeries(const Ciphertext& ct, const st
   ChebEval e{*this, {}};
   e.T[1] = y;
   return e.eval(c);
+}
+
+// ---- integer combinations (the chains of EvalAdd / EvalSub / MultByInteger they replace give the
+// same residues: every step is exact modulo each q_i) ----
+Ciphertext CryptoContextImpl::IntegerCombination(const std::vector<Ciphertext>& cts, const std::vector<int64_t>& w, double constant) {
+  if (cts.empty() || cts.size() != w.size() || cts.size() > 16) throw std::runtime_error("fideslib: IntegerCombination: 1..16 terms");
+  uint32_t t = ~0u;
+  for (const auto& c : cts) {
+    if (c->ncomp != cts[0]->ncomp || c->noiseScaleDeg != cts[0]->noiseScaleDeg)
+      throw std::runtime_error("fideslib: IntegerCombination: inputs differ in size or scale degree");
+    t = std::min(t, c->towers);
+  }
+  const size_t n = cts.size();
+  std::vector<const uint64_t*> ptrs(n);
+  std::vector<int32_t> towers(n);
+  std::vector<uint64_t> res(n * t);
+  for (size_t k = 0; k < n; ++k) {
+    ptrs[k] = cts[k]->data->ptr;
+    towers[k] = (int32_t)cts[k]->towers;
+    for (uint32_t i = 0; i < t; ++i) {
+      const int64_t r = w[k] % (int64_t)moduli_[i];
+      res[k * t + i] = (uint64_t)(r < 0 ? r + (int64_t)moduli_[i] : r);
+    }
+  }
+  std::vector<uint64_t> add;
+  if (constant != 0.0) {
+    add = ElemForEvalAddOrSub(t, std::fabs(constant), cts[0]->noiseScaleDeg);
+    if (constant < 0)
+      for (uint32_t i = 0; i < t; ++i) add[i] = add[i] ? moduli_[i] - add[i] : 0;
+  }
+  auto out = newCt(t, cts[0]->ncomp);
+  check(fh_linear_wsum(ctx_, (int)n, ptrs.data(), towers.data(), res.data(), (int)t, (int)cts[0]->ncomp, add.empty() ? nullptr : add.data(),
+                       out->data->ptr));
+  out->noiseScaleDeg = cts[0]->noiseScaleDeg;
+  out->scalingFactor = cts[0]->scalingFactor;
+  out->slots = cts[0]->slots;
+  return out;
 }
 
 // ---- multiplication by an integer (no scale change: multIntScalar, ApproxModEval.cu:169-175) ----
@@ -443,10 +478,7 @@ Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0, uint32_t numI
   // 5. EvalMod: sin(2 pi K y) on both parts
   auto evalmod = [&](const Ciphertext& y) {
     Ciphertext c = EvalChebyshevSeries(y, st.cheb, -1.0, 1.0);
-    for (int i = 0; i < st.r; ++i) {
-      c = Rescale(EvalSquare(c));
-      c = EvalSub(EvalAdd(c, c), 1.0);
-    }
+    for (int i = 0; i < st.r; ++i) c = IntegerCombination({Rescale(EvalSquare(c))}, {2}, -1.0);
     return c;
   };
   Ciphertext fre = evalmod(re), fim = evalmod(im);

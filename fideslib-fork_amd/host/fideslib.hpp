@@ -197,6 +197,9 @@ class CryptoContextImpl {
   // EvalLinearWSum; the reference's evalLinearWSumMutable, src/CKKS/Ciphertext.cuh:573): one fused
   // kernel per 16 inputs, the residues those of EvalMult(ct, w) summed
   Ciphertext EvalLinearWSum(const std::vector<Ciphertext>& cts, const std::vector<double>& weights);
+  // sum_k w[k] cts[k] + constant at the lowest input level in one kernel: the residues of the
+  // additions, negations and integer products it replaces (equal scale degrees, no scale change)
+  Ciphertext IntegerCombination(const std::vector<Ciphertext>& cts, const std::vector<int64_t>& w, double constant);
   Ciphertext Rescale(const Ciphertext& ct);
   Ciphertext LevelReduce(const Ciphertext& ct, uint32_t levels);
 
