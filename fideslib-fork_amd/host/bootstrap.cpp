@@ -205,8 +205,24 @@ std::vector<double> CryptoContextImpl::EvalChebyshevCoefficients(double (*f)(dou
 namespace {
 // Evaluation of sum_k c_k T_k(y) (full constant term) by recursive splitting at powers of two:
 // p = q T_M + r with q_0 = c_M, q_j = 2 c_{M+j}, r_k = c_k - c_{2M-k} (T_{M+j} = 2 T_M T_j - T_{M-j});
-// pieces below degree kBaby are linear combinations of T_1 .. T_{kBaby-1}.
-constexpr uint32_t kBaby = 8;
+// a piece of degree below 8, or below 16 where that costs no extra level, is a linear combination
+// of T_1 .. T_d (one fused weighted sum); a constant q multiplies T_M as a scalar.
+uint32_t clog2(uint32_t d) {
+  uint32_t j = 0;
+  while ((1u << j) < d) ++j;
+  return j;
+}
+uint32_t cheb_depth(uint32_t d);
+// levels a piece of degree d >= 1 consumes below T_1's, split at the largest power of two M <= d
+uint32_t cheb_split_depth(uint32_t d) {
+  uint32_t M = 1;
+  while (2 * M <= d) M *= 2;
+  const uint32_t lm = clog2(M);
+  const uint32_t prod = (d == M ? lm : std::max(cheb_depth(d - M), lm)) + 1;
+  return std::max(prod, cheb_depth(M - 1));
+}
+bool cheb_leaf(uint32_t d) { return d < 8 || (d < 16 && clog2(d) + 1 <= cheb_split_depth(d)); }
+uint32_t cheb_depth(uint32_t d) { return cheb_leaf(d) ? clog2(d) + 1 : cheb_split_depth(d); }
 
 struct ChebEval {
   CryptoContextImpl& cc;
@@ -228,7 +244,7 @@ struct ChebEval {
 
   Ciphertext eval(const std::vector<double>& c) {
     const uint32_t d = (uint32_t)c.size() - 1;
-    if (d < kBaby) {
+    if (cheb_leaf(d)) {
       // at the level of the deepest T_k used: one fused weighted sum of the nonzero terms, one
       // rescale (the residues of the scalar products summed term by term)
       std::vector<Ciphertext> ts;
@@ -257,8 +273,9 @@ struct ChebEval {
       q[j] = 2.0 * c[M + j];
       r[M - j] -= c[M + j];
     }
-    Ciphertext Q = eval(q), R = eval(r);
-    return cc.IntegerCombination({cc.Rescale(cc.EvalMult(Q, t(M))), R}, {1, 1}, 0.0);
+    Ciphertext P = q.size() == 1 ? cc.Rescale(cc.EvalMult(t(M), q[0])) : cc.Rescale(cc.EvalMult(eval(q), t(M)));
+    Ciphertext R = eval(r);
+    return cc.IntegerCombination({P, R}, {1, 1}, 0.0);
   }
 };
 }  // namespace
@@ -436,11 +453,8 @@ BootstrapConstants CryptoContextImpl::GetBootstrapConstants() const {
 uint32_t CryptoContextImpl::GetBootstrapDepth() const {
   if (!boot_) throw std::runtime_error("fideslib: EvalBootstrapSetup first");
   // prescale 1 + one per CoeffsToSlots group + series + double angle r + one per SlotsToCoeffs
-  // group; the series of degree d >= 7 takes ceil(log2(d + 1)) + 1 levels (pieces below degree 8
-  // take 4)
-  uint32_t series = 1;
-  while ((1u << series) < boot_->degree + 1) ++series;
-  return 1 + (uint32_t)boot_->cts.size() + (series + 1) + (uint32_t)boot_->r + (uint32_t)boot_->stc.size();
+  // group (a series of degree 2^j - 1 >= 7 takes j + 1 levels)
+  return 1 + (uint32_t)boot_->cts.size() + cheb_depth(boot_->degree) + (uint32_t)boot_->r + (uint32_t)boot_->stc.size();
 }
 
 Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0, uint32_t numIterations, uint32_t precision) {

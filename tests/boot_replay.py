@@ -11,7 +11,33 @@ import numpy as np
 
 from oracle import oracle as O
 
-BABY = 8  # pieces below this degree are linear combinations of T_1 .. T_{BABY-1}
+
+
+def _clog2(d):
+    j = 0
+    while (1 << j) < d:
+        j += 1
+    return j
+
+
+def cheb_split_depth(d):
+    """Levels a piece of degree d >= 1 consumes below T_1's when split at the largest M <= d."""
+    M = 1
+    while 2 * M <= d:
+        M *= 2
+    lm = _clog2(M)
+    prod = (lm if d == M else max(cheb_depth(d - M), lm)) + 1
+    return max(prod, cheb_depth(M - 1))
+
+
+def cheb_leaf(d):
+    """A piece of degree below 8, or below 16 where that costs no extra level, is a linear
+    combination of T_1 .. T_d."""
+    return d < 8 or (d < 16 and _clog2(d) + 1 <= cheb_split_depth(d))
+
+
+def cheb_depth(d):
+    return _clog2(d) + 1 if cheb_leaf(d) else cheb_split_depth(d)
 
 
 # ---- constants (mirrors of the facade's double arithmetic) ----
@@ -354,7 +380,7 @@ def chebyshev_series(ops, y, coeffs, a=-1.0, b=1.0):
 
     def ev(c):
         d = len(c) - 1
-        if d < BABY:
+        if cheb_leaf(d):
             towers = min(t(k)[1] for k in range(1, d + 1))
             acc = None
             for k in range(1, d + 1):
@@ -374,8 +400,8 @@ def chebyshev_series(ops, y, coeffs, a=-1.0, b=1.0):
         for j in range(1, d - M + 1):
             q[j] = 2.0 * c[M + j]
             r[M - j] -= c[M + j]
-        Q, R = ev(q), ev(r)
-        return ops.add(ops.rescale(ops.mult(Q, t(M))), R)
+        P = ops.rescale(ops.mul_scalar(t(M), q[0])) if len(q) == 1 else ops.rescale(ops.mult(ev(q), t(M)))
+        return ops.add(P, ev(r))
 
     return ev(c)
 

@@ -42,7 +42,45 @@ def test_chebyshev_coefficients_closed_form():
     assert np.allclose(c, want, atol=1e-13)
 
 
-def test_chebyshev_series_decrypts():
+class _LevelOps:
+    """Tracks (towers, scale degree) only: the level structure of a series evaluation."""
+
+    def rescale(self, y):
+        return (None, y[1] - 1, y[2] - 1)
+
+    def mult(self, a, b):
+        return (None, min(a[1], b[1]), a[2] + b[2])
+
+    def add(self, a, b):
+        assert a[2] == b[2]
+        return (None, min(a[1], b[1]), a[2])
+
+    sub = add
+
+    def add_scalar(self, a, c):
+        return a
+
+    def mul_scalar(self, a, c):
+        return (None, a[1], a[2] + 1)
+
+    def reduce(self, a, t):
+        return (None, t, a[2])
+
+
+def test_chebyshev_depth_rule():
+    """The levels a series consumes equal the planned depth, never exceed ceil(log2(d + 1)) + 1,
+    and are j + 1 for degree 2^j - 1 (the bootstrap's 63, 127, 255)."""
+    for d in range(1, 300):
+        out = B.chebyshev_series(_LevelOps(), (None, 40, 1), [1.0] * (d + 1))
+        assert out[2] == 1
+        assert 40 - out[1] == B.cheb_depth(d) <= (d).bit_length() + 1, d
+    for j in range(3, 9):
+        assert B.cheb_depth(2 ** j - 1) == j + 1
+    assert B.cheb_leaf(15) and B.cheb_leaf(13) and not B.cheb_leaf(12) and not B.cheb_leaf(16)
+
+
+@pytest.mark.parametrize("degree,depth", [(27, 5), (16, 5), (15, 5)])
+def test_chebyshev_series_decrypts(degree, depth):
     o = _ctx()
     N, n, scale = o.N, 16, 2.0 ** 50
     s = o.gen_secret(11, 16)
@@ -53,11 +91,12 @@ def test_chebyshev_series_decrypts():
     l = o.L1 - 1
     ct = (o.encrypt(o.encode(x, scale, l), l, s, 5), l + 1, 1)
     f = lambda v: math.tanh(1.5 * v) + 0.25 * v * v
-    c = B.cheb_coefficients(f, -2.0, 2.0, 27)
+    c = B.cheb_coefficients(f, -2.0, 2.0, degree)
     out = B.chebyshev_series(ops, ct, c, -2.0, 2.0)
     got = o.decode(o.decrypt(out[0], out[1] - 1, s), out[1] - 1, n, scale).real
-    assert np.max(np.abs(got - [f(v) for v in x])) < 1e-6
-    assert out[1] == l + 1 - 1 - 5     # input scaling 1 + degree 27 (depth 5)
+    want = np.polynomial.chebyshev.chebval(np.asarray(x) / 2.0, np.concatenate([[c[0] / 2], c[1:]]))
+    assert np.max(np.abs(got - want)) < 1e-6
+    assert out[1] == l + 1 - 1 - depth     # input scaling 1 + the series
 
 
 def test_factored_transforms_compose_to_the_special_fft():
