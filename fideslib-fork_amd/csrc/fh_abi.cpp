@@ -240,7 +240,7 @@ int fh_eltwise_scalar(fh_context* ctx, int op, const uint64_t* a, const uint64_t
 }
 
 int fh_linear_wsum(fh_context* ctx, int n, const uint64_t* const* cts, const int32_t* cts_towers, const uint64_t* weights, int towers,
-                   int ncomp, const uint64_t* add_c0, uint64_t* out) {
+                   int ncomp, const uint64_t* add, uint32_t add_components, uint64_t* out) {
   return guard([&] {
     auto& c = C(ctx);
     need(cts, "cts"); need(cts_towers, "cts_towers"); need(weights, "weights"); need(out, "out");
@@ -250,7 +250,7 @@ int fh_linear_wsum(fh_context* ctx, int n, const uint64_t* const* cts, const int
       if (cts_towers[k] < towers) throw fh::Error(FH_E_INVALID, "an input has fewer towers than the output");
       limbs[k] = (uint32_t)cts_towers[k];
     }
-    fh::ew_wsum(c, n, cts, limbs.data(), weights, towers, ncomp, add_c0, out, c.stream);
+    fh::ew_wsum(c, n, cts, limbs.data(), weights, towers, ncomp, add, add_components, out, c.stream);
   });
 }
 
@@ -269,7 +269,7 @@ int fh_rescale(fh_context* ctx, const uint64_t* ct, int level, int ncomp, uint64
     need(ct, "ct"); need(out, "out");
     check_level(c, level);
     if (level < 1) throw fh::Error(FH_E_INVALID, "cannot rescale at level 0");
-    if (ncomp < 1 || ncomp > 3) throw fh::Error(FH_E_INVALID, "ncomp must be 1..3");
+    if (ncomp < 1 || ncomp > 8) throw fh::Error(FH_E_INVALID, "ncomp must be 1..8");
     fh::rescale(c, ct, level, ncomp, out, c.stream);
   });
 }
@@ -400,6 +400,16 @@ int fh_eval_mult(fh_context* ctx, const uint64_t* ct1, const uint64_t* ct2, int 
     need(ct1, "ct1"); need(ct2, "ct2"); need(out, "out");
     check_level(c, level);
     fh::eval_mult(c, ct1, ct2, level, key_of(c, relin_key), out, c.stream);
+  });
+}
+
+int fh_eval_mult_many(fh_context* ctx, int n, const uint64_t* ct1, const uint64_t* ct2, int level, const fh_key* relin_key, uint64_t* out) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(ct1, "ct1"); need(ct2, "ct2"); need(out, "out");
+    check_level(c, level);
+    if (n < 1 || n > 4096) throw fh::Error(FH_E_INVALID, "n must be 1..4096");
+    fh::eval_mult_many(c, n, ct1, ct2, level, key_of(c, relin_key), out, c.stream);
   });
 }
 

@@ -68,10 +68,15 @@ __global__ __launch_bounds__(kEwThreads) void k_unary(LimbSet a, LimbSet o, Scal
 }
 
 // (c0, c1) x (d0, d1) -> (c0 d0, c0 d1 + c1 d0, c1 d1); square when SQ.
+// blockIdx.z: item (inputs in_stride words apart, outputs out_stride apart)
 template <bool SQ>
-__global__ __launch_bounds__(kEwThreads) void k_tensor(const uint64_t* c, const uint64_t* d, uint64_t* o, uint32_t nl, const PrimeConst* pr, uint32_t N) {
+__global__ __launch_bounds__(kEwThreads) void k_tensor(const uint64_t* c, const uint64_t* d, uint64_t* o, uint32_t nl, const PrimeConst* pr, uint32_t N,
+                                                       uint64_t in_stride, uint64_t out_stride) {
   const uint32_t limb = blockIdx.y;
   const PrimeConst P = pr[limb];
+  c += blockIdx.z * in_stride;
+  d += blockIdx.z * in_stride;
+  o += blockIdx.z * out_stride;
   const size_t comp = (size_t)nl * N, off = (size_t)limb * N;
   const uint64_t *c0 = c + off, *c1 = c + comp + off, *d0 = d + off, *d1 = d + comp + off;
   uint64_t *o0 = o + off, *o1 = o + comp + off, *o2 = o + 2 * comp + off;
@@ -96,18 +101,19 @@ __global__ __launch_bounds__(kEwThreads) void k_tensor(const uint64_t* c, const 
 }
 
 // Weighted sum of n ciphertexts over their first `towers` limbs: out_c = sum_k w_k * in_k,c (+ add
-// on component 0). in_k holds ncomp components of limbs_k >= towers limbs; w is [n][towers]
-// residues, add (optional) [towers] residues. One block row per (component, limb).
+// on the components whose bit is set in add_mask). in_k holds ncomp components of limbs_k >= towers
+// limbs; w is [n][towers] residues, add (optional) [towers] residues. One block row per
+// (component, limb).
 struct WSumIn {
   const uint64_t* p[kMaxWSum];
   uint32_t limbs[kMaxWSum];
 };
 
 __global__ __launch_bounds__(kEwThreads) void k_wsum(WSumIn in, int n, const uint64_t* __restrict__ w, const uint64_t* __restrict__ add,
-                                                     uint32_t towers, uint64_t* out, const PrimeConst* pr, uint32_t N) {
+                                                     uint32_t add_mask, uint32_t towers, uint64_t* out, const PrimeConst* pr, uint32_t N) {
   const uint32_t comp = blockIdx.y / towers, limb = blockIdx.y % towers;
   const PrimeConst P = pr[limb];
-  const uint64_t a0 = add && comp == 0 ? add[limb] : 0;
+  const uint64_t a0 = add && ((add_mask >> comp) & 1u) ? add[limb] : 0;
   uint64_t* po = out + ((size_t)comp * towers + limb) * N;
   for (uint32_t i = 2 * (blockIdx.x * blockDim.x + threadIdx.x); i < N; i += 2 * gridDim.x * blockDim.x) {
     uint64_t r0 = a0, r1 = a0;
@@ -164,9 +170,9 @@ void ew_binary(const Context& c, int op, const LimbSet& a, const LimbSet& b, con
 }
 
 void ew_wsum(Context& c, int n, const uint64_t* const* in, const uint32_t* in_limbs, const uint64_t* weights, int towers, int ncomp,
-             const uint64_t* add, uint64_t* out, hipStream_t st) {
+             const uint64_t* add, uint32_t add_mask, uint64_t* out, hipStream_t st) {
   if (n < 1 || n > kMaxWSum) throw Error(-1, "weighted sum of 1.." + std::to_string(kMaxWSum) + " ciphertexts");
-  if (towers < 1 || towers > c.L1 || ncomp < 1 || ncomp > 3) throw Error(-1, "weighted sum: shape out of range");
+  if (towers < 1 || towers > c.L1 || ncomp < 1 || ncomp > 8) throw Error(-1, "weighted sum: shape out of range");
   WSumIn wi{};
   for (int k = 0; k < n; ++k) {
     if (!in[k]) throw Error(-1, "weighted sum: null input");
@@ -184,7 +190,7 @@ void ew_wsum(Context& c, int n, const uint64_t* const* in, const uint32_t* in_li
   FH_HIP(hipMemcpyAsync(d.p, h.data(), h.size() * 8, hipMemcpyHostToDevice, st));
   dim3 g = ew_grid(c.N, (uint32_t)(towers * ncomp));
   ProfScope ps_(c, "eltwise", st);
-  k_wsum<<<g, kEwThreads, 0, st>>>(wi, n, d.p, add ? d.p + (size_t)n * towers : nullptr, (uint32_t)towers, out, c.d_prime, c.N);
+  k_wsum<<<g, kEwThreads, 0, st>>>(wi, n, d.p, add ? d.p + (size_t)n * towers : nullptr, add_mask, (uint32_t)towers, out, c.d_prime, c.N);
   FH_HIP(hipGetLastError());
 }
 
@@ -231,12 +237,15 @@ void fill_uniform(const Context& c, const LimbSet& s, uint64_t seed, hipStream_t
   FH_HIP(hipGetLastError());
 }
 
-void ew_tensor(const Context& c, const uint64_t* ct1, const uint64_t* ct2, uint64_t* out3, int nlimbs, hipStream_t st) {
-  if (nlimbs <= 0) return;
+void ew_tensor(const Context& c, const uint64_t* ct1, const uint64_t* ct2, uint64_t* out3, int nlimbs, hipStream_t st, int n) {
+  if (nlimbs <= 0 || n <= 0) return;
+  if (n > 65535) throw Error(-1, "too many tensor products in one launch");
   dim3 g = ew_grid(c.N, nlimbs);
+  g.z = (uint32_t)n;
+  const uint64_t P = (uint64_t)nlimbs * c.N;
   ProfScope ps_(c, "tensor", st);
-  if (ct1 == ct2) k_tensor<true><<<g, kEwThreads, 0, st>>>(ct1, ct1, out3, nlimbs, c.d_prime, c.N);
-  else k_tensor<false><<<g, kEwThreads, 0, st>>>(ct1, ct2, out3, nlimbs, c.d_prime, c.N);
+  if (ct1 == ct2) k_tensor<true><<<g, kEwThreads, 0, st>>>(ct1, ct1, out3, nlimbs, c.d_prime, c.N, 2 * P, 3 * P);
+  else k_tensor<false><<<g, kEwThreads, 0, st>>>(ct1, ct2, out3, nlimbs, c.d_prime, c.N, 2 * P, 3 * P);
   FH_HIP(hipGetLastError());
 }
 

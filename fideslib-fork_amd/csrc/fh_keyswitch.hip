@@ -667,6 +667,26 @@ void eval_mult(Context& c, const uint64_t* ct1, const uint64_t* ct2, int l, cons
   relinearize(c, t.p, l, key, out2, st);
 }
 
+void eval_mult_many(Context& c, int n, const uint64_t* ct1, const uint64_t* ct2, int l, const uint64_t* key, uint64_t* out2, hipStream_t st) {
+  if (n < 1) return;
+  if (n == 1) return eval_mult(c, ct1, ct2, l, key, out2, st);
+  const uint32_t lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);
+  const size_t P = (size_t)lq * c.N, E = (size_t)ext * c.N;
+  DBuf t(c.pool, (size_t)n * 3 * P), dig(c.pool, (size_t)n * beta * E), eo(c.pool, (size_t)n * 2 * E), ks(c.pool, (size_t)n * 2 * P);
+  ew_tensor(c, ct1, ct2, t.p, (int)lq, st, n);
+  // every d2 through one ModUp, one grouped key inner product (the key read once) and one ModDown
+  modup_batch(c, t.p + 2 * P, l, n, 3 * P, dig.p, st, false);
+  std::vector<const uint64_t*> keys(n, key);
+  std::vector<uint32_t> gal(n, 1);
+  ksk_items(c, n, 1, dig.p, (uint64_t)beta * E, nullptr, 0, false, false, l, keys.data(), gal.data(), eo.p, 2 * E, st, t.p + 2 * P, 3 * P);
+  moddown(c, eo.p, l, ks.p, 2 * n, E, P, st);
+  std::vector<int> ids(lq);
+  for (uint32_t i = 0; i < lq; ++i) ids[i] = i;
+  for (int b = 0; b < n; ++b)
+    ew_binary(c, 0, make_set(t.p + (size_t)b * 3 * P, lq, ids.data(), 2, P), make_set(ks.p + (size_t)b * 2 * P, lq, ids.data(), 2, P),
+              make_set(out2 + (size_t)b * 2 * P, lq, ids.data(), 2, P), st);
+}
+
 // hoisted formulation: ext key switch of c1 (+ P c0), automorph, result over Q_l ∪ P
 void rotate_ext(Context& c, const uint64_t* ct, int l, const uint64_t* key, uint32_t galois, bool add_c0, uint64_t* out_ext, hipStream_t st) {
   const uint32_t lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);

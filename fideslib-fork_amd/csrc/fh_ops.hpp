@@ -11,13 +11,15 @@ void ew_binary(const Context& c, int op, const LimbSet& a, const LimbSet& b, con
 // op: 0 add scalar, 1 mul scalar, 2 negate (scalars per limb, may be null for negate)
 void ew_unary(const Context& c, int op, const LimbSet& a, const LimbSet& o, const uint64_t* scalars, hipStream_t st);
 // out (ncomp x towers limbs, packed) = sum_k weights[k] * in[k] over the first towers limbs of each
-// component (in[k]: ncomp components of in_limbs[k] limbs), plus add[] on component 0 if given;
-// weights: [n][towers] residues (host), add: [towers] residues (host) or null. n <= kMaxWSum.
+// component (in[k]: ncomp components of in_limbs[k] limbs), plus add[] on the components whose bit
+// is set in add_mask; weights: [n][towers] residues (host), add: [towers] residues (host) or null.
+// n <= kMaxWSum, ncomp <= 8.
 constexpr int kMaxWSum = 16;
 void ew_wsum(Context& c, int n, const uint64_t* const* in, const uint32_t* in_limbs, const uint64_t* weights, int towers, int ncomp,
-             const uint64_t* add, uint64_t* out, hipStream_t st);
+             const uint64_t* add, uint32_t add_mask, uint64_t* out, hipStream_t st);
 // ct1, ct2: 2 x nlimbs over Q_l; out3: 3 x nlimbs. ct1 == ct2 selects the square form.
-void ew_tensor(const Context& c, const uint64_t* ct1, const uint64_t* ct2, uint64_t* out3, int nlimbs, hipStream_t st);
+// n tensor products of ciphertexts back to back (2 x nlimbs limbs each) into n x 3 x nlimbs limbs
+void ew_tensor(const Context& c, const uint64_t* ct1, const uint64_t* ct2, uint64_t* out3, int nlimbs, hipStream_t st, int n = 1);
 // in place x -> x 2^64 mod q (resident key-switching keys are kept in Montgomery form)
 void to_montgomery(const Context& c, const LimbSet& s, hipStream_t st);
 // uniform residues mod each limb's prime (synthetic inputs)

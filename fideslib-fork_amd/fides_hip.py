@@ -55,7 +55,8 @@ def _declare(L):
         "fh_eltwise_binary": (I, [P, I, VP, VP, VP, i32p, I]),
         "fh_eltwise_scalar": (I, [P, I, VP, u64p, VP, i32p, I]),
         "fh_tensor": (I, [P, VP, VP, I, VP]),
-        "fh_linear_wsum": (I, [P, I, VP, i32p, u64p, I, I, u64p, VP]),
+        "fh_linear_wsum": (I, [P, I, VP, i32p, u64p, I, I, u64p, C.c_uint32, VP]),
+        "fh_eval_mult_many": (I, [P, I, VP, VP, I, VP, VP]),
         "fh_rescale": (I, [P, VP, I, I, VP]),
         "fh_mod_raise": (I, [P, VP, I, I, I, VP]),
         "fh_key_create": (I, [P, u64p, PP]),
@@ -231,15 +232,20 @@ class Context:
         s = None if scalars is None else np.ascontiguousarray(scalars, dtype=np.uint64)
         _check(load().fh_eltwise_scalar(self.h, op, a.ptr, None if s is None else _u64p(s), out.ptr, _i32(prime_ids), nl))
 
-    def linear_wsum(self, cts, towers_in, weights, towers, ncomp, out, add=None):
+    def linear_wsum(self, cts, towers_in, weights, towers, ncomp, out, add=None, add_components=1):
         """out = sum_k weights[k] * cts[k] over the first `towers` limbs of each of ncomp components
-        (cts[k] has towers_in[k] limbs per component), plus add on component 0."""
+        (cts[k] has towers_in[k] limbs per component), plus add on the components in the bit mask
+        add_components."""
         n = len(cts)
         ptrs = (C.c_void_p * n)(*[c.ptr for c in cts])
         w = np.ascontiguousarray(weights, dtype=np.uint64).reshape(n, towers)
         a = None if add is None else np.ascontiguousarray(add, dtype=np.uint64)
         _check(load().fh_linear_wsum(self.h, n, ptrs, _i32(list(towers_in)), _u64p(w), towers, ncomp, None if a is None else _u64p(a),
-                                     out.ptr))
+                                     add_components, out.ptr))
+
+    def eval_mult_many(self, n, ct1, ct2, level, relin_key, out):
+        """n products of ciphertexts back to back with one relinearization key."""
+        _check(load().fh_eval_mult_many(self.h, n, ct1.ptr, ct2.ptr, level, relin_key.h, out.ptr))
 
     def tensor(self, ct1, ct2, level, out3):
         _check(load().fh_tensor(self.h, ct1.ptr, ct2.ptr, level, out3.ptr))

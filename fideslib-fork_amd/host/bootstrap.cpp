@@ -202,6 +202,48 @@ std::vector<double> CryptoContextImpl::EvalChebyshevCoefficients(double (*f)(dou
   return c;
 }
 
+// Two ciphertexts at one level as one of 4 components (back to back): the EvalMod of the real and
+// imaginary parts runs as one sequence whose products go through fh_eval_mult_many (one launch
+// sequence and one read of the relinearization key for both) and whose element-wise steps,
+// rescales and weighted sums cover both parts in each launch.
+struct BootstrapOps {
+  static Ciphertext mult(CryptoContextImpl& cc, const Ciphertext& a0, const Ciphertext& b0) {
+    if (a0->ncomp == 2 && b0->ncomp == 2) return cc.EvalMult(a0, b0);
+    if (a0->ncomp != 4 || b0->ncomp != 4) throw std::runtime_error("fideslib: paired product of unpaired ciphertexts");
+    if (!cc.relin_) throw std::runtime_error("fideslib: EvalMultKeyGen was not called");
+    const uint32_t t = std::min(a0->towers, b0->towers);
+    auto a = cc.adjusted(a0, t);
+    auto b = a0 == b0 ? a : cc.adjusted(b0, t);
+    auto out = cc.newCt(t, 4);
+    cc.check(fh_eval_mult_many(cc.ctx_, 2, a->data->ptr, b->data->ptr, (int)t - 1, cc.relin_, out->data->ptr));
+    out->noiseScaleDeg = a->noiseScaleDeg + b->noiseScaleDeg;
+    out->scalingFactor = a->scalingFactor * b->scalingFactor;
+    out->slots = a->slots;
+    return out;
+  }
+  static Ciphertext pack(CryptoContextImpl& cc, const Ciphertext& a, const Ciphertext& b) {
+    if (a->ncomp != 2 || b->ncomp != 2 || a->towers != b->towers || a->noiseScaleDeg != b->noiseScaleDeg)
+      throw std::runtime_error("fideslib: pairing needs two relinearized ciphertexts at one level");
+    const size_t half = (size_t)2 * a->towers * cc.N_;
+    auto p = cc.newCt(a->towers, 4);
+    cc.check(fh_copy_d2d(cc.ctx_, p->data->ptr, a->data->ptr, half * 8));
+    cc.check(fh_copy_d2d(cc.ctx_, p->data->ptr + half, b->data->ptr, half * 8));
+    p->noiseScaleDeg = a->noiseScaleDeg;
+    p->scalingFactor = a->scalingFactor;
+    p->slots = a->slots;
+    return p;
+  }
+  static Ciphertext half(CryptoContextImpl& cc, const Ciphertext& p, int which) {
+    const size_t h = (size_t)2 * p->towers * cc.N_;
+    auto x = cc.newCt(p->towers, 2);
+    cc.check(fh_copy_d2d(cc.ctx_, x->data->ptr, p->data->ptr + which * h, h * 8));
+    x->noiseScaleDeg = p->noiseScaleDeg;
+    x->scalingFactor = p->scalingFactor;
+    x->slots = p->slots;
+    return x;
+  }
+};
+
 namespace {
 // Evaluation of sum_k c_k T_k(y) (full constant term) by recursive splitting at powers of two:
 // p = q T_M + r with q_0 = c_M, q_j = 2 c_{M+j}, r_k = c_k - c_{2M-k} (T_{M+j} = 2 T_M T_j - T_{M-j});
@@ -227,6 +269,7 @@ uint32_t cheb_depth(uint32_t d) { return cheb_leaf(d) ? clog2(d) + 1 : cheb_spli
 struct ChebEval {
   CryptoContextImpl& cc;
   std::map<uint32_t, Ciphertext> T;
+  Ciphertext mul(const Ciphertext& a, const Ciphertext& b) { return BootstrapOps::mult(cc, a, b); }
 
   // T_k: T_2k = 2 T_k^2 - 1, T_2k+1 = 2 T_k T_k+1 - T_1, each with one rescale
   Ciphertext t(uint32_t k) {
@@ -234,9 +277,9 @@ struct ChebEval {
     if (it != T.end()) return it->second;
     Ciphertext r;
     if (k % 2 == 0) {
-      r = cc.IntegerCombination({cc.Rescale(cc.EvalSquare(t(k / 2)))}, {2}, -1.0);
+      r = cc.IntegerCombination({cc.Rescale(mul(t(k / 2), t(k / 2)))}, {2}, -1.0);
     } else {
-      r = cc.IntegerCombination({cc.Rescale(cc.EvalMult(t(k / 2), t(k / 2 + 1))), t(1)}, {2, -1}, 0.0);
+      r = cc.IntegerCombination({cc.Rescale(mul(t(k / 2), t(k / 2 + 1))), t(1)}, {2, -1}, 0.0);
     }
     T[k] = r;
     return r;
@@ -273,7 +316,7 @@ struct ChebEval {
       q[j] = 2.0 * c[M + j];
       r[M - j] -= c[M + j];
     }
-    Ciphertext P = q.size() == 1 ? cc.Rescale(cc.EvalMult(t(M), q[0])) : cc.Rescale(cc.EvalMult(eval(q), t(M)));
+    Ciphertext P = q.size() == 1 ? cc.Rescale(cc.EvalMult(t(M), q[0])) : cc.Rescale(mul(eval(q), t(M)));
     Ciphertext R = eval(r);
     return cc.IntegerCombination({P, R}, {1, 1}, 0.0);
   }
@@ -284,6 +327,7 @@ Ciphertext CryptoContextImpl::EvalChebyshevSeries(const Ciphertext& ct, const st
   if (coeffs.size() < 2) throw std::runtime_error("fideslib: EvalChebyshevSeries needs degree >= 1");
   if (!(b > a)) throw std::runtime_error("fideslib: EvalChebyshevSeries needs a < b");
   Ciphertext y = ct->noiseScaleDeg == 2 ? Rescale(ct) : ct;
+  if (y->ncomp == 4 && (a != -1.0 || b != 1.0)) throw std::runtime_error("fideslib: EvalChebyshevSeries of a pair runs on [-1, 1] only");
   if (a != -1.0 || b != 1.0) {
     // y = (2x - (a + b)) / (b - a)
     y = Rescale(EvalMult(y, 2.0 / (b - a)));
@@ -325,8 +369,9 @@ Ciphertext CryptoContextImpl::IntegerCombination(const std::vector<Ciphertext>& 
       for (uint32_t i = 0; i < t; ++i) add[i] = add[i] ? moduli_[i] - add[i] : 0;
   }
   auto out = newCt(t, cts[0]->ncomp);
+  // the constant goes to component 0 of each ciphertext (of both halves of a pair)
   check(fh_linear_wsum(ctx_, (int)n, ptrs.data(), towers.data(), res.data(), (int)t, (int)cts[0]->ncomp, add.empty() ? nullptr : add.data(),
-                       out->data->ptr));
+                       cts[0]->ncomp == 4 ? 5u : 1u, out->data->ptr));
   out->noiseScaleDeg = cts[0]->noiseScaleDeg;
   out->scalingFactor = cts[0]->scalingFactor;
   out->slots = cts[0]->slots;
@@ -489,13 +534,10 @@ Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0, uint32_t numI
   Ciphertext cj = EvalConjugate(t);
   Ciphertext re = EvalAdd(t, cj);
   Ciphertext im = MultByMonomial(EvalSub(t, cj), 3 * N_ / 2);
-  // 5. EvalMod: sin(2 pi K y) on both parts
-  auto evalmod = [&](const Ciphertext& y) {
-    Ciphertext c = EvalChebyshevSeries(y, st.cheb, -1.0, 1.0);
-    for (int i = 0; i < st.r; ++i) c = IntegerCombination({Rescale(EvalSquare(c))}, {2}, -1.0);
-    return c;
-  };
-  Ciphertext fre = evalmod(re), fim = evalmod(im);
+  // 5. EvalMod: sin(2 pi K y) on both parts, as one paired sequence
+  Ciphertext c = EvalChebyshevSeries(BootstrapOps::pack(*this, re, im), st.cheb, -1.0, 1.0);
+  for (int i = 0; i < st.r; ++i) c = IntegerCombination({Rescale(BootstrapOps::mult(*this, c, c))}, {2}, -1.0);
+  Ciphertext fre = BootstrapOps::half(*this, c, 0), fim = BootstrapOps::half(*this, c, 1);
   // 6. recombine (i = X^(N/2)) and SlotsToCoeffs
   Ciphertext u = EvalAdd(fre, MultByMonomial(fim, N_ / 2));
   auto sit = st.stcDiags.find(u->towers);

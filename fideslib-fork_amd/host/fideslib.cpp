@@ -712,7 +712,7 @@ Ciphertext CryptoContextImpl::EvalLinearWSum(const std::vector<Ciphertext>& cts,
       std::copy(f.begin(), f.end(), w.begin() + k * t);
     }
     auto out = newCt(t, nc);
-    check(fh_linear_wsum(ctx_, (int)n, ptrs.data(), towers.data(), w.data(), (int)t, (int)nc, nullptr, out->data->ptr));
+    check(fh_linear_wsum(ctx_, (int)n, ptrs.data(), towers.data(), w.data(), (int)t, (int)nc, nullptr, 0, out->data->ptr));
     out->noiseScaleDeg = cts[0]->noiseScaleDeg + 1;
     out->scalingFactor = cts[0]->scalingFactor * scale_;
     out->slots = cts[0]->slots;

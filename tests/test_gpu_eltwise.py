@@ -60,6 +60,23 @@ def test_linear_wsum(n, towers, with_add, gctx2, octx2):
         x.free()
 
 
+def test_linear_wsum_pair_constant(gctx2, octx2):
+    """Four components (two ciphertexts back to back), the constant on components 0 and 2."""
+    g, o = gctx2, octx2
+    towers = 6
+    pids = list(range(towers))
+    x = np.concatenate([o.random_poly(140 + j, pids) for j in range(4)])
+    w = np.array([[3 + i for i in range(towers)]], dtype=np.uint64)
+    add = np.array([int(o.primes[i]) - 7 for i in range(towers)], dtype=np.uint64)
+    d, out = g.to_device(x), g.array((4 * towers, o.N))
+    g.linear_wsum([d], [towers], w, towers, 4, out, add, add_components=5)
+    exp = np.concatenate([o.mul_scalar(x[j * towers:(j + 1) * towers], w[0]) for j in range(4)])
+    for j in (0, 2):
+        exp[j * towers:(j + 1) * towers] = o.add_scalar(exp[j * towers:(j + 1) * towers], add)
+    assert np.array_equal(out.download(), exp)
+    d.free(); out.free()
+
+
 def test_linear_wsum_rejects_bad_shapes(gctx2, fh):
     g = gctx2
     a = g.array((2 * 4, g.N))

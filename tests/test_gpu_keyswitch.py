@@ -212,6 +212,28 @@ def test_cfg2_eval_mult(level, gctx2, octx2, cfg2_material):
     d1.free(); d2.free(); out.free(); k.free()
 
 
+@pytest.mark.parametrize("n,level,square", [(2, 12, False), (3, 7, True), (1, 12, False)])
+def test_cfg2_eval_mult_many(n, level, square, gctx2, octx2, cfg2_material):
+    """n products through one batched launch sequence equal the oracle's products one by one."""
+    g, o, m = gctx2, octx2, cfg2_material
+    ids = _ids(o, level)
+    a = [np.concatenate([o.random_poly(300 + 4 * i, ids), o.random_poly(301 + 4 * i, ids)]) for i in range(n)]
+    b = a if square else [np.concatenate([o.random_poly(302 + 4 * i, ids), o.random_poly(303 + 4 * i, ids)]) for i in range(n)]
+    da = g.to_device(np.concatenate(a))
+    db = da if square else g.to_device(np.concatenate(b))
+    k = g.key(m["relin"])
+    out = g.array((n * 2 * (level + 1), o.N))
+    print(f"[eval_mult_many] n={n} level={level} square={square}", flush=True)
+    g.eval_mult_many(n, da, db, level, k, out)
+    got = out.download()
+    P = 2 * (level + 1)
+    for i in range(n):
+        assert np.array_equal(got[i * P:(i + 1) * P], o.eval_mult(a[i], b[i], level, m["relin"])), i
+    da.free(); out.free(); k.free()
+    if not square:
+        db.free()
+
+
 @pytest.mark.parametrize("rot", [1, 64])
 def test_cfg2_eval_rotate_decrypts(rot, gctx2, octx2, cfg2_material):
     g, o, m = gctx2, octx2, cfg2_material
