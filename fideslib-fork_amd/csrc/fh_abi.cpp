@@ -403,13 +403,16 @@ int fh_eval_mult(fh_context* ctx, const uint64_t* ct1, const uint64_t* ct2, int 
   });
 }
 
-int fh_eval_mult_many(fh_context* ctx, int n, const uint64_t* ct1, const uint64_t* ct2, int level, const fh_key* relin_key, uint64_t* out) {
+int fh_eval_mult_many(fh_context* ctx, int n, const uint64_t* ct1, int towers1, const uint64_t* ct2, int towers2, int level,
+                      const fh_key* relin_key, uint64_t* out) {
   return guard([&] {
     auto& c = C(ctx);
     need(ct1, "ct1"); need(ct2, "ct2"); need(out, "out");
     check_level(c, level);
     if (n < 1 || n > 4096) throw fh::Error(FH_E_INVALID, "n must be 1..4096");
-    fh::eval_mult_many(c, n, ct1, ct2, level, key_of(c, relin_key), out, c.stream);
+    if (towers1 < level + 1 || towers2 < level + 1 || towers1 > c.L1 || towers2 > c.L1)
+      throw fh::Error(FH_E_INVALID, "operand towers must be level+1..num_q");
+    fh::eval_mult_many(c, n, ct1, towers1, ct2, towers2, level, key_of(c, relin_key), out, c.stream);
   });
 }
 

@@ -67,18 +67,19 @@ __global__ __launch_bounds__(kEwThreads) void k_unary(LimbSet a, LimbSet o, Scal
   }
 }
 
-// (c0, c1) x (d0, d1) -> (c0 d0, c0 d1 + c1 d0, c1 d1); square when SQ.
-// blockIdx.z: item (inputs in_stride words apart, outputs out_stride apart)
+// (c0, c1) x (d0, d1) -> (c0 d0, c0 d1 + c1 d0, c1 d1) over the first nl limbs; square when SQ.
+// The inputs hold lc / ld limbs per component; blockIdx.z: item (ciphertexts back to back, outputs
+// 3 nl limbs apart).
 template <bool SQ>
 __global__ __launch_bounds__(kEwThreads) void k_tensor(const uint64_t* c, const uint64_t* d, uint64_t* o, uint32_t nl, const PrimeConst* pr, uint32_t N,
-                                                       uint64_t in_stride, uint64_t out_stride) {
+                                                       uint32_t lc, uint32_t ld) {
   const uint32_t limb = blockIdx.y;
   const PrimeConst P = pr[limb];
-  c += blockIdx.z * in_stride;
-  d += blockIdx.z * in_stride;
-  o += blockIdx.z * out_stride;
-  const size_t comp = (size_t)nl * N, off = (size_t)limb * N;
-  const uint64_t *c0 = c + off, *c1 = c + comp + off, *d0 = d + off, *d1 = d + comp + off;
+  const size_t cc = (size_t)lc * N, dc = (size_t)ld * N, comp = (size_t)nl * N, off = (size_t)limb * N;
+  c += blockIdx.z * 2 * cc;
+  d += blockIdx.z * 2 * dc;
+  o += blockIdx.z * 3 * comp;
+  const uint64_t *c0 = c + off, *c1 = c + cc + off, *d0 = d + off, *d1 = d + dc + off;
   uint64_t *o0 = o + off, *o1 = o + comp + off, *o2 = o + 2 * comp + off;
   for (uint32_t i = 2 * (blockIdx.x * blockDim.x + threadIdx.x); i < N; i += 2 * gridDim.x * blockDim.x) {
     V2 a0 = ld2(c0 + i), a1 = ld2(c1 + i);
@@ -108,12 +109,15 @@ struct WSumIn {
   const uint64_t* p[kMaxWSum];
   uint32_t limbs[kMaxWSum];
 };
+// weights and the added constant in the kernel arguments when n x towers <= kWSumInline (no upload)
+constexpr int kWSumInline = 320;
+struct WSumConst {
+  uint64_t w[kWSumInline];
+  uint64_t add[kMaxLimbs];
+};
 
-__global__ __launch_bounds__(kEwThreads) void k_wsum(WSumIn in, int n, const uint64_t* __restrict__ w, const uint64_t* __restrict__ add,
-                                                     uint32_t add_mask, uint32_t towers, uint64_t* out, const PrimeConst* pr, uint32_t N) {
-  const uint32_t comp = blockIdx.y / towers, limb = blockIdx.y % towers;
-  const PrimeConst P = pr[limb];
-  const uint64_t a0 = add && ((add_mask >> comp) & 1u) ? add[limb] : 0;
+__device__ __forceinline__ void wsum_body(const WSumIn& in, int n, const uint64_t* w, uint64_t a0, uint32_t comp, uint32_t limb, uint32_t towers,
+                                          uint64_t* out, const PrimeConst& P, uint32_t N) {
   uint64_t* po = out + ((size_t)comp * towers + limb) * N;
   for (uint32_t i = 2 * (blockIdx.x * blockDim.x + threadIdx.x); i < N; i += 2 * gridDim.x * blockDim.x) {
     uint64_t r0 = a0, r1 = a0;
@@ -125,6 +129,22 @@ __global__ __launch_bounds__(kEwThreads) void k_wsum(WSumIn in, int n, const uin
     }
     st2(po + i, r0, r1);
   }
+}
+
+__global__ __launch_bounds__(kEwThreads) void k_wsum_inline(WSumIn in, WSumConst kc, int n, int has_add, uint32_t add_mask, uint32_t towers,
+                                                            uint64_t* out, const PrimeConst* pr, uint32_t N) {
+  const uint32_t comp = blockIdx.y / towers, limb = blockIdx.y % towers;
+  const PrimeConst P = pr[limb];
+  const uint64_t a0 = has_add && ((add_mask >> comp) & 1u) ? kc.add[limb] : 0;
+  wsum_body(in, n, kc.w, a0, comp, limb, towers, out, P, N);
+}
+
+__global__ __launch_bounds__(kEwThreads) void k_wsum(WSumIn in, int n, const uint64_t* __restrict__ w, const uint64_t* __restrict__ add,
+                                                     uint32_t add_mask, uint32_t towers, uint64_t* out, const PrimeConst* pr, uint32_t N) {
+  const uint32_t comp = blockIdx.y / towers, limb = blockIdx.y % towers;
+  const PrimeConst P = pr[limb];
+  const uint64_t a0 = add && ((add_mask >> comp) & 1u) ? add[limb] : 0;
+  wsum_body(in, n, w, a0, comp, limb, towers, out, P, N);
 }
 
 // splitmix64 finalizer: a counter-based stream, one 64-bit word per (seed, limb job, index)
@@ -180,6 +200,18 @@ void ew_wsum(Context& c, int n, const uint64_t* const* in, const uint32_t* in_li
     wi.p[k] = in[k];
     wi.limbs[k] = in_limbs[k];
   }
+  dim3 g = ew_grid(c.N, (uint32_t)(towers * ncomp));
+  if ((size_t)n * towers <= (size_t)kWSumInline) {
+    // weights (reduced) and the constant travel in the kernel arguments
+    WSumConst kc;
+    for (int k = 0; k < n; ++k)
+      for (int i = 0; i < towers; ++i) kc.w[(size_t)k * towers + i] = weights[(size_t)k * towers + i] % c.q[i];
+    for (int i = 0; i < towers; ++i) kc.add[i] = add ? add[i] % c.q[i] : 0;
+    ProfScope ps_(c, "eltwise", st);
+    k_wsum_inline<<<g, kEwThreads, 0, st>>>(wi, kc, n, add ? 1 : 0, add_mask, (uint32_t)towers, out, c.d_prime, c.N);
+    FH_HIP(hipGetLastError());
+    return;
+  }
   // weights (reduced) and the added constant, uploaded in stream order into one pool buffer
   std::vector<uint64_t> h((size_t)(n + 1) * towers, 0);
   for (int k = 0; k < n; ++k)
@@ -188,7 +220,6 @@ void ew_wsum(Context& c, int n, const uint64_t* const* in, const uint32_t* in_li
     for (int i = 0; i < towers; ++i) h[(size_t)n * towers + i] = add[i] % c.q[i];
   DBuf d(c.pool, h.size());
   FH_HIP(hipMemcpyAsync(d.p, h.data(), h.size() * 8, hipMemcpyHostToDevice, st));
-  dim3 g = ew_grid(c.N, (uint32_t)(towers * ncomp));
   ProfScope ps_(c, "eltwise", st);
   k_wsum<<<g, kEwThreads, 0, st>>>(wi, n, d.p, add ? d.p + (size_t)n * towers : nullptr, add_mask, (uint32_t)towers, out, c.d_prime, c.N);
   FH_HIP(hipGetLastError());
@@ -237,15 +268,18 @@ void fill_uniform(const Context& c, const LimbSet& s, uint64_t seed, hipStream_t
   FH_HIP(hipGetLastError());
 }
 
-void ew_tensor(const Context& c, const uint64_t* ct1, const uint64_t* ct2, uint64_t* out3, int nlimbs, hipStream_t st, int n) {
+void ew_tensor(const Context& c, const uint64_t* ct1, const uint64_t* ct2, uint64_t* out3, int nlimbs, hipStream_t st, int n, int limbs1,
+               int limbs2) {
   if (nlimbs <= 0 || n <= 0) return;
   if (n > 65535) throw Error(-1, "too many tensor products in one launch");
+  if (limbs1 == 0) limbs1 = nlimbs;
+  if (limbs2 == 0) limbs2 = nlimbs;
+  if (limbs1 < nlimbs || limbs2 < nlimbs) throw Error(-1, "tensor inputs have fewer limbs than the product");
   dim3 g = ew_grid(c.N, nlimbs);
   g.z = (uint32_t)n;
-  const uint64_t P = (uint64_t)nlimbs * c.N;
   ProfScope ps_(c, "tensor", st);
-  if (ct1 == ct2) k_tensor<true><<<g, kEwThreads, 0, st>>>(ct1, ct1, out3, nlimbs, c.d_prime, c.N, 2 * P, 3 * P);
-  else k_tensor<false><<<g, kEwThreads, 0, st>>>(ct1, ct2, out3, nlimbs, c.d_prime, c.N, 2 * P, 3 * P);
+  if (ct1 == ct2 && limbs1 == limbs2) k_tensor<true><<<g, kEwThreads, 0, st>>>(ct1, ct1, out3, nlimbs, c.d_prime, c.N, limbs1, limbs1);
+  else k_tensor<false><<<g, kEwThreads, 0, st>>>(ct1, ct2, out3, nlimbs, c.d_prime, c.N, limbs1, limbs2);
   FH_HIP(hipGetLastError());
 }
 

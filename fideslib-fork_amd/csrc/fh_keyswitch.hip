@@ -667,13 +667,18 @@ void eval_mult(Context& c, const uint64_t* ct1, const uint64_t* ct2, int l, cons
   relinearize(c, t.p, l, key, out2, st);
 }
 
-void eval_mult_many(Context& c, int n, const uint64_t* ct1, const uint64_t* ct2, int l, const uint64_t* key, uint64_t* out2, hipStream_t st) {
+void eval_mult_many(Context& c, int n, const uint64_t* ct1, int limbs1, const uint64_t* ct2, int limbs2, int l, const uint64_t* key, uint64_t* out2,
+                    hipStream_t st) {
   if (n < 1) return;
-  if (n == 1) return eval_mult(c, ct1, ct2, l, key, out2, st);
   const uint32_t lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);
   const size_t P = (size_t)lq * c.N, E = (size_t)ext * c.N;
+  if (n == 1) {
+    DBuf t(c.pool, 3 * P);
+    ew_tensor(c, ct1, ct2, t.p, (int)lq, st, 1, limbs1, limbs2);
+    return relinearize(c, t.p, l, key, out2, st);
+  }
   DBuf t(c.pool, (size_t)n * 3 * P), dig(c.pool, (size_t)n * beta * E), eo(c.pool, (size_t)n * 2 * E), ks(c.pool, (size_t)n * 2 * P);
-  ew_tensor(c, ct1, ct2, t.p, (int)lq, st, n);
+  ew_tensor(c, ct1, ct2, t.p, (int)lq, st, n, limbs1, limbs2);
   // every d2 through one ModUp, one grouped key inner product (the key read once) and one ModDown
   modup_batch(c, t.p + 2 * P, l, n, 3 * P, dig.p, st, false);
   std::vector<const uint64_t*> keys(n, key);

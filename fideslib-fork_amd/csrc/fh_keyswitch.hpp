@@ -56,9 +56,11 @@ void automorph(Context& c, const uint64_t* x, int nlimbs, uint32_t k, uint64_t* 
 
 void relinearize(Context& c, const uint64_t* ct3, int l, const uint64_t* key, uint64_t* out2, hipStream_t st);
 void eval_mult(Context& c, const uint64_t* ct1, const uint64_t* ct2, int l, const uint64_t* key, uint64_t* out2, hipStream_t st);
-// n products of ciphertexts back to back (2 x (l+1) limbs each; ct1 == ct2: squares) with one
-// relinearization key: one tensor launch, one ModUp / key inner product / ModDown for all
-void eval_mult_many(Context& c, int n, const uint64_t* ct1, const uint64_t* ct2, int l, const uint64_t* key, uint64_t* out2, hipStream_t st);
+// n products at level l of ciphertexts back to back (2 x limbs1 / limbs2 >= l+1 limbs each, of
+// which the first l+1 per component count; ct1 == ct2: squares) with one relinearization key: one
+// tensor launch, one ModUp / key inner product / ModDown for all
+void eval_mult_many(Context& c, int n, const uint64_t* ct1, int limbs1, const uint64_t* ct2, int limbs2, int l, const uint64_t* key, uint64_t* out2,
+                    hipStream_t st);
 void rotate_ext(Context& c, const uint64_t* ct, int l, const uint64_t* key, uint32_t galois, bool add_c0, uint64_t* out_ext, hipStream_t st);
 void eval_rotate(Context& c, const uint64_t* ct, int l, const uint64_t* key, uint32_t galois, uint64_t* out2, hipStream_t st);
 

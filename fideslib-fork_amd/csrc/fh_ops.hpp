@@ -18,8 +18,10 @@ constexpr int kMaxWSum = 16;
 void ew_wsum(Context& c, int n, const uint64_t* const* in, const uint32_t* in_limbs, const uint64_t* weights, int towers, int ncomp,
              const uint64_t* add, uint32_t add_mask, uint64_t* out, hipStream_t st);
 // ct1, ct2: 2 x nlimbs over Q_l; out3: 3 x nlimbs. ct1 == ct2 selects the square form.
-// n tensor products of ciphertexts back to back (2 x nlimbs limbs each) into n x 3 x nlimbs limbs
-void ew_tensor(const Context& c, const uint64_t* ct1, const uint64_t* ct2, uint64_t* out3, int nlimbs, hipStream_t st, int n = 1);
+// n tensor products over the first nlimbs limbs of ciphertexts back to back (2 x limbs1 / limbs2
+// limbs each; 0 = nlimbs) into n x 3 x nlimbs limbs
+void ew_tensor(const Context& c, const uint64_t* ct1, const uint64_t* ct2, uint64_t* out3, int nlimbs, hipStream_t st, int n = 1, int limbs1 = 0,
+               int limbs2 = 0);
 // in place x -> x 2^64 mod q (resident key-switching keys are kept in Montgomery form)
 void to_montgomery(const Context& c, const LimbSet& s, hipStream_t st);
 // uniform residues mod each limb's prime (synthetic inputs)

@@ -56,7 +56,7 @@ def _declare(L):
         "fh_eltwise_scalar": (I, [P, I, VP, u64p, VP, i32p, I]),
         "fh_tensor": (I, [P, VP, VP, I, VP]),
         "fh_linear_wsum": (I, [P, I, VP, i32p, u64p, I, I, u64p, C.c_uint32, VP]),
-        "fh_eval_mult_many": (I, [P, I, VP, VP, I, VP, VP]),
+        "fh_eval_mult_many": (I, [P, I, VP, I, VP, I, I, VP, VP]),
         "fh_rescale": (I, [P, VP, I, I, VP]),
         "fh_mod_raise": (I, [P, VP, I, I, I, VP]),
         "fh_key_create": (I, [P, u64p, PP]),
@@ -243,9 +243,12 @@ class Context:
         _check(load().fh_linear_wsum(self.h, n, ptrs, _i32(list(towers_in)), _u64p(w), towers, ncomp, None if a is None else _u64p(a),
                                      add_components, out.ptr))
 
-    def eval_mult_many(self, n, ct1, ct2, level, relin_key, out):
-        """n products of ciphertexts back to back with one relinearization key."""
-        _check(load().fh_eval_mult_many(self.h, n, ct1.ptr, ct2.ptr, level, relin_key.h, out.ptr))
+    def eval_mult_many(self, n, ct1, ct2, level, relin_key, out, towers1=None, towers2=None):
+        """n products of ciphertexts back to back (towers1 / towers2 limbs per component, default
+        level + 1) with one relinearization key."""
+        t1 = level + 1 if towers1 is None else towers1
+        t2 = level + 1 if towers2 is None else towers2
+        _check(load().fh_eval_mult_many(self.h, n, ct1.ptr, t1, ct2.ptr, t2, level, relin_key.h, out.ptr))
 
     def tensor(self, ct1, ct2, level, out3):
         _check(load().fh_tensor(self.h, ct1.ptr, ct2.ptr, level, out3.ptr))

@@ -54,8 +54,8 @@ struct BootstrapState {
   uint32_t correction = 0;
   std::vector<double> cheb;
   std::vector<BootstrapGroup> cts, stc;         // transforms, in the order they are applied
-  std::vector<std::vector<Plaintext>> ctsDiags;  // per CoeffsToSlots group, at its level
-  std::map<uint32_t, std::vector<std::vector<Plaintext>>> stcDiags;  // by the towers entering SlotsToCoeffs
+  std::vector<CryptoContextImpl::PackedDiagonals> ctsDiags;  // per CoeffsToSlots group, at its level
+  std::map<uint32_t, std::vector<CryptoContextImpl::PackedDiagonals>> stcDiags;  // by the towers entering SlotsToCoeffs
 };
 
 namespace {
@@ -212,13 +212,12 @@ struct BootstrapOps {
     if (a0->ncomp != 4 || b0->ncomp != 4) throw std::runtime_error("fideslib: paired product of unpaired ciphertexts");
     if (!cc.relin_) throw std::runtime_error("fideslib: EvalMultKeyGen was not called");
     const uint32_t t = std::min(a0->towers, b0->towers);
-    auto a = cc.adjusted(a0, t);
-    auto b = a0 == b0 ? a : cc.adjusted(b0, t);
     auto out = cc.newCt(t, 4);
-    cc.check(fh_eval_mult_many(cc.ctx_, 2, a->data->ptr, b->data->ptr, (int)t - 1, cc.relin_, out->data->ptr));
-    out->noiseScaleDeg = a->noiseScaleDeg + b->noiseScaleDeg;
-    out->scalingFactor = a->scalingFactor * b->scalingFactor;
-    out->slots = a->slots;
+    cc.check(fh_eval_mult_many(cc.ctx_, 2, a0->data->ptr, (int)a0->towers, b0->data->ptr, (int)b0->towers, (int)t - 1, cc.relin_,
+                               out->data->ptr));
+    out->noiseScaleDeg = a0->noiseScaleDeg + b0->noiseScaleDeg;
+    out->scalingFactor = a0->scalingFactor * b0->scalingFactor;
+    out->slots = a0->slots;
     return out;
   }
   static Ciphertext pack(CryptoContextImpl& cc, const Ciphertext& a, const Ciphertext& b) {
@@ -416,8 +415,8 @@ Ciphertext CryptoContextImpl::MultByMonomial(const Ciphertext& ct, uint32_t powe
 
 // ---- bootstrapping ----
 // a group's diagonals, pre-rotated by -(bStep floor(k / bStep) stride + offset) as LinearTransform
-// expects (v'[j] = v[(j - s) mod n]), encoded extended at OpenFHE level `level`
-std::vector<Plaintext> CryptoContextImpl::groupPlaintexts(const BootstrapGroup& g, uint32_t n, uint32_t level) {
+// expects (v'[j] = v[(j - s) mod n]), encoded extended at OpenFHE level `level`, packed once
+CryptoContextImpl::PackedDiagonals CryptoContextImpl::groupDiagonals(const BootstrapGroup& g, uint32_t n, uint32_t level) {
   const uint32_t b = bstep_for(g.rowSize);
   std::vector<Plaintext> pts;
   for (uint32_t k = 0; k < g.rowSize; ++k) {
@@ -426,7 +425,7 @@ std::vector<Plaintext> CryptoContextImpl::groupPlaintexts(const BootstrapGroup& 
     for (uint32_t j = 0; j < n; ++j) v[j] = g.diags[k][(j + n - sft) % n];
     pts.push_back(MakeCKKSPackedPlaintextExtended(v, 1, level, n));
   }
-  return pts;
+  return PackDiagonals(pts, (int)g.rowSize);
 }
 
 void CryptoContextImpl::EvalBootstrapSetup(const std::vector<uint32_t>& levelBudget, const std::vector<uint32_t>& dim1, uint32_t slots,
@@ -469,7 +468,7 @@ void CryptoContextImpl::EvalBootstrapSetup(const std::vector<uint32_t>& levelBud
   st->cts = factored(n, M, levelBudget[0], true, fd);
   st->stc = factored(n, M, levelBudget[1], false, fs);
   // CoeffsToSlots group g runs at OpenFHE level 1 + g (after the prescale)
-  for (size_t g = 0; g < st->cts.size(); ++g) st->ctsDiags.push_back(groupPlaintexts(st->cts[g], n, (uint32_t)(1 + g)));
+  for (size_t g = 0; g < st->cts.size(); ++g) st->ctsDiags.push_back(groupDiagonals(st->cts[g], n, (uint32_t)(1 + g)));
   boot_ = st;
 }
 
@@ -528,8 +527,7 @@ Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0, uint32_t numI
   // 3. CoeffsToSlots: slots (I + p / q_0) / (2K), real and imaginary coefficient halves
   Ciphertext t = raised;
   for (size_t g = 0; g < st.cts.size(); ++g)
-    t = Rescale(LinearTransform(t, (int)st.cts[g].rowSize, (int)bstep_for(st.cts[g].rowSize), st.ctsDiags[g], (int)st.cts[g].stride,
-                                (int)st.cts[g].offset));
+    t = Rescale(LinearTransform(t, (int)bstep_for(st.cts[g].rowSize), st.ctsDiags[g], (int)st.cts[g].stride, (int)st.cts[g].offset));
   // 4. real and imaginary parts: t + conj(t), -i (t - conj(t))
   Ciphertext cj = EvalConjugate(t);
   Ciphertext re = EvalAdd(t, cj);
@@ -542,14 +540,13 @@ Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0, uint32_t numI
   Ciphertext u = EvalAdd(fre, MultByMonomial(fim, N_ / 2));
   auto sit = st.stcDiags.find(u->towers);
   if (sit == st.stcDiags.end()) {
-    std::vector<std::vector<Plaintext>> pts;
-    for (size_t g = 0; g < st.stc.size(); ++g) pts.push_back(groupPlaintexts(st.stc[g], n, L1_ - u->towers + (uint32_t)g));
+    std::vector<PackedDiagonals> pts;
+    for (size_t g = 0; g < st.stc.size(); ++g) pts.push_back(groupDiagonals(st.stc[g], n, L1_ - u->towers + (uint32_t)g));
     sit = st.stcDiags.emplace(u->towers, std::move(pts)).first;
   }
   Ciphertext out = u;
   for (size_t g = 0; g < st.stc.size(); ++g)
-    out = Rescale(LinearTransform(out, (int)st.stc[g].rowSize, (int)bstep_for(st.stc[g].rowSize), sit->second[g], (int)st.stc[g].stride,
-                                  (int)st.stc[g].offset));
+    out = Rescale(LinearTransform(out, (int)bstep_for(st.stc[g].rowSize), sit->second[g], (int)st.stc[g].stride, (int)st.stc[g].offset));
   if (st.correction) out = MultByInteger(out, 1ull << st.correction);
   out->slots = ct0->slots;
   return out;

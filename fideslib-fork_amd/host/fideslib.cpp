@@ -481,20 +481,23 @@ static void require_same_deg(const CiphertextImpl& a, const CiphertextImpl& b, c
 
 Ciphertext CryptoContextImpl::EvalAdd(const Ciphertext& a0, const Ciphertext& b0) {
   require_same_deg(*a0, *b0, "EvalAdd");
+  // over the first t towers of each component, read in place (no level-reduced copies)
   const uint32_t t = std::min(a0->towers, b0->towers);
-  auto a = adjusted(a0, t), b = adjusted(b0, t);
-  const uint32_t nc = std::max(a->ncomp, b->ncomp);
+  const uint32_t nc = std::max(a0->ncomp, b0->ncomp);
   auto out = newCt(t, nc);
   for (uint32_t c = 0; c < nc; ++c) {
     uint64_t* o = out->data->ptr + (size_t)c * t * N_;
-    if (c < a->ncomp && c < b->ncomp)
-      check(fh_eltwise_binary(ctx_, 0, a->data->ptr + (size_t)c * t * N_, b->data->ptr + (size_t)c * t * N_, o, nullptr, (int)t));
-    else
-      check(fh_copy_d2d(ctx_, o, (c < a->ncomp ? a : b)->data->ptr + (size_t)c * t * N_, (size_t)t * N_ * 8));
+    if (c < a0->ncomp && c < b0->ncomp)
+      check(fh_eltwise_binary(ctx_, 0, a0->data->ptr + (size_t)c * a0->towers * N_, b0->data->ptr + (size_t)c * b0->towers * N_, o, nullptr,
+                              (int)t));
+    else {
+      const Ciphertext& x = c < a0->ncomp ? a0 : b0;
+      check(fh_copy_d2d(ctx_, o, x->data->ptr + (size_t)c * x->towers * N_, (size_t)t * N_ * 8));
+    }
   }
-  out->noiseScaleDeg = a->noiseScaleDeg;
-  out->scalingFactor = a->scalingFactor;
-  out->slots = a->slots;
+  out->noiseScaleDeg = a0->noiseScaleDeg;
+  out->scalingFactor = a0->scalingFactor;
+  out->slots = a0->slots;
   return out;
 }
 
@@ -678,13 +681,13 @@ Ciphertext CryptoContextImpl::EvalMult(const Ciphertext& a0, const Ciphertext& b
   if (a0->ncomp != 2 || b0->ncomp != 2)
     throw std::runtime_error("fideslib: EvalMult: ciphertext should be relinearized before (has 3 elements; call Relinearize)");
   if (!relin_) throw std::runtime_error("fideslib: EvalMultKeyGen was not called");
+  // operands read at their own tower counts (no level-reduced copies)
   const uint32_t t = std::min(a0->towers, b0->towers);
-  auto a = adjusted(a0, t), b = adjusted(b0, t);
   auto out = newCt(t, 2);
-  check(fh_eval_mult(ctx_, a->data->ptr, b->data->ptr, (int)t - 1, relin_, out->data->ptr));
-  out->noiseScaleDeg = a->noiseScaleDeg + b->noiseScaleDeg;
-  out->scalingFactor = a->scalingFactor * b->scalingFactor;
-  out->slots = a->slots;
+  check(fh_eval_mult_many(ctx_, 1, a0->data->ptr, (int)a0->towers, b0->data->ptr, (int)b0->towers, (int)t - 1, relin_, out->data->ptr));
+  out->noiseScaleDeg = a0->noiseScaleDeg + b0->noiseScaleDeg;
+  out->scalingFactor = a0->scalingFactor * b0->scalingFactor;
+  out->slots = a0->slots;
   return out;
 }
 
@@ -837,15 +840,36 @@ std::vector<Ciphertext> CryptoContextImpl::EvalRotateMany(const std::vector<Ciph
   return out;
 }
 
+CryptoContextImpl::PackedDiagonals CryptoContextImpl::PackDiagonals(const std::vector<Plaintext>& diags, int rowSize) const {
+  if (rowSize < 1 || (int)diags.size() < rowSize) throw std::runtime_error("fideslib: LinearTransform needs rowSize diagonals");
+  const uint32_t t = diags[0]->towers, ext = t + K_;
+  for (int k = 0; k < rowSize; ++k)
+    if (!diags[k]->extended || diags[k]->towers != t) throw std::runtime_error("fideslib: diagonals must be extended plaintexts at one level");
+  PackedDiagonals p;
+  p.data = std::make_shared<DeviceBuffer>(native_, (size_t)rowSize * ext * N_);
+  for (int k = 0; k < rowSize; ++k) check(fh_copy_d2d(ctx_, p.data->ptr + (size_t)k * ext * N_, diags[k]->data->ptr, (size_t)ext * N_ * 8));
+  p.rowSize = (uint32_t)rowSize;
+  p.towers = t;
+  p.noiseScaleDeg = diags[0]->noiseScaleDeg;
+  p.scalingFactor = diags[0]->scalingFactor;
+  return p;
+}
+
 Ciphertext CryptoContextImpl::LinearTransform(const Ciphertext& ct0, int rowSize, int bStep, const std::vector<Plaintext>& diags,
                                               int stride, int offset) {
   if ((int)diags.size() < rowSize) throw std::runtime_error("fideslib: LinearTransform needs rowSize diagonals");
   Ciphertext ct = ct0->noiseScaleDeg == 2 ? Rescale(ct0) : ct0;  // rescale a fresh product first (LinearTransform.cu:77-78)
-  const uint32_t t = ct->towers, ext = t + K_;
   for (int k = 0; k < rowSize; ++k)
-    if (!diags[k]->extended || diags[k]->towers != t) throw std::runtime_error("fideslib: diagonals must be extended plaintexts at the ciphertext's level");
-  DeviceBuffer packed(native_, (size_t)rowSize * ext * N_);
-  for (int k = 0; k < rowSize; ++k) check(fh_copy_d2d(ctx_, packed.ptr + (size_t)k * ext * N_, diags[k]->data->ptr, (size_t)ext * N_ * 8));
+    if (!diags[k]->extended || diags[k]->towers != ct->towers)
+      throw std::runtime_error("fideslib: diagonals must be extended plaintexts at the ciphertext's level");
+  return LinearTransform(ct, bStep, PackDiagonals(diags, rowSize), stride, offset);
+}
+
+Ciphertext CryptoContextImpl::LinearTransform(const Ciphertext& ct0, int bStep, const PackedDiagonals& diags, int stride, int offset) {
+  Ciphertext ct = ct0->noiseScaleDeg == 2 ? Rescale(ct0) : ct0;  // rescale a fresh product first (LinearTransform.cu:77-78)
+  const uint32_t t = ct->towers;
+  if (!diags.data || diags.towers != t) throw std::runtime_error("fideslib: diagonals must be extended plaintexts at the ciphertext's level");
+  const int rowSize = (int)diags.rowSize;
   const int slots = (int)ct->slots;
   std::vector<const fh_key*> bk(bStep, nullptr);
   std::vector<uint32_t> bg(bStep, 1);
@@ -868,10 +892,10 @@ Ciphertext CryptoContextImpl::LinearTransform(const Ciphertext& ct0, int rowSize
     ok = rotKey(og);
   }
   auto out = newCt(t, 2);
-  check(fh_linear_transform(ctx_, ct->data->ptr, (int)t - 1, rowSize, bStep, stride, offset, slots, packed.ptr, bk.data(), bg.data(), gk, gg,
-                            ok, og, out->data->ptr));
-  out->noiseScaleDeg = ct->noiseScaleDeg + diags[0]->noiseScaleDeg;
-  out->scalingFactor = ct->scalingFactor * diags[0]->scalingFactor;
+  check(fh_linear_transform(ctx_, ct->data->ptr, (int)t - 1, rowSize, bStep, stride, offset, slots, diags.data->ptr, bk.data(), bg.data(), gk,
+                            gg, ok, og, out->data->ptr));
+  out->noiseScaleDeg = ct->noiseScaleDeg + diags.noiseScaleDeg;
+  out->scalingFactor = ct->scalingFactor * diags.scalingFactor;
   out->slots = ct->slots;
   return out;
 }

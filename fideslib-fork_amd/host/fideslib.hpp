@@ -217,6 +217,15 @@ class CryptoContextImpl {
   // -(bStep*(k/bStep)*stride + offset); result at the input level, noise degree + 1.
   Ciphertext LinearTransform(const Ciphertext& ct, int rowSize, int bStep, const std::vector<Plaintext>& diagonals, int stride = 1,
                              int offset = 0);
+  // the rowSize diagonals back to back in one device buffer (the layout the transform reads), for
+  // a transform applied repeatedly: the overload below skips the per-call packing
+  struct PackedDiagonals {
+    std::shared_ptr<DeviceBuffer> data;
+    uint32_t rowSize = 0, towers = 0, noiseScaleDeg = 1;
+    double scalingFactor = 1.0;
+  };
+  PackedDiagonals PackDiagonals(const std::vector<Plaintext>& diagonals, int rowSize) const;
+  Ciphertext LinearTransform(const Ciphertext& ct, int bStep, const PackedDiagonals& diagonals, int stride = 1, int offset = 0);
 
   // ---- polynomial evaluation (OpenFHE EvalChebyshevSeries / EvalChebyshevCoefficients):
   // c_0 / 2 + sum_{k>=1} c_k T_k(y), y = (2x - a - b) / (b - a); the coefficients of f on [a, b]
@@ -282,7 +291,7 @@ class CryptoContextImpl {
   friend struct BootstrapOps;
   Ciphertext MultByMonomial(const Ciphertext& ct, uint32_t power);
   Ciphertext MultByInteger(const Ciphertext& ct, uint64_t v);
-  std::vector<Plaintext> groupPlaintexts(const BootstrapGroup& g, uint32_t n, uint32_t level);
+  PackedDiagonals groupDiagonals(const BootstrapGroup& g, uint32_t n, uint32_t level);
   std::shared_ptr<struct BootstrapState> boot_;
   std::map<std::pair<uint32_t, uint32_t>, std::shared_ptr<DeviceBuffer>> monomials_;
 };

@@ -32,10 +32,11 @@ def test_scalar(op, gctx1, octx1):
     da.free(); dc.free()
 
 
-@pytest.mark.parametrize("n,towers,with_add", [(1, 5, False), (7, 12, True), (16, 10, True), (3, 25, False)])
+@pytest.mark.parametrize("n,towers,with_add", [(1, 5, False), (7, 12, True), (16, 10, True), (3, 25, False), (14, 23, True)])
 def test_linear_wsum(n, towers, with_add, gctx2, octx2):
     """fh_linear_wsum against the oracle's scalar products and additions: inputs at mixed tower
-    counts (only the first `towers` limbs of each component count), 2 components."""
+    counts (only the first `towers` limbs of each component count), 2 components; up to 320
+    weights travel in the kernel arguments, more (14 x 23) through an uploaded table."""
     g, o = gctx2, octx2
     rng = np.random.default_rng(n)
     cts, devs, tin = [], [], []

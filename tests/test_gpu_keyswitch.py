@@ -212,23 +212,31 @@ def test_cfg2_eval_mult(level, gctx2, octx2, cfg2_material):
     d1.free(); d2.free(); out.free(); k.free()
 
 
-@pytest.mark.parametrize("n,level,square", [(2, 12, False), (3, 7, True), (1, 12, False)])
-def test_cfg2_eval_mult_many(n, level, square, gctx2, octx2, cfg2_material):
-    """n products through one batched launch sequence equal the oracle's products one by one."""
+@pytest.mark.parametrize("n,level,square,e1,e2", [(2, 12, False, 0, 0), (3, 7, True, 0, 0), (1, 12, False, 0, 0), (2, 9, False, 3, 1),
+                                                  (1, 5, False, 0, 4), (2, 6, True, 2, 2)])
+def test_cfg2_eval_mult_many(n, level, square, e1, e2, gctx2, octx2, cfg2_material):
+    """n products through one batched launch sequence equal the oracle's products one by one; the
+    operands may hold e1 / e2 more towers than the product's level (read in place)."""
     g, o, m = gctx2, octx2, cfg2_material
-    ids = _ids(o, level)
-    a = [np.concatenate([o.random_poly(300 + 4 * i, ids), o.random_poly(301 + 4 * i, ids)]) for i in range(n)]
-    b = a if square else [np.concatenate([o.random_poly(302 + 4 * i, ids), o.random_poly(303 + 4 * i, ids)]) for i in range(n)]
+    t1, t2 = level + 1 + e1, level + 1 + e2
+    a = [np.concatenate([o.random_poly(300 + 4 * i, _ids(o, t1 - 1)), o.random_poly(301 + 4 * i, _ids(o, t1 - 1))]) for i in range(n)]
+    b = a if square else [np.concatenate([o.random_poly(302 + 4 * i, _ids(o, t2 - 1)), o.random_poly(303 + 4 * i, _ids(o, t2 - 1))])
+                          for i in range(n)]
     da = g.to_device(np.concatenate(a))
     db = da if square else g.to_device(np.concatenate(b))
     k = g.key(m["relin"])
     out = g.array((n * 2 * (level + 1), o.N))
-    print(f"[eval_mult_many] n={n} level={level} square={square}", flush=True)
-    g.eval_mult_many(n, da, db, level, k, out)
+    print(f"[eval_mult_many] n={n} level={level} square={square} towers {t1} {t2 if not square else t1}", flush=True)
+    g.eval_mult_many(n, da, db, level, k, out, towers1=t1, towers2=t1 if square else t2)
     got = out.download()
-    P = 2 * (level + 1)
+    P, lq = 2 * (level + 1), level + 1
+
+    def red(x, t):
+        return np.concatenate([x[:lq], x[t:t + lq]])
+
     for i in range(n):
-        assert np.array_equal(got[i * P:(i + 1) * P], o.eval_mult(a[i], b[i], level, m["relin"])), i
+        bi = red(a[i], t1) if square else red(b[i], t2)
+        assert np.array_equal(got[i * P:(i + 1) * P], o.eval_mult(red(a[i], t1), bi, level, m["relin"])), i
     da.free(); out.free(); k.free()
     if not square:
         db.free()
