@@ -248,6 +248,7 @@ def main():
     stream = uneven_stream(fh, ctx, dev_keys, gal, a) if full and a.stream_ops > 0 else None
     lt = lt_workload(fh, ctx, a) if full and a.lt_count > 0 else None
     serving = serving_sweep(ctx, dev_keys[:8], gal[:8], dev_keys[0], a) if full and a.serve_requests > 0 else None
+    enc = encode_run(fh, ctx) if full else None
     boot = bootstrap_run() if full and not a.no_bootstrap else None
     if boot is not None and serving is not None:
         serving["with_bootstrap"] = serving_bootstrap_run(a)
@@ -288,6 +289,7 @@ def main():
         "linear_transform": lt,
         "serving": serving,
         "bootstrap": boot,
+        "encode": enc,
     }
     print(json.dumps(line))
     ctx.close()
@@ -450,6 +452,54 @@ def lt_workload(fh, ctx, a):
             "batched_transforms_per_s": round(a.lt_count / dtb, 2), "batched_ms_per_transform": round(1000 * dtb / a.lt_count, 3),
             "batched_over_per_call": round(dt / dtb, 3), "bit_exact_batched_vs_per_call": same,
             "algorithmic_gb_per_transform": round(alg / 1e9, 3), "algorithmic_gb_per_batch": round(alg_b / 1e9, 3)}
+
+
+def encode_run(fh, ctx, count=64, reps=5):
+    """Device CKKS encoding (fh_encode_slots) of 64 full-slot plaintexts (32768 complex slots) at
+    N=2^16 over 25 limbs: one batched call and one call per plaintext, best of 5. Algorithmic bytes
+    per plaintext: 16 B per slot uploaded + 25 limbs written and transformed (the NTT's 16N B per
+    limb)."""
+    import time
+    N, lq = ctx.N, L1
+    rng = np.random.default_rng(11)
+    z = rng.uniform(-1, 1, (count, N // 2)) + 1j * rng.uniform(-1, 1, (count, N // 2))
+    out = ctx.array((count * lq, N))
+    pids = list(range(lq))
+    one = [_view(fh, ctx, out, b * lq * N, (lq, N)) for b in range(count)]
+
+    def batched():
+        ctx.encode_slots(z, 2.0 ** 50, pids, out)
+
+    def per_call():
+        for b in range(count):
+            ctx.encode_slots(z[b:b + 1], 2.0 ** 50, pids, one[b])
+
+    res = {}
+    for name, fn in (("batched", batched), ("per_call", per_call)):
+        fn()
+        ctx.sync()
+        best = 1e30
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ctx.sync()
+            best = min(best, time.perf_counter() - t0)
+        res[name] = best
+    # device time of one batched call: the inverse FFT, rounding and residues ("encode") and the NTT
+    ctx.profile(True)
+    ctx.profile_reset()
+    batched()
+    dev = {f: ctx.profile_read(f)[1] for f in ("encode", "ntt_fwd_cols", "ntt_fwd_rows")}
+    ctx.profile(False)
+    out.free()
+    per_pt = 16 * (N // 2) + lq * 16 * N
+    return {"plaintexts": count, "slots": N // 2, "limbs": lq,
+            "device_ms": {"ifft_round_residues": round(dev["encode"], 3), "ntt": round(dev["ntt_fwd_cols"] + dev["ntt_fwd_rows"], 3)},
+            "note": "batched_ms includes the host check and the 16 B/slot upload of the values (33.5 MB here)",
+            "batched_plaintexts_per_s": round(count / res["batched"], 1), "batched_ms": round(1000 * res["batched"], 3),
+            "per_call_plaintexts_per_s": round(count / res["per_call"], 1),
+            "batched_gbps_algorithmic": round(count * per_pt / res["batched"] / 1e9, 1),
+            "upload_bytes_per_plaintext": 16 * (N // 2), "host_encoded_upload_bytes_per_plaintext": lq * 8 * N}
 
 
 def bootstrap_run():

@@ -206,6 +206,32 @@ int fh_ntt_forward(fh_context* ctx, uint64_t* polys, const int32_t* prime_ids, i
   });
 }
 
+int fh_encode_slots(fh_context* ctx, const double* values, int count, int slots, double scale, const int32_t* prime_ids, int num_limbs,
+                    uint64_t* out, size_t out_stride) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(values, "values"); need(out, "out");
+    if (count < 1) throw fh::Error(FH_E_INVALID, "count must be >= 1");
+    if (num_limbs < 1) throw fh::Error(FH_E_INVALID, "num_limbs must be >= 1");
+    auto p = pids_or_identity(c, prime_ids, num_limbs);
+    const uint64_t stride = out_stride ? out_stride : (uint64_t)num_limbs * c.N;
+    fh::encode_slots(c, values, count, (uint32_t)slots, scale, fh::make_set(out, num_limbs, p.data(), count, stride), c.stream);
+  });
+}
+
+int fh_encode_coeffs(fh_context* ctx, const int64_t* coeffs, int count, const int32_t* prime_ids, int num_limbs, uint64_t* out,
+                     size_t out_stride) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(coeffs, "coeffs"); need(out, "out");
+    if (count < 1) throw fh::Error(FH_E_INVALID, "count must be >= 1");
+    if (num_limbs < 1) throw fh::Error(FH_E_INVALID, "num_limbs must be >= 1");
+    auto p = pids_or_identity(c, prime_ids, num_limbs);
+    const uint64_t stride = out_stride ? out_stride : (uint64_t)num_limbs * c.N;
+    fh::encode_coeffs(c, coeffs, count, fh::make_set(out, num_limbs, p.data(), count, stride), c.stream);
+  });
+}
+
 int fh_ntt_inverse(fh_context* ctx, uint64_t* polys, const int32_t* prime_ids, int num_limbs, int num_polys, size_t poly_stride) {
   return guard([&] {
     auto& c = C(ctx);

@@ -188,6 +188,11 @@ struct Context {
   uint64_t* d_rs_QlQl = nullptr;         // [L1][L1] QlQlInvModqlDivqlModq
   uint64_t* d_rs_QlQl_sh = nullptr;
 
+  // device encoder (built on first use): zeta^idx = (cos, sin)(2 pi idx / 2N) for idx < 2N, and
+  // 5^j mod 2N for j < N/2
+  double* d_ifft_w = nullptr;
+  uint32_t* d_ifft_rot = nullptr;
+
   // host copies of the conversion constants (for table checks and host helpers)
   std::vector<uint64_t> h_pmod, h_pinv;
 

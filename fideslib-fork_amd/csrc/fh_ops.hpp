@@ -18,6 +18,11 @@ constexpr int kMaxWSum = 16;
 void ew_wsum(Context& c, int n, const uint64_t* const* in, const uint32_t* in_limbs, const uint64_t* weights, int towers, int ncomp,
              const uint64_t* add, uint32_t add_mask, uint64_t* out, hipStream_t st);
 // ct1, ct2: 2 x nlimbs over Q_l; out3: 3 x nlimbs. ct1 == ct2 selects the square form.
+// CKKS encoding of `count` plaintexts into the output set (npolys == count; EVALUATION form):
+// from slot values (count x slots complex, interleaved re/im, host) with the special inverse FFT on
+// the device, or from integer coefficients (count x N, host)
+void encode_slots(Context& c, const double* values, int count, uint32_t slots, double scale, const LimbSet& out, hipStream_t st);
+void encode_coeffs(Context& c, const int64_t* coeffs, int count, const LimbSet& out, hipStream_t st);
 // n tensor products over the first nlimbs limbs of ciphertexts back to back (2 x limbs1 / limbs2
 // limbs each; 0 = nlimbs) into n x 3 x nlimbs limbs
 void ew_tensor(const Context& c, const uint64_t* ct1, const uint64_t* ct2, uint64_t* out3, int nlimbs, hipStream_t st, int n = 1, int limbs1 = 0,

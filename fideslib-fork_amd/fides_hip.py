@@ -56,6 +56,8 @@ def _declare(L):
         "fh_eltwise_scalar": (I, [P, I, VP, u64p, VP, i32p, I]),
         "fh_tensor": (I, [P, VP, VP, I, VP]),
         "fh_linear_wsum": (I, [P, I, VP, i32p, u64p, I, I, u64p, C.c_uint32, VP]),
+        "fh_encode_slots": (I, [P, VP, I, I, C.c_double, i32p, I, VP, S]),
+        "fh_encode_coeffs": (I, [P, VP, I, i32p, I, VP, S]),
         "fh_eval_mult_many": (I, [P, I, VP, I, VP, I, I, VP, VP]),
         "fh_rescale": (I, [P, VP, I, I, VP]),
         "fh_mod_raise": (I, [P, VP, I, I, I, VP]),
@@ -231,6 +233,21 @@ class Context:
         nl = len(prime_ids) if prime_ids is not None else a.shape[0]
         s = None if scalars is None else np.ascontiguousarray(scalars, dtype=np.uint64)
         _check(load().fh_eltwise_scalar(self.h, op, a.ptr, None if s is None else _u64p(s), out.ptr, _i32(prime_ids), nl))
+
+    def encode_slots(self, values, scale, prime_ids, out, out_stride=0):
+        """Encode count x slots complex values (a 2-D array) into count polynomials of
+        len(prime_ids) limbs (EVALUATION form) on the device."""
+        v = np.asarray(values, dtype=np.complex128)
+        if v.ndim == 1:
+            v = v[None, :]
+        flat = np.ascontiguousarray(np.stack([v.real, v.imag], axis=-1).reshape(-1))
+        _check(load().fh_encode_slots(self.h, flat.ctypes.data, v.shape[0], v.shape[1], float(scale), _i32(list(prime_ids)), len(prime_ids),
+                                      out.ptr, out_stride))
+
+    def encode_coeffs(self, coeffs, prime_ids, out, out_stride=0):
+        """Residues and forward NTT of count x N signed integer coefficients."""
+        m = np.ascontiguousarray(np.atleast_2d(np.asarray(coeffs, dtype=np.int64)))
+        _check(load().fh_encode_coeffs(self.h, m.ctypes.data, m.shape[0], _i32(list(prime_ids)), len(prime_ids), out.ptr, out_stride))
 
     def linear_wsum(self, cts, towers_in, weights, towers, ncomp, out, add=None, add_components=1):
         """out = sum_k weights[k] * cts[k] over the first `towers` limbs of each of ncomp components

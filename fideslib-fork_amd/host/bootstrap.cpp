@@ -82,7 +82,9 @@ std::vector<Diag> fft_stages(uint32_t n, uint64_t M, bool inverse) {
       for (uint32_t j = 0; j < lh; ++j) {
         const uint64_t idx = (g % lq) * (M / lq);
         const double ang = 2.0 * kPi * (double)idx / (double)M;
-        const std::complex<double> w(std::cos(ang), std::sin(ang));
+        double wr, wi;
+        ::sincos(ang, &wi, &wr);
+        const std::complex<double> w(wr, wi);
         const uint32_t p = i + j, q = i + j + lh;
         if (inverse) {  // x_p = (y_p + y_q) / 2, x_q = (y_p - y_q) / (2 w)
           d0[p] = 0.5, dp[p] = 0.5;
@@ -417,15 +419,29 @@ Ciphertext CryptoContextImpl::MultByMonomial(const Ciphertext& ct, uint32_t powe
 // a group's diagonals, pre-rotated by -(bStep floor(k / bStep) stride + offset) as LinearTransform
 // expects (v'[j] = v[(j - s) mod n]), encoded extended at OpenFHE level `level`, packed once
 CryptoContextImpl::PackedDiagonals CryptoContextImpl::groupDiagonals(const BootstrapGroup& g, uint32_t n, uint32_t level) {
+  // all rowSize diagonals encoded in one device call straight into the packed layout
   const uint32_t b = bstep_for(g.rowSize);
-  std::vector<Plaintext> pts;
+  std::vector<double> vals((size_t)g.rowSize * n * 2);
   for (uint32_t k = 0; k < g.rowSize; ++k) {
     const uint32_t sft = (b * (k / b) * g.stride + g.offset) % n;
-    std::vector<std::complex<double>> v(n);
-    for (uint32_t j = 0; j < n; ++j) v[j] = g.diags[k][(j + n - sft) % n];
-    pts.push_back(MakeCKKSPackedPlaintextExtended(v, 1, level, n));
+    for (uint32_t j = 0; j < n; ++j) {
+      const std::complex<double> x = g.diags[k][(j + n - sft) % n];
+      vals[((size_t)k * n + j) * 2] = x.real();
+      vals[((size_t)k * n + j) * 2 + 1] = x.imag();
+    }
   }
-  return PackDiagonals(pts, (int)g.rowSize);
+  PackedDiagonals p;
+  p.towers = L1_ - level;
+  p.rowSize = g.rowSize;
+  p.noiseScaleDeg = 1;
+  p.scalingFactor = scale_;
+  std::vector<int32_t> pids;
+  for (uint32_t i = 0; i < p.towers; ++i) pids.push_back((int32_t)i);
+  for (uint32_t k = 0; k < K_; ++k) pids.push_back((int32_t)(L1_ + k));
+  const size_t ext = pids.size() * N_;
+  p.data = std::make_shared<DeviceBuffer>(native_, (size_t)g.rowSize * ext);
+  check(fh_encode_slots(ctx_, vals.data(), (int)g.rowSize, (int)n, scale_, pids.data(), (int)pids.size(), p.data->ptr, ext));
+  return p;
 }
 
 void CryptoContextImpl::EvalBootstrapSetup(const std::vector<uint32_t>& levelBudget, const std::vector<uint32_t>& dim1, uint32_t slots,

@@ -272,23 +272,15 @@ const fh_key* CryptoContextImpl::rotKey(uint32_t galois) const {
 // ---- encoding ----
 void CryptoContextImpl::encodeInto(const std::vector<std::complex<double>>& v, uint32_t slots, double scale,
                                    const std::vector<int32_t>& pids, uint64_t* dev) const {
+  // on the device (fh_encode_slots): the special inverse FFT, rounding, residues and NTT; only
+  // the slot values are uploaded
   if (slots == 0 || (slots & (slots - 1)) || slots > N_ / 2) throw std::runtime_error("fideslib: slots must be a power of two <= N/2");
-  std::vector<Cx> z(slots, Cx{0.0, 0.0});
-  for (size_t i = 0; i < std::min<size_t>(v.size(), slots); ++i) z[i] = Cx{v[i].real(), v[i].imag()};
-  special_ifft(z, 2ull * N_);
-  std::vector<int64_t> m(N_, 0);
-  const uint32_t gap = (N_ / 2) / slots;
-  for (uint32_t i = 0; i < slots; ++i) {
-    m[i * gap] = (int64_t)std::llround(z[i].re * scale);
-    m[N_ / 2 + i * gap] = (int64_t)std::llround(z[i].im * scale);
+  std::vector<double> z(2 * (size_t)slots, 0.0);
+  for (size_t i = 0; i < std::min<size_t>(v.size(), slots); ++i) {
+    z[2 * i] = v[i].real();
+    z[2 * i + 1] = v[i].imag();
   }
-  std::vector<uint64_t> h((size_t)pids.size() * N_);
-  for (size_t l = 0; l < pids.size(); ++l) {
-    const uint64_t q = moduli_[pids[l]];
-    for (uint32_t j = 0; j < N_; ++j) h[l * N_ + j] = signed_mod(m[j], q);
-  }
-  check(fh_copy_h2d(ctx_, dev, h.data(), h.size() * 8));
-  check(fh_ntt_forward(ctx_, dev, pids.data(), (int)pids.size(), 1, 0));
+  check(fh_encode_slots(ctx_, z.data(), 1, (int)slots, scale, pids.data(), (int)pids.size(), dev, 0));
 }
 
 Plaintext CryptoContextImpl::MakeCKKSPackedPlaintext(const std::vector<std::complex<double>>& v, size_t noiseScaleDeg, uint32_t level,

@@ -1,5 +1,7 @@
 // CKKS special FFT (Cheon-Kim-Kim-Song) of the facade's encoder and of the bootstrapping
 // transforms: slots at zeta^(5^j), explicit real arithmetic so the rounding sequence is fixed.
+// Twiddles come from one sincos call per angle (glibc's sincos and separate sin / cos calls differ
+// in the last bit for some angles), as in the device encoder's tables.
 #pragma once
 #include <cmath>
 #include <cstdint>
@@ -35,7 +37,8 @@ inline void special_ifft(std::vector<Cx>& v, uint64_t M) {
       for (int j = 0; j < lenh; ++j) {
         const uint64_t idx = (lenq - (g % lenq)) * (M / lenq);
         const double ang = 2.0 * kPi * (double)idx / (double)M;
-        const double wr = std::cos(ang), wi = std::sin(ang);
+        double wr, wi;
+        ::sincos(ang, &wi, &wr);
         const Cx a = v[i + j], b = v[i + j + lenh];
         const double ur = a.re + b.re, ui = a.im + b.im;
         const double dr = a.re - b.re, di = a.im - b.im;
@@ -63,7 +66,8 @@ inline void special_fft(std::vector<Cx>& v, uint64_t M) {
       for (int j = 0; j < lenh; ++j) {
         const uint64_t idx = (g % lenq) * (M / lenq);
         const double ang = 2.0 * kPi * (double)idx / (double)M;
-        const double wr = std::cos(ang), wi = std::sin(ang);
+        double wr, wi;
+        ::sincos(ang, &wi, &wr);
         const Cx a = v[i + j], b = v[i + j + lenh];
         const double tr = b.re * wr - b.im * wi, ti = b.re * wi + b.im * wr;
         v[i + j] = Cx{a.re + tr, a.im + ti};

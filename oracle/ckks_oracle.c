@@ -4,8 +4,9 @@
  * reference code (paths relative to the FIDESlib-fork tree) or the OpenFHE
  * routine it restates. OpenFHE itself is not available here; where the
  * reference defers to OpenFHE, the published OpenFHE algorithm is restated and
- * the parity status is recorded in oracle/README.md.
+ * the parity status is stated at the routine.
  */
+#define _GNU_SOURCE /* sincos */
 #include "ckks_oracle.h"
 
 #include <complex.h>
@@ -1093,7 +1094,9 @@ static void special_fft(double complex* v, int n, u64 M) {
       for (int j = 0; j < lenh; ++j) {
         u64 idx = (g % lenq) * (M / lenq);
         double ang = 2.0 * M_PI * (double)idx / (double)M;
-        double complex w = cos(ang) + I * sin(ang);
+        double sn, cs;
+        sincos(ang, &sn, &cs); /* one call for both, as the encoders on the device and host do */
+        double complex w = cs + I * sn;
         double complex u = v[i + j], t = v[i + j + lenh] * w;
         v[i + j] = u + t;
         v[i + j + lenh] = u - t;
@@ -1112,7 +1115,9 @@ static void special_ifft(double complex* v, int n, u64 M) {
       for (int j = 0; j < lenh; ++j) {
         u64 idx = (lenq - (g % lenq)) * (M / lenq);
         double ang = 2.0 * M_PI * (double)idx / (double)M;
-        double complex w = cos(ang) + I * sin(ang);
+        double sn, cs;
+        sincos(ang, &sn, &cs); /* one call for both, as the encoders on the device and host do */
+        double complex w = cs + I * sn;
         double complex u = v[i + j] + v[i + j + lenh];
         double complex t = (v[i + j] - v[i + j + lenh]) * w;
         v[i + j] = u;
