@@ -193,6 +193,7 @@ void encode_slots(Context& c, const double* values, int count, uint32_t slots, d
   FH_HIP(hipMemcpyAsync(vals.p, values, (size_t)count * n * 2 * sizeof(double), hipMemcpyHostToDevice, st));
   D2* wk = reinterpret_cast<D2*>(work.p);
   const double2* tw = reinterpret_cast<const double2*>(c.d_ifft_w);
+  {
   ProfScope ps_(c, "encode", st);
   k_ifft_load<<<dim3((n + kEncThreads - 1) / kEncThreads, count), kEncThreads, 0, st>>>(reinterpret_cast<const double*>(vals.p), wk, n);
   FH_HIP(hipGetLastError());
@@ -207,6 +208,7 @@ void encode_slots(Context& c, const double* values, int count, uint32_t slots, d
   }
   k_encode_finish<<<enc_grid(c.N, out.nlimbs * out.npolys), kEncThreads, 0, st>>>(wk, out, c.d_prime, c.N, n, logn, loggap, scale);
   FH_HIP(hipGetLastError());
+  }
   ntt_fwd(c, out, st);
 }
 
@@ -216,9 +218,11 @@ void encode_coeffs(Context& c, const int64_t* coeffs, int count, const LimbSet& 
   if (count > 65535 || (size_t)out.nlimbs * count > 65535) throw Error(-1, "encode: too many plaintexts in one call");
   DBuf m(c.pool, (size_t)count * c.N);
   FH_HIP(hipMemcpyAsync(m.p, coeffs, (size_t)count * c.N * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  {
   ProfScope ps_(c, "encode", st);
   k_encode_coeffs<<<enc_grid(c.N, out.nlimbs * out.npolys), kEncThreads, 0, st>>>(reinterpret_cast<const int64_t*>(m.p), out, c.d_prime, c.N);
   FH_HIP(hipGetLastError());
+  }
   ntt_fwd(c, out, st);
 }
 

@@ -857,6 +857,25 @@ Ciphertext CryptoContextImpl::LinearTransform(const Ciphertext& ct0, int rowSize
   return LinearTransform(ct, bStep, PackDiagonals(diags, rowSize), stride, offset);
 }
 
+CryptoContextImpl::PreparedLinearTransform CryptoContextImpl::PrepareLinearTransform(const std::vector<Plaintext>& diagonals, int rowSize,
+                                                                                     int bStep, int stride, int offset) const {
+  if (bStep < 1 || bStep > rowSize) throw std::runtime_error("fideslib: PrepareLinearTransform: bStep must be 1..rowSize");
+  auto p = std::make_shared<PreparedLinearTransformImpl>();
+  p->rowSize = rowSize;
+  p->bStep = bStep;
+  p->stride = stride;
+  p->offset = offset;
+  p->diagonals = PackDiagonals(diagonals, rowSize);
+  return p;
+}
+
+Ciphertext CryptoContextImpl::LinearTransform(const Ciphertext& ct, const PreparedLinearTransform& prepared) {
+  if (!prepared) throw std::runtime_error("fideslib: LinearTransform: empty prepared transform");
+  return LinearTransform(ct, prepared->bStep, prepared->diagonals, prepared->stride, prepared->offset);
+}
+
+void CryptoContextImpl::LinearTransformInPlace(Ciphertext& ct, const PreparedLinearTransform& prepared) { ct = LinearTransform(ct, prepared); }
+
 Ciphertext CryptoContextImpl::LinearTransform(const Ciphertext& ct0, int bStep, const PackedDiagonals& diags, int stride, int offset) {
   Ciphertext ct = ct0->noiseScaleDeg == 2 ? Rescale(ct0) : ct0;  // rescale a fresh product first (LinearTransform.cu:77-78)
   const uint32_t t = ct->towers;

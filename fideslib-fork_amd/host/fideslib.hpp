@@ -226,6 +226,19 @@ class CryptoContextImpl {
   };
   PackedDiagonals PackDiagonals(const std::vector<Plaintext>& diagonals, int rowSize) const;
   Ciphertext LinearTransform(const Ciphertext& ct, int bStep, const PackedDiagonals& diagonals, int stride = 1, int offset = 0);
+  // a transform whose diagonals are resident, prepared once and reused (the reference's
+  // PrepareLinearTransform / LinearTransformInPlace, api/CryptoContext.hpp:576-601,
+  // api/PreparedLinearTransform.hpp:33-66): the diagonals are packed into the transform's layout at
+  // preparation; each call reads them in place
+  struct PreparedLinearTransformImpl {
+    int rowSize = 0, bStep = 0, stride = 1, offset = 0;
+    PackedDiagonals diagonals;
+  };
+  using PreparedLinearTransform = std::shared_ptr<const PreparedLinearTransformImpl>;
+  PreparedLinearTransform PrepareLinearTransform(const std::vector<Plaintext>& diagonals, int rowSize, int bStep, int stride = 1,
+                                                 int offset = 0) const;
+  Ciphertext LinearTransform(const Ciphertext& ct, const PreparedLinearTransform& prepared);
+  void LinearTransformInPlace(Ciphertext& ct, const PreparedLinearTransform& prepared);
 
   // ---- polynomial evaluation (OpenFHE EvalChebyshevSeries / EvalChebyshevCoefficients):
   // c_0 / 2 + sum_{k>=1} c_k T_k(y), y = (2x - a - b) / (b - a); the coefficients of f on [a, b]

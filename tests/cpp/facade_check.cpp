@@ -195,6 +195,11 @@ int main(int argc, char** argv) {
       for (int k = 0; k < rowSize; ++k) e[i] += x[(i + k) % slots] * D[k][i];
     }
     dump_ct(*cc, kp.secretKey, "lt", cc->LinearTransform(ct1, rowSize, bStep, diags, 1, 0), e);
+    // the same transform prepared once (resident packed diagonals), applied in place
+    auto prepared = cc->PrepareLinearTransform(diags, rowSize, bStep, 1, 0);
+    Ciphertext ltp = ct1;
+    cc->LinearTransformInPlace(ltp, prepared);
+    dump_ct(*cc, kp.secretKey, "lt_prepared", ltp, e);
 
     // scalar CRT encodings
     const double ops[] = {2.5, -0.75, 1e-9, 3.0e30, -7.25e25, 0.0};

@@ -161,6 +161,7 @@ def test_linear_transform(session, recomputed):
     g = lambda r: pow(5, r % (N // 2), 2 * N)
     res = o.linear_transform(c1, l, diags, bStep, 1, [None, R["rot"][g(1)]], [1, g(1)], R["rot"][g(2)], g(2))
     _check(session, "lt", res, 2, t, 2, sc * sc, 1e-6)
+    _check(session, "lt_prepared", res, 2, t, 2, sc * sc, 1e-6)   # PrepareLinearTransform + LinearTransformInPlace
 
 
 def test_decryption_matches_oracle_decode(session, recomputed):
