@@ -170,11 +170,34 @@ void* Pool::get(size_t bytes) {
   return p;
 }
 
+void Pool::hold() {
+  std::lock_guard<std::mutex> lk(mu_);
+  ++hold_;
+}
+
+void Pool::release_held() {
+  std::lock_guard<std::mutex> lk(mu_);
+  if (hold_ > 0 && --hold_ == 0) {
+    for (void* p : held_) {
+      auto it = sizes_.find(p);
+      if (it == sizes_.end()) continue;
+      free_.emplace(it->second, p);
+      in_use_ -= it->second;
+      cached_ += it->second;
+    }
+    held_.clear();
+  }
+}
+
 void Pool::put(void* p) {
   if (!p) return;
   std::lock_guard<std::mutex> lk(mu_);
   auto it = sizes_.find(p);
   if (it == sizes_.end()) return;
+  if (hold_ > 0) {
+    held_.push_back(p);
+    return;
+  }
   free_.emplace(it->second, p);
   in_use_ -= it->second;
   cached_ += it->second;
@@ -466,7 +489,20 @@ Context::Context(int logN_, int L1_, int K_, int dnum_, const std::vector<uint64
   FH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
 }
 
+void Context::ensure_aux_stream() {
+  if (aux_stream) return;
+  FH_HIP(hipStreamCreateWithFlags(&aux_stream, hipStreamNonBlocking));
+  FH_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+  FH_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+}
+
 Context::~Context() {
+  if (aux_stream) {
+    (void)hipStreamSynchronize(aux_stream);
+    (void)hipStreamDestroy(aux_stream);
+    (void)hipEventDestroy(ev_fork);
+    (void)hipEventDestroy(ev_join);
+  }
   if (stream) {
     (void)hipStreamSynchronize(stream);
     (void)hipStreamDestroy(stream);

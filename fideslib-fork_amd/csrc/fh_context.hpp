@@ -56,6 +56,10 @@ class Pool {
   void* get(size_t bytes);
   void put(void* p);
   void release_all();
+  // while held, returned blocks are parked instead of reused: work issued on two streams between
+  // hold() and release_held() never gets a block another stream may still be using
+  void hold();
+  void release_held();
   size_t bytes_in_use() const { return in_use_; }
   size_t bytes_cached() const { return cached_; }
 
@@ -63,6 +67,8 @@ class Pool {
   std::mutex mu_;
   std::multimap<size_t, void*> free_;
   std::map<void*, size_t> sizes_;
+  int hold_ = 0;
+  std::vector<void*> held_;
   size_t in_use_ = 0, cached_ = 0;
 };
 
@@ -115,6 +121,9 @@ struct Tune {
   // workgroup size (256 or 512 threads) of the forward column, forward row, inverse row and
   // inverse column passes; 512 applies with the default split only
   int ntt_threads[4] = {256, 256, 256, 256};
+  // 2: a batched rotation of 8 or more ciphertexts runs as two halves on two streams (ModUp,
+  // key inner product and ModDown of one half can overlap those of the other)
+  int rot_streams = 1;
 };
 
 struct Context;
@@ -197,6 +206,10 @@ struct Context {
   std::vector<uint64_t> h_pmod, h_pinv;
 
   hipStream_t stream = nullptr;
+  // second stream (default priority) and the fork / join events of two-stream batches, on first use
+  hipStream_t aux_stream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  void ensure_aux_stream();
   Pool pool;
   mutable Profiler prof;
   Tune tune;

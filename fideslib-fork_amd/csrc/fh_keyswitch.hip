@@ -790,14 +790,36 @@ void rotate_batch(Context& c, int B, const uint64_t* cts, uint64_t ct_stride, in
   if (beta > (uint32_t)kMaxDnum) throw Error(-3, "dnum above kernel limit");
   if (!ct_stride) ct_stride = 2ull * lq * c.N;
   const uint64_t out_ct = do_moddown ? 2ull * lq * c.N : 2ull * ext * c.N;
-  for (int b0 = 0; b0 < B; b0 += kMaxBatch) {
-    const int nb = std::min(B - b0, kMaxBatch);
+  auto part = [&](int b0, int nb, hipStream_t s) {
     const uint64_t* xs = cts + (size_t)b0 * ct_stride;
     DBuf dig(c.pool, (size_t)nb * beta * ext * c.N);
-    modup_batch(c, xs + (size_t)lq * c.N, l, nb, ct_stride, dig.p, st, false);
+    modup_batch(c, xs + (size_t)lq * c.N, l, nb, ct_stride, dig.p, s, false);
     rotate_core(c, nb, dig.p, (uint64_t)beta * ext * c.N, xs, ct_stride, l, keys + b0, galois + b0, add_c0, do_moddown,
-                out + (size_t)b0 * out_ct, st);
+                out + (size_t)b0 * out_ct, s);
+  };
+  if (c.tune.rot_streams > 1 && B >= 8 && B <= kMaxBatch) {
+    // two halves on two streams: the second waits for everything issued before, the caller's
+    // stream waits for the second; pool blocks freed meanwhile are parked until both are issued
+    c.ensure_aux_stream();
+    const int h = B / 2;
+    c.pool.hold();
+    try {
+      FH_HIP(hipEventRecord(c.ev_fork, st));
+      FH_HIP(hipStreamWaitEvent(c.aux_stream, c.ev_fork, 0));
+      part(0, h, st);
+      part(h, B - h, c.aux_stream);
+      FH_HIP(hipEventRecord(c.ev_join, c.aux_stream));
+      FH_HIP(hipStreamWaitEvent(st, c.ev_join, 0));
+    } catch (...) {
+      (void)hipStreamSynchronize(c.aux_stream);
+      (void)hipStreamSynchronize(st);
+      c.pool.release_held();
+      throw;
+    }
+    c.pool.release_held();
+    return;
   }
+  for (int b0 = 0; b0 < B; b0 += kMaxBatch) part(b0, std::min(B - b0, kMaxBatch), st);
 }
 
 // n rotations of ONE ciphertext from a single shared ModUp (hoisted rotation).

@@ -300,6 +300,36 @@ def test_rotate_batch_matches_per_op(mix, gctx1, octx1, cfg1_material, cfg1_rot_
         k.free()
 
 
+@pytest.mark.parametrize("B", [9, 8, 16])
+def test_rotate_batch_two_streams(B, gctx1, octx1, cfg1_material, cfg1_rot_keys):
+    """rot_streams=2 (two halves on two streams, pool blocks parked until both are issued) gives
+    the oracle's rotations; repeated calls reuse the parked blocks afterwards."""
+    g, o, m = gctx1, octx1, cfg1_material
+    steps, ks = cfg1_rot_keys
+    l = 3
+    ids = list(range(l + 1))
+    cts = [np.concatenate([o.random_poly(500 + 2 * b, ids), o.random_poly(501 + 2 * b, ids)]) for b in range(B)]
+    pick = [steps[(3 * b) % len(steps)] for b in range(B)]
+    dk = {r: g.key(ks[r]) for r in steps}
+    d = g.to_device(np.concatenate(cts))
+    rows = 2 * (l + 1)
+    out = g.array((B * rows, o.N))
+    print(f"[rotate_batch two streams] B={B}", flush=True)
+    g.tune_set("rot_streams", 2)
+    try:
+        for rep in range(2):
+            g.rotate_batch(d, B, l, [dk[r] for r in pick], [o.rot_index(r) for r in pick], out)
+            g.sync()
+            got = out.download().reshape(B, rows, o.N)
+            for b in range(B):
+                assert np.array_equal(got[b], o.eval_rotate(cts[b], l, ks[pick[b]], o.rot_index(pick[b]))), (rep, b)
+    finally:
+        g.tune_set("rot_streams", 1)
+    d.free(); out.free()
+    for k in dk.values():
+        k.free()
+
+
 @pytest.mark.parametrize("moddown", [True, False])
 def test_rotate_hoisted(moddown, gctx1, octx1, cfg1_material, cfg1_rot_keys):
     g, o, m = gctx1, octx1, cfg1_material
