@@ -176,38 +176,43 @@ void encode_slots(Context& c, const double* values, int count, uint32_t slots, d
   if (count > 65535 || (size_t)out.nlimbs * count > 65535) throw Error(-1, "encode: too many plaintexts in one call");
   if (!(scale > 0) || !std::isfinite(scale)) throw Error(-1, "encode: the scale must be positive and finite");
   // every coefficient is at most max |z| in magnitude (the inverse transform averages); llround
-  // needs |coefficient x scale| < 2^63
-  double mx = 0;
-  for (size_t i = 0; i < (size_t)count * slots; ++i) {
-    const double a = std::fabs(values[2 * i]) + std::fabs(values[2 * i + 1]);
-    if (!std::isfinite(a)) throw Error(-1, "encode: non-finite slot value");
-    mx = std::max(mx, a);
-  }
-  if (mx * scale >= 0x1p63) throw Error(-1, "encode: slot value times scale reaches 2^63");
+  // needs |coefficient x scale| < 2^63. One comparison per value also rejects NaN and infinity.
+  const double lim = 0x1p63 / scale;
+  bool bad = false;
+  for (size_t i = 0; i < (size_t)count * slots; ++i) bad |= !(std::fabs(values[2 * i]) + std::fabs(values[2 * i + 1]) < lim);
+  if (bad) throw Error(-1, "encode: a slot value is not finite or times the scale reaches 2^63");
   ensure_ifft_tables(c);
   const uint32_t n = slots;
   int logn = 0;
   while ((1u << logn) < n) ++logn;
   const int logM = c.logN + 1, loggap = c.logN - 1 - logn;
   DBuf vals(c.pool, (size_t)count * n * 2), work(c.pool, (size_t)count * n * 2);
-  FH_HIP(hipMemcpyAsync(vals.p, values, (size_t)count * n * 2 * sizeof(double), hipMemcpyHostToDevice, st));
-  D2* wk = reinterpret_cast<D2*>(work.p);
   const double2* tw = reinterpret_cast<const double2*>(c.d_ifft_w);
-  {
-  ProfScope ps_(c, "encode", st);
-  k_ifft_load<<<dim3((n + kEncThreads - 1) / kEncThreads, count), kEncThreads, 0, st>>>(reinterpret_cast<const double*>(vals.p), wk, n);
-  FH_HIP(hipGetLastError());
-  int loglen = logn;
-  for (; (1u << loglen) > kLdsElems; --loglen) {
-    k_ifft_stage<<<dim3((n / 2 + kEncThreads - 1) / kEncThreads, count), kEncThreads, 0, st>>>(wk, c.d_ifft_rot, tw, logM, n, loglen);
+  // chunks of about 8 MB of slot values: the upload of one chunk (synchronous for pageable host
+  // memory) overlaps the transforms of the previous one
+  const int chunk = std::max(1, std::min(count, (int)((8u << 20) / (16u * n))));
+  for (int b0 = 0; b0 < count; b0 += chunk) {
+    const int nb = std::min(chunk, count - b0);
+    double* v = reinterpret_cast<double*>(vals.p) + (size_t)b0 * n * 2;
+    D2* wk = reinterpret_cast<D2*>(work.p) + (size_t)b0 * n;
+    FH_HIP(hipMemcpyAsync(v, values + (size_t)b0 * n * 2, (size_t)nb * n * 2 * sizeof(double), hipMemcpyHostToDevice, st));
+    LimbSet part = out;
+    part.base = out.base + (size_t)b0 * out.poly_stride;
+    part.npolys = (uint32_t)nb;
+    ProfScope ps_(c, "encode", st);
+    k_ifft_load<<<dim3((n + kEncThreads - 1) / kEncThreads, nb), kEncThreads, 0, st>>>(v, wk, n);
     FH_HIP(hipGetLastError());
-  }
-  if (loglen >= 1) {
-    k_ifft_stages_lds<<<dim3(n >> loglen, count), kEncThreads, 0, st>>>(wk, c.d_ifft_rot, tw, logM, n, loglen);
+    int loglen = logn;
+    for (; (1u << loglen) > kLdsElems; --loglen) {
+      k_ifft_stage<<<dim3((n / 2 + kEncThreads - 1) / kEncThreads, nb), kEncThreads, 0, st>>>(wk, c.d_ifft_rot, tw, logM, n, loglen);
+      FH_HIP(hipGetLastError());
+    }
+    if (loglen >= 1) {
+      k_ifft_stages_lds<<<dim3(n >> loglen, nb), kEncThreads, 0, st>>>(wk, c.d_ifft_rot, tw, logM, n, loglen);
+      FH_HIP(hipGetLastError());
+    }
+    k_encode_finish<<<enc_grid(c.N, part.nlimbs * part.npolys), kEncThreads, 0, st>>>(wk, part, c.d_prime, c.N, n, logn, loggap, scale);
     FH_HIP(hipGetLastError());
-  }
-  k_encode_finish<<<enc_grid(c.N, out.nlimbs * out.npolys), kEncThreads, 0, st>>>(wk, out, c.d_prime, c.N, n, logn, loggap, scale);
-  FH_HIP(hipGetLastError());
   }
   ntt_fwd(c, out, st);
 }

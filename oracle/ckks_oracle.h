@@ -15,8 +15,27 @@
  * primes P. A polynomial "over Q_l" has l+1 limbs (primes 0..l); "over Q_l∪P"
  * has l+1+K limbs (primes 0..l then L1..L1+K-1).
  *
- * Parity status per function is stated in oracle/README.md and in the tests
- * that pin each function (tests/test_oracle_*.py).
+ * Parity status (the checks live in tests/test_oracle.py, tests/test_bootstrap_model.py and the
+ * GPU tests' closed-form checks):
+ *   NTT / INTT            pinned: the reference's q=17, N=8 known-answer round trip, the closed
+ *                         form A_j = sum a_i psi^((2 brev(j)+1) i) by brute force, negacyclic
+ *                         convolution
+ *   ModUp / ModDown       pinned: big-int fast base conversion and its error bounds, plain and
+ *                         ReducedNoise
+ *   Rescale               pinned: OpenFHE's QlQlInvModqlDivqlModq by the big-int floor formula,
+ *                         output by CRT
+ *   automorphism          pinned: coefficient-domain a(X^k) mod X^N + 1
+ *   key switching, rotation, relinearization, linear transform, encryption, ModRaise
+ *                         pinned by decryption invariants and closed forms
+ *   prime chain, roots    pinned to OpenFHE's published FirstPrime / PreviousPrime / root rules by
+ *                         an independent Python walk and brute force; parity unpinned against
+ *                         OpenFHE itself (not available here)
+ *   encode / decode       pinned by the round trip and the host and device encoders (same
+ *                         rounding sequence, sincos twiddles); parity unpinned against OpenFHE's
+ *                         FFTSpecialInv floating-point order
+ *   key generation, sampling (mt19937_64 streams)
+ *                         this build's own generators: the standard's known 10000th output pins
+ *                         the generator; parity unpinned against OpenFHE's samplers
  */
 #ifndef CKKS_ORACLE_H
 #define CKKS_ORACLE_H
