@@ -86,7 +86,7 @@ def main():
         name, value = kv.split("=", 1)
         ctx.tune_set(name, int(value))
     tunables = {name: ctx.tune_get(name) for name in ("ntt_split", "ntt_fwd_cols_threads", "ntt_fwd_rows_threads", "ntt_inv_rows_threads",
-                                                      "ntt_inv_cols_threads", "ntt_chunk_mb", "ntt_mix", "rot_streams")}
+                                                      "ntt_inv_cols_threads", "ntt_chunk_mb", "ntt_mix", "rot_streams", "ntt_streams")}
     N, B, lvl = ctx.N, a.batch, L1 - 1
     lq, ext, beta = lvl + 1, lvl + 1 + K, ctx.num_digits(lvl)
     ct_words = 2 * lq * N

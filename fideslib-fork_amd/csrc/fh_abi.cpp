@@ -684,6 +684,7 @@ static int* tune_slot(fh::Tune& t, const char* name, int* min_value = nullptr) {
   else if (!std::strcmp(name, "ntt_mix")) slot = &t.ntt_mix, lo = 0;
   else if (!std::strcmp(name, "ntt_split")) slot = &t.ntt_split, lo = 0;
   else if (!std::strcmp(name, "rot_streams")) slot = &t.rot_streams;
+  else if (!std::strcmp(name, "ntt_streams")) slot = &t.ntt_streams;
   for (int i = 0; i < 4 && !slot; ++i)
     if (!std::strcmp(name, kThreadNames[i])) slot = &t.ntt_threads[i], lo = 256;
   if (!slot) throw fh::Error(FH_E_INVALID, std::string("unknown tunable: ") + name);
@@ -699,6 +700,7 @@ int fh_tune_set(fh_context* ctx, const char* name, int value) {
     if (value < lo) throw fh::Error(FH_E_INVALID, std::string("tunable ") + name + " must be >= " + std::to_string(lo));
     if (!std::strcmp(name, "ntt_mix") && value > 1) throw fh::Error(FH_E_INVALID, "ntt_mix must be 0 or 1");
     if (!std::strcmp(name, "rot_streams") && value > 2) throw fh::Error(FH_E_INVALID, "rot_streams must be 1 or 2");
+    if (!std::strcmp(name, "ntt_streams") && value > 2) throw fh::Error(FH_E_INVALID, "ntt_streams must be 1 or 2");
     if (!std::strcmp(name, "ntt_split") && value != 0 && !fh::ntt_split_ok(c.logN, value))
       throw fh::Error(FH_E_INVALID, "ntt_split " + std::to_string(value) + " is not built for logN " + std::to_string(c.logN));
     if (lo == 256 && value != 256 && value != 512) throw fh::Error(FH_E_INVALID, std::string(name) + " must be 256 or 512");

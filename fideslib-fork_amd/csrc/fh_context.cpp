@@ -487,6 +487,7 @@ Context::Context(int logN_, int L1_, int K_, int dnum_, const std::vector<uint64
   d_rs_QlQl_sh = upload(QlQl_sh);
 
   FH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  ensure_aux_stream();
 }
 
 void Context::ensure_aux_stream() {

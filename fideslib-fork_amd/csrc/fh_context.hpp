@@ -124,6 +124,8 @@ struct Tune {
   // 2: a batched rotation of 8 or more ciphertexts runs as two halves on two streams (ModUp,
   // key inner product and ModDown of one half can overlap those of the other)
   int rot_streams = 1;
+  // 2: the chunks of a chunked NTT (ntt_chunk_mb below the batch) alternate between two streams
+  int ntt_streams = 1;
 };
 
 struct Context;
@@ -206,7 +208,7 @@ struct Context {
   std::vector<uint64_t> h_pmod, h_pinv;
 
   hipStream_t stream = nullptr;
-  // second stream (default priority) and the fork / join events of two-stream batches, on first use
+  // second stream (default priority) and the fork / join events of two-stream work
   hipStream_t aux_stream = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   void ensure_aux_stream();

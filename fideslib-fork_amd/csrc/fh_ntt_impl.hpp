@@ -690,12 +690,39 @@ void launch_inv_cols(const Context& c, const AddrOut& o, uint32_t n, const NttTa
 
 // WIDE: the wide (512-thread) tiles are built for this split (the default split of logN); the
 // per-pass workgroup sizes come from the context's tunables.
+// Chunks of a chunked transform alternate between the caller's stream and the context's second
+// stream when the tunable ntt_streams is 2: a chunk's column pass then runs beside the previous
+// chunk's row pass, so launch tails overlap and a chunk's intermediate can still be in the MALL
+// when its second pass reads it. The second stream starts after everything issued before on the
+// caller's stream, and the caller's stream waits for it at the end.
+struct ChunkStreams {
+  const Context& c;
+  hipStream_t st;
+  bool two;
+  ChunkStreams(const Context& ctx, hipStream_t s, uint32_t chunk, uint32_t nlimbs)
+      : c(ctx), st(s), two(ctx.tune.ntt_streams > 1 && chunk < nlimbs && ctx.aux_stream && s != ctx.aux_stream) {
+    if (two) {
+      FH_HIP(hipEventRecord(c.ev_fork, st));
+      FH_HIP(hipStreamWaitEvent(c.aux_stream, c.ev_fork, 0));
+    }
+  }
+  hipStream_t of(uint32_t k) const { return two && (k & 1) ? c.aux_stream : st; }
+  void join() const {
+    if (two) {
+      FH_HIP(hipEventRecord(c.ev_join, c.aux_stream));
+      FH_HIP(hipStreamWaitEvent(st, c.ev_join, 0));
+    }
+  }
+};
+
 template <class Addr, class AddrOut, int L1S, int L2S, bool WIDE, class Epi>
-void fwd_impl(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlimbs, hipStream_t st, const Epi& epi, const char* rows_name) {
+void fwd_impl(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlimbs, hipStream_t st0, const Epi& epi, const char* rows_name) {
   NttTables tb = ntt_tables(c, L1S);
   const bool wc = WIDE && c.tune.ntt_threads[0] == kNttWide, wr = WIDE && c.tune.ntt_threads[1] == kNttWide;
   const uint32_t chunk = chunk_limbs(c);
-  for (uint32_t base = 0; base < nlimbs; base += chunk) {
+  const ChunkStreams cs(c, st0, chunk, nlimbs);
+  for (uint32_t base = 0, k = 0; base < nlimbs; base += chunk, ++k) {
+    const hipStream_t st = cs.of(k);
     uint32_t n = std::min(chunk, nlimbs - base);
     Shift<Addr> sa{ad, base};
     Shift<AddrOut> so{ao, base};
@@ -710,14 +737,17 @@ void fwd_impl(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlim
       launch_fwd_rows<L2S, kNttThreads>(c, so, n, tb, st, se, rows_name);
     }
   }
+  cs.join();
 }
 
 template <class Addr, class AddrOut, int L1S, int L2S, bool WIDE>
-void inv_impl(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlimbs, hipStream_t st) {
+void inv_impl(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlimbs, hipStream_t st0) {
   NttTables tb = ntt_tables(c, L1S);
   const bool wr = WIDE && c.tune.ntt_threads[2] == kNttWide, wc = WIDE && c.tune.ntt_threads[3] == kNttWide;
   const uint32_t chunk = chunk_limbs(c);
-  for (uint32_t base = 0; base < nlimbs; base += chunk) {
+  const ChunkStreams cs(c, st0, chunk, nlimbs);
+  for (uint32_t base = 0, k = 0; base < nlimbs; base += chunk, ++k) {
+    const hipStream_t st = cs.of(k);
     uint32_t n = std::min(chunk, nlimbs - base);
     Shift<Addr> sa{ad, base};
     Shift<AddrOut> so{ao, base};
@@ -731,6 +761,7 @@ void inv_impl(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlim
       launch_inv_cols<L1S, kNttThreads>(c, so, n, tb, st);
     }
   }
+  cs.join();
 }
 
 template <class Addr, class AddrOut, class Epi = EpiStore>

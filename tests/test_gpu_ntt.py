@@ -121,8 +121,10 @@ def test_ntt_radix_splits_and_tiles(fh, oracle_mod, logN, l1):
         g.close()
 
 
-def test_ntt_wide_tiles_chunked_strided(gctx2, octx2):
-    """512-thread tiles with 2-limb launch chunks over strided polynomials at N=2^16."""
+@pytest.mark.parametrize("streams", [1, 2])
+def test_ntt_wide_tiles_chunked_strided(streams, gctx2, octx2):
+    """512-thread tiles with 2-limb launch chunks over strided polynomials at N=2^16, the chunks on
+    one stream or alternating between two (ntt_streams)."""
     g, o = gctx2, octx2
     N = o.N
     pids = [24, 0, 30, 3, 25]
@@ -130,11 +132,12 @@ def test_ntt_wide_tiles_chunked_strided(gctx2, octx2):
     host = np.zeros((npoly * stride_limbs, N), dtype=np.uint64)
     for p in range(npoly):
         host[p * stride_limbs:p * stride_limbs + 5] = o.random_poly(200 + p, pids)
-    saved = {k: g.tune_get(k) for k in THREAD_KNOBS + ["ntt_chunk_mb"]}
+    saved = {k: g.tune_get(k) for k in THREAD_KNOBS + ["ntt_chunk_mb", "ntt_streams"]}
     try:
         for k in THREAD_KNOBS:
             g.tune_set(k, 512)
         g.tune_set("ntt_chunk_mb", 1)
+        g.tune_set("ntt_streams", streams)
         d = g.to_device(host)
         g.ntt_forward(d, pids, num_polys=npoly, poly_stride=stride_limbs * N)
         got = d.download()
@@ -159,6 +162,10 @@ def test_tunable_validation(gctx1, fh):
         g.tune_set("ntt_fwd_cols_threads", 384)
     with pytest.raises(fh.FHError):
         g.tune_set("ntt_mix", 2)
+    with pytest.raises(fh.FHError):
+        g.tune_set("ntt_streams", 3)
+    with pytest.raises(fh.FHError):
+        g.tune_set("rot_streams", 0)
     with pytest.raises(fh.FHError):
         g.tune_set("no_such_knob", 1)
     assert g.tune_get("ntt_split") == 0
