@@ -1,0 +1,74 @@
+"""Edge cases of the C ABI: the lowest level (one limb), empty batches and transforms, and the
+contract violations each entry point rejects with an error code (never a launch)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def level0(octx1):
+    o = octx1
+    s = o.gen_secret(41)
+    return dict(s=s, relin=o.gen_relin_key(s, 42), rot=o.gen_rot_key(s, o.rot_index(1), 43))
+
+
+def _ct(o, seed, l):
+    ids = list(range(l + 1))
+    return np.concatenate([o.random_poly(seed, ids), o.random_poly(seed + 1, ids)])
+
+
+def test_level0_mult_and_rotate(gctx1, octx1, level0):
+    """Key switching at level 0 (one Q limb, one digit): products and rotations equal the oracle."""
+    g, o, m = gctx1, octx1, level0
+    a, b = _ct(o, 600, 0), _ct(o, 602, 0)
+    da, db = g.to_device(a), g.to_device(b)
+    out = g.array(a.shape)
+    k = g.key(m["relin"])
+    g.eval_mult(da, db, 0, k, out)
+    assert np.array_equal(out.download(), o.eval_mult(a, b, 0, m["relin"]))
+    rk = g.key(m["rot"])
+    g.eval_rotate(da, 0, rk, o.rot_index(1), out)
+    assert np.array_equal(out.download(), o.eval_rotate(a, 0, m["rot"], o.rot_index(1)))
+    for x in (da, db, out, k, rk):
+        x.free()
+
+
+def test_empty_work_is_a_no_op(gctx1, octx1, level0):
+    g, o, m = gctx1, octx1, level0
+    d = g.to_device(_ct(o, 610, 2))
+    before = d.download()
+    g.ntt_forward(d, [], num_limbs=0)
+    rk = g.key(m["rot"])
+    g.rotate_batch(d, 0, 2, [], [], d)
+    assert np.array_equal(d.download(), before)
+    d.free(); rk.free()
+
+
+def test_contract_violations(gctx1, octx1, level0, fh):
+    g, o, m = gctx1, octx1, level0
+    l = 2
+    d = g.to_device(_ct(o, 620, l))
+    out = g.array((2 * (l + 1), o.N))
+    k = g.key(m["relin"])
+    rk = g.key(m["rot"])
+    with pytest.raises(fh.FHError):
+        g.rescale(d, 0, 2, out)                                   # no limb to drop at level 0
+    with pytest.raises(fh.FHError):
+        g.rescale(d, l, 9, out)                                   # more than 8 components
+    with pytest.raises(fh.FHError):
+        g.rescale(d, o.L1, 2, out)                                # level beyond the chain
+    with pytest.raises(fh.FHError):
+        g.eval_rotate(d, l, rk, 2, out)                           # even Galois element
+    with pytest.raises(fh.FHError):
+        g.eval_mult_many(0, d, d, l, k, out)                      # empty batch of products
+    with pytest.raises(fh.FHError):
+        g.eval_mult_many(1, d, d, l, k, out, towers1=l)           # operand below the product's level
+    with pytest.raises(fh.FHError):
+        g.mod_raise(d, l, 2, o.L1, out)                           # raise beyond the chain
+    with pytest.raises(fh.FHError):
+        g.linear_wsum([d], [l + 1], np.ones((1, o.L1 + 1), dtype=np.uint64), o.L1 + 1, 2, out)
+    with pytest.raises(fh.FHError):
+        g.encode_coeffs(np.zeros((0, o.N), dtype=np.int64), [0], out)
+    for x in (d, out, k, rk):
+        x.free()
