@@ -539,7 +539,8 @@ def serving_bootstrap_run(a):
     keep = ("p50_ms", "p99_ms", "mean_ms", "requests_per_s", "mean_batch_ops", "bootstraps", "inference", "polynomial")
     return {"rate_per_s": 50.0, "requests": 96, "mix": "70% inference, 25% polynomial, 5% bootstrap", "towers": res["L1"],
             "batched": {k: res["batched"][k] for k in keep}, "per_op": {k: res["per_op"][k] for k in keep},
-            "same_checksum": res["batched"]["checksum"] == res["per_op"]["checksum"]}
+            "same_checksum": res["batched"]["checksum"] == res["per_op"]["checksum"],
+            "each_result_same_alone_and_in_mix": res["each_result_same_alone_and_in_mix"]}
 
 
 def serving_sweep(ctx, rot_keys, rot_gal, relin, a):

@@ -652,7 +652,8 @@ int fh_serve_run(fh_context* ctx, const fh_serve_config* config, const fh_key* c
     cfg.bootstrap_share = config->bootstrap_share;
     cfg.bootstrap = config->bootstrap;
     cfg.bootstrap_user = config->bootstrap_user;
-    if (cfg.mode != 0 && cfg.mode != 1) throw fh::Error(FH_E_INVALID, "mode must be 0 (batched) or 1 (per operation)");
+    cfg.request_checksums = config->request_checksums;
+    if (cfg.mode < 0 || cfg.mode > 2) throw fh::Error(FH_E_INVALID, "mode must be 0 (batched), 1 (per operation) or 2 (one request at a time)");
     const uint64_t* rk[8];
     for (int k = 0; k < 8; ++k) rk[k] = key_of(c, rot_keys[k]);
     fh::ServeResult r = fh::serve_run(c, cfg, rk, rot_galois, key_of(c, relin_key));

@@ -112,7 +112,7 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
     kind[i] = u < cfg.inference_share ? 1 : (u >= 1.0 - cfg.bootstrap_share ? 2 : 0);
   }
   const size_t per_request = 5 * reg_words * 8;
-  const int max_active = std::max(1, (int)std::min<double>(1e6, cfg.hbm_budget_gb * 1e9 / (double)per_request));
+  const int max_active = cfg.mode == 2 ? 1 : std::max(1, (int)std::min<double>(1e6, cfg.hbm_budget_gb * 1e9 / (double)per_request));
 
   ServeResult res;
   std::vector<double> lat;
@@ -243,10 +243,13 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
       Request* r = it->get();
       if (++r->pc == progs[r->kind].size()) {
         lat.push_back(td - r->arrival);
-        if (cfg.checksum) {
+        if (cfg.checksum || cfg.request_checksums) {
           std::vector<uint64_t> h(2ull * (r->level[3] + 1) * N);
           FH_HIP(hipMemcpy(h.data(), r->reg[3].p, h.size() * 8, hipMemcpyDeviceToHost));
-          for (uint64_t w : h) res.checksum += w;
+          uint64_t sum = 0;
+          for (uint64_t w : h) sum += w;
+          res.checksum += sum;
+          if (cfg.request_checksums) cfg.request_checksums[r->id] = sum;
         }
         ++res.completed;
         (r->kind == 1 ? res.inference : r->kind == 2 ? res.bootstraps : res.polynomial)++;

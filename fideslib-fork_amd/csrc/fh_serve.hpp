@@ -30,7 +30,7 @@ struct ServeConfig {
   int start_level = 24;        // level of every incoming ciphertext
   double hbm_budget_gb = 64.0; // admission budget for request registers
   int mode = 0;                // 0: one fused batch per tick (ticks of 1-2 operations issued directly),
-                               // 1: one call per operation
+                               // 1: one call per operation, 2: one request at a time (each alone)
   double inference_share = 0.74;
   uint64_t seed = 1;
   bool checksum = false;       // sum every request's result words (to compare the two modes)
@@ -39,6 +39,8 @@ struct ServeConfig {
   double bootstrap_share = 0.0;
   int (*bootstrap)(void* user, const uint64_t* in, int level, uint64_t* out, int* out_level) = nullptr;
   void* bootstrap_user = nullptr;
+  // optional, `requests` entries: the sum of request i's result words (mod 2^64) at index i
+  uint64_t* request_checksums = nullptr;
 };
 
 struct ServeResult {

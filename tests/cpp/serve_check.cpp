@@ -1,7 +1,8 @@
 // Serving sessions with bootstrap requests: the continuous-batching loop (fh_serve_run) over a
 // fideslib facade context whose EvalBootstrap serves the bootstrap requests through the loop's
-// callback. Runs the same seeded session batched (mode 0) and one call per operation (mode 1) and
-// prints both results as one JSON line; tests/test_gpu_serve.py compares the checksums.
+// callback. Runs the same seeded session batched (mode 0), one call per operation (mode 1) and one
+// request at a time (mode 2), after a discarded warm-up session, and prints the results as one JSON
+// line with whether every request's result is the same in all three (tests/test_gpu_serve.py).
 //
 // usage: serve_check <logN> <slots> <cts_budget> <stc_budget> <depth> <scale_bits> <rate_per_s>
 //                    <requests> <bootstrap_share> [seed] [inference_share] [hamming_weight]
@@ -92,10 +93,12 @@ int main(int argc, char** argv) {
       rg.push_back(g);
     }
     BootUser user{cc.get(), slots, 0};
-    fh_serve_result res[3];
+    fh_serve_result res[4];
+    std::vector<std::vector<uint64_t>> per(4, std::vector<uint64_t>(requests, 0));
     // a warm-up session (per operation, discarded) fills the context's buffer pool and the
-    // bootstrap's first-use state, so that neither measured session pays for them
-    for (int run = 0; run < 3; ++run) {
+    // bootstrap's first-use state, so that no measured session pays for them; then batched, per
+    // operation, and one request at a time (each alone)
+    for (int run = 0; run < 4; ++run) {
       const int mode = run == 0 ? 1 : run - 1;
       fh_serve_config cfg{};
       cfg.rate_per_s = rate;
@@ -109,15 +112,18 @@ int main(int argc, char** argv) {
       cfg.bootstrap_share = boot_share;
       cfg.bootstrap = boot_cb;
       cfg.bootstrap_user = &user;
+      cfg.request_checksums = per[run].data();
       if (fh_serve_run(cc->Native(), &cfg, rk.data(), rg.data(), cc->RelinKeyHandle(), &res[run]) != FH_OK) {
         std::fprintf(stderr, "fh_serve_run: %s\n", fh_last_error());
         return 1;
       }
     }
+    const bool same = per[1] == per[3] && per[2] == per[3];
     std::printf("{\"logN\": %d, \"slots\": %u, \"L1\": %u, \"boot_depth\": %u, \"rate_per_s\": %.3f, \"requests\": %d, "
-                "\"bootstrap_share\": %.3f, \"callbacks\": %d, \"batched\": %s, \"per_op\": %s}\n",
-                logN, slots, cc->GetNumQ(), cc->GetBootstrapDepth(), rate, requests, boot_share, user.calls, result_json(res[1]).c_str(),
-                result_json(res[2]).c_str());
+                "\"bootstrap_share\": %.3f, \"callbacks\": %d, \"each_result_same_alone_and_in_mix\": %s, \"batched\": %s, "
+                "\"per_op\": %s, \"alone\": %s}\n",
+                logN, slots, cc->GetNumQ(), cc->GetBootstrapDepth(), rate, requests, boot_share, user.calls, same ? "true" : "false",
+                result_json(res[1]).c_str(), result_json(res[2]).c_str(), result_json(res[3]).c_str());
   } catch (const std::exception& ex) {
     std::fprintf(stderr, "error: %s\n", ex.what());
     return 1;

@@ -41,12 +41,28 @@ def test_serving_batched_equals_per_op(sctx, rate, requests):
         k.free()
 
 
+def test_each_result_same_alone_and_in_the_mix(sctx):
+    """Every request's result (its result-word sum) is the same in the batched mix, in per-operation
+    execution and when it runs alone (mode 2: one request admitted at a time)."""
+    rk, rg, relin = _keys(sctx, 83)
+    runs = {mode: sctx.serve(rk, rg, relin, 3000.0, 12, mode=mode, seed=9, request_checksums=True) for mode in (0, 1, 2)}
+    assert runs[2]["max_active"] == 1 and runs[0]["max_active"] >= 3
+    per = {mode: r["request_checksums"] for mode, r in runs.items()}
+    assert all(v != 0 for v in per[2])
+    assert per[0] == per[2], [i for i in range(12) if per[0][i] != per[2][i]]
+    assert per[1] == per[2]
+    for k in rk + [relin]:
+        k.free()
+
+
 def test_serving_rejects_bad_config(sctx, fh):
     rk, rg, relin = _keys(sctx, 91)
     with pytest.raises(fh.FHError):
         sctx.serve(rk, rg, relin, 100.0, 4, start_level=5)      # too few levels for a request
     with pytest.raises(fh.FHError):
         sctx.serve(rk, rg, relin, 100.0, 4, mode=3)
+    with pytest.raises(fh.FHError):
+        sctx.serve(rk, rg, relin, 100.0, 4, mode=-1)
     with pytest.raises(fh.FHError):
         sctx.serve(rk, rg, relin, 100.0, 4, bootstrap_share=0.2)  # no bootstrap function
     with pytest.raises(fh.FHError):
@@ -85,5 +101,6 @@ def test_serving_with_bootstrap_requests(share):
     assert b["bootstraps"] == p["bootstraps"] > 0
     if share == 1.0:
         assert b["bootstraps"] == 16
-    assert res["callbacks"] == 3 * b["bootstraps"]  # warm-up, batched and per-operation sessions
+    assert res["callbacks"] == 4 * b["bootstraps"]  # warm-up, batched, per-operation and alone sessions
     assert b["checksum"] == p["checksum"], "fused batches differ from per-operation execution"
+    assert res["each_result_same_alone_and_in_mix"]
