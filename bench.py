@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--tune", action="append", default=[], metavar="NAME=VALUE", help="set a context tunable (repeatable)")
     ap.add_argument("--rotation-only", action="store_true",
                     help="time the batched rotation step only (for profiler runs: every dispatch after set-up belongs to it)")
+    ap.add_argument("--stream-only", action="store_true",
+                    help="run the mixed-level op stream only and print its result (for profiler runs)")
     ap.add_argument("--traffic-file", type=str, default=os.path.join(ROOT, "profiles", "traffic_rotation_n65536_l25.txt"),
                     help="rocprofv3 FETCH_SIZE + WRITE_SIZE bytes per launch of each kernel family of this workload")
     ap.add_argument("--autotune-polys", type=int, default=32,
@@ -81,6 +83,17 @@ def main():
     import fides_hip as fh
 
     ctx = fh.Context(LOGN, L1, K, DNUM)
+    if a.stream_only:
+        keys, gals = [], []
+        kb = ctx.array((DNUM * 2 * (L1 + K), ctx.N))
+        for i, r in enumerate(STEPS16):
+            ctx.fill_uniform(kb, list(range(L1 + K)), num_polys=DNUM * 2, seed=a.seed + 100 + i)
+            keys.append(ctx.key_from_device(kb))
+            gals.append(ctx.galois_index(r))
+        kb.free()
+        print(json.dumps({"uneven_stream": uneven_stream(fh, ctx, keys, gals, a)}))
+        ctx.close()
+        return
     autotune = ctx.autotune(a.autotune_polys) if a.autotune_polys > 0 else None
     for kv in a.tune:
         name, value = kv.split("=", 1)
