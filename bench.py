@@ -177,8 +177,21 @@ def main():
     rot_per_s = B * a.steps / t_batched
 
     per_op_rot_s = ntt_gbs = t_ntt = mult_rescale_s = None
+    pcie = None
     nprof = {}
     if not a.rotation_only:
+        # ---- the same step with the ciphertexts handed over in host memory (PCIe both ways) ----
+        host_in = cts.download()
+        ks, gs, _ = choose(4000)
+        ctx.sync()
+        t0 = time.perf_counter()
+        cts.upload(host_in)
+        ctx.rotate_batch(cts, B, lvl, ks, gs, out_b)
+        host_out = out_b.download()
+        t_pcie = time.perf_counter() - t0
+        pcie = {"rotations_per_s": round(B / t_pcie, 1), "ms_per_step": round(1000 * t_pcie, 3),
+                "bytes_each_way": int(host_in.nbytes), "host_memory": "pageable numpy arrays"}
+        del host_in, host_out
         # ---- per-op rotations (same inputs, one call per ciphertext) ----
         per_steps = max(1, min(a.steps, 3))
         for s in range(1):
@@ -292,6 +305,7 @@ def main():
         "per_op_rotations_per_s": round(per_op_rot_s, 1) if per_op_rot_s else None,
         "batched_over_per_op": round(rot_per_s / per_op_rot_s, 3) if per_op_rot_s else None,
         "bit_exact_batched_vs_per_op": bit_exact,
+        "pcie_inclusive": pcie,
         "ntt_gbps": round(ntt_gbs, 1) if ntt_gbs else None,
         "ntt_roofline_frac": round(ntt_gbs / HBM_PEAK_GBS, 4) if ntt_gbs else None,
         "ntt_ms_fwd_inv": round(1000 * t_ntt, 3) if t_ntt else None,
