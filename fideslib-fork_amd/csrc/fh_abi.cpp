@@ -334,6 +334,29 @@ int fh_key_create(fh_context* ctx, const uint64_t* host_key, fh_key** out) {
   });
 }
 
+int fh_key_export(fh_context* ctx, const fh_key* key, uint64_t* host_key) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(key, "key"); need(host_key, "host_key");
+    if (key->owner != &c) throw fh::Error(FH_E_INVALID, "key belongs to another context");
+    const size_t words = (size_t)c.dnum * 2 * c.np * c.N;
+    FH_HIP(hipStreamSynchronize(c.stream));
+    FH_HIP(hipMemcpy(host_key, key->data.p, words * 8, hipMemcpyDeviceToHost));
+    // out of the resident Montgomery form: v (2^64)^-1 mod q per limb
+    std::vector<uint64_t> rinv(c.np);
+    for (int p = 0; p < c.np; ++p) {
+      const uint64_t q = c.q[p];
+      const uint64_t r = (uint64_t)(((unsigned __int128)1 << 64) % q);
+      rinv[p] = fh::h_invmod(r, q);
+    }
+    for (size_t limb = 0; limb < (size_t)c.dnum * 2 * c.np; ++limb) {
+      const int p = (int)(limb % c.np);
+      uint64_t* v = host_key + limb * c.N;
+      for (uint32_t i = 0; i < c.N; ++i) v[i] = fh::h_mulmod(v[i], rinv[p], c.q[p]);
+    }
+  });
+}
+
 int fh_key_destroy(fh_context* ctx, fh_key* key) {
   return guard([&] {
     if (!key) return;

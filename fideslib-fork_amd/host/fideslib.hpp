@@ -16,6 +16,7 @@
 #pragma once
 #include <complex>
 #include <cstdint>
+#include <iosfwd>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -148,6 +149,9 @@ class CryptoContextImpl;
 using CryptoContext = std::shared_ptr<CryptoContextImpl>;
 CryptoContext GenCryptoContext(const CCParams& params);
 
+// api/Serialize.hpp:10-15
+enum SerType { BINARY = 0, JSON = 1 };
+
 class CryptoContextImpl {
  public:
   explicit CryptoContextImpl(const CCParams& p);
@@ -267,6 +271,17 @@ class CryptoContextImpl {
   const fh_key* RotationKeyHandle(int32_t index) const { return rotKey(galoisFor(index)); }
   const fh_key* RelinKeyHandle() const { return relin_; }
 
+  // ---- serialization (api/CryptoContext.hpp:110-127, api/Serialize.hpp; this build's own binary
+  // format, since OpenFHE's cereal archives are not available): evaluation keys of this context
+  // to / from a stream. BINARY only; JSON returns false.
+  bool SerializeEvalMultKey(std::ostream& ser, const SerType& sertype) const;
+  bool DeserializeEvalMultKey(std::istream& ser, const SerType& sertype);
+  bool SerializeEvalAutomorphismKey(std::ostream& ser, const SerType& sertype) const;
+  bool DeserializeEvalAutomorphismKey(std::istream& ser, const SerType& sertype);
+  const CCParams& GetParams() const { return params_; }
+  // a device buffer owned like the context's own (for objects rebuilt from host data)
+  std::shared_ptr<DeviceBuffer> AllocateDevice(size_t words) const;
+
   // ---- scalar CRT encodings (src/CKKS/Context.cu:669-863), exposed for tests
   std::vector<uint64_t> ElemForEvalMult(uint32_t towers, double operand) const;
   std::vector<uint64_t> ElemForEvalAddOrSub(uint32_t towers, double operand, uint32_t noiseDeg) const;
@@ -308,5 +323,18 @@ class CryptoContextImpl {
   std::shared_ptr<struct BootstrapState> boot_;
   std::map<std::pair<uint32_t, uint32_t>, std::shared_ptr<DeviceBuffer>> monomials_;
 };
+
+namespace Serial {
+// api/Serialize.hpp:23-29. Keys and ciphertexts are read back into an existing context (the
+// reference resolves it through OpenFHE's context registry): pass the context deserialized first.
+bool SerializeToFile(const std::string& filename, const CryptoContext& obj, const SerType& sertype);
+bool DeserializeFromFile(const std::string& filename, CryptoContext& obj, const SerType& sertype);
+bool SerializeToFile(const std::string& filename, const PublicKey& obj, const SerType& sertype);
+bool DeserializeFromFile(const std::string& filename, PublicKey& obj, const SerType& sertype, const CryptoContext& cc);
+bool SerializeToFile(const std::string& filename, const PrivateKey& obj, const SerType& sertype);
+bool DeserializeFromFile(const std::string& filename, PrivateKey& obj, const SerType& sertype, const CryptoContext& cc);
+bool SerializeToFile(const std::string& filename, const Ciphertext& obj, const SerType& sertype);
+bool DeserializeFromFile(const std::string& filename, Ciphertext& obj, const SerType& sertype, const CryptoContext& cc);
+}  // namespace Serial
 
 }  // namespace fideslib
