@@ -24,6 +24,11 @@ uint64_t powmod(uint64_t a, uint64_t e, uint64_t q) {
   return r;
 }
 
+// extended ciphertexts (EvalFastRotationExt) only go to KeySwitchDown
+static void require_standard(const CiphertextImpl& c, const char* op) {
+  if (c.extended) throw std::runtime_error(std::string("fideslib: ") + op + ": extended ciphertext (call KeySwitchDown first)");
+}
+
 uint64_t signed_mod(int64_t v, uint64_t q) {
   if (v >= 0) return (uint64_t)v % q;
   uint64_t r = (uint64_t)(-(v + 1)) % q;
@@ -373,6 +378,7 @@ Ciphertext CryptoContextImpl::Encrypt(const PublicKey& pk, const Plaintext& pt) 
 }
 
 Plaintext CryptoContextImpl::Decrypt(const PrivateKey& sk, const Ciphertext& ct) {
+  require_standard(*ct, "Decrypt");
   const uint32_t t = ct->towers;
   DeviceBuffer m(native_, (size_t)t * N_), tmp(native_, (size_t)t * N_);
   const uint64_t* c0 = ct->data->ptr;
@@ -438,6 +444,7 @@ void CryptoContextImpl::StoreCiphertext(const Ciphertext& ct, uint64_t* dev) con
 
 // ---- level alignment (FIXEDMANUAL: drop towers, no rescale) ----
 Ciphertext CryptoContextImpl::adjusted(const Ciphertext& ct, uint32_t towers) const {
+  require_standard(*ct, "level alignment");
   if (ct->towers == towers) return ct;
   if (ct->towers < towers) throw std::runtime_error("fideslib: cannot raise a ciphertext's level");
   auto out = newCt(towers, ct->ncomp);
@@ -456,6 +463,7 @@ Ciphertext CryptoContextImpl::LevelReduce(const Ciphertext& ct, uint32_t levels)
 }
 
 Ciphertext CryptoContextImpl::EvalNegate(const Ciphertext& ct) {
+  require_standard(*ct, "EvalNegate");
   auto out = newCt(ct->towers, ct->ncomp);
   for (uint32_t c = 0; c < ct->ncomp; ++c)
     check(fh_eltwise_scalar(ctx_, 2, ct->data->ptr + (size_t)c * ct->towers * N_, nullptr, out->data->ptr + (size_t)c * ct->towers * N_,
@@ -467,6 +475,8 @@ Ciphertext CryptoContextImpl::EvalNegate(const Ciphertext& ct) {
 }
 
 static void require_same_deg(const CiphertextImpl& a, const CiphertextImpl& b, const char* op) {
+  require_standard(a, op);
+  require_standard(b, op);
   if (a.noiseScaleDeg != b.noiseScaleDeg)
     throw std::runtime_error(std::string("fideslib: ") + op + ": operands have different noise scale degrees (rescale first)");
 }
@@ -602,6 +612,7 @@ std::vector<uint64_t> CryptoContextImpl::ElemForEvalAddOrSub(uint32_t towers, do
 }
 
 Ciphertext CryptoContextImpl::EvalAdd(const Ciphertext& a, double scalar) {
+  require_standard(*a, "EvalAdd");
   auto out = newCt(a->towers, a->ncomp);
   check(fh_copy_d2d(ctx_, out->data->ptr, a->data->ptr, (size_t)a->ncomp * a->towers * N_ * 8));
   const auto s = ElemForEvalAddOrSub(a->towers, std::fabs(scalar), a->noiseScaleDeg);
@@ -621,6 +632,7 @@ Ciphertext CryptoContextImpl::EvalAdd(const Ciphertext& a, double scalar) {
 Ciphertext CryptoContextImpl::EvalSub(const Ciphertext& a, double scalar) { return EvalAdd(a, -scalar); }
 
 Ciphertext CryptoContextImpl::EvalMult(const Ciphertext& a, double scalar) {
+  require_standard(*a, "EvalMult");
   const auto f = ElemForEvalMult(a->towers, scalar);
   auto out = newCt(a->towers, a->ncomp);
   for (uint32_t c = 0; c < a->ncomp; ++c)
@@ -658,6 +670,7 @@ Ciphertext CryptoContextImpl::EvalMultNoRelin(const Ciphertext& a0, const Cipher
 }
 
 Ciphertext CryptoContextImpl::Relinearize(const Ciphertext& ct) {
+  require_standard(*ct, "Relinearize");
   if (ct->ncomp == 2) return ct;
   if (!relin_) throw std::runtime_error("fideslib: EvalMultKeyGen was not called");
   auto out = newCt(ct->towers, 2);
@@ -670,6 +683,8 @@ Ciphertext CryptoContextImpl::Relinearize(const Ciphertext& ct) {
 
 // RequireDegree1 (api/CryptoContext.cpp:994-998), then the key-switch-first product
 Ciphertext CryptoContextImpl::EvalMult(const Ciphertext& a0, const Ciphertext& b0) {
+  require_standard(*a0, "EvalMult");
+  require_standard(*b0, "EvalMult");
   if (a0->ncomp != 2 || b0->ncomp != 2)
     throw std::runtime_error("fideslib: EvalMult: ciphertext should be relinearized before (has 3 elements; call Relinearize)");
   if (!relin_) throw std::runtime_error("fideslib: EvalMultKeyGen was not called");
@@ -727,6 +742,7 @@ Ciphertext CryptoContextImpl::EvalAddMany(const std::vector<Ciphertext>& cts) {
 // the tracked scale by OpenFHE's ModReduceFactor, which the FIXED techniques set to the nominal
 // scaling factor 2^scaleModSize (not q_l).
 Ciphertext CryptoContextImpl::Rescale(const Ciphertext& ct) {
+  require_standard(*ct, "Rescale");
   if (ct->towers < 2) throw std::runtime_error("fideslib: Rescale at the last tower");
   auto out = newCt(ct->towers - 1, ct->ncomp);
   check(fh_rescale(ctx_, ct->data->ptr, (int)ct->towers - 1, (int)ct->ncomp, out->data->ptr));
@@ -741,6 +757,7 @@ Ciphertext CryptoContextImpl::Rescale(const Ciphertext& ct) {
 uint32_t CryptoContextImpl::galoisFor(int32_t index) const { return FindAutomorphismIndex(normalize_index(index, N_ / 2)); }
 
 Ciphertext CryptoContextImpl::EvalRotate(const Ciphertext& ct, int32_t index) {
+  require_standard(*ct, "EvalRotate");
   if (ct->ncomp != 2) throw std::runtime_error("fideslib: EvalRotate: ciphertext should be relinearized before");
   if (normalize_index(index, N_ / 2) == 0) return ct;
   const uint32_t g = galoisFor(index);
@@ -771,6 +788,7 @@ struct FastRotationDigits {
 
 // EvalFastRotationPrecompute: the shared ModUp of c1 (CudaEngine.cpp:85-106)
 std::shared_ptr<void> CryptoContextImpl::EvalFastRotationPrecompute(const Ciphertext& ct) {
+  require_standard(*ct, "EvalFastRotationPrecompute");
   const int l = (int)ct->towers - 1;
   const int beta = fh_num_digits(ctx_, l);
   if (beta < 0) check(beta);
@@ -801,6 +819,43 @@ std::vector<Ciphertext> CryptoContextImpl::EvalFastRotation(const Ciphertext& ct
                                                             const std::shared_ptr<void>& precomp) {
   std::vector<Ciphertext> out;
   for (int32_t i : indices) out.push_back(EvalFastRotation(ct, i, m, precomp));
+  return out;
+}
+
+Ciphertext CryptoContextImpl::EvalFastRotationExt(const Ciphertext& ct, int32_t index, const std::shared_ptr<void>& precomp, bool addFirst) {
+  if (ct->ncomp != 2 || ct->extended) throw std::runtime_error("fideslib: EvalFastRotationExt: ciphertext should be relinearized, not extended");
+  auto h = std::static_pointer_cast<FastRotationDigits>(precomp);
+  if (!h || h->towers != ct->towers) throw std::runtime_error("fideslib: stale EvalFastRotation precomputation");
+  if (normalize_index(index, N_ / 2) == 0) throw std::runtime_error("fideslib: EvalFastRotationExt by a multiple of N/2 is not supported");
+  const uint32_t ext = ct->towers + K_;
+  const uint32_t g = galoisFor(index);
+  auto out = std::make_shared<CiphertextImpl>();
+  out->data = std::make_shared<DeviceBuffer>(native_, (size_t)2 * ext * N_);
+  check(fh_key_inner_product(ctx_, h->digits->ptr, (int)ct->towers - 1, rotKey(g), addFirst ? ct->data->ptr : nullptr, g, out->data->ptr,
+                             out->data->ptr + (size_t)ext * N_));
+  out->towers = ct->towers;
+  out->ncomp = 2;
+  out->extended = true;
+  out->noiseScaleDeg = ct->noiseScaleDeg;
+  out->scalingFactor = ct->scalingFactor;
+  out->slots = ct->slots;
+  return out;
+}
+
+std::vector<Ciphertext> CryptoContextImpl::EvalFastRotationExt(const Ciphertext& ct, const std::vector<int32_t>& indices,
+                                                               const std::shared_ptr<void>& precomp, bool addFirst) {
+  std::vector<Ciphertext> out;
+  for (int32_t i : indices) out.push_back(EvalFastRotationExt(ct, i, precomp, addFirst));
+  return out;
+}
+
+Ciphertext CryptoContextImpl::KeySwitchDown(const Ciphertext& x) {
+  if (!x->extended) throw std::runtime_error("fideslib: KeySwitchDown needs an extended ciphertext");
+  auto out = newCt(x->towers, 2);
+  check(fh_moddown(ctx_, x->data->ptr, (int)x->towers - 1, 2, out->data->ptr));
+  out->noiseScaleDeg = x->noiseScaleDeg;
+  out->scalingFactor = x->scalingFactor;
+  out->slots = x->slots;
   return out;
 }
 
@@ -877,6 +932,7 @@ Ciphertext CryptoContextImpl::LinearTransform(const Ciphertext& ct, const Prepar
 void CryptoContextImpl::LinearTransformInPlace(Ciphertext& ct, const PreparedLinearTransform& prepared) { ct = LinearTransform(ct, prepared); }
 
 Ciphertext CryptoContextImpl::LinearTransform(const Ciphertext& ct0, int bStep, const PackedDiagonals& diags, int stride, int offset) {
+  require_standard(*ct0, "LinearTransform");
   Ciphertext ct = ct0->noiseScaleDeg == 2 ? Rescale(ct0) : ct0;  // rescale a fresh product first (LinearTransform.cu:77-78)
   const uint32_t t = ct->towers;
   if (!diags.data || diags.towers != t) throw std::runtime_error("fideslib: diagonals must be extended plaintexts at the ciphertext's level");

@@ -87,6 +87,9 @@ class CiphertextImpl {
   uint32_t noiseScaleDeg = 1;
   double scalingFactor = 1.0;
   uint32_t slots = 0;
+  // extended form (EvalFastRotationExt): each component over Q_l ∪ P (towers + K limbs), for
+  // KeySwitchDown only
+  bool extended = false;
   // OpenFHE level: towers consumed from the top of the chain
   uint32_t GetLevel(uint32_t L1) const { return L1 - towers; }
   uint32_t GetNoiseScaleDeg() const { return noiseScaleDeg; }
@@ -215,6 +218,12 @@ class CryptoContextImpl {
   std::vector<Ciphertext> EvalFastRotation(const Ciphertext& ct, const std::vector<int32_t>& indices, uint32_t m,
                                            const std::shared_ptr<void>& precomp);
   std::vector<Ciphertext> EvalRotateMany(const std::vector<Ciphertext>& cts, int32_t index);
+  // rotation from the precomputed digits stopping before ModDown: components over Q_l ∪ P, P*c0
+  // folded when addFirst (api/CryptoContext.hpp:372-375); KeySwitchDown finishes it
+  Ciphertext EvalFastRotationExt(const Ciphertext& ct, int32_t index, const std::shared_ptr<void>& precomp, bool addFirst);
+  std::vector<Ciphertext> EvalFastRotationExt(const Ciphertext& ct, const std::vector<int32_t>& indices, const std::shared_ptr<void>& precomp,
+                                              bool addFirst);
+  Ciphertext KeySwitchDown(const Ciphertext& ctExt);
 
   // ---- linear transform (api/CryptoContext.hpp:460-517): diagonals built with
   // MakeCKKSPackedPlaintextExtended at the ciphertext's level, D_k pre-rotated by

@@ -177,6 +177,10 @@ int main(int argc, char** argv) {
     dump_ct(*cc, kp.secretKey, "conj", cc->EvalConjugate(ct1), e);
     auto pre = cc->EvalFastRotationPrecompute(ct1);
     dump_ct(*cc, kp.secretKey, "fast7", cc->EvalFastRotation(ct1, 7, 2 * N, pre), rotate(x, 7));
+    // the extended form from the same digits, finished by KeySwitchDown
+    auto exts = cc->EvalFastRotationExt(ct1, std::vector<int32_t>{7, -3}, pre, true);
+    dump_ct(*cc, kp.secretKey, "fastext7", cc->KeySwitchDown(exts[0]), rotate(x, 7));
+    dump_ct(*cc, kp.secretKey, "fastextm3", cc->KeySwitchDown(exts[1]), rotate(x, -3));
     auto many = cc->EvalRotateMany({ct1, ct2}, 2);
     dump_ct(*cc, kp.secretKey, "many0", many[0], rotate(x, 2));
     dump_ct(*cc, kp.secretKey, "many1", many[1], rotate(y, 2));
@@ -212,6 +216,11 @@ int main(int argc, char** argv) {
     int errors = 0;
     try {
       cc->EvalMult(nr, ct1);
+    } catch (const std::runtime_error&) {
+      ++errors;
+    }
+    try {
+      cc->Rescale(exts[0]);  // an extended ciphertext only goes to KeySwitchDown
     } catch (const std::runtime_error&) {
       ++errors;
     }

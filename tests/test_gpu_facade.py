@@ -146,6 +146,9 @@ def test_rotations(session, recomputed):
     _check(session, "rotm3", o.eval_rotate(c2, l, R["rot"][g(-3)], g(-3)), 2, t, 1, sc, 1e-6)
     _check(session, "conj", o.eval_rotate(c1, l, R["rot"][2 * N - 1], 2 * N - 1), 2, t, 1, sc, 1e-6)
     _check(session, "fast7", o.eval_rotate(c1, l, R["rot"][g(7)], g(7)), 2, t, 1, sc, 1e-6)
+    # EvalFastRotationExt + KeySwitchDown from the same precomputed digits
+    _check(session, "fastext7", o.eval_rotate(c1, l, R["rot"][g(7)], g(7)), 2, t, 1, sc, 1e-6)
+    _check(session, "fastextm3", o.eval_rotate(c1, l, R["rot"][g(-3)], g(-3)), 2, t, 1, sc, 1e-6)
     _check(session, "many0", o.eval_rotate(c1, l, R["rot"][g(2)], g(2)), 2, t, 1, sc, 1e-6)
     _check(session, "many1", o.eval_rotate(c2, l, R["rot"][g(2)], g(2)), 2, t, 1, sc, 1e-6)
 
@@ -191,4 +194,4 @@ def test_scalar_encodings(session, recomputed, oracle_mod):
 
 def test_error_paths(session):
     _, man = session
-    assert man["errors_thrown"] == 3
+    assert man["errors_thrown"] == 4
