@@ -510,6 +510,14 @@ BootstrapConstants CryptoContextImpl::GetBootstrapConstants() const {
   return BootstrapConstants{boot_->slots, boot_->correction, boot_->K, boot_->r, boot_->pre, boot_->cheb, boot_->cts, boot_->stc};
 }
 
+Ciphertext CryptoContextImpl::EvalBootstrapToLevel(const Ciphertext& ct, uint32_t outputLevel, uint32_t numIterations, uint32_t precision) {
+  Ciphertext out = EvalBootstrap(ct, numIterations, precision);
+  if (outputLevel >= L1_) throw std::runtime_error("fideslib: EvalBootstrapToLevel: level out of range");
+  const uint32_t towers = L1_ - outputLevel;
+  if (towers > out->towers) throw std::runtime_error("fideslib: EvalBootstrapToLevel: the bootstrap ends below that level");
+  return towers < out->towers ? LevelReduce(out, out->towers - towers) : out;
+}
+
 uint32_t CryptoContextImpl::GetBootstrapDepth() const {
   if (!boot_) throw std::runtime_error("fideslib: EvalBootstrapSetup first");
   // prescale 1 + one per CoeffsToSlots group + series + double angle r + one per SlotsToCoeffs

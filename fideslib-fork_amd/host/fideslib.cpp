@@ -931,6 +931,82 @@ Ciphertext CryptoContextImpl::LinearTransform(const Ciphertext& ct, const Prepar
 
 void CryptoContextImpl::LinearTransformInPlace(Ciphertext& ct, const PreparedLinearTransform& prepared) { ct = LinearTransform(ct, prepared); }
 
+CryptoContextImpl::PreparedLinearTransform CryptoContextImpl::PrepareLinearTransformMany(const std::vector<std::vector<Plaintext>>& sets,
+                                                                                         int rowSize, int bStep, int stride, int offset) const {
+  if (sets.empty()) throw std::runtime_error("fideslib: PrepareLinearTransformMany needs at least one diagonal set");
+  if (bStep < 1 || bStep > rowSize) throw std::runtime_error("fideslib: PrepareLinearTransformMany: bStep must be 1..rowSize");
+  auto p = std::make_shared<PreparedLinearTransformImpl>();
+  p->rowSize = rowSize;
+  p->bStep = bStep;
+  p->stride = stride;
+  p->offset = offset;
+  p->diagonals = PackDiagonals(sets[0], rowSize);
+  for (size_t i = 1; i < sets.size(); ++i) {
+    p->more.push_back(PackDiagonals(sets[i], rowSize));
+    if (p->more.back().towers != p->diagonals.towers) throw std::runtime_error("fideslib: PrepareLinearTransformMany: sets at different levels");
+  }
+  return p;
+}
+
+std::vector<Ciphertext> CryptoContextImpl::LinearTransformMany(const Ciphertext& ct, const PreparedLinearTransform& prepared) {
+  if (!prepared) throw std::runtime_error("fideslib: LinearTransformMany: empty prepared transform");
+  std::vector<Ciphertext> out{LinearTransform(ct, prepared->bStep, prepared->diagonals, prepared->stride, prepared->offset)};
+  for (const auto& d : prepared->more) out.push_back(LinearTransform(ct, prepared->bStep, d, prepared->stride, prepared->offset));
+  return out;
+}
+
+std::vector<int> CryptoContextImpl::GetLinearTransformRotationIndices(int bStep, int stride, int offset) {
+  std::vector<int> r;
+  for (int i = 1; i <= bStep; ++i) r.push_back(i * stride);
+  if (offset != 0) r.push_back(offset);
+  return r;
+}
+
+std::vector<int> CryptoContextImpl::GetLinearTransformPlaintextRotationIndices(int rowSize, int bStep, int stride, int offset) {
+  std::vector<int> r(rowSize);
+  for (int k = 0; k < rowSize; ++k) r[k] = -bStep * (k / bStep) * stride - offset;
+  return r;
+}
+
+std::vector<Plaintext> CryptoContextImpl::MakeCKKSPackedPlaintextDeviceMany(const std::vector<std::vector<double>>& values, size_t noiseScaleDeg,
+                                                                           uint32_t level, uint32_t slots) {
+  std::vector<Plaintext> out;
+  if (values.empty()) return out;
+  if (level >= L1_) throw std::runtime_error("fideslib: level out of range");
+  const uint32_t n = slots ? slots : (params_.batch_ ? params_.batch_ : N_ / 2);
+  const uint32_t towers = L1_ - level;
+  const double scale = std::pow(scale_, (double)noiseScaleDeg);
+  // one batched device encode into a staging buffer, then one buffer per plaintext
+  std::vector<double> z(values.size() * 2 * (size_t)n, 0.0);
+  for (size_t b = 0; b < values.size(); ++b)
+    for (size_t i = 0; i < std::min<size_t>(values[b].size(), n); ++i) z[(b * n + i) * 2] = values[b][i];
+  std::vector<int32_t> pids(towers);
+  for (uint32_t i = 0; i < towers; ++i) pids[i] = (int32_t)i;
+  const size_t words = (size_t)towers * N_;
+  DeviceBuffer staging(native_, values.size() * words);
+  check(fh_encode_slots(ctx_, z.data(), (int)values.size(), (int)n, scale, pids.data(), (int)towers, staging.ptr, words));
+  for (size_t b = 0; b < values.size(); ++b) {
+    auto pt = std::make_shared<PlaintextImpl>();
+    pt->towers = towers;
+    pt->slots = n;
+    pt->noiseScaleDeg = (uint32_t)noiseScaleDeg;
+    pt->scalingFactor = scale;
+    pt->values.assign(n, 0.0);
+    for (size_t i = 0; i < std::min<size_t>(values[b].size(), n); ++i) pt->values[i] = values[b][i];
+    pt->data = std::make_shared<DeviceBuffer>(native_, words);
+    check(fh_copy_d2d(ctx_, pt->data->ptr, staging.ptr + b * words, words * 8));
+    out.push_back(pt);
+  }
+  return out;
+}
+
+Ciphertext CryptoContextImpl::Compress(const Ciphertext& ct, uint32_t towersLeft, size_t noiseScaleDeg) {
+  Ciphertext c = ct;
+  while (c->noiseScaleDeg > noiseScaleDeg && c->towers > 1) c = Rescale(c);
+  if (c->towers > towersLeft && towersLeft >= 1) c = LevelReduce(c, c->towers - towersLeft);
+  return c;
+}
+
 Ciphertext CryptoContextImpl::LinearTransform(const Ciphertext& ct0, int bStep, const PackedDiagonals& diags, int stride, int offset) {
   require_standard(*ct0, "LinearTransform");
   Ciphertext ct = ct0->noiseScaleDeg == 2 ? Rescale(ct0) : ct0;  // rescale a fresh product first (LinearTransform.cu:77-78)

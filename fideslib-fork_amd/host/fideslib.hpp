@@ -245,7 +245,8 @@ class CryptoContextImpl {
   // preparation; each call reads them in place
   struct PreparedLinearTransformImpl {
     int rowSize = 0, bStep = 0, stride = 1, offset = 0;
-    PackedDiagonals diagonals;
+    PackedDiagonals diagonals;                 // the first (or only) set
+    std::vector<PackedDiagonals> more;         // further sets of PrepareLinearTransformMany
   };
   using PreparedLinearTransform = std::shared_ptr<const PreparedLinearTransformImpl>;
   PreparedLinearTransform PrepareLinearTransform(const std::vector<Plaintext>& diagonals, int rowSize, int bStep, int stride = 1,
@@ -268,6 +269,51 @@ class CryptoContextImpl {
                           uint32_t slots = 0, uint32_t correctionFactor = 0);
   void EvalBootstrapKeyGen(const PrivateKey& sk, uint32_t slots);
   Ciphertext EvalBootstrap(const Ciphertext& ct, uint32_t numIterations = 1, uint32_t precision = 0);
+  // EvalBootstrap, then the level reduced to outputLevel (OpenFHE level: towers consumed)
+  Ciphertext EvalBootstrapToLevel(const Ciphertext& ct, uint32_t outputLevel, uint32_t numIterations = 1, uint32_t precision = 0);
+
+  // ---- in-place and convenience forms of the operations above (api/CryptoContext.hpp:300-456):
+  // each replaces its first argument with the result of the corresponding operation
+  void EvalAddInPlace(Ciphertext& a, const Ciphertext& b) { a = EvalAdd(a, b); }
+  void EvalAddInPlace(Ciphertext& a, const Plaintext& pt) { a = EvalAdd(a, pt); }
+  void EvalAddInPlace(Ciphertext& a, double scalar) { a = EvalAdd(a, scalar); }
+  void EvalSubInPlace(Ciphertext& a, const Ciphertext& b) { a = EvalSub(a, b); }
+  void EvalSubInPlace(Ciphertext& a, const Plaintext& pt) { a = EvalSub(a, pt); }
+  void EvalSubInPlace(Ciphertext& a, double scalar) { a = EvalSub(a, scalar); }
+  void EvalMultInPlace(Ciphertext& a, const Ciphertext& b) { a = EvalMult(a, b); }
+  void EvalMultInPlace(Ciphertext& a, const Plaintext& pt) { a = EvalMult(a, pt); }
+  void EvalMultInPlace(Ciphertext& a, double scalar) { a = EvalMult(a, scalar); }
+  void EvalNegateInPlace(Ciphertext& a) { a = EvalNegate(a); }
+  void EvalSquareInPlace(Ciphertext& a) { a = EvalSquare(a); }
+  void RelinearizeInPlace(Ciphertext& a) { a = Relinearize(a); }
+  void RescaleInPlace(Ciphertext& a) { a = Rescale(a); }
+  void EvalRotateInPlace(Ciphertext& a, int32_t index) { a = EvalRotate(a, index); }
+  void EvalChebyshevSeriesInPlace(Ciphertext& a, const std::vector<double>& coeffs, double lo, double hi) {
+    a = EvalChebyshevSeries(a, coeffs, lo, hi);
+  }
+  void EvalBootstrapInPlace(Ciphertext& a, uint32_t numIterations = 1, uint32_t precision = 0) { a = EvalBootstrap(a, numIterations, precision); }
+  void EvalBootstrapToLevelInPlace(Ciphertext& a, uint32_t outputLevel, uint32_t numIterations = 1, uint32_t precision = 0) {
+    a = EvalBootstrapToLevel(a, outputLevel, numIterations, precision);
+  }
+  // OpenFHE Compress: rescale down to noise degree noiseScaleDeg, then keep towersLeft towers
+  Ciphertext Compress(const Ciphertext& ct, uint32_t towersLeft = 1, size_t noiseScaleDeg = 1);
+  // the plaintexts are encoded on the device in any case; the Many form encodes the group in one
+  // batched call (api/CryptoContext.hpp:240-281)
+  Plaintext MakeCKKSPackedPlaintextDevice(const std::vector<double>& v, size_t noiseScaleDeg = 1, uint32_t level = 0, uint32_t slots = 0) {
+    return MakeCKKSPackedPlaintext(v, noiseScaleDeg, level, slots);
+  }
+  std::vector<Plaintext> MakeCKKSPackedPlaintextDeviceMany(const std::vector<std::vector<double>>& values, size_t noiseScaleDeg = 1,
+                                                           uint32_t level = 0, uint32_t slots = 0);
+  // rotation indices a BSGS transform needs keys for ({i stride : 1 <= i <= bStep}, and offset
+  // when nonzero) and the rotation each diagonal k is stored with (-bStep floor(k / bStep) stride
+  // - offset); api/CryptoContext.hpp:478-500
+  static std::vector<int> GetLinearTransformRotationIndices(int bStep, int stride = 1, int offset = 0);
+  static std::vector<int> GetLinearTransformPlaintextRotationIndices(int rowSize, int bStep, int stride = 1, int offset = 0);
+  // several diagonal sets applied to one ciphertext (api/CryptoContext.hpp:544-608); one transform
+  // per set here
+  PreparedLinearTransform PrepareLinearTransformMany(const std::vector<std::vector<Plaintext>>& diagonalSets, int rowSize, int bStep,
+                                                     int stride = 1, int offset = 0) const;
+  std::vector<Ciphertext> LinearTransformMany(const Ciphertext& ct, const PreparedLinearTransform& prepared);
   // levels one bootstrap consumes after the raise (prescale, CoeffsToSlots, EvalMod, SlotsToCoeffs)
   uint32_t GetBootstrapDepth() const;
   BootstrapConstants GetBootstrapConstants() const;

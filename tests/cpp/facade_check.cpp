@@ -205,6 +205,46 @@ int main(int argc, char** argv) {
     cc->LinearTransformInPlace(ltp, prepared);
     dump_ct(*cc, kp.secretKey, "lt_prepared", ltp, e);
 
+    // API forms: in-place operations, the batched device encoder, rotation-index helpers,
+    // LinearTransformMany, Compress (each compared with the plain form on the device words)
+    auto dev = [&](const uint64_t* p, size_t words) {
+      std::vector<uint64_t> h(words);
+      if (fh_copy_d2h(cc->Native(), h.data(), p, words * 8) != FH_OK) throw std::runtime_error(fh_last_error());
+      return h;
+    };
+    auto ctw = [&](const Ciphertext& c) { return dev(c->data->ptr, (size_t)c->ncomp * c->towers * N); };
+    int api_ok = 0, api_total = 0;
+    auto expect = [&](bool v) { ++api_total; api_ok += v ? 1 : 0; };
+    {
+      Ciphertext t = ct1;
+      cc->EvalAddInPlace(t, ct2);
+      expect(ctw(t) == ctw(cc->EvalAdd(ct1, ct2)));
+      t = ct1;
+      cc->EvalMultInPlace(t, ct2);
+      cc->RescaleInPlace(t);
+      expect(ctw(t) == ctw(cc->Rescale(cc->EvalMult(ct1, ct2))));
+      t = ct1;
+      cc->EvalRotateInPlace(t, 1);
+      expect(ctw(t) == ctw(cc->EvalRotate(ct1, 1)));
+      std::vector<double> r0(slots), r1(slots);
+      for (uint32_t i = 0; i < slots; ++i) r0[i] = x[i].real(), r1[i] = 0.5 * x[i].imag();
+      auto many = cc->MakeCKKSPackedPlaintextDeviceMany({r0, r1}, 1, 1);
+      auto one0 = cc->MakeCKKSPackedPlaintext(r0, 1, 1), one1 = cc->MakeCKKSPackedPlaintext(r1, 1, 1);
+      const size_t pw = (size_t)one0->towers * N;
+      expect(many.size() == 2 && many[0]->towers == one0->towers && dev(many[0]->data->ptr, pw) == dev(one0->data->ptr, pw) &&
+             dev(many[1]->data->ptr, pw) == dev(one1->data->ptr, pw));
+      expect(CryptoContextImpl::GetLinearTransformRotationIndices(2, 1, 0) == std::vector<int>{1, 2});
+      expect(CryptoContextImpl::GetLinearTransformPlaintextRotationIndices(4, 2, 1, 0) == std::vector<int>{0, 0, -2, -2});
+      auto pm = cc->PrepareLinearTransformMany({diags, diags}, rowSize, bStep, 1, 0);
+      auto lts = cc->LinearTransformMany(ct1, pm);
+      const auto ref = ctw(ltp);
+      expect(lts.size() == 2 && ctw(lts[0]) == ref && ctw(lts[1]) == ref);
+      auto cmp = cc->Compress(cc->EvalMult(ct1, ct2), 1);
+      expect(cmp->towers == 1 && cmp->noiseScaleDeg == 1);
+    }
+    entry_sep();
+    std::fprintf(g_manifest, "  \"api_forms\": [%d, %d]", api_ok, api_total);
+
     // scalar CRT encodings
     const double ops[] = {2.5, -0.75, 1e-9, 3.0e30, -7.25e25, 0.0};
     for (int i = 0; i < 6; ++i) dump_u64_list("em" + std::to_string(i), cc->ElemForEvalMult(cc->GetNumQ(), ops[i]));

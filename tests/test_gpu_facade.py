@@ -192,6 +192,14 @@ def test_scalar_encodings(session, recomputed, oracle_mod):
             assert [int(v) for v in man[f"ea{i}_{d}"]] == [int(v) for v in oracle_mod.elem_for_eval_add(q, c, sc, d)], (c, d)
 
 
+def test_api_forms(session):
+    """In-place forms, MakeCKKSPackedPlaintextDeviceMany, the rotation-index helpers,
+    LinearTransformMany and Compress agree with the plain forms on every device word."""
+    _, man = session
+    ok, total = man["api_forms"]
+    assert total == 8 and ok == total, man["api_forms"]
+
+
 def test_error_paths(session):
     _, man = session
     assert man["errors_thrown"] == 4
