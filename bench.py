@@ -52,6 +52,8 @@ def parse():
                     help="time the batched rotation step only (for profiler runs: every dispatch after set-up belongs to it)")
     ap.add_argument("--stream-only", action="store_true",
                     help="run the mixed-level op stream only and print its result (for profiler runs)")
+    ap.add_argument("--lt-only", action="store_true",
+                    help="run the BSGS linear-transform workload only and print its result (for profiler runs)")
     ap.add_argument("--traffic-file", type=str, default=os.path.join(ROOT, "profiles", "traffic_rotation_n65536_l25.txt"),
                     help="rocprofv3 FETCH_SIZE + WRITE_SIZE bytes per launch of each kernel family of this workload")
     ap.add_argument("--autotune-polys", type=int, default=32,
@@ -83,6 +85,10 @@ def main():
     import fides_hip as fh
 
     ctx = fh.Context(LOGN, L1, K, DNUM)
+    if a.lt_only:
+        print(json.dumps({"linear_transform": lt_workload(fh, ctx, a)}))
+        ctx.close()
+        return
     if a.stream_only:
         keys, gals = [], []
         kb = ctx.array((DNUM * 2 * (L1 + K), ctx.N))

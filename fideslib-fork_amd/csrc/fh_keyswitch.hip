@@ -179,6 +179,7 @@ struct KskParams {
   const uint64_t* pmod_sh;
   const uint64_t* own;      // optional: the key-switched polynomial in EVAL form (lq limbs); digit d's
   uint32_t alpha;           // own limbs are read from it instead of from the digits
+  int accumulate;           // 1: add the result to what out0 / out1 hold (after the automorphism)
 };
 
 // blockIdx.y = extended position. out0 = sum_d digit_d b_d (+ P c0), out1 = sum_d digit_d a_d,
@@ -216,6 +217,10 @@ __global__ __launch_bounds__(kT) void k_ksk_inner(const uint64_t* __restrict__ d
       // invert the gather map: find j with auto_src(j) = n. auto_src with k^-1 is the inverse map.
       j = auto_src(n, kp.galois, kp.logN);
     }
+    if (kp.accumulate) {
+      r0 = add_mod(out0[(size_t)pos * N + j], r0, P.q);
+      r1 = add_mod(out1[(size_t)pos * N + j], r1, P.q);
+    }
     out0[(size_t)pos * N + j] = r0;
     out1[(size_t)pos * N + j] = r1;
   }
@@ -251,6 +256,7 @@ struct KskBatchParams {
   const uint64_t* own;      // optional: source b's key-switched polynomial (EVAL, lq limbs) at
   uint64_t own_stride;      // own + b * own_stride; digit d's own limbs are read from it
   uint32_t alpha;
+  int accumulate;           // 1: add each result to what its output holds (after the automorphism)
 };
 
 template <class M>
@@ -327,6 +333,12 @@ __global__ __launch_bounds__(kT) void k_ksk_batch(const uint64_t* __restrict__ d
         s0 = add_mod(s0, c.y, P.q);
       }
       uint64_t* o = out + b * kp.out_stride;
+      if (kp.accumulate) {
+        r0 = add_mod(o[(size_t)pos * N + j0], r0, P.q);
+        r1 = add_mod(o[((size_t)kp.ext + pos) * N + j0], r1, P.q);
+        s0 = add_mod(o[(size_t)pos * N + j1], s0, P.q);
+        s1 = add_mod(o[((size_t)kp.ext + pos) * N + j1], s1, P.q);
+      }
       o[(size_t)pos * N + j0] = r0;
       o[((size_t)kp.ext + pos) * N + j0] = r1;
       o[(size_t)pos * N + j1] = s0;
@@ -544,10 +556,10 @@ void modup_batch(Context& c, const uint64_t* x, int l, int npolys, uint64_t x_st
 }
 
 void ksk_inner(Context& c, const uint64_t* digits, int l, const uint64_t* key, uint64_t* out0, uint64_t* out1,
-               const uint64_t* c0, uint32_t galois, hipStream_t st, bool c0_ext, const uint64_t* own) {
+               const uint64_t* c0, uint32_t galois, hipStream_t st, bool c0_ext, const uint64_t* own, bool accumulate) {
   const uint32_t lq = l + 1, ext = lq + c.K;
   KskParams kp{c.N, lq, ext, (uint32_t)c.L1, (uint32_t)c.K, (uint32_t)c.num_digits(l), (uint32_t)c.logN, galois,
-               c.d_prime, key, c0, c0_ext ? 1 : 0, c.d_pmod, c.d_pmod_sh, own, (uint32_t)c.alpha};
+               c.d_prime, key, c0, c0_ext ? 1 : 0, c.d_pmod, c.d_pmod_sh, own, (uint32_t)c.alpha, accumulate ? 1 : 0};
   if (galois != 1) {
     // the kernel scatters through the inverse map: pass k^-1 so that out[j] = in[src_k(j)]
     kp.galois = (uint32_t)h_inv_odd(galois, 2ull * c.N);
@@ -714,7 +726,7 @@ void eval_rotate(Context& c, const uint64_t* ct, int l, const uint64_t* key, uin
 // add_c0 folds c0: P c0 over Q_l (c0_ext false) or an extended c0 added on every limb.
 void ksk_items(Context& c, int nb, int per_src, const uint64_t* digits, uint64_t dig_stride, const uint64_t* c0s, uint64_t c0_stride,
                bool add_c0, bool c0_ext, int l, const uint64_t* const* keys, const uint32_t* galois, uint64_t* eo, uint64_t out_stride,
-               hipStream_t st, const uint64_t* own, uint64_t own_stride) {
+               hipStream_t st, const uint64_t* own, uint64_t own_stride, bool accumulate) {
   const uint32_t lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);
   if (beta > (uint32_t)kMaxDnum) throw Error(-3, "dnum above kernel limit");
   if (per_src < 1) throw Error(-1, "per_src must be >= 1");
@@ -727,7 +739,7 @@ void ksk_items(Context& c, int nb, int per_src, const uint64_t* digits, uint64_t
   });
   KskBatchParams kp{c.N, lq, ext, (uint32_t)c.L1, (uint32_t)c.K, beta, (uint32_t)c.logN, c0_stride, dig_stride, out_stride,
                     (uint32_t)per_src, add_c0 ? 1 : 0, c0_ext ? 1 : 0, c.d_prime, c0s, c.d_pmod, c.d_pmod_sh, own, own_stride,
-                    (uint32_t)c.alpha};
+                    (uint32_t)c.alpha, accumulate ? 1 : 0};
   int i = 0;
   while (i < nb) {
     RotGroups g{};

@@ -22,8 +22,10 @@ void rotate_hoisted(Context& c, const uint64_t* ct, int l, int n, const uint64_t
 // galois != 1 writes both outputs through the automorphism X -> X^galois.
 // c0_ext: c0 is already P-scaled over Q_l ∪ P (l+1+K limbs) and is added on every limb.
 // own (optional): the ModUp input (EVAL, l+1 limbs), read for each digit's own limbs.
+// accumulate: out0/out1 += the result instead of out0/out1 = the result.
 void ksk_inner(Context& c, const uint64_t* digits, int l, const uint64_t* key, uint64_t* out0, uint64_t* out1,
-               const uint64_t* c0, uint32_t galois, hipStream_t st, bool c0_ext = false, const uint64_t* own = nullptr);
+               const uint64_t* c0, uint32_t galois, hipStream_t st, bool c0_ext = false, const uint64_t* own = nullptr,
+               bool accumulate = false);
 // BSGS linear transform in the extended-basis formulation (fh_lt.hip); keys[i] for the baby step
 // i*stride (null where i*stride = 0 mod slots), giant/offset keys may be null when unused.
 void linear_transform(Context& c, const uint64_t* ct, int l, int rowSize, int bStep, int stride, int offset, int slots,
@@ -33,10 +35,10 @@ void linear_transform(Context& c, const uint64_t* ct, int l, int rowSize, int bS
 // apart) and c0 (c0_stride apart) of source b / per_src and writes 2 x (l+1+K) limbs at
 // eo + b * out_stride. add_c0 folds c0 as P c0 over Q_l, or (c0_ext) as given over Q_l ∪ P.
 // own (optional): source s's ModUp input (EVAL, l+1 limbs) at own + s * own_stride, read for each
-// digit's own limbs (digits built with copy_own = false).
+// digit's own limbs (digits built with copy_own = false). accumulate: each output += its result.
 void ksk_items(Context& c, int nb, int per_src, const uint64_t* digits, uint64_t dig_stride, const uint64_t* c0s, uint64_t c0_stride,
                bool add_c0, bool c0_ext, int l, const uint64_t* const* keys, const uint32_t* galois, uint64_t* eo, uint64_t out_stride,
-               hipStream_t st, const uint64_t* own = nullptr, uint64_t own_stride = 0);
+               hipStream_t st, const uint64_t* own = nullptr, uint64_t own_stride = 0, bool accumulate = false);
 // count linear transforms of ciphertexts at level l (ct_stride words apart, 0 = packed) sharing one
 // diagonal set and key set; outputs packed (2 x (l+1) limbs each). Keys and diagonals are read once
 // per batch instead of once per ciphertext.

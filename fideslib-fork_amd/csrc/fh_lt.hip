@@ -206,23 +206,20 @@ void linear_transform(Context& c, const uint64_t* ct, int l, int rowSize, int bS
   // ---- Horner over the giant steps ----
   std::vector<int> epids(ext);
   for (uint32_t p = 0; p < ext; ++p) epids[p] = c.ext_prime(l, p);
-  DBuf acc(c.pool, 2 * E), c1q(c.pool, (size_t)lq * N), dig(c.pool, (size_t)beta * E);
-  bool have_acc = false;
+  // R_{j-1} += rot_giant(R_j) for j = gStep-1 .. 1: the key switch adds its result into R_{j-1}.
+  DBuf c1q(c.pool, (size_t)lq * N), dig(c.pool, (size_t)beta * E);
   for (int j = gStep - 1; j >= 0; --j) {
     uint64_t* Rj = R.p + (size_t)j * 2 * E;
-    if (have_acc) {
-      LimbSet a = make_set(Rj, ext, epids.data(), 2, E), b = make_set(acc.p, ext, epids.data(), 2, E);
-      ew_binary(c, 0, a, b, a, st);
-    }
     if (j > 0) {
+      uint64_t* Rp = Rj - 2 * E;
       if (giant) {
         moddown(c, Rj + E, l, c1q.p, 1, 0, 0, st);
         modup(c, c1q.p, l, dig.p, st);
-        ksk_inner(c, dig.p, l, giant_key, acc.p, acc.p + E, Rj, giant_gal, st, true);
+        ksk_inner(c, dig.p, l, giant_key, Rp, Rp + E, Rj, giant_gal, st, true, nullptr, true);
       } else {
-        FH_HIP(hipMemcpyAsync(acc.p, Rj, 2 * E * 8, hipMemcpyDeviceToDevice, st));
+        LimbSet a = make_set(Rp, ext, epids.data(), 2, E), b = make_set(Rj, ext, epids.data(), 2, E);
+        ew_binary(c, 0, a, b, a, st);
       }
-      have_acc = true;
     } else if (offset != 0) {
       moddown(c, Rj + E, l, c1q.p, 1, 0, 0, st);
       modup(c, c1q.p, l, dig.p, st);
@@ -311,27 +308,24 @@ void linear_transform_batch(Context& c, int count, const uint64_t* cts, uint64_t
   std::vector<int> epids(ext);
   for (uint32_t p = 0; p < ext; ++p) epids[p] = c.ext_prime(l, p);
   const uint64_t rstride = (uint64_t)gStep * 2 * E;  // between ciphertexts in R
-  DBuf acc(c.pool, (size_t)B * 2 * E), c1q(c.pool, (size_t)B * lq * N);
+  DBuf c1q(c.pool, (size_t)B * lq * N);
   std::vector<const uint64_t*> gk(B, giant_key), okv(B, off_key);
   std::vector<uint32_t> gg(B, giant_gal), og(B, off_gal);
-  bool have_acc = false;
   for (int j = gStep - 1; j >= 0; --j) {
     uint64_t* Rj = R.p + (size_t)j * 2 * E;
-    if (have_acc) {
-      for (int comp = 0; comp < 2; ++comp) {
-        LimbSet a = make_set(Rj + comp * E, ext, epids.data(), B, rstride), b = make_set(acc.p + comp * E, ext, epids.data(), B, 2 * E);
-        ew_binary(c, 0, a, b, a, st);
-      }
-    }
     if (j > 0) {
+      uint64_t* Rp = Rj - 2 * E;  // every ciphertext's R_{j-1} += rot_giant(R_j)
       if (giant) {
         moddown(c, Rj + E, l, c1q.p, (int)B, rstride, (uint64_t)lq * N, st);
         modup_batch(c, c1q.p, l, (int)B, (uint64_t)lq * N, dig.p, st);
-        ksk_items(c, (int)B, 1, dig.p, (uint64_t)beta * E, Rj, rstride, true, true, l, gk.data(), gg.data(), acc.p, 2 * E, st);
+        ksk_items(c, (int)B, 1, dig.p, (uint64_t)beta * E, Rj, rstride, true, true, l, gk.data(), gg.data(), Rp, rstride, st,
+                  nullptr, 0, true);
       } else {
-        FH_HIP(hipMemcpy2DAsync(acc.p, 2 * E * 8, Rj, rstride * 8, 2 * E * 8, B, hipMemcpyDeviceToDevice, st));
+        for (int comp = 0; comp < 2; ++comp) {
+          LimbSet a = make_set(Rp + comp * E, ext, epids.data(), B, rstride), b = make_set(Rj + comp * E, ext, epids.data(), B, rstride);
+          ew_binary(c, 0, a, b, a, st);
+        }
       }
-      have_acc = true;
     } else if (offset != 0) {
       moddown(c, Rj + E, l, c1q.p, (int)B, rstride, (uint64_t)lq * N, st);
       modup_batch(c, c1q.p, l, (int)B, (uint64_t)lq * N, dig.p, st);
