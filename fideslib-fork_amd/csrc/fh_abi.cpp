@@ -537,6 +537,54 @@ int fh_linear_transform(fh_context* ctx, const uint64_t* ct, int level, int row_
   });
 }
 
+namespace {
+// fold rotation indices; throws on an invalid radix or stride
+std::vector<int32_t> fold_indices(int radix, int stride, int size, int start) {
+  if (radix < 2 || radix > 64 || (radix & (radix - 1))) throw fh::Error(FH_E_INVALID, "radix must be a power of two in [2, 64]");
+  if (stride < 1) throw fh::Error(FH_E_INVALID, "stride must be >= 1");
+  std::vector<int32_t> v;
+  if (start <= 0 || size <= 0) return v;
+  for (long long s = start; s < size; s *= radix)
+    for (long long i = 1; i < radix && i * s < size; ++i) {
+      const long long idx = i * s * stride;
+      if (idx > INT32_MAX) throw fh::Error(FH_E_INVALID, "rotation index out of range");
+      v.push_back((int32_t)idx);
+    }
+  return v;
+}
+}  // namespace
+
+int fh_accumulate_indices(int radix, int stride, int size, int start, int32_t* out, int capacity) {
+  int n = 0;
+  int rc = guard([&] {
+    auto v = fold_indices(radix, stride, size, start);
+    if (out)
+      for (size_t i = 0; i < v.size() && (int)i < capacity; ++i) out[i] = v[i];
+    n = (int)v.size();
+  });
+  return rc != FH_OK ? rc : n;
+}
+
+int fh_accumulate_sum(fh_context* ctx, const uint64_t* ct, int level, int radix, int stride, int size, int start,
+                      const fh_key* const* keys, const uint32_t* galois, int num_keys, uint64_t* out) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(ct, "ct"); need(out, "out");
+    check_level(c, level);
+    const auto idx = fold_indices(radix, stride, size, start);
+    if (num_keys != (int)idx.size()) throw fh::Error(FH_E_INVALID, "num_keys must equal the fold's rotation count");
+    std::vector<const uint64_t*> kp(idx.size());
+    std::vector<uint32_t> gl(idx.size());
+    for (size_t i = 0; i < idx.size(); ++i) {
+      need(keys, "keys"); need(galois, "galois");
+      kp[i] = key_of(c, keys[i]);
+      gl[i] = galois[i];
+      if ((gl[i] & 1) == 0 || gl[i] < 3 || gl[i] >= 2 * c.N) throw fh::Error(FH_E_INVALID, "galois must be odd, not 1, and < 2N");
+    }
+    fh::accumulate_sum(c, ct, level, radix, size, start, kp.data(), gl.data(), out, c.stream);
+  });
+}
+
 int fh_linear_transform_batch(fh_context* ctx, int count, const uint64_t* cts, size_t ct_stride, int level, int row_size, int b_step,
                               int stride, int offset, int slots, const uint64_t* diags, const fh_key* const* baby_keys,
                               const uint32_t* baby_galois, const fh_key* giant_key, uint32_t giant_galois, const fh_key* offset_key,

@@ -339,4 +339,37 @@ void linear_transform_batch(Context& c, int count, const uint64_t* cts, uint64_t
   }
 }
 
+// Rotation-fold sum (CKKS::Accumulate, src/CKKS/AccumulateBroadcast.cu:42-84): per fold level s =
+// start, start*radix, ... < size, the rotations by i*stride*s (0 < i < radix, i*s < size) share one
+// ModUp and stay in the extended basis with P*c0 folded in, are added to P*ct, and each component
+// takes one ModDown. keys / galois: one per rotation in fold order.
+void accumulate_sum(Context& c, const uint64_t* ct, int l, int radix, int size, int start, const uint64_t* const* keys,
+                    const uint32_t* galois, uint64_t* out, hipStream_t st) {
+  const uint32_t N = c.N, lq = l + 1, ext = lq + c.K;
+  const size_t E = (size_t)ext * N, Q = (size_t)lq * N;
+  if (out != ct) FH_HIP(hipMemcpyAsync(out, ct, 2 * Q * 8, hipMemcpyDeviceToDevice, st));
+  if (start <= 0 || size <= 0 || start >= size) return;
+  std::vector<int> epids(ext);
+  for (uint32_t p = 0; p < ext; ++p) epids[p] = c.ext_prime(l, p);
+  DBuf fast(c.pool, (size_t)radix * 2 * E);
+  int used = 0;
+  for (long long s = start; s < size; s *= radix) {
+    int n = 0;
+    while (n + 1 < radix && (long long)(n + 1) * s < size) ++n;
+    {
+      ProfScope ps_(c, "lt_extend", st);
+      k_extend_p<<<grid_n(N, 2 * ext), kT, 0, st>>>(out, fast.p, N, lq, ext, c.d_prime, c.d_pmod, c.d_pmod_sh);
+      FH_HIP(hipGetLastError());
+    }
+    rotate_hoisted(c, out, l, n, keys + used, galois + used, true, false, fast.p + 2 * E, st);
+    LimbSet a = make_set(fast.p, ext, epids.data(), 2, E);
+    for (int i = 1; i <= n; ++i) {
+      LimbSet b = make_set(fast.p + (size_t)i * 2 * E, ext, epids.data(), 2, E);
+      ew_binary(c, 0, a, b, a, st);
+    }
+    moddown(c, fast.p, l, out, 2, 0, 0, st);
+    used += n;
+  }
+}
+
 }  // namespace fh

@@ -93,6 +93,8 @@ def _declare(L):
         "fh_serve_run": (I, [P, C.POINTER(ServeConfig), C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), P, C.POINTER(ServeResult)]),
         "fh_linear_transform_batch": (I, [P, I, VP, S, I, I, I, I, I, I, VP, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), P,
                                           C.c_uint32, P, C.c_uint32, VP]),
+        "fh_accumulate_sum": (I, [P, VP, I, I, I, I, I, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), I, VP]),
+        "fh_accumulate_indices": (I, [I, I, I, I, C.POINTER(C.c_int32), I]),
         "fh_tune_set": (I, [P, C.c_char_p, I]),
         "fh_tune_get": (I, [P, C.c_char_p, C.POINTER(C.c_int)]),
         "fh_autotune": (I, [P, I, C.POINTER(AutotuneReport)]),
@@ -349,6 +351,22 @@ class Context:
         _check(load().fh_linear_transform(self.h, ct.ptr, level, row_size, b_step, stride, offset, slots, diags.ptr, bk, bg,
                                           giant_key.h if giant_key is not None else None, giant_galois,
                                           offset_key.h if offset_key is not None else None, offset_galois, out.ptr))
+
+    @staticmethod
+    def accumulate_indices(radix, stride, size, start=1):
+        n = load().fh_accumulate_indices(radix, stride, size, start, None, 0)
+        if n < 0:
+            _check(n)
+        buf = (C.c_int32 * max(n, 1))()
+        _check(min(0, load().fh_accumulate_indices(radix, stride, size, start, buf, n)))
+        return list(buf[:n])
+
+    def accumulate_sum(self, ct, level, radix, stride, size, keys, galois, out, start=1):
+        """keys / galois: one per index of accumulate_indices(radix, stride, size, start)."""
+        n = len(keys)
+        ka = (C.c_void_p * max(n, 1))(*[k.h for k in keys])
+        ga = (C.c_uint32 * max(n, 1))(*[int(g) for g in galois])
+        _check(load().fh_accumulate_sum(self.h, ct.ptr, level, radix, stride, size, start, ka, ga, n, out.ptr))
 
     def linear_transform_batch(self, cts, count, level, row_size, b_step, stride, diags, baby_keys, baby_galois, out, giant_key=None,
                                giant_galois=1, offset_key=None, offset_galois=1, offset=0, slots=None, ct_stride=0):

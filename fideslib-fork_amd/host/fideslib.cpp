@@ -931,6 +931,42 @@ Ciphertext CryptoContextImpl::LinearTransform(const Ciphertext& ct, const Prepar
 
 void CryptoContextImpl::LinearTransformInPlace(Ciphertext& ct, const PreparedLinearTransform& prepared) { ct = LinearTransform(ct, prepared); }
 
+// ---- rotation-fold sum ----
+std::vector<int32_t> CryptoContextImpl::GetAccumulateSumRotationIndices(int slots, int stride, int start) {
+  const int n = fh_accumulate_indices(kAccumulateSumRadix, stride, slots, start, nullptr, 0);
+  if (n < 0) throw std::runtime_error(std::string("fideslib: ") + fh_last_error());
+  std::vector<int32_t> v(n);
+  if (n) fh_accumulate_indices(kAccumulateSumRadix, stride, slots, start, v.data(), n);
+  return v;
+}
+
+Ciphertext CryptoContextImpl::accumulate(const Ciphertext& ct, int slots, int stride, int start) {
+  require_standard(*ct, "AccumulateSum");
+  if (ct->ncomp != 2) throw std::runtime_error("fideslib: AccumulateSum: ciphertext should be relinearized before");
+  const auto idx = GetAccumulateSumRotationIndices(slots, stride, start);
+  std::vector<const fh_key*> keys(idx.size());
+  std::vector<uint32_t> gal(idx.size());
+  for (size_t i = 0; i < idx.size(); ++i) {
+    gal[i] = galoisFor(idx[i]);
+    keys[i] = rotKey(gal[i]);
+  }
+  auto out = newCt(ct->towers, 2);
+  check(fh_accumulate_sum(ctx_, ct->data->ptr, (int)ct->towers - 1, kAccumulateSumRadix, stride, slots, start, keys.data(), gal.data(),
+                          (int)idx.size(), out->data->ptr));
+  out->noiseScaleDeg = ct->noiseScaleDeg;
+  out->scalingFactor = ct->scalingFactor;
+  out->slots = ct->slots;
+  return out;
+}
+
+Ciphertext CryptoContextImpl::AccumulateSum(const Ciphertext& ct, int slots, int stride) { return accumulate(ct, slots, stride, 1); }
+void CryptoContextImpl::AccumulateSumInPlace(Ciphertext& ct, int slots, int stride) { ct = accumulate(ct, slots, stride, 1); }
+void CryptoContextImpl::AccumulateSumInPlace(Ciphertext& ct, int slots, int stride, int start) {
+  const uint32_t before = ct->slots;
+  ct = accumulate(ct, slots, stride, start);
+  if (start > 0 && slots > 0 && (long long)slots * stride == before) ct->slots = (uint32_t)(stride * start);
+}
+
 CryptoContextImpl::PreparedLinearTransform CryptoContextImpl::PrepareLinearTransformMany(const std::vector<std::vector<Plaintext>>& sets,
                                                                                          int rowSize, int bStep, int stride, int offset) const {
   if (sets.empty()) throw std::runtime_error("fideslib: PrepareLinearTransformMany needs at least one diagonal set");

@@ -254,6 +254,18 @@ class CryptoContextImpl {
   Ciphertext LinearTransform(const Ciphertext& ct, const PreparedLinearTransform& prepared);
   void LinearTransformInPlace(Ciphertext& ct, const PreparedLinearTransform& prepared);
 
+  // ---- rotation-fold sum (AccumulateSum, api/CryptoContext.hpp:456-458; CudaEngine::accumulateSum,
+  // api/engine/cuda/CudaEngine.cpp:755-782): slot j of the result is the sum over i < slots/start
+  // of slot j + i*start*stride, folded at radix kAccumulateSumRadix with one ModDown per fold
+  // level. The two- and three-argument forms keep the slot count; the start form sets it to
+  // stride*start when slots*stride equals it (src/CKKS/AccumulateBroadcast.cu:82-83). Needs the
+  // rotation keys GetAccumulateSumRotationIndices lists.
+  static constexpr int kAccumulateSumRadix = 4;
+  Ciphertext AccumulateSum(const Ciphertext& ct, int slots, int stride = 1);
+  void AccumulateSumInPlace(Ciphertext& ct, int slots, int stride = 1);
+  void AccumulateSumInPlace(Ciphertext& ct, int slots, int stride, int start);
+  static std::vector<int32_t> GetAccumulateSumRotationIndices(int slots, int stride = 1, int start = 1);
+
   // ---- polynomial evaluation (OpenFHE EvalChebyshevSeries / EvalChebyshevCoefficients):
   // c_0 / 2 + sum_{k>=1} c_k T_k(y), y = (2x - a - b) / (b - a); the coefficients of f on [a, b]
   // at degree d are c_k = 2/(d+1) sum_j f(x_j) cos(pi k (j + 1/2) / (d+1)).
@@ -355,6 +367,7 @@ class CryptoContextImpl {
   std::vector<int64_t> gaussian();
   fh_key* switchKey(const DeviceBuffer& s_old, const DeviceBuffer& s_new);
   const fh_key* rotKey(uint32_t galois) const;
+  Ciphertext accumulate(const Ciphertext& ct, int slots, int stride, int start);
   uint32_t galoisFor(int32_t index) const;
   void automorphismKey(const PrivateKey& sk, uint32_t galois);
 

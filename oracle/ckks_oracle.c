@@ -896,6 +896,47 @@ void orc_linear_transform(const orc_ctx* c, const u64* ct, int l, int rowSize, i
   free(fast); free(acc); free(R); free(c1q);
 }
 
+/* Rotation-fold sum at a power-of-two radix (CKKS::Accumulate, src/CKKS/AccumulateBroadcast.cu:42-84,
+   the lazy fold of BITCOMPAT.md:69-80): for s = start, start*radix, ... < size, the rotations by
+   i*stride*s (0 < i < radix, i*s < size) of the current ciphertext are formed in the extended basis
+   (one shared decomposition, P*c0 folded in), added to P*ct, and each component takes ONE ModDown.
+   keys / ks: one key and Galois element per rotation, in fold order. out may alias ct. */
+void orc_accumulate_sum(const orc_ctx* c, const u64* ct, int l, int radix, int stride, int size, int start,
+                        const u64* keys, const uint32_t* ks, u64* out) {
+  u64 N = c->N;
+  int ext = l + 1 + c->K;
+  size_t E = (size_t)ext * N, Q = (size_t)(l + 1) * N;
+  size_t key_words = (size_t)c->dnum * 2 * c->np * N;
+  (void)stride;
+  if (out != ct) memcpy(out, ct, 2 * Q * 8);
+  if (start <= 0 || size <= 0) return;
+  u64* acc = (u64*)malloc(2 * E * 8);
+  u64* t = (u64*)malloc(2 * E * 8);
+  int used = 0;
+  for (long long s = start; s < size; s *= radix) {
+    memset(acc, 0, 2 * E * 8);
+    for (int comp = 0; comp < 2; ++comp)
+      for (int p = 0; p <= l; ++p)
+        for (u64 j = 0; j < N; ++j)
+          acc[comp * E + (size_t)p * N + j] = orc_mulmod(out[comp * Q + (size_t)p * N + j], c->Pmod[p], c->q[p]);
+    for (long long i = 1; i < radix && i * s < size; ++i, ++used) {
+      rotate_core(c, out, 1, out + Q, l, keys + (size_t)used * key_words, ks[used], t);
+      for (int comp = 0; comp < 2; ++comp)
+#pragma omp parallel for
+        for (int pos = 0; pos < ext; ++pos) {
+          u64 q = c->q[ext_prime(c, l, pos)];
+          for (u64 n = 0; n < N; ++n) {
+            size_t x = comp * E + (size_t)pos * N + n;
+            acc[x] = modadd(acc[x], t[x], q);
+          }
+        }
+    }
+    orc_moddown(c, acc, l, out);
+    orc_moddown(c, acc + E, l, out + Q);
+  }
+  free(acc); free(t);
+}
+
 void orc_eval_rotate(const orc_ctx* c, const u64* ct, int l, const u64* key, uint32_t k, u64* out) {
   u64 N = c->N;
   int ext = l + 1 + c->K;

@@ -25,8 +25,10 @@
  *   Rescale               pinned: OpenFHE's QlQlInvModqlDivqlModq by the big-int floor formula,
  *                         output by CRT
  *   automorphism          pinned: coefficient-domain a(X^k) mod X^N + 1
- *   key switching, rotation, relinearization, linear transform, encryption, ModRaise
- *                         pinned by decryption invariants and closed forms
+ *   key switching, rotation, relinearization, linear transform, encryption, ModRaise,
+ *   rotation-fold sum     pinned by decryption invariants and closed forms (the fold sum: its
+ *                         decryption is the sum of the slot rotations, and each fold level equals
+ *                         P*ct plus the extended rotations under one ModDown)
  *   prime chain, roots    pinned to OpenFHE's published FirstPrime / PreviousPrime / root rules by
  *                         an independent Python walk and brute force; parity unpinned against
  *                         OpenFHE itself (not available here)
@@ -130,6 +132,9 @@ void orc_eval_rotate(const orc_ctx* c, const uint64_t* ct, int l, const uint64_t
 void orc_eval_rotate_ext(const orc_ctx* c, const uint64_t* ct, int l, const uint64_t* key, uint32_t k, int add_c0, uint64_t* out_ext);
 
 /* BSGS linear transform in the reference's extended-basis (hoisted) formulation; see the .c file */
+/* Rotation-fold sum (lazy, one ModDown per fold level); keys / ks one per rotation in fold order. */
+void orc_accumulate_sum(const orc_ctx* c, const uint64_t* ct, int l, int radix, int stride, int size, int start,
+                        const uint64_t* keys, const uint32_t* ks, uint64_t* out);
 void orc_linear_transform(const orc_ctx* c, const uint64_t* ct, int l, int rowSize, int bStep, int stride, int offset, int slots,
                           const uint64_t* diags, const uint64_t* baby_keys, const uint32_t* baby_k, const uint64_t* giant_key,
                           uint32_t giant_k, const uint64_t* off_key, uint32_t off_k, uint64_t* out);

@@ -81,6 +81,8 @@ def lib():
             "orc_eval_rotate_ext": (V, [P, u64p, I, u64p, U32, I, u64p]),
             "orc_linear_transform": (V, [P, u64p, I, I, I, I, I, I, u64p, u64p, np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"),
                                          u64p, U32, u64p, U32, u64p]),
+            "orc_accumulate_sum": (V, [P, u64p, I, I, I, I, I, u64p, np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"),
+                                       u64p]),
             "orc_gen_secret": (V, [P, U, I, u64p]),
             "orc_gen_switch_key": (V, [P, u64p, u64p, U, u64p]),
             "orc_gen_relin_key": (V, [P, u64p, U, u64p]),
@@ -337,6 +339,30 @@ class Ctx:
         out = self.zeros(l + 1, 2)
         lib().orc_linear_transform(self.h, np.ascontiguousarray(ct), l, rowSize, bStep, stride, offset, slots,
                                    np.ascontiguousarray(diags), bk, bkk, gk, giant_k, ok, off_k, out)
+        return out
+
+    @staticmethod
+    def accumulate_indices(radix, stride, size, start=1):
+        """Rotation indices of the fold, in fold order (GetAccumulateRotationIndices,
+        src/CKKS/AccumulateBroadcast.cu:86-96, from start instead of 1)."""
+        out, s = [], start
+        while start > 0 and s < size:
+            i = 1
+            while i < radix and i * s < size:
+                out.append(i * s * stride)
+                i += 1
+            s *= radix
+        return out
+
+    def accumulate_sum(self, ct, l, radix, stride, size, keys, ks, start=1):
+        """keys / ks: one rotation key and Galois element per index of accumulate_indices."""
+        n = len(self.accumulate_indices(radix, stride, size, start))
+        if len(keys) != n or len(ks) != n:
+            raise ValueError("one key per fold rotation")
+        kk = np.ascontiguousarray(np.concatenate(keys)) if n else np.zeros(self.key_shape(), dtype=np.uint64)
+        kx = np.ascontiguousarray(np.array([int(x) for x in ks] or [1], dtype=np.uint32))
+        out = self.zeros(l + 1, 2)
+        lib().orc_accumulate_sum(self.h, np.ascontiguousarray(ct), l, radix, stride, size, start, kk, kx, out)
         return out
 
     # ---- keys / encoding / encryption ----

@@ -173,6 +173,12 @@ int main(int argc, char** argv) {
 
     dump_ct(*cc, kp.secretKey, "rot1", cc->EvalRotate(ct1, 1), rotate(x, 1));
     dump_ct(*cc, kp.secretKey, "rotm3", cc->EvalRotate(ct2, -3), rotate(y, -3));
+    {
+      // rotation-fold sum over 3 summands (rotations by 1 and 2 at radix 4)
+      const cvec r1 = rotate(x, 1), r2 = rotate(x, 2);
+      for (uint32_t i = 0; i < slots; ++i) e[i] = x[i] + r1[i] + r2[i];
+      dump_ct(*cc, kp.secretKey, "acc3", cc->AccumulateSum(ct1, 3), e);
+    }
     for (uint32_t i = 0; i < slots; ++i) e[i] = std::conj(x[i]);
     dump_ct(*cc, kp.secretKey, "conj", cc->EvalConjugate(ct1), e);
     auto pre = cc->EvalFastRotationPrecompute(ct1);

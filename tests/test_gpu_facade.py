@@ -145,6 +145,7 @@ def test_rotations(session, recomputed):
     _check(session, "rot1", o.eval_rotate(c1, l, R["rot"][g(1)], g(1)), 2, t, 1, sc, 1e-6)
     _check(session, "rotm3", o.eval_rotate(c2, l, R["rot"][g(-3)], g(-3)), 2, t, 1, sc, 1e-6)
     _check(session, "conj", o.eval_rotate(c1, l, R["rot"][2 * N - 1], 2 * N - 1), 2, t, 1, sc, 1e-6)
+    _check(session, "acc3", o.accumulate_sum(c1, l, 4, 1, 3, [R["rot"][g(1)], R["rot"][g(2)]], [g(1), g(2)]), 2, t, 1, sc, 1e-6)
     _check(session, "fast7", o.eval_rotate(c1, l, R["rot"][g(7)], g(7)), 2, t, 1, sc, 1e-6)
     # EvalFastRotationExt + KeySwitchDown from the same precomputed digits
     _check(session, "fastext7", o.eval_rotate(c1, l, R["rot"][g(7)], g(7)), 2, t, 1, sc, 1e-6)

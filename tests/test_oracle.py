@@ -539,3 +539,52 @@ def test_context_rejects_inconsistent_prime_count():
     primes = O.gen_primes(10, 4, 60, 50, 2)
     with pytest.raises(ValueError):
         O.Ctx(10, primes, 4, 3, 2)   # 6 primes cannot be L1=4 plus K=3
+
+
+def test_accumulate_sum_invariants():
+    """The rotation-fold sum decrypts to sum_k roll(z, -k*start*stride) over the fold's offsets, and a
+    one-level fold equals ModDown(P*ct + the extended rotations): one ModDown of the sum, not one per
+    rotation (the lazy fold of the reference's Accumulate)."""
+    c = small_ctx(13, 4, 2, 2)
+    n, l, Dl = c.N // 2, 3, 2.0 ** 50
+    s = c.gen_secret(21)
+    z = _slots(n, 22)
+    ct = c.encrypt(c.encode(z, Dl, l), l, s, 23)
+    cases = [((4, 1, 16, 1), range(16)), ((2, 3, 8, 1), range(0, 24, 3)), ((4, 2, 8, 1), range(0, 16, 2)),
+             ((4, 1, 16, 2), range(0, 16, 2)), ((8, 1, 8, 1), range(8)), ((4, 1, 9, 1), range(12))]
+    for (radix, stride, size, start), offsets in cases:
+        idx = c.accumulate_indices(radix, stride, size, start)
+        ks = [c.rot_index(r) for r in idx]
+        keys = [c.gen_rot_key(s, k, 300 + i) for i, k in enumerate(ks)]
+        out = c.accumulate_sum(ct, l, radix, stride, size, keys, ks, start)
+        dec = c.decode(c.decrypt(out, l, s), l, n, Dl)
+        exp = sum(np.roll(z, -k) for k in offsets)
+        assert np.max(np.abs(dec - exp)) < 1e-7, (radix, stride, size, start)
+    assert c.accumulate_indices(4, 1, 16) == [1, 2, 3, 4, 8, 12]
+    assert c.accumulate_indices(4, 1, 9) == [1, 2, 3, 4, 8]
+    assert c.accumulate_indices(4, 2, 16, 2) == [4, 8, 12, 16]
+    # one fold level by composition: P*ct over Q_l (0 over P) plus the extended rotations, one ModDown
+    idx = c.accumulate_indices(4, 1, 4)
+    ks = [c.rot_index(r) for r in idx]
+    keys = [c.gen_rot_key(s, k, 400 + i) for i, k in enumerate(ks)]
+    ext = l + 1 + c.K
+    pids = c.ext_pids(l)
+    P = 1
+    for k in range(c.K):
+        P *= c.q(c.L1 + k)
+    acc = np.zeros((2 * ext, c.N), dtype=np.uint64)
+    for comp in range(2):
+        for p in range(l + 1):
+            q = c.q(p)
+            acc[comp * ext + p] = np.array([int(v) * (P % q) % q for v in ct[comp * (l + 1) + p]], dtype=np.uint64)
+    for k, key in zip(ks, keys):
+        r = c.eval_rotate_ext(ct, l, key, k)
+        for row in range(2 * ext):
+            q = np.uint64(c.q(pids[row % ext]))
+            acc[row] = (acc[row] + r[row]) % q
+    lazy = np.concatenate([c.moddown(acc[:ext], l), c.moddown(acc[ext:], l)])
+    assert np.array_equal(c.accumulate_sum(ct, l, 4, 1, 4, keys, ks), lazy)
+    eager = ct.copy()
+    for k, key in zip(ks, keys):
+        eager = c.add(eager, c.eval_rotate(ct, l, key, k))
+    assert not np.array_equal(eager, lazy)       # the per-rotation ModDown rounds differently
