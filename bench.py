@@ -86,7 +86,7 @@ def main():
 
     ctx = fh.Context(LOGN, L1, K, DNUM)
     if a.lt_only:
-        print(json.dumps({"linear_transform": lt_workload(fh, ctx, a)}))
+        print(json.dumps({"linear_transform": lt_workload(fh, ctx, a), "accumulate": accumulate_run(fh, ctx, a)}))
         ctx.close()
         return
     if a.stream_only:
@@ -281,6 +281,7 @@ def main():
     lt = lt_workload(fh, ctx, a) if full and a.lt_count > 0 else None
     serving = serving_sweep(ctx, dev_keys[:8], gal[:8], dev_keys[0], a) if full and a.serve_requests > 0 else None
     enc = encode_run(fh, ctx) if full else None
+    accum = accumulate_run(fh, ctx, a) if full else None
     boot = bootstrap_run() if full and not a.no_bootstrap else None
     if boot is not None and serving is not None:
         serving["with_bootstrap"] = serving_bootstrap_run(a)
@@ -323,6 +324,7 @@ def main():
         "serving": serving,
         "bootstrap": boot,
         "encode": enc,
+        "accumulate": accum,
     }
     print(json.dumps(line))
     ctx.close()
@@ -485,6 +487,55 @@ def lt_workload(fh, ctx, a):
             "batched_transforms_per_s": round(a.lt_count / dtb, 2), "batched_ms_per_transform": round(1000 * dtb / a.lt_count, 3),
             "batched_over_per_call": round(dt / dtb, 3), "bit_exact_batched_vs_per_call": same,
             "algorithmic_gb_per_transform": round(alg / 1e9, 3), "algorithmic_gb_per_batch": round(alg_b / 1e9, 3)}
+
+
+def accumulate_run(fh, ctx, a, reps=5):
+    """Rotation-fold sum of one ciphertext at level 24 over 256 summands (the dot product of a
+    256-feature request): fh_accumulate_sum at radix 4 (12 rotations sharing 4 ModUps, one ModDown
+    per component per fold level) against the doubling loop of 8 rotations and additions.
+    Synthetic keys and residues (the arithmetic is data-independent)."""
+    N, lvl, size = ctx.N, 24, 256
+    lq = lvl + 1
+    full = list(range(L1 + K))
+    idx4 = fh.Context.accumulate_indices(4, 1, size)
+    steps = sorted(set(idx4) | {1 << i for i in range(8)})
+    kbuf = ctx.array((DNUM * 2 * (L1 + K), N))
+    keys = {}
+    for i, r in enumerate(steps):
+        ctx.fill_uniform(kbuf, full, num_polys=DNUM * 2, seed=a.seed + 900 + i)
+        keys[r] = ctx.key_from_device(kbuf)
+    kbuf.free()
+    ct = ctx.array((2 * lq, N))
+    ctx.fill_uniform(ct, list(range(lq)), num_polys=2, seed=a.seed + 950)
+    out, tmp, acc = ctx.array((2 * lq, N)), ctx.array((2 * lq, N)), ctx.array((2 * lq, N))
+    ks, gs = [keys[r] for r in idx4], [ctx.galois_index(r) for r in idx4]
+    pids = list(range(lq)) * 2
+
+    def fold():
+        ctx.accumulate_sum(ct, lvl, 4, 1, size, ks, gs, out)
+
+    def doubling():
+        src = ct
+        for i in range(8):
+            r = 1 << i
+            ctx.eval_rotate(src, lvl, keys[r], ctx.galois_index(r), tmp)
+            ctx.eltwise(0, src, tmp, acc, pids)
+            src = acc
+
+    res = {}
+    for name, fn in (("fold_radix4", fold), ("doubling", doubling)):
+        fn()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ctx.sync()
+        res[name] = 1000 * (time.perf_counter() - t0) / reps
+    for x in [ct, out, tmp, acc] + list(keys.values()):
+        x.free()
+    return {"summands": size, "level": lvl, "radix4_rotations": len(idx4), "doubling_rotations": 8,
+            "fold_radix4_ms": round(res["fold_radix4"], 3), "doubling_ms": round(res["doubling"], 3),
+            "doubling_over_fold": round(res["doubling"] / res["fold_radix4"], 3)}
 
 
 def encode_run(fh, ctx, count=64, reps=5):
