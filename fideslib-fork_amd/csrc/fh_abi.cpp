@@ -565,23 +565,45 @@ int fh_accumulate_indices(int radix, int stride, int size, int start, int32_t* o
   return rc != FH_OK ? rc : n;
 }
 
-int fh_accumulate_sum(fh_context* ctx, const uint64_t* ct, int level, int radix, int stride, int size, int start,
-                      const fh_key* const* keys, const uint32_t* galois, int num_keys, uint64_t* out) {
+namespace {
+// one key and Galois element per fold rotation; throws on a mismatch or an invalid element
+void fold_keys(fh::Context& c, size_t n, const fh_key* const* keys, const uint32_t* galois, int num_keys, std::vector<const uint64_t*>& kp,
+               std::vector<uint32_t>& gl) {
+  if (num_keys != (int)n) throw fh::Error(FH_E_INVALID, "num_keys must equal the fold's rotation count");
+  kp.resize(n);
+  gl.resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    need(keys, "keys"); need(galois, "galois");
+    kp[i] = key_of(c, keys[i]);
+    gl[i] = galois[i];
+    if ((gl[i] & 1) == 0 || gl[i] < 3 || gl[i] >= 2 * c.N) throw fh::Error(FH_E_INVALID, "galois must be odd, not 1, and < 2N");
+  }
+}
+}  // namespace
+
+int fh_accumulate_sum(fh_context* ctx, const uint64_t* ct, int level, int radix, int stride, int size, const fh_key* const* keys,
+                      const uint32_t* galois, int num_keys, uint64_t* out) {
   return guard([&] {
     auto& c = C(ctx);
     need(ct, "ct"); need(out, "out");
     check_level(c, level);
-    const auto idx = fold_indices(radix, stride, size, start);
-    if (num_keys != (int)idx.size()) throw fh::Error(FH_E_INVALID, "num_keys must equal the fold's rotation count");
-    std::vector<const uint64_t*> kp(idx.size());
-    std::vector<uint32_t> gl(idx.size());
-    for (size_t i = 0; i < idx.size(); ++i) {
-      need(keys, "keys"); need(galois, "galois");
-      kp[i] = key_of(c, keys[i]);
-      gl[i] = galois[i];
-      if ((gl[i] & 1) == 0 || gl[i] < 3 || gl[i] >= 2 * c.N) throw fh::Error(FH_E_INVALID, "galois must be odd, not 1, and < 2N");
-    }
-    fh::accumulate_sum(c, ct, level, radix, size, start, kp.data(), gl.data(), out, c.stream);
+    std::vector<const uint64_t*> kp;
+    std::vector<uint32_t> gl;
+    fold_keys(c, fold_indices(radix, stride, size, 1).size(), keys, galois, num_keys, kp, gl);
+    fh::accumulate_sum(c, ct, level, radix, size, kp.data(), gl.data(), out, c.stream);
+  });
+}
+
+int fh_accumulate_sum_from(fh_context* ctx, const uint64_t* ct, int level, int radix, int stride, int size, int start,
+                           const fh_key* const* keys, const uint32_t* galois, int num_keys, uint64_t* out) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(ct, "ct"); need(out, "out");
+    check_level(c, level);
+    std::vector<const uint64_t*> kp;
+    std::vector<uint32_t> gl;
+    fold_keys(c, fold_indices(radix, stride, size, start).size(), keys, galois, num_keys, kp, gl);
+    fh::accumulate_cascade(c, ct, level, radix, size, start, kp.data(), gl.data(), out, c.stream);
   });
 }
 

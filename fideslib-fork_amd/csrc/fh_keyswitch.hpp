@@ -31,10 +31,14 @@ void ksk_inner(Context& c, const uint64_t* digits, int l, const uint64_t* key, u
 void linear_transform(Context& c, const uint64_t* ct, int l, int rowSize, int bStep, int stride, int offset, int slots,
                       const uint64_t* diags, const uint64_t* const* baby_keys, const uint32_t* baby_gal, const uint64_t* giant_key,
                       uint32_t giant_gal, const uint64_t* off_key, uint32_t off_gal, uint64_t* out, hipStream_t st);
-// Rotation-fold sum at a power-of-two radix from fold factor start (fh_lt.hip); keys / galois one
-// per fold rotation in order; out may alias ct.
-void accumulate_sum(Context& c, const uint64_t* ct, int l, int radix, int size, int start, const uint64_t* const* keys,
-                    const uint32_t* galois, uint64_t* out, hipStream_t st);
+// Rotation-fold sums at a power-of-two radix (fh_lt.hip); keys / galois one per fold rotation in
+// order; out may alias ct. accumulate_sum: the lazy fold (c0 stays in Q_l ∪ P across the levels and
+// takes one ModDown at the end); accumulate_cascade: the fold from factor start with a ModDown of
+// both components at every level.
+void accumulate_sum(Context& c, const uint64_t* ct, int l, int radix, int size, const uint64_t* const* keys, const uint32_t* galois,
+                    uint64_t* out, hipStream_t st);
+void accumulate_cascade(Context& c, const uint64_t* ct, int l, int radix, int size, int start, const uint64_t* const* keys,
+                        const uint32_t* galois, uint64_t* out, hipStream_t st);
 // Key inner products of nb items grouped by (key, galois): item b uses the digits (dig_stride
 // apart) and c0 (c0_stride apart) of source b / per_src and writes 2 x (l+1+K) limbs at
 // eo + b * out_stride. add_c0 folds c0 as P c0 over Q_l, or (c0_ext) as given over Q_l ∪ P.

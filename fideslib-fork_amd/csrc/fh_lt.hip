@@ -339,11 +339,12 @@ void linear_transform_batch(Context& c, int count, const uint64_t* cts, uint64_t
   }
 }
 
-// Rotation-fold sum (CKKS::Accumulate, src/CKKS/AccumulateBroadcast.cu:42-84): per fold level s =
-// start, start*radix, ... < size, the rotations by i*stride*s (0 < i < radix, i*s < size) share one
-// ModUp and stay in the extended basis with P*c0 folded in, are added to P*ct, and each component
-// takes one ModDown. keys / galois: one per rotation in fold order.
-void accumulate_sum(Context& c, const uint64_t* ct, int l, int radix, int size, int start, const uint64_t* const* keys,
+// Rotation-fold sum from a start factor (the cascade: CKKS::Accumulate(ct, bStep, stride, size,
+// startFactor), src/CKKS/AccumulateBroadcast.cu:42-84): per fold level s = start, start*radix, ...
+// < size, the rotations by i*stride*s (0 < i < radix, i*s < size) share one ModUp and stay in the
+// extended basis with P*c0 folded in, are added to P*ct, and both components take a ModDown at
+// every level. keys / galois: one per rotation in fold order.
+void accumulate_cascade(Context& c, const uint64_t* ct, int l, int radix, int size, int start, const uint64_t* const* keys,
                     const uint32_t* galois, uint64_t* out, hipStream_t st) {
   const uint32_t N = c.N, lq = l + 1, ext = lq + c.K;
   const size_t E = (size_t)ext * N, Q = (size_t)lq * N;
@@ -370,6 +371,54 @@ void accumulate_sum(Context& c, const uint64_t* ct, int l, int radix, int size, 
     moddown(c, fast.p, l, out, 2, 0, 0, st);
     used += n;
   }
+}
+
+
+// The lazy rotation-fold sum (CKKS::Accumulate(ct, bStep, stride, size),
+// src/CKKS/AccumulateBroadcast.cu:122-160). Per fold level s = 1, radix, ... < size: one ModUp of c1
+// shared by the level's rotations (by i*stride*s, 0 < i < radix, i*s < size), each kept in Q_l ∪ P;
+// c0 is folded as P*c0 on the first level and afterwards stays in Q_l ∪ P, where it is added on
+// every limb before the automorphism (the c0_modup branch of hoistedRotateDotKSK_2_,
+// src/CKKS/ElemenwiseBatchKernels.cu:302-307). The rotations are summed into (c0, P*c1); c1 takes a
+// ModDown at the end of each level and c0 one ModDown after the last. Buffer fast: slot 0 holds the
+// accumulators (extended c0, extended c1), slots 1..radix-1 the level's rotations.
+void accumulate_sum(Context& c, const uint64_t* ct, int l, int radix, int size, const uint64_t* const* keys, const uint32_t* galois,
+                    uint64_t* out, hipStream_t st) {
+  const uint32_t N = c.N, lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);
+  const size_t E = (size_t)ext * N, Q = (size_t)lq * N;
+  if (out != ct) FH_HIP(hipMemcpyAsync(out, ct, 2 * Q * 8, hipMemcpyDeviceToDevice, st));
+  if (size <= 1 || radix < 2) return;
+  std::vector<int> epids(ext);
+  for (uint32_t p = 0; p < ext; ++p) epids[p] = c.ext_prime(l, p);
+  DBuf fast(c.pool, (size_t)radix * 2 * E);
+  DBuf dig(c.pool, (size_t)beta * E);
+  bool c0_ext = false;
+  int used = 0;
+  for (long long s = 1; s < size; s *= radix) {
+    int n = 0;
+    while (n + 1 < radix && (long long)(n + 1) * s < size) ++n;
+    // the level's rotations, all from the level's input (c0 or its extension, and c1)
+    modup_batch(c, out + Q, l, 1, 0, dig.p, st, false);
+    ksk_items(c, n, n, dig.p, 0, c0_ext ? fast.p : out, 0, true, c0_ext, l, keys + used, galois + used, fast.p + 2 * E, 2 * E, st,
+              out + Q, 0);
+    {
+      // accumulators: P*c1 (and P*c0 on the first level; afterwards slot 0 already holds c0)
+      ProfScope ps_(c, "lt_extend", st);
+      const uint32_t comps = c0_ext ? 1 : 2;
+      k_extend_p<<<grid_n(N, comps * ext), kT, 0, st>>>(c0_ext ? out + Q : out, c0_ext ? fast.p + E : fast.p, N, lq, ext, c.d_prime,
+                                                       c.d_pmod, c.d_pmod_sh);
+      FH_HIP(hipGetLastError());
+    }
+    LimbSet a = make_set(fast.p, ext, epids.data(), 2, E);
+    for (int i = 1; i <= n; ++i) {
+      LimbSet b = make_set(fast.p + (size_t)i * 2 * E, ext, epids.data(), 2, E);
+      ew_binary(c, 0, a, b, a, st);
+    }
+    moddown(c, fast.p + E, l, out + Q, 1, 0, 0, st);
+    c0_ext = true;
+    used += n;
+  }
+  moddown(c, fast.p, l, out, 1, 0, 0, st);
 }
 
 }  // namespace fh

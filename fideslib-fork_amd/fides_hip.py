@@ -93,7 +93,8 @@ def _declare(L):
         "fh_serve_run": (I, [P, C.POINTER(ServeConfig), C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), P, C.POINTER(ServeResult)]),
         "fh_linear_transform_batch": (I, [P, I, VP, S, I, I, I, I, I, I, VP, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), P,
                                           C.c_uint32, P, C.c_uint32, VP]),
-        "fh_accumulate_sum": (I, [P, VP, I, I, I, I, I, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), I, VP]),
+        "fh_accumulate_sum": (I, [P, VP, I, I, I, I, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), I, VP]),
+        "fh_accumulate_sum_from": (I, [P, VP, I, I, I, I, I, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), I, VP]),
         "fh_accumulate_indices": (I, [I, I, I, I, C.POINTER(C.c_int32), I]),
         "fh_tune_set": (I, [P, C.c_char_p, I]),
         "fh_tune_get": (I, [P, C.c_char_p, C.POINTER(C.c_int)]),
@@ -361,12 +362,20 @@ class Context:
         _check(min(0, load().fh_accumulate_indices(radix, stride, size, start, buf, n)))
         return list(buf[:n])
 
-    def accumulate_sum(self, ct, level, radix, stride, size, keys, galois, out, start=1):
-        """keys / galois: one per index of accumulate_indices(radix, stride, size, start)."""
+    def accumulate_sum(self, ct, level, radix, stride, size, keys, galois, out):
+        """The lazy fold; keys / galois: one per index of accumulate_indices(radix, stride, size)."""
         n = len(keys)
         ka = (C.c_void_p * max(n, 1))(*[k.h for k in keys])
         ga = (C.c_uint32 * max(n, 1))(*[int(g) for g in galois])
-        _check(load().fh_accumulate_sum(self.h, ct.ptr, level, radix, stride, size, start, ka, ga, n, out.ptr))
+        _check(load().fh_accumulate_sum(self.h, ct.ptr, level, radix, stride, size, ka, ga, n, out.ptr))
+
+    def accumulate_sum_from(self, ct, level, radix, stride, size, start, keys, galois, out):
+        """The cascade from a start factor; keys / galois: one per index of
+        accumulate_indices(radix, stride, size, start)."""
+        n = len(keys)
+        ka = (C.c_void_p * max(n, 1))(*[k.h for k in keys])
+        ga = (C.c_uint32 * max(n, 1))(*[int(g) for g in galois])
+        _check(load().fh_accumulate_sum_from(self.h, ct.ptr, level, radix, stride, size, start, ka, ga, n, out.ptr))
 
     def linear_transform_batch(self, cts, count, level, row_size, b_step, stride, diags, baby_keys, baby_galois, out, giant_key=None,
                                giant_galois=1, offset_key=None, offset_galois=1, offset=0, slots=None, ct_stride=0):

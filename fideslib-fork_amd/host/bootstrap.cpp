@@ -2,7 +2,7 @@
 //
 // The bootstrap follows the reference's pipeline (src/CKKS/Bootstrap.cu:437-648):
 //   ModRaise (fh_mod_raise: tower 0 lifted centred to the whole chain, Bootstrap.cu:708-930)
-//   -> for sparse packing, the partial trace over the rotations by slots * 2^j
+//   -> for sparse packing, the partial trace: the lazy rotation-fold sum at stride slots
 //      (AccumulateBroadcast.cu:122-160), which keeps the coefficients at multiples of N/(2 slots)
 //   -> CoeffsToSlots: the inverse special FFT as levelBudget[0] BSGS LinearTransforms, one per
 //      group of butterfly stages (CoeffsToSlots.cu:75-151 applies the reference's factors the
@@ -492,7 +492,8 @@ void CryptoContextImpl::EvalBootstrapKeyGen(const PrivateKey& sk, uint32_t slots
   if (!boot_ || boot_->slots != (slots ? slots : N_ / 2)) throw std::runtime_error("fideslib: EvalBootstrapSetup for these slots first");
   const uint32_t n = boot_->slots;
   std::vector<int32_t> steps;
-  for (uint32_t s = n; s < N_ / 2; s *= 2) steps.push_back((int32_t)s);  // partial trace
+  if (n < N_ / 2)  // partial trace: the rotation-fold sum over the N/(2n) copies at stride n
+    for (int32_t r : GetAccumulateSumRotationIndices((int)(N_ / 2 / n), (int)n)) steps.push_back(r);
   std::vector<BootstrapGroup> all = boot_->cts;
   all.insert(all.end(), boot_->stc.begin(), boot_->stc.end());
   for (const auto& g : all) {
@@ -547,7 +548,9 @@ Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0, uint32_t numI
   raised->slots = n;
   raised = Rescale(EvalMult(raised, st.pre));
   // 2. sparse packing: trace onto the coefficients at multiples of gap (times gap)
-  for (uint32_t s = n; s < N_ / 2; s *= 2) raised = EvalAdd(raised, EvalRotate(raised, (int32_t)s));
+  //    by the lazy rotation-fold sum (CKKS::Accumulate(ct, bStep, slots, N/2/slots),
+  //    src/CKKS/Bootstrap.cu:548, src/CKKS/AccumulateBroadcast.cu:122-160)
+  if (n < N_ / 2) raised = AccumulateSum(raised, (int)(N_ / 2 / n), (int)n);
   // 3. CoeffsToSlots: slots (I + p / q_0) / (2K), real and imaginary coefficient halves
   Ciphertext t = raised;
   for (size_t g = 0; g < st.cts.size(); ++g)

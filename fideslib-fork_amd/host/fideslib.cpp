@@ -940,10 +940,13 @@ std::vector<int32_t> CryptoContextImpl::GetAccumulateSumRotationIndices(int slot
   return v;
 }
 
+// start 0: the lazy fold (CKKS::Accumulate(ct, bStep, stride, size) behind accumulateSum); start
+// >= 1: the cascade from that factor (Accumulate(ct, 4, stride, size, start) behind
+// accumulateSumInPlace(ct, slots, stride, start), api/engine/cuda/CudaEngine.cpp:780-784)
 Ciphertext CryptoContextImpl::accumulate(const Ciphertext& ct, int slots, int stride, int start) {
   require_standard(*ct, "AccumulateSum");
   if (ct->ncomp != 2) throw std::runtime_error("fideslib: AccumulateSum: ciphertext should be relinearized before");
-  const auto idx = GetAccumulateSumRotationIndices(slots, stride, start);
+  const auto idx = GetAccumulateSumRotationIndices(slots, stride, start == 0 ? 1 : start);
   std::vector<const fh_key*> keys(idx.size());
   std::vector<uint32_t> gal(idx.size());
   for (size_t i = 0; i < idx.size(); ++i) {
@@ -951,18 +954,23 @@ Ciphertext CryptoContextImpl::accumulate(const Ciphertext& ct, int slots, int st
     keys[i] = rotKey(gal[i]);
   }
   auto out = newCt(ct->towers, 2);
-  check(fh_accumulate_sum(ctx_, ct->data->ptr, (int)ct->towers - 1, kAccumulateSumRadix, stride, slots, start, keys.data(), gal.data(),
-                          (int)idx.size(), out->data->ptr));
+  if (start == 0)
+    check(fh_accumulate_sum(ctx_, ct->data->ptr, (int)ct->towers - 1, kAccumulateSumRadix, stride, slots, keys.data(), gal.data(),
+                            (int)idx.size(), out->data->ptr));
+  else
+    check(fh_accumulate_sum_from(ctx_, ct->data->ptr, (int)ct->towers - 1, kAccumulateSumRadix, stride, slots, start, keys.data(),
+                                 gal.data(), (int)idx.size(), out->data->ptr));
   out->noiseScaleDeg = ct->noiseScaleDeg;
   out->scalingFactor = ct->scalingFactor;
   out->slots = ct->slots;
   return out;
 }
 
-Ciphertext CryptoContextImpl::AccumulateSum(const Ciphertext& ct, int slots, int stride) { return accumulate(ct, slots, stride, 1); }
-void CryptoContextImpl::AccumulateSumInPlace(Ciphertext& ct, int slots, int stride) { ct = accumulate(ct, slots, stride, 1); }
+Ciphertext CryptoContextImpl::AccumulateSum(const Ciphertext& ct, int slots, int stride) { return accumulate(ct, slots, stride, 0); }
+void CryptoContextImpl::AccumulateSumInPlace(Ciphertext& ct, int slots, int stride) { ct = accumulate(ct, slots, stride, 0); }
 void CryptoContextImpl::AccumulateSumInPlace(Ciphertext& ct, int slots, int stride, int start) {
   const uint32_t before = ct->slots;
+  if (start <= 0) return;  // the reference's cascade returns at once for startFactor <= 0
   ct = accumulate(ct, slots, stride, start);
   if (start > 0 && slots > 0 && (long long)slots * stride == before) ct->slots = (uint32_t)(stride * start);
 }

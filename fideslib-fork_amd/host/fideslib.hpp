@@ -255,11 +255,13 @@ class CryptoContextImpl {
   void LinearTransformInPlace(Ciphertext& ct, const PreparedLinearTransform& prepared);
 
   // ---- rotation-fold sum (AccumulateSum, api/CryptoContext.hpp:456-458; CudaEngine::accumulateSum,
-  // api/engine/cuda/CudaEngine.cpp:755-782): slot j of the result is the sum over i < slots/start
-  // of slot j + i*start*stride, folded at radix kAccumulateSumRadix with one ModDown per fold
-  // level. The two- and three-argument forms keep the slot count; the start form sets it to
-  // stride*start when slots*stride equals it (src/CKKS/AccumulateBroadcast.cu:82-83). Needs the
-  // rotation keys GetAccumulateSumRotationIndices lists.
+  // api/engine/cuda/CudaEngine.cpp:755-784): slot j of the result is the sum over i < slots/start
+  // of slot j + i*start*stride, folded at radix kAccumulateSumRadix. The two- and three-argument
+  // forms run the lazy fold (c0 stays in Q_l ∪ P across the fold levels and takes one ModDown at
+  // the end, src/CKKS/AccumulateBroadcast.cu:122-160) and keep the slot count; the start form runs
+  // the cascade (a ModDown of both components per level, :42-84) and sets the slot count to
+  // stride*start when slots*stride equals it (:82-83). Needs the rotation keys
+  // GetAccumulateSumRotationIndices lists.
   static constexpr int kAccumulateSumRadix = 4;
   Ciphertext AccumulateSum(const Ciphertext& ct, int slots, int stride = 1);
   void AccumulateSumInPlace(Ciphertext& ct, int slots, int stride = 1);

@@ -896,10 +896,12 @@ void orc_linear_transform(const orc_ctx* c, const u64* ct, int l, int rowSize, i
   free(fast); free(acc); free(R); free(c1q);
 }
 
-/* Rotation-fold sum at a power-of-two radix (CKKS::Accumulate, src/CKKS/AccumulateBroadcast.cu:42-84,
-   the lazy fold of BITCOMPAT.md:69-80): for s = start, start*radix, ... < size, the rotations by
-   i*stride*s (0 < i < radix, i*s < size) of the current ciphertext are formed in the extended basis
-   (one shared decomposition, P*c0 folded in), added to P*ct, and each component takes ONE ModDown.
+/* Rotation-fold sum from a start factor (the cascade: CKKS::Accumulate(ct, bStep, stride, size,
+   startFactor), src/CKKS/AccumulateBroadcast.cu:42-84, behind CudaEngine::accumulateSumInPlace(ct,
+   slots, stride, start), api/engine/cuda/CudaEngine.cpp:780-783): for s = start, start*radix, ... <
+   size, the rotations by i*stride*s (0 < i < radix, i*s < size) of the current ciphertext are formed
+   in the extended basis (one shared decomposition, P*c0 folded in), added to P*ct
+   (Ciphertext::extend), and BOTH components take one ModDown at every level.
    keys / ks: one key and Galois element per rotation, in fold order. out may alias ct. */
 void orc_accumulate_sum(const orc_ctx* c, const u64* ct, int l, int radix, int stride, int size, int start,
                         const u64* keys, const uint32_t* ks, u64* out) {
@@ -935,6 +937,64 @@ void orc_accumulate_sum(const orc_ctx* c, const u64* ct, int l, int radix, int s
     orc_moddown(c, acc + E, l, out + Q);
   }
   free(acc); free(t);
+}
+
+/* The lazy rotation-fold sum (CKKS::Accumulate(ct, bStep, stride, size),
+   src/CKKS/AccumulateBroadcast.cu:122-160, behind CudaEngine::accumulateSum and
+   accumulateSumInPlace(ct, slots, stride), api/engine/cuda/CudaEngine.cpp:755-777). For s = 1,
+   radix, radix^2, ... < size: the rotations by i*stride*s (0 < i < radix, i*s < size) share one
+   decomposition of c1 and stay in Q_l∪P (Ciphertext::rotate_hoisted with ext, src/CKKS/Ciphertext.cpp
+   :1221-1350). c0 enters the first level as P*c0 over Q_l (0 over P) and from then on stays in
+   Q_l∪P: every later rotation adds it on every limb before the automorphism (the c0_modup branch of
+   hoistedRotateDotKSK_2_, src/CKKS/ElemenwiseBatchKernels.cu:302-307). The rotations are added to
+   (c0, P*c1) (RNSPoly::add of an extended operand into a Q_l one scales it by P first,
+   src/CKKS/RNSPoly.cpp:272-286 and src/AddSub.cu:57-70), c1 takes its ModDown at the end of every
+   level, and c0 takes ONE ModDown after the last level. keys / ks: one per rotation in fold order
+   (orc_accumulate_sum's index order with start 1). out may alias ct. */
+void orc_accumulate_lazy(const orc_ctx* c, const u64* ct, int l, int radix, int size, const u64* keys, const uint32_t* ks,
+                         u64* out) {
+  u64 N = c->N;
+  int ext = l + 1 + c->K;
+  size_t E = (size_t)ext * N, Q = (size_t)(l + 1) * N;
+  size_t key_words = (size_t)c->dnum * 2 * c->np * N;
+  if (out != ct) memcpy(out, ct, 2 * Q * 8);
+  if (size <= 1 || radix < 2) return;
+  u64* c0e = (u64*)malloc(E * 8);     /* c0 over Q_l∪P once the first level has run */
+  u64* a0 = (u64*)malloc(E * 8);
+  u64* a1 = (u64*)malloc(E * 8);
+  u64* t = (u64*)malloc(2 * E * 8);
+  int c0_ext = 0, used = 0;
+  for (long long s = 1; s < size; s *= radix) {
+    /* accumulators: c0 (as P*c0 on the first level, extended afterwards) and P*c1 */
+    memset(a1, 0, E * 8);
+    for (int p = 0; p <= l; ++p)
+      for (u64 j = 0; j < N; ++j) a1[(size_t)p * N + j] = orc_mulmod(out[Q + (size_t)p * N + j], c->Pmod[p], c->q[p]);
+    if (c0_ext) {
+      memcpy(a0, c0e, E * 8);
+    } else {
+      memset(a0, 0, E * 8);
+      for (int p = 0; p <= l; ++p)
+        for (u64 j = 0; j < N; ++j) a0[(size_t)p * N + j] = orc_mulmod(out[(size_t)p * N + j], c->Pmod[p], c->q[p]);
+    }
+    /* every rotation of this level reads the level's input (c0 or its extension, c1) */
+    for (long long i = 1; i < radix && i * s < size; ++i, ++used) {
+      rotate_core(c, c0_ext ? c0e : out, c0_ext ? 2 : 1, out + Q, l, keys + (size_t)used * key_words, ks[used], t);
+#pragma omp parallel for
+      for (int pos = 0; pos < ext; ++pos) {
+        u64 q = c->q[ext_prime(c, l, pos)];
+        for (u64 n = 0; n < N; ++n) {
+          size_t x = (size_t)pos * N + n;
+          a0[x] = modadd(a0[x], t[x], q);
+          a1[x] = modadd(a1[x], t[E + x], q);
+        }
+      }
+    }
+    orc_moddown(c, a1, l, out + Q);
+    memcpy(c0e, a0, E * 8);
+    c0_ext = 1;
+  }
+  orc_moddown(c, c0e, l, out);
+  free(c0e); free(a0); free(a1); free(t);
 }
 
 void orc_eval_rotate(const orc_ctx* c, const u64* ct, int l, const u64* key, uint32_t k, u64* out) {

@@ -132,9 +132,13 @@ void orc_eval_rotate(const orc_ctx* c, const uint64_t* ct, int l, const uint64_t
 void orc_eval_rotate_ext(const orc_ctx* c, const uint64_t* ct, int l, const uint64_t* key, uint32_t k, int add_c0, uint64_t* out_ext);
 
 /* BSGS linear transform in the reference's extended-basis (hoisted) formulation; see the .c file */
-/* Rotation-fold sum (lazy, one ModDown per fold level); keys / ks one per rotation in fold order. */
+/* Rotation-fold sum from a start factor (the cascade: both components take a ModDown at every fold
+ * level); keys / ks one per rotation in fold order. */
 void orc_accumulate_sum(const orc_ctx* c, const uint64_t* ct, int l, int radix, int stride, int size, int start,
                         const uint64_t* keys, const uint32_t* ks, uint64_t* out);
+/* The lazy rotation-fold sum (c0 stays in Q_l∪P across the levels, one final ModDown of c0). */
+void orc_accumulate_lazy(const orc_ctx* c, const uint64_t* ct, int l, int radix, int size, const uint64_t* keys,
+                         const uint32_t* ks, uint64_t* out);
 void orc_linear_transform(const orc_ctx* c, const uint64_t* ct, int l, int rowSize, int bStep, int stride, int offset, int slots,
                           const uint64_t* diags, const uint64_t* baby_keys, const uint32_t* baby_k, const uint64_t* giant_key,
                           uint32_t giant_k, const uint64_t* off_key, uint32_t off_k, uint64_t* out);

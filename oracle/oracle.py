@@ -83,6 +83,8 @@ def lib():
                                          u64p, U32, u64p, U32, u64p]),
             "orc_accumulate_sum": (V, [P, u64p, I, I, I, I, I, u64p, np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"),
                                        u64p]),
+            "orc_accumulate_lazy": (V, [P, u64p, I, I, I, u64p, np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"),
+                                        u64p]),
             "orc_gen_secret": (V, [P, U, I, u64p]),
             "orc_gen_switch_key": (V, [P, u64p, u64p, U, u64p]),
             "orc_gen_relin_key": (V, [P, u64p, U, u64p]),
@@ -354,13 +356,25 @@ class Ctx:
             s *= radix
         return out
 
-    def accumulate_sum(self, ct, l, radix, stride, size, keys, ks, start=1):
-        """keys / ks: one rotation key and Galois element per index of accumulate_indices."""
-        n = len(self.accumulate_indices(radix, stride, size, start))
+    def _fold_keys(self, n, keys, ks):
         if len(keys) != n or len(ks) != n:
             raise ValueError("one key per fold rotation")
         kk = np.ascontiguousarray(np.concatenate(keys)) if n else np.zeros(self.key_shape(), dtype=np.uint64)
         kx = np.ascontiguousarray(np.array([int(x) for x in ks] or [1], dtype=np.uint32))
+        return kk, kx
+
+    def accumulate_sum(self, ct, l, radix, stride, size, keys, ks):
+        """The lazy fold (reference Accumulate(ct, bStep, stride, size), behind accumulateSum):
+        keys / ks one rotation key and Galois element per index of accumulate_indices."""
+        kk, kx = self._fold_keys(len(self.accumulate_indices(radix, stride, size)), keys, ks)
+        out = self.zeros(l + 1, 2)
+        lib().orc_accumulate_lazy(self.h, np.ascontiguousarray(ct), l, radix, size, kk, kx, out)
+        return out
+
+    def accumulate_sum_from(self, ct, l, radix, stride, size, keys, ks, start):
+        """The cascade from a start factor (reference Accumulate(ct, bStep, stride, size, startFactor),
+        behind accumulateSumInPlace(ct, slots, stride, start))."""
+        kk, kx = self._fold_keys(len(self.accumulate_indices(radix, stride, size, start)), keys, ks)
         out = self.zeros(l + 1, 2)
         lib().orc_accumulate_sum(self.h, np.ascontiguousarray(ct), l, radix, stride, size, start, kk, kx, out)
         return out

@@ -212,15 +212,28 @@ def bstep_for(rowSize):
     return b
 
 
+TRACE_RADIX = 4
+
+
+def trace_indices(n, N):
+    """rotations of the partial trace: the lazy fold at radix 4 over the N/(2n) copies at stride n"""
+    if n >= N // 2:
+        return []
+    out, s, size = [], 1, N // 2 // n
+    while s < size:
+        i = 1
+        while i < TRACE_RADIX and i * s < size:
+            out.append(i * s * n)
+            i += 1
+        s *= TRACE_RADIX
+    return out
+
+
 def rotation_steps(k, N):
     """slot rotations a bootstrap with constants k uses: the partial trace, then every group's
     baby steps, giant step and offset"""
     n = k["slots"]
-    steps = []
-    t = n
-    while t < N // 2:
-        steps.append(t)
-        t *= 2
+    steps = list(trace_indices(n, N))
     for rowSize, stride, offset, _ in k["cts"] + k["stc"]:
         b = bstep_for(rowSize)
         steps += [(i * stride) % n for i in range(1, b)]
@@ -309,6 +322,12 @@ class Ops:
     def rescale(self, a):
         x, t, d = a
         return (self.o.rescale_ct(x, t - 1), t - 1, max(1, d - 1))
+
+    def accumulate(self, a, stride, size):
+        """the lazy rotation-fold sum at radix 4 (metadata unchanged)"""
+        x, t, d = a
+        ks = [self.o.rot_index(r) for r in trace_indices(stride, 2 * stride * size)]
+        return (self.o.accumulate_sum(x, t - 1, TRACE_RADIX, stride, size, [self.rot[k] for k in ks], ks), t, d)
 
     def rotate(self, a, k):
         x, t, d = a
@@ -419,10 +438,8 @@ def bootstrap(ops, ct, k):
     L1 = o.L1
     raised = (o.mod_raise(ct[0], 0, L1 - 1), L1, 1)
     raised = ops.rescale(ops.mul_scalar(raised, k["pre"]))
-    s = n
-    while s < N // 2:
-        raised = ops.add(raised, ops.rotate(raised, o.rot_index(s)))
-        s *= 2
+    if n < N // 2:
+        raised = ops.accumulate(raised, n, N // 2 // n)
     t = raised
     for g in k["cts"]:
         t = ops.rescale(ops.linear_transform(t, g, n))

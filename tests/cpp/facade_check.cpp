@@ -178,6 +178,22 @@ int main(int argc, char** argv) {
       const cvec r1 = rotate(x, 1), r2 = rotate(x, 2);
       for (uint32_t i = 0; i < slots; ++i) e[i] = x[i] + r1[i] + r2[i];
       dump_ct(*cc, kp.secretKey, "acc3", cc->AccumulateSum(ct1, 3), e);
+      // two fold levels (16 summands: rotations 1, 2, 3, then 4, 8, 12): the lazy fold, and the
+      // cascade from start factor 4 (rotations 4, 8, 12 of the input: 4 summands at stride 4)
+      cc->EvalRotateKeyGen(kp.secretKey, std::vector<int32_t>{3, 4, 8, 12});  // +10 .. +13
+      for (uint32_t i = 0; i < slots; ++i) {
+        e[i] = 0;
+        for (uint32_t k = 0; k < 16; ++k) e[i] += x[(i + k) % slots];
+      }
+      dump_ct(*cc, kp.secretKey, "acc16", cc->AccumulateSum(ct1, 16), e);
+      for (uint32_t i = 0; i < slots; ++i) {
+        e[i] = 0;
+        for (uint32_t k = 0; k < 16; k += 4) e[i] += x[(i + k) % slots];
+      }
+      Ciphertext t4 = ct1;
+      cc->AccumulateSumInPlace(t4, 16, 1, 4);
+      t4->slots = ct1->slots;
+      dump_ct(*cc, kp.secretKey, "acc16s4", t4, e);
     }
     for (uint32_t i = 0; i < slots; ++i) e[i] = std::conj(x[i]);
     dump_ct(*cc, kp.secretKey, "conj", cc->EvalConjugate(ct1), e);

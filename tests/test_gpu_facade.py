@@ -67,6 +67,8 @@ def recomputed(session, oracle_mod):
     assert gal == [pow(5, r % (N // 2), 2 * N) for r in (1, 2, -3, 7)]
     rot = {g: o.gen_rot_key(s, g, seed + 3 + i) for i, g in enumerate(gal)}
     rot[2 * N - 1] = o.gen_rot_key(s, 2 * N - 1, seed + 7)
+    for i, r in enumerate((3, 4, 8, 12)):      # the fold keys, drawn after the two encryptions
+        rot[pow(5, r, 2 * N)] = o.gen_rot_key(s, pow(5, r, 2 * N), seed + 10 + i)
     x, y = _c128(out, "x"), _c128(out, "y")
     ct1 = o.encrypt(o.encode(x, scale, l), l, s, seed + 8)
     ct2 = o.encrypt_pk(o.encode(y, scale, l), l, pk_b, pk_a, seed + 9)
@@ -146,6 +148,10 @@ def test_rotations(session, recomputed):
     _check(session, "rotm3", o.eval_rotate(c2, l, R["rot"][g(-3)], g(-3)), 2, t, 1, sc, 1e-6)
     _check(session, "conj", o.eval_rotate(c1, l, R["rot"][2 * N - 1], 2 * N - 1), 2, t, 1, sc, 1e-6)
     _check(session, "acc3", o.accumulate_sum(c1, l, 4, 1, 3, [R["rot"][g(1)], R["rot"][g(2)]], [g(1), g(2)]), 2, t, 1, sc, 1e-6)
+    ks16 = [g(r) for r in (1, 2, 3, 4, 8, 12)]
+    _check(session, "acc16", o.accumulate_sum(c1, l, 4, 1, 16, [R["rot"][k] for k in ks16], ks16), 2, t, 1, sc, 1e-5)
+    ks4 = [g(r) for r in (4, 8, 12)]
+    _check(session, "acc16s4", o.accumulate_sum_from(c1, l, 4, 1, 16, [R["rot"][k] for k in ks4], ks4, 4), 2, t, 1, sc, 1e-5)
     _check(session, "fast7", o.eval_rotate(c1, l, R["rot"][g(7)], g(7)), 2, t, 1, sc, 1e-6)
     # EvalFastRotationExt + KeySwitchDown from the same precomputed digits
     _check(session, "fastext7", o.eval_rotate(c1, l, R["rot"][g(7)], g(7)), 2, t, 1, sc, 1e-6)

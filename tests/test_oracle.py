@@ -542,9 +542,9 @@ def test_context_rejects_inconsistent_prime_count():
 
 
 def test_accumulate_sum_invariants():
-    """The rotation-fold sum decrypts to sum_k roll(z, -k*start*stride) over the fold's offsets, and a
-    one-level fold equals ModDown(P*ct + the extended rotations): one ModDown of the sum, not one per
-    rotation (the lazy fold of the reference's Accumulate)."""
+    """Both rotation-fold sums (the lazy fold and the cascade from a start factor) decrypt to
+    sum_k roll(z, -k*start*stride) over the fold's offsets, and a one-level fold equals
+    ModDown(P*ct + the extended rotations) in both forms: one ModDown of the sum, not one per rotation."""
     c = small_ctx(13, 4, 2, 2)
     n, l, Dl = c.N // 2, 3, 2.0 ** 50
     s = c.gen_secret(21)
@@ -556,10 +556,13 @@ def test_accumulate_sum_invariants():
         idx = c.accumulate_indices(radix, stride, size, start)
         ks = [c.rot_index(r) for r in idx]
         keys = [c.gen_rot_key(s, k, 300 + i) for i, k in enumerate(ks)]
-        out = c.accumulate_sum(ct, l, radix, stride, size, keys, ks, start)
-        dec = c.decode(c.decrypt(out, l, s), l, n, Dl)
+        outs = [c.accumulate_sum_from(ct, l, radix, stride, size, keys, ks, start)]
+        if start == 1:
+            outs.append(c.accumulate_sum(ct, l, radix, stride, size, keys, ks))
         exp = sum(np.roll(z, -k) for k in offsets)
-        assert np.max(np.abs(dec - exp)) < 1e-7, (radix, stride, size, start)
+        for out in outs:
+            dec = c.decode(c.decrypt(out, l, s), l, n, Dl)
+            assert np.max(np.abs(dec - exp)) < 1e-7, (radix, stride, size, start)
     assert c.accumulate_indices(4, 1, 16) == [1, 2, 3, 4, 8, 12]
     assert c.accumulate_indices(4, 1, 9) == [1, 2, 3, 4, 8]
     assert c.accumulate_indices(4, 2, 16, 2) == [4, 8, 12, 16]
@@ -568,23 +571,84 @@ def test_accumulate_sum_invariants():
     ks = [c.rot_index(r) for r in idx]
     keys = [c.gen_rot_key(s, k, 400 + i) for i, k in enumerate(ks)]
     ext = l + 1 + c.K
-    pids = c.ext_pids(l)
-    P = 1
-    for k in range(c.K):
-        P *= c.q(c.L1 + k)
-    acc = np.zeros((2 * ext, c.N), dtype=np.uint64)
-    for comp in range(2):
-        for p in range(l + 1):
-            q = c.q(p)
-            acc[comp * ext + p] = np.array([int(v) * (P % q) % q for v in ct[comp * (l + 1) + p]], dtype=np.uint64)
+    acc = _p_scaled(c, ct, l)
     for k, key in zip(ks, keys):
-        r = c.eval_rotate_ext(ct, l, key, k)
-        for row in range(2 * ext):
-            q = np.uint64(c.q(pids[row % ext]))
-            acc[row] = (acc[row] + r[row]) % q
+        acc = _add_ext(c, l, acc, c.eval_rotate_ext(ct, l, key, k))
     lazy = np.concatenate([c.moddown(acc[:ext], l), c.moddown(acc[ext:], l)])
     assert np.array_equal(c.accumulate_sum(ct, l, 4, 1, 4, keys, ks), lazy)
+    assert np.array_equal(c.accumulate_sum_from(ct, l, 4, 1, 4, keys, ks, 1), lazy)
     eager = ct.copy()
     for k, key in zip(ks, keys):
         eager = c.add(eager, c.eval_rotate(ct, l, key, k))
     assert not np.array_equal(eager, lazy)       # the per-rotation ModDown rounds differently
+
+
+def _p_scaled(c, ct, l, comps=(0, 1)):
+    """(P*c_comp over Q_l, 0 over P) for each listed component, stacked."""
+    ext = l + 1 + c.K
+    P = 1
+    for k in range(c.K):
+        P *= c.q(c.L1 + k)
+    acc = np.zeros((len(comps) * ext, c.N), dtype=np.uint64)
+    for r, comp in enumerate(comps):
+        for p in range(l + 1):
+            q = c.q(p)
+            acc[r * ext + p] = np.array([int(v) * (P % q) % q for v in ct[comp * (l + 1) + p]], dtype=np.uint64)
+    return acc
+
+
+def _add_ext(c, l, a, b):
+    """a + b over stacked polynomials in Q_l ∪ P (exact, through Python integers)."""
+    ext = l + 1 + c.K
+    pids = c.ext_pids(l)
+    out = np.empty_like(a)
+    for row in range(a.shape[0]):
+        q = c.q(pids[row % ext])
+        out[row] = np.array([(int(x) + int(y)) % q for x, y in zip(a[row], b[row])], dtype=np.uint64)
+    return out
+
+
+@pytest.mark.parametrize("radix,size,terms", [(4, 16, 16), (2, 8, 8), (4, 9, 12)])
+def test_accumulate_lazy_composed(radix, size, terms):
+    """The lazy fold over two or more levels, composed from pinned pieces (the extended rotation of c1
+    alone, the coefficient-domain automorphism, ModDown): c0 enters as P*c0 and then stays in Q_l∪P,
+    where every later rotation adds the automorphism of the extended c0 itself; c1 takes a ModDown per
+    level, c0 one at the end. It decrypts to the sum and differs in bits from the cascade, which takes
+    a ModDown of c0 at every level."""
+    c = small_ctx(12, 3, 2, 2)
+    n, l, Dl = c.N // 2, 2, 2.0 ** 50
+    s = c.gen_secret(31)
+    z = _slots(n, 32)
+    ct = c.encrypt(c.encode(z, Dl, l), l, s, 33)
+    ext, Q = l + 1 + c.K, l + 1
+    idx = c.accumulate_indices(radix, 1, size)
+    ks = [c.rot_index(r) for r in idx]
+    keys = [c.gen_rot_key(s, k, 600 + i) for i, k in enumerate(ks)]
+    c0e, c1 = None, ct[Q:].copy()
+    used, lev = 0, 1
+    while lev < size:
+        cur = np.concatenate([ct[:Q] if c0e is None else np.zeros((Q, c.N), dtype=np.uint64), c1])
+        a0 = _p_scaled(c, cur, l, (0,)) if c0e is None else c0e.copy()
+        a1 = _p_scaled(c, cur, l, (1,))
+        i = 1
+        while i < radix and i * lev < size:
+            k, key = ks[used], keys[used]
+            if c0e is None:
+                r = c.eval_rotate_ext(cur, l, key, k)                  # P*c0 folded before the automorphism
+            else:
+                r = c.eval_rotate_ext(cur, l, key, k, add_c0=False)   # the key switch of c1 alone ...
+                r[:ext] = _add_ext(c, l, r[:ext], c.automorph(c0e, k))  # ... plus the automorphism of c0e
+            a0 = _add_ext(c, l, a0, r[:ext])
+            a1 = _add_ext(c, l, a1, r[ext:])
+            used, i = used + 1, i + 1
+        c1 = c.moddown(a1, l)
+        c0e = a0
+        lev *= radix
+    composed = np.concatenate([c.moddown(c0e, l), c1])
+    lazy = c.accumulate_sum(ct, l, radix, 1, size, keys, ks)
+    assert np.array_equal(lazy, composed)
+    cascade = c.accumulate_sum_from(ct, l, radix, 1, size, keys, ks, 1)
+    assert np.array_equal(lazy[Q:], cascade[Q:])          # c1 never depends on c0
+    assert not np.array_equal(lazy[:Q], cascade[:Q])      # c0 rounds once instead of once per level
+    dec = c.decode(c.decrypt(lazy, l, s), l, n, Dl)
+    assert np.max(np.abs(dec - sum(np.roll(z, -k) for k in range(terms)))) < 1e-7
