@@ -45,13 +45,15 @@ def parse():
     ap.add_argument("--lt-count", type=int, default=32, help="linear transforms of the BSGS workload (0 = skip)")
     ap.add_argument("--serve-requests", type=int, default=200, help="requests per serving session (0 = skip)")
     ap.add_argument("--serve-rates", type=str, default="50,150,400", help="arrival rates (requests/s) of the serving sweep")
-    ap.add_argument("--copy-mb", type=int, default=256, help="device copy of this size first (counter calibration; 0 = skip)")
+    ap.add_argument("--copy-mb", type=int, default=1024, help="device copy of this size first (counter calibration; 0 = skip)")
     ap.add_argument("--no-bootstrap", action="store_true", help="skip the full-slot bootstrap at N=2^16")
     ap.add_argument("--tune", action="append", default=[], metavar="NAME=VALUE", help="set a context tunable (repeatable)")
     ap.add_argument("--rotation-only", action="store_true",
                     help="time the batched rotation step only (for profiler runs: every dispatch after set-up belongs to it)")
     ap.add_argument("--stream-only", action="store_true",
                     help="run the mixed-level op stream only and print its result (for profiler runs)")
+    ap.add_argument("--ntt-only", action="store_true",
+                    help="time forward + inverse NTTs over the batch's 64 x 2 x 25 limbs only and print the result")
     ap.add_argument("--lt-only", action="store_true",
                     help="run the BSGS linear-transform workload only and print its result (for profiler runs)")
     ap.add_argument("--traffic-file", type=str, default=os.path.join(ROOT, "profiles", "traffic_rotation_n65536_l25.txt"),
@@ -100,6 +102,13 @@ def main():
         print(json.dumps({"uneven_stream": uneven_stream(fh, ctx, keys, gals, a)}))
         ctx.close()
         return
+    if a.ntt_only:
+        for kv in a.tune:
+            name, value = kv.split("=", 1)
+            ctx.tune_set(name, int(value))
+        print(json.dumps({"ntt": ntt_run(fh, ctx, a)}))
+        ctx.close()
+        return
     autotune = ctx.autotune(a.autotune_polys) if a.autotune_polys > 0 else None
     for kv in a.tune:
         name, value = kv.split("=", 1)
@@ -112,7 +121,7 @@ def main():
     rng = np.random.default_rng(a.seed)
 
     # ---- a plain device copy of known size: the reference point that scales hardware byte counters ----
-    copy_gbs = None
+    copy_gbs = copy_dma_gbs = None
     if a.copy_mb > 0:
         words = a.copy_mb * (1 << 17)
         src, dst = ctx.array((words,)), ctx.array((words,))
@@ -122,7 +131,17 @@ def main():
         t0 = time.perf_counter()
         lib.fh_copy_d2d(ctx.h, dst.ptr, src.ptr, words * 8)
         ctx.sync()
-        copy_gbs = round(2 * words * 8 / (time.perf_counter() - t0) / 1e9, 1)
+        copy_dma_gbs = round(2 * words * 8 / (time.perf_counter() - t0) / 1e9, 1)
+        # STREAM copy kernel: read + write of the buffer, best of a few timed runs
+        fh._check(lib.fh_stream_copy(ctx.h, dst.ptr, src.ptr, words * 8))
+        ctx.sync()
+        best = 1e30
+        for _ in range(5):
+            t0 = time.perf_counter()
+            fh._check(lib.fh_stream_copy(ctx.h, dst.ptr, src.ptr, words * 8))
+            ctx.sync()
+            best = min(best, time.perf_counter() - t0)
+        copy_gbs = round(2 * words * 8 / best / 1e9, 1)
         src.free()
         dst.free()
 
@@ -179,7 +198,7 @@ def main():
     ms_per_step = 1000.0 * t_batched / a.steps
     rot_per_s = B * a.steps / t_batched
 
-    per_op_rot_s = ntt_gbs = t_ntt = mult_rescale_s = None
+    per_op_rot_s = ntt_gbs = t_ntt = mult_rescale_s = mult_rescale_batched_s = None
     pcie = None
     nprof = {}
     if not a.rotation_only:
@@ -245,6 +264,30 @@ def main():
             ctx.rescale(prod, lvl, 2, rs)
         ctx.sync()
         mult_rescale_s = nmul / (time.perf_counter() - t0)
+        # the same products as one batch: fh_eval_mult_many (one tensor launch, one ModUp / key inner
+        # product / ModDown for all) followed by one Rescale over every component
+        nb = min(B // 2, 32)
+        prods = ctx.array((nb * 2 * lq, N))
+        rss = ctx.array((nb * 2 * (lq - 1), N))
+        lo = _view(fh, ctx, cts, 0, (nb * 2 * lq, N))
+        hi = _view(fh, ctx, cts, nb * ct_words, (nb * 2 * lq, N))
+        def mult_batch():
+            ctx.eval_mult_many(nb, lo, hi, lvl, dev_keys[0], prods)
+            for j in range(0, 2 * nb, 8):   # Rescale takes up to 8 components per call
+                nc = min(8, 2 * nb - j)
+                ctx.rescale(_view(fh, ctx, prods, j * lq * N, (nc * lq, N)), lvl, nc,
+                            _view(fh, ctx, rss, j * (lq - 1) * N, (nc * (lq - 1), N)))
+
+        mult_batch()
+        ctx.sync()
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            mult_batch()
+        ctx.sync()
+        mult_rescale_batched_s = reps * nb / (time.perf_counter() - t0)
+        prods.free()
+        rss.free()
 
     # ---- roofline of the dominant kernel of the timed rotation step ----
     dom = max(prof, key=lambda f: prof[f][1])
@@ -301,6 +344,7 @@ def main():
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic (uniform residues and keys generated on the device)",
+        "seed": a.seed,
         "config": {"workload": "batched hybrid key-switch rotation, N=2^16, 25 limbs, dnum=3, K=9, 64 ciphertexts, 16 keyed steps",
                    "ring_dim": N, "limbs": lq, "dnum": DNUM, "special_primes": K, "batch": B, "keyed_steps": len(STEPS16),
                    "tunables": tunables, "autotune": autotune},
@@ -315,7 +359,10 @@ def main():
         "ntt_ms_fwd_inv": round(1000 * t_ntt, 3) if t_ntt else None,
         "ntt_avg_launch_ms": ntt_kernel_ms,
         "eval_mult_rescale_per_s": round(mult_rescale_s, 1) if mult_rescale_s else None,
+        "eval_mult_rescale_batched_per_s": round(mult_rescale_batched_s, 1) if mult_rescale_batched_s else None,
         "device_copy_gbps": copy_gbs,
+        "device_copy_source": "STREAM copy kernel (fh_stream_copy), bytes read + written, best of 5" if copy_gbs else None,
+        "device_copy_dma_gbps": copy_dma_gbs,
         "uneven_stream": stream,
         "linear_transform": lt,
         "serving": serving,
@@ -325,6 +372,35 @@ def main():
     }
     print(json.dumps(line))
     ctx.close()
+
+
+def ntt_run(fh, ctx, a):
+    """Forward + inverse NTT over 64 x 2 x 25 limbs (16N algorithmic bytes per limb-transform)."""
+    N, B, lq = ctx.N, a.batch, L1
+    qids = list(range(lq))
+    buf = ctx.array((B * 2 * lq, N))
+    ctx.fill_uniform(buf, qids, num_polys=2 * B, seed=a.seed)
+    ctx.ntt_forward(buf, qids, num_polys=2 * B)
+    ctx.ntt_inverse(buf, qids, num_polys=2 * B)
+    ctx.sync()
+    nrep = max(1, a.steps)
+    ctx.profile(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(nrep):
+        ctx.ntt_forward(buf, qids, num_polys=2 * B)
+        ctx.ntt_inverse(buf, qids, num_polys=2 * B)
+    ctx.sync()
+    t = (time.perf_counter() - t0) / nrep
+    fams = ("ntt_fwd_cols", "ntt_fwd_rows", "ntt_inv_rows", "ntt_inv_cols", "ntt_fwd_team", "ntt_inv_team")
+    prof = {f: ctx.profile_read(f) for f in fams}
+    ctx.profile(False)
+    buf.free()
+    limbs = 2 * B * lq
+    gbs = 2 * limbs * 16 * N / t / 1e9
+    return {"limb_transforms": 2 * limbs, "ms_fwd_inv": round(1000 * t, 3), "gbps": round(gbs, 1),
+            "roofline_frac": round(gbs / HBM_PEAK_GBS, 4), "team": ctx.tune_get("ntt_team"),
+            "avg_launch_ms": {f: round(v[1] / v[0], 4) for f, v in prof.items() if v[0]}}
 
 
 def read_traffic(path):
@@ -670,8 +746,31 @@ def cpu_baseline(a):
         n += 1
     dt = time.perf_counter() - t0
     threads = O.lib().orc_num_threads()
+
+    def rate(fn, budget, cap):
+        fn()  # warm-up
+        m, t = 0, time.perf_counter()
+        while m < cap and time.perf_counter() - t < budget:
+            fn()
+            m += 1
+        return m, time.perf_counter() - t
+
+    # the other op classes of the single-op workload, each on a bounded sample
+    ids = list(range(L1))
+    poly2 = ct.copy()
+    m_ntt, t_ntt = rate(lambda: o.intt(o.ntt(poly2, ids * 2), ids * 2), a.cpu_seconds / 3, 64)
+    ct2 = np.concatenate([o.random_poly(4, ids), o.random_poly(5, ids)])
+    m_mul, t_mul = rate(lambda: o.rescale_ct(o.eval_mult(ct, ct2, lvl, key), lvl), a.cpu_seconds / 3, 64)
+    per_class = {
+        "ntt_limb_transforms_per_s": round(2 * 2 * L1 * m_ntt / t_ntt, 2),
+        "eval_mult_rescale_per_s": round(m_mul / t_mul, 3),
+        "rotations_per_s": round(n / dt, 3),
+        "samples": f"{m_ntt} forward+inverse NTTs of a 2 x 25-limb ciphertext ({t_ntt:.1f} s), {m_mul} EvalMult+Rescale at level 24 "
+                   f"({t_mul:.1f} s), {n} rotations ({dt:.1f} s)",
+    }
     return {"value": round(n / dt, 3), "unit": "rotations/s", "cores": threads, "kind": "port",
-            "sample": f"{n} rotations of one N=2^16, 25-limb ciphertext (dnum=3, K=9) by the C oracle, {dt:.1f} s"}
+            "sample": f"{n} rotations of one N=2^16, 25-limb ciphertext (dnum=3, K=9) by the C oracle, {dt:.1f} s",
+            "per_op_class": per_class}
 
 
 if __name__ == "__main__":

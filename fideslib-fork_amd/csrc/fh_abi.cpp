@@ -68,6 +68,12 @@ void check_level(const fh::Context& c, int level) {
   if (level < 0 || level >= c.L1) throw fh::Error(FH_E_INVALID, "level out of range");
 }
 
+// An automorphism index must be a unit modulo 2N (odd) below 2N: anything else is not a bijection of
+// the slots, and the key inner product's scatter would leave outputs unwritten.
+void check_galois(const fh::Context& c, uint32_t g, const char* what) {
+  if ((g & 1) == 0 || g >= 2 * c.N) throw fh::Error(FH_E_INVALID, std::string(what) + ": galois index must be odd and < 2N");
+}
+
 }  // namespace
 
 extern "C" {
@@ -150,7 +156,17 @@ int fh_context_moduli(const fh_context* ctx, uint64_t* moduli, uint64_t* psi) {
 }
 
 int fh_synchronize(fh_context* ctx) {
-  return guard([&] { FH_HIP(hipStreamSynchronize(C(ctx).stream)); });
+  return guard([&] {
+    auto& c = C(ctx);
+    FH_HIP(hipStreamSynchronize(c.stream));
+    // a team NTT launch whose barrier wait ran past its bound left its results incomplete
+    uint32_t err = 0;
+    FH_HIP(hipMemcpy(&err, c.team_err(), sizeof(err), hipMemcpyDeviceToHost));
+    if (err) {
+      FH_HIP(hipMemset(c.team_err(), 0, sizeof(err)));
+      throw fh::Error(FH_E_DEVICE, "a team NTT barrier wait timed out: results since the last synchronization are invalid");
+    }
+  });
 }
 
 int fh_device_alloc(fh_context* ctx, size_t bytes, void** out) {
@@ -192,6 +208,14 @@ int fh_copy_d2d(fh_context* ctx, void* dst, const void* src, size_t bytes) {
     auto& c = C(ctx);
     need(dst, "dst"); need(src, "src");
     FH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c.stream));
+  });
+}
+
+int fh_stream_copy(fh_context* ctx, void* dst, const void* src, size_t bytes) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(dst, "dst"); need(src, "src");
+    fh::stream_copy(c, dst, src, bytes, c.stream);
   });
 }
 
@@ -494,7 +518,10 @@ int fh_rotate_batch(fh_context* ctx, int count, const uint64_t* cts, size_t ct_s
     check_level(c, level);
     if (count < 0) throw fh::Error(FH_E_INVALID, "count must be >= 0");
     std::vector<const uint64_t*> kp(count);
-    for (int i = 0; i < count; ++i) kp[i] = key_of(c, keys[i]);
+    for (int i = 0; i < count; ++i) {
+      kp[i] = key_of(c, keys[i]);
+      check_galois(c, galois[i], "fh_rotate_batch");
+    }
     fh::rotate_batch(c, count, cts, ct_stride, level, kp.data(), galois, true, true, out, c.stream);
   });
 }
@@ -507,7 +534,10 @@ int fh_rotate_hoisted(fh_context* ctx, const uint64_t* ct, int level, int n, con
     check_level(c, level);
     if (n < 0) throw fh::Error(FH_E_INVALID, "n must be >= 0");
     std::vector<const uint64_t*> kp(n);
-    for (int i = 0; i < n; ++i) kp[i] = key_of(c, keys[i]);
+    for (int i = 0; i < n; ++i) {
+      kp[i] = key_of(c, keys[i]);
+      check_galois(c, galois[i], "fh_rotate_hoisted");
+    }
     fh::rotate_hoisted(c, ct, level, n, kp.data(), galois, add_c0 != 0, moddown != 0, out, c.stream);
   });
 }
@@ -529,13 +559,18 @@ int fh_linear_transform(fh_context* ctx, const uint64_t* ct, int level, int row_
       need(baby_galois, "baby_galois");
       bk[i] = key_of(c, baby_keys[i]);
       bg[i] = baby_galois[i];
+      check_galois(c, bg[i], "baby step");
     }
     const uint64_t* gk = giant_key ? key_of(c, giant_key) : nullptr;
     const uint64_t* ok = offset_key ? key_of(c, offset_key) : nullptr;
+    if (gk) check_galois(c, giant_galois, "giant step");
+    if (ok) check_galois(c, offset_galois, "offset");
     fh::linear_transform(c, ct, level, row_size, b_step, stride, offset, slots, diags, bk.data(), bg.data(), gk, giant_galois, ok,
                          offset_galois, out, c.stream);
   });
 }
+
+}  // extern "C"
 
 namespace {
 // fold rotation indices; throws on an invalid radix or stride
@@ -553,6 +588,8 @@ std::vector<int32_t> fold_indices(int radix, int stride, int size, int start) {
   return v;
 }
 }  // namespace
+
+extern "C" {
 
 int fh_accumulate_indices(int radix, int stride, int size, int start, int32_t* out, int capacity) {
   int n = 0;
@@ -626,9 +663,12 @@ int fh_linear_transform_batch(fh_context* ctx, int count, const uint64_t* cts, s
       need(baby_galois, "baby_galois");
       bk[i] = key_of(c, baby_keys[i]);
       bg[i] = baby_galois[i];
+      check_galois(c, bg[i], "baby step");
     }
     const uint64_t* gk = giant_key ? key_of(c, giant_key) : nullptr;
     const uint64_t* ok = offset_key ? key_of(c, offset_key) : nullptr;
+    if (gk) check_galois(c, giant_galois, "giant step");
+    if (ok) check_galois(c, offset_galois, "offset");
     fh::linear_transform_batch(c, count, cts, ct_stride, level, row_size, b_step, stride, offset, slots, diags, bk.data(), bg.data(), gk,
                                giant_galois, ok, offset_galois, out, c.stream);
   });
@@ -779,6 +819,8 @@ static int* tune_slot(fh::Tune& t, const char* name, int* min_value = nullptr) {
   else if (!std::strcmp(name, "ntt_split")) slot = &t.ntt_split, lo = 0;
   else if (!std::strcmp(name, "rot_streams")) slot = &t.rot_streams;
   else if (!std::strcmp(name, "ntt_streams")) slot = &t.ntt_streams;
+  else if (!std::strcmp(name, "ntt_team")) slot = &t.ntt_team, lo = 0;
+  else if (!std::strcmp(name, "ntt_team_per_xcd")) slot = &t.ntt_team_per_xcd;
   for (int i = 0; i < 4 && !slot; ++i)
     if (!std::strcmp(name, kThreadNames[i])) slot = &t.ntt_threads[i], lo = 256;
   if (!slot) throw fh::Error(FH_E_INVALID, std::string("unknown tunable: ") + name);
@@ -795,6 +837,8 @@ int fh_tune_set(fh_context* ctx, const char* name, int value) {
     if (!std::strcmp(name, "ntt_mix") && value > 1) throw fh::Error(FH_E_INVALID, "ntt_mix must be 0 or 1");
     if (!std::strcmp(name, "rot_streams") && value > 2) throw fh::Error(FH_E_INVALID, "rot_streams must be 1 or 2");
     if (!std::strcmp(name, "ntt_streams") && value > 2) throw fh::Error(FH_E_INVALID, "ntt_streams must be 1 or 2");
+    if (!std::strcmp(name, "ntt_team") && value > 1) throw fh::Error(FH_E_INVALID, "ntt_team must be 0 or 1");
+    if (!std::strcmp(name, "ntt_team_per_xcd") && value > 64) throw fh::Error(FH_E_INVALID, "ntt_team_per_xcd must be 1..64");
     if (!std::strcmp(name, "ntt_split") && value != 0 && !fh::ntt_split_ok(c.logN, value))
       throw fh::Error(FH_E_INVALID, "ntt_split " + std::to_string(value) + " is not built for logN " + std::to_string(c.logN));
     if (lo == 256 && value != 256 && value != 512) throw fh::Error(FH_E_INVALID, std::string(name) + " must be 256 or 512");

@@ -126,6 +126,11 @@ struct Tune {
   int rot_streams = 1;
   // 2: the chunks of a chunked NTT (ntt_chunk_mb below the batch) alternate between two streams
   int ntt_streams = 1;
+  // 1: NTTs with equal column and row lengths run as one launch of workgroup teams (one HBM round
+  // trip per limb: the intermediate stays in the L2 of the team's XCD); 0: two launches
+  int ntt_team = 0;
+  // teams per XCD of a team launch (each team holds one limb of intermediate in its XCD's L2)
+  int ntt_team_per_xcd = 4;
 };
 
 struct Context;
@@ -206,6 +211,15 @@ struct Context {
 
   // host copies of the conversion constants (for table checks and host helpers)
   std::vector<uint64_t> h_pmod, h_pinv;
+
+  // team NTT launches: arrival counters [kMaxTeams], the XCD of every member [kMaxTeams][16] and an
+  // error word (a barrier wait ran past its bound), zeroed at creation
+  static constexpr int kMaxTeams = 1024;
+  uint32_t* d_team = nullptr;
+  uint32_t* team_ctr() const { return d_team; }
+  uint32_t* team_xcc() const { return d_team + kMaxTeams; }
+  uint32_t* team_err() const { return d_team + kMaxTeams * 17; }
+  int cu_count = 0;
 
   hipStream_t stream = nullptr;
   // second stream (default priority) and the fork / join events of two-stream work

@@ -167,6 +167,18 @@ __global__ __launch_bounds__(kEwThreads) void k_fill_uniform(LimbSet s, const Pr
   }
 }
 
+// STREAM copy (b[i] = a[i]) in 16-byte words, each lane moving 4 of them per iteration: the
+// achievable HBM bandwidth the kernels are compared against (a DMA-engine hipMemcpy reaches less).
+__global__ __launch_bounds__(256) void k_stream_copy(const uint4* __restrict__ a, uint4* __restrict__ b, size_t n16) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n16; i += 4 * stride) {
+    const uint4 x0 = a[i], x1 = a[i + stride], x2 = a[i + 2 * stride], x3 = a[i + 3 * stride];
+    b[i] = x0; b[i + stride] = x1; b[i + 2 * stride] = x2; b[i + 3 * stride] = x3;
+  }
+  for (; i < n16; i += stride) b[i] = a[i];
+}
+
 inline dim3 ew_grid(uint32_t N, uint32_t nlimbs) {
   uint32_t per_block = 2 * kEwThreads;
   uint32_t gx = (N + per_block - 1) / per_block;
@@ -257,6 +269,17 @@ void to_montgomery(const Context& c, const LimbSet& s, hipStream_t st) {
   uint32_t n = s.nlimbs * s.npolys;
   if (!n) return;
   k_to_mont<<<ew_grid(c.N, n), kEwThreads, 0, st>>>(s, c.d_prime, c.N);
+  FH_HIP(hipGetLastError());
+}
+
+void stream_copy(const Context& c, void* dst, const void* src, size_t bytes, hipStream_t st) {
+  if (!bytes) return;
+  if ((bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15)) throw Error(-1, "stream copy: 16-byte aligned sizes and pointers only");
+  const size_t n16 = bytes / 16;
+  // 8 workgroups of 256 lanes per CU (256 CUs), each lane with 4 loads in flight
+  const size_t g = std::min<size_t>(2048, (n16 + 255) / 256);
+  ProfScope ps_(c, "stream_copy", st);
+  k_stream_copy<<<(unsigned)g, 256, 0, st>>>(static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16);
   FH_HIP(hipGetLastError());
 }
 

@@ -175,12 +175,16 @@ void encode_slots(Context& c, const double* values, int count, uint32_t slots, d
   if ((uint32_t)count != out.npolys) throw Error(-1, "encode: output set and count differ");
   if (count > 65535 || (size_t)out.nlimbs * count > 65535) throw Error(-1, "encode: too many plaintexts in one call");
   if (!(scale > 0) || !std::isfinite(scale)) throw Error(-1, "encode: the scale must be positive and finite");
-  // every coefficient is at most max |z| in magnitude (the inverse transform averages); llround
-  // needs |coefficient x scale| < 2^63. One comparison per value also rejects NaN and infinity.
-  const double lim = 0x1p63 / scale;
+  // Every coefficient is at most max |z| in magnitude (the inverse transform averages, up to a
+  // rounding error of a few ulps). Above 2^61 (OpenFHE's MAX_BITS_IN_WORD) OpenFHE's
+  // CKKSPackedEncoding::Encode switches to an approximate path with different bits, and the
+  // reference's device encoder rejects such coefficients (src/CKKS/DeviceEncodeBatch.cu:628-652);
+  // this encoder rounds exactly, so it rejects every input that could reach 2^61: |re| + |im| times
+  // the scale must stay below 2^61 (1 - 2^-40). One comparison per value also rejects NaN and infinity.
+  const double lim = 0x1p61 * (1.0 - 0x1p-40) / scale;
   bool bad = false;
   for (size_t i = 0; i < (size_t)count * slots; ++i) bad |= !(std::fabs(values[2 * i]) + std::fabs(values[2 * i + 1]) < lim);
-  if (bad) throw Error(-1, "encode: a slot value is not finite or times the scale reaches 2^63");
+  if (bad) throw Error(-1, "encode: a slot value is not finite, or times the scale it reaches 2^61 (OpenFHE's MAX_BITS_IN_WORD)");
   ensure_ifft_tables(c);
   const uint32_t n = slots;
   int logn = 0;

@@ -37,6 +37,7 @@
 //    element-wise step that consumes them (ModDown, Rescale).
 #pragma once
 #include <algorithm>
+#include <cstring>
 
 #include "fh_kernels.hpp"
 
@@ -125,6 +126,19 @@ struct FpA {
   }
 };
 
+// Loads of the second pass. PlainLd: ordinary loads (the two-launch form: the first launch's
+// stores are visible to the next launch). L2Ld: loads that skip the CU's L1 (sc0) and are served by
+// the L2 -- the single-launch team form, where the data was written by other workgroups of the same
+// launch after this CU may have cached the addresses.
+struct PlainLd {
+  __device__ __forceinline__ static uint64_t ld(const uint64_t* p) { return *p; }
+};
+struct L2Ld {
+  __device__ __forceinline__ static uint64_t ld(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+};
+
 // per-limb arithmetic selection
 struct Sel {
   bool fp;
@@ -156,11 +170,11 @@ __device__ __forceinline__ Sel select(const NttTables& tb, int pid, bool inverse
 // First l1 stages. Input: canonical (in place or out of place), output: internal form.
 template <int LS, class A, class TwT>
 __device__ __forceinline__ void fwd_cols_body(const A& ar, const TwT* sw, const TwT* swp, const uint64_t* p, uint64_t* po,
-                                              uint64_t* tile, uint32_t N2, int C) {
+                                              uint64_t* tile, uint32_t N2, int C, uint32_t bx) {
   constexpr int S = 1 << LS, T = S / 16, PADS = S + 1;
   const int tid = threadIdx.x;
   const int cc = tid % C, r0 = tid / C;
-  const uint32_t col = blockIdx.x * C + cc;
+  const uint32_t col = bx * C + cc;
   typename A::T x[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) x[k] = ar.from_canon(p[(size_t)(r0 + T * k) * N2 + col]);
@@ -220,10 +234,10 @@ __global__ __launch_bounds__(TH) void k_ntt_fwd_cols(Addr ad, AddrOut ao, NttTab
   if (sel.fp) {
     double* swd = reinterpret_cast<double*>(sw);
     for (int i = threadIdx.x; i < S; i += blockDim.x) swd[i] = sel.twd[i];
-    fwd_cols_body<LS>(sel.fa, (const double*)swd, (const double*)swd, p, po, tile, N2, C);
+    fwd_cols_body<LS>(sel.fa, (const double*)swd, (const double*)swd, p, po, tile, N2, C, blockIdx.x);
   } else {
     for (int i = threadIdx.x; i < S; i += blockDim.x) { sw[i] = sel.tw[i]; swp[i] = sel.twsh[i]; }
-    fwd_cols_body<LS>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, p, po, tile, N2, C);
+    fwd_cols_body<LS>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, p, po, tile, N2, C, blockIdx.x);
   }
 }
 
@@ -245,18 +259,18 @@ struct RowTile {
 // transform of x_c tau_rho^c (c = position in the row): one coalesced twist multiply per element,
 // then stages whose twiddles are the first S2 entries of the table, the same for every row, read
 // from LDS. Input: internal form, output: canonical residues handed to the epilogue.
-template <int LS, class A, class TwT, class Epi>
+template <int LS, class Ld = PlainLd, class A, class TwT, class Epi>
 __device__ __forceinline__ void fwd_rows_body(const A& ar, const TwT* sw, const TwT* swp, const TwT* ts, const TwT* tsp, uint64_t* p,
-                                              uint64_t* tile, int R, const Epi& epi) {
+                                              uint64_t* tile, int R, const Epi& epi, uint32_t bx) {
   constexpr int S = 1 << LS, T = S / 16, PADS = RowTile<LS>::PAD;
   const int tid = threadIdx.x;
   const int rr = tid / T, c0 = tid % T;
-  const uint32_t rho = blockIdx.x * R + rr;
+  const uint32_t rho = bx * R + rr;
   uint64_t* row = p + (size_t)rho * S;
   const uint32_t rbase = rho * S;
   typename A::T x[16];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(row[c0 + T * k]);
+  for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(Ld::ld(row + c0 + T * k));
 #pragma unroll
   for (int k = 0; k < 16; ++k) x[k] = ar.mul(x[k], ar.tw(ts + rbase, tsp + rbase, c0 + T * k));
 #pragma unroll
@@ -374,11 +388,11 @@ __global__ __launch_bounds__(TH) void k_ntt_fwd_rows(Addr ad, NttTables tb, Epi 
     double* swd = reinterpret_cast<double*>(sw);
     for (int i = threadIdx.x; i < S; i += blockDim.x) swd[i] = sel.twd[i];
     __syncthreads();
-    fwd_rows_body<LS>(sel.fa, (const double*)swd, (const double*)swd, sel.tsd, sel.tsd, p, tile, R, epi);
+    fwd_rows_body<LS>(sel.fa, (const double*)swd, (const double*)swd, sel.tsd, sel.tsd, p, tile, R, epi, blockIdx.x);
   } else {
     for (int i = threadIdx.x; i < S; i += blockDim.x) { sw[i] = sel.tw[i]; swp[i] = sel.twsh[i]; }
     __syncthreads();
-    fwd_rows_body<LS>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, sel.ts, sel.tssh, p, tile, R, epi);
+    fwd_rows_body<LS>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, sel.ts, sel.tssh, p, tile, R, epi, blockIdx.x);
   }
 }
 
@@ -389,11 +403,11 @@ __global__ __launch_bounds__(TH) void k_ntt_fwd_rows(Addr ad, NttTables tb, Epi 
 // re-centred after every second stage (Gentleman-Sande sums double per stage).
 template <int LS, class A, class TwT>
 __device__ __forceinline__ void inv_rows_body(const A& ar, const TwT* sw, const TwT* swp, const TwT* ts, const TwT* tsp, const uint64_t* p,
-                                              uint64_t* po, uint64_t* tile, int R) {
+                                              uint64_t* po, uint64_t* tile, int R, uint32_t bx) {
   constexpr int S = 1 << LS, T = S / 16, PADS = RowTile<LS>::PAD;
   const int tid = threadIdx.x;
   const int rr = tid / T, c0 = tid % T;
-  const uint32_t rho = blockIdx.x * R + rr;
+  const uint32_t rho = bx * R + rr;
   const uint64_t* row = p + (size_t)rho * S;
   uint64_t* rowo = po + (size_t)rho * S;
   const uint32_t rbase = rho * S;
@@ -464,29 +478,29 @@ __global__ __launch_bounds__(TH) void k_ntt_inv_rows(Addr ad, AddrOut ao, NttTab
     double* swd = reinterpret_cast<double*>(sw);
     for (int i = threadIdx.x; i < S; i += blockDim.x) swd[i] = sel.twd[i];
     __syncthreads();
-    inv_rows_body<LS>(sel.fa, (const double*)swd, (const double*)swd, sel.tsd, sel.tsd, p, po, tile, R);
+    inv_rows_body<LS>(sel.fa, (const double*)swd, (const double*)swd, sel.tsd, sel.tsd, p, po, tile, R, blockIdx.x);
   } else {
     for (int i = threadIdx.x; i < S; i += blockDim.x) { sw[i] = sel.tw[i]; swp[i] = sel.twsh[i]; }
     __syncthreads();
-    inv_rows_body<LS>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, sel.ts, sel.tssh, p, po, tile, R);
+    inv_rows_body<LS>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, sel.ts, sel.tssh, p, po, tile, R, blockIdx.x);
   }
 }
 
 // ------------------------------------------------------------------------------- inverse, columns
 // Second launch of the INTT (first l1 stages, reversed) with the N^-1 scaling. Input: internal
 // form, output: canonical residues.
-template <int LS, class A, class TwT, class Fin>
+template <int LS, class Ld = PlainLd, class A, class TwT, class Fin>
 __device__ __forceinline__ void inv_cols_body(const A& ar, const TwT* sw, const TwT* swp, uint64_t* p, uint64_t* tile, uint32_t N2,
-                                              int C, const Fin& fin) {
+                                              int C, const Fin& fin, uint32_t bx) {
   constexpr int S = 1 << LS, T = S / 16, PADS = S + 1;
   const int tid = threadIdx.x;
   const int cc = tid % C, r0 = tid / C;
-  const uint32_t col = blockIdx.x * C + cc;
+  const uint32_t col = bx * C + cc;
   typename A::T x[16];
   uint64_t* t = tile + cc * PADS;
   if constexpr (LS > 4) {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(p[(size_t)(16 * r0 + k) * N2 + col]);
+    for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(Ld::ld(p + (size_t)(16 * r0 + k) * N2 + col));
     __syncthreads();
 #pragma unroll
     for (int s = LS - 1; s >= 4; --s) {
@@ -512,7 +526,7 @@ __device__ __forceinline__ void inv_cols_body(const A& ar, const TwT* sw, const 
     for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(t[r0 + T * k]);
   } else {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(p[(size_t)(r0 + T * k) * N2 + col]);
+    for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(Ld::ld(p + (size_t)(r0 + T * k) * N2 + col));
     __syncthreads();
   }
 #pragma unroll
@@ -562,10 +576,177 @@ __global__ __launch_bounds__(TH) void k_ntt_inv_cols(Addr ad, NttTables tb) {
   if (sel.fp) {
     double* swd = reinterpret_cast<double*>(sw);
     for (int i = threadIdx.x; i < S; i += blockDim.x) swd[i] = sel.twd[i];
-    inv_cols_body<LS>(sel.fa, (const double*)swd, (const double*)swd, p, tile, N2, C, FpFin{sel.fa, P.ninv_d});
+    inv_cols_body<LS>(sel.fa, (const double*)swd, (const double*)swd, p, tile, N2, C, FpFin{sel.fa, P.ninv_d}, blockIdx.x);
   } else {
     for (int i = threadIdx.x; i < S; i += blockDim.x) { sw[i] = sel.tw[i]; swp[i] = sel.twsh[i]; }
-    inv_cols_body<LS>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, p, tile, N2, C, IntFin{sel.ia, P.ninv, P.ninv_sh});
+    inv_cols_body<LS>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, p, tile, N2, C, IntFin{sel.ia, P.ninv, P.ninv_sh}, blockIdx.x);
+  }
+}
+
+// ------------------------------------------------------------------------------- team form
+// One launch transforms every limb with a single HBM round trip. For N = S x S, a limb is split
+// into W = S / C column tiles and W row tiles; a team of W workgroups owns one limb at a time:
+// member m runs the first pass on tile m and stores the intermediate in the output limb, the team
+// meets at a barrier, and member m finishes row tile m (forward; the inverse runs rows first).
+// The intermediate (8N bytes per limb, a few teams per XCD) stays in the L2 of the team's XCD
+// between the passes, so HBM sees only the input read and the output write.
+//
+// Members of a team are workgroups b = x + 8 (t W + m) (the same b mod 8). Workgroups are dispatched
+// to the XCDs round-robin, so a team normally shares one XCD and its L2, the point of coherence of
+// all of that XCD's CUs: a store acknowledged to the wave (s_waitcnt vmcnt(0)) is visible to every
+// CU of the XCD, and the second pass reads with loads that skip the CU's L1 (L2Ld). Whether a team
+// does share an XCD is checked, not assumed: every member records its XCC id (hardware register
+// XCC_ID) at launch, and a team whose members differ synchronizes with agent-scope release /
+// acquire fences instead (L2 write-back and invalidate), which is correct on any placement.
+// The launch is cooperative (all workgroups resident), every barrier wait is bounded, and a wait
+// that runs past its bound sets the error word and makes every team leave the launch.
+constexpr uint32_t kTeamSpinLimit = 1u << 20;
+
+struct TeamArgs {
+  uint32_t* ctr;      // [teams] arrival counters, zero at launch
+  uint32_t* xcc;      // [teams][W] XCC id of each member
+  uint32_t* err;      // error word
+  uint32_t njobs, nteams, W;
+};
+
+__device__ __forceinline__ uint32_t xcc_id() { return (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | 20) & 0xF; }  // HW_REG_XCC_ID[3:0]
+
+__device__ __forceinline__ uint32_t team_poll(uint32_t* p) { return __hip_atomic_fetch_add(p, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// All W members arrive (target = W times the number of barriers so far); false when the launch is
+// being abandoned. formal: agent-scope fences around the barrier (members on different XCDs).
+__device__ __forceinline__ bool team_barrier(const TeamArgs& ta, uint32_t team, uint32_t target, bool formal, int* sflag) {
+  if (formal) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  else __builtin_amdgcn_s_waitcnt(0);   // this wave's stores are acknowledged by the L2
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(ta.ctr + team, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int ok = 1;
+    for (uint32_t spins = 0; team_poll(ta.ctr + team) < target; ++spins) {
+      if ((spins & 63) == 63 && (spins > kTeamSpinLimit || team_poll(ta.err))) {
+        __hip_atomic_fetch_or(ta.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    *sflag = ok;
+  }
+  __syncthreads();
+  const bool ok = *sflag != 0;
+  if (formal) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return ok;
+}
+
+// A copy of v the compiler cannot compute before this point: work addressed through it (a pass's
+// element addresses) is not hoisted above the preceding barrier, where it would hold registers
+// across the other pass.
+__device__ __forceinline__ uint32_t opaque(uint32_t v) {
+  asm volatile("" : "+s"(v));
+  return v;
+}
+
+// Team identity of this workgroup and the launch-time placement check; returns false to leave.
+__device__ __forceinline__ bool team_setup(const TeamArgs& ta, uint32_t& team, uint32_t& m, bool& formal, int* sflag) {
+  const uint32_t b = blockIdx.x, j = b / 8;
+  team = (b % 8) + 8 * (j / ta.W);
+  m = j % ta.W;
+  if (threadIdx.x == 0) ta.xcc[team * ta.W + m] = xcc_id();
+  if (!team_barrier(ta, team, ta.W, true, sflag)) return false;
+  __syncthreads();   // every thread has read the barrier's flag
+  if (threadIdx.x == 0) {
+    int same = 1;
+    const uint32_t x0 = ta.xcc[team * ta.W];
+    for (uint32_t i = 1; i < ta.W; ++i) same &= ta.xcc[team * ta.W + i] == x0;
+    *sflag = same;
+  }
+  __syncthreads();
+  formal = *sflag == 0;
+  __syncthreads();
+  return true;
+}
+
+// One launch per arithmetic (FP: the fp64 path of the 50-bit primes, else integer Shoup): a team
+// skips the limbs of the other kind, and each kernel holds one path's registers only (both paths in
+// one kernel double its register use).
+template <int LS, int TH, bool FP, class Addr, class AddrOut, class Epi>
+__global__ __launch_bounds__(TH) void k_ntt_fwd_team(Addr ad, AddrOut ao, NttTables tb, Epi epi, TeamArgs ta) {
+  constexpr int S = 1 << LS, T = S / 16, CMAX = TH / T;
+  constexpr int TW = CMAX * RowTile<LS>::PAD > CMAX * (S + 1) ? CMAX * RowTile<LS>::PAD : CMAX * (S + 1);
+  __shared__ uint64_t tile[TW];
+  __shared__ uint64_t sw[S], swp[S];
+  __shared__ int sflag;
+  uint32_t team, m;
+  bool formal;
+  if (!team_setup(ta, team, m, formal, &sflag)) return;
+  const uint32_t N2 = tb.N >> LS;
+  // tile width as a run-time value (= CMAX): a compile-time width lets the compiler precompute all
+  // 32 element addresses of a pass up front, which doubles the kernel's register use
+  const int C = (int)(blockDim.x / T);
+  uint32_t nb = 1;   // barriers passed
+  for (uint32_t y = team; y < ta.njobs; y += ta.nteams) {
+    uint64_t *p, *po;
+    int pid, pido;
+    ad.at(y, tb.N, p, pid);
+    if ((tb.prime[pid].fp64 != 0) != FP) continue;   // the other launch's limb (uniform across the team)
+    ao.at(y, tb.N, po, pido);
+    Epi e = epi;
+    e.init(y, tb.N, tb.prime);
+    const Sel sel = select(tb, pid, false);
+    __syncthreads();   // the previous limb's readers of sw / tile are done
+    if constexpr (FP) {
+      double* swd = reinterpret_cast<double*>(sw);
+      for (int i = threadIdx.x; i < S; i += TH) swd[i] = sel.twd[i];
+      __syncthreads();
+      fwd_cols_body<LS>(sel.fa, (const double*)swd, (const double*)swd, p, po, tile, N2, C, opaque(m));
+      if (!team_barrier(ta, team, ta.W * ++nb, formal, &sflag)) return;
+      fwd_rows_body<LS, L2Ld>(sel.fa, (const double*)swd, (const double*)swd, sel.tsd, sel.tsd, po, tile, C, e, opaque(m));
+    } else {
+      for (int i = threadIdx.x; i < S; i += TH) { sw[i] = sel.tw[i]; swp[i] = sel.twsh[i]; }
+      __syncthreads();
+      fwd_cols_body<LS>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, p, po, tile, N2, C, opaque(m));
+      if (!team_barrier(ta, team, ta.W * ++nb, formal, &sflag)) return;
+      fwd_rows_body<LS, L2Ld>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, sel.ts, sel.tssh, po, tile, C, e, opaque(m));
+    }
+  }
+}
+
+template <int LS, int TH, bool FP, class Addr, class AddrOut>
+__global__ __launch_bounds__(TH) void k_ntt_inv_team(Addr ad, AddrOut ao, NttTables tb, TeamArgs ta) {
+  constexpr int S = 1 << LS, T = S / 16, CMAX = TH / T;
+  constexpr int TW = CMAX * RowTile<LS>::PAD > CMAX * (S + 1) ? CMAX * RowTile<LS>::PAD : CMAX * (S + 1);
+  __shared__ uint64_t tile[TW];
+  __shared__ uint64_t sw[S], swp[S];
+  __shared__ int sflag;
+  uint32_t team, m;
+  bool formal;
+  if (!team_setup(ta, team, m, formal, &sflag)) return;
+  const uint32_t N2 = tb.N >> LS;
+  const int C = (int)(blockDim.x / T);
+  uint32_t nb = 1;
+  for (uint32_t y = team; y < ta.njobs; y += ta.nteams) {
+    uint64_t *p, *po;
+    int pid, pido;
+    ad.at(y, tb.N, p, pid);
+    if ((tb.prime[pid].fp64 != 0) != FP) continue;
+    ao.at(y, tb.N, po, pido);
+    const Sel sel = select(tb, pid, true);
+    const PrimeConst& P = tb.prime[pid];
+    __syncthreads();
+    if constexpr (FP) {
+      double* swd = reinterpret_cast<double*>(sw);
+      for (int i = threadIdx.x; i < S; i += TH) swd[i] = sel.twd[i];
+      __syncthreads();
+      inv_rows_body<LS>(sel.fa, (const double*)swd, (const double*)swd, sel.tsd, sel.tsd, p, po, tile, C, opaque(m));
+      if (!team_barrier(ta, team, ta.W * ++nb, formal, &sflag)) return;
+      inv_cols_body<LS, L2Ld>(sel.fa, (const double*)swd, (const double*)swd, po, tile, N2, C, FpFin{sel.fa, P.ninv_d}, opaque(m));
+    } else {
+      for (int i = threadIdx.x; i < S; i += TH) { sw[i] = sel.tw[i]; swp[i] = sel.twsh[i]; }
+      __syncthreads();
+      inv_rows_body<LS>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, sel.ts, sel.tssh, p, po, tile, C, opaque(m));
+      if (!team_barrier(ta, team, ta.W * ++nb, formal, &sflag)) return;
+      inv_cols_body<LS, L2Ld>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, po, tile, N2, C, IntFin{sel.ia, P.ninv, P.ninv_sh}, opaque(m));
+    }
   }
 }
 
@@ -688,6 +869,90 @@ void launch_inv_cols(const Context& c, const AddrOut& o, uint32_t n, const NttTa
   k_ntt_inv_cols<LS, TH><<<dim3((c.N >> LS) / C, n), b, 0, st>>>(o, tb);
 }
 
+// ---- team launches
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+// the team form applies to N = S x S with whole columns per tile (S >= 64), outside graph capture
+// (the launch is cooperative)
+template <int L1S, int L2S>
+inline bool team_ok(const Context& c, hipStream_t st) {
+  if constexpr (L1S != L2S || L1S < 6) return false;
+  if (!c.tune.ntt_team) return false;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  FH_HIP(hipStreamIsCapturing(st, &cs));
+  return cs == hipStreamCaptureStatusNone;
+}
+
+// team count of a launch: a few teams per XCD (each holds one limb in flight in its L2), no more
+// than the jobs need, and all of the launch's workgroups co-resident (cooperative launch)
+inline bool team_check(const Context& c, uint32_t njobs, uint32_t W, int max_blocks_per_cu, TeamArgs& ta) {
+  const uint32_t per_xcd = (uint32_t)std::max(1, c.tune.ntt_team_per_xcd);
+  uint32_t nteams = std::min<uint32_t>(8 * per_xcd, (njobs + 7) / 8 * 8);
+  const uint32_t resident = (uint32_t)std::max(0, max_blocks_per_cu) * (uint32_t)std::max(0, c.cu_count);
+  nteams = std::min<uint32_t>(nteams, resident / W / 8 * 8);
+  nteams = std::min<uint32_t>(nteams, Context::kMaxTeams);
+  if (nteams < 8) return false;
+  ta = TeamArgs{c.team_ctr(), c.team_xcc(), c.team_err(), njobs, nteams, W};
+  return true;
+}
+
+// arrival counters of the launch back to zero (a fill kernel: capturable, unlike a memset node)
+inline void team_reset(const Context& c, const TeamArgs& ta, hipStream_t st) {
+  k_zero_u32<<<1, 256, 0, st>>>(c.team_ctr(), ta.nteams);
+  FH_HIP(hipGetLastError());
+}
+
+inline const char* team_name(const char* rows_name) {
+  if (!std::strcmp(rows_name, "ntt_fwd_rows_moddown")) return "ntt_fwd_team_moddown";
+  if (!std::strcmp(rows_name, "ntt_fwd_rows_rescale")) return "ntt_fwd_team_rescale";
+  return "ntt_fwd_team";
+}
+
+template <int LS, class Addr, class AddrOut, class Epi>
+bool launch_fwd_team(const Context& c, const Addr& a, const AddrOut& o, uint32_t n, const NttTables& tb, hipStream_t st, const Epi& e,
+                     const char* rows_name) {
+  constexpr int TH = kNttThreads, T = (1 << LS) / 16, C = TH / T;
+  constexpr uint32_t W = (1u << LS) / C;
+  const void* fns[2] = {reinterpret_cast<const void*>(&k_ntt_fwd_team<LS, TH, true, Addr, AddrOut, Epi>),
+                        reinterpret_cast<const void*>(&k_ntt_fwd_team<LS, TH, false, Addr, AddrOut, Epi>)};
+  static int occ[2] = {-1, -1};
+  TeamArgs ta[2];
+  for (int k = 0; k < 2; ++k) {
+    if (occ[k] < 0) FH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ[k], fns[k], TH, 0));
+    if (!team_check(c, n, W, occ[k], ta[k])) return false;
+  }
+  ProfScope ps_(c, team_name(rows_name), st);
+  for (int k = 0; k < 2; ++k) {
+    team_reset(c, ta[k], st);
+    void* args[] = {(void*)&a, (void*)&o, (void*)&tb, (void*)&e, (void*)&ta[k]};
+    FH_HIP(hipLaunchCooperativeKernel(fns[k], dim3(ta[k].nteams * W), dim3(TH), args, 0, st));
+  }
+  return true;
+}
+
+template <int LS, class Addr, class AddrOut>
+bool launch_inv_team(const Context& c, const Addr& a, const AddrOut& o, uint32_t n, const NttTables& tb, hipStream_t st) {
+  constexpr int TH = kNttThreads, T = (1 << LS) / 16, C = TH / T;
+  constexpr uint32_t W = (1u << LS) / C;
+  const void* fns[2] = {reinterpret_cast<const void*>(&k_ntt_inv_team<LS, TH, true, Addr, AddrOut>),
+                        reinterpret_cast<const void*>(&k_ntt_inv_team<LS, TH, false, Addr, AddrOut>)};
+  static int occ[2] = {-1, -1};
+  TeamArgs ta[2];
+  for (int k = 0; k < 2; ++k) {
+    if (occ[k] < 0) FH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ[k], fns[k], TH, 0));
+    if (!team_check(c, n, W, occ[k], ta[k])) return false;
+  }
+  ProfScope ps_(c, "ntt_inv_team", st);
+  for (int k = 0; k < 2; ++k) {
+    team_reset(c, ta[k], st);
+    void* args[] = {(void*)&a, (void*)&o, (void*)&tb, (void*)&ta[k]};
+    FH_HIP(hipLaunchCooperativeKernel(fns[k], dim3(ta[k].nteams * W), dim3(TH), args, 0, st));
+  }
+  return true;
+}
+
 // WIDE: the wide (512-thread) tiles are built for this split (the default split of logN); the
 // per-pass workgroup sizes come from the context's tunables.
 // Chunks of a chunked transform alternate between the caller's stream and the context's second
@@ -718,6 +983,9 @@ struct ChunkStreams {
 template <class Addr, class AddrOut, int L1S, int L2S, bool WIDE, class Epi>
 void fwd_impl(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlimbs, hipStream_t st0, const Epi& epi, const char* rows_name) {
   NttTables tb = ntt_tables(c, L1S);
+  if constexpr (L1S == L2S && L1S >= 6) {
+    if (team_ok<L1S, L2S>(c, st0) && launch_fwd_team<L1S>(c, ad, ao, nlimbs, tb, st0, epi, rows_name)) return;
+  }
   const bool wc = WIDE && c.tune.ntt_threads[0] == kNttWide, wr = WIDE && c.tune.ntt_threads[1] == kNttWide;
   const uint32_t chunk = chunk_limbs(c);
   const ChunkStreams cs(c, st0, chunk, nlimbs);
@@ -743,6 +1011,9 @@ void fwd_impl(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlim
 template <class Addr, class AddrOut, int L1S, int L2S, bool WIDE>
 void inv_impl(const Context& c, const Addr& ad, const AddrOut& ao, uint32_t nlimbs, hipStream_t st0) {
   NttTables tb = ntt_tables(c, L1S);
+  if constexpr (L1S == L2S && L1S >= 6) {
+    if (team_ok<L1S, L2S>(c, st0) && launch_inv_team<L1S>(c, ad, ao, nlimbs, tb, st0)) return;
+  }
   const bool wr = WIDE && c.tune.ntt_threads[2] == kNttWide, wc = WIDE && c.tune.ntt_threads[3] == kNttWide;
   const uint32_t chunk = chunk_limbs(c);
   const ChunkStreams cs(c, st0, chunk, nlimbs);

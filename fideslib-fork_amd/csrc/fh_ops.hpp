@@ -29,6 +29,8 @@ void ew_tensor(const Context& c, const uint64_t* ct1, const uint64_t* ct2, uint6
                int limbs2 = 0);
 // in place x -> x 2^64 mod q (resident key-switching keys are kept in Montgomery form)
 void to_montgomery(const Context& c, const LimbSet& s, hipStream_t st);
+// STREAM copy kernel (bandwidth reference; 16-byte aligned)
+void stream_copy(const Context& c, void* dst, const void* src, size_t bytes, hipStream_t st);
 // uniform residues mod each limb's prime (synthetic inputs)
 void fill_uniform(const Context& c, const LimbSet& s, uint64_t seed, hipStream_t st);
 

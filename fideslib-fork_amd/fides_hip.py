@@ -60,6 +60,7 @@ def _declare(L):
         "fh_encode_coeffs": (I, [P, VP, I, i32p, I, VP, S]),
         "fh_eval_mult_many": (I, [P, I, VP, I, VP, I, I, VP, VP]),
         "fh_rescale": (I, [P, VP, I, I, VP]),
+        "fh_stream_copy": (I, [P, VP, VP, S]),
         "fh_mod_raise": (I, [P, VP, I, I, I, VP]),
         "fh_key_create": (I, [P, u64p, PP]),
         "fh_key_destroy": (I, [P, P]),

@@ -6,6 +6,7 @@
 // A context is stored as its parameters and moduli and rebuilt with GenCryptoContext (the moduli
 // must come out equal); keys and ciphertexts store their canonical residues in the EVALUATION
 // layout of the C ABI.
+#include <cmath>
 #include <cstring>
 #include <fstream>
 #include <istream>
@@ -50,11 +51,22 @@ bool put_device(std::ostream& os, fh_context* ctx, const uint64_t* dev, size_t w
   put_raw(os, h.data(), words * 8);
   return (bool)os;
 }
-std::shared_ptr<DeviceBuffer> get_device(std::istream& is, const CryptoContextImpl& cc, size_t expect) {
+// Every residue must be canonical for its limb's prime: the data is blocks of `block` limbs whose
+// limb j is over moduli[j] (a component of a ciphertext, a key's Q ∪ P block). Non-canonical words
+// would feed the lazy Shoup / Montgomery kernels values outside their input ranges.
+bool canonical(const std::vector<uint64_t>& h, const std::vector<uint64_t>& moduli, size_t block, size_t N) {
+  if (block == 0 || block > moduli.size()) return false;
+  for (size_t w = 0; w < h.size(); ++w)
+    if (h[w] >= moduli[(w / N) % block]) return false;
+  return true;
+}
+
+std::shared_ptr<DeviceBuffer> get_device(std::istream& is, const CryptoContextImpl& cc, size_t expect, size_t block) {
   uint64_t words = 0;
   if (!get_u64(is, words) || words != expect) return nullptr;
   std::vector<uint64_t> h(words);
   if (!get_raw(is, h.data(), words * 8)) return nullptr;
+  if (!canonical(h, cc.GetModuli(), block, cc.GetRingDimension())) return nullptr;
   auto d = cc.AllocateDevice(words);
   if (fh_copy_h2d(cc.Native(), d->ptr, h.data(), words * 8) != FH_OK) return nullptr;
   return d;
@@ -71,11 +83,12 @@ bool put_key(std::ostream& os, fh_context* ctx, const fh_key* k, size_t words) {
   put_raw(os, h.data(), words * 8);
   return (bool)os;
 }
-fh_key* get_key(std::istream& is, fh_context* ctx, size_t expect) {
+fh_key* get_key(std::istream& is, const CryptoContextImpl& cc, fh_context* ctx, size_t expect) {
   uint64_t words = 0;
   if (!get_u64(is, words) || words != expect) return nullptr;
   std::vector<uint64_t> h(words);
   if (!get_raw(is, h.data(), words * 8)) return nullptr;
+  if (!canonical(h, cc.GetModuli(), cc.GetModuli().size(), cc.GetRingDimension())) return nullptr;
   fh_key* k = nullptr;
   if (fh_key_create(ctx, h.data(), &k) != FH_OK) return nullptr;
   return k;
@@ -93,7 +106,7 @@ bool CryptoContextImpl::SerializeEvalMultKey(std::ostream& ser, const SerType& s
 
 bool CryptoContextImpl::DeserializeEvalMultKey(std::istream& ser, const SerType& sertype) {
   if (sertype != SerType::BINARY || !get_header(ser, "MULTKEY")) return false;
-  fh_key* k = get_key(ser, ctx_, key_words(*this));
+  fh_key* k = get_key(ser, *this, ctx_, key_words(*this));
   if (!k) return false;
   if (relin_) fh_key_destroy(ctx_, relin_);
   relin_ = k;
@@ -117,7 +130,7 @@ bool CryptoContextImpl::DeserializeEvalAutomorphismKey(std::istream& ser, const 
   for (uint32_t i = 0; i < n; ++i) {
     uint32_t g = 0;
     if (!get_u32(ser, g) || (g & 1) == 0 || g >= 2 * N_) return false;
-    fh_key* k = get_key(ser, ctx_, key_words(*this));
+    fh_key* k = get_key(ser, *this, ctx_, key_words(*this));
     if (!k) return false;
     auto it = rot_.find(g);
     if (it != rot_.end()) fh_key_destroy(ctx_, it->second);
@@ -184,8 +197,8 @@ bool DeserializeFromFile(const std::string& filename, PublicKey& obj, const SerT
   if (!f || !get_header(f, "PUBKEY")) return false;
   const size_t words = (size_t)cc->GetNumQ() * cc->GetRingDimension();
   auto pk = std::make_shared<PublicKeyImpl>();
-  pk->b = get_device(f, *cc, words);
-  pk->a = pk->b ? get_device(f, *cc, words) : nullptr;
+  pk->b = get_device(f, *cc, words, cc->GetNumQ());
+  pk->a = pk->b ? get_device(f, *cc, words, cc->GetNumQ()) : nullptr;
   if (!pk->a) return false;
   obj = pk;
   return true;
@@ -204,7 +217,7 @@ bool DeserializeFromFile(const std::string& filename, PrivateKey& obj, const Ser
   std::ifstream f(filename, std::ios::binary);
   if (!f || !get_header(f, "SECKEY")) return false;
   auto sk = std::make_shared<PrivateKeyImpl>();
-  sk->s = get_device(f, *cc, cc->GetModuli().size() * (size_t)cc->GetRingDimension());
+  sk->s = get_device(f, *cc, cc->GetModuli().size() * (size_t)cc->GetRingDimension(), cc->GetModuli().size());
   if (!sk->s) return false;
   obj = sk;
   return true;
@@ -233,7 +246,10 @@ bool DeserializeFromFile(const std::string& filename, Ciphertext& obj, const Ser
     return false;
   if (ct->towers < 1 || ct->towers > cc->GetNumQ() || ct->ncomp < 2 || ct->ncomp > 3 || ct->noiseScaleDeg < 1 || ct->noiseScaleDeg > 2)
     return false;
-  ct->data = get_device(f, *cc, (size_t)ct->ncomp * ct->towers * cc->GetRingDimension());
+  // slots: a power of two in [1, N/2]; the scale a finite positive number
+  if (ct->slots < 1 || ct->slots > cc->GetRingDimension() / 2 || (ct->slots & (ct->slots - 1)) != 0) return false;
+  if (!(ct->scalingFactor > 0) || !std::isfinite(ct->scalingFactor)) return false;
+  ct->data = get_device(f, *cc, (size_t)ct->ncomp * ct->towers * cc->GetRingDimension(), ct->towers);
   if (!ct->data) return false;
   obj = ct;
   return true;

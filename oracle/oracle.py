@@ -162,6 +162,16 @@ def elem_for_eval_add(moduli, operand, sc_factor, noise_deg):
     return np.array(c, dtype=np.uint64)
 
 
+def _check_encode_bound(z, scale):
+    """The encoders round every coefficient exactly, which is OpenFHE's CKKSPackedEncoding::Encode
+    only while the scaled coefficients stay below 2^61 (MAX_BITS_IN_WORD; above it OpenFHE takes an
+    approximate path). A coefficient is at most max(|re| + |im|) of the slots, so inputs at or above
+    2^61 (1 - 2^-40) / scale are rejected, as the device encoder does."""
+    lim = 2.0 ** 61 * (1.0 - 2.0 ** -40) / scale
+    if z.size and not np.all(np.abs(z.real) + np.abs(z.imag) < lim):
+        raise ValueError("encode: a slot value times the scale reaches 2^61 (OpenFHE's MAX_BITS_IN_WORD)")
+
+
 def gen_primes(logN, L1, first_bits, scale_bits, K, aux_bits=60):
     out = np.zeros(L1 + K, dtype=np.uint64)
     rc = lib().orc_gen_primes(logN, L1, first_bits, scale_bits, K, aux_bits, out)
@@ -405,6 +415,7 @@ class Ctx:
 
     def encode(self, z, scale, l):
         z = np.asarray(z, dtype=np.complex128)
+        _check_encode_bound(z, scale)
         flat = np.ascontiguousarray(np.stack([z.real, z.imag], axis=1).reshape(-1))
         out = self.zeros(l + 1)
         lib().orc_encode(self.h, flat, len(z), scale, l, out)
@@ -412,6 +423,7 @@ class Ctx:
 
     def encode_ext(self, z, scale, l):
         z = np.asarray(z, dtype=np.complex128)
+        _check_encode_bound(z, scale)
         flat = np.ascontiguousarray(np.stack([z.real, z.imag], axis=1).reshape(-1))
         out = self.zeros(l + 1 + self.K)
         lib().orc_encode_ext(self.h, flat, len(z), scale, l, out)

@@ -7,6 +7,8 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstring>
+#include <iterator>
 #include <fstream>
 #include <random>
 #include <string>
@@ -76,6 +78,30 @@ int main(int argc, char** argv) {
       std::printf("{\"ok\": false}\n");
       return 1;
     }
+    // corrupt copies are refused: a residue equal to its prime, a slot count that is not a power of
+    // two, a key word above its prime
+    auto patched = [&](const std::string& src, const std::string& dst, size_t off, const void* v, size_t n) {
+      std::ifstream in(src, std::ios::binary);
+      std::string bytes((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+      std::memcpy(&bytes[off], v, n);
+      std::ofstream out(dst, std::ios::binary);
+      out.write(bytes.data(), (std::streamsize)bytes.size());
+    };
+    const size_t N = b->GetRingDimension();
+    const uint64_t q0 = b->GetModuli()[0], q1 = b->GetModuli()[1];
+    const uint32_t three = 3;
+    patched(dir + "/ct.bin", dir + "/ct_noncanon.bin", 16 + 24 + 8 + 8 * 5, &q0, 8);   // header, metadata, word count
+    patched(dir + "/ct.bin", dir + "/ct_slots.bin", 16 + 12, &three, 4);
+    patched(dir + "/mult.bin", dir + "/mult_noncanon.bin", 16 + 8 + 8 * (N + 7), &q1, 8);
+    Ciphertext bad;
+    bool corrupt_refused = !Serial::DeserializeFromFile(dir + "/ct_noncanon.bin", bad, SerType::BINARY, b) &&
+                           !Serial::DeserializeFromFile(dir + "/ct_slots.bin", bad, SerType::BINARY, b);
+    {
+      std::ifstream f(dir + "/mult_noncanon.bin", std::ios::binary);
+      corrupt_refused = corrupt_refused && !b->DeserializeEvalMultKey(f, SerType::BINARY);
+    }
+    // the untouched copy still loads
+    corrupt_refused = corrupt_refused && Serial::DeserializeFromFile(dir + "/ct.bin", bad, SerType::BINARY, b);
     Ciphertext ct3;
     Serial::DeserializeFromFile(dir + "/ct.bin", ct3, SerType::BINARY, b);
 
@@ -93,9 +119,10 @@ int main(int argc, char** argv) {
     double err = 0;
     for (uint32_t i = 0; i < n; ++i) err = std::max(err, std::abs(df->values[i] - z[i]));
     std::printf("{\"ok\": true, \"same_moduli\": %s, \"same_ciphertext\": %s, \"same_result\": %s, \"same_decryption\": %s, "
-                "\"fresh_encryption_err\": %.3g, \"json_refused\": %s, \"wrong_kind_refused\": %s}\n",
+                "\"fresh_encryption_err\": %.3g, \"json_refused\": %s, \"wrong_kind_refused\": %s, \"corrupt_refused\": %s}\n",
                 a->GetModuli() == b->GetModuli() ? "true" : "false", same_ct ? "true" : "false", same_result ? "true" : "false",
-                same_dec ? "true" : "false", err, json_refused ? "true" : "false", wrong_kind_refused ? "true" : "false");
+                same_dec ? "true" : "false", err, json_refused ? "true" : "false", wrong_kind_refused ? "true" : "false",
+                corrupt_refused ? "true" : "false");
   } catch (const std::exception& ex) {
     std::fprintf(stderr, "error: %s\n", ex.what());
     return 1;

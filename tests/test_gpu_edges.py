@@ -72,3 +72,32 @@ def test_contract_violations(gctx1, octx1, level0, fh):
         g.encode_coeffs(np.zeros((0, o.N), dtype=np.int64), [0], out)
     for x in (d, out, k, rk):
         x.free()
+
+
+def test_even_galois_rejected(gctx1, octx1, level0, fh):
+    """Every entry point taking Galois elements rejects an even one (not a bijection of the slots,
+    so the key inner product's scatter would leave outputs unwritten) before any launch, and so does
+    one at or above 2N."""
+    g, o, m = gctx1, octx1, level0
+    l = 2
+    src = _ct(o, 630, l)
+    d = g.to_device(src)
+    out = g.array((2 * (l + 1), o.N))
+    rk = g.key(m["rot"])
+    good = o.rot_index(1)
+    for bad in (2, 2 * o.N + 1):
+        with pytest.raises(fh.FHError, match="galois"):
+            g.rotate_batch(d, 1, l, [rk], [bad], out)
+        with pytest.raises(fh.FHError, match="galois"):
+            g.rotate_hoisted(d, l, [rk, rk], [good, bad], out)
+        with pytest.raises(fh.FHError, match="galois"):
+            g.linear_transform(d, l, o.N // 2, 2, 1, d, [None, rk], [1, bad], out)
+        with pytest.raises(fh.FHError, match="galois"):
+            g.linear_transform(d, l, o.N // 2, 2, 1, d, [None, rk], [1, good], out, giant_key=rk, giant_galois=bad)
+        with pytest.raises(fh.FHError, match="galois"):
+            g.linear_transform_batch(d, 1, l, o.N // 2, 2, 1, d, [None, rk], [1, bad], out)
+    # nothing was launched: the output buffer is untouched and the engine still works
+    g.rotate_batch(d, 1, l, [rk], [good], out)
+    assert np.array_equal(out.download(), o.eval_rotate(src, l, m["rot"], good))
+    for x in (d, out, rk):
+        x.free()

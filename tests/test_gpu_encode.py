@@ -65,6 +65,22 @@ def test_encode_rejects_bad_input(gctx1, fh):
         g.encode_slots(np.ones(6), 2.0 ** 40, [0, 1], out)             # slots not a power of two
     with pytest.raises(fh.FHError):
         g.encode_slots(np.full(4, 4.0), 2.0 ** 62, [0, 1], out)         # |value| * scale beyond 2^63
+    with pytest.raises(fh.FHError, match="2\\^61"):
+        g.encode_slots(np.array([8.0, 0.0]), 2.0 ** 58, [0, 1], out)    # coefficient 2^61: OpenFHE's approximate path
     with pytest.raises(fh.FHError):
         g.encode_slots(np.array([np.nan, 1.0]), 2.0 ** 40, [0, 1], out)
+    out.free()
+
+
+def test_encode_at_the_2_61_boundary(gctx2, octx2):
+    """Coefficients just below 2^61 (OpenFHE's MAX_BITS_IN_WORD) still encode exactly, bit for bit
+    with the oracle; 2^61 itself is rejected by both encoders."""
+    g, o = gctx2, octx2
+    pids = [0, 1, 2]
+    z = np.array([5.0 - 2.999j])                                        # one slot: coefficients z * 2^58
+    out = g.array((3, o.N))
+    g.encode_slots(z, 2.0 ** 58, pids, out)
+    assert np.array_equal(out.download(), o.encode(z, 2.0 ** 58, 2))
+    with pytest.raises(ValueError):
+        o.encode(np.array([8.0 + 0j]), 2.0 ** 58, 2)
     out.free()

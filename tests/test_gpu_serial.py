@@ -2,7 +2,8 @@
 file has the same moduli; deserialized keys and ciphertexts hold the same device words; the same
 computation (EvalMult, Rescale, EvalRotate with deserialized evaluation keys) gives the same
 residues and decryption in both contexts; a fresh encryption under the deserialized public key
-decrypts under the deserialized secret key; JSON and files of another kind are refused."""
+decrypts under the deserialized secret key; JSON, files of another kind and corrupt files (a residue
+not below its prime in a ciphertext or key, a slot count that is not a power of two) are refused."""
 import json
 import os
 import subprocess
@@ -27,3 +28,4 @@ def test_serialization_round_trip(tmp_path):
     assert res["same_result"] and res["same_decryption"]
     assert res["fresh_encryption_err"] < 1e-6
     assert res["json_refused"] and res["wrong_kind_refused"]
+    assert res["corrupt_refused"]

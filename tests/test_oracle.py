@@ -106,6 +106,38 @@ def test_root_of_unity_is_minimal_primitive():
             assert pow(psi, k, q) >= psi
 
 
+@pytest.mark.parametrize("logN,L1,K", [(13, 4, 2), (16, 25, 9)])
+def test_root_of_unity_minimal_over_all_odd_powers(logN, L1, K):
+    """At the workloads' real primes: psi has order exactly 2N, and it is the smallest of all
+    primitive 2N-th roots, which are exactly psi^k for the N odd k < 2N (exhaustive)."""
+    c = O.Ctx(logN, O.gen_primes(logN, L1, 60, 50, K), L1, K, 2 if logN == 13 else 3)
+    N = c.N
+    for i in range(L1 + K):
+        q, psi = c.q(i), c.psi[i]
+        assert pow(psi, N, q) == q - 1
+        p2 = psi * psi % q
+        w = psi
+        for _ in range(N - 1):                      # psi^3, psi^5, ..., psi^(2N-1)
+            w = w * p2 % q
+            assert w > psi, (i, q)
+
+
+def test_encode_bound_2_61(sc):
+    """The encoders round exactly, which is OpenFHE's Encode only below 2^61 (MAX_BITS_IN_WORD): the
+    oracle rejects slot values whose coefficients could reach 2^61 and encodes the largest ones
+    below it exactly (one slot: the coefficients are re * scale and im * scale)."""
+    c = sc
+    with pytest.raises(ValueError):
+        c.encode(np.array([8.0 + 0j]), 2.0 ** 58, 1)
+    with pytest.raises(ValueError):
+        c.encode(np.array([4.0 + 4.0j]), 2.0 ** 58, 1)            # |re| + |im| counts
+    m = c.encode(np.array([5.0 - 2.999j]), 2.0 ** 58, 1)
+    coef = c.intt(m, [0, 1])                                   # back to coefficients mod q_0, q_1
+    q0 = c.q(0)
+    re, im = 5 * 2 ** 58, int(round(-2.999 * 2 ** 58))
+    assert int(coef[0][0]) == re % q0 and int(coef[0][c.N // 2]) == im % q0
+
+
 def test_kat_q17_n8():
     """Reference known-answer case (test/NegacyclicNTTtests.cu:31-122): q=17, N=8,
     v = {1,2,3,7,5,4,1,2}: round trip, and NTT product == schoolbook negacyclic product."""
