@@ -89,7 +89,9 @@ int fh_context_create(int log_n, int num_q, int num_p, int dnum, const uint64_t*
     auto* h = new fh_context{nullptr};
     try {
       h->c = new fh::Context(log_n, num_q, num_p, dnum, m);
+      fh::noalloc_init(*h->c);
     } catch (...) {
+      delete h->c;
       delete h;
       throw;
     }
@@ -112,7 +114,9 @@ int fh_context_create_generated(int log_n, int num_q, int num_p, int dnum, int f
     auto* h = new fh_context{nullptr};
     try {
       h->c = new fh::Context(log_n, num_q, K, dnum, m);
+      fh::noalloc_init(*h->c);
     } catch (...) {
+      delete h->c;
       delete h;
       throw;
     }
@@ -321,6 +325,19 @@ int fh_rescale(fh_context* ctx, const uint64_t* ct, int level, int ncomp, uint64
     if (level < 1) throw fh::Error(FH_E_INVALID, "cannot rescale at level 0");
     if (ncomp < 1 || ncomp > 8) throw fh::Error(FH_E_INVALID, "ncomp must be 1..8");
     fh::rescale(c, ct, level, ncomp, out, c.stream);
+  });
+}
+
+int fh_rescale_noalloc(fh_context* ctx, const uint64_t* ct, int level, int ncomp, uint64_t* out) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(ct, "ct"); need(out, "out");
+    check_level(c, level);
+    if (level < 1) throw fh::Error(FH_E_INVALID, "cannot rescale at level 0");
+    if (ncomp < 1 || ncomp > 8) throw fh::Error(FH_E_INVALID, "ncomp must be 1..8");
+    const size_t in_words = (size_t)ncomp * (level + 1) * c.N, out_words = (size_t)ncomp * level * c.N;
+    if (ct < out + out_words && out < ct + in_words) throw fh::Error(FH_E_INVALID, "fh_rescale_noalloc cannot run in place: ct and out overlap");
+    fh::rescale_noalloc(c, ct, level, ncomp, out, c.stream);
   });
 }
 

@@ -147,6 +147,7 @@ struct ProfScope {
 struct Context {
   int logN = 0;
   uint32_t N = 0;
+  int device = -1;   // HIP device current at creation (set by noalloc_init)
   int L1 = 0, K = 0, dnum = 0, alpha = 0, np = 0;
   std::vector<uint64_t> q, psi;
   std::vector<uint8_t> fp64;   // per prime: 1 when its NTT runs in fp64 arithmetic

@@ -622,18 +622,27 @@ void rescale(Context& c, const uint64_t* x, int l, int ncomp, uint64_t* out, hip
     FH_HIP(hipMemcpyAsync(out, tmp.p, (size_t)ncomp * l * c.N * 8, hipMemcpyDeviceToDevice, st));
     return;
   }
-  DBuf top(c.pool, (size_t)ncomp * c.N), e(c.pool, (size_t)ncomp * l * c.N);
+  DBuf work(c.pool, rescale_work_words(c, l, ncomp));
+  rescale_core(c, x, l, ncomp, out, work.p, st);
+}
+
+void rescale_core(Context& c, const uint64_t* x, int l, int ncomp, uint64_t* out, uint64_t* work, hipStream_t st) {
+  if (l < 1) throw Error(-1, "cannot rescale a level-0 ciphertext");
+  if (ncomp < 1) return;
+  if (overlaps(x, (size_t)ncomp * (l + 1) * c.N, out, (size_t)ncomp * l * c.N)) throw Error(-1, "rescale_core cannot run in place");
+  uint64_t* top = work;                          // ncomp limbs
+  uint64_t* e = work + (size_t)ncomp * c.N;      // ncomp x l limbs
   int pid_top = l;
   // INTT of the top limb of every component straight into a packed buffer
   ntt_inv(c, make_set(const_cast<uint64_t*>(x) + (size_t)l * c.N, 1, &pid_top, ncomp, (uint64_t)(l + 1) * c.N),
-          make_set(top.p, 1, &pid_top, ncomp, c.N), st);
+          make_set(top, 1, &pid_top, ncomp, c.N), st);
   RescaleParams rp{c.N, (uint32_t)l, (uint32_t)c.L1, c.d_prime, c.d_rs_qlinv, c.d_rs_qlinv_sh, c.d_rs_QlQl, c.d_rs_QlQl_sh};
-  { ProfScope ps_(c, "rescale_expand", st); k_rescale_expand<<<grid_n(c.N, ncomp * l), kT, 0, st>>>(top.p, e.p, rp); }
+  { ProfScope ps_(c, "rescale_expand", st); k_rescale_expand<<<grid_n(c.N, ncomp * l), kT, 0, st>>>(top, e, rp); }
   FH_HIP(hipGetLastError());
-  std::vector<int> ids(l);
+  int ids[kMaxLimbs];   // on the stack: this path makes no allocation of any kind
   for (int i = 0; i < l; ++i) ids[i] = i;
   // forward NTT of the correction with x_i * q_l^-1 + e_i fused into its last launch
-  ntt_fwd_rescale(c, make_set(e.p, l, ids.data(), ncomp, (uint64_t)l * c.N), x, (uint32_t)l, out, st);
+  ntt_fwd_rescale(c, make_set(e, l, ids, ncomp, (uint64_t)l * c.N), x, (uint32_t)l, out, st);
 }
 
 void mod_raise(Context& c, const uint64_t* x, int l_in, int ncomp, int l, uint64_t* out, hipStream_t st) {

@@ -58,6 +58,15 @@ void linear_transform_batch(Context& c, int count, const uint64_t* cts, uint64_t
 void moddown(Context& c, const uint64_t* x, int l, uint64_t* out, int npolys, uint64_t x_stride, uint64_t out_stride, hipStream_t st);
 // ncomp components over Q_l -> over Q_{l-1}
 void rescale(Context& c, const uint64_t* x, int l, int ncomp, uint64_t* out, hipStream_t st);
+// Rescale with its work buffer supplied by the caller (rescale_work_words words: the INTT of the top
+// limbs, then the correction over l limbs per component); x and out must not overlap. Allocates nothing.
+inline size_t rescale_work_words(const Context& c, int l, int ncomp) { return (size_t)ncomp * (l + 1) * c.N; }
+void rescale_core(Context& c, const uint64_t* x, int l, int ncomp, uint64_t* out, uint64_t* work, hipStream_t st);
+// fh_noalloc.hip: Rescale through a fixed per-device array (no allocation); noalloc_init looks the
+// array's device address up for the current device and records the device in the context.
+void noalloc_init(Context& c);
+size_t noalloc_scratch_words();
+void rescale_noalloc(Context& c, const uint64_t* x, int l, int ncomp, uint64_t* out, hipStream_t st);
 // CKKS bootstrapping ModRaise: tower 0 of ncomp components (of l_in + 1 limbs each) to
 // coefficient form, lifted centred to q_0..q_l (SwitchModulus) and transformed back: out holds
 // ncomp components of l + 1 limbs.

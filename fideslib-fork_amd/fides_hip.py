@@ -60,6 +60,7 @@ def _declare(L):
         "fh_encode_coeffs": (I, [P, VP, I, i32p, I, VP, S]),
         "fh_eval_mult_many": (I, [P, I, VP, I, VP, I, I, VP, VP]),
         "fh_rescale": (I, [P, VP, I, I, VP]),
+        "fh_rescale_noalloc": (I, [P, VP, I, I, VP]),
         "fh_stream_copy": (I, [P, VP, VP, S]),
         "fh_mod_raise": (I, [P, VP, I, I, I, VP]),
         "fh_key_create": (I, [P, u64p, PP]),
@@ -278,6 +279,9 @@ class Context:
     # ---- rescale / key switching ----
     def rescale(self, ct, level, ncomp, out):
         _check(load().fh_rescale(self.h, ct.ptr, level, ncomp, out.ptr))
+
+    def rescale_noalloc(self, ct, level, ncomp, out):
+        _check(load().fh_rescale_noalloc(self.h, ct.ptr, level, ncomp, out.ptr))
 
     def mod_raise(self, ct, level_in, ncomp, level, out):
         _check(load().fh_mod_raise(self.h, ct.ptr, level_in, ncomp, level, out.ptr))
