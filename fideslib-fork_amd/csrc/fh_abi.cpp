@@ -803,6 +803,9 @@ int fh_serve_run(fh_context* ctx, const fh_serve_config* config, const fh_key* c
     cfg.bootstrap = config->bootstrap;
     cfg.bootstrap_user = config->bootstrap_user;
     cfg.request_checksums = config->request_checksums;
+    cfg.request_outputs = config->request_outputs;
+    cfg.request_output_capacity = config->request_output_capacity;
+    cfg.request_output_words = config->request_output_words;
     if (cfg.mode < 0 || cfg.mode > 2) throw fh::Error(FH_E_INVALID, "mode must be 0 (batched), 1 (per operation) or 2 (one request at a time)");
     const uint64_t* rk[8];
     for (int k = 0; k < 8; ++k) rk[k] = key_of(c, rot_keys[k]);

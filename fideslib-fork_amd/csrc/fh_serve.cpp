@@ -243,6 +243,12 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
       Request* r = it->get();
       if (++r->pc == progs[r->kind].size()) {
         lat.push_back(td - r->arrival);
+        if (cfg.request_outputs && cfg.request_outputs[r->id]) {
+          const size_t words = 2ull * (r->level[3] + 1) * N;
+          if (words > cfg.request_output_capacity) throw Error(-1, "request_output_capacity below a request's result");
+          FH_HIP(hipMemcpy(cfg.request_outputs[r->id], r->reg[3].p, words * 8, hipMemcpyDeviceToHost));
+          if (cfg.request_output_words) cfg.request_output_words[r->id] = words;
+        }
         if (cfg.checksum || cfg.request_checksums) {
           std::vector<uint64_t> h(2ull * (r->level[3] + 1) * N);
           FH_HIP(hipMemcpy(h.data(), r->reg[3].p, h.size() * 8, hipMemcpyDeviceToHost));

@@ -41,6 +41,10 @@ struct ServeConfig {
   void* bootstrap_user = nullptr;
   // optional, `requests` entries: the sum of request i's result words (mod 2^64) at index i
   uint64_t* request_checksums = nullptr;
+  // optional, `requests` host buffers of request_output_capacity words: each result copied whole
+  uint64_t* const* request_outputs = nullptr;
+  size_t request_output_capacity = 0;
+  uint64_t* request_output_words = nullptr;
 };
 
 struct ServeResult {

@@ -120,7 +120,8 @@ class ServeConfig(C.Structure):
     _fields_ = [("rate_per_s", C.c_double), ("requests", C.c_int), ("start_level", C.c_int), ("hbm_budget_gb", C.c_double),
                 ("mode", C.c_int), ("inference_share", C.c_double), ("seed", C.c_uint64), ("checksum", C.c_int),
                 ("bootstrap_share", C.c_double), ("bootstrap", C.c_void_p), ("bootstrap_user", C.c_void_p),
-                ("request_checksums", C.POINTER(C.c_uint64))]
+                ("request_checksums", C.POINTER(C.c_uint64)), ("request_outputs", C.POINTER(C.c_void_p)),
+                ("request_output_capacity", C.c_size_t), ("request_output_words", C.POINTER(C.c_uint64))]
 
 
 class ServeResult(C.Structure):
@@ -393,15 +394,22 @@ class Context:
 
     # ---- serving loop ----
     def serve(self, rot_keys, rot_galois, relin_key, rate_per_s, requests, start_level=None, mode=0, hbm_budget_gb=64.0,
-              inference_share=0.74, seed=1, checksum=False, bootstrap_share=0.0, request_checksums=False):
+              inference_share=0.74, seed=1, checksum=False, bootstrap_share=0.0, request_checksums=False, request_outputs=False):
         """bootstrap_share > 0 needs a bootstrap function, which only C/C++ callers pass (the C++
         facade's EvalBootstrap, tests/cpp/serve_check.cpp); from here it is rejected. mode 0:
         batched ticks, 1: one call per operation, 2: one request at a time. request_checksums:
-        also return each request's result-word sum (list indexed by request)."""
+        also return each request's result-word sum (list indexed by request). request_outputs: also
+        return each request's whole result (numpy arrays indexed by request)."""
         per = (C.c_uint64 * max(1, requests))() if request_checksums else None
+        cap = 2 * self.L1 * self.N
+        bufs = [np.zeros(cap, dtype=np.uint64) for _ in range(requests)] if request_outputs else None
+        ptrs = (C.c_void_p * max(1, requests))(*[b.ctypes.data for b in bufs]) if bufs else None
+        words = (C.c_uint64 * max(1, requests))() if bufs else None
         cfg = ServeConfig(rate_per_s, requests, self.L1 - 1 if start_level is None else start_level, hbm_budget_gb, mode,
                           inference_share, seed, int(checksum), float(bootstrap_share), None, None,
-                          C.cast(per, C.POINTER(C.c_uint64)) if per is not None else None)
+                          C.cast(per, C.POINTER(C.c_uint64)) if per is not None else None,
+                          C.cast(ptrs, C.POINTER(C.c_void_p)) if ptrs is not None else None, cap if bufs else 0,
+                          C.cast(words, C.POINTER(C.c_uint64)) if words is not None else None)
         kp = (C.c_void_p * 8)(*[k.h for k in rot_keys])
         gl = (C.c_uint32 * 8)(*[int(g) for g in rot_galois])
         r = ServeResult()
@@ -409,6 +417,8 @@ class Context:
         out = {f: getattr(r, f) for f, _ in ServeResult._fields_}
         if per is not None:
             out["request_checksums"] = [int(per[i]) for i in range(requests)]
+        if bufs is not None:
+            out["request_outputs"] = [bufs[i][:int(words[i])] for i in range(requests)]
         return out
 
     # ---- launch-shape tunables ----

@@ -327,7 +327,10 @@ struct ChebEval {
 Ciphertext CryptoContextImpl::EvalChebyshevSeries(const Ciphertext& ct, const std::vector<double>& coeffs, double a, double b) {
   if (coeffs.size() < 2) throw std::runtime_error("fideslib: EvalChebyshevSeries needs degree >= 1");
   if (!(b > a)) throw std::runtime_error("fideslib: EvalChebyshevSeries needs a < b");
-  Ciphertext y = ct->noiseScaleDeg == 2 ? Rescale(ct) : ct;
+  Ciphertext y = ct->noiseScaleDeg == 2 ? modReduce(ct) : ct;
+  // the series manages its rescales and tracked scales explicitly (FIXEDMANUAL semantics inside,
+  // also under FLEXIBLEAUTO: the result carries the scale it tracked)
+  ManualScope manual(*this);
   if (y->ncomp == 4 && (a != -1.0 || b != 1.0)) throw std::runtime_error("fideslib: EvalChebyshevSeries of a pair runs on [-1, 1] only");
   if (a != -1.0 || b != 1.0) {
     // y = (2x - (a + b)) / (b - a)
@@ -534,8 +537,21 @@ Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0, uint32_t numI
   BootstrapState& st = *boot_;
   const uint32_t n = st.slots;
   if (GetBootstrapDepth() + 1 > L1_) throw std::runtime_error("fideslib: EvalBootstrap: the chain is too short for the bootstrap depth");
-  Ciphertext ct = ct0->noiseScaleDeg == 2 ? Rescale(ct0) : ct0;
-  if (st.correction) {
+  Ciphertext ct = ct0->noiseScaleDeg == 2 ? modReduce(ct0) : ct0;
+  if (flex()) {
+    // FLEXIBLEAUTO input (scale s at level factor sf): one scalar product and rescale bring the
+    // message to the nominal scale 2^scaleModSize times 2^-correction, the state the steps below
+    // assume (the reference's adjustment factor (target / source) (dropped prime / source)
+    // 2^-correction, src/CKKS/Bootstrap.cu:680-690, with this bootstrap's nominal target)
+    if (ct->towers < 2) throw std::runtime_error("fideslib: EvalBootstrap under FLEXIBLEAUTO needs two towers");
+    const uint32_t l = ct->towers - 1;
+    const double adj = scale_ / ct->scalingFactor * (mrf_[l] / sfReal_[l]) * std::ldexp(1.0, -(int)st.correction);
+    ct = modReduce(multCore(ct, adj));
+    ct->scalingFactor = scale_;
+  }
+  // the steps below manage their rescales and scales explicitly (FIXEDMANUAL semantics)
+  ManualScope manual(*this);
+  if (st.correction && GetScalingTechnique() == ScalingTechnique::FIXEDMANUAL) {
     if (ct->towers < 2) throw std::runtime_error("fideslib: EvalBootstrap with a correction factor needs two towers");
     ct = Rescale(EvalMult(ct, std::ldexp(1.0, -(int)st.correction)));
   }

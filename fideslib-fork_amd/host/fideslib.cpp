@@ -116,6 +116,40 @@ CryptoContextImpl::CryptoContextImpl(const CCParams& p) : params_(p) {
   }
   rng_ = std::make_unique<Rng>();
   rng_->base = p.seed_;
+  // per-level scale tables (by towers - 1). FLEXIBLEAUTO: the top level's factor is the top Q prime,
+  // level k's the previous one squared over the prime the k-th rescale drops, in OpenFHE's order of
+  // operations (prevSF * prevSF / q); the technique is refused when a factor leaves (0.5, 2) times
+  // the top one, as OpenFHE does.
+  sfReal_.assign(L1_, scale_);
+  sfBig_.assign(L1_, scale_ * scale_);
+  mrf_.assign(L1_, scale_);
+  if (p.tech_ == ScalingTechnique::FLEXIBLEAUTO) {
+    sfReal_[L1_ - 1] = (double)moduli_[L1_ - 1];
+    for (int l = (int)L1_ - 2; l >= 0; --l) {
+      const double prev = sfReal_[l + 1];
+      sfReal_[l] = prev * prev / (double)moduli_[l + 1];
+      const double ratio = sfReal_[l] / sfReal_[L1_ - 1];
+      if (ratio <= 0.5 || ratio >= 2.0)
+        throw std::runtime_error("fideslib: FLEXIBLEAUTO cannot support this number of levels in this parameter setting (use FIXEDMANUAL)");
+    }
+    for (uint32_t l = 0; l < L1_; ++l) {
+      sfBig_[l] = sfReal_[l] * sfReal_[l];
+      mrf_[l] = (double)moduli_[l];
+    }
+  }
+}
+
+double CryptoContextImpl::GetScalingFactorReal(uint32_t level) const {
+  if (level >= L1_) throw std::runtime_error("fideslib: level out of range");
+  return sfReal_[L1_ - 1 - level];
+}
+double CryptoContextImpl::GetScalingFactorRealBig(uint32_t level) const {
+  if (level >= L1_) throw std::runtime_error("fideslib: level out of range");
+  return sfBig_[L1_ - 1 - level];
+}
+double CryptoContextImpl::GetModReduceFactor(uint32_t level) const {
+  if (level >= L1_) throw std::runtime_error("fideslib: level out of range");
+  return mrf_[L1_ - 1 - level];
 }
 
 CryptoContextImpl::~CryptoContextImpl() {
@@ -288,6 +322,16 @@ void CryptoContextImpl::encodeInto(const std::vector<std::complex<double>>& v, u
   check(fh_encode_slots(ctx_, z.data(), 1, (int)slots, scale, pids.data(), (int)pids.size(), dev, 0));
 }
 
+// FIXEDMANUAL: the nominal scale to the degree; FLEXIBLEAUTO: the level's factor (OpenFHE's
+// MakeCKKSPackedPlaintext takes GetScalingFactorReal(level)); a degree-2 plaintext would be encoded at
+// the level's factor and multiplied by it again as an integer, which this encoder does not do.
+double CryptoContextImpl::ptScale(uint32_t towers, size_t noiseScaleDeg) const {
+  if (noiseScaleDeg < 1) throw std::runtime_error("fideslib: noiseScaleDeg must be >= 1");
+  if (!flex()) return std::pow(scale_, (double)noiseScaleDeg);
+  if (noiseScaleDeg != 1) throw std::runtime_error("fideslib: FLEXIBLEAUTO plaintexts are encoded at noise scale degree 1");
+  return sfReal_[towers - 1];
+}
+
 Plaintext CryptoContextImpl::MakeCKKSPackedPlaintext(const std::vector<std::complex<double>>& v, size_t noiseScaleDeg, uint32_t level,
                                                      uint32_t slots) {
   if (level >= L1_) throw std::runtime_error("fideslib: level out of range");
@@ -295,7 +339,7 @@ Plaintext CryptoContextImpl::MakeCKKSPackedPlaintext(const std::vector<std::comp
   pt->towers = L1_ - level;
   pt->slots = slots ? slots : (params_.batch_ ? params_.batch_ : N_ / 2);
   pt->noiseScaleDeg = (uint32_t)noiseScaleDeg;
-  pt->scalingFactor = std::pow(scale_, (double)noiseScaleDeg);
+  pt->scalingFactor = ptScale(pt->towers, noiseScaleDeg);
   pt->values = v;
   pt->values.resize(pt->slots);
   std::vector<int32_t> pids(pt->towers);
@@ -318,7 +362,7 @@ Plaintext CryptoContextImpl::MakeCKKSPackedPlaintextExtended(const std::vector<s
   pt->extended = true;
   pt->slots = slots ? slots : (params_.batch_ ? params_.batch_ : N_ / 2);
   pt->noiseScaleDeg = (uint32_t)noiseScaleDeg;
-  pt->scalingFactor = std::pow(scale_, (double)noiseScaleDeg);
+  pt->scalingFactor = ptScale(pt->towers, noiseScaleDeg);
   pt->values = v;
   pt->values.resize(pt->slots);
   std::vector<int32_t> pids;
@@ -377,8 +421,10 @@ Ciphertext CryptoContextImpl::Encrypt(const PublicKey& pk, const Plaintext& pt) 
   return ct;
 }
 
-Plaintext CryptoContextImpl::Decrypt(const PrivateKey& sk, const Ciphertext& ct) {
-  require_standard(*ct, "Decrypt");
+Plaintext CryptoContextImpl::Decrypt(const PrivateKey& sk, const Ciphertext& ct0) {
+  require_standard(*ct0, "Decrypt");
+  // FLEXIBLEAUTO: a depth-2 result is rescaled before decoding
+  const Ciphertext ct = flex() && ct0->noiseScaleDeg == 2 && ct0->towers > 1 ? modReduce(ct0) : ct0;
   const uint32_t t = ct->towers;
   DeviceBuffer m(native_, (size_t)t * N_), tmp(native_, (size_t)t * N_);
   const uint64_t* c0 = ct->data->ptr;
@@ -481,7 +527,13 @@ static void require_same_deg(const CiphertextImpl& a, const CiphertextImpl& b, c
     throw std::runtime_error(std::string("fideslib: ") + op + ": operands have different noise scale degrees (rescale first)");
 }
 
-Ciphertext CryptoContextImpl::EvalAdd(const Ciphertext& a0, const Ciphertext& b0) {
+Ciphertext CryptoContextImpl::EvalAdd(const Ciphertext& a1, const Ciphertext& b1) {
+  Ciphertext a0 = a1, b0 = b1;
+  if (flex()) {
+    require_standard(*a0, "EvalAdd");
+    require_standard(*b0, "EvalAdd");
+    adjustLevelsAndDepth(a0, b0);   // OpenFHE AdjustForAddOrSub: one level and one depth
+  }
   require_same_deg(*a0, *b0, "EvalAdd");
   // over the first t towers of each component, read in place (no level-reduced copies)
   const uint32_t t = std::min(a0->towers, b0->towers);
@@ -507,6 +559,11 @@ Ciphertext CryptoContextImpl::EvalSub(const Ciphertext& a0, const Ciphertext& b0
 
 Ciphertext CryptoContextImpl::EvalAdd(const Ciphertext& a0, const Plaintext& pt) {
   if (pt->extended) throw std::runtime_error("fideslib: extended plaintexts are for LinearTransform");
+  if (flex()) {
+    Ciphertext a = a0, p = morph(pt);
+    adjustLevelsAndDepth(a, p);
+    return addPlain(a, p, 0);
+  }
   if (pt->noiseScaleDeg != a0->noiseScaleDeg) throw std::runtime_error("fideslib: EvalAdd: plaintext scale degree differs");
   const uint32_t t = std::min(a0->towers, pt->towers);
   auto a = adjusted(a0, t);
@@ -521,6 +578,11 @@ Ciphertext CryptoContextImpl::EvalAdd(const Ciphertext& a0, const Plaintext& pt)
 
 Ciphertext CryptoContextImpl::EvalSub(const Ciphertext& a0, const Plaintext& pt) {
   if (pt->extended) throw std::runtime_error("fideslib: extended plaintexts are for LinearTransform");
+  if (flex()) {
+    Ciphertext a = a0, p = morph(pt);
+    adjustLevelsAndDepth(a, p);
+    return addPlain(a, p, 1);
+  }
   if (pt->noiseScaleDeg != a0->noiseScaleDeg) throw std::runtime_error("fideslib: EvalSub: plaintext scale degree differs");
   const uint32_t t = std::min(a0->towers, pt->towers);
   auto a = adjusted(a0, t);
@@ -537,7 +599,8 @@ Ciphertext CryptoContextImpl::EvalSub(const Ciphertext& a0, const Plaintext& pt)
 // (OpenFHE's non-FMA form operand / approxFactor * scFactor + 0.5, truncated), reduced per tower,
 // times 2^logApprox by CRT multiplication.
 std::vector<uint64_t> CryptoContextImpl::ElemForEvalMult(uint32_t towers, double operand) const {
-  const double scFactor = scale_;  // FIXEDMANUAL: the same scaling factor at every level
+  // the level's scaling factor: FIXEDMANUAL the same at every level, FLEXIBLEAUTO the level's own
+  const double scFactor = sfReal(towers);
   const int32_t MAX_BITS = 125, MAX_LOG_STEP = 60;
   int32_t logApprox = 0;
   const double res = std::fabs(operand * scFactor);
@@ -580,7 +643,7 @@ std::vector<uint64_t> CryptoContextImpl::ElemForEvalMult(uint32_t towers, double
 
 // ElemForEvalAddOrSub (src/CKKS/Context.cu:777-863) for a non-negative operand.
 std::vector<uint64_t> CryptoContextImpl::ElemForEvalAddOrSub(uint32_t towers, double operand, uint32_t noiseDeg) const {
-  const double scFactor = scale_;
+  const double scFactor = sfReal(towers);
   const int32_t MAX_BITS = 61, MAX_LOG_STEP = 60;
   int32_t logApprox = 0;
   const double res = std::fabs(operand * scFactor);
@@ -631,21 +694,143 @@ Ciphertext CryptoContextImpl::EvalAdd(const Ciphertext& a, double scalar) {
 
 Ciphertext CryptoContextImpl::EvalSub(const Ciphertext& a, double scalar) { return EvalAdd(a, -scalar); }
 
-Ciphertext CryptoContextImpl::EvalMult(const Ciphertext& a, double scalar) {
-  require_standard(*a, "EvalMult");
-  const auto f = ElemForEvalMult(a->towers, scalar);
+// FLEXIBLEAUTO rescales a degree-2 operand first (OpenFHE LeveledSHECKKSRNS::EvalMult(ct, double))
+Ciphertext CryptoContextImpl::EvalMult(const Ciphertext& a0, double scalar) {
+  require_standard(*a0, "EvalMult");
+  const Ciphertext a = flex() && a0->noiseScaleDeg == 2 ? modReduce(a0) : a0;
+  return multCore(a, scalar);
+}
+
+// EvalMultCore(ct, double): every component times the CRT encoding of operand at the level's
+// scaling factor; noise degree + 1, scale times that factor
+Ciphertext CryptoContextImpl::multCore(const Ciphertext& a, double operand) const {
+  const auto f = ElemForEvalMult(a->towers, operand);
   auto out = newCt(a->towers, a->ncomp);
   for (uint32_t c = 0; c < a->ncomp; ++c)
     check(fh_eltwise_scalar(ctx_, 1, a->data->ptr + (size_t)c * a->towers * N_, f.data(), out->data->ptr + (size_t)c * a->towers * N_,
                             nullptr, (int)a->towers));
   out->noiseScaleDeg = a->noiseScaleDeg + 1;
-  out->scalingFactor = a->scalingFactor * scale_;
+  out->scalingFactor = a->scalingFactor * sfReal(a->towers);
+  out->slots = a->slots;
+  return out;
+}
+
+// ModReduceInternal: the top tower dropped, noise degree - 1, the scale divided by the technique's
+// ModReduceFactor (FLEXIBLEAUTO: the dropped prime; FIXEDMANUAL: the nominal 2^scaleModSize, not q_l)
+Ciphertext CryptoContextImpl::modReduce(const Ciphertext& ct) const {
+  require_standard(*ct, "Rescale");
+  if (ct->towers < 2) throw std::runtime_error("fideslib: Rescale at the last tower");
+  auto out = newCt(ct->towers - 1, ct->ncomp);
+  check(fh_rescale(ctx_, ct->data->ptr, (int)ct->towers - 1, (int)ct->ncomp, out->data->ptr));
+  out->noiseScaleDeg = ct->noiseScaleDeg > 1 ? ct->noiseScaleDeg - 1 : 1;
+  out->scalingFactor = ct->scalingFactor / (flex() ? mrf_[ct->towers - 1] : scale_);
+  out->slots = ct->slots;
+  return out;
+}
+
+// OpenFHE LeveledSHECKKSRNS::AdjustLevelsAndDepthInPlace for an operand c1 with more towers than
+// the other (towers2, deg2, scale2), as the reference restates it (src/CKKS/Ciphertext.cpp:1917-2014):
+// a scalar product that brings the scale to what a rescale (or two) will turn into the other's,
+// the rescales, level drops, and the other operand's scale copied over. Device level l = towers - 1;
+// OpenFHE's GetScalingFactorRealBig(c2lvl - 1) is the square of the factor one level above c2.
+Ciphertext CryptoContextImpl::adjustDown(const Ciphertext& c1, uint32_t towers2, uint32_t deg2, double scale2) const {
+  const uint32_t l1 = c1->towers - 1, l2 = towers2 - 1;
+  Ciphertext x;
+  if (c1->noiseScaleDeg == 2) {
+    if (deg2 == 2) {
+      const double scf1 = c1->scalingFactor, scf2 = scale2, scf = sfReal_[l1], q1 = mrf_[l1];
+      x = modReduce(multCore(c1, scf2 / scf1 * q1 / scf));
+      if (x->towers > towers2) x = adjusted(x, towers2);
+      x->scalingFactor = scale2;
+    } else if (l1 - 1 == l2) {
+      x = modReduce(c1);
+    } else {
+      const double scf1 = c1->scalingFactor, scf2 = sfBig_[l2 + 1], scf = sfReal_[l1], q1 = mrf_[l1];
+      x = modReduce(multCore(c1, scf2 / scf1 * q1 / scf));
+      if (x->towers > towers2 + 1) x = adjusted(x, towers2 + 1);
+      x = modReduce(x);
+      x->scalingFactor = scale2;
+    }
+  } else {
+    if (deg2 == 2) {
+      const double scf1 = c1->scalingFactor, scf2 = scale2, scf = sfReal_[l1];
+      x = adjusted(multCore(c1, scf2 / scf1 / scf), towers2);
+      x->scalingFactor = scf2;
+    } else {
+      const double scf1 = c1->scalingFactor, scf2 = sfBig_[l2 + 1], scf = sfReal_[l1];
+      x = multCore(c1, scf2 / scf1 / scf);
+      if (x->towers > towers2 + 1) x = adjusted(x, towers2 + 1);
+      x = modReduce(x);
+      x->scalingFactor = scale2;
+    }
+  }
+  return x;
+}
+
+// one level and one depth: the operand with more towers is brought down; at equal levels the
+// lower-depth operand is multiplied by 1.0 (EvalMultCore), raising its depth
+void CryptoContextImpl::adjustLevelsAndDepth(Ciphertext& a, Ciphertext& b) const {
+  if (a->noiseScaleDeg > 2 || b->noiseScaleDeg > 2) throw std::runtime_error("fideslib: operands of noise degree above 2");
+  if (a->towers > b->towers) a = adjustDown(a, b->towers, b->noiseScaleDeg, b->scalingFactor);
+  else if (a->towers < b->towers) b = adjustDown(b, a->towers, a->noiseScaleDeg, a->scalingFactor);
+  else if (a->noiseScaleDeg < b->noiseScaleDeg) a = multCore(a, 1.0);
+  else if (b->noiseScaleDeg < a->noiseScaleDeg) b = multCore(b, 1.0);
+}
+
+// AdjustLevelsAndDepthToOneInPlace: aligned, then both rescaled when at depth 2
+void CryptoContextImpl::adjustLevelsAndDepthToOne(Ciphertext& a, Ciphertext& b) const {
+  adjustLevelsAndDepth(a, b);
+  if (a->noiseScaleDeg == 2) {
+    a = modReduce(a);
+    b = modReduce(b);
+  }
+}
+
+Ciphertext CryptoContextImpl::morph(const Plaintext& pt) const {
+  auto c = std::make_shared<CiphertextImpl>();
+  c->data = pt->data;
+  c->towers = pt->towers;
+  c->ncomp = 1;
+  c->noiseScaleDeg = pt->noiseScaleDeg;
+  c->scalingFactor = pt->scalingFactor;
+  c->slots = pt->slots;
+  return c;
+}
+
+// a times the (aligned) plaintext p, component by component
+Ciphertext CryptoContextImpl::mulPlain(const Ciphertext& a, const Ciphertext& p) const {
+  const uint32_t t = std::min(a->towers, p->towers);
+  auto out = newCt(t, a->ncomp);
+  for (uint32_t c = 0; c < a->ncomp; ++c)
+    check(fh_eltwise_binary(ctx_, 2, a->data->ptr + (size_t)c * a->towers * N_, p->data->ptr, out->data->ptr + (size_t)c * t * N_, nullptr,
+                            (int)t));
+  out->noiseScaleDeg = a->noiseScaleDeg + p->noiseScaleDeg;
+  out->scalingFactor = a->scalingFactor * p->scalingFactor;
+  out->slots = a->slots;
+  return out;
+}
+
+// a +/- the (aligned) plaintext p on component 0 (op 0 add, 1 sub)
+Ciphertext CryptoContextImpl::addPlain(const Ciphertext& a, const Ciphertext& p, int op) const {
+  const uint32_t t = std::min(a->towers, p->towers);
+  auto out = newCt(t, a->ncomp);
+  for (uint32_t c = 1; c < a->ncomp; ++c)
+    check(fh_copy_d2d(ctx_, out->data->ptr + (size_t)c * t * N_, a->data->ptr + (size_t)c * a->towers * N_, (size_t)t * N_ * 8));
+  check(fh_eltwise_binary(ctx_, op, a->data->ptr, p->data->ptr, out->data->ptr, nullptr, (int)t));
+  out->noiseScaleDeg = a->noiseScaleDeg;
+  out->scalingFactor = a->scalingFactor;
   out->slots = a->slots;
   return out;
 }
 
 Ciphertext CryptoContextImpl::EvalMult(const Ciphertext& a0, const Plaintext& pt) {
   if (pt->extended) throw std::runtime_error("fideslib: extended plaintexts are for LinearTransform");
+  if (flex()) {
+    require_standard(*a0, "EvalMult");
+    Ciphertext a = a0, p = morph(pt);
+    adjustLevelsAndDepthToOne(a, p);   // OpenFHE AdjustForMult: aligned, then rescaled at depth 2
+    return mulPlain(a, p);
+  }
   const uint32_t t = std::min(a0->towers, pt->towers);
   auto a = adjusted(a0, t);
   auto out = newCt(t, a->ncomp);
@@ -657,8 +842,10 @@ Ciphertext CryptoContextImpl::EvalMult(const Ciphertext& a0, const Plaintext& pt
   return out;
 }
 
-Ciphertext CryptoContextImpl::EvalMultNoRelin(const Ciphertext& a0, const Ciphertext& b0) {
-  if (a0->ncomp != 2 || b0->ncomp != 2) throw std::runtime_error("fideslib: EvalMultNoRelin: ciphertext should be relinearized before");
+Ciphertext CryptoContextImpl::EvalMultNoRelin(const Ciphertext& a1, const Ciphertext& b1) {
+  if (a1->ncomp != 2 || b1->ncomp != 2) throw std::runtime_error("fideslib: EvalMultNoRelin: ciphertext should be relinearized before");
+  Ciphertext a0 = a1, b0 = b1;
+  if (flex()) adjustLevelsAndDepthToOne(a0, b0);
   const uint32_t t = std::min(a0->towers, b0->towers);
   auto a = adjusted(a0, t), b = adjusted(b0, t);
   auto out = newCt(t, 3);
@@ -682,12 +869,16 @@ Ciphertext CryptoContextImpl::Relinearize(const Ciphertext& ct) {
 }
 
 // RequireDegree1 (api/CryptoContext.cpp:994-998), then the key-switch-first product
-Ciphertext CryptoContextImpl::EvalMult(const Ciphertext& a0, const Ciphertext& b0) {
-  require_standard(*a0, "EvalMult");
-  require_standard(*b0, "EvalMult");
-  if (a0->ncomp != 2 || b0->ncomp != 2)
+Ciphertext CryptoContextImpl::EvalMult(const Ciphertext& a1, const Ciphertext& b1) {
+  require_standard(*a1, "EvalMult");
+  require_standard(*b1, "EvalMult");
+  if (a1->ncomp != 2 || b1->ncomp != 2)
     throw std::runtime_error("fideslib: EvalMult: ciphertext should be relinearized before (has 3 elements; call Relinearize)");
   if (!relin_) throw std::runtime_error("fideslib: EvalMultKeyGen was not called");
+  Ciphertext a0 = a1, b0 = b1;
+  // FLEXIBLEAUTO: OpenFHE AdjustForMult (one level and depth, depth-2 operands rescaled); the product
+  // is left at depth 2 and rescaled by the next operation that needs it
+  if (flex()) adjustLevelsAndDepthToOne(a0, b0);
   // operands read at their own tower counts (no level-reduced copies)
   const uint32_t t = std::min(a0->towers, b0->towers);
   auto out = newCt(t, 2);
@@ -700,8 +891,20 @@ Ciphertext CryptoContextImpl::EvalMult(const Ciphertext& a0, const Ciphertext& b
 
 Ciphertext CryptoContextImpl::EvalSquare(const Ciphertext& ct) { return EvalMult(ct, ct); }
 
-Ciphertext CryptoContextImpl::EvalLinearWSum(const std::vector<Ciphertext>& cts, const std::vector<double>& weights) {
-  if (cts.empty() || cts.size() != weights.size()) throw std::runtime_error("fideslib: EvalLinearWSum needs one weight per ciphertext");
+Ciphertext CryptoContextImpl::EvalLinearWSum(const std::vector<Ciphertext>& cts0, const std::vector<double>& weights) {
+  if (cts0.empty() || cts0.size() != weights.size()) throw std::runtime_error("fideslib: EvalLinearWSum needs one weight per ciphertext");
+  std::vector<Ciphertext> cts = cts0;
+  if (flex()) {
+    // OpenFHE EvalLinearWSumMutable under FLEXIBLEAUTO: every input aligned with the one of fewest
+    // towers, all rescaled when that one is at depth 2
+    size_t mi = 0;
+    for (size_t i = 1; i < cts.size(); ++i)
+      if (cts[i]->towers < cts[mi]->towers) mi = i;
+    for (size_t i = 0; i < cts.size(); ++i)
+      if (i != mi) adjustLevelsAndDepth(cts[i], cts[mi]);
+    if (cts[mi]->noiseScaleDeg == 2)
+      for (auto& c : cts) c = modReduce(c);
+  }
   uint32_t t = ~0u;
   for (const auto& c : cts) {
     if (c->ncomp != cts[0]->ncomp || c->noiseScaleDeg != cts[0]->noiseScaleDeg)
@@ -724,7 +927,7 @@ Ciphertext CryptoContextImpl::EvalLinearWSum(const std::vector<Ciphertext>& cts,
     auto out = newCt(t, nc);
     check(fh_linear_wsum(ctx_, (int)n, ptrs.data(), towers.data(), w.data(), (int)t, (int)nc, nullptr, 0, out->data->ptr));
     out->noiseScaleDeg = cts[0]->noiseScaleDeg + 1;
-    out->scalingFactor = cts[0]->scalingFactor * scale_;
+    out->scalingFactor = cts[0]->scalingFactor * sfReal(t);
     out->slots = cts[0]->slots;
     acc = acc ? EvalAdd(acc, out) : out;
   }
@@ -741,15 +944,12 @@ Ciphertext CryptoContextImpl::EvalAddMany(const std::vector<Ciphertext>& cts) {
 // Ciphertext::rescale (src/CKKS/Ciphertext.cpp:569-607): drop the top tower, degree - 1, and divide
 // the tracked scale by OpenFHE's ModReduceFactor, which the FIXED techniques set to the nominal
 // scaling factor 2^scaleModSize (not q_l).
+// Under FLEXIBLEAUTO the operations rescale when needed and Rescale does nothing (OpenFHE's
+// ModReduceInPlace acts under FIXEDMANUAL only).
 Ciphertext CryptoContextImpl::Rescale(const Ciphertext& ct) {
   require_standard(*ct, "Rescale");
-  if (ct->towers < 2) throw std::runtime_error("fideslib: Rescale at the last tower");
-  auto out = newCt(ct->towers - 1, ct->ncomp);
-  check(fh_rescale(ctx_, ct->data->ptr, (int)ct->towers - 1, (int)ct->ncomp, out->data->ptr));
-  out->noiseScaleDeg = ct->noiseScaleDeg > 1 ? ct->noiseScaleDeg - 1 : 1;
-  out->scalingFactor = ct->scalingFactor / scale_;
-  out->slots = ct->slots;
-  return out;
+  if (flex()) return ct;
+  return modReduce(ct);
 }
 
 // OpenFHE FindAutomorphismIndex2nComplex: 5^index mod 2N, 5^-|index| for negative indices; 5 has
@@ -905,7 +1105,7 @@ CryptoContextImpl::PackedDiagonals CryptoContextImpl::PackDiagonals(const std::v
 Ciphertext CryptoContextImpl::LinearTransform(const Ciphertext& ct0, int rowSize, int bStep, const std::vector<Plaintext>& diags,
                                               int stride, int offset) {
   if ((int)diags.size() < rowSize) throw std::runtime_error("fideslib: LinearTransform needs rowSize diagonals");
-  Ciphertext ct = ct0->noiseScaleDeg == 2 ? Rescale(ct0) : ct0;  // rescale a fresh product first (LinearTransform.cu:77-78)
+  Ciphertext ct = ct0->noiseScaleDeg == 2 ? modReduce(ct0) : ct0;  // rescale a fresh product first (LinearTransform.cu:77-78)
   for (int k = 0; k < rowSize; ++k)
     if (!diags[k]->extended || diags[k]->towers != ct->towers)
       throw std::runtime_error("fideslib: diagonals must be extended plaintexts at the ciphertext's level");
@@ -1019,7 +1219,7 @@ std::vector<Plaintext> CryptoContextImpl::MakeCKKSPackedPlaintextDeviceMany(cons
   if (level >= L1_) throw std::runtime_error("fideslib: level out of range");
   const uint32_t n = slots ? slots : (params_.batch_ ? params_.batch_ : N_ / 2);
   const uint32_t towers = L1_ - level;
-  const double scale = std::pow(scale_, (double)noiseScaleDeg);
+  const double scale = ptScale(towers, noiseScaleDeg);
   // one batched device encode into a staging buffer, then one buffer per plaintext
   std::vector<double> z(values.size() * 2 * (size_t)n, 0.0);
   for (size_t b = 0; b < values.size(); ++b)
@@ -1046,14 +1246,14 @@ std::vector<Plaintext> CryptoContextImpl::MakeCKKSPackedPlaintextDeviceMany(cons
 
 Ciphertext CryptoContextImpl::Compress(const Ciphertext& ct, uint32_t towersLeft, size_t noiseScaleDeg) {
   Ciphertext c = ct;
-  while (c->noiseScaleDeg > noiseScaleDeg && c->towers > 1) c = Rescale(c);
+  while (c->noiseScaleDeg > noiseScaleDeg && c->towers > 1) c = modReduce(c);
   if (c->towers > towersLeft && towersLeft >= 1) c = LevelReduce(c, c->towers - towersLeft);
   return c;
 }
 
 Ciphertext CryptoContextImpl::LinearTransform(const Ciphertext& ct0, int bStep, const PackedDiagonals& diags, int stride, int offset) {
   require_standard(*ct0, "LinearTransform");
-  Ciphertext ct = ct0->noiseScaleDeg == 2 ? Rescale(ct0) : ct0;  // rescale a fresh product first (LinearTransform.cu:77-78)
+  Ciphertext ct = ct0->noiseScaleDeg == 2 ? modReduce(ct0) : ct0;  // rescale a fresh product first (LinearTransform.cu:77-78)
   const uint32_t t = ct->towers;
   if (!diags.data || diags.towers != t) throw std::runtime_error("fideslib: diagonals must be extended plaintexts at the ciphertext's level");
   const int rowSize = (int)diags.rowSize;

@@ -10,9 +10,10 @@
 // CiphertextImpl does (src/CKKS/Ciphertext.cpp:163-237, 569-607) and the scalar CRT encodings of
 // ElemForEvalMult / ElemForEvalAddOrSub (src/CKKS/Context.cu:669-863).
 //
-// Scope: the FIXEDMANUAL scaling technique (the caller rescales), HYBRID key switching,
-// uniform-ternary or sparse-ternary secrets. Errors are thrown as std::runtime_error, the
-// reference facade's contract.
+// Scope: the FIXEDMANUAL (the caller rescales) and FLEXIBLEAUTO (per-level scaling factors,
+// rescaling and level / depth alignment done by the operations) scaling techniques, HYBRID key
+// switching, uniform-ternary or sparse-ternary secrets. Errors are thrown as std::runtime_error,
+// the reference facade's contract.
 #pragma once
 #include <complex>
 #include <cstdint>
@@ -27,7 +28,7 @@
 
 namespace fideslib {
 
-enum class ScalingTechnique { FIXEDMANUAL };
+enum class ScalingTechnique { FIXEDMANUAL, FLEXIBLEAUTO };
 enum class SecretKeyDist { UNIFORM_TERNARY, SPARSE_TERNARY };
 
 // api/CCParams.hpp:37-62 (the setters this path reads)
@@ -357,6 +358,16 @@ class CryptoContextImpl {
 
   void Synchronize() const;
   double ScalingFactor() const { return scale_; }
+  ScalingTechnique GetScalingTechnique() const { return params_.tech_; }
+  // OpenFHE's per-level tables (level = towers consumed, OpenFHE's numbering): the scaling factor
+  // of a degree-1 ciphertext, its square for degree 2, and the factor a rescale divides by. Under
+  // FIXEDMANUAL all three are the nominal 2^scaleModSize; under FLEXIBLEAUTO the top level's factor
+  // is the top Q prime, each further level's the previous one squared over the prime dropped, and
+  // a rescale divides by that prime (CryptoParametersCKKSRNS, restated by the reference at
+  // src/CKKS/openfhe-interface/RawCiphertext.cu:305-317).
+  double GetScalingFactorReal(uint32_t level) const;
+  double GetScalingFactorRealBig(uint32_t level) const;
+  double GetModReduceFactor(uint32_t level) const;
 
  private:
   void check(int rc) const;
@@ -372,6 +383,29 @@ class CryptoContextImpl {
   Ciphertext accumulate(const Ciphertext& ct, int slots, int stride, int start);
   uint32_t galoisFor(int32_t index) const;
   void automorphismKey(const PrivateKey& sk, uint32_t galois);
+
+  // FLEXIBLEAUTO metadata algebra (OpenFHE's LeveledSHECKKSRNS, as the reference restates it in
+  // src/CKKS/Ciphertext.cpp:624-658, 1917-2070 and src/CKKS/Context.cu:669-863)
+  bool flex() const { return params_.tech_ == ScalingTechnique::FLEXIBLEAUTO && manual_ == 0; }
+  double sfReal(uint32_t towers) const { return flex() ? sfReal_[towers - 1] : scale_; }
+  double ptScale(uint32_t towers, size_t noiseScaleDeg) const;      // the scale a plaintext is encoded at
+  Ciphertext modReduce(const Ciphertext& ct) const;                 // rescale with the technique's metadata
+  Ciphertext multCore(const Ciphertext& ct, double operand) const;  // EvalMultCore(ct, double): degree + 1
+  Ciphertext adjustDown(const Ciphertext& c1, uint32_t towers2, uint32_t deg2, double scale2) const;
+  void adjustLevelsAndDepth(Ciphertext& a, Ciphertext& b) const;
+  void adjustLevelsAndDepthToOne(Ciphertext& a, Ciphertext& b) const;
+  Ciphertext morph(const Plaintext& pt) const;                      // a plaintext as a 1-component ciphertext
+  Ciphertext mulPlain(const Ciphertext& a, const Ciphertext& p) const;
+  Ciphertext addPlain(const Ciphertext& a, const Ciphertext& p, int op) const;
+  // FIXEDMANUAL semantics for the duration of a scope (the bootstrap and Chebyshev internals manage
+  // their rescales explicitly, with the scales they track)
+  struct ManualScope {
+    const CryptoContextImpl& cc;
+    explicit ManualScope(const CryptoContextImpl& c) : cc(c) { ++cc.manual_; }
+    ~ManualScope() { --cc.manual_; }
+  };
+  mutable int manual_ = 0;
+  std::vector<double> sfReal_, sfBig_, mrf_;   // by towers - 1
 
   std::shared_ptr<NativeContext> native_;
   fh_context* ctx_ = nullptr;

@@ -166,7 +166,7 @@ bool DeserializeFromFile(const std::string& filename, CryptoContext& obj, const 
   uint32_t v[10];
   for (auto& x : v)
     if (!get_u32(f, x)) return false;
-  if (v[8] != (uint32_t)ScalingTechnique::FIXEDMANUAL || v[9] > (uint32_t)SecretKeyDist::SPARSE_TERNARY) return false;
+  if (v[8] > (uint32_t)ScalingTechnique::FLEXIBLEAUTO || v[9] > (uint32_t)SecretKeyDist::SPARSE_TERNARY) return false;
   p.depth_ = v[0], p.scale_bits_ = v[1], p.first_bits_ = v[2], p.ring_dim_ = v[3], p.dnum_ = v[4], p.batch_ = v[5], p.hw_ = v[6];
   p.num_special_ = v[7];
   p.tech_ = (ScalingTechnique)v[8];

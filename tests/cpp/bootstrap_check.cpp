@@ -7,8 +7,8 @@
 // The inputs, outputs and bootstrap constants are written to <out_dir> so that the test can replay
 // the same sequence with the oracle and compare residues.
 //
-// usage: bootstrap_check <out_dir> <logN> <slots> [seed] [cts_budget stc_budget depth scale_bits hw]
-// (prints one JSON line)
+// usage: bootstrap_check <out_dir> <logN> <slots> [seed] [cts_budget stc_budget depth scale_bits hw technique]
+// technique: 0 FIXEDMANUAL (default), 1 FLEXIBLEAUTO. (prints one JSON line)
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -81,6 +81,7 @@ int main(int argc, char** argv) {
   const uint32_t depth = argc > 7 ? (uint32_t)std::atoi(argv[7]) : 15;
   const uint32_t sbits = argc > 8 ? (uint32_t)std::atoi(argv[8]) : 50;
   const uint32_t hw = argc > 9 ? (uint32_t)std::atoi(argv[9]) : 16;
+  const bool flex = argc > 10 && std::atoi(argv[10]) == 1;
   try {
     CCParams p;
     p.SetRingDim(1u << logN);
@@ -92,6 +93,7 @@ int main(int argc, char** argv) {
     p.SetHammingWeight(hw);
     p.SetBatchSize(slots);
     p.SetSeed(seed);
+    p.SetScalingTechnique(flex ? ScalingTechnique::FLEXIBLEAUTO : ScalingTechnique::FIXEDMANUAL);
     auto cc = GenCryptoContext(p);
     auto kp = cc->KeyGen();
     cc->EvalMultKeyGen(kp.secretKey);
@@ -104,8 +106,9 @@ int main(int argc, char** argv) {
     cvec z(slots);
     for (auto& v : z) v = {U(rng), U(rng)};
 
-    // a ciphertext with the towers the bootstrap needs left (one, two with a correction factor)
-    const uint32_t tin = cc->GetBootstrapConstants().correction ? 2 : 1;
+    // a ciphertext with the towers the bootstrap needs left (one, two with a correction factor or
+    // under FLEXIBLEAUTO, whose scale adjustment takes a rescale)
+    const uint32_t tin = cc->GetBootstrapConstants().correction || flex ? 2 : 1;
     auto pt = cc->MakeCKKSPackedPlaintext(z, 1, L1 - tin, slots);
     auto ct = cc->Encrypt(kp.secretKey, pt);
     auto bt = cc->EvalBootstrap(ct);
@@ -129,7 +132,7 @@ int main(int argc, char** argv) {
     }
     std::sort(ms.begin(), ms.end());
     // the bootstrapped ciphertext computes: one more product
-    auto sq = cc->Rescale(cc->EvalMult(bt, bt));
+    auto sq = cc->Rescale(cc->EvalMult(bt, bt));   // (Rescale does nothing under FLEXIBLEAUTO; Decrypt rescales)
     auto dsq = cc->Decrypt(kp.secretKey, sq);
     cvec zz(slots);
     for (uint32_t i = 0; i < slots; ++i) zz[i] = z[i] * z[i];
@@ -177,9 +180,9 @@ int main(int argc, char** argv) {
     }
     {
       FILE* f = std::fopen((g_dir + "/meta.json").c_str(), "w");
-      std::fprintf(f, "{\"logN\": %d, \"slots\": %u, \"seed\": %llu, \"L1\": %u, \"K\": %u, \"dnum\": 3, \"hw\": %u, "
+      std::fprintf(f, "{\"flex\": %d, \"logN\": %d, \"slots\": %u, \"seed\": %llu, \"L1\": %u, \"K\": %u, \"dnum\": 3, \"hw\": %u, "
                       "\"scale\": %.17g, \"Krange\": %.17g, \"r\": %d, \"pre\": %.17g, \"correction\": %u, %s, \"moduli\": [",
-                   logN, slots, (unsigned long long)seed, L1, (unsigned)(cc->GetModuli().size() - L1), hw, cc->ScalingFactor(), bc.K, bc.r, bc.pre,
+                   flex ? 1 : 0, logN, slots, (unsigned long long)seed, L1, (unsigned)(cc->GetModuli().size() - L1), hw, cc->ScalingFactor(), bc.K, bc.r, bc.pre,
                    bc.correction, groups.c_str());
       for (size_t i = 0; i < cc->GetModuli().size(); ++i)
         std::fprintf(f, "%s\"%llu\"", i ? ", " : "", (unsigned long long)cc->GetModuli()[i]);
@@ -187,10 +190,10 @@ int main(int argc, char** argv) {
       std::fclose(f);
     }
 
-    std::printf("{\"logN\": %d, \"slots\": %u, \"budget\": [%u, %u], \"L1\": %u, \"depth\": %u, \"towers_in\": %u, \"towers_out\": %u, "
+    std::printf("{\"technique\": \"%s\", \"logN\": %d, \"slots\": %u, \"budget\": [%u, %u], \"L1\": %u, \"depth\": %u, \"towers_in\": %u, \"towers_out\": %u, "
                 "\"boot_err\": %.6g, \"boot_bits\": %.2f, \"deterministic\": %s, \"square_err\": %.6g, "
                 "\"cheb_towers_out\": %u, \"cheb_err_vs_series\": %.6g, \"cheb_err_vs_function\": %.6g, \"boot_ms\": %.3f}\n",
-                logN, slots, b1, b2, L1, cc->GetBootstrapDepth(), ct->towers, bt->towers, e1, -std::log2(e1), same ? "true" : "false", e2,
+                flex ? "FLEXIBLEAUTO" : "FIXEDMANUAL", logN, slots, b1, b2, L1, cc->GetBootstrapDepth(), ct->towers, bt->towers, e1, -std::log2(e1), same ? "true" : "false", e2,
                 cy->towers, e3, e4, ms[1]);
   } catch (const std::exception& ex) {
     std::fprintf(stderr, "error: %s\n", ex.what());

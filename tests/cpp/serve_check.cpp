@@ -5,7 +5,11 @@
 // line with whether every request's result is the same in all three (tests/test_gpu_serve.py).
 //
 // usage: serve_check <logN> <slots> <cts_budget> <stc_budget> <depth> <scale_bits> <rate_per_s>
-//                    <requests> <bootstrap_share> [seed] [inference_share] [hamming_weight]
+//                    <requests> <bootstrap_share> [seed] [inference_share] [hamming_weight] [technique]
+// technique 1: the facade context (and so the bootstrap requests' EvalBootstrap) is FLEXIBLEAUTO;
+// the loop's other request programs are integer programs at the C ABI, the same under either.
+// Every request's whole result is captured in each session and compared bit for bit.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <stdexcept>
@@ -20,13 +24,16 @@ struct BootUser {
   CryptoContextImpl* cc;
   uint32_t slots;
   int calls;
+  bool flex;
 };
 
 // the loop's bootstrap callback: the request register as a facade ciphertext, bootstrapped
 static int boot_cb(void* user, const uint64_t* in, int level, uint64_t* out, int* out_level) {
   auto* u = static_cast<BootUser*>(user);
   try {
-    auto ct = u->cc->LoadCiphertext(in, (uint32_t)level + 1, 2, 1, u->cc->ScalingFactor(), u->slots);
+    // a fresh request ciphertext carries its level's scaling factor
+    const double scale = u->flex ? u->cc->GetScalingFactorReal(u->cc->GetNumQ() - 1 - (uint32_t)level) : u->cc->ScalingFactor();
+    auto ct = u->cc->LoadCiphertext(in, (uint32_t)level + 1, 2, 1, scale, u->slots);
     auto bt = u->cc->EvalBootstrap(ct);
     u->cc->StoreCiphertext(bt, out);
     *out_level = (int)bt->towers - 1;
@@ -64,6 +71,7 @@ int main(int argc, char** argv) {
   const uint64_t seed = argc > 10 ? std::strtoull(argv[10], nullptr, 10) : 5;
   const double inf_share = argc > 11 ? std::atof(argv[11]) : 0.5;
   const uint32_t hw = argc > 12 ? (uint32_t)std::atoi(argv[12]) : 16;
+  const bool flex = argc > 13 && std::atoi(argv[13]) == 1;
   try {
     CCParams p;
     p.SetRingDim(1u << logN);
@@ -75,6 +83,7 @@ int main(int argc, char** argv) {
     p.SetHammingWeight(hw);
     p.SetBatchSize(slots);
     p.SetSeed(seed);
+    p.SetScalingTechnique(flex ? ScalingTechnique::FLEXIBLEAUTO : ScalingTechnique::FIXEDMANUAL);
     auto cc = GenCryptoContext(p);
     auto kp = cc->KeyGen();
     cc->EvalMultKeyGen(kp.secretKey);
@@ -92,9 +101,18 @@ int main(int argc, char** argv) {
       if (fh_galois_index(cc->Native(), s, &g) != FH_OK) throw std::runtime_error(fh_last_error());
       rg.push_back(g);
     }
-    BootUser user{cc.get(), slots, 0};
+    BootUser user{cc.get(), slots, 0, flex};
     fh_serve_result res[4];
     std::vector<std::vector<uint64_t>> per(4, std::vector<uint64_t>(requests, 0));
+    // whole results of the batched, per-operation and alone sessions
+    const size_t cap = 2ull * cc->GetNumQ() * cc->GetRingDimension();
+    std::vector<std::vector<std::vector<uint64_t>>> outs(4);
+    std::vector<std::vector<uint64_t*>> optr(4);
+    std::vector<std::vector<uint64_t>> owords(4, std::vector<uint64_t>(requests, 0));
+    for (int run = 1; run < 4; ++run) {
+      outs[run].assign(requests, std::vector<uint64_t>(cap));
+      for (int i = 0; i < requests; ++i) optr[run].push_back(outs[run][i].data());
+    }
     // a warm-up session (per operation, discarded) fills the context's buffer pool and the
     // bootstrap's first-use state, so that no measured session pays for them; then batched, per
     // operation, and one request at a time (each alone)
@@ -113,16 +131,30 @@ int main(int argc, char** argv) {
       cfg.bootstrap = boot_cb;
       cfg.bootstrap_user = &user;
       cfg.request_checksums = per[run].data();
+      if (run > 0) {
+        cfg.request_outputs = optr[run].data();
+        cfg.request_output_capacity = cap;
+        cfg.request_output_words = owords[run].data();
+      }
       if (fh_serve_run(cc->Native(), &cfg, rk.data(), rg.data(), cc->RelinKeyHandle(), &res[run]) != FH_OK) {
         std::fprintf(stderr, "fh_serve_run: %s\n", fh_last_error());
         return 1;
       }
     }
     const bool same = per[1] == per[3] && per[2] == per[3];
+    bool bitwise = true;
+    for (int i = 0; i < requests; ++i) {
+      const size_t w = owords[3][i];
+      bitwise = bitwise && w > 0 && owords[1][i] == w && owords[2][i] == w &&
+                std::equal(outs[1][i].begin(), outs[1][i].begin() + w, outs[3][i].begin()) &&
+                std::equal(outs[2][i].begin(), outs[2][i].begin() + w, outs[3][i].begin());
+    }
     std::printf("{\"logN\": %d, \"slots\": %u, \"L1\": %u, \"boot_depth\": %u, \"rate_per_s\": %.3f, \"requests\": %d, "
-                "\"bootstrap_share\": %.3f, \"callbacks\": %d, \"each_result_same_alone_and_in_mix\": %s, \"batched\": %s, "
+                "\"bootstrap_share\": %.3f, \"callbacks\": %d, \"technique\": \"%s\", \"each_result_same_alone_and_in_mix\": %s, "
+                "\"bitwise_same_alone_and_in_mix\": %s, \"batched\": %s, "
                 "\"per_op\": %s, \"alone\": %s}\n",
-                logN, slots, cc->GetNumQ(), cc->GetBootstrapDepth(), rate, requests, boot_share, user.calls, same ? "true" : "false",
+                logN, slots, cc->GetNumQ(), cc->GetBootstrapDepth(), rate, requests, boot_share, user.calls, flex ? "FLEXIBLEAUTO" : "FIXEDMANUAL",
+                same ? "true" : "false", bitwise ? "true" : "false",
                 result_json(res[1]).c_str(), result_json(res[2]).c_str(), result_json(res[3]).c_str());
   } catch (const std::exception& ex) {
     std::fprintf(stderr, "error: %s\n", ex.what());

@@ -34,22 +34,29 @@ BCHECK = os.path.join(PKG, "_build", "bootstrap_check")
 
 
 # (logN, slots, CoeffsToSlots budget, SlotsToCoeffs budget, multiplicative depth, scaling bits,
-# bits required, secret weight); the CPU model of the same sequences measures 15.8, 15.0, 10.0 and
-# 21.4 bits; weight 32 takes a degree-255 series
-RUNS = [(12, 8, 1, 1, 15, 50, 14, 16), (12, 64, 2, 3, 17, 50, 13, 16), (12, 2048, 3, 3, 18, 50, 9, 16), (12, 2048, 3, 3, 19, 59, 19, 16),
-        (12, 2048, 3, 3, 20, 59, 15, 32)]
+# bits required, secret weight, technique 0 FIXEDMANUAL / 1 FLEXIBLEAUTO); the CPU model of the same
+# sequences measures 15.8, 15.0, 10.0 and 21.4 bits; weight 32 takes a degree-255 series. The N=2^16
+# runs are the serving workload's bootstrap (all 32768 slots, budget {3,3}, h=32, 59-bit scaling
+# under a 60-bit q_0): the reference states 11.5-15 bits for its GPU path at N=2^16
+# (test/BootstrapPrecisionTests.cu:4-7), the bound they are held to; their residues are not replayed
+# (the oracle's bootstrap at N=2^16 takes too long), nor those of FLEXIBLEAUTO runs (this
+# bootstrap's own scale adjustment, not a reference sequence).
+RUNS = [(12, 8, 1, 1, 15, 50, 14, 16, 0), (12, 64, 2, 3, 17, 50, 13, 16, 0), (12, 2048, 3, 3, 18, 50, 9, 16, 0),
+        (12, 2048, 3, 3, 19, 59, 19, 16, 0), (12, 2048, 3, 3, 20, 59, 15, 32, 0), (12, 2048, 3, 3, 20, 59, 15, 32, 1),
+        (16, 32768, 3, 3, 24, 59, 11.5, 32, 0), (16, 32768, 3, 3, 24, 59, 11.5, 32, 1)]
 
 
 @pytest.fixture(scope="module", params=RUNS, ids=["N4096_slots8_b11", "N4096_slots64_b23", "N4096_full_b33", "N4096_full_b33_scale59",
-                                                  "N4096_full_b33_scale59_h32"])
+                                                  "N4096_full_b33_scale59_h32", "N4096_full_b33_scale59_h32_flexibleauto",
+                                                  "N65536_full_b33_scale59_h32", "N65536_full_b33_scale59_h32_flexibleauto"])
 def boot_run(request, tmp_path_factory):
-    logN, slots, b1, b2, depth, sbits, bits, hw = request.param
+    logN, slots, b1, b2, depth, sbits, bits, hw, tech = request.param
     if not os.path.exists(BCHECK):
         pytest.fail("bootstrap_check is not built (run __graft_entry__.build())")
     out = tmp_path_factory.mktemp(f"boot{slots}")
-    args = [str(out), str(logN), str(slots), "7", str(b1), str(b2), str(depth), str(sbits), str(hw)]
+    args = [str(out), str(logN), str(slots), "7", str(b1), str(b2), str(depth), str(sbits), str(hw), str(tech)]
     print(f"[bootstrap_check] {BCHECK} {' '.join(args)}", flush=True)
-    r = subprocess.run([BCHECK] + args, capture_output=True, text=True, timeout=300)
+    r = subprocess.run([BCHECK] + args, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
     res = json.loads(r.stdout.strip().splitlines()[-1])
     with open(out / "meta.json") as f:
@@ -64,7 +71,7 @@ def test_bootstrap_precision(boot_run):
     product."""
     out, res, meta = boot_run
     print(res)
-    assert res["towers_in"] == (2 if meta["correction"] else 1) and res["towers_out"] == res["L1"] - res["depth"]
+    assert res["towers_in"] == (2 if meta["correction"] or meta["flex"] else 1) and res["towers_out"] == res["L1"] - res["depth"]
     assert res["boot_err"] < 2.0 ** -res["bits_required"], res
     assert res["deterministic"]
     assert res["square_err"] < 4 * 2.0 ** -res["bits_required"]
@@ -93,6 +100,8 @@ def test_bootstrap_matches_oracle_replay(boot_run, oracle_mod):
     """The oracle replays the session (keys and encryptions from the same seeds, the same
     bootstrap and series sequences over the dumped constants): every residue agrees."""
     out, res, meta = boot_run
+    if meta["logN"] > 12 or meta["flex"]:
+        pytest.skip("residue replay at N=2^12 with FIXEDMANUAL only (the precision test covers this run)")
     O = oracle_mod
     logN, n, seed, L1, K, scale = meta["logN"], meta["slots"], meta["seed"], meta["L1"], meta["K"], meta["scale"]
     N = 1 << logN

@@ -1,6 +1,7 @@
 """The continuous-batching serving loop (fh_serve_run): every request completes, the fused
 mixed-level batches produce exactly the results of one-call-per-operation execution (checksum of
 every request's result), and latency statistics are consistent."""
+import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
@@ -42,15 +43,20 @@ def test_serving_batched_equals_per_op(sctx, rate, requests):
 
 
 def test_each_result_same_alone_and_in_the_mix(sctx):
-    """Every request's result (its result-word sum) is the same in the batched mix, in per-operation
+    """Every request's result is bit for bit the same in the batched mix, in per-operation
     execution and when it runs alone (mode 2: one request admitted at a time)."""
     rk, rg, relin = _keys(sctx, 83)
-    runs = {mode: sctx.serve(rk, rg, relin, 3000.0, 12, mode=mode, seed=9, request_checksums=True) for mode in (0, 1, 2)}
+    runs = {mode: sctx.serve(rk, rg, relin, 3000.0, 12, mode=mode, seed=9, request_checksums=True, request_outputs=True)
+            for mode in (0, 1, 2)}
     assert runs[2]["max_active"] == 1 and runs[0]["max_active"] >= 3
     per = {mode: r["request_checksums"] for mode, r in runs.items()}
     assert all(v != 0 for v in per[2])
-    assert per[0] == per[2], [i for i in range(12) if per[0][i] != per[2][i]]
-    assert per[1] == per[2]
+    outs = {mode: r["request_outputs"] for mode, r in runs.items()}
+    for i in range(12):
+        assert outs[2][i].size > 0
+        assert np.array_equal(outs[0][i], outs[2][i]), f"request {i}: batched result differs from the request alone"
+        assert np.array_equal(outs[1][i], outs[2][i]), f"request {i}: per-operation result differs from the request alone"
+    assert per[0] == per[2] and per[1] == per[2]
     for k in rk + [relin]:
         k.free()
 
@@ -104,3 +110,28 @@ def test_serving_with_bootstrap_requests(share):
     assert res["callbacks"] == 4 * b["bootstraps"]  # warm-up, batched, per-operation and alone sessions
     assert b["checksum"] == p["checksum"], "fused batches differ from per-operation execution"
     assert res["each_result_same_alone_and_in_mix"]
+    assert res["bitwise_same_alone_and_in_mix"], "a request's result differs between the mix and the request alone"
+
+
+@pytest.mark.parametrize("technique", [0, 1], ids=["fixedmanual", "flexibleauto"])
+def test_serving_workload_n65536_with_bootstraps(technique):
+    """The serving workload's own shape: N=2^16, 25 towers, real keys (sparse ternary h=32), all
+    32768 slots, bootstrap level budget {3,3} with 59-bit scaling; arrivals at 50/s, 70% inference,
+    25% polynomial and 5% bootstrap requests (the bootstrap requests under the context's scaling
+    technique). Every request's result is bit for bit the same in the batched mix, in
+    per-operation execution and alone."""
+    if not os.path.exists(SCHECK):
+        pytest.fail("serve_check is not built (run __graft_entry__.build())")
+    args = ["16", "32768", "3", "3", "24", "59", "50", "40", "0.05", "2026", "0.70", "32", str(technique)]
+    print(f"[serve_check] {SCHECK} {' '.join(args)}", flush=True)
+    r = subprocess.run([SCHECK] + args, capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    print({k: res[k] for k in ("technique", "callbacks", "each_result_same_alone_and_in_mix", "bitwise_same_alone_and_in_mix")})
+    b, p, a = res["batched"], res["per_op"], res["alone"]
+    for x in (b, p, a):
+        assert x["completed"] == 40
+        assert x["inference"] + x["polynomial"] + x["bootstraps"] == 40
+    assert b["bootstraps"] == p["bootstraps"] == a["bootstraps"] > 0, "the seeded mix holds bootstrap requests"
+    assert a["max_active"] == 1 and b["max_active"] >= 2
+    assert res["bitwise_same_alone_and_in_mix"], "a request's result differs between the mix and the request alone"
