@@ -400,6 +400,7 @@ def ntt_run(fh, ctx, a):
     gbs = 2 * limbs * 16 * N / t / 1e9
     return {"limb_transforms": 2 * limbs, "ms_fwd_inv": round(1000 * t, 3), "gbps": round(gbs, 1),
             "roofline_frac": round(gbs / HBM_PEAK_GBS, 4), "team": ctx.tune_get("ntt_team"),
+            "team_per_xcd": ctx.tune_get("ntt_team_per_xcd"), "teams_spanning_xcds": ctx.tune_get("ntt_team_split_teams"),
             "avg_launch_ms": {f: round(v[1] / v[0], 4) for f, v in prof.items() if v[0]}}
 
 
@@ -664,20 +665,22 @@ def bootstrap_run():
     """EvalBootstrap through the C++ facade (tests/cpp/bootstrap_check.cpp) at N=2^16 with all
     32768 slots, level budget {3, 3}, 59-bit scaling under a 60-bit q_0 (correction factor 9),
     sparse ternary secret h=32 (one double angle), 25 towers: median latency of three bootstraps
-    and the decryption error of the bootstrapped slots."""
+    and the decryption error of the bootstrapped slots; the same for the two-iteration bootstrap
+    (Meta-BTS) at precision 12."""
     import subprocess
     import tempfile
     exe = os.path.join(ROOT, "fideslib-fork_amd", "_build", "bootstrap_check")
     if not os.path.exists(exe):
         return {"error": "bootstrap_check is not built"}
     with tempfile.TemporaryDirectory() as d:
-        r = subprocess.run([exe, d, "16", "32768", "7", "3", "3", "24", "59", "32"], capture_output=True, text=True, timeout=600)
+        r = subprocess.run([exe, d, "16", "32768", "7", "3", "3", "24", "59", "32", "0", "12"], capture_output=True, text=True, timeout=600)
     if r.returncode != 0:
         return {"error": (r.stdout + r.stderr)[-500:]}
     res = json.loads(r.stdout.strip().splitlines()[-1])
     return {"workload": "N=2^16, 32768 slots, level budget {3,3}, 59-bit scaling, sparse ternary h=32", "ms": res["boot_ms"],
             "bits": res["boot_bits"], "towers_in": res["towers_in"], "towers_out": res["towers_out"], "depth": res["depth"],
-            "deterministic": res["deterministic"]}
+            "deterministic": res["deterministic"],
+            "meta_bts": {"iterations": 2, "precision": res["meta_precision"], "bits": res["meta_bits"], "ms": res["meta_ms"]}}
 
 
 def serving_bootstrap_run(a):

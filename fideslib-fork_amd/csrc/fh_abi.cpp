@@ -886,6 +886,14 @@ int fh_tune_get(const fh_context* ctx, const char* name, int* value) {
   return guard([&] {
     auto& c = const_cast<fh::Context&>(C(ctx));
     need(value, "value");
+    if (name && !std::strcmp(name, "ntt_team_split_teams")) {
+      // diagnostic (read only): team launches so far whose members spanned XCDs (synchronizes)
+      uint32_t v = 0;
+      FH_HIP(hipStreamSynchronize(c.stream));
+      FH_HIP(hipMemcpy(&v, c.team_err() + 1, sizeof(v), hipMemcpyDeviceToHost));
+      *value = (int)v;
+      return;
+    }
     *value = *tune_slot(c.tune, name);
   });
 }

@@ -487,7 +487,7 @@ Context::Context(int logN_, int L1_, int K_, int dnum_, const std::vector<uint64
   d_rs_QlQl_sh = upload(QlQl_sh);
 
   {
-    const size_t words = (size_t)kMaxTeams * 17 + 64;
+    const size_t words = (size_t)kMaxTeams * 19 + 64;
     FH_HIP(hipMalloc(&d_team, words * sizeof(uint32_t)));
     owned.push_back(d_team);
     FH_HIP(hipMemset(d_team, 0, words * sizeof(uint32_t)));

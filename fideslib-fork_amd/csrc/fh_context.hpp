@@ -213,13 +213,14 @@ struct Context {
   // host copies of the conversion constants (for table checks and host helpers)
   std::vector<uint64_t> h_pmod, h_pinv;
 
-  // team NTT launches: arrival counters [kMaxTeams], the XCD of every member [kMaxTeams][16] and an
-  // error word (a barrier wait ran past its bound), zeroed at creation
+  // team NTT launches: arrival counters [kMaxTeams][3], the XCD of every member [kMaxTeams][16], an
+  // error word (a barrier wait ran past its bound) and the count of teams that spanned XCDs,
+  // zeroed at creation
   static constexpr int kMaxTeams = 1024;
   uint32_t* d_team = nullptr;
   uint32_t* team_ctr() const { return d_team; }
-  uint32_t* team_xcc() const { return d_team + kMaxTeams; }
-  uint32_t* team_err() const { return d_team + kMaxTeams * 17; }
+  uint32_t* team_xcc() const { return d_team + 3 * kMaxTeams; }
+  uint32_t* team_err() const { return d_team + 19 * kMaxTeams; }
   int cu_count = 0;
 
   hipStream_t stream = nullptr;

@@ -603,9 +603,9 @@ __global__ __launch_bounds__(TH) void k_ntt_inv_cols(Addr ad, NttTables tb) {
 constexpr uint32_t kTeamSpinLimit = 1u << 20;
 
 struct TeamArgs {
-  uint32_t* ctr;      // [teams] arrival counters, zero at launch
+  uint32_t* ctr;      // [teams][3] arrival counters (placement, two limb slots), zero at launch
   uint32_t* xcc;      // [teams][W] XCC id of each member
-  uint32_t* err;      // error word
+  uint32_t* err;      // error word; err[1] counts teams whose members spanned XCDs
   uint32_t njobs, nteams, W;
 };
 
@@ -613,16 +613,21 @@ __device__ __forceinline__ uint32_t xcc_id() { return (uint32_t)__builtin_amdgcn
 
 __device__ __forceinline__ uint32_t team_poll(uint32_t* p) { return __hip_atomic_fetch_add(p, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
-// All W members arrive (target = W times the number of barriers so far); false when the launch is
-// being abandoned. formal: agent-scope fences around the barrier (members on different XCDs).
-__device__ __forceinline__ bool team_barrier(const TeamArgs& ta, uint32_t team, uint32_t target, bool formal, int* sflag) {
+// Arrival of this workgroup at a team counter (after its stores are visible to the team: the
+// wave's stores acknowledged by the L2, or an agent-scope release when the team spans XCDs).
+__device__ __forceinline__ void team_arrive(uint32_t* ctr, bool formal) {
   if (formal) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  else __builtin_amdgcn_s_waitcnt(0);   // this wave's stores are acknowledged by the L2
+  else __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wait until the counter reaches target; false when the launch is being abandoned (a wait ran past
+// its bound here or elsewhere).
+__device__ __forceinline__ bool team_wait(const TeamArgs& ta, uint32_t* ctr, uint32_t target, bool formal, int* sflag) {
   if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(ta.ctr + team, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     int ok = 1;
-    for (uint32_t spins = 0; team_poll(ta.ctr + team) < target; ++spins) {
+    for (uint32_t spins = 0; team_poll(ctr) < target; ++spins) {
       if ((spins & 63) == 63 && (spins > kTeamSpinLimit || team_poll(ta.err))) {
         __hip_atomic_fetch_or(ta.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ok = 0;
@@ -634,31 +639,32 @@ __device__ __forceinline__ bool team_barrier(const TeamArgs& ta, uint32_t team, 
   }
   __syncthreads();
   const bool ok = *sflag != 0;
+  __syncthreads();   // every thread has read the flag before it is written again
   if (formal) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return ok;
 }
 
-// A copy of v the compiler cannot compute before this point: work addressed through it (a pass's
-// element addresses) is not hoisted above the preceding barrier, where it would hold registers
-// across the other pass.
+// A copy of v the compiler cannot compute before this point.
 __device__ __forceinline__ uint32_t opaque(uint32_t v) {
   asm volatile("" : "+s"(v));
   return v;
 }
 
 // Team identity of this workgroup and the launch-time placement check; returns false to leave.
+// Counters: [team][3]: placement barrier, then the two alternating limb slots.
 __device__ __forceinline__ bool team_setup(const TeamArgs& ta, uint32_t& team, uint32_t& m, bool& formal, int* sflag) {
   const uint32_t b = blockIdx.x, j = b / 8;
   team = (b % 8) + 8 * (j / ta.W);
   m = j % ta.W;
   if (threadIdx.x == 0) ta.xcc[team * ta.W + m] = xcc_id();
-  if (!team_barrier(ta, team, ta.W, true, sflag)) return false;
-  __syncthreads();   // every thread has read the barrier's flag
+  team_arrive(ta.ctr + 3 * team, true);
+  if (!team_wait(ta, ta.ctr + 3 * team, ta.W, true, sflag)) return false;
   if (threadIdx.x == 0) {
     int same = 1;
     const uint32_t x0 = ta.xcc[team * ta.W];
     for (uint32_t i = 1; i < ta.W; ++i) same &= ta.xcc[team * ta.W + i] == x0;
     *sflag = same;
+    if (!same && m == 0) __hip_atomic_fetch_add(ta.err + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);   // diagnostic count
   }
   __syncthreads();
   formal = *sflag == 0;
@@ -666,9 +672,11 @@ __device__ __forceinline__ bool team_setup(const TeamArgs& ta, uint32_t& team, u
   return true;
 }
 
+// Each workgroup runs its limbs software-pipelined: the first pass of the team's next limb is done
+// before waiting for the current limb's team barrier, so the barrier's latency hides behind a pass
+// of work; two limbs per team are in flight (two alternating counter slots, targets W per use).
 // One launch per arithmetic (FP: the fp64 path of the 50-bit primes, else integer Shoup): a team
-// skips the limbs of the other kind, and each kernel holds one path's registers only (both paths in
-// one kernel double its register use).
+// skips the limbs of the other kind, and each kernel holds one path's registers only.
 template <int LS, int TH, bool FP, class Addr, class AddrOut, class Epi>
 __global__ __launch_bounds__(TH) void k_ntt_fwd_team(Addr ad, AddrOut ao, NttTables tb, Epi epi, TeamArgs ta) {
   constexpr int S = 1 << LS, T = S / 16, CMAX = TH / T;
@@ -680,34 +688,68 @@ __global__ __launch_bounds__(TH) void k_ntt_fwd_team(Addr ad, AddrOut ao, NttTab
   bool formal;
   if (!team_setup(ta, team, m, formal, &sflag)) return;
   const uint32_t N2 = tb.N >> LS;
-  // tile width as a run-time value (= CMAX): a compile-time width lets the compiler precompute all
-  // 32 element addresses of a pass up front, which doubles the kernel's register use
   const int C = (int)(blockDim.x / T);
-  uint32_t nb = 1;   // barriers passed
-  for (uint32_t y = team; y < ta.njobs; y += ta.nteams) {
+  auto mine = [&](uint32_t y) {
+    uint64_t* p;
+    int pid;
+    ad.at(y, tb.N, p, pid);
+    return (tb.prime[pid].fp64 != 0) == FP;
+  };
+  auto next = [&](uint32_t y) {   // the team's next limb of this kernel's arithmetic after y
+    for (y += ta.nteams; y < ta.njobs && !mine(y); y += ta.nteams) {}
+    return y;
+  };
+  uint32_t* slot = ta.ctr + 3 * team + 1;
+  uint32_t uses[2] = {0, 0};
+  auto pass1 = [&](uint32_t y, uint32_t k) {
     uint64_t *p, *po;
     int pid, pido;
     ad.at(y, tb.N, p, pid);
-    if ((tb.prime[pid].fp64 != 0) != FP) continue;   // the other launch's limb (uniform across the team)
     ao.at(y, tb.N, po, pido);
-    Epi e = epi;
-    e.init(y, tb.N, tb.prime);
     const Sel sel = select(tb, pid, false);
-    __syncthreads();   // the previous limb's readers of sw / tile are done
+    __syncthreads();   // earlier readers of sw / tile are done
     if constexpr (FP) {
       double* swd = reinterpret_cast<double*>(sw);
       for (int i = threadIdx.x; i < S; i += TH) swd[i] = sel.twd[i];
       __syncthreads();
       fwd_cols_body<LS>(sel.fa, (const double*)swd, (const double*)swd, p, po, tile, N2, C, opaque(m));
-      if (!team_barrier(ta, team, ta.W * ++nb, formal, &sflag)) return;
-      fwd_rows_body<LS, L2Ld>(sel.fa, (const double*)swd, (const double*)swd, sel.tsd, sel.tsd, po, tile, C, e, opaque(m));
     } else {
       for (int i = threadIdx.x; i < S; i += TH) { sw[i] = sel.tw[i]; swp[i] = sel.twsh[i]; }
       __syncthreads();
       fwd_cols_body<LS>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, p, po, tile, N2, C, opaque(m));
-      if (!team_barrier(ta, team, ta.W * ++nb, formal, &sflag)) return;
+    }
+    team_arrive(slot + (k & 1), formal);
+  };
+  auto pass2 = [&](uint32_t y) {
+    uint64_t *p, *po;
+    int pid, pido;
+    ad.at(y, tb.N, p, pid);
+    ao.at(y, tb.N, po, pido);
+    Epi e = epi;
+    e.init(y, tb.N, tb.prime);
+    const Sel sel = select(tb, pid, false);
+    __syncthreads();
+    if constexpr (FP) {
+      double* swd = reinterpret_cast<double*>(sw);
+      for (int i = threadIdx.x; i < S; i += TH) swd[i] = sel.twd[i];
+      __syncthreads();
+      fwd_rows_body<LS, L2Ld>(sel.fa, (const double*)swd, (const double*)swd, sel.tsd, sel.tsd, po, tile, C, e, opaque(m));
+    } else {
+      for (int i = threadIdx.x; i < S; i += TH) { sw[i] = sel.tw[i]; swp[i] = sel.twsh[i]; }
+      __syncthreads();
       fwd_rows_body<LS, L2Ld>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, sel.ts, sel.tssh, po, tile, C, e, opaque(m));
     }
+  };
+  uint32_t y = team < ta.njobs && mine(team) ? team : next(team);
+  uint32_t k = 0;
+  if (y < ta.njobs) pass1(y, k);
+  while (y < ta.njobs) {
+    const uint32_t yn = next(y);
+    if (yn < ta.njobs) pass1(yn, k + 1);
+    if (!team_wait(ta, slot + (k & 1), ta.W * ++uses[k & 1], formal, &sflag)) return;
+    pass2(y);
+    y = yn;
+    ++k;
   }
 }
 
@@ -723,12 +765,41 @@ __global__ __launch_bounds__(TH) void k_ntt_inv_team(Addr ad, AddrOut ao, NttTab
   if (!team_setup(ta, team, m, formal, &sflag)) return;
   const uint32_t N2 = tb.N >> LS;
   const int C = (int)(blockDim.x / T);
-  uint32_t nb = 1;
-  for (uint32_t y = team; y < ta.njobs; y += ta.nteams) {
+  auto mine = [&](uint32_t y) {
+    uint64_t* p;
+    int pid;
+    ad.at(y, tb.N, p, pid);
+    return (tb.prime[pid].fp64 != 0) == FP;
+  };
+  auto next = [&](uint32_t y) {
+    for (y += ta.nteams; y < ta.njobs && !mine(y); y += ta.nteams) {}
+    return y;
+  };
+  uint32_t* slot = ta.ctr + 3 * team + 1;
+  uint32_t uses[2] = {0, 0};
+  auto pass1 = [&](uint32_t y, uint32_t k) {
     uint64_t *p, *po;
     int pid, pido;
     ad.at(y, tb.N, p, pid);
-    if ((tb.prime[pid].fp64 != 0) != FP) continue;
+    ao.at(y, tb.N, po, pido);
+    const Sel sel = select(tb, pid, true);
+    __syncthreads();
+    if constexpr (FP) {
+      double* swd = reinterpret_cast<double*>(sw);
+      for (int i = threadIdx.x; i < S; i += TH) swd[i] = sel.twd[i];
+      __syncthreads();
+      inv_rows_body<LS>(sel.fa, (const double*)swd, (const double*)swd, sel.tsd, sel.tsd, p, po, tile, C, opaque(m));
+    } else {
+      for (int i = threadIdx.x; i < S; i += TH) { sw[i] = sel.tw[i]; swp[i] = sel.twsh[i]; }
+      __syncthreads();
+      inv_rows_body<LS>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, sel.ts, sel.tssh, p, po, tile, C, opaque(m));
+    }
+    team_arrive(slot + (k & 1), formal);
+  };
+  auto pass2 = [&](uint32_t y) {
+    uint64_t *p, *po;
+    int pid, pido;
+    ad.at(y, tb.N, p, pid);
     ao.at(y, tb.N, po, pido);
     const Sel sel = select(tb, pid, true);
     const PrimeConst& P = tb.prime[pid];
@@ -737,16 +808,24 @@ __global__ __launch_bounds__(TH) void k_ntt_inv_team(Addr ad, AddrOut ao, NttTab
       double* swd = reinterpret_cast<double*>(sw);
       for (int i = threadIdx.x; i < S; i += TH) swd[i] = sel.twd[i];
       __syncthreads();
-      inv_rows_body<LS>(sel.fa, (const double*)swd, (const double*)swd, sel.tsd, sel.tsd, p, po, tile, C, opaque(m));
-      if (!team_barrier(ta, team, ta.W * ++nb, formal, &sflag)) return;
       inv_cols_body<LS, L2Ld>(sel.fa, (const double*)swd, (const double*)swd, po, tile, N2, C, FpFin{sel.fa, P.ninv_d}, opaque(m));
     } else {
       for (int i = threadIdx.x; i < S; i += TH) { sw[i] = sel.tw[i]; swp[i] = sel.twsh[i]; }
       __syncthreads();
-      inv_rows_body<LS>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, sel.ts, sel.tssh, p, po, tile, C, opaque(m));
-      if (!team_barrier(ta, team, ta.W * ++nb, formal, &sflag)) return;
-      inv_cols_body<LS, L2Ld>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, po, tile, N2, C, IntFin{sel.ia, P.ninv, P.ninv_sh}, opaque(m));
+      inv_cols_body<LS, L2Ld>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, po, tile, N2, C, IntFin{sel.ia, P.ninv, P.ninv_sh},
+                              opaque(m));
     }
+  };
+  uint32_t y = team < ta.njobs && mine(team) ? team : next(team);
+  uint32_t k = 0;
+  if (y < ta.njobs) pass1(y, k);
+  while (y < ta.njobs) {
+    const uint32_t yn = next(y);
+    if (yn < ta.njobs) pass1(yn, k + 1);
+    if (!team_wait(ta, slot + (k & 1), ta.W * ++uses[k & 1], formal, &sflag)) return;
+    pass2(y);
+    y = yn;
+    ++k;
   }
 }
 
@@ -900,7 +979,7 @@ inline bool team_check(const Context& c, uint32_t njobs, uint32_t W, int max_blo
 
 // arrival counters of the launch back to zero (a fill kernel: capturable, unlike a memset node)
 inline void team_reset(const Context& c, const TeamArgs& ta, hipStream_t st) {
-  k_zero_u32<<<1, 256, 0, st>>>(c.team_ctr(), ta.nteams);
+  k_zero_u32<<<1, 256, 0, st>>>(c.team_ctr(), 3 * ta.nteams);
   FH_HIP(hipGetLastError());
 }
 

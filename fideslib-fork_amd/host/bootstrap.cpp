@@ -531,7 +531,26 @@ uint32_t CryptoContextImpl::GetBootstrapDepth() const {
 
 Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0, uint32_t numIterations, uint32_t precision) {
   if (!boot_) throw std::runtime_error("fideslib: EvalBootstrapSetup first");
-  if (numIterations != 1) throw std::runtime_error("fideslib: EvalBootstrap: one iteration only");
+  if (numIterations == 2) {
+    // Meta-BTS (OpenFHE's EvalBootstrap with two iterations, as the reference composes it around
+    // its device bootstrap, api/engine/cuda/CudaEngine.cpp:930-1040): bootstrap, scale the result
+    // and the input up by 2^precision, bootstrap their difference (the first bootstrap's error,
+    // now large enough to survive a second refresh), subtract it and scale back down.
+    if (precision > 40) throw std::runtime_error("fideslib: EvalBootstrap: precision must be at most 40 bits");
+    const uint64_t pw = 1ull << precision;
+    Ciphertext b1 = EvalBootstrap(ct0, 1, 0);
+    if (b1->noiseScaleDeg > 1) b1 = modReduce(b1);
+    Ciphertext c1 = MultByInteger(b1, pw);
+    if (c1->towers <= ct0->towers) return ct0;   // the bootstrap gains no towers: the input as it is
+    Ciphertext up = ct0;
+    if (!flex() && up->noiseScaleDeg > 1) up = modReduce(up);
+    up = MultByInteger(up, pw);
+    Ciphertext err = EvalSub(c1, up);
+    Ciphertext b2 = EvalBootstrap(err, 1, 0);
+    if (b2->noiseScaleDeg > 1) b2 = modReduce(b2);
+    return EvalMult(EvalSub(c1, b2), 1.0 / (double)pw);
+  }
+  if (numIterations != 1) throw std::runtime_error("fideslib: EvalBootstrap: one or two iterations");
   (void)precision;
   if (ct0->ncomp != 2) throw std::runtime_error("fideslib: EvalBootstrap: relinearize first");
   BootstrapState& st = *boot_;

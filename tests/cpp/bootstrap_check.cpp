@@ -7,8 +7,9 @@
 // The inputs, outputs and bootstrap constants are written to <out_dir> so that the test can replay
 // the same sequence with the oracle and compare residues.
 //
-// usage: bootstrap_check <out_dir> <logN> <slots> [seed] [cts_budget stc_budget depth scale_bits hw technique]
-// technique: 0 FIXEDMANUAL (default), 1 FLEXIBLEAUTO. (prints one JSON line)
+// usage: bootstrap_check <out_dir> <logN> <slots> [seed] [cts_budget stc_budget depth scale_bits hw technique meta_precision]
+// technique: 0 FIXEDMANUAL (default), 1 FLEXIBLEAUTO; meta_precision > 0 also runs the two-iteration
+// bootstrap (Meta-BTS) at that precision and reports its error. (prints one JSON line)
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -82,6 +83,7 @@ int main(int argc, char** argv) {
   const uint32_t sbits = argc > 8 ? (uint32_t)std::atoi(argv[8]) : 50;
   const uint32_t hw = argc > 9 ? (uint32_t)std::atoi(argv[9]) : 16;
   const bool flex = argc > 10 && std::atoi(argv[10]) == 1;
+  const uint32_t meta_prec = argc > 11 ? (uint32_t)std::atoi(argv[11]) : 0;
   try {
     CCParams p;
     p.SetRingDim(1u << logN);
@@ -114,6 +116,16 @@ int main(int argc, char** argv) {
     auto bt = cc->EvalBootstrap(ct);
     auto dec = cc->Decrypt(kp.secretKey, bt);
     const double e1 = max_err(dec->values, z, slots);
+    // the two-iteration bootstrap (Meta-BTS) of the same ciphertext
+    double e_meta = -1, meta_ms = -1;
+    if (meta_prec) {
+      cc->Synchronize();
+      const auto t0 = std::chrono::steady_clock::now();
+      auto bm = cc->EvalBootstrap(ct, 2, meta_prec);
+      cc->Synchronize();
+      meta_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      e_meta = max_err(cc->Decrypt(kp.secretKey, bm)->values, z, slots);
+    }
     // determinism: a second bootstrap of the same ciphertext gives the same residues
     auto bt2 = cc->EvalBootstrap(ct);
     const size_t words = (size_t)2 * bt->towers * cc->GetRingDimension();
@@ -192,9 +204,10 @@ int main(int argc, char** argv) {
 
     std::printf("{\"technique\": \"%s\", \"logN\": %d, \"slots\": %u, \"budget\": [%u, %u], \"L1\": %u, \"depth\": %u, \"towers_in\": %u, \"towers_out\": %u, "
                 "\"boot_err\": %.6g, \"boot_bits\": %.2f, \"deterministic\": %s, \"square_err\": %.6g, "
-                "\"cheb_towers_out\": %u, \"cheb_err_vs_series\": %.6g, \"cheb_err_vs_function\": %.6g, \"boot_ms\": %.3f}\n",
+                "\"cheb_towers_out\": %u, \"cheb_err_vs_series\": %.6g, \"cheb_err_vs_function\": %.6g, \"boot_ms\": %.3f, "
+                "\"meta_precision\": %u, \"meta_err\": %.6g, \"meta_bits\": %.2f, \"meta_ms\": %.3f}\n",
                 flex ? "FLEXIBLEAUTO" : "FIXEDMANUAL", logN, slots, b1, b2, L1, cc->GetBootstrapDepth(), ct->towers, bt->towers, e1, -std::log2(e1), same ? "true" : "false", e2,
-                cy->towers, e3, e4, ms[1]);
+                cy->towers, e3, e4, ms[1], meta_prec, e_meta, e_meta > 0 ? -std::log2(e_meta) : 0.0, meta_ms);
   } catch (const std::exception& ex) {
     std::fprintf(stderr, "error: %s\n", ex.what());
     return 1;
