@@ -54,6 +54,8 @@ def parse():
                     help="run the mixed-level op stream only and print its result (for profiler runs)")
     ap.add_argument("--ntt-only", action="store_true",
                     help="time forward + inverse NTTs over the batch's 64 x 2 x 25 limbs only and print the result")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run the serving sessions only (synthetic keys, then bootstrap requests on real keys) and print them")
     ap.add_argument("--lt-only", action="store_true",
                     help="run the BSGS linear-transform workload only and print its result (for profiler runs)")
     ap.add_argument("--traffic-file", type=str, default=os.path.join(ROOT, "profiles", "traffic_rotation_n65536_l25.txt"),
@@ -88,6 +90,7 @@ def main():
 
     ctx = fh.Context(LOGN, L1, K, DNUM)
     if a.lt_only:
+        calibration_copy(fh, ctx, a.copy_mb)
         print(json.dumps({"linear_transform": lt_workload(fh, ctx, a), "accumulate": accumulate_run(fh, ctx, a)}))
         ctx.close()
         return
@@ -100,6 +103,20 @@ def main():
             gals.append(ctx.galois_index(r))
         kb.free()
         print(json.dumps({"uneven_stream": uneven_stream(fh, ctx, keys, gals, a)}))
+        ctx.close()
+        return
+    if a.serve_only:
+        keys, gals = [], []
+        kb = ctx.array((DNUM * 2 * (L1 + K), ctx.N))
+        for i, r in enumerate(STEPS16[:8]):
+            ctx.fill_uniform(kb, list(range(L1 + K)), num_polys=DNUM * 2, seed=a.seed + 100 + i)
+            keys.append(ctx.key_from_device(kb))
+            gals.append(ctx.galois_index(r))
+        kb.free()
+        serving = serving_sweep(ctx, keys, gals, keys[0], a)
+        if not a.no_bootstrap:
+            serving["with_bootstrap"] = serving_bootstrap_run(a)
+        print(json.dumps({"serving": serving}))
         ctx.close()
         return
     if a.ntt_only:
@@ -294,20 +311,30 @@ def main():
     launches, tot = prof[dom]
     avg_ms = tot / max(launches, 1)
     groups_avg = groups_total / a.steps
+    pass_gbs = None
     if dom.startswith("ntt_"):
-        # per launch: every limb of the launch read and written once
-        # (the ModDown rows launch also reads x_i and writes the result: 24N bytes per limb)
-        per_step_bytes = {"ntt_inv_rows": 16 * N * (B * lq + 2 * B * K), "ntt_inv_cols": 16 * N * (B * lq + 2 * B * K),
-                          "ntt_fwd_cols": 16 * N * (B * (beta * ext - lq) + 2 * B * lq),
-                          "ntt_fwd_rows": 16 * N * B * (beta * ext - lq), "ntt_fwd_rows_moddown": 24 * N * 2 * B * lq}
-        alg = per_step_bytes[dom] * a.steps / max(launches, 1)
+        # Algorithmic bytes of a transform: 16N per limb-transform (read 8N + write 8N), of which
+        # each of the two passes completes half: 8N per limb per launch. The ModDown rows launch
+        # also reads x_i for its fused epilogue (8N per limb; its write is the transform's write).
+        limbs_per_step = {"ntt_inv_rows": B * lq + 2 * B * K, "ntt_inv_cols": B * lq + 2 * B * K,
+                          "ntt_fwd_cols": B * (beta * ext - lq) + 2 * B * lq, "ntt_fwd_rows": B * (beta * ext - lq),
+                          "ntt_fwd_rows_moddown": 2 * B * lq}
+        per_limb = 16 * N if dom == "ntt_fwd_rows_moddown" else 8 * N
+        limbs = limbs_per_step[dom] * a.steps / max(launches, 1)
+        alg = per_limb * limbs
+        # each pass itself reads and writes every limb once: its own bandwidth, reported beside
+        pass_bytes = (24 if dom == "ntt_fwd_rows_moddown" else 16) * N * limbs
+        pass_gbs = pass_bytes / (avg_ms * 1e-3) / 1e9
     else:
         alg = algorithmic_bytes(dom, N, lq, ext, beta, B, groups_avg)
     achieved = alg / (avg_ms * 1e-3) / 1e9 if alg else None
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": None,
-            "avg_launch_ms": round(avg_ms, 4), "launches": launches,
+            "algorithmic_bytes_per_launch": int(alg) if alg else None, "avg_launch_ms": round(avg_ms, 4), "launches": launches,
             "kernel_ms_per_step": {f: round(prof[f][1] / a.steps, 3) for f in fams}}
+    if pass_gbs is not None:
+        roof["pass_gbps"] = round(pass_gbs, 1)
+        roof["pass_note"] = "the launch's own reads and writes (every limb once each) per second; achieved counts 8N of a limb's 16N transform bytes per pass"
     ntt_kernel_ms = {f: round(nprof[f][1] / max(nprof[f][0], 1), 4) for f in nprof}
     # HBM bytes per launch of the dominant kernel, from the rocprofv3 counter runs of this workload
     traffic = read_traffic(a.traffic_file).get(dom)
@@ -402,6 +429,19 @@ def ntt_run(fh, ctx, a):
             "roofline_frac": round(gbs / HBM_PEAK_GBS, 4), "team": ctx.tune_get("ntt_team"),
             "team_per_xcd": ctx.tune_get("ntt_team_per_xcd"), "teams_spanning_xcds": ctx.tune_get("ntt_team_split_teams"),
             "avg_launch_ms": {f: round(v[1] / v[0], 4) for f, v in prof.items() if v[0]}}
+
+
+def calibration_copy(fh, ctx, mb):
+    """One STREAM copy kernel (k_stream_copy) of mb MiB: in a counter run, the reference whose
+    FETCH_SIZE / WRITE_SIZE readings scale the other kernels' (it reads and writes mb MiB)."""
+    if mb <= 0:
+        return
+    words = mb * (1 << 17)
+    src, dst = ctx.array((words,)), ctx.array((words,))
+    fh._check(fh.load().fh_stream_copy(ctx.h, dst.ptr, src.ptr, words * 8))
+    ctx.sync()
+    src.free()
+    dst.free()
 
 
 def read_traffic(path):
@@ -683,6 +723,9 @@ def bootstrap_run():
             "meta_bts": {"iterations": 2, "precision": res["meta_precision"], "bits": res["meta_bits"], "ms": res["meta_ms"]}}
 
 
+BOOT_SERVE_REQUESTS = 200
+
+
 def serving_bootstrap_run(a):
     """A serving session whose arrivals include bootstrap requests (tests/cpp/serve_check.cpp):
     the bootstrap context above (N=2^16, 25 towers, full-slot bootstrapping, h=32), the mix of 70%
@@ -692,16 +735,24 @@ def serving_bootstrap_run(a):
     exe = os.path.join(ROOT, "fideslib-fork_amd", "_build", "serve_check")
     if not os.path.exists(exe):
         return {"error": "serve_check is not built"}
-    args = ["16", "32768", "3", "3", "24", "59", "50", "96", "0.05", str(a.seed), "0.70", "32"]
-    r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=600)
-    if r.returncode != 0:
-        return {"error": (r.stdout + r.stderr)[-500:]}
-    res = json.loads(r.stdout.strip().splitlines()[-1])
-    keep = ("p50_ms", "p99_ms", "mean_ms", "requests_per_s", "mean_batch_ops", "bootstraps", "inference", "polynomial")
-    return {"rate_per_s": 50.0, "requests": 96, "mix": "70% inference, 25% polynomial, 5% bootstrap", "towers": res["L1"],
-            "batched": {k: res["batched"][k] for k in keep}, "per_op": {k: res["per_op"][k] for k in keep},
-            "same_checksum": res["batched"]["checksum"] == res["per_op"]["checksum"],
-            "each_result_same_alone_and_in_mix": res["each_result_same_alone_and_in_mix"]}
+    keep = ("p50_ms", "p99_ms", "p50_other_ms", "p99_other_ms", "mean_ms", "requests_per_s", "mean_batch_ops", "bootstraps", "inference",
+            "polynomial")
+    out = {"rate_per_s": 50.0, "requests": BOOT_SERVE_REQUESTS, "mix": "70% inference, 25% polynomial, 5% bootstrap",
+           "p99_other": "p99 over the requests other than bootstraps"}
+    # serial: each bootstrap runs in the tick that reaches it; concurrent: on the loop's worker
+    # thread through a second context (same keys, its own stream) while the other requests tick
+    for name, conc in (("serial_bootstrap", "0"), ("concurrent_bootstrap", "1")):
+        args = ["16", "32768", "3", "3", "24", "59", "50", str(BOOT_SERVE_REQUESTS), "0.05", str(a.seed), "0.70", "32", "0", conc]
+        r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            out[name] = {"error": (r.stdout + r.stderr)[-500:]}
+            continue
+        res = json.loads(r.stdout.strip().splitlines()[-1])
+        out["towers"] = res["L1"]
+        out[name] = {"batched": {k: res["batched"][k] for k in keep}, "per_op": {k: res["per_op"][k] for k in keep},
+                     "same_checksum": res["batched"]["checksum"] == res["per_op"]["checksum"],
+                     "bitwise_same_alone_and_in_mix": res["bitwise_same_alone_and_in_mix"]}
+    return out
 
 
 def serving_sweep(ctx, rot_keys, rot_gal, relin, a):
@@ -714,12 +765,18 @@ def serving_sweep(ctx, rot_keys, rot_gal, relin, a):
     out = {"requests": a.serve_requests, "mix": "74% inference (dot product + degree-7 polynomial), 26% depth-6 polynomial",
            "bootstrap_share": "0 in these sessions (synthetic keys); with_bootstrap: a session with bootstrap requests on real keys", "sessions": []}
     keep = ("p50_ms", "p99_ms", "mean_ms", "requests_per_s", "mean_batch_ops", "max_active", "ticks")
+    out["ticks_in_flight"] = "ticks issued ahead of the GPU: 3 by default, 1 = the host waits for every tick"
+    # a discarded warm-up session (batched, at a rate that fills batches) takes the first-use
+    # allocations of the context pool and the batch's table staging out of the measured sessions
+    ctx.serve(rot_keys, rot_gal, relin, 2000.0, 24, mode=0, seed=a.seed + 99)
     for i, rate in enumerate(rates):
         r = ctx.serve(rot_keys, rot_gal, relin, rate, a.serve_requests, mode=0, seed=a.seed + i)
-        out["sessions"].append(dict({"rate_per_s": rate, "mode": "batched"}, **{k: round(r[k], 3) for k in keep}))
+        out["sessions"].append(dict({"rate_per_s": rate, "mode": "batched", "ticks_in_flight": 3}, **{k: round(r[k], 3) for k in keep}))
     mid = rates[len(rates) // 2]
-    r = ctx.serve(rot_keys, rot_gal, relin, mid, a.serve_requests, mode=1, seed=a.seed + len(rates) // 2)
-    out["sessions"].append(dict({"rate_per_s": mid, "mode": "per_op"}, **{k: round(r[k], 3) for k in keep}))
+    for mode, tif in ((0, 1), (1, 3), (1, 1)):
+        r = ctx.serve(rot_keys, rot_gal, relin, mid, a.serve_requests, mode=mode, seed=a.seed + len(rates) // 2, ticks_in_flight=tif)
+        out["sessions"].append(dict({"rate_per_s": mid, "mode": ["batched", "per_op"][mode], "ticks_in_flight": tif},
+                                    **{k: round(r[k], 3) for k in keep}))
     return out
 
 

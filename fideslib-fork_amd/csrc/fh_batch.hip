@@ -484,6 +484,7 @@ struct BatchPlan {
 
   // scratch and tables (kept across reset() and regrown on demand)
   DBuf arena;
+  DBuf tables;  // device job tables (from the context pool: kept across sessions)
   void* d_tables = nullptr;
   size_t tables_cap = 0;
   size_t scratch_words = 0;
@@ -522,6 +523,11 @@ struct BatchPlan {
   hipGraphExec_t exec = nullptr;
   hipStream_t graph_stream = nullptr;
 
+  hipEvent_t uploaded = nullptr;  // after the last table upload (the context's staging ring)
+  // a run on a stream other than the context stream: the next upload waits for it on the GPU
+  hipEvent_t run_ev = nullptr;
+  bool run_other = false;
+
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -531,7 +537,10 @@ struct BatchPlan {
   }
   ~BatchPlan() {
     drop_graph();
-    if (d_tables) (void)hipFree(d_tables);
+    if (run_ev) {
+      if (run_other) (void)hipEventSynchronize(run_ev);
+      (void)hipEventDestroy(run_ev);
+    }
   }
 };
 
@@ -847,17 +856,22 @@ void Batch::finalize() {
                o_md = tb.put(mdj), o_md_ntt = tb.put(md_ntt), o_md_epi = tb.put(md_epi), o_rs_in = tb.put(rs_in), o_rs_out = tb.put(rs_out),
                o_rs = tb.put(rj), o_rs_ntt = tb.put(rs_ntt), o_rs_epi = tb.put(rs_epi);
   if (P.tables_cap < tb.bytes.size() || !P.d_tables) {
-    if (P.d_tables) {
-      FH_HIP(hipStreamSynchronize(c.stream));  // earlier runs may still read the old tables
-      FH_HIP(hipFree(P.d_tables));
-      P.d_tables = nullptr;
-    }
-    P.tables_cap = std::max<size_t>(tb.bytes.size() + tb.bytes.size() / 4, 4096);
-    FH_HIP(hipMalloc(&P.d_tables, P.tables_cap));
+    // a run on another stream may still read the old tables; runs on the context stream are
+    // ordered before any later use of the block returned to the pool
+    if (P.d_tables && P.run_other) FH_HIP(hipEventSynchronize(P.run_ev));
+    const size_t words = (std::max<size_t>(tb.bytes.size() + tb.bytes.size() / 4, 4096) + 7) / 8;
+    P.tables = DBuf(c.pool, words);
+    P.d_tables = P.tables.p;
+    P.tables_cap = words * 8;
   }
-  // synchronous copy: ordered after every earlier run of this batch on the context stream
-  FH_HIP(hipStreamSynchronize(c.stream));
-  FH_HIP(hipMemcpy(P.d_tables, tb.bytes.data(), tb.bytes.size(), hipMemcpyHostToDevice));
+  // Asynchronous copy on the context stream through the context's pinned staging ring: ordered
+  // on the GPU after every earlier run of this batch on that stream (and, through run_ev, after
+  // the last run on another stream), so the host never waits for the batch in flight.
+  if (P.run_other) {
+    FH_HIP(hipStreamWaitEvent(c.stream, P.run_ev, 0));
+    P.run_other = false;
+  }
+  P.uploaded = c.uploads.upload(P.d_tables, tb.bytes.data(), tb.bytes.size(), c.stream);
   const uint8_t* D = static_cast<const uint8_t*>(P.d_tables);
   P.t_mulpt = reinterpret_cast<const MulPtJob*>(D + o_mulpt);
   P.n_mulpt = (uint32_t)pj.size();
@@ -959,6 +973,19 @@ void launch_all(Context& c, BatchPlan& P, hipStream_t st) {
 void Batch::run(hipStream_t st, bool graph) {
   BatchPlan& P = *p_;
   if (!P.final) finalize();
+  const bool other = st != c_.stream;
+  // the tables were uploaded on the context stream
+  if (other) FH_HIP(hipStreamWaitEvent(st, P.uploaded, 0));
+  struct Mark {  // after the launches on another stream: the next upload waits for them
+    BatchPlan& P;
+    hipStream_t st;
+    bool on;
+    ~Mark() {
+      if (!on) return;
+      if (!P.run_ev && hipEventCreateWithFlags(&P.run_ev, hipEventDisableTiming) != hipSuccess) return;
+      if (hipEventRecord(P.run_ev, st) == hipSuccess) P.run_other = true;
+    }
+  } mark{P, st, other};
   if (!graph) {
     launch_all(c_, P, st);
     return;

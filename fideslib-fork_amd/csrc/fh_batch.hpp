@@ -47,7 +47,8 @@ class Batch {
   // drop every item (keeping the scratch arena and table buffer for the next finalize)
   void reset();
 
-  // Lay out scratch and upload the job tables. No items may be added afterwards.
+  // Lay out scratch and upload the job tables (a copy ordered on the context stream after earlier
+  // runs; the host does not wait for them). No items may be added afterwards.
   void finalize();
   // Issue the batch on st. graph: capture the launch sequence into a HIP graph on the first
   // call and replay it afterwards.

@@ -210,6 +210,38 @@ void Pool::release_all() {
   cached_ = 0;
 }
 
+// ---- UploadRing ----
+UploadRing::~UploadRing() {
+  for (Slot& g : slot_) {
+    if (g.ev) {
+      if (g.used) (void)hipEventSynchronize(g.ev);
+      (void)hipEventDestroy(g.ev);
+    }
+    if (g.h) (void)hipHostFree(g.h);
+  }
+}
+
+hipEvent_t UploadRing::upload(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  std::lock_guard<std::mutex> lk(mu_);
+  Slot& g = slot_[next_];
+  next_ = (next_ + 1) % kSlots;
+  if (!g.ev) FH_HIP(hipEventCreateWithFlags(&g.ev, hipEventDisableTiming));
+  if (g.used) FH_HIP(hipEventSynchronize(g.ev));  // the copy that last read this slot
+  if (g.cap < bytes) {
+    if (g.h) FH_HIP(hipHostFree(g.h));
+    g.h = nullptr;
+    g.cap = 0;
+    const size_t cap = std::max<size_t>(bytes + bytes / 4, 16384);
+    FH_HIP(hipHostMalloc(&g.h, cap, hipHostMallocDefault));
+    g.cap = cap;
+  }
+  std::memcpy(g.h, src, bytes);
+  FH_HIP(hipMemcpyAsync(dst, g.h, bytes, hipMemcpyHostToDevice, st));
+  FH_HIP(hipEventRecord(g.ev, st));
+  g.used = true;
+  return g.ev;
+}
+
 // ---- Profiler ----
 hipEvent_t Profiler::take() {
   if (!spare.empty()) {

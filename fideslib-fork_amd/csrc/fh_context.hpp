@@ -72,6 +72,29 @@ class Pool {
   size_t in_use_ = 0, cached_ = 0;
 };
 
+// Pinned host staging for small host-to-device uploads (batch job tables): a ring of slots, each
+// rewritten only once the copy that read it has completed, so an upload never waits for work
+// queued before it. Slots are allocated on first use and kept for the context's lifetime.
+class UploadRing {
+ public:
+  ~UploadRing();
+  // Copy `bytes` from host `src` to device `dst` on `st` through the next slot; returns the event
+  // recorded after the copy (valid until that slot's next upload, which completes no earlier).
+  hipEvent_t upload(void* dst, const void* src, size_t bytes, hipStream_t st);
+
+ private:
+  static constexpr int kSlots = 8;
+  struct Slot {
+    void* h = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    bool used = false;
+  };
+  Slot slot_[kSlots];
+  int next_ = 0;
+  std::mutex mu_;
+};
+
 // RAII device buffer from the pool.
 struct DBuf {
   Pool* pool = nullptr;
@@ -224,6 +247,7 @@ struct Context {
   int cu_count = 0;
 
   hipStream_t stream = nullptr;
+  UploadRing uploads;  // staging of job-table uploads on `stream`
   // second stream (default priority) and the fork / join events of two-stream work
   hipStream_t aux_stream = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;

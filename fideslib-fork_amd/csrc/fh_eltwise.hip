@@ -167,16 +167,27 @@ __global__ __launch_bounds__(kEwThreads) void k_fill_uniform(LimbSet s, const Pr
   }
 }
 
-// STREAM copy (b[i] = a[i]) in 16-byte words, each lane moving 4 of them per iteration: the
-// achievable HBM bandwidth the kernels are compared against (a DMA-engine hipMemcpy reaches less).
+// STREAM copy (b[i] = a[i]) in 16-byte words: each workgroup moves contiguous chunks of 4 x 256
+// words (16 KiB), its lanes holding 4 loads in flight, chunks dealt out grid-stride. Contiguous
+// per-workgroup chunks keep DRAM pages open; on MI355X this reaches about 5.7 TB/s at 1 GiB against
+// 4.6 TB/s for a lane-interleaved grid-stride loop and 5.2 TB/s for a DMA-engine hipMemcpy. It is
+// the achievable HBM bandwidth the kernels are compared against.
+constexpr int kCopyU = 4;
 __global__ __launch_bounds__(256) void k_stream_copy(const uint4* __restrict__ a, uint4* __restrict__ b, size_t n16) {
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (; i + 3 * stride < n16; i += 4 * stride) {
-    const uint4 x0 = a[i], x1 = a[i + stride], x2 = a[i + 2 * stride], x3 = a[i + 3 * stride];
-    b[i] = x0; b[i + stride] = x1; b[i + 2 * stride] = x2; b[i + 3 * stride] = x3;
+  constexpr size_t per = (size_t)kCopyU * 256;
+  for (size_t base = (size_t)blockIdx.x * per; base < n16; base += (size_t)gridDim.x * per) {
+    uint4 x[kCopyU];
+#pragma unroll
+    for (int u = 0; u < kCopyU; ++u) {
+      const size_t i = base + (size_t)u * 256 + threadIdx.x;
+      if (i < n16) x[u] = a[i];
+    }
+#pragma unroll
+    for (int u = 0; u < kCopyU; ++u) {
+      const size_t i = base + (size_t)u * 256 + threadIdx.x;
+      if (i < n16) b[i] = x[u];
+    }
   }
-  for (; i < n16; i += stride) b[i] = a[i];
 }
 
 inline dim3 ew_grid(uint32_t N, uint32_t nlimbs) {
@@ -276,8 +287,9 @@ void stream_copy(const Context& c, void* dst, const void* src, size_t bytes, hip
   if (!bytes) return;
   if ((bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15)) throw Error(-1, "stream copy: 16-byte aligned sizes and pointers only");
   const size_t n16 = bytes / 16;
-  // 8 workgroups of 256 lanes per CU (256 CUs), each lane with 4 loads in flight
-  const size_t g = std::min<size_t>(2048, (n16 + 255) / 256);
+  // one 16 KiB chunk per workgroup up to 32768 workgroups (128 per CU), then grid-stride
+  const size_t per = (size_t)kCopyU * 256;
+  const size_t g = std::min<size_t>(32768, (n16 + per - 1) / per);
   ProfScope ps_(c, "stream_copy", st);
   k_stream_copy<<<(unsigned)g, 256, 0, st>>>(static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16);
   FH_HIP(hipGetLastError());

@@ -40,37 +40,42 @@ struct MacParams {
 
 // R[j][comp][pos] = sum_{i: bStep j + i < rowSize} fast[i][comp][pos] (.) diag[bStep j + i][pos]
 // One lane holds the bStep baby-step words of both components for its (position, coefficient)
-// and streams the diagonals once.
-template <int BB>
+// and streams the diagonals once; each sum (at most 16 products) is accumulated exactly under the
+// multiply policy M and reduced once.
+template <int BB, class M>
 __global__ __launch_bounds__(kT) void k_lt_mac(const uint64_t* __restrict__ fast, const uint64_t* __restrict__ diags,
                                                 uint64_t* __restrict__ R, MacParams mp) {
+  using Op = typename M::Op;
   const uint32_t pos = blockIdx.y;
   const PrimeConst P = mp.prime[pos < mp.lq ? pos : mp.L1 + (pos - mp.lq)];
   const size_t N = mp.N, E = (size_t)mp.ext * N;
   for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < mp.N; n += gridDim.x * blockDim.x) {
-    uint64_t f0[BB], f1[BB];
+    Op f0[BB], f1[BB];
 #pragma unroll
     for (int i = 0; i < BB; ++i) {
       if (i < (int)mp.bStep) {
-        f0[i] = fast[((size_t)i * 2 + 0) * E + (size_t)pos * N + n];
-        f1[i] = fast[((size_t)i * 2 + 1) * E + (size_t)pos * N + n];
+        f0[i] = M::prep(fast[((size_t)i * 2 + 0) * E + (size_t)pos * N + n]);
+        f1[i] = M::prep(fast[((size_t)i * 2 + 1) * E + (size_t)pos * N + n]);
       }
     }
     for (uint32_t j = 0; j < mp.gStep; ++j) {
-      U128 a0, a1;
+      M a0, a1;
       a0.zero();
       a1.zero();
 #pragma unroll
       for (int i = 0; i < BB; ++i) {
         const uint32_t k = j * mp.bStep + i;
         if (i < (int)mp.bStep && k < mp.rowSize) {
-          const uint64_t d = diags[(size_t)k * E + (size_t)pos * N + n];
+          const Op d = M::prep(diags[(size_t)k * E + (size_t)pos * N + n]);
           a0.mac(f0[i], d);
           a1.mac(f1[i], d);
         }
       }
-      R[((size_t)j * 2 + 0) * E + (size_t)pos * N + n] = mod128(a0.hi, a0.lo, P);
-      R[((size_t)j * 2 + 1) * E + (size_t)pos * N + n] = mod128(a1.hi, a1.lo, P);
+      uint64_t hi, lo;
+      a0.get(hi, lo);
+      R[((size_t)j * 2 + 0) * E + (size_t)pos * N + n] = mod128(hi, lo, P);
+      a1.get(hi, lo);
+      R[((size_t)j * 2 + 1) * E + (size_t)pos * N + n] = mod128(hi, lo, P);
     }
   }
 }
@@ -104,45 +109,53 @@ struct MacBatchParams {
 
 // Giant-step partial sums for B ciphertexts sharing one diagonal set. blockIdx.y = extended
 // position, blockIdx.x = a 64-coefficient tile: the tile of every diagonal is staged in LDS once
-// and applied to all B ciphertexts (four ciphertexts in flight per workgroup), so the diagonals
-// are read once per batch. R[b][j][comp][pos] = sum_i fast_b[i][comp][pos] (.) diag[bStep j + i][pos].
-template <int BB>
+// (as multiply operands, split into 30-bit halves under Mac30) and applied to all B ciphertexts
+// (four ciphertexts in flight per workgroup), so the diagonals are read once per batch.
+// R[b][j][comp][pos] = sum_i fast_b[i][comp][pos] (.) diag[bStep j + i][pos], each sum of at most
+// 16 products accumulated exactly and reduced once.
+template <int BB, class M>
 __global__ __launch_bounds__(kT) void k_lt_mac_b(const uint64_t* __restrict__ rot, const uint64_t* __restrict__ ident,
                                                   const uint64_t* __restrict__ diags, uint64_t* __restrict__ R, MacBatchParams mp) {
-  extern __shared__ uint64_t sd[];   // rowSize x kMacTile
+  using Op = typename M::Op;
+  static_assert(sizeof(Op) == 8, "one LDS word per diagonal coefficient");
+  extern __shared__ uint64_t sd_raw[];   // rowSize x kMacTile
+  Op* sd = reinterpret_cast<Op*>(sd_raw);
   const uint32_t pos = blockIdx.y;
   const PrimeConst P = mp.prime[pos < mp.lq ? pos : mp.L1 + (pos - mp.lq)];
   const size_t N = mp.N, E = (size_t)mp.ext * N;
   const uint32_t lane = threadIdx.x % kMacTile, cl = threadIdx.x / kMacTile;
   const uint32_t n = blockIdx.x * kMacTile + lane;
-  for (uint32_t k = cl; k < mp.rowSize; k += kMacLanes) sd[k * kMacTile + lane] = diags[k * E + (size_t)pos * N + n];
+  for (uint32_t k = cl; k < mp.rowSize; k += kMacLanes) sd[k * kMacTile + lane] = M::prep(diags[k * E + (size_t)pos * N + n]);
   __syncthreads();
   for (uint32_t b = cl; b < mp.B; b += kMacLanes) {
-    uint64_t f0[BB], f1[BB];
+    Op f0[BB], f1[BB];
 #pragma unroll
     for (int i = 0; i < BB; ++i) {
       if (i < (int)mp.bStep) {
         const uint64_t* src = mp.slot[i] >= 0 ? rot + ((size_t)b * mp.nrot + mp.slot[i]) * 2 * E : ident + (size_t)b * 2 * E;
-        f0[i] = src[(size_t)pos * N + n];
-        f1[i] = src[E + (size_t)pos * N + n];
+        f0[i] = M::prep(src[(size_t)pos * N + n]);
+        f1[i] = M::prep(src[E + (size_t)pos * N + n]);
       }
     }
     uint64_t* Rb = R + (size_t)b * mp.gStep * 2 * E;
     for (uint32_t j = 0; j < mp.gStep; ++j) {
-      U128 a0, a1;
+      M a0, a1;
       a0.zero();
       a1.zero();
 #pragma unroll
       for (int i = 0; i < BB; ++i) {
         const uint32_t k = j * mp.bStep + i;
         if (i < (int)mp.bStep && k < mp.rowSize) {
-          const uint64_t d = sd[k * kMacTile + lane];
+          const Op d = sd[k * kMacTile + lane];
           a0.mac(f0[i], d);
           a1.mac(f1[i], d);
         }
       }
-      Rb[((size_t)j * 2 + 0) * E + (size_t)pos * N + n] = mod128(a0.hi, a0.lo, P);
-      Rb[((size_t)j * 2 + 1) * E + (size_t)pos * N + n] = mod128(a1.hi, a1.lo, P);
+      uint64_t hi, lo;
+      a0.get(hi, lo);
+      Rb[((size_t)j * 2 + 0) * E + (size_t)pos * N + n] = mod128(hi, lo, P);
+      a1.get(hi, lo);
+      Rb[((size_t)j * 2 + 1) * E + (size_t)pos * N + n] = mod128(hi, lo, P);
     }
   }
 }
@@ -197,9 +210,16 @@ void linear_transform(Context& c, const uint64_t* ct, int l, int rowSize, int bS
   MacParams mp{c.N, lq, ext, (uint32_t)c.L1, (uint32_t)rowSize, (uint32_t)bStep, (uint32_t)gStep, c.d_prime};
   {
     ProfScope ps_(c, "lt_mac", st);
-    if (bStep <= 4) k_lt_mac<4><<<grid_n(c.N, ext), kT, 0, st>>>(fast.p, diags, R.p, mp);
-    else if (bStep <= 8) k_lt_mac<8><<<grid_n(c.N, ext), kT, 0, st>>>(fast.p, diags, R.p, mp);
-    else k_lt_mac<16><<<grid_n(c.N, ext), kT, 0, st>>>(fast.p, diags, R.p, mp);
+    const dim3 g = grid_n(c.N, ext);
+    if (c.split_mac_ok) {
+      if (bStep <= 4) k_lt_mac<4, Mac30><<<g, kT, 0, st>>>(fast.p, diags, R.p, mp);
+      else if (bStep <= 8) k_lt_mac<8, Mac30><<<g, kT, 0, st>>>(fast.p, diags, R.p, mp);
+      else k_lt_mac<16, Mac30><<<g, kT, 0, st>>>(fast.p, diags, R.p, mp);
+    } else {
+      if (bStep <= 4) k_lt_mac<4, MacWide><<<g, kT, 0, st>>>(fast.p, diags, R.p, mp);
+      else if (bStep <= 8) k_lt_mac<8, MacWide><<<g, kT, 0, st>>>(fast.p, diags, R.p, mp);
+      else k_lt_mac<16, MacWide><<<g, kT, 0, st>>>(fast.p, diags, R.p, mp);
+    }
     FH_HIP(hipGetLastError());
   }
 
@@ -298,9 +318,16 @@ void linear_transform_batch(Context& c, int count, const uint64_t* cts, uint64_t
     ProfScope ps_(c, "lt_mac", st);
     const dim3 g(c.N / kMacTile, ext);
     const size_t lds = (size_t)rowSize * kMacTile * 8;
-    if (bStep <= 4) k_lt_mac_b<4><<<g, kT, lds, st>>>(rot.p, ident.p, diags, R.p, mp);
-    else if (bStep <= 8) k_lt_mac_b<8><<<g, kT, lds, st>>>(rot.p, ident.p, diags, R.p, mp);
-    else k_lt_mac_b<16><<<g, kT, lds, st>>>(rot.p, ident.p, diags, R.p, mp);
+    // 30-bit split products when every prime is below 2^60 (sums of at most bStep <= 16 terms)
+    if (c.split_mac_ok) {
+      if (bStep <= 4) k_lt_mac_b<4, Mac30><<<g, kT, lds, st>>>(rot.p, ident.p, diags, R.p, mp);
+      else if (bStep <= 8) k_lt_mac_b<8, Mac30><<<g, kT, lds, st>>>(rot.p, ident.p, diags, R.p, mp);
+      else k_lt_mac_b<16, Mac30><<<g, kT, lds, st>>>(rot.p, ident.p, diags, R.p, mp);
+    } else {
+      if (bStep <= 4) k_lt_mac_b<4, MacWide><<<g, kT, lds, st>>>(rot.p, ident.p, diags, R.p, mp);
+      else if (bStep <= 8) k_lt_mac_b<8, MacWide><<<g, kT, lds, st>>>(rot.p, ident.p, diags, R.p, mp);
+      else k_lt_mac_b<16, MacWide><<<g, kT, lds, st>>>(rot.p, ident.p, diags, R.p, mp);
+    }
     FH_HIP(hipGetLastError());
   }
 

@@ -4,9 +4,12 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <deque>
+#include <mutex>
 #include <memory>
 #include <random>
+#include <string>
 #include <thread>
 
 #include "fh_keyswitch.hpp"
@@ -64,9 +67,120 @@ struct Request {
   int kind = 1;  // 0 polynomial, 1 inference, 2 bootstrap
   double arrival = 0;
   size_t pc = 0;
+  bool in_boot = false;  // a bootstrap step running on the worker thread
   int level[5] = {0, 0, 0, 0, 0};
   DBuf reg[5];
 };
+
+// Timing events, created on demand and reused (single-threaded: the loop's thread only).
+struct EventPool {
+  std::vector<hipEvent_t> all, spare;
+  hipEvent_t take() {
+    if (!spare.empty()) {
+      hipEvent_t e = spare.back();
+      spare.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    FH_HIP(hipEventCreate(&e));
+    all.push_back(e);
+    return e;
+  }
+  void give(hipEvent_t e) { spare.push_back(e); }
+  ~EventPool() {
+    for (hipEvent_t e : all) (void)hipEventDestroy(e);
+  }
+};
+
+// Bootstrap steps served concurrently: a worker thread waits until the request's input is
+// written (an event on the loop's stream), runs the caller's bootstrap function (which works on
+// its own stream and returns once the output is written) and reports the completion time, while
+// the loop keeps ticking the other requests.
+class BootWorker {
+ public:
+  struct Job {
+    Request* r;
+    hipEvent_t ready;
+  };
+  struct Done {
+    Request* r;
+    hipEvent_t ready;
+    int rc, level;
+    double t;
+  };
+  BootWorker(const ServeConfig& cfg, std::chrono::steady_clock::time_point t0) : cfg_(cfg), t0_(t0), th_([this] { loop(); }) {}
+  ~BootWorker() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    th_.join();
+  }
+  void submit(const Job& j) {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      q_.push_back(j);
+    }
+    cv_.notify_all();
+  }
+  bool pop(Done* d) {
+    std::lock_guard<std::mutex> lk(mu_);
+    if (done_.empty()) return false;
+    *d = done_.front();
+    done_.pop_front();
+    return true;
+  }
+  // until a bootstrap completes or `seconds` pass
+  void wait(double seconds) {
+    std::unique_lock<std::mutex> lk(mu_);
+    cv_done_.wait_for(lk, std::chrono::duration<double>(std::max(seconds, 0.0)), [&] { return !done_.empty(); });
+  }
+
+ private:
+  void loop() {
+    for (;;) {
+      Job j;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return stop_ || !q_.empty(); });
+        if (stop_) return;  // the loop is unwinding: queued bootstraps are dropped
+        j = q_.front();
+        q_.pop_front();
+      }
+      int lo = -1, rc = -1;
+      if (hipEventSynchronize(j.ready) == hipSuccess)
+        rc = cfg_.bootstrap(cfg_.bootstrap_user, j.r->reg[0].p, j.r->level[0], j.r->reg[3].p, &lo);
+      const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        done_.push_back({j.r, j.ready, rc, lo, t});
+      }
+      cv_done_.notify_all();
+    }
+  }
+  const ServeConfig& cfg_;
+  const std::chrono::steady_clock::time_point t0_;
+  std::mutex mu_;
+  std::condition_variable cv_, cv_done_;
+  std::deque<Job> q_;
+  std::deque<Done> done_;
+  bool stop_ = false;
+  std::thread th_;  // last: started once the members above exist
+};
+
+// A tick issued on the stream: its completion event and the requests it completed.
+struct Tick {
+  hipEvent_t ev = nullptr;
+  std::vector<std::unique_ptr<Request>> done;
+};
+
+double percentile(std::vector<double> v, double p) {
+  if (v.empty()) return 0;
+  std::sort(v.begin(), v.end());
+  const size_t i = std::min(v.size() - 1, (size_t)std::ceil(p * v.size()) - (p > 0 ? 1 : 0));
+  return 1000.0 * v[i];
+}
 
 }  // namespace
 
@@ -78,9 +192,11 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
   for (int k = 0; k < 8; ++k)
     if (!rot_keys[k]) throw Error(-1, "null rotation key");
   if (!relin_key) throw Error(-1, "null relinearization key");
+  if (cfg.ticks_in_flight < 0 || cfg.ticks_in_flight > 64) throw Error(-1, "ticks_in_flight must be in [0, 64]");
   const size_t N = c.N;
   const size_t reg_words = 2ull * (L + 1) * N;
   const hipStream_t st = c.stream;
+  const int depth = cfg.ticks_in_flight > 0 ? cfg.ticks_in_flight : kDefaultTicksInFlight;
 
   // shared plaintexts: 0 = weights, 1..8 = polynomial coefficients c0..c7 (synthetic residues)
   DBuf pts(c.pool, 9 * (size_t)(L + 1) * N);
@@ -115,18 +231,91 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
   const int max_active = cfg.mode == 2 ? 1 : std::max(1, (int)std::min<double>(1e6, cfg.hbm_budget_gb * 1e9 / (double)per_request));
 
   ServeResult res;
-  std::vector<double> lat;
+  std::vector<double> lat, lat_other;
   lat.reserve(cfg.requests);
+  EventPool evs;
+  std::deque<Tick> inflight;
   std::deque<std::unique_ptr<Request>> active;
   Batch batch(c);
-  int next = 0;
+  int next = 0, boots_running = 0;
   double ops_total = 0;
   using clock = std::chrono::steady_clock;
   FH_HIP(hipStreamSynchronize(st));
   const auto t0 = clock::now();
   auto now = [&] { return std::chrono::duration<double>(clock::now() - t0).count(); };
+  // GPU-side time origin: the stream is idle, so the event completes at once
+  const hipEvent_t ev0 = evs.take();
+  FH_HIP(hipEventRecord(ev0, st));
+  std::unique_ptr<BootWorker> worker;  // after every buffer it uses: stopped and joined first
+  if (cfg.bootstrap_concurrent && cfg.bootstrap_share > 0) worker.reset(new BootWorker(cfg, t0));
+
+  auto finish = [&](Request* r, double td) {
+    lat.push_back(td - r->arrival);
+    if (r->kind != 2) lat_other.push_back(td - r->arrival);
+    if (cfg.request_outputs && cfg.request_outputs[r->id]) {
+      const size_t words = 2ull * (r->level[3] + 1) * N;
+      if (words > cfg.request_output_capacity) throw Error(-1, "request_output_capacity below a request's result");
+      FH_HIP(hipMemcpy(cfg.request_outputs[r->id], r->reg[3].p, words * 8, hipMemcpyDeviceToHost));
+      if (cfg.request_output_words) cfg.request_output_words[r->id] = words;
+    }
+    if (cfg.checksum || cfg.request_checksums) {
+      std::vector<uint64_t> h(2ull * (r->level[3] + 1) * N);
+      FH_HIP(hipMemcpy(h.data(), r->reg[3].p, h.size() * 8, hipMemcpyDeviceToHost));
+      uint64_t sum = 0;
+      for (uint64_t w : h) sum += w;
+      res.checksum += sum;
+      if (cfg.request_checksums) cfg.request_checksums[r->id] = sum;
+    }
+    ++res.completed;
+    (r->kind == 1 ? res.inference : r->kind == 2 ? res.bootstraps : res.polynomial)++;
+  };
+  // the oldest tick in flight, once complete (wait: block until it is): its requests finish at
+  // the tick's completion time on the GPU clock
+  auto retire = [&](bool wait) -> bool {
+    if (inflight.empty()) return false;
+    Tick& k = inflight.front();
+    if (wait) {
+      FH_HIP(hipEventSynchronize(k.ev));
+    } else {
+      const hipError_t e = hipEventQuery(k.ev);
+      if (e == hipErrorNotReady) {
+        (void)hipGetLastError();  // a pending event is not an error
+        return false;
+      }
+      FH_HIP(e);
+    }
+    float ms = 0;
+    FH_HIP(hipEventElapsedTime(&ms, ev0, k.ev));
+    for (auto& r : k.done) finish(r.get(), ms / 1000.0);
+    evs.give(k.ev);
+    inflight.pop_front();
+    return true;
+  };
+  // bootstraps the worker completed: the request takes its next step (the program's last)
+  auto collect_boots = [&] {
+    BootWorker::Done d;
+    while (worker && worker->pop(&d)) {
+      evs.give(d.ready);
+      --boots_running;
+      if (d.rc != 0 || d.level < 0) throw Error(-6, "the bootstrap function failed");
+      Request* r = d.r;
+      r->in_boot = false;
+      r->level[3] = d.level;
+      auto it = std::find_if(active.begin(), active.end(), [&](const std::unique_ptr<Request>& p) { return p.get() == r; });
+      if (it == active.end()) throw Error(-6, "bootstrap completion for an unknown request");
+      if (++r->pc == progs[r->kind].size()) {
+        std::unique_ptr<Request> up = std::move(*it);
+        active.erase(it);
+        finish(up.get(), d.t);
+      }
+    }
+  };
 
   while (res.completed < cfg.requests) {
+    while (retire(false)) {
+    }
+    collect_boots();
+    if (res.completed >= cfg.requests) break;
     const double tn = now();
     while (next < cfg.requests && arrival[next] <= tn && (int)active.size() < max_active) {
       std::unique_ptr<Request> r(new Request);
@@ -143,21 +332,46 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
       ++next;
     }
     res.max_active = std::max(res.max_active, (int)active.size());
-    if (active.empty()) {
-      if (next < cfg.requests) {
-        const double wait = arrival[next] - now();
-        if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+    size_t runnable = 0;
+    for (auto& r : active) runnable += !r->in_boot;
+    if (runnable == 0) {
+      if (retire(true)) continue;  // the loop condition sees what it completed
+      // until the next arrival, unless none is left or admission is full
+      const bool admit = next < cfg.requests && (int)active.size() < max_active;
+      const double wait = admit ? arrival[next] - now() : 1.0;
+      if (boots_running > 0) {
+        worker->wait(std::min(wait, 1.0));
+        continue;
       }
+      if (next >= cfg.requests)
+        throw Error(-6, "serving loop stalled: " + std::to_string(res.completed) + " of " + std::to_string(cfg.requests) +
+                            " requests completed, " + std::to_string(active.size()) + " admitted");
+      if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
       continue;
     }
-    // one tick: the next step of every admitted request. A tick of one or two operations is
+    // at most `depth` ticks in flight: the host builds the next tick while the GPU runs these
+    if ((int)inflight.size() >= depth) retire(true);
+
+    // one tick: the next step of every runnable request. A tick of one or two operations is
     // issued directly (the batch's table upload would cost more than it saves).
-    const bool use_batch = cfg.mode == 0 && active.size() >= 3;
+    const bool use_batch = cfg.mode == 0 && runnable >= 3;
     if (use_batch) batch.reset();
-    std::vector<Request*> boots;
+    std::vector<Request*> boots;  // bootstraps run in the tick, after its other operations
+    size_t ops = 0;
     for (auto& rp : active) {
       Request* r = rp.get();
+      if (r->in_boot) continue;
       const Step& s = progs[r->kind][r->pc];
+      if (s.kind == kBoot && worker) {
+        // to the worker, once everything issued so far (the input included) is done
+        const hipEvent_t ready = evs.take();
+        FH_HIP(hipEventRecord(ready, st));
+        r->in_boot = true;
+        ++boots_running;
+        worker->submit({r, ready});
+        continue;
+      }
+      ++ops;
       const int la = r->level[s.a];
       uint64_t* a = r->reg[s.a].p;
       uint64_t* d = r->reg[s.dst].p;
@@ -225,8 +439,9 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
       }
       if (s.kind != kBoot) r->level[s.dst] = ld;
     }
-    ops_total += (double)active.size();
-    if (use_batch) {
+    if (ops == 0) continue;  // every runnable request went to the worker
+    ops_total += (double)ops;
+    if (use_batch && batch.items() > 0) {
       batch.finalize();
       batch.run(st, false);
     }
@@ -236,44 +451,28 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
         throw Error(-6, "the bootstrap function failed");
       r->level[3] = lo;
     }
-    FH_HIP(hipStreamSynchronize(st));
-    const double td = now();
     ++res.ticks;
+    Tick tick;
+    tick.ev = evs.take();
+    FH_HIP(hipEventRecord(tick.ev, st));
+    // the stepped requests advance; those at the end of their program leave with the tick
     for (auto it = active.begin(); it != active.end();) {
       Request* r = it->get();
-      if (++r->pc == progs[r->kind].size()) {
-        lat.push_back(td - r->arrival);
-        if (cfg.request_outputs && cfg.request_outputs[r->id]) {
-          const size_t words = 2ull * (r->level[3] + 1) * N;
-          if (words > cfg.request_output_capacity) throw Error(-1, "request_output_capacity below a request's result");
-          FH_HIP(hipMemcpy(cfg.request_outputs[r->id], r->reg[3].p, words * 8, hipMemcpyDeviceToHost));
-          if (cfg.request_output_words) cfg.request_output_words[r->id] = words;
-        }
-        if (cfg.checksum || cfg.request_checksums) {
-          std::vector<uint64_t> h(2ull * (r->level[3] + 1) * N);
-          FH_HIP(hipMemcpy(h.data(), r->reg[3].p, h.size() * 8, hipMemcpyDeviceToHost));
-          uint64_t sum = 0;
-          for (uint64_t w : h) sum += w;
-          res.checksum += sum;
-          if (cfg.request_checksums) cfg.request_checksums[r->id] = sum;
-        }
-        ++res.completed;
-        (r->kind == 1 ? res.inference : r->kind == 2 ? res.bootstraps : res.polynomial)++;
+      if (!r->in_boot && ++r->pc == progs[r->kind].size()) {
+        tick.done.push_back(std::move(*it));
         it = active.erase(it);
       } else {
         ++it;
       }
     }
+    inflight.push_back(std::move(tick));
   }
   res.wall_s = now();
-  std::sort(lat.begin(), lat.end());
-  auto pct = [&](double p) {
-    const size_t i = std::min(lat.size() - 1, (size_t)std::ceil(p * lat.size()) - (p > 0 ? 1 : 0));
-    return 1000.0 * lat[i];
-  };
-  res.p50_ms = pct(0.50);
-  res.p99_ms = pct(0.99);
-  res.max_ms = 1000.0 * lat.back();
+  res.p50_ms = percentile(lat, 0.50);
+  res.p99_ms = percentile(lat, 0.99);
+  res.p50_other_ms = percentile(lat_other, 0.50);
+  res.p99_other_ms = percentile(lat_other, 0.99);
+  res.max_ms = 1000.0 * *std::max_element(lat.begin(), lat.end());
   double sum = 0;
   for (double v : lat) sum += v;
   res.mean_ms = 1000.0 * sum / lat.size();

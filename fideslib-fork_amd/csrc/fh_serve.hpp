@@ -9,11 +9,17 @@
 //     plaintext coefficient additions, level alignment of the argument);
 //   polynomial ("depth-6 polynomial"): Horner's rule for a degree-6 polynomial of the input;
 //   bootstrap: one bootstrap of the input through a caller-supplied function (the bootstrap is
-//     host-orchestrated above this layer), run directly in the tick that reaches it.
+//     host-orchestrated above this layer), run either in the tick that reaches it (after the
+//     tick's other operations) or, with bootstrap_concurrent, on a worker thread of the loop while
+//     the other requests keep ticking (the function then works on a stream of its own: a second
+//     context with the same keys).
 // Admission is bounded by an HBM budget (each request's register footprint). Every scheduler
 // tick gathers the next operation of every admitted request into one batch, whatever its kind
 // and level, and executes it as one launch sequence (or, in the per-operation baseline mode, one
-// operation call at a time); requests that arrive during a tick join at the next one.
+// operation call at a time); requests that arrive during a tick join at the next one. Ticks are
+// issued without waiting for the GPU, up to ticks_in_flight ahead of it, and each is timed by an
+// event on the stream: a request's latency ends at its last tick's completion on the GPU clock
+// (or, for a concurrent bootstrap, when the bootstrap function returns).
 // The reference has no request loop; its closest analogue is the concurrent-ops lanes of
 // api/ConcurrentOps.hpp:39-62 and api/CryptoContext.hpp:174-195.
 #pragma once
@@ -23,6 +29,8 @@
 #include "fh_batch.hpp"
 
 namespace fh {
+
+constexpr int kDefaultTicksInFlight = 3;
 
 struct ServeConfig {
   double rate_per_s = 100.0;   // mean arrival rate
@@ -45,6 +53,12 @@ struct ServeConfig {
   uint64_t* const* request_outputs = nullptr;
   size_t request_output_capacity = 0;
   uint64_t* request_output_words = nullptr;
+  // bootstrap requests on a worker thread, concurrently with the other requests: `bootstrap` is
+  // then called from that thread, must use a stream other than the context's and must return only
+  // once `out` is written
+  bool bootstrap_concurrent = false;
+  // ticks the host may issue ahead of the GPU (0: kDefaultTicksInFlight; 1: wait for each tick)
+  int ticks_in_flight = 0;
 };
 
 struct ServeResult {
@@ -52,6 +66,7 @@ struct ServeResult {
   double requests_per_s = 0, wall_s = 0, mean_batch_ops = 0;
   int ticks = 0, completed = 0, inference = 0, polynomial = 0, bootstraps = 0, max_active = 0;
   uint64_t checksum = 0;       // sum of all result words mod 2^64 (when requested)
+  double p50_other_ms = 0, p99_other_ms = 0;  // over the requests other than bootstraps
 };
 
 // rot_keys / rot_galois: the 8 keys of rotations by 2^0 .. 2^7.

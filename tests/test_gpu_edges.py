@@ -101,3 +101,30 @@ def test_even_galois_rejected(gctx1, octx1, level0, fh):
     assert np.array_equal(out.download(), o.eval_rotate(src, l, m["rot"], good))
     for x in (d, out, rk):
         x.free()
+
+
+def test_stream_copy(gctx1, fh):
+    """The STREAM copy kernel (the bandwidth reference of the counter calibration) copies exactly:
+    a size that is no multiple of its 16 KiB chunk, one chunk, a single 16-byte word; and it rejects
+    sizes and pointers that are not 16-byte aligned."""
+    g = gctx1
+    lib = fh.load()
+    rng = np.random.default_rng(5)
+    for words in (2 * (3 * 2048 + 17), 2048, 2):
+        src = g.array((words,))
+        dst = g.array((words + 2,))
+        host = rng.integers(0, 2**63, size=words, dtype=np.uint64)
+        src.upload(host)
+        dst.upload(np.full(words + 2, 7, dtype=np.uint64))
+        fh._check(lib.fh_stream_copy(g.h, dst.ptr, src.ptr, words * 8))
+        got = dst.download()
+        assert np.array_equal(got[:words], host), words
+        assert np.all(got[words:] == 7), "the copy wrote past its end"
+        src.free()
+        dst.free()
+    a = g.array((4,))
+    with pytest.raises(fh.FHError, match="16-byte"):
+        fh._check(lib.fh_stream_copy(g.h, a.ptr, a.ptr + 8, 16))
+    with pytest.raises(fh.FHError, match="16-byte"):
+        fh._check(lib.fh_stream_copy(g.h, a.ptr, a.ptr, 8))
+    a.free()

@@ -48,15 +48,20 @@ def test_each_result_same_alone_and_in_the_mix(sctx):
     rk, rg, relin = _keys(sctx, 83)
     runs = {mode: sctx.serve(rk, rg, relin, 3000.0, 12, mode=mode, seed=9, request_checksums=True, request_outputs=True)
             for mode in (0, 1, 2)}
+    # the host waiting for every tick, and running far ahead of the GPU
+    runs["sync"] = sctx.serve(rk, rg, relin, 3000.0, 12, mode=0, seed=9, request_checksums=True, request_outputs=True, ticks_in_flight=1)
+    runs["ahead"] = sctx.serve(rk, rg, relin, 3000.0, 12, mode=0, seed=9, request_checksums=True, request_outputs=True, ticks_in_flight=16)
     assert runs[2]["max_active"] == 1 and runs[0]["max_active"] >= 3
     per = {mode: r["request_checksums"] for mode, r in runs.items()}
     assert all(v != 0 for v in per[2])
     outs = {mode: r["request_outputs"] for mode, r in runs.items()}
     for i in range(12):
         assert outs[2][i].size > 0
-        assert np.array_equal(outs[0][i], outs[2][i]), f"request {i}: batched result differs from the request alone"
-        assert np.array_equal(outs[1][i], outs[2][i]), f"request {i}: per-operation result differs from the request alone"
-    assert per[0] == per[2] and per[1] == per[2]
+        for mode in (0, 1, "sync", "ahead"):
+            assert np.array_equal(outs[mode][i], outs[2][i]), f"request {i}: result in mode {mode} differs from the request alone"
+    assert all(per[mode] == per[2] for mode in per)
+    for r in runs.values():
+        assert r["p99_other_ms"] == r["p99_ms"]  # no bootstrap requests in this mix
     for k in rk + [relin]:
         k.free()
 
@@ -73,6 +78,8 @@ def test_serving_rejects_bad_config(sctx, fh):
         sctx.serve(rk, rg, relin, 100.0, 4, bootstrap_share=0.2)  # no bootstrap function
     with pytest.raises(fh.FHError):
         sctx.serve(rk, rg, relin, 100.0, 4, inference_share=0.9, bootstrap_share=0.2)
+    with pytest.raises(fh.FHError):
+        sctx.serve(rk, rg, relin, 100.0, 4, ticks_in_flight=65)
     for k in rk + [relin]:
         k.free()
 
@@ -87,14 +94,15 @@ from conftest import PKG
 SCHECK = os.path.join(PKG, "_build", "serve_check")
 
 
-@pytest.mark.parametrize("share", [0.3, 1.0])
-def test_serving_with_bootstrap_requests(share):
+@pytest.mark.parametrize("share,concurrent", [(0.3, 0), (1.0, 0), (0.3, 1), (1.0, 1)])
+def test_serving_with_bootstrap_requests(share, concurrent):
     """A session whose arrivals include bootstrap requests, served by the facade's EvalBootstrap
     (N=2^12, 8 slots, level budget {1,1}, 16 towers): batched and per-operation execution give the
-    same checksum, every bootstrap request went through the callback once per mode."""
+    same checksum, every bootstrap request went through the callback once per mode. concurrent:
+    the bootstraps run on the loop's worker thread through a second context with the same keys."""
     if not os.path.exists(SCHECK):
         pytest.fail("serve_check is not built (run __graft_entry__.build())")
-    args = ["12", "8", "1", "1", "15", "50", "2000", "16", str(share), "5", str(0.5 * (1 - share))]
+    args = ["12", "8", "1", "1", "15", "50", "2000", "16", str(share), "5", str(0.5 * (1 - share)), "16", "0", str(concurrent)]
     print(f"[serve_check] {SCHECK} {' '.join(args)}", flush=True)
     r = subprocess.run([SCHECK] + args, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
@@ -111,23 +119,26 @@ def test_serving_with_bootstrap_requests(share):
     assert b["checksum"] == p["checksum"], "fused batches differ from per-operation execution"
     assert res["each_result_same_alone_and_in_mix"]
     assert res["bitwise_same_alone_and_in_mix"], "a request's result differs between the mix and the request alone"
+    assert res["concurrent"] == bool(concurrent)
 
 
-@pytest.mark.parametrize("technique", [0, 1], ids=["fixedmanual", "flexibleauto"])
-def test_serving_workload_n65536_with_bootstraps(technique):
+@pytest.mark.parametrize("technique,concurrent", [(0, 0), (1, 0), (0, 1)], ids=["fixedmanual", "flexibleauto", "fixedmanual-concurrent"])
+def test_serving_workload_n65536_with_bootstraps(technique, concurrent):
     """The serving workload's own shape: N=2^16, 25 towers, real keys (sparse ternary h=32), all
     32768 slots, bootstrap level budget {3,3} with 59-bit scaling; arrivals at 50/s, 70% inference,
     25% polynomial and 5% bootstrap requests (the bootstrap requests under the context's scaling
     technique). Every request's result is bit for bit the same in the batched mix, in
-    per-operation execution and alone."""
+    per-operation execution and alone; concurrent: with the bootstraps on the loop's worker thread
+    (a second context, same keys) while the other requests keep ticking."""
     if not os.path.exists(SCHECK):
         pytest.fail("serve_check is not built (run __graft_entry__.build())")
-    args = ["16", "32768", "3", "3", "24", "59", "50", "40", "0.05", "2026", "0.70", "32", str(technique)]
+    args = ["16", "32768", "3", "3", "24", "59", "50", "40", "0.05", "2026", "0.70", "32", str(technique), str(concurrent)]
     print(f"[serve_check] {SCHECK} {' '.join(args)}", flush=True)
     r = subprocess.run([SCHECK] + args, capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
     res = json.loads(r.stdout.strip().splitlines()[-1])
-    print({k: res[k] for k in ("technique", "callbacks", "each_result_same_alone_and_in_mix", "bitwise_same_alone_and_in_mix")})
+    print({k: res[k] for k in ("technique", "concurrent", "callbacks", "each_result_same_alone_and_in_mix", "bitwise_same_alone_and_in_mix")})
+    print({m: {k: res[m][k] for k in ("p50_ms", "p99_ms", "p99_other_ms")} for m in ("batched", "per_op", "alone")})
     b, p, a = res["batched"], res["per_op"], res["alone"]
     for x in (b, p, a):
         assert x["completed"] == 40
