@@ -771,12 +771,16 @@ def serving_sweep(ctx, rot_keys, rot_gal, relin, a):
     ctx.serve(rot_keys, rot_gal, relin, 2000.0, 24, mode=0, seed=a.seed + 99)
     for i, rate in enumerate(rates):
         r = ctx.serve(rot_keys, rot_gal, relin, rate, a.serve_requests, mode=0, seed=a.seed + i)
-        out["sessions"].append(dict({"rate_per_s": rate, "mode": "batched", "ticks_in_flight": 3}, **{k: round(r[k], 3) for k in keep}))
-    mid = rates[len(rates) // 2]
-    for mode, tif in ((0, 1), (1, 3), (1, 1)):
-        r = ctx.serve(rot_keys, rot_gal, relin, mid, a.serve_requests, mode=mode, seed=a.seed + len(rates) // 2, ticks_in_flight=tif)
-        out["sessions"].append(dict({"rate_per_s": mid, "mode": ["batched", "per_op"][mode], "ticks_in_flight": tif},
+        out["sessions"].append(dict({"rate_per_s": rate, "mode": "batched", "ticks_in_flight": 3, "batch_min_ops": 3},
                                     **{k: round(r[k], 3) for k in keep}))
+    mid = rates[len(rates) // 2]
+    for mode, tif, bmin in ((0, 1, 3), (0, 3, 2), (1, 3, 3), (1, 1, 3)):
+        r = ctx.serve(rot_keys, rot_gal, relin, mid, a.serve_requests, mode=mode, seed=a.seed + len(rates) // 2, ticks_in_flight=tif,
+                      batch_min_ops=bmin)
+        s = {"rate_per_s": mid, "mode": ["batched", "per_op"][mode], "ticks_in_flight": tif}
+        if mode == 0:
+            s["batch_min_ops"] = bmin
+        out["sessions"].append(dict(s, **{k: round(r[k], 3) for k in keep}))
     return out
 
 

@@ -808,6 +808,7 @@ int fh_serve_run(fh_context* ctx, const fh_serve_config* config, const fh_key* c
     cfg.request_output_words = config->request_output_words;
     cfg.bootstrap_concurrent = config->bootstrap_concurrent != 0;
     cfg.ticks_in_flight = config->ticks_in_flight;
+    cfg.batch_min_ops = config->batch_min_ops;
     if (cfg.mode < 0 || cfg.mode > 2) throw fh::Error(FH_E_INVALID, "mode must be 0 (batched), 1 (per operation) or 2 (one request at a time)");
     const uint64_t* rk[8];
     for (int k = 0; k < 8; ++k) rk[k] = key_of(c, rot_keys[k]);

@@ -737,12 +737,21 @@ void Batch::finalize() {
     for (int comp = 0; comp < d.ncomp; ++comp)
       for (size_t j = 0; j < lo; ++j) aj.push_back(AddJob{d.a + (comp * li + j) * N, nullptr, d.out + (comp * lo + j) * N, (uint32_t)j, 0});
   }
+  // A plaintext product followed by a Rescale forms only its top limb here (the INTT's input);
+  // the Rescale epilogue forms the other limbs' products as it reads them (unless the output
+  // overlaps the input, when the whole product is materialized first).
+  auto fused_mulpt = [&](const BatchPlan::MultPt& m) {
+    if (!m.rescale) return false;
+    const size_t lq = m.l + 1;
+    const uint64_t *a0 = m.a, *a1 = m.a + 2 * lq * N, *o0 = m.out, *o1 = m.out + 2 * (lq - 1) * N;
+    return o1 <= a0 || a1 <= o0;
+  };
   std::vector<MulPtJob> pj;
   for (size_t i = 0; i < P.mulpts.size(); ++i) {
     const auto& m = P.mulpts[i];
     const size_t lq = m.l + 1;
     uint64_t* o = m.rescale ? A + ps[i].relin : m.out;
-    for (size_t j = 0; j < lq; ++j)
+    for (size_t j = fused_mulpt(m) ? lq - 1 : 0; j < lq; ++j)
       pj.push_back(MulPtJob{m.a + j * N, m.a + (lq + j) * N, m.pt + j * N, o + j * N, o + (lq + j) * N, (uint32_t)j, 0});
   }
   std::vector<LimbJob> ks_in, ks_out, modup_ntt, md_in, md_out, md_ntt, rs_in, rs_out, rs_ntt;
@@ -811,12 +820,17 @@ void Batch::finalize() {
   struct RsSrc {
     uint32_t l;
     uint64_t *R, *top, *e, *out;
+    const uint64_t *a, *pt;   // fused plaintext product: the epilogue reads a_i and pt_i
   };
   std::vector<RsSrc> rss;
   for (size_t i = 0; i < P.mults.size(); ++i)
-    if (P.mults[i].rescale) rss.push_back(RsSrc{(uint32_t)P.mults[i].l, A + ms[i].relin, A + ms[i].top, A + ms[i].e, P.mults[i].out});
-  for (size_t i = 0; i < P.mulpts.size(); ++i)
-    if (P.mulpts[i].rescale) rss.push_back(RsSrc{(uint32_t)P.mulpts[i].l, A + ps[i].relin, A + ps[i].top, A + ps[i].e, P.mulpts[i].out});
+    if (P.mults[i].rescale)
+      rss.push_back(RsSrc{(uint32_t)P.mults[i].l, A + ms[i].relin, A + ms[i].top, A + ms[i].e, P.mults[i].out, nullptr, nullptr});
+  for (size_t i = 0; i < P.mulpts.size(); ++i) {
+    const auto& m = P.mulpts[i];
+    if (m.rescale)
+      rss.push_back(RsSrc{(uint32_t)m.l, A + ps[i].relin, A + ps[i].top, A + ps[i].e, m.out, fused_mulpt(m) ? m.a : nullptr, m.pt});
+  }
   for (const RsSrc& rs : rss) {
     const uint32_t l = rs.l, lq = l + 1;
     uint64_t* R = rs.R;
@@ -829,7 +843,10 @@ void Batch::finalize() {
       for (uint32_t ii = 0; ii < l; ++ii) {
         rj.push_back(RsJob{top + (size_t)comp * N, e + ((size_t)comp * l + ii) * N, l, ii});
         rs_ntt.push_back(LimbJob{e + ((size_t)comp * l + ii) * N, ii, 0});
-        rs_epi.push_back(EpiJob{R + ((size_t)comp * lq + ii) * N, rout + ((size_t)comp * l + ii) * N, nullptr, ii, l});
+        if (rs.a)
+          rs_epi.push_back(EpiJob{rs.a + ((size_t)comp * lq + ii) * N, rout + ((size_t)comp * l + ii) * N, rs.pt + (size_t)ii * N, ii, l});
+        else
+          rs_epi.push_back(EpiJob{R + ((size_t)comp * lq + ii) * N, rout + ((size_t)comp * l + ii) * N, nullptr, ii, l});
       }
     }
   }

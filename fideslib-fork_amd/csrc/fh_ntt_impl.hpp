@@ -166,13 +166,28 @@ __device__ __forceinline__ Sel select(const NttTables& tb, int pid, bool inverse
   return s;
 }
 
+// Thread index policies of the pass bodies. Inside a loop (the team kernels run their limbs in
+// one), the compiler hoists the column passes' 16 strided per-lane addresses out of the loop as
+// loop invariants, which doubled those kernels' registers; an index the compiler cannot see
+// through (OpaqueTid) keeps them in the body. Single-pass launches use the plain index.
+struct PlainTid {
+  __device__ __forceinline__ static int get() { return threadIdx.x; }
+};
+struct OpaqueTid {
+  __device__ __forceinline__ static int get() {
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+  }
+};
+
 // ------------------------------------------------------------------------------- forward, columns
 // First l1 stages. Input: canonical (in place or out of place), output: internal form.
-template <int LS, class A, class TwT>
+template <int LS, class Tid = PlainTid, class A, class TwT>
 __device__ __forceinline__ void fwd_cols_body(const A& ar, const TwT* sw, const TwT* swp, const uint64_t* p, uint64_t* po,
                                               uint64_t* tile, uint32_t N2, int C, uint32_t bx) {
   constexpr int S = 1 << LS, T = S / 16, PADS = S + 1;
-  const int tid = threadIdx.x;
+  const int tid = Tid::get();
   const int cc = tid % C, r0 = tid / C;
   const uint32_t col = bx * C + cc;
   typename A::T x[16];
@@ -489,11 +504,11 @@ __global__ __launch_bounds__(TH) void k_ntt_inv_rows(Addr ad, AddrOut ao, NttTab
 // ------------------------------------------------------------------------------- inverse, columns
 // Second launch of the INTT (first l1 stages, reversed) with the N^-1 scaling. Input: internal
 // form, output: canonical residues.
-template <int LS, class Ld = PlainLd, class A, class TwT, class Fin>
+template <int LS, class Ld = PlainLd, class Tid = PlainTid, class A, class TwT, class Fin>
 __device__ __forceinline__ void inv_cols_body(const A& ar, const TwT* sw, const TwT* swp, uint64_t* p, uint64_t* tile, uint32_t N2,
                                               int C, const Fin& fin, uint32_t bx) {
   constexpr int S = 1 << LS, T = S / 16, PADS = S + 1;
-  const int tid = threadIdx.x;
+  const int tid = Tid::get();
   const int cc = tid % C, r0 = tid / C;
   const uint32_t col = bx * C + cc;
   typename A::T x[16];
@@ -678,7 +693,7 @@ __device__ __forceinline__ bool team_setup(const TeamArgs& ta, uint32_t& team, u
 // One launch per arithmetic (FP: the fp64 path of the 50-bit primes, else integer Shoup): a team
 // skips the limbs of the other kind, and each kernel holds one path's registers only.
 template <int LS, int TH, bool FP, class Addr, class AddrOut, class Epi>
-__global__ __launch_bounds__(TH) void k_ntt_fwd_team(Addr ad, AddrOut ao, NttTables tb, Epi epi, TeamArgs ta) {
+__global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(3))) void k_ntt_fwd_team(Addr ad, AddrOut ao, NttTables tb, Epi epi, TeamArgs ta) {
   constexpr int S = 1 << LS, T = S / 16, CMAX = TH / T;
   constexpr int TW = CMAX * RowTile<LS>::PAD > CMAX * (S + 1) ? CMAX * RowTile<LS>::PAD : CMAX * (S + 1);
   __shared__ uint64_t tile[TW];
@@ -712,11 +727,11 @@ __global__ __launch_bounds__(TH) void k_ntt_fwd_team(Addr ad, AddrOut ao, NttTab
       double* swd = reinterpret_cast<double*>(sw);
       for (int i = threadIdx.x; i < S; i += TH) swd[i] = sel.twd[i];
       __syncthreads();
-      fwd_cols_body<LS>(sel.fa, (const double*)swd, (const double*)swd, p, po, tile, N2, C, opaque(m));
+      fwd_cols_body<LS, OpaqueTid>(sel.fa, (const double*)swd, (const double*)swd, p, po, tile, N2, C, opaque(m));
     } else {
       for (int i = threadIdx.x; i < S; i += TH) { sw[i] = sel.tw[i]; swp[i] = sel.twsh[i]; }
       __syncthreads();
-      fwd_cols_body<LS>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, p, po, tile, N2, C, opaque(m));
+      fwd_cols_body<LS, OpaqueTid>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, p, po, tile, N2, C, opaque(m));
     }
     team_arrive(slot + (k & 1), formal);
   };
@@ -754,7 +769,7 @@ __global__ __launch_bounds__(TH) void k_ntt_fwd_team(Addr ad, AddrOut ao, NttTab
 }
 
 template <int LS, int TH, bool FP, class Addr, class AddrOut>
-__global__ __launch_bounds__(TH) void k_ntt_inv_team(Addr ad, AddrOut ao, NttTables tb, TeamArgs ta) {
+__global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(3))) void k_ntt_inv_team(Addr ad, AddrOut ao, NttTables tb, TeamArgs ta) {
   constexpr int S = 1 << LS, T = S / 16, CMAX = TH / T;
   constexpr int TW = CMAX * RowTile<LS>::PAD > CMAX * (S + 1) ? CMAX * RowTile<LS>::PAD : CMAX * (S + 1);
   __shared__ uint64_t tile[TW];
@@ -808,11 +823,11 @@ __global__ __launch_bounds__(TH) void k_ntt_inv_team(Addr ad, AddrOut ao, NttTab
       double* swd = reinterpret_cast<double*>(sw);
       for (int i = threadIdx.x; i < S; i += TH) swd[i] = sel.twd[i];
       __syncthreads();
-      inv_cols_body<LS, L2Ld>(sel.fa, (const double*)swd, (const double*)swd, po, tile, N2, C, FpFin{sel.fa, P.ninv_d}, opaque(m));
+      inv_cols_body<LS, L2Ld, OpaqueTid>(sel.fa, (const double*)swd, (const double*)swd, po, tile, N2, C, FpFin{sel.fa, P.ninv_d}, opaque(m));
     } else {
       for (int i = threadIdx.x; i < S; i += TH) { sw[i] = sel.tw[i]; swp[i] = sel.twsh[i]; }
       __syncthreads();
-      inv_cols_body<LS, L2Ld>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, po, tile, N2, C, IntFin{sel.ia, P.ninv, P.ninv_sh},
+      inv_cols_body<LS, L2Ld, OpaqueTid>(sel.ia, (const uint64_t*)sw, (const uint64_t*)swp, po, tile, N2, C, IntFin{sel.ia, P.ninv, P.ninv_sh},
                               opaque(m));
     }
   };
@@ -881,25 +896,33 @@ struct EpiModDownJobs {
   }
 };
 
+// A job whose `add` is set rescales a ciphertext-plaintext product formed here: x_i = a_i pt_i,
+// read from the ciphertext and the plaintext (job.x = a_i, job.add = pt_i) instead of from a
+// materialized product (the ct x pt + Rescale fusion, src/NTTfusions.cuh NTT_MULTPT).
 struct EpiRescaleJobs {
   const EpiJob* jobs;
   const uint64_t* w;       // [L1][L1] [q_l^-1]_{q_i}, Shoup companions
   const uint64_t* wsh;
   uint32_t L1;
   const uint64_t* xi;
+  const uint64_t* pti;
   uint64_t* oi;
-  uint64_t q, wi, wish;
+  uint64_t wi, wish;
+  PrimeConst P;
   __device__ __forceinline__ void init(uint32_t y, uint32_t, const PrimeConst* prime) {
     const EpiJob j = jobs[y];
     xi = j.x;
+    pti = j.add;
     oi = j.out;
-    q = prime[j.pid].q;
+    P = prime[j.pid];
     const size_t o = (size_t)j.l * L1 + j.pid;
     wi = w[o];
     wish = wsh[o];
   }
   __device__ __forceinline__ void store(uint64_t*, uint32_t rbase, uint32_t jj, uint64_t v) const {
-    oi[rbase + jj] = add_mod(shoup(xi[rbase + jj], wi, wish, q), v, q);
+    uint64_t x = xi[rbase + jj];
+    if (pti) x = mul_mod(x, pti[rbase + jj], P);
+    oi[rbase + jj] = add_mod(shoup(x, wi, wish, P.q), v, P.q);
   }
 };
 

@@ -17,7 +17,7 @@ void ntt_fwd_jobs_moddown(const Context& c, const LimbJob* jobs, const EpiJob* e
   dispatch(c, JobAddr{jobs}, JobAddr{jobs}, n, st, true, e, "ntt_fwd_rows_moddown");
 }
 void ntt_fwd_jobs_rescale(const Context& c, const LimbJob* jobs, const EpiJob* epi, uint32_t n, hipStream_t st) {
-  EpiRescaleJobs e{epi, c.d_rs_qlinv, c.d_rs_qlinv_sh, (uint32_t)c.L1, nullptr, nullptr, 0, 0, 0};
+  EpiRescaleJobs e{epi, c.d_rs_qlinv, c.d_rs_qlinv_sh, (uint32_t)c.L1, nullptr, nullptr, nullptr, 0, 0, PrimeConst{}};
   dispatch(c, JobAddr{jobs}, JobAddr{jobs}, n, st, true, e, "ntt_fwd_rows_rescale");
 }
 

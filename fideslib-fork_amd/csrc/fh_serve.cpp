@@ -197,6 +197,8 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
   const size_t reg_words = 2ull * (L + 1) * N;
   const hipStream_t st = c.stream;
   const int depth = cfg.ticks_in_flight > 0 ? cfg.ticks_in_flight : kDefaultTicksInFlight;
+  if (cfg.batch_min_ops < 0) throw Error(-1, "batch_min_ops must be >= 0");
+  const int batch_min = cfg.batch_min_ops > 0 ? cfg.batch_min_ops : kDefaultBatchMinOps;
 
   // shared plaintexts: 0 = weights, 1..8 = polynomial coefficients c0..c7 (synthetic residues)
   DBuf pts(c.pool, 9 * (size_t)(L + 1) * N);
@@ -230,6 +232,13 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
   const size_t per_request = 5 * reg_words * 8;
   const int max_active = cfg.mode == 2 ? 1 : std::max(1, (int)std::min<double>(1e6, cfg.hbm_budget_gb * 1e9 / (double)per_request));
 
+  // device-side store of the results (one slot of 2 x L1 limbs per request) when they are asked for
+  const bool capturing = cfg.checksum || cfg.request_checksums || cfg.request_outputs;
+  const size_t slot_words = 2ull * c.L1 * N;
+  DBuf store;
+  if (capturing) store = DBuf(c.pool, (size_t)cfg.requests * slot_words);
+  std::vector<size_t> result_words(cfg.requests, 0);
+
   ServeResult res;
   std::vector<double> lat, lat_other;
   lat.reserve(cfg.requests);
@@ -249,23 +258,21 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
   std::unique_ptr<BootWorker> worker;  // after every buffer it uses: stopped and joined first
   if (cfg.bootstrap_concurrent && cfg.bootstrap_share > 0) worker.reset(new BootWorker(cfg, t0));
 
+  // A finished request's result is copied on the stream into its slot of a device-side store
+  // (ordered after the work that wrote it) and read back after the session, so capturing results
+  // never stalls the loop.
+  auto capture = [&](Request* r) {
+    if (!capturing) return;
+    const size_t words = 2ull * (r->level[3] + 1) * N;
+    if (words > slot_words) throw Error(-6, "a request's result exceeds its capture slot");
+    if (cfg.request_outputs && cfg.request_outputs[r->id] && words > cfg.request_output_capacity)
+      throw Error(-1, "request_output_capacity below a request's result");
+    FH_HIP(hipMemcpyAsync(store.p + (size_t)r->id * slot_words, r->reg[3].p, words * 8, hipMemcpyDeviceToDevice, st));
+    result_words[r->id] = words;
+  };
   auto finish = [&](Request* r, double td) {
     lat.push_back(td - r->arrival);
     if (r->kind != 2) lat_other.push_back(td - r->arrival);
-    if (cfg.request_outputs && cfg.request_outputs[r->id]) {
-      const size_t words = 2ull * (r->level[3] + 1) * N;
-      if (words > cfg.request_output_capacity) throw Error(-1, "request_output_capacity below a request's result");
-      FH_HIP(hipMemcpy(cfg.request_outputs[r->id], r->reg[3].p, words * 8, hipMemcpyDeviceToHost));
-      if (cfg.request_output_words) cfg.request_output_words[r->id] = words;
-    }
-    if (cfg.checksum || cfg.request_checksums) {
-      std::vector<uint64_t> h(2ull * (r->level[3] + 1) * N);
-      FH_HIP(hipMemcpy(h.data(), r->reg[3].p, h.size() * 8, hipMemcpyDeviceToHost));
-      uint64_t sum = 0;
-      for (uint64_t w : h) sum += w;
-      res.checksum += sum;
-      if (cfg.request_checksums) cfg.request_checksums[r->id] = sum;
-    }
     ++res.completed;
     (r->kind == 1 ? res.inference : r->kind == 2 ? res.bootstraps : res.polynomial)++;
   };
@@ -304,6 +311,7 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
       auto it = std::find_if(active.begin(), active.end(), [&](const std::unique_ptr<Request>& p) { return p.get() == r; });
       if (it == active.end()) throw Error(-6, "bootstrap completion for an unknown request");
       if (++r->pc == progs[r->kind].size()) {
+        capture(r);  // the bootstrap function returned once the output was written
         std::unique_ptr<Request> up = std::move(*it);
         active.erase(it);
         finish(up.get(), d.t);
@@ -352,9 +360,9 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
     // at most `depth` ticks in flight: the host builds the next tick while the GPU runs these
     if ((int)inflight.size() >= depth) retire(true);
 
-    // one tick: the next step of every runnable request. A tick of one or two operations is
-    // issued directly (the batch's table upload would cost more than it saves).
-    const bool use_batch = cfg.mode == 0 && runnable >= 3;
+    // one tick: the next step of every runnable request, as one batch from batch_min_ops
+    // operations on (a single operation is issued directly: its batch would only add a table)
+    const bool use_batch = cfg.mode == 0 && (int)runnable >= batch_min;
     if (use_batch) batch.reset();
     std::vector<Request*> boots;  // bootstraps run in the tick, after its other operations
     size_t ops = 0;
@@ -453,21 +461,39 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
     }
     ++res.ticks;
     Tick tick;
-    tick.ev = evs.take();
-    FH_HIP(hipEventRecord(tick.ev, st));
-    // the stepped requests advance; those at the end of their program leave with the tick
+    // the stepped requests advance; those at the end of their program have their results
+    // captured and leave with the tick
     for (auto it = active.begin(); it != active.end();) {
       Request* r = it->get();
       if (!r->in_boot && ++r->pc == progs[r->kind].size()) {
+        capture(r);
         tick.done.push_back(std::move(*it));
         it = active.erase(it);
       } else {
         ++it;
       }
     }
+    tick.ev = evs.take();
+    FH_HIP(hipEventRecord(tick.ev, st));
     inflight.push_back(std::move(tick));
   }
   res.wall_s = now();
+  if (capturing) {
+    FH_HIP(hipStreamSynchronize(st));
+    std::vector<uint64_t> h(slot_words);
+    for (int i = 0; i < cfg.requests; ++i) {
+      const size_t words = result_words[i];
+      FH_HIP(hipMemcpy(h.data(), store.p + (size_t)i * slot_words, words * 8, hipMemcpyDeviceToHost));
+      uint64_t sum = 0;
+      for (size_t k = 0; k < words; ++k) sum += h[k];
+      res.checksum += sum;
+      if (cfg.request_checksums) cfg.request_checksums[i] = sum;
+      if (cfg.request_outputs && cfg.request_outputs[i]) {
+        std::copy(h.begin(), h.begin() + words, cfg.request_outputs[i]);
+        if (cfg.request_output_words) cfg.request_output_words[i] = words;
+      }
+    }
+  }
   res.p50_ms = percentile(lat, 0.50);
   res.p99_ms = percentile(lat, 0.99);
   res.p50_other_ms = percentile(lat_other, 0.50);

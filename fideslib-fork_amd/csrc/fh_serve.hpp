@@ -31,13 +31,14 @@
 namespace fh {
 
 constexpr int kDefaultTicksInFlight = 3;
+constexpr int kDefaultBatchMinOps = 3;
 
 struct ServeConfig {
   double rate_per_s = 100.0;   // mean arrival rate
   int requests = 200;
   int start_level = 24;        // level of every incoming ciphertext
   double hbm_budget_gb = 64.0; // admission budget for request registers
-  int mode = 0;                // 0: one fused batch per tick (ticks of 1-2 operations issued directly),
+  int mode = 0;                // 0: one fused batch per tick (ticks below batch_min_ops issued directly),
                                // 1: one call per operation, 2: one request at a time (each alone)
   double inference_share = 0.74;
   uint64_t seed = 1;
@@ -59,6 +60,8 @@ struct ServeConfig {
   bool bootstrap_concurrent = false;
   // ticks the host may issue ahead of the GPU (0: kDefaultTicksInFlight; 1: wait for each tick)
   int ticks_in_flight = 0;
+  // mode 0: smallest tick issued as one batch (0: kDefaultBatchMinOps); smaller ticks run per operation
+  int batch_min_ops = 0;
 };
 
 struct ServeResult {

@@ -195,3 +195,29 @@ def test_autotune_choice_is_valid_and_exact(fh, oracle_mod, logN, polys):
         d.free()
     finally:
         g.close()
+
+
+@pytest.mark.skip(reason="CU-masked context streams are switched off: the process crashed (SIGSEGV) inside "
+                         "the CU-masked stream's creation on the GPU host, also after the mask-size argument "
+                         "was corrected; the stream_cu_share tunable was withdrawn")
+def test_ntt_on_cu_limited_stream(fh, octx1):
+    """A context whose stream is restricted to a share of the CUs (stream_cu_share, a CU-masked
+    stream) computes the same transforms; the share is validated and can be set back to all CUs."""
+    from conftest import CFG1
+    o = octx1
+    g = fh.Context(CFG1["logN"], CFG1["L1"], CFG1["K"], CFG1["dnum"])
+    try:
+        pids = list(range(o.L1 + o.K))
+        x = o.random_poly(17, pids)
+        for share in (50, 13, 100):
+            g.tune_set("stream_cu_share", share)
+            assert g.tune_get("stream_cu_share") == share
+            d = g.to_device(x)
+            g.ntt_forward(d, pids)
+            assert np.array_equal(d.download(), o.ntt(x, pids)), f"forward NTT differs at a {share}% CU share"
+            d.free()
+        for bad in (0, 101):
+            with pytest.raises(fh.FHError):
+                g.tune_set("stream_cu_share", bad)
+    finally:
+        g.close()
