@@ -149,16 +149,18 @@ def main():
         lib.fh_copy_d2d(ctx.h, dst.ptr, src.ptr, words * 8)
         ctx.sync()
         copy_dma_gbs = round(2 * words * 8 / (time.perf_counter() - t0) / 1e9, 1)
-        # STREAM copy kernel: read + write of the buffer, best of a few timed runs
+        # STREAM copy kernel: read + write of the buffer, the kernel timed with HIP events on the
+        # context stream (average of 5 launches after one untimed)
         fh._check(lib.fh_stream_copy(ctx.h, dst.ptr, src.ptr, words * 8))
         ctx.sync()
-        best = 1e30
+        ctx.profile(True)
+        ctx.profile_reset()
         for _ in range(5):
-            t0 = time.perf_counter()
             fh._check(lib.fh_stream_copy(ctx.h, dst.ptr, src.ptr, words * 8))
-            ctx.sync()
-            best = min(best, time.perf_counter() - t0)
-        copy_gbs = round(2 * words * 8 / best / 1e9, 1)
+        ctx.sync()
+        nl, tms = ctx.profile_read("stream_copy")
+        ctx.profile(False)
+        copy_gbs = round(2 * words * 8 / (tms / max(nl, 1) * 1e-3) / 1e9, 1)
         src.free()
         dst.free()
 
@@ -388,7 +390,7 @@ def main():
         "eval_mult_rescale_per_s": round(mult_rescale_s, 1) if mult_rescale_s else None,
         "eval_mult_rescale_batched_per_s": round(mult_rescale_batched_s, 1) if mult_rescale_batched_s else None,
         "device_copy_gbps": copy_gbs,
-        "device_copy_source": "STREAM copy kernel (fh_stream_copy), bytes read + written, best of 5" if copy_gbs else None,
+        "device_copy_source": "STREAM copy kernel (fh_stream_copy), bytes read + written, HIP-event kernel time, mean of 5" if copy_gbs else None,
         "device_copy_dma_gbps": copy_dma_gbs,
         "uneven_stream": stream,
         "linear_transform": lt,
@@ -717,10 +719,26 @@ def bootstrap_run():
     if r.returncode != 0:
         return {"error": (r.stdout + r.stderr)[-500:]}
     res = json.loads(r.stdout.strip().splitlines()[-1])
-    return {"workload": "N=2^16, 32768 slots, level budget {3,3}, 59-bit scaling, sparse ternary h=32", "ms": res["boot_ms"],
-            "bits": res["boot_bits"], "towers_in": res["towers_in"], "towers_out": res["towers_out"], "depth": res["depth"],
-            "deterministic": res["deterministic"],
-            "meta_bts": {"iterations": 2, "precision": res["meta_precision"], "bits": res["meta_bits"], "ms": res["meta_ms"]}}
+    out = {"workload": "N=2^16, 32768 slots, level budget {3,3}, 59-bit scaling, sparse ternary h=32", "ms": res["boot_ms"],
+           "bits": res["boot_bits"], "towers_in": res["towers_in"], "towers_out": res["towers_out"], "depth": res["depth"],
+           "deterministic": res["deterministic"],
+           "meta_bts": {"iterations": 2, "precision": res["meta_precision"], "bits": res["meta_bits"], "ms": res["meta_ms"]}}
+    # the reference's own precision point (test/BootstrapPrecisionTests.cu): depth 25, 55-bit
+    # scaling under a 56-bit q_0, budget {3,2}, dnum 6, FLEXIBLEAUTO, weight-192 secret, real slots
+    # encrypted at level 21; the reference states 18.6 bits (OpenFHE), 11.5-15 (its GPU path) and
+    # 29-32 (its GPU Meta-BTS) there
+    with tempfile.TemporaryDirectory() as d:
+        r = subprocess.run([exe, d, "16", "32768", "7", "3", "2", "25", "55", "192", "1", "12", "56", "6", "1", "21"], capture_output=True,
+                           text=True, timeout=600)
+    if r.returncode != 0:
+        out["reference_point"] = {"error": (r.stdout + r.stderr)[-500:]}
+    else:
+        res = json.loads(r.stdout.strip().splitlines()[-1])
+        out["reference_point"] = {
+            "workload": "N=2^16, 32768 real slots at level 21, budget {3,2}, depth 25, 55-bit scaling, q_0 56 bits, dnum 6, FLEXIBLEAUTO, h=192",
+            "ms": res["boot_ms"], "bits": res["boot_bits"], "meta_bts_bits": res["meta_bits"], "meta_bts_ms": res["meta_ms"],
+            "reference_states_bits": {"openfhe": 18.6, "gpu": [11.5, 15], "gpu_meta_bts": [29, 32]}}
+    return out
 
 
 BOOT_SERVE_REQUESTS = 200

@@ -51,6 +51,7 @@ struct BootstrapState {
   int r = 0;             // double-angle steps
   uint32_t degree = 0;   // Chebyshev degree of the series
   double pre = 1.0;      // exact power of two applied after the raise
+  double scale_fix = 1.0;  // prod(q_r / scale) over the raise's rescales (in the first CoeffsToSlots group)
   uint32_t correction = 0;
   std::vector<double> cheb;
   std::vector<BootstrapGroup> cts, stc;         // transforms, in the order they are applied
@@ -170,9 +171,11 @@ std::vector<BootstrapGroup> factored(uint32_t n, uint64_t M, uint32_t budget, bo
   return groups;
 }
 
+// the smallest power of two whose square covers the row, at most 16 (the linear transform's
+// baby-step bound; b + g - 2 rotations either way, so the cap costs no rotations)
 uint32_t bstep_for(uint32_t rowSize) {
   uint32_t b = 1;
-  while (b * b < rowSize) b *= 2;
+  while (b * b < rowSize && b < 16) b *= 2;
   return b;
 }
 
@@ -234,6 +237,88 @@ struct BootstrapOps {
     p->slots = a->slots;
     return p;
   }
+  // ---- exact-scale forms (the bootstrap's EvalMod): scalingFactor is the ciphertext's true scale ----
+  // a rescale that divides the tracked scale by the dropped prime itself
+  static Ciphertext rescale_exact(CryptoContextImpl& cc, const Ciphertext& ct) {
+    const double q = (double)cc.moduli_[ct->towers - 1];
+    Ciphertext r = cc.modReduce(ct);
+    r->scalingFactor = ct->scalingFactor / q;
+    return r;
+  }
+  // every component times operand encoded at scale enc: degree + 1, scale times enc
+  static Ciphertext mult_at(CryptoContextImpl& cc, const Ciphertext& a, double operand, double enc) {
+    const auto f = cc.ElemForEvalMultAt(a->towers, operand, enc);
+    auto out = cc.newCt(a->towers, a->ncomp);
+    for (uint32_t c = 0; c < a->ncomp; ++c)
+      cc.check(fh_eltwise_scalar(cc.ctx_, 1, a->data->ptr + (size_t)c * a->towers * cc.N_, f.data(),
+                                 out->data->ptr + (size_t)c * a->towers * cc.N_, nullptr, (int)a->towers));
+    out->noiseScaleDeg = a->noiseScaleDeg + 1;
+    out->scalingFactor = a->scalingFactor * enc;
+    out->slots = a->slots;
+    return out;
+  }
+  // sum_k w_k cts[k] with weight k encoded at encs[k] (every term then at scale S): one fused
+  // weighted sum at the level of the lowest input, degree + 1, scale S
+  static Ciphertext wsum_at(CryptoContextImpl& cc, const std::vector<Ciphertext>& cts, const std::vector<double>& w,
+                            const std::vector<double>& encs, double S) {
+    if (cts.empty() || cts.size() > 16 || cts.size() != w.size() || w.size() != encs.size())
+      throw std::runtime_error("fideslib: weighted sum of 1..16 terms");
+    uint32_t t = ~0u;
+    for (const auto& c : cts) t = std::min(t, c->towers);
+    const size_t n = cts.size();
+    std::vector<const uint64_t*> ptrs(n);
+    std::vector<int32_t> towers(n);
+    std::vector<uint64_t> res(n * t);
+    for (size_t k = 0; k < n; ++k) {
+      if (cts[k]->ncomp != cts[0]->ncomp || cts[k]->noiseScaleDeg != cts[0]->noiseScaleDeg)
+        throw std::runtime_error("fideslib: weighted sum inputs differ in size or scale degree");
+      ptrs[k] = cts[k]->data->ptr;
+      towers[k] = (int32_t)cts[k]->towers;
+      const auto f = cc.ElemForEvalMultAt(t, w[k], encs[k]);
+      std::copy(f.begin(), f.end(), res.begin() + k * t);
+    }
+    auto out = cc.newCt(t, cts[0]->ncomp);
+    cc.check(fh_linear_wsum(cc.ctx_, (int)n, ptrs.data(), towers.data(), res.data(), (int)t, (int)cts[0]->ncomp, nullptr, 0, out->data->ptr));
+    out->noiseScaleDeg = cts[0]->noiseScaleDeg + 1;
+    out->scalingFactor = S;
+    out->slots = cts[0]->slots;
+    return out;
+  }
+  // sum_k w_k cts[k] (integers) plus constant encoded at the inputs' common true scale
+  static Ciphertext icomb_exact(CryptoContextImpl& cc, const std::vector<Ciphertext>& cts, const std::vector<int64_t>& w, double constant) {
+    for (const auto& c : cts)
+      if (c->scalingFactor != cts[0]->scalingFactor) throw std::runtime_error("fideslib: exact-scale combination of different scales");
+    uint32_t t = ~0u;
+    for (const auto& c : cts) t = std::min(t, c->towers);
+    const size_t n = cts.size();
+    std::vector<const uint64_t*> ptrs(n);
+    std::vector<int32_t> towers(n);
+    std::vector<uint64_t> res(n * t);
+    for (size_t k = 0; k < n; ++k) {
+      ptrs[k] = cts[k]->data->ptr;
+      towers[k] = (int32_t)cts[k]->towers;
+      for (uint32_t i = 0; i < t; ++i) {
+        const int64_t r = w[k] % (int64_t)cc.moduli_[i];
+        res[k * t + i] = (uint64_t)(r < 0 ? r + (int64_t)cc.moduli_[i] : r);
+      }
+    }
+    std::vector<uint64_t> add;
+    if (constant != 0.0) {
+      add = cc.ElemForEvalAddOrSubAt(t, std::fabs(constant), 1, cts[0]->scalingFactor);
+      if (constant < 0)
+        for (uint32_t i = 0; i < t; ++i) add[i] = add[i] ? cc.moduli_[i] - add[i] : 0;
+    }
+    auto out = cc.newCt(t, cts[0]->ncomp);
+    cc.check(fh_linear_wsum(cc.ctx_, (int)n, ptrs.data(), towers.data(), res.data(), (int)t, (int)cts[0]->ncomp,
+                            add.empty() ? nullptr : add.data(), cts[0]->ncomp == 4 ? 5u : 1u, out->data->ptr));
+    out->noiseScaleDeg = cts[0]->noiseScaleDeg;
+    out->scalingFactor = cts[0]->scalingFactor;
+    out->slots = cts[0]->slots;
+    return out;
+  }
+  static double prime(const CryptoContextImpl& cc, uint32_t i) { return (double)cc.moduli_[i]; }
+  static double nominal(const CryptoContextImpl& cc) { return cc.scale_; }
+
   static Ciphertext half(CryptoContextImpl& cc, const Ciphertext& p, int which) {
     const size_t h = (size_t)2 * p->towers * cc.N_;
     auto x = cc.newCt(p->towers, 2);
@@ -320,6 +405,99 @@ struct ChebEval {
     Ciphertext P = q.size() == 1 ? cc.Rescale(cc.EvalMult(t(M), q[0])) : cc.Rescale(mul(eval(q), t(M)));
     Ciphertext R = eval(r);
     return cc.IntegerCombination({P, R}, {1, 1}, 0.0);
+  }
+};
+// The same series with every ciphertext's true scale tracked (the rescales divide by the dropped
+// primes, not the nominal scale): every constant is encoded at the true scale of what it is added
+// to, every leaf term's weight at the scale that puts all terms of the leaf at one scale S, and
+// each remainder piece at the scale of the product it is added to, so no sum combines different
+// scales. With the nominal bookkeeping instead, the primes' distance from the scale (about 2^-31
+// at N=2^16) enters every T_k and is doubled by each T_2k = 2 T_k^2 - 1, which bounded the
+// bootstrap's precision. T_2k+1 = 2 T_k T_k+1 - T_1 takes T_1 through a scalar product that lands
+// it on the product's scale.
+struct ChebExact {
+  CryptoContextImpl& cc;
+  std::map<uint32_t, Ciphertext> T;
+  std::map<uint32_t, uint32_t> lv;  // output towers of pieces, by degree (structure only)
+  Ciphertext mul(const Ciphertext& a, const Ciphertext& b) { return BootstrapOps::mult(cc, a, b); }
+  Ciphertext rs(const Ciphertext& a) { return BootstrapOps::rescale_exact(cc, a); }
+
+  Ciphertext t(uint32_t k) {
+    auto it = T.find(k);
+    if (it != T.end()) return it->second;
+    Ciphertext r;
+    if (k % 2 == 0) {
+      r = BootstrapOps::icomb_exact(cc, {rs(mul(t(k / 2), t(k / 2)))}, {2}, -1.0);
+    } else {
+      Ciphertext p = rs(mul(t(k / 2), t(k / 2 + 1)));
+      const Ciphertext& t1 = T.at(1);
+      // T_1 times 1 encoded at p's scale q / s_1: after the rescale by q, exactly p's scale
+      Ciphertext a = rs(BootstrapOps::mult_at(cc, t1, 1.0, p->scalingFactor * BootstrapOps::prime(cc, t1->towers - 1) / t1->scalingFactor));
+      a->scalingFactor = p->scalingFactor;
+      r = BootstrapOps::icomb_exact(cc, {p, a}, {2, -1}, 0.0);
+    }
+    T[k] = r;
+    return r;
+  }
+  // the towers the evaluation of a piece of degree d ends at
+  uint32_t towers_of(uint32_t d) {
+    if (cheb_leaf(d)) {
+      uint32_t tw = ~0u;
+      for (uint32_t k = 1; k <= d; ++k) tw = std::min(tw, t(k)->towers);
+      return tw - 1;
+    }
+    uint32_t M = 1;
+    while (2 * M <= d) M *= 2;
+    const uint32_t tq = d - M == 0 ? t(M)->towers : towers_of(d - M);
+    return std::min(tq, t(M)->towers) - 1;
+  }
+  // the piece at scale `target` (0: the nominal scale at its leaf)
+  Ciphertext eval(const std::vector<double>& c, double target) {
+    const uint32_t d = (uint32_t)c.size() - 1;
+    if (cheb_leaf(d)) {
+      uint32_t towers = ~0u;
+      for (uint32_t k = 1; k <= d; ++k) towers = std::min(towers, t(k)->towers);
+      const double S = (target != 0.0 ? target : BootstrapOps::nominal(cc)) * BootstrapOps::prime(cc, towers - 1);
+      std::vector<Ciphertext> ts;
+      std::vector<double> ws, es;
+      for (uint32_t k = 1; k <= d; ++k) {
+        if (c[k] == 0.0) continue;
+        ts.push_back(t(k));
+        ws.push_back(c[k]);
+        es.push_back(S / t(k)->scalingFactor);
+      }
+      uint32_t used = ~0u;
+      for (const auto& x : ts) used = std::min(used, x->towers);
+      if (ts.empty() || used > towers) {
+        // the sum's level is that of the deepest T_k, used or not
+        ts.push_back(cc.LevelReduce(t(1), t(1)->towers - towers));
+        ws.push_back(0.0);
+        es.push_back(S / t(1)->scalingFactor);
+      }
+      return BootstrapOps::icomb_exact(cc, {rs(BootstrapOps::wsum_at(cc, ts, ws, es, S))}, {1}, c[0]);
+    }
+    uint32_t M = 1;
+    while (2 * M <= d) M *= 2;
+    std::vector<double> q(d - M + 1), r(c.begin(), c.begin() + M);
+    q[0] = c[M];
+    for (uint32_t j = 1; j <= d - M; ++j) {
+      q[j] = 2.0 * c[M + j];
+      r[M - j] -= c[M + j];
+    }
+    const Ciphertext tM = t(M);
+    Ciphertext P;
+    if (q.size() == 1) {
+      const double enc = target != 0.0 ? target * BootstrapOps::prime(cc, tM->towers - 1) / tM->scalingFactor : BootstrapOps::nominal(cc);
+      P = rs(BootstrapOps::mult_at(cc, tM, q[0], enc));
+    } else {
+      const uint32_t tp = std::min(towers_of(d - M), tM->towers);
+      const double tq = target != 0.0 ? target * BootstrapOps::prime(cc, tp - 1) / tM->scalingFactor : 0.0;
+      P = rs(mul(eval(q, tq), tM));
+    }
+    if (target != 0.0) P->scalingFactor = target;
+    Ciphertext R = eval(r, P->scalingFactor);
+    R->scalingFactor = P->scalingFactor;
+    return BootstrapOps::icomb_exact(cc, {P, R}, {1, 1}, 0.0);
   }
 };
 }  // namespace
@@ -461,9 +639,12 @@ void CryptoContextImpl::EvalBootstrapSetup(const std::vector<uint32_t>& levelBud
   // 0 = automatic: keep the message 2^-10 below q_0
   const int room = (int)params_.first_bits_ - (int)params_.scale_bits_;
   st->correction = correctionFactor ? correctionFactor : (uint32_t)std::max(0, 10 - room);
-  // EvalMod range and shape: |I| <= (h + 1) / 2 for a sparse secret of weight h
+  // EvalMod range and shape: |I| <= (h + 1) / 2 for a sparse secret of weight h, and at most
+  // OpenFHE's K_SPARSE = 28 (above that bound the overflow's spread, about sqrt(h / 12) per
+  // coefficient, keeps |I| below 28 with overwhelming probability: OpenFHE's choice for its
+  // SPARSE_TERNARY secrets of weight 192)
   const bool sparse = params_.dist_ == SecretKeyDist::SPARSE_TERNARY;
-  st->K = sparse ? std::ceil(params_.hw_ / 2.0) + 4.0 : 64.0;
+  st->K = sparse ? std::min(std::ceil(params_.hw_ / 2.0) + 4.0, 28.0) : 64.0;
   // sparse: a Chebyshev series of sin(a y) of degree d converges to double precision while
   // a <= 0.63 d (its coefficients fall like the Bessel values J_k(a): ~1e-14 at k = 127 for a = 80),
   // and each double angle multiplies the series' error by ~4, so: no double angle and degree 127
@@ -484,7 +665,16 @@ void CryptoContextImpl::EvalBootstrapSetup(const std::vector<uint32_t>& levelBud
   const int e = (int)std::ceil(-std::log2(fc));
   st->pre = std::ldexp(1.0, -e);
   const double fd = fc * std::ldexp(1.0, e);
-  st->cts = factored(n, M, levelBudget[0], true, fd);
+  // The raise's rescales (the prescale's, then one per CoeffsToSlots group) divide by the chain's
+  // top primes, not by the nominal scale the constants are encoded at, so the EvalMod input would
+  // arrive scaled by prod(scale / q_r). Its sine period must be exact (a relative error delta in
+  // that scale moves the result by 2 pi I delta for an overflow I, about 2^-13 at N=2^16 where
+  // the NTT-friendly primes sit ~2^-31 from the scale), so the first group carries the inverse.
+  const uint32_t ncts = std::max(1u, std::min(levelBudget[0], (uint32_t)fft_stages(n, M, true).size()));
+  double fix = 1.0;
+  for (uint32_t j = 0; j <= ncts; ++j) fix *= (double)moduli_[L1_ - 1 - j] / scale_;
+  st->scale_fix = fix;
+  st->cts = factored(n, M, levelBudget[0], true, fd * fix);
   st->stc = factored(n, M, levelBudget[1], false, fs);
   // CoeffsToSlots group g runs at OpenFHE level 1 + g (after the prescale)
   for (size_t g = 0; g < st->cts.size(); ++g) st->ctsDiags.push_back(groupDiagonals(st->cts[g], n, (uint32_t)(1 + g)));
@@ -511,7 +701,7 @@ void CryptoContextImpl::EvalBootstrapKeyGen(const PrivateKey& sk, uint32_t slots
 
 BootstrapConstants CryptoContextImpl::GetBootstrapConstants() const {
   if (!boot_) throw std::runtime_error("fideslib: EvalBootstrapSetup first");
-  return BootstrapConstants{boot_->slots, boot_->correction, boot_->K, boot_->r, boot_->pre, boot_->cheb, boot_->cts, boot_->stc};
+  return BootstrapConstants{boot_->slots, boot_->correction, boot_->K, boot_->r, boot_->pre, boot_->scale_fix, boot_->cheb, boot_->cts, boot_->stc};
 }
 
 Ciphertext CryptoContextImpl::EvalBootstrapToLevel(const Ciphertext& ct, uint32_t outputLevel, uint32_t numIterations, uint32_t precision) {
@@ -594,9 +784,19 @@ Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0, uint32_t numI
   Ciphertext cj = EvalConjugate(t);
   Ciphertext re = EvalAdd(t, cj);
   Ciphertext im = MultByMonomial(EvalSub(t, cj), 3 * N_ / 2);
-  // 5. EvalMod: sin(2 pi K y) on both parts, as one paired sequence
-  Ciphertext c = EvalChebyshevSeries(BootstrapOps::pack(*this, re, im), st.cheb, -1.0, 1.0);
-  for (int i = 0; i < st.r; ++i) c = IntegerCombination({Rescale(BootstrapOps::mult(*this, c, c))}, {2}, -1.0);
+  // 5. EvalMod: sin(2 pi K y) on both parts, as one paired sequence with exact scales (the slots
+  //    are exact at the nominal scale: the first CoeffsToSlots group carries the raise's drift)
+  Ciphertext y = BootstrapOps::pack(*this, re, im);
+  y->scalingFactor = scale_;
+  std::vector<double> cheb = st.cheb;
+  cheb[0] *= 0.5;
+  ChebExact ex{*this, {}, {}};
+  ex.T[1] = y;
+  Ciphertext c = ex.eval(cheb, 0.0);
+  for (int i = 0; i < st.r; ++i) c = BootstrapOps::icomb_exact(*this, {BootstrapOps::rescale_exact(*this, BootstrapOps::mult(*this, c, c))}, {2}, -1.0);
+  // the rest keeps the nominal bookkeeping (the output's relative scale error is the drift of the
+  // EvalMod and SlotsToCoeffs levels, about 2^-27 at N=2^16)
+  c->scalingFactor = scale_;
   Ciphertext fre = BootstrapOps::half(*this, c, 0), fim = BootstrapOps::half(*this, c, 1);
   // 6. recombine (i = X^(N/2)) and SlotsToCoeffs
   Ciphertext u = EvalAdd(fre, MultByMonomial(fim, N_ / 2));

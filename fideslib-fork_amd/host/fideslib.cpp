@@ -600,7 +600,10 @@ Ciphertext CryptoContextImpl::EvalSub(const Ciphertext& a0, const Plaintext& pt)
 // times 2^logApprox by CRT multiplication.
 std::vector<uint64_t> CryptoContextImpl::ElemForEvalMult(uint32_t towers, double operand) const {
   // the level's scaling factor: FIXEDMANUAL the same at every level, FLEXIBLEAUTO the level's own
-  const double scFactor = sfReal(towers);
+  return ElemForEvalMultAt(towers, operand, sfReal(towers));
+}
+
+std::vector<uint64_t> CryptoContextImpl::ElemForEvalMultAt(uint32_t towers, double operand, double scFactor) const {
   const int32_t MAX_BITS = 125, MAX_LOG_STEP = 60;
   int32_t logApprox = 0;
   const double res = std::fabs(operand * scFactor);
@@ -643,7 +646,10 @@ std::vector<uint64_t> CryptoContextImpl::ElemForEvalMult(uint32_t towers, double
 
 // ElemForEvalAddOrSub (src/CKKS/Context.cu:777-863) for a non-negative operand.
 std::vector<uint64_t> CryptoContextImpl::ElemForEvalAddOrSub(uint32_t towers, double operand, uint32_t noiseDeg) const {
-  const double scFactor = sfReal(towers);
+  return ElemForEvalAddOrSubAt(towers, operand, noiseDeg, sfReal(towers));
+}
+
+std::vector<uint64_t> CryptoContextImpl::ElemForEvalAddOrSubAt(uint32_t towers, double operand, uint32_t noiseDeg, double scFactor) const {
   const int32_t MAX_BITS = 61, MAX_LOG_STEP = 60;
   int32_t logApprox = 0;
   const double res = std::fabs(operand * scFactor);

@@ -145,6 +145,7 @@ struct BootstrapConstants {
   double K;                   // EvalMod input range
   int r;                      // double-angle steps
   double pre;                 // power of two applied after the raise
+  double scale_fix;           // prod(q_r / scale) over the raise's rescales, folded into the first CoeffsToSlots group
   std::vector<double> cheb;   // Chebyshev coefficients of cos((2 pi K y - pi/2) / 2^r) on [-1, 1]
   std::vector<BootstrapGroup> cts, stc;  // in the order they are applied
 };
@@ -355,6 +356,9 @@ class CryptoContextImpl {
   // ---- scalar CRT encodings (src/CKKS/Context.cu:669-863), exposed for tests
   std::vector<uint64_t> ElemForEvalMult(uint32_t towers, double operand) const;
   std::vector<uint64_t> ElemForEvalAddOrSub(uint32_t towers, double operand, uint32_t noiseDeg) const;
+  // the same encodings at an explicit scaling factor (the bootstrap's exact-scale EvalMod)
+  std::vector<uint64_t> ElemForEvalMultAt(uint32_t towers, double operand, double scFactor) const;
+  std::vector<uint64_t> ElemForEvalAddOrSubAt(uint32_t towers, double operand, uint32_t noiseDeg, double scFactor) const;
 
   void Synchronize() const;
   double ScalingFactor() const { return scale_; }

@@ -207,7 +207,7 @@ def factored(n, M, budget, inverse, factor):
 
 def bstep_for(rowSize):
     b = 1
-    while b * b < rowSize:
+    while b * b < rowSize and b < 16:
         b *= 2
     return b
 
@@ -244,9 +244,11 @@ def rotation_steps(k, N):
     return [r for r in steps if r % (N // 2)]
 
 
-def constants(N, n, hw, scale, q0, sparse=True, budget=(1, 1), correction=0):
+def constants(N, n, hw, scale, q0, sparse=True, budget=(1, 1), correction=0, top_primes=None):
+    """top_primes: the chain's primes from the top down; the raise's rescales (the prescale's and
+    one per CoeffsToSlots group) divide by them, and the first group carries prod(q_r / scale)."""
     gap = (N // 2) // n
-    K = math.ceil(hw / 2.0) + 4.0 if sparse else 64.0
+    K = min(math.ceil(hw / 2.0) + 4.0, 28.0) if sparse else 64.0
     r, degree = 5, 63
     if sparse:
         r = 0
@@ -260,8 +262,13 @@ def constants(N, n, hw, scale, q0, sparse=True, budget=(1, 1), correction=0):
     # diagonals carry fc 2^e in [1, 2)
     e = math.ceil(-math.log2(fc))
     fs = q0 / (2.0 * math.pi * scale)
-    return dict(slots=n, gap=gap, K=K, r=r, cheb=cheb, pre=math.ldexp(1.0, -e), correction=correction,
-                cts=factored(n, M, budget[0], True, fc * math.ldexp(1.0, e)), stc=factored(n, M, budget[1], False, fs))
+    ncts = max(1, min(budget[0], n.bit_length() - 1))
+    fix = 1.0
+    if top_primes is not None:
+        for j in range(ncts + 1):
+            fix *= float(top_primes[j]) / scale
+    return dict(slots=n, gap=gap, K=K, r=r, cheb=cheb, pre=math.ldexp(1.0, -e), correction=correction, scale_fix=fix,
+                cts=factored(n, M, budget[0], True, fc * math.ldexp(1.0, e) * fix), stc=factored(n, M, budget[1], False, fs))
 
 
 # ---- ciphertext operations with the facade's metadata rules (towers t, noise degree) ----
@@ -425,6 +432,137 @@ def chebyshev_series(ops, y, coeffs, a=-1.0, b=1.0):
     return ev(c)
 
 
+# ---- the bootstrap's EvalMod with exact scales (the facade's ChebExact / BootstrapOps exact forms) ----
+class ExactOps:
+    """Ciphertexts (residues, towers, noise degree, true scale): rescales divide the tracked scale by
+    the dropped prime; constants are encoded at the true scale of what they are added to; scalar
+    products at an explicit encoding scale."""
+
+    def __init__(self, ops):
+        self.b, self.o, self.q = ops, ops.o, ops.q
+
+    def rs(self, a):
+        x, t, d, s = a
+        r = self.b.rescale((x, t, d))
+        return (*r, s / float(self.q[t - 1]))
+
+    def mult(self, a, b):
+        r = self.b.mult(a[:3], b[:3])
+        return (*r, a[3] * b[3])
+
+    def mult_at(self, a, c, enc):
+        x, t, d, s = a
+        f = O.elem_for_eval_mult(self.q[:t], c, enc)
+        return (np.concatenate([self.o.mul_scalar(self.b.comp(x, t, i), f) for i in range(2)]), t, d + 1, s * enc)
+
+    def reduce(self, a, t_to):
+        return (*self.b.reduce(a[:3], t_to), a[3])
+
+    def add(self, a, b):
+        assert a[3] == b[3], "exact-scale sum of different scales"
+        return (*self.b.add(a[:3], b[:3]), a[3])
+
+    def sub(self, a, b):
+        assert a[3] == b[3], "exact-scale sum of different scales"
+        return (*self.b.sub(a[:3], b[:3]), a[3])
+
+    def add_const(self, a, c):
+        x, t, d, s = a
+        if c == 0.0:
+            return a
+        e = O.elem_for_eval_add(self.q[:t], abs(c), s, 1)
+        if c < 0:
+            e = np.array([(qi - int(v)) % qi for v, qi in zip(e, self.q[:t])], dtype=np.uint64)
+        return (np.concatenate([self.o.add_scalar(self.b.comp(x, t, 0), e), self.b.comp(x, t, 1)]), t, d, s)
+
+
+def chebyshev_exact(xo, y, c):
+    """sum_k c_k T_k(y) (c_0 already halved), y = (residues, towers, degree, true scale)."""
+    T = {1: y}
+
+    def t(k):
+        if k in T:
+            return T[k]
+        if k % 2 == 0:
+            r = xo.rs(xo.mult(t(k // 2), t(k // 2)))
+            r = xo.add_const(xo.add(r, r), -1.0)
+        else:
+            p = xo.rs(xo.mult(t(k // 2), t(k // 2 + 1)))
+            t1 = T[1]
+            a = xo.rs(xo.mult_at(t1, 1.0, p[3] * float(xo.q[t1[1] - 1]) / t1[3]))
+            a = (a[0], a[1], a[2], p[3])
+            r = xo.sub(xo.add(p, p), a)
+        T[k] = r
+        return r
+
+    def towers_of(d):
+        if cheb_leaf(d):
+            return min(t(k)[1] for k in range(1, d + 1)) - 1
+        M = 1
+        while 2 * M <= d:
+            M *= 2
+        tq = t(M)[1] if d - M == 0 else towers_of(d - M)
+        return min(tq, t(M)[1]) - 1
+
+    def ev(cc, target):
+        d = len(cc) - 1
+        if cheb_leaf(d):
+            towers = min(t(k)[1] for k in range(1, d + 1))
+            S = (target if target != 0.0 else xo.b.scale) * float(xo.q[towers - 1])
+            acc = None
+            used = None
+            for k in range(1, d + 1):
+                if cc[k] == 0.0:
+                    continue
+                tk = t(k)
+                used = tk[1] if used is None else min(used, tk[1])
+                term = xo.mult_at(xo.reduce(tk, towers), cc[k], S / tk[3])
+                term = (term[0], term[1], term[2], S)
+                acc = term if acc is None else xo.add(acc, term)
+            if acc is None or used > towers:
+                t1 = t(1)
+                z = xo.mult_at(xo.reduce(t1, towers), 0.0, S / t1[3])
+                z = (z[0], z[1], z[2], S)
+                acc = z if acc is None else xo.add(acc, z)
+            return xo.add_const(xo.rs(acc), cc[0])
+        M = 1
+        while 2 * M <= d:
+            M *= 2
+        q = [0.0] * (d - M + 1)
+        r = list(cc[:M])
+        q[0] = cc[M]
+        for j in range(1, d - M + 1):
+            q[j] = 2.0 * cc[M + j]
+            r[M - j] -= cc[M + j]
+        tM = t(M)
+        if len(q) == 1:
+            enc = target * float(xo.q[tM[1] - 1]) / tM[3] if target != 0.0 else xo.b.scale
+            P = xo.rs(xo.mult_at(tM, q[0], enc))
+        else:
+            tp = min(towers_of(d - M), tM[1])
+            tq = target * float(xo.q[tp - 1]) / tM[3] if target != 0.0 else 0.0
+            P = xo.rs(xo.mult(ev(q, tq), tM))
+        if target != 0.0:
+            P = (P[0], P[1], P[2], target)
+        R = ev(r, P[3])
+        R = (R[0], R[1], R[2], P[3])
+        return xo.add(P, R)
+
+    return ev(c, 0.0)
+
+
+def evalmod_exact(ops, y, k):
+    """EvalMod of y (slots exact at the nominal scale): the series and double angles with exact scales."""
+    xo = ExactOps(ops)
+    c = list(k["cheb"])
+    c[0] *= 0.5
+    v = chebyshev_exact(xo, (y[0], y[1], y[2], ops.scale), c)
+    for _ in range(k["r"]):
+        v = xo.rs(xo.mult(v, v))
+        v = xo.add_const(xo.add(v, v), -1.0)
+    return v[:3]
+
+
 def bootstrap(ops, ct, k):
     """EvalBootstrap of ct = (residues, towers, noise degree) with constants k."""
     o, N = ops.o, ops.N
@@ -447,14 +585,7 @@ def bootstrap(ops, ct, k):
     re = ops.add(t, cj)
     im = ops.monomial(ops.sub(t, cj), 3 * N // 2)
 
-    def evalmod(y):
-        c = chebyshev_series(ops, y, k["cheb"])
-        for _ in range(k["r"]):
-            c = ops.rescale(ops.mult(c, c))
-            c = ops.add_scalar(ops.add(c, c), -1.0)
-        return c
-
-    u = ops.add(evalmod(re), ops.monomial(evalmod(im), N // 2))
+    u = ops.add(evalmod_exact(ops, re, k), ops.monomial(evalmod_exact(ops, im, k), N // 2))
     for g in k["stc"]:
         u = ops.rescale(ops.linear_transform(u, g, n))
     if corr:

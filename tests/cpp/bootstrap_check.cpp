@@ -7,9 +7,13 @@
 // The inputs, outputs and bootstrap constants are written to <out_dir> so that the test can replay
 // the same sequence with the oracle and compare residues.
 //
-// usage: bootstrap_check <out_dir> <logN> <slots> [seed] [cts_budget stc_budget depth scale_bits hw technique meta_precision]
+// usage: bootstrap_check <out_dir> <logN> <slots> [seed] [cts_budget stc_budget depth scale_bits hw technique meta_precision
+//                        first_mod_bits dnum real_input input_level]
 // technique: 0 FIXEDMANUAL (default), 1 FLEXIBLEAUTO; meta_precision > 0 also runs the two-iteration
-// bootstrap (Meta-BTS) at that precision and reports its error. (prints one JSON line)
+// bootstrap (Meta-BTS) at that precision and reports its error; first_mod_bits (default 60) and
+// dnum (default 3) shape the modulus chain; real_input 1 draws real slot values (the reference's
+// precision test); input_level > 0 encrypts at that OpenFHE level (levels consumed) instead of the
+// lowest one the bootstrap accepts. (prints one JSON line)
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -84,13 +88,17 @@ int main(int argc, char** argv) {
   const uint32_t hw = argc > 9 ? (uint32_t)std::atoi(argv[9]) : 16;
   const bool flex = argc > 10 && std::atoi(argv[10]) == 1;
   const uint32_t meta_prec = argc > 11 ? (uint32_t)std::atoi(argv[11]) : 0;
+  const uint32_t first_bits = argc > 12 ? (uint32_t)std::atoi(argv[12]) : 60;
+  const uint32_t dnum = argc > 13 ? (uint32_t)std::atoi(argv[13]) : 3;
+  const bool real_input = argc > 14 && std::atoi(argv[14]) == 1;
+  const uint32_t in_level = argc > 15 ? (uint32_t)std::atoi(argv[15]) : 0;
   try {
     CCParams p;
     p.SetRingDim(1u << logN);
     p.SetMultiplicativeDepth(depth);
     p.SetScalingModSize(sbits);
-    p.SetFirstModSize(60);
-    p.SetNumLargeDigits(3);
+    p.SetFirstModSize(first_bits);
+    p.SetNumLargeDigits(dnum);
     p.SetSecretKeyDist(SecretKeyDist::SPARSE_TERNARY);
     p.SetHammingWeight(hw);
     p.SetBatchSize(slots);
@@ -106,12 +114,13 @@ int main(int argc, char** argv) {
     std::mt19937_64 rng(seed);
     std::uniform_real_distribution<double> U(-1.0, 1.0);
     cvec z(slots);
-    for (auto& v : z) v = {U(rng), U(rng)};
+    for (auto& v : z) v = real_input ? std::complex<double>(U(rng), 0.0) : std::complex<double>(U(rng), U(rng));
 
     // a ciphertext with the towers the bootstrap needs left (one, two with a correction factor or
     // under FLEXIBLEAUTO, whose scale adjustment takes a rescale)
     const uint32_t tin = cc->GetBootstrapConstants().correction || flex ? 2 : 1;
-    auto pt = cc->MakeCKKSPackedPlaintext(z, 1, L1 - tin, slots);
+    if (in_level > L1 - tin) throw std::runtime_error("input level leaves the bootstrap too few towers");
+    auto pt = cc->MakeCKKSPackedPlaintext(z, 1, in_level ? in_level : L1 - tin, slots);
     auto ct = cc->Encrypt(kp.secretKey, pt);
     auto bt = cc->EvalBootstrap(ct);
     auto dec = cc->Decrypt(kp.secretKey, bt);
@@ -192,9 +201,9 @@ int main(int argc, char** argv) {
     }
     {
       FILE* f = std::fopen((g_dir + "/meta.json").c_str(), "w");
-      std::fprintf(f, "{\"flex\": %d, \"logN\": %d, \"slots\": %u, \"seed\": %llu, \"L1\": %u, \"K\": %u, \"dnum\": 3, \"hw\": %u, "
-                      "\"scale\": %.17g, \"Krange\": %.17g, \"r\": %d, \"pre\": %.17g, \"correction\": %u, %s, \"moduli\": [",
-                   flex ? 1 : 0, logN, slots, (unsigned long long)seed, L1, (unsigned)(cc->GetModuli().size() - L1), hw, cc->ScalingFactor(), bc.K, bc.r, bc.pre,
+      std::fprintf(f, "{\"flex\": %d, \"logN\": %d, \"slots\": %u, \"seed\": %llu, \"L1\": %u, \"K\": %u, \"dnum\": %u, \"hw\": %u, "
+                      "\"scale\": %.17g, \"Krange\": %.17g, \"r\": %d, \"pre\": %.17g, \"scale_fix\": %.17g, \"correction\": %u, %s, \"moduli\": [",
+                   flex ? 1 : 0, logN, slots, (unsigned long long)seed, L1, (unsigned)(cc->GetModuli().size() - L1), dnum, hw, cc->ScalingFactor(), bc.K, bc.r, bc.pre, bc.scale_fix,
                    bc.correction, groups.c_str());
       for (size_t i = 0; i < cc->GetModuli().size(); ++i)
         std::fprintf(f, "%s\"%llu\"", i ? ", " : "", (unsigned long long)cc->GetModuli()[i]);

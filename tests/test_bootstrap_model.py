@@ -120,9 +120,12 @@ def test_factored_transforms_compose_to_the_special_fft():
             assert np.max(np.abs(A - want)) < 1e-12, (budget, inverse)
 
 
-@pytest.mark.parametrize("n,budget,L1,sbits,corr,bits,hw", [(8, (1, 1), 16, 50, 0, 19, 16), (64, (1, 1), 16, 50, 0, 17, 16),
-                                                           (64, (2, 3), 18, 50, 0, 14, 16), (512, (3, 3), 18, 50, 0, 12, 16),
-                                                           (512, (3, 3), 19, 59, 9, 20, 16), (512, (3, 3), 20, 59, 9, 18, 32)])
+# bits: what the series with exact scales reaches with a margin of about one bit; the nominal
+# bookkeeping (rescales taken to divide by the scale itself) loses 1 to 3.5 bits at 50-bit scaling
+# and misses the first four bounds
+@pytest.mark.parametrize("n,budget,L1,sbits,corr,bits,hw", [(8, (1, 1), 16, 50, 0, 20, 16), (64, (1, 1), 16, 50, 0, 20.5, 16),
+                                                           (64, (2, 3), 18, 50, 0, 17.5, 16), (512, (3, 3), 18, 50, 0, 15.5, 16),
+                                                           (512, (3, 3), 19, 59, 9, 23.5, 16), (512, (3, 3), 20, 59, 9, 23.5, 32)])
 def test_bootstrap_decrypts(n, budget, L1, sbits, corr, bits, hw):
     o = _ctx(L1=L1, K=9 if sbits > 50 else 6, sbits=sbits)
     N, scale = o.N, 2.0 ** sbits

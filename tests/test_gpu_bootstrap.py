@@ -34,16 +34,18 @@ BCHECK = os.path.join(PKG, "_build", "bootstrap_check")
 
 
 # (logN, slots, CoeffsToSlots budget, SlotsToCoeffs budget, multiplicative depth, scaling bits,
-# bits required, secret weight, technique 0 FIXEDMANUAL / 1 FLEXIBLEAUTO); the CPU model of the same
-# sequences measures 15.8, 15.0, 10.0 and 21.4 bits; weight 32 takes a degree-255 series. The N=2^16
-# runs are the serving workload's bootstrap (all 32768 slots, budget {3,3}, h=32, 59-bit scaling
-# under a 60-bit q_0): the reference states 11.5-15 bits for its GPU path at N=2^16
-# (test/BootstrapPrecisionTests.cu:4-7), the bound they are held to; their residues are not replayed
-# (the oracle's bootstrap at N=2^16 takes too long), nor those of FLEXIBLEAUTO runs (this
-# bootstrap's own scale adjustment, not a reference sequence).
-RUNS = [(12, 8, 1, 1, 15, 50, 14, 16, 0), (12, 64, 2, 3, 17, 50, 13, 16, 0), (12, 2048, 3, 3, 18, 50, 9, 16, 0),
-        (12, 2048, 3, 3, 19, 59, 19, 16, 0), (12, 2048, 3, 3, 20, 59, 15, 32, 0), (12, 2048, 3, 3, 20, 59, 15, 32, 1),
-        (16, 32768, 3, 3, 24, 59, 11.5, 32, 0), (16, 32768, 3, 3, 24, 59, 11.5, 32, 1)]
+# bits required, secret weight, technique 0 FIXEDMANUAL / 1 FLEXIBLEAUTO). The bounds sit 1.5-2 bits
+# under what the runs measure (19.7, 21.0, 13.9, 26.9, 26.5, 26.6, 23.4 and 23.3 bits): the
+# EvalMod series tracks exact scales, and with the nominal bookkeeping (each rescale taken to divide
+# by the scale itself) the N=2^16 runs measured 12.3 bits. Weight 32 takes a degree-255 series. The
+# N=2^16 runs are the serving workload's bootstrap (all 32768 slots, budget {3,3}, h=32, 59-bit
+# scaling under a 60-bit q_0): the reference states 11.5-15 bits for its GPU path at N=2^16
+# (test/BootstrapPrecisionTests.cu:4-7); their residues are not replayed (the oracle's bootstrap at
+# N=2^16 takes too long), nor those of FLEXIBLEAUTO runs (this bootstrap's own scale adjustment,
+# not a reference sequence).
+RUNS = [(12, 8, 1, 1, 15, 50, 18, 16, 0), (12, 64, 2, 3, 17, 50, 19, 16, 0), (12, 2048, 3, 3, 18, 50, 12, 16, 0),
+        (12, 2048, 3, 3, 19, 59, 25, 16, 0), (12, 2048, 3, 3, 20, 59, 24.5, 32, 0), (12, 2048, 3, 3, 20, 59, 24.5, 32, 1),
+        (16, 32768, 3, 3, 24, 59, 21.5, 32, 0), (16, 32768, 3, 3, 24, 59, 21.5, 32, 1)]
 
 
 @pytest.fixture(scope="module", params=RUNS, ids=["N4096_slots8_b11", "N4096_slots64_b23", "N4096_full_b33", "N4096_full_b33_scale59",
@@ -80,7 +82,29 @@ def test_bootstrap_precision(boot_run):
     if res["meta_precision"]:
         # the second iteration refreshes the first one's error: far beyond the single bootstrap
         print({"boot_bits": res["boot_bits"], "meta_bits": res["meta_bits"]})
-        assert res["meta_bits"] >= res["boot_bits"] + 6, res
+        assert res["meta_bits"] >= res["boot_bits"] + 10, res
+
+
+def test_bootstrap_precision_at_reference_point(tmp_path):
+    """The reference's own bootstrap precision point (test/BootstrapPrecisionTests.cu:1-7, 58-82):
+    N=2^16, depth 25, 55-bit scaling under a 56-bit q_0, level budget {3,2}, dnum 6, FLEXIBLEAUTO,
+    sparse ternary secret (OpenFHE's SPARSE_TERNARY draws weight 192), 2^15 slots of real values in
+    [-1, 1] encrypted at level 21. The reference states 18.6 bits for its OpenFHE path, 11.5-15
+    bits for its GPU path and 29-32 bits for the two-iteration (Meta-BTS) refresh there. This
+    bootstrap measures 16.3 bits and Meta-BTS 28.4: one iteration is held to the GPU path's upper
+    bound, Meta-BTS to 27 bits."""
+    if not os.path.exists(BCHECK):
+        pytest.fail("bootstrap_check is not built (run __graft_entry__.build())")
+    args = [str(tmp_path), "16", "32768", "7", "3", "2", "25", "55", "192", "1", "12", "56", "6", "1", "21"]
+    print(f"[bootstrap_check] {BCHECK} {' '.join(args)}", flush=True)
+    r = subprocess.run([BCHECK] + args, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    print({k: res[k] for k in ("technique", "L1", "depth", "towers_in", "towers_out", "boot_bits", "meta_bits", "boot_ms", "meta_ms")})
+    assert res["technique"] == "FLEXIBLEAUTO" and res["L1"] == 26 and res["towers_in"] == 26 - 21
+    assert res["boot_bits"] >= 15.0, res
+    assert res["deterministic"]
+    assert res["meta_bits"] >= 27.0, res
 
 
 def _u64(out, name, rows, N):
@@ -171,11 +195,19 @@ def test_bootstrap_constants_closed_form(boot_run):
                 A = G @ A
             return A
 
-        assert np.allclose(product(_groups(out, meta, "cts")) * meta["pre"], fc * np.linalg.inv(UP), rtol=0, atol=1e-12 * fc)
+        # (the first group also carries the rescale-drift factor prod(q_r / scale))
+        assert np.allclose(product(_groups(out, meta, "cts")) * meta["pre"], fc * meta["scale_fix"] * np.linalg.inv(UP), rtol=0,
+                           atol=1e-12 * fc)
         assert np.allclose(product(_groups(out, meta, "stc")), q0 / (2 * math.pi * scale) * UP, rtol=0, atol=1e-9)
     assert 1.0 <= fc / meta["pre"] < 2.0 and math.log2(meta["pre"]).is_integer()
     cheb = np.fromfile(out / "cheb.f64", dtype=np.float64)
-    assert Kr == math.ceil(meta["hw"] / 2) + 4 and r == 0 and len(cheb) == (128 if meta["hw"] <= 21 else 256)
+    assert Kr == min(math.ceil(meta["hw"] / 2) + 4, 28) and r == 0 and len(cheb) == (128 if meta["hw"] <= 21 else 256)
+    # the drift factor: the top primes the prescale and the CoeffsToSlots groups rescale by, over the scale
+    L1 = meta["L1"]
+    fix = 1.0
+    for j in range(len(meta["cts"]) + 1):
+        fix *= float(int(meta["moduli"][L1 - 1 - j])) / scale
+    assert abs(meta["scale_fix"] / fix - 1) < 1e-15
     ys = np.linspace(-1, 1, 101)
     approx = np.polynomial.chebyshev.chebval(ys, np.concatenate([[cheb[0] / 2], cheb[1:]]))
     assert np.max(np.abs(approx - np.cos((2 * math.pi * Kr * ys - math.pi / 2) / 2 ** r))) < 1e-12
