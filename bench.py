@@ -758,8 +758,9 @@ def serving_bootstrap_run(a):
     out = {"rate_per_s": 50.0, "requests": BOOT_SERVE_REQUESTS, "mix": "70% inference, 25% polynomial, 5% bootstrap",
            "p99_other": "p99 over the requests other than bootstraps"}
     # serial: each bootstrap runs in the tick that reaches it; concurrent: on the loop's worker
-    # thread through a second context (same keys, its own stream) while the other requests tick
-    for name, conc in (("serial_bootstrap", "0"), ("concurrent_bootstrap", "1")):
+    # thread through a second context (same keys, its own stream) while the other requests tick;
+    # yielding: the worker issues one bootstrap operation at a time, in the gaps between ticks
+    for name, conc in (("serial_bootstrap", "0"), ("concurrent_bootstrap", "1"), ("yielding_bootstrap", "2")):
         args = ["16", "32768", "3", "3", "24", "59", "50", str(BOOT_SERVE_REQUESTS), "0.05", str(a.seed), "0.70", "32", "0", conc]
         r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=600)
         if r.returncode != 0:

@@ -23,6 +23,7 @@ struct fh_key {
 namespace {
 
 thread_local std::string g_err;
+thread_local fh::CallGate g_gate;
 
 template <class F>
 int guard(F&& f) {
@@ -41,8 +42,16 @@ int guard(F&& f) {
   }
 }
 
+}  // namespace
+
+fh::CallGate& fh::thread_call_gate() { return g_gate; }
+
+namespace {
+
 fh::Context& C(fh_context* h) {
   if (!h || !h->c) throw fh::Error(FH_E_INVALID, "null context");
+  const fh::CallGate& g = fh::thread_call_gate();
+  if (g.fn) g.fn(g.user, *h->c);
   return *h->c;
 }
 const fh::Context& C(const fh_context* h) {
@@ -806,7 +815,7 @@ int fh_serve_run(fh_context* ctx, const fh_serve_config* config, const fh_key* c
     cfg.request_outputs = config->request_outputs;
     cfg.request_output_capacity = config->request_output_capacity;
     cfg.request_output_words = config->request_output_words;
-    cfg.bootstrap_concurrent = config->bootstrap_concurrent != 0;
+    cfg.bootstrap_concurrent = config->bootstrap_concurrent;
     cfg.ticks_in_flight = config->ticks_in_flight;
     cfg.batch_min_ops = config->batch_min_ops;
     if (cfg.mode < 0 || cfg.mode > 2) throw fh::Error(FH_E_INVALID, "mode must be 0 (batched), 1 (per operation) or 2 (one request at a time)");

@@ -274,4 +274,13 @@ struct Context {
   T* upload(const std::vector<T>& v);
 };
 
+// A hook run at the start of every engine call this thread makes on a context (before the call
+// issues any work): the serving loop's bootstrap worker sets it to issue a bootstrap one
+// operation at a time, in the gaps between the loop's ticks. Unset (fn null) by default.
+struct CallGate {
+  void (*fn)(void* user, Context& c) = nullptr;
+  void* user = nullptr;
+};
+CallGate& thread_call_gate();
+
 }  // namespace fh

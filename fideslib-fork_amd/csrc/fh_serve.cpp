@@ -108,7 +108,8 @@ class BootWorker {
     int rc, level;
     double t;
   };
-  BootWorker(const ServeConfig& cfg, std::chrono::steady_clock::time_point t0) : cfg_(cfg), t0_(t0), th_([this] { loop(); }) {}
+  BootWorker(const ServeConfig& cfg, std::chrono::steady_clock::time_point t0, hipStream_t loop_stream)
+      : cfg_(cfg), t0_(t0), loop_stream_(loop_stream), th_([this] { loop(); }) {}
   ~BootWorker() {
     {
       std::lock_guard<std::mutex> lk(mu_);
@@ -138,7 +139,24 @@ class BootWorker {
   }
 
  private:
+  // The yielding gate (run before each engine call of the bootstrap on this thread): the
+  // bootstrap's previous operation has finished, and the loop's stream holds no tick (or
+  // kYieldCapMs have passed, so that a loop that never idles does not starve the bootstrap).
+  static void yield_gate(void* user, Context& bc) {
+    const hipStream_t loop = static_cast<const BootWorker*>(user)->loop_stream_;
+    FH_HIP(hipStreamSynchronize(bc.stream));
+    const auto start = std::chrono::steady_clock::now();
+    for (;;) {
+      const hipError_t e = hipStreamQuery(loop);
+      if (e == hipSuccess) return;
+      if (e != hipErrorNotReady) FH_HIP(e);
+      (void)hipGetLastError();  // a not-ready query leaves its status behind on this thread
+      if (std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count() >= kYieldCapMs) return;
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+  }
   void loop() {
+    if (cfg_.bootstrap_concurrent == 2) thread_call_gate() = CallGate{&BootWorker::yield_gate, this};
     for (;;) {
       Job j;
       {
@@ -161,6 +179,7 @@ class BootWorker {
   }
   const ServeConfig& cfg_;
   const std::chrono::steady_clock::time_point t0_;
+  const hipStream_t loop_stream_;
   std::mutex mu_;
   std::condition_variable cv_, cv_done_;
   std::deque<Job> q_;
@@ -198,6 +217,8 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
   const hipStream_t st = c.stream;
   const int depth = cfg.ticks_in_flight > 0 ? cfg.ticks_in_flight : kDefaultTicksInFlight;
   if (cfg.batch_min_ops < 0) throw Error(-1, "batch_min_ops must be >= 0");
+  if (cfg.bootstrap_concurrent < 0 || cfg.bootstrap_concurrent > 2)
+    throw Error(-1, "bootstrap_concurrent must be 0 (in the tick), 1 (worker thread) or 2 (worker thread, yielding)");
   const int batch_min = cfg.batch_min_ops > 0 ? cfg.batch_min_ops : kDefaultBatchMinOps;
 
   // shared plaintexts: 0 = weights, 1..8 = polynomial coefficients c0..c7 (synthetic residues)
@@ -256,7 +277,7 @@ ServeResult serve_run(Context& c, const ServeConfig& cfg, const uint64_t* const*
   const hipEvent_t ev0 = evs.take();
   FH_HIP(hipEventRecord(ev0, st));
   std::unique_ptr<BootWorker> worker;  // after every buffer it uses: stopped and joined first
-  if (cfg.bootstrap_concurrent && cfg.bootstrap_share > 0) worker.reset(new BootWorker(cfg, t0));
+  if (cfg.bootstrap_concurrent && cfg.bootstrap_share > 0) worker.reset(new BootWorker(cfg, t0, st));
 
   // A finished request's result is copied on the stream into its slot of a device-side store
   // (ordered after the work that wrote it) and read back after the session, so capturing results

@@ -12,7 +12,8 @@
 //     host-orchestrated above this layer), run either in the tick that reaches it (after the
 //     tick's other operations) or, with bootstrap_concurrent, on a worker thread of the loop while
 //     the other requests keep ticking (the function then works on a stream of its own: a second
-//     context with the same keys).
+//     context with the same keys), optionally yielding: one bootstrap operation at a time, issued
+//     in the gaps between the loop's ticks.
 // Admission is bounded by an HBM budget (each request's register footprint). Every scheduler
 // tick gathers the next operation of every admitted request into one batch, whatever its kind
 // and level, and executes it as one launch sequence (or, in the per-operation baseline mode, one
@@ -32,6 +33,7 @@ namespace fh {
 
 constexpr int kDefaultTicksInFlight = 3;
 constexpr int kDefaultBatchMinOps = 3;
+constexpr double kYieldCapMs = 2.0;  // longest a yielding bootstrap operation waits for the loop
 
 struct ServeConfig {
   double rate_per_s = 100.0;   // mean arrival rate
@@ -56,8 +58,11 @@ struct ServeConfig {
   uint64_t* request_output_words = nullptr;
   // bootstrap requests on a worker thread, concurrently with the other requests: `bootstrap` is
   // then called from that thread, must use a stream other than the context's and must return only
-  // once `out` is written
-  bool bootstrap_concurrent = false;
+  // once `out` is written. 1: the worker issues the bootstrap freely; 2 (yielding): each engine
+  // call the bootstrap makes waits until its previous one has finished and the loop has no tick
+  // on the GPU (or kYieldCapMs have passed), so a tick shares the GPU with at most one bootstrap
+  // operation
+  int bootstrap_concurrent = 0;
   // ticks the host may issue ahead of the GPU (0: kDefaultTicksInFlight; 1: wait for each tick)
   int ticks_in_flight = 0;
   // mode 0: smallest tick issued as one batch (0: kDefaultBatchMinOps); smaller ticks run per operation

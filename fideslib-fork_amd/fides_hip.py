@@ -396,14 +396,16 @@ class Context:
     # ---- serving loop ----
     def serve(self, rot_keys, rot_galois, relin_key, rate_per_s, requests, start_level=None, mode=0, hbm_budget_gb=64.0,
               inference_share=0.74, seed=1, checksum=False, bootstrap_share=0.0, request_checksums=False, request_outputs=False,
-              ticks_in_flight=0, batch_min_ops=0):
+              ticks_in_flight=0, batch_min_ops=0, bootstrap_concurrent=0):
         """bootstrap_share > 0 needs a bootstrap function, which only C/C++ callers pass (the C++
         facade's EvalBootstrap, tests/cpp/serve_check.cpp); from here it is rejected. mode 0:
         batched ticks, 1: one call per operation, 2: one request at a time. request_checksums:
         also return each request's result-word sum (list indexed by request). request_outputs: also
         return each request's whole result (numpy arrays indexed by request). ticks_in_flight:
         scheduler ticks issued ahead of the GPU (0: the default, 1: wait for every tick).
-        batch_min_ops: mode 0's smallest tick run as one batch (0: the default, 3)."""
+        batch_min_ops: mode 0's smallest tick run as one batch (0: the default, 3).
+        bootstrap_concurrent: where bootstrap requests run (0 in the tick, 1 on a worker thread, 2
+        on a worker thread that yields to the ticks); validated even without bootstrap requests."""
         per = (C.c_uint64 * max(1, requests))() if request_checksums else None
         cap = 2 * self.L1 * self.N
         bufs = [np.zeros(cap, dtype=np.uint64) for _ in range(requests)] if request_outputs else None
@@ -413,7 +415,7 @@ class Context:
                           inference_share, seed, int(checksum), float(bootstrap_share), None, None,
                           C.cast(per, C.POINTER(C.c_uint64)) if per is not None else None,
                           C.cast(ptrs, C.POINTER(C.c_void_p)) if ptrs is not None else None, cap if bufs else 0,
-                          C.cast(words, C.POINTER(C.c_uint64)) if words is not None else None, 0, int(ticks_in_flight),
+                          C.cast(words, C.POINTER(C.c_uint64)) if words is not None else None, int(bootstrap_concurrent), int(ticks_in_flight),
                           int(batch_min_ops))
         kp = (C.c_void_p * 8)(*[k.h for k in rot_keys])
         gl = (C.c_uint32 * 8)(*[int(g) for g in rot_galois])

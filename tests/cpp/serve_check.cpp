@@ -11,7 +11,8 @@
 // the loop's other request programs are integer programs at the C ABI, the same under either.
 // concurrent 1: the bootstrap requests run on the loop's worker thread, through a second facade
 // context built from the same parameters and seed (so the same keys) with its own stream, while
-// the other requests keep ticking on the first. Every request's whole result is captured in each
+// the other requests keep ticking on the first; 2: the same with the worker yielding (one
+// bootstrap operation at a time, between the loop's ticks). Every request's whole result is captured in each
 // session and compared bit for bit.
 #include <algorithm>
 #include <cstdio>
@@ -78,7 +79,7 @@ int main(int argc, char** argv) {
   const double inf_share = argc > 11 ? std::atof(argv[11]) : 0.5;
   const uint32_t hw = argc > 12 ? (uint32_t)std::atoi(argv[12]) : 16;
   const bool flex = argc > 13 && std::atoi(argv[13]) == 1;
-  const bool concurrent = argc > 14 && std::atoi(argv[14]) == 1;
+  const int concurrent = argc > 14 ? std::atoi(argv[14]) : 0;
   const int ticks_in_flight = argc > 15 ? std::atoi(argv[15]) : 0;
   try {
     CCParams p;
@@ -115,7 +116,7 @@ int main(int argc, char** argv) {
       if (fh_galois_index(cc->Native(), s, &g) != FH_OK) throw std::runtime_error(fh_last_error());
       rg.push_back(g);
     }
-    BootUser user{cc_boot.get(), slots, 0, flex, concurrent};
+    BootUser user{cc_boot.get(), slots, 0, flex, concurrent != 0};
     fh_serve_result res[4];
     std::vector<std::vector<uint64_t>> per(4, std::vector<uint64_t>(requests, 0));
     // whole results of the batched, per-operation and alone sessions
@@ -145,7 +146,7 @@ int main(int argc, char** argv) {
       cfg.bootstrap = boot_cb;
       cfg.bootstrap_user = &user;
       cfg.request_checksums = per[run].data();
-      cfg.bootstrap_concurrent = concurrent ? 1 : 0;
+      cfg.bootstrap_concurrent = concurrent;
       cfg.ticks_in_flight = ticks_in_flight;
       if (run > 0) {
         cfg.request_outputs = optr[run].data();
@@ -166,12 +167,12 @@ int main(int argc, char** argv) {
                 std::equal(outs[2][i].begin(), outs[2][i].begin() + w, outs[3][i].begin());
     }
     std::printf("{\"logN\": %d, \"slots\": %u, \"L1\": %u, \"boot_depth\": %u, \"rate_per_s\": %.3f, \"requests\": %d, "
-                "\"bootstrap_share\": %.3f, \"callbacks\": %d, \"technique\": \"%s\", \"concurrent\": %s, \"ticks_in_flight\": %d, "
+                "\"bootstrap_share\": %.3f, \"callbacks\": %d, \"technique\": \"%s\", \"concurrent\": %d, \"ticks_in_flight\": %d, "
                 "\"each_result_same_alone_and_in_mix\": %s, "
                 "\"bitwise_same_alone_and_in_mix\": %s, \"batched\": %s, "
                 "\"per_op\": %s, \"alone\": %s}\n",
                 logN, slots, cc->GetNumQ(), cc->GetBootstrapDepth(), rate, requests, boot_share, user.calls, flex ? "FLEXIBLEAUTO" : "FIXEDMANUAL",
-                concurrent ? "true" : "false", ticks_in_flight, same ? "true" : "false", bitwise ? "true" : "false",
+                concurrent, ticks_in_flight, same ? "true" : "false", bitwise ? "true" : "false",
                 result_json(res[1]).c_str(), result_json(res[2]).c_str(), result_json(res[3]).c_str());
   } catch (const std::exception& ex) {
     std::fprintf(stderr, "error: %s\n", ex.what());

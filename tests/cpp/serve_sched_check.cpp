@@ -6,10 +6,12 @@
 // first word of its output register.
 //
 // Checks, for every mode (batched, per operation, one request at a time), bootstrap placement
-// (in the tick, on the worker) and ticks_in_flight (1, 3, 16): every request completes, with a
-// result written and the same result as in the first session; latencies are ordered; bootstrap
-// callbacks run on the worker thread exactly when asked; the failure of a bootstrap surfaces as an
-// error. Prints one JSON line per session and "ok" at the end; exits non-zero on a failed check.
+// (in the tick, on the worker, on the yielding worker) and ticks_in_flight (1, 3, 16): every
+// request completes, with a result written and the same result as in the first session; latencies
+// are ordered; bootstrap callbacks run on the worker thread exactly when asked; on the yielding
+// worker each engine call of a bootstrap passes the call gate, which returns with the bootstrap's
+// stream drained and the loop's stream idle or its wait cap spent (and no gate is set elsewhere);
+// the failure of a bootstrap surfaces as an error. Prints one JSON line per session and "ok" at the end; exits non-zero on a failed check.
 #include <atomic>
 #include <chrono>
 #include <cstdio>
@@ -116,7 +118,27 @@ hipError_t hipStreamSynchronize(hipStream_t s) {
   advance(s, g_issued[s]);
   return hipSuccess;
 }
+thread_local int g_last_query = 0;  // this thread's last stream query: 1 idle, 2 busy
+hipError_t hipStreamQuery(hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_done[s] < g_issued[s] && g_rng() % 3 == 0) advance(s, std::min(g_issued[s], g_done[s] + 1 + g_rng() % 3));
+  g_last_query = g_done[s] == g_issued[s] ? 1 : 2;
+  return g_done[s] == g_issued[s] ? hipSuccess : hipErrorNotReady;
 }
+}
+
+namespace {
+bool stream_idle(hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return g_done[s] == g_issued[s];
+}
+// the bootstrap's stand-in context: its own stream (a distinct handle), with work pending
+hipStream_t g_boot_stream = reinterpret_cast<hipStream_t>(0x5eed);
+void boot_issue() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  ++g_issued[g_boot_stream];
+}
+}  // namespace
 
 // host stand-ins of the library: the rescaling steps (plaintext and ciphertext products) add one to
 // the first word of their output register, so a result's first word counts the rescaled products
@@ -130,6 +152,10 @@ Context::Context(int logN, int L1_, int K_, int dnum_, const std::vector<uint64_
 }
 Context::~Context() {}
 Pool::~Pool() {}
+CallGate& thread_call_gate() {
+  thread_local CallGate g;
+  return g;
+}
 Profiler::~Profiler() {}
 UploadRing::~UploadRing() {}
 void* Pool::get(size_t b) { return std::calloc(1, b); }
@@ -163,14 +189,32 @@ size_t Batch::items() const { return p_->outs.size(); }
 namespace {
 struct BootUser {
   std::thread::id main_id;
-  std::atomic<int> calls{0}, on_worker{0};
+  std::atomic<int> calls{0}, on_worker{0}, gated{0}, gate_bad{0};
   int fail_at = -1;  // the call that reports failure (-1: none)
+  Context* boot = nullptr;  // the bootstrap's own context (stand-in)
+  hipStream_t loop = nullptr;
 };
+constexpr int kBootEngineCalls = 4;
 int boot_cb(void* user, const uint64_t*, int level, uint64_t* out, int* out_level) {
   auto* u = static_cast<BootUser*>(user);
   const int k = u->calls++;
   if (std::this_thread::get_id() != u->main_id) ++u->on_worker;
-  std::this_thread::sleep_for(std::chrono::microseconds(300));  // a long step
+  // the bootstrap's engine calls: each runs the thread's call gate first (as the C ABI does), then
+  // issues work on the bootstrap's stream
+  for (int i = 0; i < kBootEngineCalls; ++i) {
+    const CallGate& g = thread_call_gate();
+    if (g.fn) {
+      const auto t = std::chrono::steady_clock::now();
+      g_last_query = 0;
+      g.fn(g.user, *u->boot);
+      const double waited = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+      ++u->gated;
+      // the gate returns once the loop's stream was seen idle (its last query) or the cap is spent
+      if (!stream_idle(g_boot_stream) || (g_last_query != 1 && waited < kYieldCapMs)) ++u->gate_bad;
+    }
+    boot_issue();
+    std::this_thread::sleep_for(std::chrono::microseconds(75));  // a long step
+  }
   if (k == u->fail_at) return 1;
   out[0] = 1000 + (uint64_t)level;
   *out_level = level + 4;
@@ -188,13 +232,17 @@ int main() {
     rk[i] = &keys[i];
     rg[i] = 5;
   }
+  Context boot(8, 16, 2, 3, {});
+  boot.stream = g_boot_stream;
   const size_t cap = 2ull * c.L1 * c.N;
   std::vector<uint64_t> words_ref;
   for (int mode = 0; mode < 3; ++mode)
-    for (int conc = 0; conc < 2; ++conc)
+    for (int conc = 0; conc < 3; ++conc)
       for (int tif : {1, 3, 16}) {
         BootUser u;
         u.main_id = std::this_thread::get_id();
+        u.boot = &boot;
+        u.loop = c.stream;
         std::vector<std::vector<uint64_t>> outs(requests, std::vector<uint64_t>(cap));
         std::vector<uint64_t*> optr(requests);
         for (int i = 0; i < requests; ++i) optr[i] = outs[i].data();
@@ -208,7 +256,7 @@ int main() {
         cfg.bootstrap_share = 0.25;
         cfg.bootstrap = boot_cb;
         cfg.bootstrap_user = &u;
-        cfg.bootstrap_concurrent = conc != 0;
+        cfg.bootstrap_concurrent = conc;
         cfg.ticks_in_flight = tif;
         cfg.seed = 7;
         cfg.request_outputs = optr.data();
@@ -223,13 +271,16 @@ int main() {
         }
         std::printf("{\"mode\": %d, \"concurrent\": %d, \"ticks_in_flight\": %d, \"completed\": %d, \"ticks\": %d, \"bootstraps\": %d, "
                     "\"inference\": %d, \"polynomial\": %d, \"max_active\": %d, \"callbacks\": %d, \"on_worker\": %d, \"p50_ms\": %.3f, "
-                    "\"p99_ms\": %.3f, \"p99_other_ms\": %.3f}\n",
+                    "\"p99_ms\": %.3f, \"p99_other_ms\": %.3f, \"gated\": %d}\n",
                     mode, conc, tif, r.completed, r.ticks, r.bootstraps, r.inference, r.polynomial, r.max_active, u.calls.load(),
-                    u.on_worker.load(), r.p50_ms, r.p99_ms, r.p99_other_ms);
+                    u.on_worker.load(), r.p50_ms, r.p99_ms, r.p99_other_ms, u.gated.load());
         CHECK(r.completed == requests, "completed %d", r.completed);
         CHECK(r.inference + r.polynomial + r.bootstraps == requests, "kinds do not add up");
         CHECK(r.bootstraps > 0 && u.calls == r.bootstraps, "callbacks %d for %d bootstraps", u.calls.load(), r.bootstraps);
         CHECK(u.on_worker == (conc ? r.bootstraps : 0), "callbacks on the worker: %d", u.on_worker.load());
+        CHECK(u.gated == (conc == 2 ? kBootEngineCalls * r.bootstraps : 0), "gated engine calls: %d", u.gated.load());
+        CHECK(u.gate_bad == 0, "the gate returned with work in flight %d times", u.gate_bad.load());
+        CHECK(!thread_call_gate().fn, "a call gate is set on the loop's thread");
         CHECK(0 <= r.p50_ms && r.p50_ms <= r.p99_ms && r.p99_ms <= r.max_ms, "latency order");
         if (mode == 2) CHECK(r.max_active == 1, "one request at a time");
         // every request's result written once, with the step count of its program
@@ -240,9 +291,11 @@ int main() {
         CHECK(w0 == words_ref, "results differ from the first session (mode %d concurrent %d ticks_in_flight %d)", mode, conc, tif);
       }
   // a failing bootstrap surfaces as an error, in the tick and on the worker
-  for (int conc = 0; conc < 2; ++conc) {
+  for (int conc = 0; conc < 3; ++conc) {
     BootUser u;
     u.main_id = std::this_thread::get_id();
+    u.boot = &boot;
+    u.loop = c.stream;
     u.fail_at = 1;
     ServeConfig cfg;
     cfg.rate_per_s = 4000;
@@ -251,7 +304,7 @@ int main() {
     cfg.bootstrap_share = 0.25;
     cfg.bootstrap = boot_cb;
     cfg.bootstrap_user = &u;
-    cfg.bootstrap_concurrent = conc != 0;
+    cfg.bootstrap_concurrent = conc;
     cfg.seed = 7;
     bool threw = false;
     try {

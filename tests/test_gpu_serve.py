@@ -80,6 +80,8 @@ def test_serving_rejects_bad_config(sctx, fh):
         sctx.serve(rk, rg, relin, 100.0, 4, inference_share=0.9, bootstrap_share=0.2)
     with pytest.raises(fh.FHError):
         sctx.serve(rk, rg, relin, 100.0, 4, ticks_in_flight=65)
+    with pytest.raises(fh.FHError):
+        sctx.serve(rk, rg, relin, 100.0, 4, bootstrap_concurrent=3)
     for k in rk + [relin]:
         k.free()
 
@@ -94,12 +96,13 @@ from conftest import PKG
 SCHECK = os.path.join(PKG, "_build", "serve_check")
 
 
-@pytest.mark.parametrize("share,concurrent", [(0.3, 0), (1.0, 0), (0.3, 1), (1.0, 1)])
+@pytest.mark.parametrize("share,concurrent", [(0.3, 0), (1.0, 0), (0.3, 1), (1.0, 1), (0.3, 2)])
 def test_serving_with_bootstrap_requests(share, concurrent):
     """A session whose arrivals include bootstrap requests, served by the facade's EvalBootstrap
     (N=2^12, 8 slots, level budget {1,1}, 16 towers): batched and per-operation execution give the
     same checksum, every bootstrap request went through the callback once per mode. concurrent:
-    the bootstraps run on the loop's worker thread through a second context with the same keys."""
+    the bootstraps run on the loop's worker thread through a second context with the same keys
+    (2: yielding, one bootstrap operation at a time between the loop's ticks)."""
     if not os.path.exists(SCHECK):
         pytest.fail("serve_check is not built (run __graft_entry__.build())")
     args = ["12", "8", "1", "1", "15", "50", "2000", "16", str(share), "5", str(0.5 * (1 - share)), "16", "0", str(concurrent)]
@@ -119,17 +122,19 @@ def test_serving_with_bootstrap_requests(share, concurrent):
     assert b["checksum"] == p["checksum"], "fused batches differ from per-operation execution"
     assert res["each_result_same_alone_and_in_mix"]
     assert res["bitwise_same_alone_and_in_mix"], "a request's result differs between the mix and the request alone"
-    assert res["concurrent"] == bool(concurrent)
+    assert res["concurrent"] == concurrent
 
 
-@pytest.mark.parametrize("technique,concurrent", [(0, 0), (1, 0), (0, 1)], ids=["fixedmanual", "flexibleauto", "fixedmanual-concurrent"])
+@pytest.mark.parametrize("technique,concurrent", [(0, 0), (1, 0), (0, 1), (0, 2)],
+                         ids=["fixedmanual", "flexibleauto", "fixedmanual-concurrent", "fixedmanual-yielding"])
 def test_serving_workload_n65536_with_bootstraps(technique, concurrent):
     """The serving workload's own shape: N=2^16, 25 towers, real keys (sparse ternary h=32), all
     32768 slots, bootstrap level budget {3,3} with 59-bit scaling; arrivals at 50/s, 70% inference,
     25% polynomial and 5% bootstrap requests (the bootstrap requests under the context's scaling
     technique). Every request's result is bit for bit the same in the batched mix, in
     per-operation execution and alone; concurrent: with the bootstraps on the loop's worker thread
-    (a second context, same keys) while the other requests keep ticking."""
+    (a second context, same keys) while the other requests keep ticking (yielding: one bootstrap
+    operation at a time, between ticks)."""
     if not os.path.exists(SCHECK):
         pytest.fail("serve_check is not built (run __graft_entry__.build())")
     args = ["16", "32768", "3", "3", "24", "59", "50", "40", "0.05", "2026", "0.70", "32", str(technique), str(concurrent)]

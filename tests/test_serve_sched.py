@@ -1,6 +1,7 @@
 """The serving loop's scheduler on the CPU (tests/cpp/serve_sched_check.cpp): fh_serve.cpp built with
 the host compiler against host stand-ins of the HIP runtime calls and library operations it makes,
-run over every mode, bootstrap placement (in the tick, on the worker thread) and ticks_in_flight.
+run over every mode, bootstrap placement (in the tick, on the worker thread, on the yielding
+worker thread) and ticks_in_flight.
 Every request completes with the same result as in the first session, latencies are ordered,
 bootstrap callbacks run where asked, and a failing bootstrap surfaces as an error."""
 import os
@@ -27,4 +28,4 @@ def test_serving_scheduler_on_cpu(tmp_path):
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     lines = r.stdout.strip().splitlines()
     assert lines[-1] == "ok"
-    assert len(lines) == 1 + 3 * 2 * 3  # one line per (mode, placement, ticks_in_flight) session
+    assert len(lines) == 1 + 3 * 3 * 3  # one line per (mode, placement, ticks_in_flight) session
