@@ -68,6 +68,10 @@ def parse():
 def algorithmic_bytes(family, N, lq, ext, beta, B, groups):
     """HBM bytes one launch of a kernel family must move at this shape (8-byte words)."""
     limb = 8 * N
+    if family == "ntt_rows_ksk":
+        # the fused row stages + key inner product: reads each digit's target limbs after the
+        # column pass, the own limbs, the keys (once per group), c0; writes both extended outputs
+        return limb * (B * (beta * ext - lq) + B * lq + groups * 2 * beta * ext + B * lq + 2 * B * ext)
     if family.startswith("ntt_"):
         return None  # depends on the launch; handled by the caller
     if family == "ksk_batch":
@@ -131,7 +135,8 @@ def main():
         name, value = kv.split("=", 1)
         ctx.tune_set(name, int(value))
     tunables = {name: ctx.tune_get(name) for name in ("ntt_split", "ntt_fwd_cols_threads", "ntt_fwd_rows_threads", "ntt_inv_rows_threads",
-                                                      "ntt_inv_cols_threads", "ntt_chunk_mb", "ntt_mix", "rot_streams", "ntt_streams")}
+                                                      "ntt_inv_cols_threads", "ntt_chunk_mb", "ntt_mix", "rot_streams", "ntt_streams",
+                                                      "rot_fused")}
     N, B, lvl = ctx.N, a.batch, L1 - 1
     lq, ext, beta = lvl + 1, lvl + 1 + K, ctx.num_digits(lvl)
     ct_words = 2 * lq * N
@@ -184,7 +189,7 @@ def main():
         return [dev_keys[i] for i in r], [gal[i] for i in r], len(set(r.tolist()))
 
     # ---- bit-exactness: batched == per-op on the same inputs ----
-    bit_exact = None
+    bit_exact = bit_exact_fused = None
     if not a.rotation_only:
         ks, gs, _ = choose(0)
         ctx.rotate_batch(cts, B, lvl, ks, gs, out_b)
@@ -193,7 +198,17 @@ def main():
             ctx_out = _view(fh, ctx, out_p, b * ct_words, (2 * lq, N))
             ctx.eval_rotate(ctx_ct, lvl, ks[b], gs[b], ctx_out)
         ctx.sync()
-        bit_exact = bool(np.array_equal(out_b.download(), out_p.download()))   # all B ciphertexts, every limb
+        ref = out_b.download()
+        bit_exact = bool(np.array_equal(ref, out_p.download()))   # all B ciphertexts, every limb
+        # the same rotations with the targets' row stages fused into the key inner product
+        ctx.tune_set("rot_fused", 1)
+        try:
+            ctx.rotate_batch(cts, B, lvl, ks, gs, out_p)
+            ctx.sync()
+        finally:
+            ctx.tune_set("rot_fused", 0)
+        bit_exact_fused = bool(np.array_equal(ref, out_p.download()))
+        del ref
 
     # ---- timed: batched rotations ----
     for w in range(a.warmup):
@@ -211,11 +226,37 @@ def main():
     ctx.sync()
     t_batched = time.perf_counter() - t0
     fams = ["ntt_inv_rows", "ntt_inv_cols", "ntt_fwd_cols", "ntt_fwd_rows", "modup_conv", "ksk_batch",
-            "moddown_conv", "ntt_fwd_rows_moddown"]
+            "moddown_conv", "ntt_fwd_rows_moddown", "ntt_rows_ksk"]
     prof = {f: ctx.profile_read(f) for f in fams}
     ctx.profile(False)
     ms_per_step = 1000.0 * t_batched / a.steps
     rot_per_s = B * a.steps / t_batched
+
+    # ---- the same timed step with the key inner product fused into the targets' row stages ----
+    fused = None
+    if not a.rotation_only:
+        ctx.tune_set("rot_fused", 1)
+        try:
+            ks, gs, _ = choose(1)
+            ctx.rotate_batch(cts, B, lvl, ks, gs, out_b)
+            ctx.sync()
+            ctx.profile(True)
+            ctx.profile_reset()
+            t0 = time.perf_counter()
+            for s in range(a.steps):
+                ks, gs, _ = choose(1000 + s)
+                ctx.rotate_batch(cts, B, lvl, ks, gs, out_b)
+            ctx.sync()
+            t_fused = time.perf_counter() - t0
+            fused_ms = ctx.profile_read("ntt_rows_ksk")[1] / a.steps
+            ctx.profile(False)
+        finally:
+            ctx.tune_set("rot_fused", 0)
+        fused = {"ms_per_step": round(1000.0 * t_fused / a.steps, 3), "rotations_per_s": round(B * a.steps / t_fused, 1),
+                 "fused_kernel_ms_per_step": round(fused_ms, 3),
+                 "separate_row_pass_and_ksk_ms_per_step": round((prof["ntt_fwd_rows"][1] + prof["ksk_batch"][1]) / a.steps, 3),
+                 "bit_exact_vs_separate": bit_exact_fused,
+                 "note": "tunable rot_fused (off by default): the ModUp targets' row stages and the key inner product in one kernel"}
 
     per_op_rot_s = ntt_gbs = t_ntt = mult_rescale_s = mult_rescale_batched_s = None
     pcie = None
@@ -314,7 +355,7 @@ def main():
     avg_ms = tot / max(launches, 1)
     groups_avg = groups_total / a.steps
     pass_gbs = None
-    if dom.startswith("ntt_"):
+    if dom.startswith("ntt_") and dom != "ntt_rows_ksk":
         # Algorithmic bytes of a transform: 16N per limb-transform (read 8N + write 8N), of which
         # each of the two passes completes half: 8N per limb per launch. The ModDown rows launch
         # also reads x_i for its fused epilogue (8N per limb; its write is the transform's write).
@@ -382,6 +423,7 @@ def main():
         "per_op_rotations_per_s": round(per_op_rot_s, 1) if per_op_rot_s else None,
         "batched_over_per_op": round(rot_per_s / per_op_rot_s, 3) if per_op_rot_s else None,
         "bit_exact_batched_vs_per_op": bit_exact,
+        "fused_rows_ksk": fused,
         "pcie_inclusive": pcie,
         "ntt_gbps": round(ntt_gbs, 1) if ntt_gbs else None,
         "ntt_roofline_frac": round(ntt_gbs / HBM_PEAK_GBS, 4) if ntt_gbs else None,
@@ -745,10 +787,12 @@ BOOT_SERVE_REQUESTS = 200
 
 
 def serving_bootstrap_run(a):
-    """A serving session whose arrivals include bootstrap requests (tests/cpp/serve_check.cpp):
-    the bootstrap context above (N=2^16, 25 towers, full-slot bootstrapping, h=32), the mix of 70%
-    inference, 25% polynomial and 5% bootstrap requests at 50 requests/s, batched and per-operation;
-    the same checksum from both."""
+    """Serving sessions whose arrivals include bootstrap requests (tests/cpp/serve_check.cpp): the
+    bootstrap context above under FLEXIBLEAUTO (N=2^16, 25 towers, full-slot bootstrapping, h=32),
+    the mix of 70% inference, 25% polynomial and 5% bootstrap requests at 50 requests/s with the
+    bootstraps in the tick, on the loop's worker thread and on its yielding worker thread, and the
+    yielding placement again at 150 requests/s; batched and per-operation, the same checksum and
+    every result bitwise the same alone and in the mix."""
     import subprocess
     exe = os.path.join(ROOT, "fideslib-fork_amd", "_build", "serve_check")
     if not os.path.exists(exe):
@@ -756,19 +800,21 @@ def serving_bootstrap_run(a):
     keep = ("p50_ms", "p99_ms", "p50_other_ms", "p99_other_ms", "mean_ms", "requests_per_s", "mean_batch_ops", "bootstraps", "inference",
             "polynomial")
     out = {"rate_per_s": 50.0, "requests": BOOT_SERVE_REQUESTS, "mix": "70% inference, 25% polynomial, 5% bootstrap",
-           "p99_other": "p99 over the requests other than bootstraps"}
+           "technique": "FLEXIBLEAUTO", "p99_other": "p99 over the requests other than bootstraps"}
     # serial: each bootstrap runs in the tick that reaches it; concurrent: on the loop's worker
     # thread through a second context (same keys, its own stream) while the other requests tick;
-    # yielding: the worker issues one bootstrap operation at a time, in the gaps between ticks
-    for name, conc in (("serial_bootstrap", "0"), ("concurrent_bootstrap", "1"), ("yielding_bootstrap", "2")):
-        args = ["16", "32768", "3", "3", "24", "59", "50", str(BOOT_SERVE_REQUESTS), "0.05", str(a.seed), "0.70", "32", "0", conc]
+    # yielding: the worker issues one bootstrap operation at a time, in the gaps between ticks.
+    # The last session repeats the yielding placement at 150 requests/s.
+    for name, conc, rate in (("serial_bootstrap", "0", 50), ("concurrent_bootstrap", "1", 50), ("yielding_bootstrap", "2", 50),
+                             ("yielding_bootstrap_150", "2", 150)):
+        args = ["16", "32768", "3", "3", "24", "59", str(rate), str(BOOT_SERVE_REQUESTS), "0.05", str(a.seed), "0.70", "32", "1", conc]
         r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=600)
         if r.returncode != 0:
             out[name] = {"error": (r.stdout + r.stderr)[-500:]}
             continue
         res = json.loads(r.stdout.strip().splitlines()[-1])
         out["towers"] = res["L1"]
-        out[name] = {"batched": {k: res["batched"][k] for k in keep}, "per_op": {k: res["per_op"][k] for k in keep},
+        out[name] = {"rate_per_s": float(rate), "batched": {k: res["batched"][k] for k in keep}, "per_op": {k: res["per_op"][k] for k in keep},
                      "same_checksum": res["batched"]["checksum"] == res["per_op"]["checksum"],
                      "bitwise_same_alone_and_in_mix": res["bitwise_same_alone_and_in_mix"]}
     return out

@@ -184,6 +184,9 @@ void ntt_inv(const Context& c, const LimbSet& s, hipStream_t st);
 void ntt_fwd(const Context& c, const LimbSet& in, const LimbSet& out, hipStream_t st);  // out-of-place
 void ntt_inv(const Context& c, const LimbSet& in, const LimbSet& out, hipStream_t st);
 void ntt_fwd_modup(const Context& c, const ModUpAddr& a, uint32_t n, hipStream_t st);
+// the column pass alone over the ModUp targets: their row stages then run in the fused key inner
+// product (rows_ksk, fh_rows_ksk.hip)
+void ntt_fwd_modup_cols(const Context& c, const ModUpAddr& a, uint32_t n, hipStream_t st);
 
 // NTT launch-shape autotuner: times forward + inverse transforms of `polys` synthetic
 // polynomials over every prime of the context under candidate radix splits, per-pass workgroup

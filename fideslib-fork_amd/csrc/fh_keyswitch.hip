@@ -528,7 +528,8 @@ inline dim3 grid_n(uint32_t N, uint32_t y, uint32_t per_thread = 1) {
 
 void modup(Context& c, const uint64_t* x, int l, uint64_t* digits, hipStream_t st) { modup_batch(c, x, l, 1, 0, digits, st); }
 
-void modup_batch(Context& c, const uint64_t* x, int l, int npolys, uint64_t x_stride, uint64_t* digits, hipStream_t st, bool copy_own) {
+void modup_batch(Context& c, const uint64_t* x, int l, int npolys, uint64_t x_stride, uint64_t* digits, hipStream_t st, bool copy_own,
+                 bool rows) {
   const uint32_t lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);
   if (c.K < 1) throw Error(-1, "key switching needs at least one special prime");
   if ((uint32_t)c.alpha > (uint32_t)kMaxAlpha) throw Error(-3, "digit size above kernel limit");
@@ -552,7 +553,8 @@ void modup_batch(Context& c, const uint64_t* x, int l, int npolys, uint64_t x_st
   }
   FH_HIP(hipGetLastError());
   ModUpAddr a{digits, ext, lq, (uint32_t)c.alpha, (uint32_t)c.L1, beta, (uint32_t)npolys, (uint64_t)beta * ext * c.N};
-  ntt_fwd_modup(c, a, a.count(), st);
+  if (rows) ntt_fwd_modup(c, a, a.count(), st);
+  else ntt_fwd_modup_cols(c, a, a.count(), st);
 }
 
 void ksk_inner(Context& c, const uint64_t* digits, int l, const uint64_t* key, uint64_t* out0, uint64_t* out1,
@@ -811,12 +813,26 @@ void rotate_batch(Context& c, int B, const uint64_t* cts, uint64_t ct_stride, in
   if (beta > (uint32_t)kMaxDnum) throw Error(-3, "dnum above kernel limit");
   if (!ct_stride) ct_stride = 2ull * lq * c.N;
   const uint64_t out_ct = do_moddown ? 2ull * lq * c.N : 2ull * ext * c.N;
+  const bool fused = rows_ksk_ok(c);
   auto part = [&](int b0, int nb, hipStream_t s) {
     const uint64_t* xs = cts + (size_t)b0 * ct_stride;
     DBuf dig(c.pool, (size_t)nb * beta * ext * c.N);
-    modup_batch(c, xs + (size_t)lq * c.N, l, nb, ct_stride, dig.p, s, false);
-    rotate_core(c, nb, dig.p, (uint64_t)beta * ext * c.N, xs, ct_stride, l, keys + b0, galois + b0, add_c0, do_moddown,
-                out + (size_t)b0 * out_ct, s);
+    modup_batch(c, xs + (size_t)lq * c.N, l, nb, ct_stride, dig.p, s, false, !fused);
+    if (!fused) {
+      rotate_core(c, nb, dig.p, (uint64_t)beta * ext * c.N, xs, ct_stride, l, keys + b0, galois + b0, add_c0, do_moddown,
+                  out + (size_t)b0 * out_ct, s);
+      return;
+    }
+    // the targets' row stages fused with the key inner product (the digits stay after the column pass)
+    DBuf ext_buf;
+    uint64_t* eo = out + (size_t)b0 * out_ct;
+    if (do_moddown) {
+      ext_buf = DBuf(c.pool, (size_t)nb * 2 * ext * c.N);
+      eo = ext_buf.p;
+    }
+    rows_ksk(c, nb, dig.p, (uint64_t)beta * ext * c.N, xs, ct_stride, add_c0, xs + (size_t)lq * c.N, ct_stride, l, keys + b0,
+             galois + b0, eo, 2ull * ext * c.N, s);
+    if (do_moddown) moddown(c, eo, l, out + (size_t)b0 * out_ct, 2 * nb, 0, 0, s);
   };
   if (c.tune.rot_streams > 1 && B >= 8 && B <= kMaxBatch) {
     // two halves on two streams: the second waits for everything issued before, the caller's

@@ -11,7 +11,16 @@ void modup(Context& c, const uint64_t* x, int l, uint64_t* digits, hipStream_t s
 // npolys inputs x_stride words apart (0 = packed) -> npolys packed digit blocks. copy_own false
 // leaves each digit's own limbs unwritten: the key inner product then reads them from x (own).
 void modup_batch(Context& c, const uint64_t* x, int l, int npolys, uint64_t x_stride, uint64_t* digits, hipStream_t st,
-                 bool copy_own = true);
+                 bool copy_own = true, bool rows = true);
+// rows false above leaves the ModUp targets after the column pass of their forward NTT; rows_ksk
+// then runs their row stages fused with the key inner product of nb rotations (item b: digits at
+// digits + b dig_stride, own limbs at own + b own_stride, c0 at c0s + b c0_stride folded as P c0
+// when add_c0, key keys[b], automorphism galois[b]) and writes item b's extended result (2 x
+// (l+1+K) limbs) at out + b out_stride. rows_ksk_ok: the context's split and tunables allow it.
+bool rows_ksk_ok(const Context& c);
+void rows_ksk(Context& c, int nb, const uint64_t* digits, uint64_t dig_stride, const uint64_t* c0s, uint64_t c0_stride, bool add_c0,
+              const uint64_t* own, uint64_t own_stride, int l, const uint64_t* const* keys, const uint32_t* galois, uint64_t* out,
+              uint64_t out_stride, hipStream_t st);
 // B rotations sharing nothing but the level; ciphertexts with equal (key, galois) share the key stream
 void rotate_batch(Context& c, int B, const uint64_t* cts, uint64_t ct_stride, int l, const uint64_t* const* keys,
                   const uint32_t* galois, bool add_c0, bool do_moddown, uint64_t* out, hipStream_t st);

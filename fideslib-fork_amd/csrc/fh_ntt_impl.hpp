@@ -274,15 +274,16 @@ struct RowTile {
 // transform of x_c tau_rho^c (c = position in the row): one coalesced twist multiply per element,
 // then stages whose twiddles are the first S2 entries of the table, the same for every row, read
 // from LDS. Input: internal form, output: canonical residues handed to the epilogue.
+//
+// fwd_rows_core: the stages of one row (at `row`, its first word at limb position rbase) by the T
+// threads of row slot rr of the tile. With LS > 4 the canonical outputs end in the LDS tile (word
+// rsw(c0 + T k) of the slot holds position c0 + T k) after a barrier, where the row pass's store
+// phase and the fused key inner product read them; with LS <= 4 (no LDS exchange) they go to the
+// epilogue directly.
 template <int LS, class Ld = PlainLd, class A, class TwT, class Epi>
-__device__ __forceinline__ void fwd_rows_body(const A& ar, const TwT* sw, const TwT* swp, const TwT* ts, const TwT* tsp, uint64_t* p,
-                                              uint64_t* tile, int R, const Epi& epi, uint32_t bx) {
+__device__ __forceinline__ void fwd_rows_core(const A& ar, const TwT* sw, const TwT* swp, const TwT* ts, const TwT* tsp, const uint64_t* row,
+                                              uint32_t rbase, uint64_t* tile, int rr, int c0, const Epi& epi) {
   constexpr int S = 1 << LS, T = S / 16, PADS = RowTile<LS>::PAD;
-  const int tid = threadIdx.x;
-  const int rr = tid / T, c0 = tid % T;
-  const uint32_t rho = bx * R + rr;
-  uint64_t* row = p + (size_t)rho * S;
-  const uint32_t rbase = rho * S;
   typename A::T x[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) x[k] = ar.from_mid(Ld::ld(row + c0 + T * k));
@@ -298,8 +299,8 @@ __device__ __forceinline__ void fwd_rows_body(const A& ar, const TwT* sw, const 
       ar.fwd(x[k], x[k + h], ar.tw(sw, swp, idx));
     }
   }
-  uint64_t* t = tile + rr * PADS;
   if constexpr (LS > 4) {
+    uint64_t* t = tile + rr * PADS;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       ar.recenter(x[k]);
@@ -322,11 +323,26 @@ __device__ __forceinline__ void fwd_rows_body(const A& ar, const TwT* sw, const 
 #pragma unroll
     for (int k = 0; k < 16; ++k) t[rsw(16 * c0 + k)] = ar.canon(x[k]);
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 16; ++k) epi.store(row, rbase, c0 + T * k, t[rsw(c0 + T * k)]);
   } else {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) epi.store(row, rbase, c0 + T * k, ar.canon(x[k]));
+    for (int k = 0; k < 16; ++k) epi.store(const_cast<uint64_t*>(row), rbase, c0 + T * k, ar.canon(x[k]));
+  }
+}
+
+template <int LS, class Ld = PlainLd, class A, class TwT, class Epi>
+__device__ __forceinline__ void fwd_rows_body(const A& ar, const TwT* sw, const TwT* swp, const TwT* ts, const TwT* tsp, uint64_t* p,
+                                              uint64_t* tile, int R, const Epi& epi, uint32_t bx) {
+  constexpr int S = 1 << LS, T = S / 16, PADS = RowTile<LS>::PAD;
+  const int tid = threadIdx.x;
+  const int rr = tid / T, c0 = tid % T;
+  const uint32_t rho = bx * R + rr;
+  uint64_t* row = p + (size_t)rho * S;
+  const uint32_t rbase = rho * S;
+  fwd_rows_core<LS, Ld>(ar, sw, swp, ts, tsp, row, rbase, tile, rr, c0, epi);
+  if constexpr (LS > 4) {
+    const uint64_t* t = tile + rr * PADS;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) epi.store(row, rbase, c0 + T * k, t[rsw(c0 + T * k)]);
   }
 }
 

@@ -3,7 +3,8 @@
 * fh_rotate_batch at N=2^16, 25 limbs (level 24), dnum 3, K 9: 20 ciphertexts rotated by steps
   drawn (seeded) from the 16 keyed steps of the batched rotation workload; every limb of every
   output compared with the oracle's EvalRotate (reference: CudaEngine::evalRotateMany /
-  Ciphertext::rotate_many, api/engine/cuda/CudaEngine.cpp:656-702, src/CKKS/Ciphertext.cpp:1519-1599).
+  Ciphertext::rotate_many, api/engine/cuda/CudaEngine.cpp:656-702, src/CKKS/Ciphertext.cpp:1519-1599),
+  with the key inner product separate and fused into the targets' row stages (NTT_KSK_DOT).
 * the mixed-level batch (fh_batch) at N=2^16 with levels drawn uniformly from [1, 24]: 40%
   EvalRotate, 40% EvalMult + relinearization + Rescale, 20% EvalAdd, run eagerly and replayed from
   the captured HIP graph with fresh inputs (plain conversions, not ReducedNoise).
@@ -27,8 +28,12 @@ def _ct(o, seed, l, ncomp=2):
     return np.concatenate([o.random_poly(seed * 7 + c, list(range(l + 1))) for c in range(ncomp)])
 
 
-def test_rotate_batch_headline_shape(gctx2, octx2):
+@pytest.mark.parametrize("fused", [0, 1])
+def test_rotate_batch_headline_shape(fused, gctx2, octx2):
+    """fused 1: the ModUp targets' row stages run fused with the key inner product (tunable
+    rot_fused), the same words expected."""
     g, o = gctx2, octx2
+    g.tune_set("rot_fused", fused)
     l, B = 24, 20
     rng = np.random.default_rng(2026)
     pick = [STEPS16[i] for i in rng.integers(0, len(STEPS16), size=B)]
@@ -40,8 +45,11 @@ def test_rotate_batch_headline_shape(gctx2, octx2):
     d = g.to_device(np.concatenate(cts))
     rows = 2 * (l + 1)
     out = g.array((B * rows, o.N))
-    g.rotate_batch(d, B, l, [dk[r] for r in pick], [o.rot_index(r) for r in pick], out)
-    g.sync()
+    try:
+        g.rotate_batch(d, B, l, [dk[r] for r in pick], [o.rot_index(r) for r in pick], out)
+        g.sync()
+    finally:
+        g.tune_set("rot_fused", 0)
     got = out.download().reshape(B, rows, o.N)
     for b in range(B):
         exp = o.eval_rotate(cts[b], l, hkeys[pick[b]], o.rot_index(pick[b]))

@@ -271,12 +271,14 @@ def cfg1_rot_keys(octx1, cfg1_material):
     return steps, ks
 
 
-@pytest.mark.parametrize("mix", [0, 1])
-def test_rotate_batch_matches_per_op(mix, gctx1, octx1, cfg1_material, cfg1_rot_keys):
+@pytest.mark.parametrize("mix,fused", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_rotate_batch_matches_per_op(mix, fused, gctx1, octx1, cfg1_material, cfg1_rot_keys):
     """fh_rotate_batch over 7 ciphertexts with repeated keys == per-ciphertext oracle rotations
-    (prime-major NTT job order, and with 60-bit-prime jobs spread among the others)."""
+    (prime-major NTT job order, and with 60-bit-prime jobs spread among the others; with the
+    targets' row stages fused into the key inner product, and separate)."""
     g, o, m = gctx1, octx1, cfg1_material
     g.tune_set("ntt_mix", mix)
+    g.tune_set("rot_fused", fused)
     steps, ks = cfg1_rot_keys
     l = 2
     ids = list(range(l + 1))
@@ -291,6 +293,7 @@ def test_rotate_batch_matches_per_op(mix, gctx1, octx1, cfg1_material, cfg1_rot_
         g.sync()
     finally:
         g.tune_set("ntt_mix", 0)
+        g.tune_set("rot_fused", 0)
     got = out.download().reshape(B, 2 * (l + 1), o.N)
     for b in range(B):
         exp = o.eval_rotate(cts[b], l, ks[pick[b]], o.rot_index(pick[b]))
