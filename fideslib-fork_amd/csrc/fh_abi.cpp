@@ -755,6 +755,15 @@ int fh_batch_mult(fh_batch* b, const uint64_t* a, const uint64_t* bb, int level,
   });
 }
 
+int fh_batch_mult_strided(fh_batch* b, const uint64_t* a, int towers_a, const uint64_t* bb, int towers_b, int level, const fh_key* relin_key,
+                          int rescale, uint64_t* out) {
+  return guard([&] {
+    auto& h = B(b);
+    auto& c = C(h.ctx);
+    h.b.add_mult(a, bb, level, key_of(c, relin_key), rescale != 0, out, towers_a, towers_b);
+  });
+}
+
 int fh_batch_add(fh_batch* b, const uint64_t* a, const uint64_t* bb, int level, int ncomp, uint64_t* out) {
   return guard([&] { B(b).b.add_add(a, bb, level, ncomp, out); });
 }

@@ -452,6 +452,7 @@ struct BatchPlan {
     const uint64_t* key;
     bool rescale;
     uint64_t* out;
+    uint32_t ta, tb;  // limbs per component of a and b (their first l+1 limbs are read)
   };
   struct Add {
     const uint64_t *a, *b;
@@ -576,12 +577,18 @@ void Batch::add_rotate(const uint64_t* ct, int l, const uint64_t* key, uint32_t 
   p_->rots.push_back({ct, l, key, galois, out});
 }
 
-void Batch::add_mult(const uint64_t* a, const uint64_t* b, int l, const uint64_t* key, bool rescale, uint64_t* out) {
+void Batch::add_mult(const uint64_t* a, const uint64_t* b, int l, const uint64_t* key, bool rescale, uint64_t* out, int towers_a,
+                     int towers_b) {
   if (p_->final) throw Error(-1, "batch already finalized");
   check_level(c_, l);
   if (!a || !b || !key || !out) throw Error(-1, "null pointer in batch multiplication");
   if (rescale && l < 1) throw Error(-1, "cannot rescale a level-0 ciphertext");
-  p_->mults.push_back({a, b, l, key, rescale, out});
+  const int lq = l + 1;
+  if (towers_a == 0) towers_a = lq;
+  if (towers_b == 0) towers_b = lq;
+  if (towers_a < lq || towers_b < lq || towers_a > c_.L1 || towers_b > c_.L1)
+    throw Error(-1, "batch multiplication: operand towers must lie in [level + 1, number of Q primes]");
+  p_->mults.push_back({a, b, l, key, rescale, out, (uint32_t)towers_a, (uint32_t)towers_b});
 }
 
 void Batch::add_add(const uint64_t* a, const uint64_t* b, int l, int ncomp, uint64_t* out) {
@@ -713,7 +720,7 @@ void Batch::finalize() {
     const size_t lq = m.l + 1;
     uint64_t* t = A + ms[i].tensor;
     for (size_t j = 0; j < lq; ++j)
-      tj.push_back(TensorJob{m.a + j * N, m.a + (lq + j) * N, m.b + j * N, m.b + (lq + j) * N, t + j * N, t + (lq + j) * N,
+      tj.push_back(TensorJob{m.a + j * N, m.a + (m.ta + j) * N, m.b + j * N, m.b + (m.tb + j) * N, t + j * N, t + (lq + j) * N,
                              t + (2 * lq + j) * N, (uint32_t)j, 0});
   }
   std::vector<AddJob> aj;

@@ -34,8 +34,10 @@ class Batch {
   // ct, out: 2 x (l+1) limbs. key: device key ([dnum][2][L1+K][N], Montgomery form).
   void add_rotate(const uint64_t* ct, int l, const uint64_t* key, uint32_t galois, uint64_t* out);
   // EvalMult of a and b at level l (tensor + relinearization); with rescale the result is
-  // rescaled to level l-1 (out: 2 x l limbs), else out has 2 x (l+1) limbs.
-  void add_mult(const uint64_t* a, const uint64_t* b, int l, const uint64_t* relin_key, bool rescale, uint64_t* out);
+  // rescaled to level l-1 (out: 2 x l limbs), else out has 2 x (l+1) limbs. towers_a / towers_b:
+  // limbs per component of a / b when they sit above level l (0: l+1); their first l+1 limbs are read.
+  void add_mult(const uint64_t* a, const uint64_t* b, int l, const uint64_t* relin_key, bool rescale, uint64_t* out, int towers_a = 0,
+                int towers_b = 0);
   // out = a + b over ncomp components of l+1 limbs
   void add_add(const uint64_t* a, const uint64_t* b, int l, int ncomp, uint64_t* out);
   // ciphertext a (2 x (l+1) limbs) times plaintext pt (l+1 limbs), optionally rescaled

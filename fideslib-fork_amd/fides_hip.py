@@ -84,6 +84,7 @@ def _declare(L):
         "fh_batch_destroy": (I, [P]),
         "fh_batch_rotate": (I, [P, VP, I, P, C.c_uint32, VP]),
         "fh_batch_mult": (I, [P, VP, VP, I, P, I, VP]),
+        "fh_batch_mult_strided": (I, [P, VP, I, VP, I, I, P, I, VP]),
         "fh_batch_add": (I, [P, VP, VP, I, I, VP]),
         "fh_batch_mult_plain": (I, [P, VP, VP, I, I, VP]),
         "fh_batch_add_plain": (I, [P, VP, VP, I, VP]),
@@ -475,6 +476,10 @@ class Batch:
 
     def mult(self, a_addr, b_addr, level, relin_key, rescale, out_addr):
         _check(load().fh_batch_mult(self.h, a_addr, b_addr, level, relin_key.h, int(rescale), out_addr))
+
+    def mult_strided(self, a_addr, towers_a, b_addr, towers_b, level, relin_key, rescale, out_addr):
+        """a / b hold towers_a / towers_b limbs per component (>= level + 1; the first level + 1 are read)"""
+        _check(load().fh_batch_mult_strided(self.h, a_addr, towers_a, b_addr, towers_b, level, relin_key.h, int(rescale), out_addr))
 
     def add(self, a_addr, b_addr, level, ncomp, out_addr):
         _check(load().fh_batch_add(self.h, a_addr, b_addr, level, ncomp, out_addr))

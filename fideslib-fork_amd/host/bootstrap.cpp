@@ -13,11 +13,8 @@
 //      (the monomial X^(3N/2), Bootstrap.cu:577-584)
 //   -> EvalMod on both parts: a Chebyshev series of cos((2 pi K y - pi/2) / 2^r) on [-1, 1]
 //      followed by r double-angle steps c <- 2c^2 - 1, which leaves sin(2 pi K y)
-//      (ApproxModEval.cu:341-873); for a sparse secret r = 0 with degree 127 up to weight 21 and
-//      255 up to weight 42, double angles only beyond (each double angle
-//      quadruples the series' noise, which the final transform then scales by q_0 / Delta: at
-//      N=2^12 and 64 slots r = 3 with degree 63 measured 11.7 bits, r = 0 with degree 127 14.3
-//      bits, one level cheaper)
+//      (ApproxModEval.cu:341-873); for a sparse secret the fewest double angles that bring
+//      2 pi K / 2^r to at most 48, with a series of degree 63 or 127 (EvalBootstrapSetup)
 //   -> recombine with i = X^(N/2) -> SlotsToCoeffs: the butterfly stages of the special FFT as
 //      levelBudget[1] BSGS LinearTransforms.
 // A group of m stages whose smallest butterfly half-length is h has its nonzero diagonals at the
@@ -36,6 +33,7 @@
 #include <cmath>
 #include <complex>
 #include <map>
+#include <set>
 
 #include "fideslib.hpp"
 #include "special_fft.hpp"
@@ -316,6 +314,35 @@ struct BootstrapOps {
     out->slots = cts[0]->slots;
     return out;
   }
+  // rescale_exact(mult(a_i, b_i)) of several independent (paired or plain) pairs as one
+  // mixed-level batch: one launch sequence for all of them, bit for bit the products and rescales
+  // issued one at a time (operands may sit at different levels; the product's is the lower)
+  static std::vector<Ciphertext> mult_rescale_many(CryptoContextImpl& cc, const std::vector<std::pair<Ciphertext, Ciphertext>>& ps) {
+    if (!cc.relin_) throw std::runtime_error("fideslib: EvalMultKeyGen was not called");
+    if (!cc.batch_) cc.check(fh_batch_create(cc.ctx_, &cc.batch_));
+    cc.check(fh_batch_reset(cc.batch_));
+    std::vector<Ciphertext> outs;
+    for (const auto& pr : ps) {
+      const Ciphertext &a = pr.first, &b = pr.second;
+      if (a->ncomp != b->ncomp || (a->ncomp != 2 && a->ncomp != 4)) throw std::runtime_error("fideslib: batched product of mismatched ciphertexts");
+      const uint32_t t = std::min(a->towers, b->towers);
+      if (t < 2) throw std::runtime_error("fideslib: Rescale at the last tower");
+      auto out = cc.newCt(t - 1, a->ncomp);
+      for (uint32_t h = 0; h < a->ncomp / 2; ++h)
+        cc.check(fh_batch_mult_strided(cc.batch_, a->data->ptr + (size_t)h * 2 * a->towers * cc.N_, (int)a->towers,
+                                       b->data->ptr + (size_t)h * 2 * b->towers * cc.N_, (int)b->towers, (int)t - 1, cc.relin_, 1,
+                                       out->data->ptr + (size_t)h * 2 * (t - 1) * cc.N_));
+      const uint32_t deg = a->noiseScaleDeg + b->noiseScaleDeg;
+      out->noiseScaleDeg = deg > 1 ? deg - 1 : 1;
+      const double s = a->scalingFactor * b->scalingFactor;
+      out->scalingFactor = s / (double)cc.moduli_[t - 1];
+      out->slots = a->slots;
+      outs.push_back(out);
+    }
+    cc.check(fh_batch_finalize(cc.batch_));
+    cc.check(fh_batch_run(cc.batch_, 0));
+    return outs;
+  }
   static double prime(const CryptoContextImpl& cc, uint32_t i) { return (double)cc.moduli_[i]; }
   static double nominal(const CryptoContextImpl& cc) { return cc.scale_; }
 
@@ -351,6 +378,43 @@ uint32_t cheb_split_depth(uint32_t d) {
 }
 bool cheb_leaf(uint32_t d) { return d < 8 || (d < 16 && clog2(d) + 1 <= cheb_split_depth(d)); }
 uint32_t cheb_depth(uint32_t d) { return cheb_leaf(d) ? clog2(d) + 1 : cheb_split_depth(d); }
+
+// the T_k a series of degree d reads: its leaves' T_1..T_d and its split powers
+void cheb_powers(uint32_t d, std::set<uint32_t>& need) {
+  if (cheb_leaf(d)) {
+    for (uint32_t k = 1; k <= d; ++k) need.insert(k);
+    return;
+  }
+  uint32_t M = 1;
+  while (2 * M <= d) M *= 2;
+  need.insert(M);
+  if (d > M) cheb_powers(d - M, need);
+  cheb_powers(M - 1, need);
+}
+// the same with every T_k those need (T_2k from T_k, T_2k+1 from T_k and T_k+1)
+std::set<uint32_t> cheb_powers_closed(uint32_t d) {
+  std::set<uint32_t> need, all;
+  cheb_powers(d, need);
+  std::vector<uint32_t> stack(need.begin(), need.end());
+  while (!stack.empty()) {
+    const uint32_t k = stack.back();
+    stack.pop_back();
+    if (k <= 1 || !all.insert(k).second) continue;
+    stack.push_back(k / 2);
+    stack.push_back(k / 2 + k % 2);
+  }
+  return all;
+}
+// ciphertext products of the series' split pieces (a piece of degree d > M multiplies T_M by
+// its upper part's value; exactly M a scalar product)
+uint32_t cheb_tree_products(uint32_t d) {
+  if (cheb_leaf(d)) return 0;
+  uint32_t M = 1;
+  while (2 * M <= d) M *= 2;
+  return (d > M ? 1 + cheb_tree_products(d - M) : 0) + cheb_tree_products(M - 1);
+}
+// all ciphertext products of a series of degree d: its powers T_k (k >= 2) and its pieces'
+uint32_t cheb_products(uint32_t d) { return (uint32_t)cheb_powers_closed(d).size() + cheb_tree_products(d); }
 
 struct ChebEval {
   CryptoContextImpl& cc;
@@ -422,22 +486,45 @@ struct ChebExact {
   Ciphertext mul(const Ciphertext& a, const Ciphertext& b) { return BootstrapOps::mult(cc, a, b); }
   Ciphertext rs(const Ciphertext& a) { return BootstrapOps::rescale_exact(cc, a); }
 
+  // T_k from the product p = T_k/2 T_k/2 or T_k/2 T_k/2+1 (already rescaled)
+  Ciphertext finish(uint32_t k, const Ciphertext& p) {
+    if (k % 2 == 0) return BootstrapOps::icomb_exact(cc, {p}, {2}, -1.0);
+    const Ciphertext& t1 = T.at(1);
+    // T_1 times 1 encoded at p's scale q / s_1: after the rescale by q, exactly p's scale
+    Ciphertext a = rs(BootstrapOps::mult_at(cc, t1, 1.0, p->scalingFactor * BootstrapOps::prime(cc, t1->towers - 1) / t1->scalingFactor));
+    a->scalingFactor = p->scalingFactor;
+    return BootstrapOps::icomb_exact(cc, {p, a}, {2, -1}, 0.0);
+  }
   Ciphertext t(uint32_t k) {
     auto it = T.find(k);
     if (it != T.end()) return it->second;
-    Ciphertext r;
-    if (k % 2 == 0) {
-      r = BootstrapOps::icomb_exact(cc, {rs(mul(t(k / 2), t(k / 2)))}, {2}, -1.0);
-    } else {
-      Ciphertext p = rs(mul(t(k / 2), t(k / 2 + 1)));
-      const Ciphertext& t1 = T.at(1);
-      // T_1 times 1 encoded at p's scale q / s_1: after the rescale by q, exactly p's scale
-      Ciphertext a = rs(BootstrapOps::mult_at(cc, t1, 1.0, p->scalingFactor * BootstrapOps::prime(cc, t1->towers - 1) / t1->scalingFactor));
-      a->scalingFactor = p->scalingFactor;
-      r = BootstrapOps::icomb_exact(cc, {p, a}, {2, -1}, 0.0);
-    }
+    const Ciphertext r = finish(k, rs(mul(t(k / 2), t(k / 2 + k % 2))));
     T[k] = r;
     return r;
+  }
+  // Every T_k the series reads, computed ahead in dependency waves: the products of one wave
+  // (independent of each other) run as one batch. Values and scales are those of the lazy t(k).
+  void precompute(uint32_t d) {
+    const std::set<uint32_t> all = cheb_powers_closed(d);
+    std::map<uint32_t, uint32_t> wave{{1, 0}};
+    uint32_t waves = 0;
+    for (uint32_t k : all) {  // ascending: a T_k's inputs have smaller indices
+      const uint32_t w = 1 + std::max(wave.at(k / 2), wave.at(k / 2 + k % 2));
+      wave[k] = w;
+      waves = std::max(waves, w);
+    }
+    for (uint32_t w = 1; w <= waves; ++w) {
+      std::vector<uint32_t> ks;
+      std::vector<std::pair<Ciphertext, Ciphertext>> ops;
+      for (uint32_t k : all)
+        if (wave.at(k) == w && !T.count(k)) {
+          ks.push_back(k);
+          ops.push_back({T.at(k / 2), T.at(k / 2 + k % 2)});
+        }
+      if (ks.empty()) continue;
+      const std::vector<Ciphertext> ps = BootstrapOps::mult_rescale_many(cc, ops);
+      for (size_t i = 0; i < ks.size(); ++i) T[ks[i]] = finish(ks[i], ps[i]);
+    }
   }
   // the towers the evaluation of a piece of degree d ends at
   uint32_t towers_of(uint32_t d) {
@@ -645,17 +732,30 @@ void CryptoContextImpl::EvalBootstrapSetup(const std::vector<uint32_t>& levelBud
   // SPARSE_TERNARY secrets of weight 192)
   const bool sparse = params_.dist_ == SecretKeyDist::SPARSE_TERNARY;
   st->K = sparse ? std::min(std::ceil(params_.hw_ / 2.0) + 4.0, 28.0) : 64.0;
-  // sparse: a Chebyshev series of sin(a y) of degree d converges to double precision while
-  // a <= 0.63 d (its coefficients fall like the Bessel values J_k(a): ~1e-14 at k = 127 for a = 80),
-  // and each double angle multiplies the series' error by ~4, so: no double angle and degree 127
-  // up to a = 80 (h <= 21), degree 255 up to a = 160 (h <= 42; one more product level, the same
-  // depth as degree 127 plus a double angle), then the fewest double angles that reach a <= 160
+  // sparse: the series approximates cos(a y - pi / 2^(r+1)) with a = 2 pi K / 2^r, which degree
+  // 63 does to ~2^-45 up to a = 32, degree 127 up to a = 80 and degree 255 up to a = 160 (the
+  // Chebyshev coefficients fall like the Bessel values J_k(a)). Among those (r, degree) the lowest
+  // EvalMod depth, then the fewest products (the series' plus one per double angle). With the
+  // series' scales exact a double angle costs well under a bit: at N=2^16, h=32 this picks r = 2
+  // with degree 63, measured 22.2 bits in 25 ms against 23.4 bits in 35 ms for r = 0 with degree
+  // 255 at the same depth
   st->r = 5;
   st->degree = 63;
   if (sparse) {
-    st->r = 0;
-    while (2.0 * M_PI * st->K / std::ldexp(1.0, st->r) > 160.0) ++st->r;
-    st->degree = 2.0 * M_PI * st->K / std::ldexp(1.0, st->r) > 80.0 ? 255 : 127;
+    static const uint32_t kDeg[3] = {63, 127, 255};
+    static const double kArg[3] = {32.0, 80.0, 160.0};
+    uint32_t best_depth = ~0u, best_prod = ~0u;
+    for (int r = 0; r <= 10; ++r)
+      for (int j = 0; j < 3; ++j) {
+        if (2.0 * M_PI * st->K / std::ldexp(1.0, r) > kArg[j]) continue;
+        const uint32_t depth = cheb_depth(kDeg[j]) + (uint32_t)r, prod = cheb_products(kDeg[j]) + (uint32_t)r;
+        if (depth < best_depth || (depth == best_depth && prod < best_prod)) {
+          best_depth = depth;
+          best_prod = prod;
+          st->r = r;
+          st->degree = kDeg[j];
+        }
+      }
   }
   CosArg arg{st->K, st->r};
   st->cheb = EvalChebyshevCoefficients(shifted_cos, &arg, -1.0, 1.0, st->degree);
@@ -792,6 +892,7 @@ Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0, uint32_t numI
   cheb[0] *= 0.5;
   ChebExact ex{*this, {}, {}};
   ex.T[1] = y;
+  ex.precompute((uint32_t)cheb.size() - 1);
   Ciphertext c = ex.eval(cheb, 0.0);
   for (int i = 0; i < st.r; ++i) c = BootstrapOps::icomb_exact(*this, {BootstrapOps::rescale_exact(*this, BootstrapOps::mult(*this, c, c))}, {2}, -1.0);
   // the rest keeps the nominal bookkeeping (the output's relative scale error is the drift of the

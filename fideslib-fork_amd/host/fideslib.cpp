@@ -155,6 +155,7 @@ double CryptoContextImpl::GetModReduceFactor(uint32_t level) const {
 CryptoContextImpl::~CryptoContextImpl() {
   if (!ctx_) return;
   fh_synchronize(ctx_);
+  if (batch_) fh_batch_destroy(batch_);
   if (relin_) fh_key_destroy(ctx_, relin_);
   for (auto& kv : rot_) fh_key_destroy(ctx_, kv.second);
   rot_.clear();

@@ -425,6 +425,7 @@ class CryptoContextImpl {
 
   // bootstrapping state (bootstrap.cpp)
   friend struct BootstrapOps;
+  fh_batch* batch_ = nullptr;  // the bootstrap's batched products (created on first use)
   Ciphertext MultByMonomial(const Ciphertext& ct, uint32_t power);
   Ciphertext MultByInteger(const Ciphertext& ct, uint64_t v);
   PackedDiagonals groupDiagonals(const BootstrapGroup& g, uint32_t n, uint32_t level);

@@ -41,6 +41,44 @@ def cheb_depth(d):
 
 
 # ---- constants (mirrors of the facade's double arithmetic) ----
+def cheb_powers(d, need):
+    """The T_k a series of degree d reads: its leaves' T_1..T_d and its split powers."""
+    if cheb_leaf(d):
+        need.update(range(1, d + 1))
+        return
+    M = 1
+    while 2 * M <= d:
+        M *= 2
+    need.add(M)
+    if d > M:
+        cheb_powers(d - M, need)
+    cheb_powers(M - 1, need)
+
+
+def cheb_products(d):
+    """Ciphertext products of a series of degree d: its powers T_k (k >= 2, with every T_k they
+    need) and its split pieces' products."""
+    need, seen = set(), set()
+    cheb_powers(d, need)
+    stack = list(need)
+    while stack:
+        k = stack.pop()
+        if k <= 1 or k in seen:
+            continue
+        seen.add(k)
+        stack += [k // 2, k // 2 + k % 2]
+
+    def tree(d):
+        if cheb_leaf(d):
+            return 0
+        M = 1
+        while 2 * M <= d:
+            M *= 2
+        return (1 + tree(d - M) if d > M else 0) + tree(M - 1)
+
+    return len(seen) + tree(d)
+
+
 def cheb_coefficients(f, a, b, degree):
     n = degree + 1
     fx = [f(0.5 * (b - a) * math.cos(math.pi * (j + 0.5) / n) + 0.5 * (b + a)) for j in range(n)]
@@ -251,10 +289,17 @@ def constants(N, n, hw, scale, q0, sparse=True, budget=(1, 1), correction=0, top
     K = min(math.ceil(hw / 2.0) + 4.0, 28.0) if sparse else 64.0
     r, degree = 5, 63
     if sparse:
-        r = 0
-        while 2.0 * math.pi * K / math.ldexp(1.0, r) > 160.0:
-            r += 1
-        degree = 255 if 2.0 * math.pi * K / math.ldexp(1.0, r) > 80.0 else 127
+        # (r, degree) with 2 pi K / 2^r within what the degree approximates: the lowest depth,
+        # then the fewest products
+        best = None
+        for rr in range(11):
+            for deg, lim in ((63, 32.0), (127, 80.0), (255, 160.0)):
+                if 2.0 * math.pi * K / math.ldexp(1.0, rr) > lim:
+                    continue
+                key = (cheb_depth(deg) + rr, cheb_products(deg) + rr)
+                if best is None or key < best[0]:
+                    best = (key, rr, deg)
+        r, degree = best[1], best[2]
     cheb = cheb_coefficients(lambda y: math.cos((2.0 * math.pi * K * y - 0.5 * math.pi) / math.ldexp(1.0, r)), -1.0, 1.0, degree)
     M = 2 * N
     fc = scale / (2.0 * gap * q0 * K)

@@ -173,7 +173,7 @@ Batch::Batch(Context& c) : c_(c), p_(new BatchPlan) {}
 Batch::~Batch() = default;
 void Batch::reset() { p_->outs.clear(); }
 void Batch::add_rotate(const uint64_t*, int, const uint64_t*, uint32_t, uint64_t*) { p_->outs.push_back(nullptr); }
-void Batch::add_mult(const uint64_t*, const uint64_t*, int, const uint64_t*, bool, uint64_t* out) { p_->outs.push_back(out); }
+void Batch::add_mult(const uint64_t*, const uint64_t*, int, const uint64_t*, bool, uint64_t* out, int, int) { p_->outs.push_back(out); }
 void Batch::add_add(const uint64_t*, const uint64_t*, int, int, uint64_t*) { p_->outs.push_back(nullptr); }
 void Batch::add_mult_plain(const uint64_t*, const uint64_t*, int, bool, uint64_t* out) { p_->outs.push_back(out); }
 void Batch::add_add_plain(const uint64_t*, const uint64_t*, int, uint64_t*) { p_->outs.push_back(nullptr); }

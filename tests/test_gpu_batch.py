@@ -189,6 +189,45 @@ def test_batch_reset_reuse(env, fh):
     drl.free()
 
 
+def test_batch_mult_operands_above_level(env, fh):
+    """fh_batch_mult_strided: products (with and without Rescale) whose operands sit above the
+    product's level, read in place, equal the oracle's product of the operands' first level + 1
+    limbs; operand towers outside [level + 1, L1] are rejected."""
+    o, g = env
+    relin = _key(o, 310)
+    drl = g.key(relin)
+    b = fh.Batch(g)
+    rng = np.random.default_rng(77)
+    bufs = []
+    for i in range(9):
+        l = int(rng.integers(1, o.L1 - 1))
+        ta, tb = int(rng.integers(l + 1, o.L1 + 1)), int(rng.integers(l + 1, o.L1 + 1))
+        x, y = _rand_ct(o, 9500 + 2 * i, ta - 1), _rand_ct(o, 9501 + 2 * i, tb - 1)
+        rs = bool(i % 2)
+        dx, dy, do = g.to_device(x), g.to_device(y), g.array((2 * (l if rs else l + 1), o.N))
+        b.mult_strided(dx.ptr, ta, dy.ptr, tb, l, drl, rs, do.ptr)
+        bufs.append((l, ta, tb, rs, x, y, dx, dy, do))
+    with pytest.raises(fh.FHError):
+        b.mult_strided(bufs[0][6].ptr, bufs[0][0], bufs[0][7].ptr, bufs[0][2], bufs[0][0], drl, False, bufs[0][8].ptr)  # towers_a = level
+    with pytest.raises(fh.FHError):
+        b.mult_strided(bufs[0][6].ptr, o.L1 + 1, bufs[0][7].ptr, bufs[0][2], bufs[0][0], drl, False, bufs[0][8].ptr)   # above L1
+    b.finalize()
+    b.run(graph=False)
+    g.sync()
+    for l, ta, tb, rs, x, y, dx, dy, do in bufs:
+        lo = l + 1
+        xa = np.concatenate([x[:lo], x[ta:ta + lo]])
+        yb = np.concatenate([y[:lo], y[tb:tb + lo]])
+        exp = o.eval_mult(xa, yb, l, relin)
+        if rs:
+            exp = o.rescale_ct(exp, l)
+        assert np.array_equal(do.download(), exp), (l, ta, tb, rs)
+        for d in (dx, dy, do):
+            d.free()
+    b.free()
+    drl.free()
+
+
 def test_batch_errors(env, fh):
     o, g = env
     b = fh.Batch(g)

@@ -34,17 +34,18 @@ BCHECK = os.path.join(PKG, "_build", "bootstrap_check")
 
 
 # (logN, slots, CoeffsToSlots budget, SlotsToCoeffs budget, multiplicative depth, scaling bits,
-# bits required, secret weight, technique 0 FIXEDMANUAL / 1 FLEXIBLEAUTO). The bounds sit 1.5-2 bits
-# under what the runs measure (19.7, 21.0, 13.9, 26.9, 26.5, 26.6, 23.4 and 23.3 bits): the
+# bits required, secret weight, technique 0 FIXEDMANUAL / 1 FLEXIBLEAUTO). The bounds sit 0.7-2 bits
+# under what the runs measure (19.7, 21.0, 13.9, 26.9, 26.0, 25.7, 22.2 and 22.2 bits): the
 # EvalMod series tracks exact scales, and with the nominal bookkeeping (each rescale taken to divide
-# by the scale itself) the N=2^16 runs measured 12.3 bits. Weight 32 takes a degree-255 series. The
+# by the scale itself) the N=2^16 runs measured 12.3 bits. Weight 32 takes a degree-63 series and two
+# double angles (23.4 bits at N=2^16 with one degree-255 series, 10 ms slower). The
 # N=2^16 runs are the serving workload's bootstrap (all 32768 slots, budget {3,3}, h=32, 59-bit
 # scaling under a 60-bit q_0): the reference states 11.5-15 bits for its GPU path at N=2^16
 # (test/BootstrapPrecisionTests.cu:4-7); their residues are not replayed (the oracle's bootstrap at
 # N=2^16 takes too long), nor those of FLEXIBLEAUTO runs (this bootstrap's own scale adjustment,
 # not a reference sequence).
 RUNS = [(12, 8, 1, 1, 15, 50, 18, 16, 0), (12, 64, 2, 3, 17, 50, 19, 16, 0), (12, 2048, 3, 3, 18, 50, 12, 16, 0),
-        (12, 2048, 3, 3, 19, 59, 25, 16, 0), (12, 2048, 3, 3, 20, 59, 24.5, 32, 0), (12, 2048, 3, 3, 20, 59, 24.5, 32, 1),
+        (12, 2048, 3, 3, 19, 59, 25, 16, 0), (12, 2048, 3, 3, 20, 59, 24, 32, 0), (12, 2048, 3, 3, 20, 59, 24, 32, 1),
         (16, 32768, 3, 3, 24, 59, 21.5, 32, 0), (16, 32768, 3, 3, 24, 59, 21.5, 32, 1)]
 
 
@@ -91,8 +92,8 @@ def test_bootstrap_precision_at_reference_point(tmp_path):
     sparse ternary secret (OpenFHE's SPARSE_TERNARY draws weight 192), 2^15 slots of real values in
     [-1, 1] encrypted at level 21. The reference states 18.6 bits for its OpenFHE path, 11.5-15
     bits for its GPU path and 29-32 bits for the two-iteration (Meta-BTS) refresh there. This
-    bootstrap measures 16.3 bits and Meta-BTS 28.4: one iteration is held to the GPU path's upper
-    bound, Meta-BTS to 27 bits."""
+    bootstrap (degree-63 series, three double angles) measures 15.7 bits and Meta-BTS 27.8: one
+    iteration is held to the GPU path's upper bound, Meta-BTS to 27 bits."""
     if not os.path.exists(BCHECK):
         pytest.fail("bootstrap_check is not built (run __graft_entry__.build())")
     args = [str(tmp_path), "16", "32768", "7", "3", "2", "25", "55", "192", "1", "12", "56", "6", "1", "21"]
@@ -201,7 +202,10 @@ def test_bootstrap_constants_closed_form(boot_run):
         assert np.allclose(product(_groups(out, meta, "stc")), q0 / (2 * math.pi * scale) * UP, rtol=0, atol=1e-9)
     assert 1.0 <= fc / meta["pre"] < 2.0 and math.log2(meta["pre"]).is_integer()
     cheb = np.fromfile(out / "cheb.f64", dtype=np.float64)
-    assert Kr == min(math.ceil(meta["hw"] / 2) + 4, 28) and r == 0 and len(cheb) == (128 if meta["hw"] <= 21 else 256)
+    # weight <= 21: one degree-127 series; weight 32: degree 63 and two double angles (the same
+    # depth as one degree-255 series, 21 instead of 33 products)
+    assert Kr == min(math.ceil(meta["hw"] / 2) + 4, 28)
+    assert (r, len(cheb)) == ((0, 128) if meta["hw"] <= 21 else (2, 64))
     # the drift factor: the top primes the prescale and the CoeffsToSlots groups rescale by, over the scale
     L1 = meta["L1"]
     fix = 1.0
