@@ -862,6 +862,7 @@ static int* tune_slot(fh::Tune& t, const char* name, int* min_value = nullptr) {
   else if (!std::strcmp(name, "ntt_split")) slot = &t.ntt_split, lo = 0;
   else if (!std::strcmp(name, "rot_streams")) slot = &t.rot_streams;
   else if (!std::strcmp(name, "rot_fused")) slot = &t.rot_fused, lo = 0;
+  else if (!std::strcmp(name, "ksk_group_chunk")) slot = &t.ksk_group_chunk;
   else if (!std::strcmp(name, "ntt_streams")) slot = &t.ntt_streams;
   else if (!std::strcmp(name, "ntt_team")) slot = &t.ntt_team, lo = 0;
   else if (!std::strcmp(name, "ntt_team_per_xcd")) slot = &t.ntt_team_per_xcd;
@@ -881,6 +882,7 @@ int fh_tune_set(fh_context* ctx, const char* name, int value) {
     if (!std::strcmp(name, "ntt_mix") && value > 1) throw fh::Error(FH_E_INVALID, "ntt_mix must be 0 or 1");
     if (!std::strcmp(name, "rot_streams") && value > 2) throw fh::Error(FH_E_INVALID, "rot_streams must be 1 or 2");
     if (!std::strcmp(name, "rot_fused") && value > 1) throw fh::Error(FH_E_INVALID, "rot_fused must be 0 or 1");
+    if (!std::strcmp(name, "ksk_group_chunk") && value > 256) throw fh::Error(FH_E_INVALID, "ksk_group_chunk must be 1..256");
     if (!std::strcmp(name, "ntt_streams") && value > 2) throw fh::Error(FH_E_INVALID, "ntt_streams must be 1 or 2");
     if (!std::strcmp(name, "ntt_team") && value > 1) throw fh::Error(FH_E_INVALID, "ntt_team must be 0 or 1");
     if (!std::strcmp(name, "ntt_team_per_xcd") && value > 64) throw fh::Error(FH_E_INVALID, "ntt_team_per_xcd must be 1..64");

@@ -800,9 +800,9 @@ void Batch::finalize() {
     }
   }
   // groups of key-switch items sharing (key, galois), levels descending inside a group. A group is
-  // cut into chunks of at most kGroupChunk items (each chunk reads the key once), so no workgroup
+  // cut into chunks of at most ksk_group_chunk items (each chunk reads the key once), so no workgroup
   // runs far longer than the others, and the chunks are launched largest first.
-  constexpr uint32_t kGroupChunk = 16;
+  const uint32_t kGroupChunk = (uint32_t)std::max(1, c.tune.ksk_group_chunk);
   std::vector<size_t> order(ks.size());
   for (size_t i = 0; i < ks.size(); ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {

@@ -154,6 +154,10 @@ struct Tune {
   // the row stages (240 VGPRs, 2 waves per SIMD), and the work is VALU-bound, so the bytes it
   // saves do not pay; off by default.
   int rot_fused = 0;
+  // key-switch items per workgroup group of a batch's key inner product (items sharing a key are
+  // cut into chunks of this many; each chunk reads the key once). On the 256-operation mixed-level
+  // stream at N=2^16, 4 to 64 measured within 1% of each other.
+  int ksk_group_chunk = 16;
   // 2: the chunks of a chunked NTT (ntt_chunk_mb below the batch) alternate between two streams
   int ntt_streams = 1;
   // 1: NTTs with equal column and row lengths run as one launch of workgroup teams (one HBM round
