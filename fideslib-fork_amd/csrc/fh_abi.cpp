@@ -596,6 +596,45 @@ int fh_linear_transform(fh_context* ctx, const uint64_t* ct, int level, int row_
   });
 }
 
+int fh_linear_transform_flat(fh_context* ctx, const uint64_t* ct, int level, int row_size, int b_step, int stride, int offset,
+                             int slots, const uint64_t* diags, const fh_key* const* baby_keys, const uint32_t* baby_galois,
+                             const fh_key* const* giant_keys, const uint32_t* giant_galois, const fh_key* offset_key,
+                             uint32_t offset_galois, uint64_t* out) {
+  return guard([&] {
+    auto& c = C(ctx);
+    need(ct, "ct"); need(diags, "diags"); need(out, "out");
+    check_level(c, level);
+    if (b_step < 1 || b_step > 16) throw fh::Error(FH_E_INVALID, "b_step must be in [1, 16]");
+    if (row_size < 1) throw fh::Error(FH_E_INVALID, "row_size must be >= 1");
+    if (slots < 1 || slots > (int)c.N / 2) throw fh::Error(FH_E_INVALID, "slots out of range");
+    std::vector<const uint64_t*> bk(b_step, nullptr);
+    std::vector<uint32_t> bg(b_step, 1);
+    for (int i = 0; i < b_step; ++i) {
+      if (((long long)i * stride) % slots == 0) continue;
+      need(baby_keys, "baby_keys");
+      need(baby_galois, "baby_galois");
+      bk[i] = key_of(c, baby_keys[i]);
+      bg[i] = baby_galois[i];
+      check_galois(c, bg[i], "baby step");
+    }
+    const int g_step = (row_size + b_step - 1) / b_step;
+    std::vector<const uint64_t*> gk(g_step > 1 ? g_step - 1 : 0, nullptr);
+    std::vector<uint32_t> gg(gk.size(), 1);
+    for (int j = 1; j < g_step; ++j) {
+      if (((long long)j * b_step * stride) % slots == 0) continue;
+      need(giant_keys, "giant_keys");
+      need(giant_galois, "giant_galois");
+      gk[j - 1] = key_of(c, giant_keys[j - 1]);
+      gg[j - 1] = giant_galois[j - 1];
+      check_galois(c, gg[j - 1], "giant step");
+    }
+    const uint64_t* ok = offset_key ? key_of(c, offset_key) : nullptr;
+    if (ok) check_galois(c, offset_galois, "offset");
+    fh::linear_transform_flat(c, ct, level, row_size, b_step, stride, offset, slots, diags, bk.data(), bg.data(), gk.data(), gg.data(),
+                              ok, offset_galois, out, c.stream);
+  });
+}
+
 }  // extern "C"
 
 namespace {

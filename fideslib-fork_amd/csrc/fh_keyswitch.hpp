@@ -40,6 +40,14 @@ void ksk_inner(Context& c, const uint64_t* digits, int l, const uint64_t* key, u
 void linear_transform(Context& c, const uint64_t* ct, int l, int rowSize, int bStep, int stride, int offset, int slots,
                       const uint64_t* diags, const uint64_t* const* baby_keys, const uint32_t* baby_gal, const uint64_t* giant_key,
                       uint32_t giant_gal, const uint64_t* off_key, uint32_t off_gal, uint64_t* out, hipStream_t st);
+// The same linear map with the giant steps as independent rotations (giant_keys[j-1] /
+// giant_gal[j-1] rotate the j-th partial sum by j*bStep*stride; entries whose multiple is 0 mod
+// slots are unused): one batched key switch for all of them instead of a sequential Horner chain.
+// Its own sequence (the same map, different rounding than linear_transform).
+void linear_transform_flat(Context& c, const uint64_t* ct, int l, int rowSize, int bStep, int stride, int offset, int slots,
+                           const uint64_t* diags, const uint64_t* const* baby_keys, const uint32_t* baby_gal,
+                           const uint64_t* const* giant_keys, const uint32_t* giant_gal, const uint64_t* off_key, uint32_t off_gal,
+                           uint64_t* out, hipStream_t st);
 // Rotation-fold sums at a power-of-two radix (fh_lt.hip); keys / galois one per fold rotation in
 // order; out may alias ct. accumulate_sum: the lazy fold (c0 stays in Q_l ∪ P across the levels and
 // takes one ModDown at the end); accumulate_cascade: the fold from factor start with a ModDown of

@@ -162,20 +162,15 @@ __global__ __launch_bounds__(kT) void k_lt_mac_b(const uint64_t* __restrict__ ro
 
 }  // namespace
 
-void linear_transform(Context& c, const uint64_t* ct, int l, int rowSize, int bStep, int stride, int offset, int slots,
-                      const uint64_t* diags, const uint64_t* const* baby_keys, const uint32_t* baby_gal, const uint64_t* giant_key,
-                      uint32_t giant_gal, const uint64_t* off_key, uint32_t off_gal, uint64_t* out, hipStream_t st) {
-  if (bStep < 1 || bStep > kMaxBaby) throw Error(-1, "bStep must be in [1, 16]");
-  if (rowSize < 1) throw Error(-1, "rowSize must be >= 1");
-  if (slots < 1 || slots > (int)c.N / 2) throw Error(-1, "slots out of range");
-  const uint32_t lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);
-  const size_t N = c.N, E = (size_t)ext * N;
-  const int gStep = (rowSize + bStep - 1) / bStep;
-  const bool giant = ((long long)bStep * stride) % slots != 0;
-  if (gStep > 1 && giant && !giant_key) throw Error(-1, "giant-step key required");
-  if (offset != 0 && !off_key) throw Error(-1, "offset key required");
+namespace {
 
-  // ---- baby steps: identity in P-scaled extended form, the others from one shared ModUp ----
+// Baby steps (the identity in P-scaled extended form, the others from one shared ModUp) and the
+// multiply-accumulate of every giant-step partial sum: R_j, 2 x (l+1+K) limbs each, j < gStep.
+DBuf lt_partials(Context& c, const uint64_t* ct, int l, int rowSize, int bStep, int stride, int slots, const uint64_t* diags,
+                 const uint64_t* const* baby_keys, const uint32_t* baby_gal, hipStream_t st) {
+  const uint32_t lq = l + 1, ext = lq + c.K;
+  const size_t E = (size_t)ext * c.N;
+  const int gStep = (rowSize + bStep - 1) / bStep;
   DBuf fast(c.pool, (size_t)bStep * 2 * E);
   std::vector<int> rot_idx;
   std::vector<const uint64_t*> rk;
@@ -204,8 +199,6 @@ void linear_transform(Context& c, const uint64_t* ct, int l, int rowSize, int bS
         FH_HIP(hipMemcpyAsync(fast.p + (size_t)rot_idx[t] * 2 * E, tmp.p + t * 2 * E, 2 * E * 8, hipMemcpyDeviceToDevice, st));
     }
   }
-
-  // ---- all giant-step partial sums in one multiply-accumulate pass ----
   DBuf R(c.pool, (size_t)gStep * 2 * E);
   MacParams mp{c.N, lq, ext, (uint32_t)c.L1, (uint32_t)rowSize, (uint32_t)bStep, (uint32_t)gStep, c.d_prime};
   {
@@ -222,34 +215,109 @@ void linear_transform(Context& c, const uint64_t* ct, int l, int rowSize, int bS
     }
     FH_HIP(hipGetLastError());
   }
+  return R;
+}
+
+// The last step of both forms on the combined extended sum: the offset rotation (ModDown of c1,
+// the extended c0 folded directly) and ModDown of both, or the ModDown of both.
+void lt_finish(Context& c, uint64_t* acc, int l, int offset, const uint64_t* off_key, uint32_t off_gal, uint64_t* out, hipStream_t st) {
+  const uint32_t lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);
+  const size_t N = c.N, E = (size_t)ext * N;
+  if (offset != 0) {
+    DBuf c1q(c.pool, (size_t)lq * N), dig(c.pool, (size_t)beta * E), t(c.pool, 2 * E);
+    moddown(c, acc + E, l, c1q.p, 1, 0, 0, st);
+    modup(c, c1q.p, l, dig.p, st);
+    ksk_inner(c, dig.p, l, off_key, t.p, t.p + E, acc, off_gal, st, true);
+    moddown(c, t.p, l, out, 2, 0, 0, st);
+  } else {
+    moddown(c, acc, l, out, 2, 0, 0, st);
+  }
+}
+
+}  // namespace
+
+void linear_transform(Context& c, const uint64_t* ct, int l, int rowSize, int bStep, int stride, int offset, int slots,
+                      const uint64_t* diags, const uint64_t* const* baby_keys, const uint32_t* baby_gal, const uint64_t* giant_key,
+                      uint32_t giant_gal, const uint64_t* off_key, uint32_t off_gal, uint64_t* out, hipStream_t st) {
+  if (bStep < 1 || bStep > kMaxBaby) throw Error(-1, "bStep must be in [1, 16]");
+  if (rowSize < 1) throw Error(-1, "rowSize must be >= 1");
+  if (slots < 1 || slots > (int)c.N / 2) throw Error(-1, "slots out of range");
+  const uint32_t lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);
+  const size_t N = c.N, E = (size_t)ext * N;
+  const int gStep = (rowSize + bStep - 1) / bStep;
+  const bool giant = ((long long)bStep * stride) % slots != 0;
+  if (gStep > 1 && giant && !giant_key) throw Error(-1, "giant-step key required");
+  if (offset != 0 && !off_key) throw Error(-1, "offset key required");
+  DBuf R = lt_partials(c, ct, l, rowSize, bStep, stride, slots, diags, baby_keys, baby_gal, st);
 
   // ---- Horner over the giant steps ----
   std::vector<int> epids(ext);
   for (uint32_t p = 0; p < ext; ++p) epids[p] = c.ext_prime(l, p);
   // R_{j-1} += rot_giant(R_j) for j = gStep-1 .. 1: the key switch adds its result into R_{j-1}.
   DBuf c1q(c.pool, (size_t)lq * N), dig(c.pool, (size_t)beta * E);
-  for (int j = gStep - 1; j >= 0; --j) {
+  for (int j = gStep - 1; j > 0; --j) {
     uint64_t* Rj = R.p + (size_t)j * 2 * E;
-    if (j > 0) {
-      uint64_t* Rp = Rj - 2 * E;
-      if (giant) {
-        moddown(c, Rj + E, l, c1q.p, 1, 0, 0, st);
-        modup(c, c1q.p, l, dig.p, st);
-        ksk_inner(c, dig.p, l, giant_key, Rp, Rp + E, Rj, giant_gal, st, true, nullptr, true);
-      } else {
-        LimbSet a = make_set(Rp, ext, epids.data(), 2, E), b = make_set(Rj, ext, epids.data(), 2, E);
-        ew_binary(c, 0, a, b, a, st);
-      }
-    } else if (offset != 0) {
+    uint64_t* Rp = Rj - 2 * E;
+    if (giant) {
       moddown(c, Rj + E, l, c1q.p, 1, 0, 0, st);
       modup(c, c1q.p, l, dig.p, st);
-      DBuf t(c.pool, 2 * E);
-      ksk_inner(c, dig.p, l, off_key, t.p, t.p + E, Rj, off_gal, st, true);
-      moddown(c, t.p, l, out, 2, 0, 0, st);
+      ksk_inner(c, dig.p, l, giant_key, Rp, Rp + E, Rj, giant_gal, st, true, nullptr, true);
     } else {
-      moddown(c, Rj, l, out, 2, 0, 0, st);
+      LimbSet a = make_set(Rp, ext, epids.data(), 2, E), b = make_set(Rj, ext, epids.data(), 2, E);
+      ew_binary(c, 0, a, b, a, st);
     }
   }
+  lt_finish(c, R.p, l, offset, off_key, off_gal, out, st);
+}
+
+void linear_transform_flat(Context& c, const uint64_t* ct, int l, int rowSize, int bStep, int stride, int offset, int slots,
+                           const uint64_t* diags, const uint64_t* const* baby_keys, const uint32_t* baby_gal,
+                           const uint64_t* const* giant_keys, const uint32_t* giant_gal, const uint64_t* off_key, uint32_t off_gal,
+                           uint64_t* out, hipStream_t st) {
+  if (bStep < 1 || bStep > kMaxBaby) throw Error(-1, "bStep must be in [1, 16]");
+  if (rowSize < 1) throw Error(-1, "rowSize must be >= 1");
+  if (slots < 1 || slots > (int)c.N / 2) throw Error(-1, "slots out of range");
+  const uint32_t lq = l + 1, ext = lq + c.K, beta = c.num_digits(l);
+  const size_t N = c.N, E = (size_t)ext * N;
+  const int gStep = (rowSize + bStep - 1) / bStep;
+  if (offset != 0 && !off_key) throw Error(-1, "offset key required");
+  // the partial sums rotated by a nonzero multiple of the giant step, and their keys
+  std::vector<int> rot;
+  std::vector<const uint64_t*> keys;
+  std::vector<uint32_t> gals;
+  for (int j = 1; j < gStep; ++j) {
+    if (((long long)j * bStep * stride) % slots == 0) continue;
+    if (!giant_keys || !giant_keys[j - 1]) throw Error(-1, "giant-step key required");
+    rot.push_back(j);
+    keys.push_back(giant_keys[j - 1]);
+    gals.push_back(giant_gal[j - 1]);
+  }
+  DBuf R = lt_partials(c, ct, l, rowSize, bStep, stride, slots, diags, baby_keys, baby_gal, st);
+  std::vector<int> epids(ext);
+  for (uint32_t p = 0; p < ext; ++p) epids[p] = c.ext_prime(l, p);
+  const LimbSet acc = make_set(R.p, ext, epids.data(), 2, E);
+  if (!rot.empty()) {
+    // the rotated partial sums side by side: one ModDown of their c1, one ModUp, one key inner
+    // product with each item's own key (the extended c0 folded directly, as a Horner step does)
+    const int nr = (int)rot.size();
+    bool contiguous = true;
+    for (int t = 1; t < nr; ++t) contiguous &= rot[t] == rot[0] + t;
+    DBuf gathered, c1q(c.pool, (size_t)nr * lq * N), dig(c.pool, (size_t)nr * beta * E), T(c.pool, (size_t)nr * 2 * E);
+    const uint64_t* src = R.p + (size_t)rot[0] * 2 * E;
+    if (!contiguous) {
+      gathered = DBuf(c.pool, (size_t)nr * 2 * E);
+      for (int t = 0; t < nr; ++t)
+        FH_HIP(hipMemcpyAsync(gathered.p + (size_t)t * 2 * E, R.p + (size_t)rot[t] * 2 * E, 2 * E * 8, hipMemcpyDeviceToDevice, st));
+      src = gathered.p;
+    }
+    moddown(c, src + E, l, c1q.p, nr, 2 * E, (uint64_t)lq * N, st);
+    modup_batch(c, c1q.p, l, nr, (uint64_t)lq * N, dig.p, st, true);
+    ksk_items(c, nr, 1, dig.p, (uint64_t)beta * E, src, 2 * E, true, true, l, keys.data(), gals.data(), T.p, 2 * E, st);
+    for (int t = 0; t < nr; ++t) ew_binary(c, 0, acc, make_set(T.p + (size_t)t * 2 * E, ext, epids.data(), 2, E), acc, st);
+  }
+  for (int j = 1; j < gStep; ++j)
+    if (((long long)j * bStep * stride) % slots == 0) ew_binary(c, 0, acc, make_set(R.p + (size_t)j * 2 * E, ext, epids.data(), 2, E), acc, st);
+  lt_finish(c, R.p, l, offset, off_key, off_gal, out, st);
 }
 
 // The same transform for count ciphertexts sharing the diagonal set and keys: the baby-step

@@ -80,6 +80,8 @@ def _declare(L):
         "fh_fill_uniform": (I, [P, VP, i32p, I, I, S, U]),
         "fh_linear_transform": (I, [P, VP, I, I, I, I, I, I, VP, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), P, C.c_uint32, P,
                                     C.c_uint32, VP]),
+        "fh_linear_transform_flat": (I, [P, VP, I, I, I, I, I, I, VP, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32),
+                                         C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), P, C.c_uint32, VP]),
         "fh_batch_create": (I, [P, PP]),
         "fh_batch_destroy": (I, [P]),
         "fh_batch_rotate": (I, [P, VP, I, P, C.c_uint32, VP]),
@@ -360,6 +362,19 @@ class Context:
         _check(load().fh_linear_transform(self.h, ct.ptr, level, row_size, b_step, stride, offset, slots, diags.ptr, bk, bg,
                                           giant_key.h if giant_key is not None else None, giant_galois,
                                           offset_key.h if offset_key is not None else None, offset_galois, out.ptr))
+
+    def linear_transform_flat(self, ct, level, row_size, b_step, stride, diags, baby_keys, baby_galois, out, giant_keys=(),
+                              giant_galois=(), offset_key=None, offset_galois=1, offset=0, slots=None):
+        """The transform with giant steps as independent rotations: giant_keys[j-1] rotates the
+        j-th partial sum by j*b_step*stride (None where that is 0 mod slots)."""
+        slots = slots or self.N // 2
+        bk = (C.c_void_p * b_step)(*[(k.h if k is not None else None) for k in baby_keys])
+        bg = (C.c_uint32 * b_step)(*[int(g) for g in baby_galois])
+        ng = max(len(giant_keys), 1)
+        gk = (C.c_void_p * ng)(*[(k.h if k is not None else None) for k in giant_keys])
+        gg = (C.c_uint32 * ng)(*([int(g) for g in giant_galois] or [1]))
+        _check(load().fh_linear_transform_flat(self.h, ct.ptr, level, row_size, b_step, stride, offset, slots, diags.ptr, bk, bg, gk, gg,
+                                               offset_key.h if offset_key is not None else None, offset_galois, out.ptr))
 
     @staticmethod
     def accumulate_indices(radix, stride, size, start=1):

@@ -792,7 +792,8 @@ void CryptoContextImpl::EvalBootstrapKeyGen(const PrivateKey& sk, uint32_t slots
   for (const auto& g : all) {
     const uint32_t b = bstep_for(g.rowSize);
     for (uint32_t i = 1; i < b; ++i) steps.push_back((int32_t)((i * g.stride) % n));  // baby steps
-    if (g.rowSize > b) steps.push_back((int32_t)((b * g.stride) % n));                // giant step
+    for (uint32_t j = 1; j * b < g.rowSize; ++j)  // giant steps: each multiple rotated by its own key
+      if ((uint64_t)j * b * g.stride % n) steps.push_back((int32_t)((uint64_t)j * b * g.stride % n));
     if (g.offset) steps.push_back((int32_t)g.offset);
   }
   EvalRotateKeyGen(sk, steps);
@@ -879,7 +880,7 @@ Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0, uint32_t numI
   // 3. CoeffsToSlots: slots (I + p / q_0) / (2K), real and imaginary coefficient halves
   Ciphertext t = raised;
   for (size_t g = 0; g < st.cts.size(); ++g)
-    t = Rescale(LinearTransform(t, (int)bstep_for(st.cts[g].rowSize), st.ctsDiags[g], (int)st.cts[g].stride, (int)st.cts[g].offset));
+    t = Rescale(LinearTransformFlat(t, (int)bstep_for(st.cts[g].rowSize), st.ctsDiags[g], (int)st.cts[g].stride, (int)st.cts[g].offset));
   // 4. real and imaginary parts: t + conj(t), -i (t - conj(t))
   Ciphertext cj = EvalConjugate(t);
   Ciphertext re = EvalAdd(t, cj);
@@ -909,7 +910,7 @@ Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0, uint32_t numI
   }
   Ciphertext out = u;
   for (size_t g = 0; g < st.stc.size(); ++g)
-    out = Rescale(LinearTransform(out, (int)bstep_for(st.stc[g].rowSize), sit->second[g], (int)st.stc[g].stride, (int)st.stc[g].offset));
+    out = Rescale(LinearTransformFlat(out, (int)bstep_for(st.stc[g].rowSize), sit->second[g], (int)st.stc[g].stride, (int)st.stc[g].offset));
   if (st.correction) out = MultByInteger(out, 1ull << st.correction);
   out->slots = ct0->slots;
   return out;

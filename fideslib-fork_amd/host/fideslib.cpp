@@ -1294,4 +1294,42 @@ Ciphertext CryptoContextImpl::LinearTransform(const Ciphertext& ct0, int bStep, 
   return out;
 }
 
+Ciphertext CryptoContextImpl::LinearTransformFlat(const Ciphertext& ct0, int bStep, const PackedDiagonals& diags, int stride, int offset) {
+  require_standard(*ct0, "LinearTransform");
+  Ciphertext ct = ct0->noiseScaleDeg == 2 ? modReduce(ct0) : ct0;
+  const uint32_t t = ct->towers;
+  if (!diags.data || diags.towers != t) throw std::runtime_error("fideslib: diagonals must be extended plaintexts at the ciphertext's level");
+  const int rowSize = (int)diags.rowSize;
+  const int slots = (int)ct->slots;
+  std::vector<const fh_key*> bk(bStep, nullptr);
+  std::vector<uint32_t> bg(bStep, 1);
+  for (int i = 1; i < bStep; ++i) {
+    if (normalize_index(i * stride, slots) == 0) continue;
+    bg[i] = galoisFor(i * stride);
+    bk[i] = rotKey(bg[i]);
+  }
+  const int gStep = (rowSize + bStep - 1) / bStep;
+  std::vector<const fh_key*> gk(std::max(gStep - 1, 1), nullptr);
+  std::vector<uint32_t> gg(gk.size(), 1);
+  for (int j = 1; j < gStep; ++j) {
+    const int32_t r = normalize_index(j * bStep * stride, slots);  // the rotation on `slots` replicated slots
+    if (r == 0) continue;
+    gg[j - 1] = galoisFor(r);
+    gk[j - 1] = rotKey(gg[j - 1]);
+  }
+  const fh_key* ok = nullptr;
+  uint32_t og = 1;
+  if (offset != 0) {
+    og = galoisFor(offset);
+    ok = rotKey(og);
+  }
+  auto out = newCt(t, 2);
+  check(fh_linear_transform_flat(ctx_, ct->data->ptr, (int)t - 1, rowSize, bStep, stride, offset, slots, diags.data->ptr, bk.data(),
+                                 bg.data(), gk.data(), gg.data(), ok, og, out->data->ptr));
+  out->noiseScaleDeg = ct->noiseScaleDeg + diags.noiseScaleDeg;
+  out->scalingFactor = ct->scalingFactor * diags.scalingFactor;
+  out->slots = ct->slots;
+  return out;
+}
+
 }  // namespace fideslib

@@ -384,6 +384,9 @@ class CryptoContextImpl {
   std::vector<int64_t> gaussian();
   fh_key* switchKey(const DeviceBuffer& s_old, const DeviceBuffer& s_new);
   const fh_key* rotKey(uint32_t galois) const;
+  // the transform with every giant-step multiple rotated by its own key (fh_linear_transform_flat):
+  // the bootstrap's CtS / StC transforms, whose keys EvalBootstrapKeyGen generates
+  Ciphertext LinearTransformFlat(const Ciphertext& ct, int bStep, const PackedDiagonals& diagonals, int stride, int offset);
   Ciphertext accumulate(const Ciphertext& ct, int slots, int stride, int start);
   uint32_t galoisFor(int32_t index) const;
   void automorphismKey(const PrivateKey& sk, uint32_t galois);

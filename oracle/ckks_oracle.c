@@ -896,6 +896,82 @@ void orc_linear_transform(const orc_ctx* c, const u64* ct, int l, int rowSize, i
   free(fast); free(acc); free(R); free(c1q);
 }
 
+/* The same transform with the giant steps as independent rotations: R_j (j >= 1) is rotated by
+ * j*bStep*stride with its own key (ModDown of c1, the extended c0 folded directly, as a Horner
+ * step does) and the results are added to R_0, which then takes the offset rotation or the
+ * ModDown of both components. This build's own sequence for its bootstrap's transforms (it has
+ * no reference counterpart; the reference's transform is the Horner form above): the same
+ * linear map, different rounding. giant_keys: gStep-1 keys back to back (entry j-1 for
+ * j*bStep*stride), giant_k their automorphism indices; an R_j whose multiple is 0 mod slots is
+ * added unrotated (its entry unused). */
+void orc_linear_transform_flat(const orc_ctx* c, const u64* ct, int l, int rowSize, int bStep, int stride, int offset, int slots,
+                               const u64* diags, const u64* baby_keys, const uint32_t* baby_k, const u64* giant_keys,
+                               const uint32_t* giant_k, const u64* off_key, uint32_t off_k, u64* out) {
+  u64 N = c->N;
+  int ext = l + 1 + c->K;
+  size_t E = (size_t)ext * N, Q = (size_t)(l + 1) * N;
+  size_t key_words = (size_t)c->dnum * 2 * c->np * N;
+  int gStep = (rowSize + bStep - 1) / bStep;
+  int giant = ((long long)bStep * stride) % slots != 0;
+  u64* fast = (u64*)calloc((size_t)bStep * 2 * E, 8);
+  for (int i = 0; i < bStep; ++i) {
+    u64* f = fast + (size_t)i * 2 * E;
+    if (((long long)i * stride) % slots == 0) {
+      for (int comp = 0; comp < 2; ++comp)
+        for (int p = 0; p <= l; ++p) {
+          u64 qp = c->q[p];
+          for (u64 j = 0; j < N; ++j) f[comp * E + (size_t)p * N + j] = orc_mulmod(ct[comp * Q + (size_t)p * N + j], c->Pmod[p], qp);
+        }
+    } else {
+      rotate_core(c, ct, 1, ct + Q, l, baby_keys + (size_t)i * key_words, baby_k[i], f);
+    }
+  }
+  u64* acc = (u64*)calloc(2 * E, 8);
+  u64* R = (u64*)malloc(2 * E * 8);
+  u64* T = (u64*)malloc(2 * E * 8);
+  u64* c1q = (u64*)malloc(Q * 8);
+  for (int j = 0; j < gStep; ++j) {
+    memset(R, 0, 2 * E * 8);
+    for (int i = 0; i < bStep && bStep * j + i < rowSize; ++i) {
+      const u64* d = diags + (size_t)(bStep * j + i) * E;
+      const u64* f = fast + (size_t)i * 2 * E;
+      for (int comp = 0; comp < 2; ++comp)
+#pragma omp parallel for
+        for (int pos = 0; pos < ext; ++pos) {
+          u64 q = c->q[ext_prime(c, l, pos)];
+          for (u64 n = 0; n < N; ++n) {
+            size_t x = (size_t)pos * N + n;
+            R[comp * E + x] = modadd(R[comp * E + x], orc_mulmod(f[comp * E + x], d[x], q), q);
+          }
+        }
+    }
+    const u64* add = R;
+    if (j > 0 && giant && ((long long)j * bStep * stride) % slots != 0) {
+      orc_moddown(c, R + E, l, c1q);
+      rotate_core(c, R, 2, c1q, l, giant_keys + (size_t)(j - 1) * key_words, giant_k[j - 1], T);
+      add = T;
+    }
+    for (int comp = 0; comp < 2; ++comp)
+      for (int pos = 0; pos < ext; ++pos) {
+        u64 q = c->q[ext_prime(c, l, pos)];
+        for (u64 n = 0; n < N; ++n) {
+          size_t x = comp * E + (size_t)pos * N + n;
+          acc[x] = modadd(acc[x], add[x], q);
+        }
+      }
+  }
+  if (offset != 0) {
+    orc_moddown(c, acc + E, l, c1q);
+    rotate_core(c, acc, 2, c1q, l, off_key, off_k, T);
+    orc_moddown(c, T, l, out);
+    orc_moddown(c, T + E, l, out + Q);
+  } else {
+    orc_moddown(c, acc, l, out);
+    orc_moddown(c, acc + E, l, out + Q);
+  }
+  free(fast); free(acc); free(R); free(T); free(c1q);
+}
+
 /* Rotation-fold sum from a start factor (the cascade: CKKS::Accumulate(ct, bStep, stride, size,
    startFactor), src/CKKS/AccumulateBroadcast.cu:42-84, behind CudaEngine::accumulateSumInPlace(ct,
    slots, stride, start), api/engine/cuda/CudaEngine.cpp:780-783): for s = start, start*radix, ... <

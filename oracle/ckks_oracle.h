@@ -142,6 +142,11 @@ void orc_accumulate_lazy(const orc_ctx* c, const uint64_t* ct, int l, int radix,
 void orc_linear_transform(const orc_ctx* c, const uint64_t* ct, int l, int rowSize, int bStep, int stride, int offset, int slots,
                           const uint64_t* diags, const uint64_t* baby_keys, const uint32_t* baby_k, const uint64_t* giant_key,
                           uint32_t giant_k, const uint64_t* off_key, uint32_t off_k, uint64_t* out);
+/* the same linear map with independent giant-step rotations (giant_keys: gStep-1 keys, entry j-1
+ * for j*bStep*stride): this build's own sequence for its bootstrap's transforms */
+void orc_linear_transform_flat(const orc_ctx* c, const uint64_t* ct, int l, int rowSize, int bStep, int stride, int offset, int slots,
+                               const uint64_t* diags, const uint64_t* baby_keys, const uint32_t* baby_k, const uint64_t* giant_keys,
+                               const uint32_t* giant_k, const uint64_t* off_key, uint32_t off_k, uint64_t* out);
 
 /* ---- key generation / encryption (seeded, mt19937_64) ---- */
 /* ternary secret, EVAL over Q∪P (L1+K limbs); hamming weight h (0 = uniform ternary) */

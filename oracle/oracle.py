@@ -81,6 +81,9 @@ def lib():
             "orc_eval_rotate_ext": (V, [P, u64p, I, u64p, U32, I, u64p]),
             "orc_linear_transform": (V, [P, u64p, I, I, I, I, I, I, u64p, u64p, np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"),
                                          u64p, U32, u64p, U32, u64p]),
+            "orc_linear_transform_flat": (V, [P, u64p, I, I, I, I, I, I, u64p, u64p,
+                                              np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"), u64p,
+                                              np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"), u64p, U32, u64p]),
             "orc_accumulate_sum": (V, [P, u64p, I, I, I, I, I, u64p, np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"),
                                        u64p]),
             "orc_accumulate_lazy": (V, [P, u64p, I, I, I, u64p, np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"),
@@ -351,6 +354,26 @@ class Ctx:
         out = self.zeros(l + 1, 2)
         lib().orc_linear_transform(self.h, np.ascontiguousarray(ct), l, rowSize, bStep, stride, offset, slots,
                                    np.ascontiguousarray(diags), bk, bkk, gk, giant_k, ok, off_k, out)
+        return out
+
+    def linear_transform_flat(self, ct, l, diags, bStep, stride, baby_keys, baby_k, giant_keys=(), giant_ks=(), off_key=None, off_k=1,
+                              offset=0, slots=None):
+        """The same linear map with the giant steps as independent rotations: giant_keys[j-1] /
+        giant_ks[j-1] rotate R_j by j*bStep*stride (this build's own sequence for its bootstrap)."""
+        slots = slots or self.N // 2
+        rowSize = diags.shape[0] // (l + 1 + self.K)
+        gStep = (rowSize + bStep - 1) // bStep
+        zero = np.zeros(self.key_shape(), dtype=np.uint64)
+        bk = np.ascontiguousarray(np.concatenate([zero if k is None else k for k in baby_keys]))
+        bkk = np.ascontiguousarray(np.array([int(x) for x in baby_k], dtype=np.uint32))
+        gkeys = list(giant_keys) + [None] * (gStep - 1 - len(giant_keys))
+        gk = np.ascontiguousarray(np.concatenate([zero if k is None else k for k in gkeys] or [zero]))
+        gks = list(giant_ks) + [1] * (gStep - 1 - len(giant_ks))
+        gkk = np.ascontiguousarray(np.array([int(x) for x in gks] or [1], dtype=np.uint32))
+        ok = zero if off_key is None else np.ascontiguousarray(off_key)
+        out = self.zeros(l + 1, 2)
+        lib().orc_linear_transform_flat(self.h, np.ascontiguousarray(ct), l, rowSize, bStep, stride, offset, slots,
+                                        np.ascontiguousarray(diags), bk, bkk, gk, gkk, ok, off_k, out)
         return out
 
     @staticmethod

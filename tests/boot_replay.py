@@ -269,14 +269,13 @@ def trace_indices(n, N):
 
 def rotation_steps(k, N):
     """slot rotations a bootstrap with constants k uses: the partial trace, then every group's
-    baby steps, giant step and offset"""
+    baby steps, giant-step multiples (each rotated by its own key) and offset"""
     n = k["slots"]
     steps = list(trace_indices(n, N))
     for rowSize, stride, offset, _ in k["cts"] + k["stc"]:
         b = bstep_for(rowSize)
         steps += [(i * stride) % n for i in range(1, b)]
-        if rowSize > b:
-            steps.append((b * stride) % n)
+        steps += [(j * b * stride) % n for j in range(1, (rowSize + b - 1) // b) if (j * b * stride) % n]
         if offset:
             steps.append(offset)
     return [r for r in steps if r % (N // 2)]
@@ -415,15 +414,19 @@ class Ops:
                 continue
             g = o.rot_index(i * stride)
             baby_keys[i], baby_k[i] = self.rot[g], g
-        gk, gg = None, 1
-        if rowSize > bStep and (bStep * stride) % n:
-            gg = o.rot_index(bStep * stride)
-            gk = self.rot[gg]
+        gStep = (rowSize + bStep - 1) // bStep
+        giant_keys, giant_k = [None] * (gStep - 1), [1] * (gStep - 1)
+        for j in range(1, gStep):
+            r = (j * bStep * stride) % n
+            if r:
+                giant_k[j - 1] = o.rot_index(r)
+                giant_keys[j - 1] = self.rot[giant_k[j - 1]]
         ok, og = None, 1
         if offset:
             og = o.rot_index(offset)
             ok = self.rot[og]
-        return (o.linear_transform(x, l, diags, bStep, stride, baby_keys, baby_k, gk, gg, ok, og, offset=offset, slots=n), t, d + 1)
+        return (o.linear_transform_flat(x, l, diags, bStep, stride, baby_keys, baby_k, giant_keys, giant_k, ok, og, offset=offset, slots=n),
+                t, d + 1)
 
 
 def chebyshev_series(ops, y, coeffs, a=-1.0, b=1.0):

@@ -456,6 +456,32 @@ def test_linear_transform_decrypts(rowSize, bStep, offset):
     assert np.max(np.abs(dec - want)) < 1e-6
 
 
+@pytest.mark.parametrize("rowSize,bStep,offset", [(12, 4, 0), (10, 4, 3), (16, 4, 0)])
+def test_linear_transform_flat_decrypts(rowSize, bStep, offset):
+    """The flat form (each giant-step partial sum rotated by its own multiple of the giant step)
+    decrypts to the same sum_k rot(z, k + offset) D_k as the Horner form, with different residues;
+    a giant key for the wrong multiple (every R_j rotated by the one giant step, as if it were
+    Horner's) misses the map by far."""
+    c = small_ctx(10, 4, 2, 2)
+    n, l, Dl = c.N // 2, 3, 2.0 ** 50
+    s = c.gen_secret(17)
+    z = _slots(n, 18)
+    ct = c.encrypt(c.encode(z, Dl, l), l, s, 19)
+    diags, bk, bkk, gk, ok, want = _lt_case(c, s, z, l, rowSize, bStep, 1, offset)
+    gStep = (rowSize + bStep - 1) // bStep
+    gks = [c.rot_index(j * bStep) for j in range(1, gStep)]
+    gkeys = [c.gen_rot_key(s, k, 800 + j) for j, k in enumerate(gks)]
+    okk = c.rot_index(offset) if offset else 1
+    flat = c.linear_transform_flat(ct, l, diags, bStep, 1, bk, bkk, gkeys, gks, ok, okk, offset)
+    horner = c.linear_transform(ct, l, diags, bStep, 1, bk, bkk, gk, c.rot_index(bStep), ok, okk, offset)
+    dec = c.decode(c.decrypt(flat, l, s), l, n, Dl * 2.0 ** 30)
+    assert np.max(np.abs(dec - want)) < 1e-6
+    assert not np.array_equal(flat, horner)
+    wrong = c.linear_transform_flat(ct, l, diags, bStep, 1, bk, bkk, [gk] * (gStep - 1), [c.rot_index(bStep)] * (gStep - 1), ok, okk,
+                                    offset)
+    assert np.max(np.abs(c.decode(c.decrypt(wrong, l, s), l, n, Dl * 2.0 ** 30) - want)) > 1e-2
+
+
 def test_public_key_encryption_decrypts(sc):
     """pk = (b, a) = an encryption of zero; c0 + c1 s = v e + e0 + e1 s + m, well inside the noise."""
     c = sc
