@@ -82,6 +82,22 @@ __global__ __launch_bounds__(kT) void k_lt_mac(const uint64_t* __restrict__ fast
 
 inline dim3 grid_n(uint32_t N, uint32_t y) { return dim3(std::max(1u, std::min((N + kT - 1) / kT, 256u)), y); }
 
+// acc += sum_{t < nt} T[t] over the extended basis of both components (acc: the LimbSet's two
+// polynomials; T[t] at T + t * t_stride in the same layout): one read of each summand and one
+// read-modify-write of acc instead of nt separate additions. blockIdx.y = (comp, pos).
+__global__ __launch_bounds__(kT) void k_lt_sum(LimbSet acc, const uint64_t* __restrict__ T, uint64_t t_stride, int nt, uint32_t N,
+                                              const PrimeConst* prime) {
+  const uint32_t comp = blockIdx.y / acc.nlimbs, pos = blockIdx.y % acc.nlimbs;
+  const uint64_t q = prime[acc.pid[pos]].q;
+  const size_t off = (size_t)comp * acc.poly_stride + (size_t)pos * N;
+  uint64_t* a = acc.base + off;
+  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+    uint64_t s = a[n];
+    for (int t = 0; t < nt; ++t) s = add_mod(s, T[t * t_stride + off + n], q);
+    a[n] = s;
+  }
+}
+
 // k_extend_p for B ciphertexts: blockIdx.y = (b, comp, pos)
 __global__ __launch_bounds__(kT) void k_extend_p_b(const uint64_t* __restrict__ x, uint64_t x_stride, uint64_t* __restrict__ out,
                                                     uint64_t out_stride, uint32_t N, uint32_t lq, uint32_t ext, const PrimeConst* prime,
@@ -313,7 +329,11 @@ void linear_transform_flat(Context& c, const uint64_t* ct, int l, int rowSize, i
     moddown(c, src + E, l, c1q.p, nr, 2 * E, (uint64_t)lq * N, st);
     modup_batch(c, c1q.p, l, nr, (uint64_t)lq * N, dig.p, st, true);
     ksk_items(c, nr, 1, dig.p, (uint64_t)beta * E, src, 2 * E, true, true, l, keys.data(), gals.data(), T.p, 2 * E, st);
-    for (int t = 0; t < nr; ++t) ew_binary(c, 0, acc, make_set(T.p + (size_t)t * 2 * E, ext, epids.data(), 2, E), acc, st);
+    {
+      ProfScope ps_(c, "lt_sum", st);
+      k_lt_sum<<<grid_n(c.N, 2 * ext), kT, 0, st>>>(acc, T.p, 2 * E, nr, c.N, c.d_prime);
+      FH_HIP(hipGetLastError());
+    }
   }
   for (int j = 1; j < gStep; ++j)
     if (((long long)j * bStep * stride) % slots == 0) ew_binary(c, 0, acc, make_set(R.p + (size_t)j * 2 * E, ext, epids.data(), 2, E), acc, st);
