@@ -192,9 +192,9 @@ int fh_device_alloc(fh_context* ctx, size_t bytes, void** out) {
 int fh_device_free(fh_context* ctx, void* ptr) {
   return guard([&] {
     auto& c = C(ctx);
-    // the pool reuses blocks in stream order; make sure no queued work still reads it
-    FH_HIP(hipStreamSynchronize(c.stream));
-    c.pool.put(ptr);
+    // no host wait: the block is reused only after the work queued on the context stream so far
+    // (the last that may read or write it) has completed
+    c.pool.put_after(ptr, c.stream);
   });
 }
 

@@ -144,10 +144,55 @@ int h_default_num_special(const std::vector<uint64_t>& q, int L1, int dnum, int 
 // ---- Pool ----
 Pool::~Pool() { release_all(); }
 
+void Pool::reap(bool wait_one) {
+  while (!pending_.empty()) {
+    Pending& f = pending_.front();
+    if (wait_one) {
+      FH_HIP(hipEventSynchronize(f.ev));
+      wait_one = false;
+    } else {
+      const hipError_t e = hipEventQuery(f.ev);
+      if (e == hipErrorNotReady) {
+        (void)hipGetLastError();  // a pending event is not an error
+        return;
+      }
+      FH_HIP(e);
+    }
+    free_.emplace(f.bytes, f.p);
+    cached_ += f.bytes;
+    pending_bytes_ -= f.bytes;
+    spare_ev_.push_back(f.ev);
+    pending_.pop_front();
+  }
+}
+
+void Pool::put_after(void* p, hipStream_t st) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = sizes_.find(p);
+  if (it == sizes_.end()) return;
+  hipEvent_t ev;
+  if (!spare_ev_.empty()) {
+    ev = spare_ev_.back();
+    spare_ev_.pop_back();
+  } else {
+    FH_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  }
+  FH_HIP(hipEventRecord(ev, st));
+  pending_.push_back(Pending{p, it->second, ev});
+  in_use_ -= it->second;
+  pending_bytes_ += it->second;
+}
+
 void* Pool::get(size_t bytes) {
   bytes = (bytes + 255) & ~(size_t)255;
   std::lock_guard<std::mutex> lk(mu_);
+  if (!pending_.empty()) reap(false);
   auto it = free_.find(bytes);
+  while (it == free_.end() && pending_bytes_ > kPendingCap) {
+    reap(true);
+    it = free_.find(bytes);
+  }
   if (it != free_.end()) {
     void* p = it->second;
     free_.erase(it);
@@ -158,7 +203,9 @@ void* Pool::get(size_t bytes) {
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, bytes);
   if (e != hipSuccess) {
-    // give back cached blocks and retry once
+    (void)hipGetLastError();
+    // wait for the pending blocks, give back every cached block and retry once
+    while (!pending_.empty()) reap(true);
     for (auto& kv : free_) { (void)hipFree(kv.second); sizes_.erase(kv.second); }
     free_.clear();
     cached_ = 0;
@@ -205,9 +252,18 @@ void Pool::put(void* p) {
 
 void Pool::release_all() {
   std::lock_guard<std::mutex> lk(mu_);
+  for (Pending& f : pending_) {
+    (void)hipEventSynchronize(f.ev);
+    free_.emplace(f.bytes, f.p);
+    spare_ev_.push_back(f.ev);
+  }
+  pending_.clear();
+  pending_bytes_ = 0;
   for (auto& kv : free_) { (void)hipFree(kv.second); sizes_.erase(kv.second); }
   free_.clear();
   cached_ = 0;
+  for (hipEvent_t ev : spare_ev_) (void)hipEventDestroy(ev);
+  spare_ev_.clear();
 }
 
 // ---- UploadRing ----
@@ -231,7 +287,7 @@ hipEvent_t UploadRing::upload(void* dst, const void* src, size_t bytes, hipStrea
     if (g.h) FH_HIP(hipHostFree(g.h));
     g.h = nullptr;
     g.cap = 0;
-    const size_t cap = std::max<size_t>(bytes + bytes / 4, 16384);
+    const size_t cap = std::max<size_t>(2 * bytes, (size_t)1 << 20);  // rare regrowth: a slot serves tables of any size
     FH_HIP(hipHostMalloc(&g.h, cap, hipHostMallocDefault));
     g.cap = cap;
   }

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -55,6 +56,11 @@ class Pool {
   ~Pool();
   void* get(size_t bytes);
   void put(void* p);
+  // p is returned once the work queued on st so far has completed: an event recorded on st marks
+  // it, and get() reuses the block only after that event has completed, so no caller on any
+  // stream (or the host) receives a block that queued work still reads or writes. The host does
+  // not wait, unless more than kPendingCap bytes are pending (then get() waits for the oldest).
+  void put_after(void* p, hipStream_t st);
   void release_all();
   // while held, returned blocks are parked instead of reused: work issued on two streams between
   // hold() and release_held() never gets a block another stream may still be using
@@ -64,9 +70,19 @@ class Pool {
   size_t bytes_cached() const { return cached_; }
 
  private:
+  struct Pending {
+    void* p;
+    size_t bytes;
+    hipEvent_t ev;
+  };
+  static constexpr size_t kPendingCap = (size_t)8 << 30;
+  void reap(bool wait_one);  // moves completed pending blocks (oldest first) to the free list
   std::mutex mu_;
   std::multimap<size_t, void*> free_;
   std::map<void*, size_t> sizes_;
+  std::deque<Pending> pending_;
+  std::vector<hipEvent_t> spare_ev_;
+  size_t pending_bytes_ = 0;
   int hold_ = 0;
   std::vector<void*> held_;
   size_t in_use_ = 0, cached_ = 0;

@@ -370,9 +370,9 @@ class Context:
         slots = slots or self.N // 2
         bk = (C.c_void_p * b_step)(*[(k.h if k is not None else None) for k in baby_keys])
         bg = (C.c_uint32 * b_step)(*[int(g) for g in baby_galois])
-        ng = max(len(giant_keys), 1)
-        gk = (C.c_void_p * ng)(*[(k.h if k is not None else None) for k in giant_keys])
-        gg = (C.c_uint32 * ng)(*([int(g) for g in giant_galois] or [1]))
+        ng = len(giant_keys)
+        gk = (C.c_void_p * ng)(*[(k.h if k is not None else None) for k in giant_keys]) if ng else None
+        gg = (C.c_uint32 * ng)(*[int(g) for g in giant_galois]) if ng else None
         _check(load().fh_linear_transform_flat(self.h, ct.ptr, level, row_size, b_step, stride, offset, slots, diags.ptr, bk, bg, gk, gg,
                                                offset_key.h if offset_key is not None else None, offset_galois, out.ptr))
 

@@ -96,6 +96,13 @@ def test_even_galois_rejected(gctx1, octx1, level0, fh):
             g.linear_transform(d, l, o.N // 2, 2, 1, d, [None, rk], [1, good], out, giant_key=rk, giant_galois=bad)
         with pytest.raises(fh.FHError, match="galois"):
             g.linear_transform_batch(d, 1, l, o.N // 2, 2, 1, d, [None, rk], [1, bad], out)
+        with pytest.raises(fh.FHError, match="galois"):
+            g.linear_transform_flat(d, l, 4, 2, 1, d, [None, rk], [1, good], out, giant_keys=[rk], giant_galois=[bad])
+    # the flat form needs a key for every giant multiple that rotates
+    with pytest.raises(fh.FHError, match="null key"):
+        g.linear_transform_flat(d, l, 4, 2, 1, d, [None, rk], [1, good], out, giant_keys=[None], giant_galois=[1])
+    with pytest.raises(fh.FHError, match="giant"):
+        g.linear_transform_flat(d, l, 4, 2, 1, d, [None, rk], [1, good], out)
     # nothing was launched: the output buffer is untouched and the engine still works
     g.rotate_batch(d, 1, l, [rk], [good], out)
     assert np.array_equal(out.download(), o.eval_rotate(src, l, m["rot"], good))
