@@ -593,7 +593,8 @@ def lt_workload(fh, ctx, a):
     full = list(range(L1 + K))
     kbuf = ctx.array((DNUM * 2 * (L1 + K), N))
     keys = {}
-    for i, r in enumerate(list(range(1, bstep)) + [bstep]):
+    gsteps = [j * bstep for j in range(1, row // bstep)]  # the flat form's giant multiples
+    for i, r in enumerate(list(range(1, bstep)) + gsteps):
         ctx.fill_uniform(kbuf, full, num_polys=DNUM * 2, seed=a.seed + 500 + i)
         keys[r] = ctx.key_from_device(kbuf)
     kbuf.free()
@@ -633,6 +634,21 @@ def lt_workload(fh, ctx, a):
     ctx.sync()
     dtb = (time.perf_counter() - t0) / reps
     same = bool(np.array_equal(outs.download(), outb.download()))
+
+    # the flat giant-step form (each multiple of the giant step rotated by its own key, one batched
+    # key switch for all 7), one call per ciphertext: the same map with its own rounding
+    def flat(i):
+        ctx.linear_transform_flat(_view(fh, ctx, cts, i * 2 * lq * N, (2 * lq, N)), lvl, row, bstep, 1, diags, bk, bg,
+                                  _view(fh, ctx, outs, i * 2 * lq * N, (2 * lq, N)), giant_keys=[keys[r] for r in gsteps],
+                                  giant_galois=[ctx.galois_index(r) for r in gsteps])
+
+    flat(0)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for i in range(a.lt_count):
+        flat(i)
+    ctx.sync()
+    dtf = time.perf_counter() - t0
     for x in [diags, cts, out, outs, outb] + list(keys.values()):
         x.free()
     # algorithmic bytes per transform: 15 key switches' keys, 64 diagonals, ciphertext in/out
@@ -644,6 +660,8 @@ def lt_workload(fh, ctx, a):
             "per_call_transforms_per_s": round(a.lt_count / dt, 2), "per_call_ms_per_transform": round(1000 * dt / a.lt_count, 3),
             "batched_transforms_per_s": round(a.lt_count / dtb, 2), "batched_ms_per_transform": round(1000 * dtb / a.lt_count, 3),
             "batched_over_per_call": round(dt / dtb, 3), "bit_exact_batched_vs_per_call": same,
+            "flat_per_call_ms_per_transform": round(1000 * dtf / a.lt_count, 3),
+            "flat_note": "fh_linear_transform_flat per call: 7 giant-step keys instead of 1, the 7 giant rotations as one batch",
             "algorithmic_gb_per_transform": round(alg / 1e9, 3), "algorithmic_gb_per_batch": round(alg_b / 1e9, 3)}
 
 
