@@ -169,12 +169,20 @@ std::vector<BootstrapGroup> factored(uint32_t n, uint64_t M, uint32_t budget, bo
   return groups;
 }
 
-// the smallest power of two whose square covers the row, at most 16 (the linear transform's
-// baby-step bound; b + g - 2 rotations either way, so the cap costs no rotations)
+// baby steps of a bootstrap transform: the power of two up to 16 (the linear transform's bound)
+// minimizing (b - 1) + 4 (ceil(rowSize / b) - 1). In the flat form a giant step costs a ModDown,
+// a ModUp and a key inner product (batched with the other giant steps), a hoisted baby step only
+// the key inner product: weighted 4 : 1. Ties go to the smaller b.
 uint32_t bstep_for(uint32_t rowSize) {
-  uint32_t b = 1;
-  while (b * b < rowSize && b < 16) b *= 2;
-  return b;
+  uint32_t best_b = 1, best_cost = UINT32_MAX;
+  for (uint32_t b = 1; b <= 16; b *= 2) {
+    const uint32_t cost = (b - 1) + 4 * ((rowSize + b - 1) / b - 1);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best_b = b;
+    }
+  }
+  return best_b;
 }
 
 struct CosArg {

@@ -244,10 +244,16 @@ def factored(n, M, budget, inverse, factor):
 
 
 def bstep_for(rowSize):
-    b = 1
-    while b * b < rowSize and b < 16:
-        b *= 2
-    return b
+    """baby steps of a bootstrap transform: the power of two up to 16 minimizing
+    (b - 1) + 4 (ceil(rowSize / b) - 1), a giant step (ModDown, ModUp and a key inner product in
+    the flat form's batch) weighted as four hoisted baby steps (a key inner product each);
+    ties go to the smaller b"""
+    best = None
+    for b in (1, 2, 4, 8, 16):
+        cost = (b - 1) + 4 * (-(-rowSize // b) - 1)
+        if best is None or cost < best[0]:
+            best = (cost, b)
+    return best[1]
 
 
 TRACE_RADIX = 4
