@@ -36,6 +36,8 @@ __global__ __launch_bounds__(kT) void k_extend_p(const uint64_t* __restrict__ x,
 struct MacParams {
   uint32_t N, lq, ext, L1, rowSize, bStep, gStep;
   const PrimeConst* prime;
+  uint32_t i0 = 0;   // first baby step of this launch (a launch covers baby steps i0 .. i0 + BB - 1)
+  uint32_t acc = 0;  // 1: add into R (a later launch over the next baby steps)
 };
 
 // R[j][comp][pos] = sum_{i: bStep j + i < rowSize} fast[i][comp][pos] (.) diag[bStep j + i][pos]
@@ -53,9 +55,9 @@ __global__ __launch_bounds__(kT) void k_lt_mac(const uint64_t* __restrict__ fast
     Op f0[BB], f1[BB];
 #pragma unroll
     for (int i = 0; i < BB; ++i) {
-      if (i < (int)mp.bStep) {
-        f0[i] = M::prep(fast[((size_t)i * 2 + 0) * E + (size_t)pos * N + n]);
-        f1[i] = M::prep(fast[((size_t)i * 2 + 1) * E + (size_t)pos * N + n]);
+      if (mp.i0 + i < mp.bStep) {
+        f0[i] = M::prep(fast[((size_t)(mp.i0 + i) * 2 + 0) * E + (size_t)pos * N + n]);
+        f1[i] = M::prep(fast[((size_t)(mp.i0 + i) * 2 + 1) * E + (size_t)pos * N + n]);
       }
     }
     for (uint32_t j = 0; j < mp.gStep; ++j) {
@@ -64,18 +66,22 @@ __global__ __launch_bounds__(kT) void k_lt_mac(const uint64_t* __restrict__ fast
       a1.zero();
 #pragma unroll
       for (int i = 0; i < BB; ++i) {
-        const uint32_t k = j * mp.bStep + i;
-        if (i < (int)mp.bStep && k < mp.rowSize) {
+        const uint32_t k = j * mp.bStep + mp.i0 + i;
+        if (mp.i0 + i < mp.bStep && k < mp.rowSize) {
           const Op d = M::prep(diags[(size_t)k * E + (size_t)pos * N + n]);
           a0.mac(f0[i], d);
           a1.mac(f1[i], d);
         }
       }
       uint64_t hi, lo;
+      uint64_t* r0 = R + ((size_t)j * 2 + 0) * E + (size_t)pos * N + n;
+      uint64_t* r1 = R + ((size_t)j * 2 + 1) * E + (size_t)pos * N + n;
       a0.get(hi, lo);
-      R[((size_t)j * 2 + 0) * E + (size_t)pos * N + n] = mod128(hi, lo, P);
+      const uint64_t v0 = mod128(hi, lo, P);
       a1.get(hi, lo);
-      R[((size_t)j * 2 + 1) * E + (size_t)pos * N + n] = mod128(hi, lo, P);
+      const uint64_t v1 = mod128(hi, lo, P);
+      *r0 = mp.acc ? add_mod(*r0, v0, P.q) : v0;
+      *r1 = mp.acc ? add_mod(*r1, v1, P.q) : v1;
     }
   }
 }
@@ -220,14 +226,17 @@ DBuf lt_partials(Context& c, const uint64_t* ct, int l, int rowSize, int bStep, 
   {
     ProfScope ps_(c, "lt_mac", st);
     const dim3 g = grid_n(c.N, ext);
+    // more than 8 baby steps: launches of 8 each, the later ones adding into R (the sums are
+    // exact residues, so the split result is the same word); 16 per lane would hold 3 waves per
+    // SIMD instead of 5
     if (c.split_mac_ok) {
       if (bStep <= 4) k_lt_mac<4, Mac30><<<g, kT, 0, st>>>(fast.p, diags, R.p, mp);
-      else if (bStep <= 8) k_lt_mac<8, Mac30><<<g, kT, 0, st>>>(fast.p, diags, R.p, mp);
-      else k_lt_mac<16, Mac30><<<g, kT, 0, st>>>(fast.p, diags, R.p, mp);
+      else
+        for (mp.i0 = 0; mp.i0 < (uint32_t)bStep; mp.i0 += 8, mp.acc = 1) k_lt_mac<8, Mac30><<<g, kT, 0, st>>>(fast.p, diags, R.p, mp);
     } else {
       if (bStep <= 4) k_lt_mac<4, MacWide><<<g, kT, 0, st>>>(fast.p, diags, R.p, mp);
-      else if (bStep <= 8) k_lt_mac<8, MacWide><<<g, kT, 0, st>>>(fast.p, diags, R.p, mp);
-      else k_lt_mac<16, MacWide><<<g, kT, 0, st>>>(fast.p, diags, R.p, mp);
+      else
+        for (mp.i0 = 0; mp.i0 < (uint32_t)bStep; mp.i0 += 8, mp.acc = 1) k_lt_mac<8, MacWide><<<g, kT, 0, st>>>(fast.p, diags, R.p, mp);
     }
     FH_HIP(hipGetLastError());
   }

@@ -44,7 +44,7 @@ def _run(g, o, s, l, rowSize, bStep, stride, offset, scale_pt=2.0 ** 30, seed=0,
         x.free()
 
 
-@pytest.mark.parametrize("rowSize,bStep,offset", [(12, 4, 0), (10, 4, 3), (8, 8, 0), (16, 4, 5)])
+@pytest.mark.parametrize("rowSize,bStep,offset", [(12, 4, 0), (10, 4, 3), (8, 8, 0), (16, 4, 5), (40, 16, 0), (29, 12, 2)])
 def test_linear_transform_cfg1(rowSize, bStep, offset, gctx1, octx1):
     s = octx1.gen_secret(70)
     _run(gctx1, octx1, s, 3, rowSize, bStep, 1, offset, seed=rowSize + offset)
@@ -126,7 +126,8 @@ def _run_flat(g, o, s, l, rowSize, bStep, stride, offset, seed, scale_pt=2.0 ** 
         x.free()
 
 
-@pytest.mark.parametrize("rowSize,bStep,stride,offset", [(12, 4, 1, 0), (10, 4, 1, 3), (16, 4, 1, 5), (8, 8, 1, 0), (12, 4, -8, 0)])
+@pytest.mark.parametrize("rowSize,bStep,stride,offset", [(12, 4, 1, 0), (10, 4, 1, 3), (16, 4, 1, 5), (8, 8, 1, 0), (12, 4, -8, 0),
+                                                         (63, 16, 1, 3)])
 def test_linear_transform_flat_cfg1(rowSize, bStep, stride, offset, gctx1, octx1):
     """stride -8 stands for a stride of slots/8: the second giant multiple (8 strides) wraps to 0 mod
     slots and is added without a rotation."""
